@@ -1,0 +1,354 @@
+"""bonsai_amd — MI355X-native GBDT hot path (histograms, split find, partition, growers,
+predict, TreeSHAP) behind the C ABI of include/bonsai_hip.h.
+
+This module is the Python host binding of libbonsai_hip.so (ctypes).  It mirrors the
+reference's booster/engine vocabulary: a ``HipContext`` holds one resident training set
+(the reference's CudaDeviceContext), ``FitConfig`` carries TreeConfig/BoosterConfig
+fields plus the device knobs, growers are named ``hip_depthwise`` / ``hip_leafwise`` /
+``hip_levelwise`` (the reference registers ``cuda_*`` under
+include/bonsai/registry/names.hpp:81-94), and errors map to ``ConfigError`` (capacity
+or configuration refusals) and ``DeviceError``.
+
+The library must be built (``python -m bonsai_amd.build`` or
+``__graft_entry__.build()``); importing the binding without it raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libbonsai_hip.so"
+
+
+class ConfigError(ValueError):
+    """A configuration or capacity refusal (the reference's bonsai::ConfigError)."""
+
+
+class DeviceError(RuntimeError):
+    """A HIP runtime failure."""
+
+
+class LibraryMissing(ImportError):
+    """libbonsai_hip.so has not been built: there is no CPU fallback."""
+
+
+GROWERS = {"hip_depthwise": 0, "hip_leafwise": 1, "hip_levelwise": 2}
+OBJECTIVES = {"mse": 0, "logloss": 1}
+
+
+class FitConfigC(C.Structure):
+    _fields_ = [
+        ("grower", C.c_int32), ("objective", C.c_int32), ("quantized", C.c_int32),
+        ("use_graph", C.c_int32), ("host_sync", C.c_int32), ("max_depth", C.c_int32),
+        ("min_data_in_leaf", C.c_int32), ("max_leaves", C.c_int32),
+        ("min_child_hess", C.c_float), ("min_gain_to_split", C.c_float),
+        ("lambda_l2", C.c_float), ("lambda_l1", C.c_float), ("learning_rate", C.c_float),
+        ("_pad", C.c_uint32), ("seed", C.c_uint64),
+    ]
+
+
+NODE_DTYPE = np.dtype([
+    ("feature", np.int32), ("bin", np.uint32), ("default_left", np.uint32),
+    ("left", np.uint32), ("right", np.uint32), ("value", np.float32),
+    ("cover", np.float32), ("gain", np.float32),
+])
+SPLIT_DTYPE = np.dtype([
+    ("gain", np.float64), ("feature", np.uint32), ("bin", np.uint32),
+    ("default_left", np.uint32), ("valid", np.uint32),
+    ("gL", np.float64), ("hL", np.float64), ("gR", np.float64), ("hR", np.float64),
+    ("igL", np.int64), ("ihL", np.int64), ("igR", np.int64), ("ihR", np.int64),
+])
+assert NODE_DTYPE.itemsize == 32 and SPLIT_DTYPE.itemsize == 88
+
+
+@dataclass
+class FitConfig:
+    """TreeConfig / BoosterConfig fields the hot path reads, with the reference's
+    defaults (include/bonsai/config/tree_config.hpp:9-29,
+    include/bonsai/config/booster_config.hpp:8-24), plus device knobs."""
+    grower: str = "hip_depthwise"
+    objective: str = "logloss"
+    quantized: bool = False
+    use_graph: bool = True
+    host_sync: bool = False
+    max_depth: int = 6
+    min_data_in_leaf: int = 20
+    max_leaves: int = 31
+    min_child_hess: float = 1.0
+    min_gain_to_split: float = 0.0
+    lambda_l2: float = 1.0
+    lambda_l1: float = 0.0
+    learning_rate: float = 0.05
+    seed: int = 42
+
+    def to_c(self) -> FitConfigC:
+        if self.grower not in GROWERS:
+            raise ConfigError(f"unknown grower_name {self.grower!r}; known: {sorted(GROWERS)}")
+        if self.objective not in OBJECTIVES:
+            raise ConfigError(f"objective {self.objective!r} has no device gradient")
+        return FitConfigC(
+            GROWERS[self.grower], OBJECTIVES[self.objective], int(self.quantized),
+            int(self.use_graph), int(self.host_sync), self.max_depth, self.min_data_in_leaf,
+            self.max_leaves, self.min_child_hess, self.min_gain_to_split, self.lambda_l2,
+            self.lambda_l1, self.learning_rate, 0, self.seed)
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libbonsai_hip.so once; raises LibraryMissing if it was never built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise LibraryMissing(f"{LIB_PATH} not built; run python -m bonsai_amd.build")
+    L = C.CDLL(str(LIB_PATH))
+    P, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32
+    sig = {
+        "bh_device_count": [C.POINTER(C.c_int)],
+        "bh_create": [C.c_int, C.POINTER(P)],
+        "bh_destroy": [P],
+        "bh_upload_bins": [P, P, u64, u32, u64, P],
+        "bh_upload_labels": [P, P],
+        "bh_configure": [P, C.POINTER(FitConfigC)],
+        "bh_set_scores": [P, P],
+        "bh_get_scores": [P, P],
+        "bh_train": [P, u32],
+        "bh_get_trees": [P, P, u32, P, u32, C.POINTER(u32)],
+        "bh_max_nodes": [P, C.POINTER(u32)],
+        "bh_set_gradients": [P, P, P, u32, P, P, P],
+        "bh_grow_tree": [P, P, u32, C.POINTER(u32), P],
+        "bh_histogram": [P, P, u64, P, P],
+        "bh_find_split": [P, P, P, u64, u32, i32, C.c_double, C.c_double, P, P],
+        "bh_partition": [P, P, u64, u32, u32, u32, P, C.POINTER(u64)],
+        "bh_time_root_histogram": [P, u32, C.POINTER(C.c_float)],
+        "bh_set_timing": [P, i32],
+        "bh_get_timing": [P, P, u32],
+        "bh_predictor_create": [C.c_int, P, P, u32, P, u32, C.c_float, C.c_float, C.POINTER(P)],
+        "bh_predictor_destroy": [P],
+        "bh_predict": [P, P, u64, u64, P],
+        "bh_shap": [P, P, u64, u64, P],
+        "bh_predict_device": [P, P, u64, u64, P, P],
+        "bh_shap_device": [P, P, u64, u64, P, P],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    L.bh_last_error.restype = C.c_char_p
+    L.bh_last_error.argtypes = []
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc == 0:
+        return
+    msg = lib().bh_last_error().decode(errors="replace")
+    if rc > 0:
+        raise ConfigError(msg)
+    raise DeviceError(msg)
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = lib().bh_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+class HipContext:
+    """One resident training set on one GPU (the reference's CudaDeviceContext)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().bh_create(device, C.byref(self._h)))
+        self.n_rows = 0
+        self.n_features = 0
+        self.n_bins = None
+        self.cfg: FitConfig | None = None
+        self.n_trained = 0
+
+    def close(self) -> None:
+        if self._h:
+            lib().bh_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- data plane --
+    def upload_bins(self, bins: np.ndarray, n_bins: np.ndarray) -> None:
+        bins = np.ascontiguousarray(bins, dtype=np.uint8)
+        n_bins = np.ascontiguousarray(n_bins, dtype=np.uint16)
+        n, f = bins.shape
+        _check(lib().bh_upload_bins(self._h, _ptr(bins), n, f, f, _ptr(n_bins)))
+        self.n_rows, self.n_features, self.n_bins = n, f, n_bins.copy()
+
+    def upload_labels(self, labels: np.ndarray) -> None:
+        labels = np.ascontiguousarray(labels, dtype=np.float32)
+        _check(lib().bh_upload_labels(self._h, _ptr(labels)))
+
+    def configure(self, cfg: FitConfig) -> None:
+        c = cfg.to_c()
+        _check(lib().bh_configure(self._h, C.byref(c)))
+        self.cfg = cfg
+
+    @property
+    def bin_stride(self) -> int:
+        return int(self.n_bins.max())
+
+    # -- whole fit --
+    def set_scores(self, scores: np.ndarray) -> None:
+        scores = np.ascontiguousarray(scores, dtype=np.float32)
+        _check(lib().bh_set_scores(self._h, _ptr(scores)))
+        self.n_trained = 0
+
+    def get_scores(self) -> np.ndarray:
+        out = np.empty(self.n_rows, np.float32)
+        _check(lib().bh_get_scores(self._h, _ptr(out)))
+        return out
+
+    def train(self, n_iters: int) -> None:
+        _check(lib().bh_train(self._h, n_iters))
+        self.n_trained += n_iters
+
+    def max_nodes(self) -> int:
+        m = C.c_uint32(0)
+        _check(lib().bh_max_nodes(self._h, C.byref(m)))
+        return m.value
+
+    def trees(self) -> list[np.ndarray]:
+        """Trees grown since the last set_scores, in bin space (NODE_DTYPE)."""
+        m = self.max_nodes()
+        cap = max(1, self.n_trained)
+        nodes = np.zeros((cap, m), NODE_DTYPE)
+        sizes = np.zeros(cap, np.uint32)
+        n = C.c_uint32(0)
+        _check(lib().bh_get_trees(self._h, _ptr(nodes), m, _ptr(sizes), cap, C.byref(n)))
+        return [nodes[t, : sizes[t]].copy() for t in range(n.value)]
+
+    def set_timing(self, on: bool) -> None:
+        _check(lib().bh_set_timing(self._h, int(on)))
+
+    def timing(self, n: int) -> np.ndarray:
+        out = np.zeros(n, np.float32)
+        _check(lib().bh_get_timing(self._h, _ptr(out), n))
+        return out
+
+    # -- engine seam --
+    def set_gradients(self, g: np.ndarray, h: np.ndarray, it: int = 0):
+        g = np.ascontiguousarray(g, np.float32)
+        h = np.ascontiguousarray(h, np.float32)
+        gq = np.zeros(self.n_rows, np.int8)
+        hq = np.zeros(self.n_rows, np.int16)
+        sc = np.zeros(2, np.float64)
+        _check(lib().bh_set_gradients(self._h, _ptr(g), _ptr(h), it, _ptr(gq), _ptr(hq), _ptr(sc)))
+        return gq, hq, sc
+
+    def grow_tree(self) -> tuple[np.ndarray, np.ndarray]:
+        cap = self.max_nodes()
+        nodes = np.zeros(cap, NODE_DTYPE)
+        nn = C.c_uint32(0)
+        lor = np.zeros(self.n_rows, np.uint32)
+        _check(lib().bh_grow_tree(self._h, _ptr(nodes), cap, C.byref(nn), _ptr(lor)))
+        return nodes[: nn.value].copy(), lor
+
+    def histogram(self, rows: np.ndarray) -> np.ndarray:
+        rows = np.ascontiguousarray(rows, np.uint32)
+        shape = (self.n_features, self.bin_stride, 2)
+        if self.cfg.quantized:
+            out = np.zeros(shape, np.int64)
+            _check(lib().bh_histogram(self._h, _ptr(rows), rows.size, None, _ptr(out)))
+        else:
+            out = np.zeros(shape, np.float32)
+            _check(lib().bh_histogram(self._h, _ptr(rows), rows.size, _ptr(out), None))
+        return out
+
+    def find_split(self, hist: np.ndarray, n_rows: int, depth: int = 0, totals=None,
+                   scales=None) -> np.ndarray:
+        out = np.zeros(1, SPLIT_DTYPE)
+        tg, th = (0.0, 0.0) if totals is None else totals
+        sc = None if scales is None else np.ascontiguousarray(scales, np.float64)
+        if self.cfg.quantized:
+            h = np.ascontiguousarray(hist, np.int64)
+            _check(lib().bh_find_split(self._h, None, _ptr(h), n_rows, depth, int(totals is not None),
+                                       float(tg), float(th), _ptr(sc), _ptr(out)))
+        else:
+            h = np.ascontiguousarray(hist, np.float32)
+            _check(lib().bh_find_split(self._h, _ptr(h), None, n_rows, depth, int(totals is not None),
+                                       float(tg), float(th), _ptr(sc), _ptr(out)))
+        return out[0]
+
+    def partition(self, rows: np.ndarray, feature: int, bin_: int, default_left: bool):
+        rows = np.ascontiguousarray(rows, np.uint32)
+        out = np.zeros(rows.size, np.uint32)
+        nl = C.c_uint64(0)
+        _check(lib().bh_partition(self._h, _ptr(rows), rows.size, feature, bin_, int(default_left),
+                                  _ptr(out), C.byref(nl)))
+        return out, nl.value
+
+    def time_root_histogram(self, reps: int = 10) -> float:
+        ms = C.c_float(0)
+        _check(lib().bh_time_root_histogram(self._h, reps, C.byref(ms)))
+        return ms.value
+
+
+class Predictor:
+    """A resident ensemble for predict / TreeSHAP (the reference's CudaPredictPlan)."""
+
+    def __init__(self, trees: list[np.ndarray], n_bins: np.ndarray, learning_rate: float,
+                 init_score: float, device: int = 0):
+        offsets = np.zeros(len(trees), np.uint32)
+        pos = 0
+        for i, t in enumerate(trees):
+            offsets[i] = pos
+            pos += len(t)
+        nodes = np.concatenate(trees) if trees else np.zeros(0, NODE_DTYPE)
+        nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+        last = np.ascontiguousarray(np.asarray(n_bins, np.uint16) - 1, np.uint16)
+        self.n_features = len(last)
+        self._h = C.c_void_p()
+        _check(lib().bh_predictor_create(device, _ptr(nodes), _ptr(offsets), len(trees), _ptr(last),
+                                         self.n_features, learning_rate, init_score, C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().bh_predictor_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def predict(self, bins: np.ndarray) -> np.ndarray:
+        bins = np.ascontiguousarray(bins, np.uint8)
+        out = np.zeros(bins.shape[0], np.float32)
+        _check(lib().bh_predict(self._h, _ptr(bins), bins.shape[0], bins.shape[1], _ptr(out)))
+        return out
+
+    def shap(self, bins: np.ndarray) -> np.ndarray:
+        bins = np.ascontiguousarray(bins, np.uint8)
+        out = np.zeros((bins.shape[0], self.n_features + 1), np.float64)
+        _check(lib().bh_shap(self._h, _ptr(bins), bins.shape[0], bins.shape[1], _ptr(out)))
+        return out

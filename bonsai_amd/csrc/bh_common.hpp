@@ -1,0 +1,146 @@
+// Shared host/device layouts of the MI355X GBDT engine.
+//
+// Data layout in HBM (one context = one resident training set):
+//   plane0   u8 [N][Fs]   pristine bin plane, original row order (Fs = F rounded up to 4)
+//   work0/1  u8 [N][Fs]   ping-pong planes: rows physically partitioned per node, each
+//                         node's rows one contiguous segment, ascending original order
+//   gh0      gradient pairs in original order (f32x2, or one packed u32 when quantized:
+//                         high 16 bits int16 g in [-127,127], low 16 bits uint16 h)
+//   ghw0/1   gradient pairs travelling with the partitioned rows
+//   hist[2]  node histograms of the current / next level, [slot][F][SB][2] cells
+//            (f32 pairs, or i64 pairs when quantized)
+#pragma once
+#include <stdint.h>
+
+namespace bh {
+
+typedef uint8_t  u8;
+typedef uint16_t u16;
+typedef uint32_t u32;
+typedef uint64_t u64;
+typedef int64_t  i64;
+
+constexpr u32 kNone       = 0xFFFFFFFFu;
+constexpr u32 kSrcPlane0  = 2; // segment lives in the pristine plane
+constexpr u32 kHistChunk  = 16384; // rows per histogram work unit (fixed: part of the
+                                   // FP32 summation order)
+constexpr u32 kPartChunk  = 4096;  // rows per partition work unit
+constexpr u32 kMaxTreeNodes = 8192;
+
+// Tree node in bin space, the layout the host reads back (feature < 0: leaf).
+struct DevNode {
+    int32_t feature;
+    u32     bin;
+    u32     default_left;
+    u32     left;
+    u32     right;
+    float   value;
+    float   cover;
+    float   gain;
+};
+static_assert(sizeof(DevNode) == 32, "DevNode");
+
+// One frontier node of the tree being grown.
+struct NodeInfo {
+    u32    id;       // node id in the tree
+    u32    off;      // first row of its segment
+    u32    cnt;      // rows in the segment
+    u32    src;      // 0/1 working plane, 2 = pristine plane
+    u32    slot;     // histogram slot (frontier position for level growth)
+    u32    depth;
+    float  tg, th;   // FP32 totals (Histogram::totals semantics)
+    i64    itg, ith; // quantized totals
+    u32    totals_ready; // 1 when totals were set from a parent split (last level)
+    u32    pad;
+};
+static_assert(sizeof(NodeInfo) == 56, "NodeInfo");
+
+// Split decision of one node (SplitOutput plus the child sums the finder saw).
+struct SplitRec {
+    double gain;
+    u32    feature, bin, default_left, valid;
+    double gL, hL, gR, hR;     // FP32 path child sums (double)
+    i64    igL, ihL, igR, ihR; // quantized path child sums (exact)
+};
+static_assert(sizeof(SplitRec) == 88, "SplitRec");
+
+// A histogram work unit: rows [begin, end) of one node's segment.
+struct HistChunk {
+    u32 slot;  // destination node histogram slot (next-level buffer)
+    u32 src;   // which plane/gh buffer holds the rows
+    u32 begin, end;
+    u32 part;  // partial index, or kNone when the chunk is the node's only one
+    u32 pad[3];
+};
+
+// Completion of one built node: small = sum of partials (fixed order), large = parent -
+// small.  large_slot == kNone for the root.
+struct FinishOp {
+    u32 small_slot, large_slot, parent_slot;
+    u32 part_first, part_count; // part_count == 0: written directly (or empty)
+    u32 empty;                  // small child has no rows: zero its histogram
+    u32 unit_first, unit_count; // histogram units of the small child
+    u32 small_off, small_cnt, src;
+    u32 pad[5];
+};
+static_assert(sizeof(FinishOp) == 64, "FinishOp");
+
+// A split's partition: parent frontier index and the routing rule.
+struct PartOp {
+    u32 parent;       // frontier index
+    u32 feature, bin, default_left, last_bin;
+    u32 off, cnt, src, dst;
+    u32 chunk_first, chunk_count;
+    u32 n_left;       // filled by the scan
+    u32 left_id, right_id;
+    u32 alive;        // 0 after demotion
+    u32 base;         // scan scratch: global prefix at the op's first unit
+};
+
+struct PartChunk {
+    u32 op;
+    u32 begin, end; // rows relative to the op's segment
+    u32 left_before; // exclusive prefix of left rows over the op's earlier chunks
+};
+
+// Device-side control block of one tree under construction.
+struct TreeCtl {
+    u32 n_front;        // frontier size of the current level
+    u32 n_nodes;        // nodes allocated in the tree
+    u32 n_ops;          // split ops of the current level
+    u32 n_part_chunks;
+    u32 n_hist_chunks;
+    u32 n_finish;
+    u32 n_parts;        // partial histograms in use
+    u32 next_front;     // frontier size after the partition
+    u32 iter;           // boosting iteration (RNG key and tree slot)
+    u32 depth;          // level being processed
+    u32 done;           // leafwise: growth finished
+    u32 live;           // leafwise: live leaves
+    u32 n_cand;         // leafwise: candidates in the pool
+    u32 n_pending;
+    u32 sel;            // leafwise: selected candidate index
+    u32 max_depth_seen;
+    u32 oblivious_depth; // levelwise: levels split so far
+    u32 pad[15];
+};
+static_assert(sizeof(TreeCtl) == 128, "TreeCtl");
+
+// Tree-growth settings shared by every kernel of a fit.
+struct GrowCfg {
+    double min_child_hess, min_gain, l2, l1;
+    int32_t max_depth, min_data_in_leaf, max_leaves, quantized;
+    double scale_g, scale_h; // unused on device (read from scales buffer)
+};
+
+// Compact node for the bin-space walks (score update, predict).
+struct WalkNode {
+    u16 feature; // 0xFFFF: leaf
+    u8  bin;
+    u8  default_left;
+    u16 left, right;
+    float value;
+};
+static_assert(sizeof(WalkNode) == 12, "WalkNode");
+
+} // namespace bh

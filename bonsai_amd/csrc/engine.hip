@@ -1,0 +1,839 @@
+// Host runtime of the MI355X GBDT engine: device buffers, the three grow loops as
+// device-resident kernel sequences, hipGraph capture of a whole boosting iteration, and
+// the extern "C" boundary declared in include/bonsai_hip.h.
+//
+// The grow loops keep the reference's control semantics (src/grower_impl.hpp) but run
+// their decisions on the device: the host only enqueues fixed-shape launches, so an
+// iteration has no host round-trip unless host_sync asks for the reference's cadence.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bonsai_hip.h"
+#include "bh_common.hpp"
+#include "kernels.hpp"
+#include "kernels_ext.hpp"
+
+using namespace bh;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, std::string const &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t const e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                                \
+            return fail(-1, std::string("hip: ") + #expr + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+template <typename T> hipError_t dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
+}
+
+u32 round_up4(u32 x) { return (x + 3u) & ~3u; }
+
+} // namespace
+
+struct bh_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // data plane
+    u64 N = 0;
+    u32 F = 0, Fs = 0, SB = 0;
+    std::vector<u16> n_bins;
+    u8 *plane0 = nullptr, *work[2] = {nullptr, nullptr};
+    u16 *d_nbins = nullptr;
+    float *label = nullptr, *score = nullptr;
+    bool have_labels = false;
+    // gradients
+    float2 *gh0f = nullptr, *ghwf[2] = {nullptr, nullptr};
+    u32 *gh0q = nullptr, *ghwq[2] = {nullptr, nullptr};
+    u32 *maxbits = nullptr;
+    double *scales = nullptr;
+    // histograms and work lists
+    void *hist[2] = {nullptr, nullptr};
+    void *partials = nullptr;
+    TreeCtl *ctl = nullptr;
+    NodeInfo *front[2] = {nullptr, nullptr};
+    SplitRec *splits = nullptr;
+    DevNode *tree = nullptr;
+    PartOp *ops = nullptr;
+    PartChunk *pch = nullptr;
+    u32 *chunk_left = nullptr;
+    HistChunk *hch = nullptr;
+    FinishOp *fops = nullptr;
+    u32 *leaf_rows = nullptr;
+    ExtState ext{};
+    // outputs
+    DevNode *trees_out = nullptr;
+    u32 *tree_sizes = nullptr;
+    u32 trees_cap = 0, trees_done = 0, max_nodes = 0;
+    // capacities
+    u32 max_front = 0, max_ops = 0, max_pch = 0, max_hch = 0, max_parts = 0, hist_slots = 0;
+    // configuration
+    bh_fit_config fc{};
+    GrowCfg gc{};
+    bool configured = false;
+    bool quant = false;
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    // timing
+    bool timing = false;
+    std::vector<float> iter_ms;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    u32 hist_fsl() const { return F <= 64 ? round_up4(F) : 128u; }
+    u32 hist_waves() const
+    {
+        u32 const groups = (std::min(F, hist_fsl()) + 3u) / 4u;
+        return std::max(1u, std::min(groups, 16u));
+    }
+    size_t cell_bytes() const { return quant ? 16 : 8; }
+    size_t part_bytes() const { return 8; }
+    u32 node_cells() const { return F * SB; }
+};
+
+namespace {
+
+void free_fit(bh_context *c)
+{
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    void *ptrs[] = {c->gh0f, c->ghwf[0], c->ghwf[1], c->gh0q, c->ghwq[0], c->ghwq[1], c->maxbits, c->scales,
+                    c->hist[0], c->hist[1], c->partials, c->ctl, c->front[0], c->front[1], c->splits, c->tree,
+                    c->ops, c->pch, c->chunk_left, c->hch, c->fops, c->trees_out, c->tree_sizes, c->leaf_rows};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    c->gh0f = c->ghwf[0] = c->ghwf[1] = nullptr;
+    c->gh0q = c->ghwq[0] = c->ghwq[1] = nullptr;
+    c->maxbits = nullptr;
+    c->scales = nullptr;
+    c->hist[0] = c->hist[1] = c->partials = nullptr;
+    c->ctl = nullptr;
+    c->front[0] = c->front[1] = nullptr;
+    c->splits = nullptr;
+    c->tree = nullptr;
+    c->ops = nullptr;
+    c->pch = nullptr;
+    c->chunk_left = nullptr;
+    c->hch = nullptr;
+    c->fops = nullptr;
+    c->trees_out = nullptr;
+    c->tree_sizes = nullptr;
+    c->leaf_rows = nullptr;
+    ext_free(c->ext);
+    c->configured = false;
+    c->trees_cap = c->trees_done = 0;
+}
+
+// Node capacity of one tree under the configured policy.
+u32 tree_capacity(bh_fit_config const &f)
+{
+    if (f.grower == BH_LEAFWISE) return std::min<u32>(kMaxTreeNodes, 4u * static_cast<u32>(f.max_leaves) + 8u);
+    return std::min<u32>(kMaxTreeNodes, (2u << static_cast<u32>(f.max_depth)) + 2u);
+}
+
+int alloc_fit(bh_context *c)
+{
+    bh_fit_config const &f = c->fc;
+    u32 const N = static_cast<u32>(c->N);
+    c->quant = f.quantized != 0;
+    // Frontier capacity: 2^max_depth for level growth, 2 for leafwise rounds.
+    u32 const depth = static_cast<u32>(f.max_depth);
+    c->max_front = f.grower == BH_LEAFWISE ? 2u : (1u << std::min(depth, 14u));
+    c->max_ops = std::max(1u, c->max_front);
+    c->max_pch = N / kPartChunk + c->max_ops + 2;
+    c->max_hch = N / kHistChunk + c->max_ops + 2;
+    c->max_parts = N / kHistChunk + c->max_ops + 2;
+    c->hist_slots = f.grower == BH_LEAFWISE ? static_cast<u32>(f.max_leaves) + 4u : std::max(1u, c->max_front / 2u);
+    c->max_nodes = tree_capacity(f);
+    size_t const nc = c->node_cells();
+    size_t const hist_bytes = static_cast<size_t>(c->hist_slots) * nc * c->cell_bytes();
+    if (c->quant) {
+        HIP_TRY(dalloc(&c->gh0q, N));
+        HIP_TRY(dalloc(&c->ghwq[0], N));
+        HIP_TRY(dalloc(&c->ghwq[1], N));
+    } else {
+        HIP_TRY(dalloc(&c->gh0f, N));
+        HIP_TRY(dalloc(&c->ghwf[0], N));
+        HIP_TRY(dalloc(&c->ghwf[1], N));
+    }
+    HIP_TRY(dalloc(&c->maxbits, 4));
+    HIP_TRY(dalloc(&c->scales, 4));
+    HIP_TRY(hipMemset(c->scales, 0, 4 * sizeof(double)));
+    HIP_TRY(hipMalloc(&c->hist[0], hist_bytes));
+    if (f.grower != BH_LEAFWISE) HIP_TRY(hipMalloc(&c->hist[1], hist_bytes));
+    HIP_TRY(hipMalloc(&c->partials, static_cast<size_t>(c->max_parts) * nc * c->part_bytes()));
+    HIP_TRY(dalloc(&c->ctl, 1));
+    HIP_TRY(hipMemset(c->ctl, 0, sizeof(TreeCtl)));
+    u32 const fcap = std::max(c->max_front * 2u, 4u);
+    HIP_TRY(dalloc(&c->front[0], fcap));
+    HIP_TRY(dalloc(&c->front[1], fcap));
+    HIP_TRY(dalloc(&c->splits, fcap));
+    HIP_TRY(dalloc(&c->tree, c->max_nodes));
+    HIP_TRY(dalloc(&c->ops, c->max_ops + 2));
+    HIP_TRY(dalloc(&c->pch, c->max_pch));
+    HIP_TRY(dalloc(&c->chunk_left, c->max_pch));
+    HIP_TRY(dalloc(&c->hch, c->max_hch));
+    HIP_TRY(dalloc(&c->fops, c->max_ops + 2));
+    HIP_TRY(dalloc(&c->leaf_rows, N));
+    if (ext_alloc(c->ext, f, c->F, c->SB, c->max_front, c->hist_slots) != hipSuccess)
+        return fail(-1, "hip: extension buffers");
+    c->gc.min_child_hess = f.min_child_hess;
+    c->gc.min_gain = f.min_gain_to_split;
+    c->gc.l2 = f.lambda_l2;
+    c->gc.l1 = f.lambda_l1;
+    c->gc.max_depth = f.max_depth;
+    c->gc.min_data_in_leaf = f.min_data_in_leaf;
+    c->gc.max_leaves = f.max_leaves;
+    c->gc.quantized = f.quantized;
+    c->configured = true;
+    return 0;
+}
+
+int ensure_tree_out(bh_context *c, u32 n_more)
+{
+    u32 const need = c->trees_done + n_more;
+    if (need <= c->trees_cap) return 0;
+    u32 const cap = std::max(need, c->trees_cap * 2u + 16u);
+    DevNode *nt = nullptr;
+    u32 *ns = nullptr;
+    HIP_TRY(dalloc(&nt, static_cast<size_t>(cap) * c->max_nodes));
+    HIP_TRY(dalloc(&ns, cap));
+    if (c->trees_out) {
+        HIP_TRY(hipMemcpy(nt, c->trees_out, static_cast<size_t>(c->trees_cap) * c->max_nodes * sizeof(DevNode),
+                          hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(ns, c->tree_sizes, c->trees_cap * sizeof(u32), hipMemcpyDeviceToDevice));
+        hipFree(c->trees_out);
+        hipFree(c->tree_sizes);
+    }
+    c->trees_out = nt;
+    c->tree_sizes = ns;
+    c->trees_cap = cap;
+    // the captured graph holds the old pointers
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    return 0;
+}
+
+void const *gh_src(bh_context *c, int which)
+{
+    if (c->quant) return which == 2 ? static_cast<void const *>(c->gh0q) : c->ghwq[which];
+    return which == 2 ? static_cast<void const *>(c->gh0f) : c->ghwf[which];
+}
+
+hipError_t host_sync_point(bh_context *c)
+{
+    // The reference's blocking fetch after find / partition (device_context.cu:862,1038):
+    // a small D2H copy and a stream synchronisation.
+    TreeCtl tmp;
+    hipError_t e = hipMemcpyAsync(&tmp, c->ctl, sizeof(TreeCtl), hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+
+// Histogram units -> partials / slots, then the fixed-order completion.
+void enqueue_build(bh_context *c, void *hist_cur, void *hist_next, u32 max_ops, u32 grid_units)
+{
+    launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
+                gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), grid_units,
+                c->partials, hist_next);
+    launch_finish(c->stream, c->quant, c->fops, c->ctl, c->partials, hist_cur, hist_next, c->node_cells(), max_ops);
+}
+
+u32 hist_grid(bh_context *c) { return std::max(1u, std::min(c->max_hch, 4096u)); }
+u32 part_grid(bh_context *c) { return std::max(1u, std::min(c->max_pch, 2048u)); }
+
+// DepthwiseGrower::grow (src/grower_impl.hpp:501-577) as a device launch sequence.
+void enqueue_depthwise(bh_context *c)
+{
+    hipStream_t s = c->stream;
+    u32 const D = static_cast<u32>(c->fc.max_depth);
+    u32 const gh_bytes = c->quant ? 4 : 8;
+    launch_tree_begin(s, c->ctl, c->front[0], c->tree, c->hch, c->fops, static_cast<u32>(c->N));
+    enqueue_build(c, c->hist[1], c->hist[0], 1, hist_grid(c));
+    for (u32 d = 0; d < D; ++d) {
+        NodeInfo *cur = c->front[d & 1u], *nxt = c->front[(d + 1) & 1u];
+        void *hcur = c->hist[d & 1u], *hnext = c->hist[(d + 1) & 1u];
+        u32 const width = 1u << std::min(d, 14u);
+        bool const last = d + 1 == D;
+        launch_find(s, c->quant, cur, c->ctl, hcur, c->d_nbins, c->F, c->SB, c->gc, c->scales, c->splits, width, 0, 0);
+        launch_plan_level(s, cur, c->splits, c->ctl, c->tree, c->ops, c->pch, c->d_nbins, c->gc, c->scales, d & 1u);
+        if (c->fc.host_sync) host_sync_point(c);
+        launch_part_count(s, c->pch, c->ops, c->ctl, c->work[0], c->work[1], c->plane0, c->Fs, c->chunk_left,
+                          part_grid(c));
+        launch_part_scan(s, c->quant, c->ops, c->pch, c->chunk_left, c->ctl, cur, nxt, c->splits, c->tree, c->hch,
+                         c->fops, c->gc, c->scales, last ? 1u : 0u, 0u);
+        if (c->fc.host_sync) host_sync_point(c);
+        if (!last) {
+            launch_part_scatter(s, c->pch, c->ops, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
+                                gh_src(c, 1), gh_src(c, 2), c->work[0], c->work[1],
+                                const_cast<void *>(gh_src(c, 0)), const_cast<void *>(gh_src(c, 1)), c->Fs,
+                                gh_bytes, part_grid(c));
+            enqueue_build(c, hcur, hnext, std::min(width, c->max_ops), hist_grid(c));
+        }
+        launch_next_level(s, c->ctl);
+        if (c->fc.host_sync) host_sync_point(c);
+    }
+    launch_finalize(s, c->front[D & 1u], c->ctl, c->tree, c->gc, c->scales, 1u << std::min(D, 14u));
+}
+
+void enqueue_grow(bh_context *c)
+{
+    if (c->fc.grower == BH_DEPTHWISE) enqueue_depthwise(c);
+    else if (c->fc.grower == BH_LEVELWISE) enqueue_levelwise(c->ext, c->stream, c->quant, grow_view(c));
+    else enqueue_leafwise(c->ext, c->stream, c->quant, grow_view(c));
+}
+
+void enqueue_gradients(bh_context *c)
+{
+    if (c->quant)
+        launch_grad_quant(c->stream, c->score, c->label, c->N, c->fc.objective, c->maxbits, c->fc.seed, c->ctl,
+                          c->gh0q, c->scales);
+    else
+        launch_grad_f32(c->stream, c->score, c->label, c->N, c->fc.objective, c->gh0f);
+}
+
+// One boosting iteration: gradients, tree, score epilogue, tree copy-out.
+void enqueue_iteration(bh_context *c)
+{
+    enqueue_gradients(c);
+    enqueue_grow(c);
+    launch_score_update(c->stream, c->plane0, c->Fs, c->N, c->tree, c->ctl, c->d_nbins, c->F, c->fc.learning_rate,
+                        c->score, nullptr);
+    launch_copy_tree(c->stream, c->tree, c->ctl, c->trees_out, c->tree_sizes, c->max_nodes);
+    launch_iter_advance(c->stream, c->ctl);
+}
+
+int check_ready(bh_context *c)
+{
+    if (!c) return fail(1, "null context");
+    if (!c->plane0) return fail(1, "no bins uploaded");
+    if (!c->configured) return fail(1, "not configured");
+    return 0;
+}
+
+} // namespace
+
+// View of the context the leafwise / levelwise loops (kernels_ext.hip) need.
+GrowView grow_view(bh_context *c)
+{
+    GrowView v;
+    v.N = c->N;
+    v.F = c->F;
+    v.Fs = c->Fs;
+    v.SB = c->SB;
+    v.plane0 = c->plane0;
+    v.work[0] = c->work[0];
+    v.work[1] = c->work[1];
+    v.gh[0] = gh_src(c, 0);
+    v.gh[1] = gh_src(c, 1);
+    v.gh[2] = gh_src(c, 2);
+    v.d_nbins = c->d_nbins;
+    v.hist[0] = c->hist[0];
+    v.hist[1] = c->hist[1];
+    v.partials = c->partials;
+    v.ctl = c->ctl;
+    v.front[0] = c->front[0];
+    v.front[1] = c->front[1];
+    v.splits = c->splits;
+    v.tree = c->tree;
+    v.ops = c->ops;
+    v.pch = c->pch;
+    v.chunk_left = c->chunk_left;
+    v.hch = c->hch;
+    v.fops = c->fops;
+    v.scales = c->scales;
+    v.gc = c->gc;
+    v.fc = c->fc;
+    v.max_pch = c->max_pch;
+    v.max_hch = c->max_hch;
+    v.max_ops = c->max_ops;
+    v.fsl = c->hist_fsl();
+    v.waves = c->hist_waves();
+    v.host_sync = c->fc.host_sync != 0;
+    return v;
+}
+
+extern "C" {
+
+const char *bh_last_error(void) { return g_err.c_str(); }
+
+int bh_device_count(int *count)
+{
+    hipError_t const e = hipGetDeviceCount(count);
+    if (e != hipSuccess) {
+        *count = 0;
+        return fail(-1, std::string("hip: hipGetDeviceCount: ") + hipGetErrorString(e));
+    }
+    return 0;
+}
+
+int bh_create(int device, bh_context **out)
+{
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(-1, "hip: no device");
+    if (device < 0 || device >= n) return fail(1, "device id out of range");
+    HIP_TRY(hipSetDevice(device));
+    auto *c = new bh_context();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(-1, "hip: stream");
+    }
+    hipEventCreate(&c->ev0);
+    hipEventCreate(&c->ev1);
+    e = set_hist_lds_limits();
+    if (e == hipSuccess) e = ext_set_limits();
+    if (e != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return fail(-1, std::string("hip: LDS limit: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return 0;
+}
+
+int bh_destroy(bh_context *c)
+{
+    if (!c) return 0;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    free_fit(c);
+    void *ptrs[] = {c->plane0, c->work[0], c->work[1], c->d_nbins, c->label, c->score};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int bh_upload_bins(bh_context *c, const uint8_t *bins, uint64_t n_rows, uint32_t n_features, uint64_t row_stride,
+                   const uint16_t *n_bins)
+{
+    if (!c) return fail(1, "null context");
+    if (n_rows == 0 || n_features == 0) return fail(1, "empty plane");
+    if (n_rows >= 0xFFFFFFF0ull) return fail(1, "n_rows exceeds the 32-bit row index");
+    if (row_stride < n_features) return fail(1, "row_stride < n_features");
+    u32 SB = 0;
+    for (u32 f = 0; f < n_features; ++f) {
+        if (n_bins[f] < 1 || n_bins[f] > 256) return fail(1, "n_bins must be in [1, 256] (u8 bins)");
+        SB = std::max<u32>(SB, n_bins[f]);
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    hipStreamSynchronize(c->stream);
+    free_fit(c);
+    void *ptrs[] = {c->plane0, c->work[0], c->work[1], c->d_nbins, c->label, c->score};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    c->plane0 = c->work[0] = c->work[1] = nullptr;
+    c->d_nbins = nullptr;
+    c->label = c->score = nullptr;
+    c->have_labels = false;
+    c->N = n_rows;
+    c->F = n_features;
+    c->Fs = round_up4(n_features);
+    c->SB = SB;
+    c->n_bins.assign(n_bins, n_bins + n_features);
+    size_t const bytes = static_cast<size_t>(n_rows) * c->Fs;
+    HIP_TRY(hipMalloc(&c->plane0, bytes));
+    HIP_TRY(hipMalloc(&c->work[0], bytes));
+    HIP_TRY(hipMalloc(&c->work[1], bytes));
+    HIP_TRY(hipMemset(c->plane0, 0, bytes));
+    if (row_stride == c->Fs) {
+        HIP_TRY(hipMemcpy(c->plane0, bins, bytes, hipMemcpyHostToDevice));
+    } else {
+        HIP_TRY(hipMemcpy2D(c->plane0, c->Fs, bins, row_stride, n_features, n_rows, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(dalloc(&c->d_nbins, n_features));
+    HIP_TRY(hipMemcpy(c->d_nbins, n_bins, n_features * sizeof(u16), hipMemcpyHostToDevice));
+    HIP_TRY(dalloc(&c->label, n_rows));
+    HIP_TRY(dalloc(&c->score, n_rows));
+    HIP_TRY(hipMemset(c->label, 0, n_rows * sizeof(float)));
+    HIP_TRY(hipMemset(c->score, 0, n_rows * sizeof(float)));
+    return 0;
+}
+
+int bh_upload_labels(bh_context *c, const float *labels)
+{
+    if (!c || !c->plane0) return fail(1, "no bins uploaded");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMemcpy(c->label, labels, c->N * sizeof(float), hipMemcpyHostToDevice));
+    c->have_labels = true;
+    return 0;
+}
+
+int bh_configure(bh_context *c, const bh_fit_config *cfg)
+{
+    if (!c || !c->plane0) return fail(1, "no bins uploaded");
+    if (cfg->grower < 0 || cfg->grower > 2) return fail(1, "unknown grower");
+    if (cfg->objective != BH_MSE && cfg->objective != BH_LOGLOSS) return fail(1, "unknown objective");
+    if (cfg->max_depth < 1 || cfg->max_depth > 12) return fail(1, "max_depth must be in [1, 12]");
+    if (cfg->min_data_in_leaf < 0) return fail(1, "min_data_in_leaf < 0");
+    if (cfg->grower == BH_LEAFWISE && (cfg->max_leaves < 2 || cfg->max_leaves > 1024))
+        return fail(1, "max_leaves must be in [2, 1024] for the leafwise grower");
+    if (cfg->grower == BH_LEVELWISE && cfg->max_depth > 11) return fail(1, "levelwise max_depth must be <= 11");
+    HIP_TRY(hipSetDevice(c->device));
+    hipStreamSynchronize(c->stream);
+    free_fit(c);
+    c->fc = *cfg;
+    return alloc_fit(c);
+}
+
+int bh_set_scores(bh_context *c, const float *scores)
+{
+    if (int r = check_ready(c)) return r;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMemcpy(c->score, scores, c->N * sizeof(float), hipMemcpyHostToDevice));
+    // A new fit starts: iteration counter and tree store reset.
+    TreeCtl z{};
+    HIP_TRY(hipMemcpy(c->ctl, &z, sizeof(TreeCtl), hipMemcpyHostToDevice));
+    c->trees_done = 0;
+    return 0;
+}
+
+int bh_get_scores(bh_context *c, float *scores)
+{
+    if (int r = check_ready(c)) return r;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(scores, c->score, c->N * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int bh_set_timing(bh_context *c, int32_t enable)
+{
+    if (!c) return fail(1, "null context");
+    c->timing = enable != 0;
+    return 0;
+}
+
+int bh_get_timing(bh_context *c, float *ms, uint32_t n)
+{
+    if (!c) return fail(1, "null context");
+    for (u32 i = 0; i < n; ++i) ms[i] = i < c->iter_ms.size() ? c->iter_ms[i] : 0.0F;
+    return 0;
+}
+
+int bh_train(bh_context *c, uint32_t n_iters)
+{
+    if (int r = check_ready(c)) return r;
+    if (!c->have_labels) return fail(1, "no labels uploaded");
+    HIP_TRY(hipSetDevice(c->device));
+    if (int r = ensure_tree_out(c, n_iters)) return r;
+    c->iter_ms.clear();
+    if (c->fc.use_graph && !c->fc.host_sync) {
+        if (!c->graph_exec) {
+            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_iteration(c);
+            hipError_t const e = hipStreamEndCapture(c->stream, &c->graph);
+            if (e != hipSuccess) return fail(-1, std::string("hip: capture: ") + hipGetErrorString(e));
+            HIP_TRY(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+        }
+        for (u32 it = 0; it < n_iters; ++it) {
+            if (c->timing) HIP_TRY(hipEventRecord(c->ev0, c->stream));
+            HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+            if (c->timing) {
+                HIP_TRY(hipEventRecord(c->ev1, c->stream));
+                HIP_TRY(hipEventSynchronize(c->ev1));
+                float ms = 0.0F;
+                hipEventElapsedTime(&ms, c->ev0, c->ev1);
+                c->iter_ms.push_back(ms);
+            }
+        }
+    } else {
+        for (u32 it = 0; it < n_iters; ++it) {
+            if (c->timing) HIP_TRY(hipEventRecord(c->ev0, c->stream));
+            enqueue_iteration(c);
+            if (c->timing) {
+                HIP_TRY(hipEventRecord(c->ev1, c->stream));
+                HIP_TRY(hipEventSynchronize(c->ev1));
+                float ms = 0.0F;
+                hipEventElapsedTime(&ms, c->ev0, c->ev1);
+                c->iter_ms.push_back(ms);
+            }
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->trees_done += n_iters;
+    return 0;
+}
+
+int bh_max_nodes(bh_context *c, uint32_t *max_nodes)
+{
+    if (int r = check_ready(c)) return r;
+    *max_nodes = c->max_nodes;
+    return 0;
+}
+
+int bh_get_trees(bh_context *c, bh_node *nodes, uint32_t max_nodes, uint32_t *sizes, uint32_t cap_trees,
+                 uint32_t *n_trees)
+{
+    if (int r = check_ready(c)) return r;
+    if (max_nodes < c->max_nodes) return fail(1, "max_nodes smaller than the fit's tree capacity");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    u32 const n = std::min(cap_trees, c->trees_done);
+    std::vector<u32> sz(n);
+    if (n) HIP_TRY(hipMemcpy(sz.data(), c->tree_sizes, n * sizeof(u32), hipMemcpyDeviceToHost));
+    std::vector<DevNode> buf(c->max_nodes);
+    for (u32 t = 0; t < n; ++t) {
+        HIP_TRY(hipMemcpy(buf.data(), c->trees_out + static_cast<size_t>(t) * c->max_nodes,
+                          c->max_nodes * sizeof(DevNode), hipMemcpyDeviceToHost));
+        std::memcpy(nodes + static_cast<size_t>(t) * max_nodes, buf.data(), sz[t] * sizeof(DevNode));
+        sizes[t] = sz[t];
+    }
+    *n_trees = n;
+    return 0;
+}
+
+int bh_set_gradients(bh_context *c, const float *grad, const float *hess, uint32_t iter, int8_t *gq, int16_t *hq,
+                     double *scales)
+{
+    if (int r = check_ready(c)) return r;
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<float2> gh(c->N);
+    for (u64 i = 0; i < c->N; ++i) gh[i] = make_float2(grad[i], hess[i]);
+    if (!c->quant) {
+        HIP_TRY(hipMemcpy(c->gh0f, gh.data(), c->N * sizeof(float2), hipMemcpyHostToDevice));
+        return 0;
+    }
+    float2 *tmp = nullptr;
+    HIP_TRY(dalloc(&tmp, c->N));
+    HIP_TRY(hipMemcpy(tmp, gh.data(), c->N * sizeof(float2), hipMemcpyHostToDevice));
+    launch_quant_from(c->stream, tmp, c->N, c->maxbits, c->fc.seed, iter, c->gh0q, c->scales);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    hipFree(tmp);
+    u32 mb[2];
+    HIP_TRY(hipMemcpy(mb, c->maxbits, sizeof(mb), hipMemcpyDeviceToHost));
+    float mg, mh;
+    std::memcpy(&mg, &mb[0], 4);
+    std::memcpy(&mh, &mb[1], 4);
+    double const sc[2] = {static_cast<double>(mg) / 127.0, static_cast<double>(mh) / 32767.0};
+    HIP_TRY(hipMemcpy(c->scales, sc, sizeof(sc), hipMemcpyHostToDevice));
+    if (scales) {
+        scales[0] = sc[0];
+        scales[1] = sc[1];
+    }
+    if (gq || hq) {
+        std::vector<u32> q(c->N);
+        HIP_TRY(hipMemcpy(q.data(), c->gh0q, c->N * sizeof(u32), hipMemcpyDeviceToHost));
+        for (u64 i = 0; i < c->N; ++i) {
+            if (gq) gq[i] = static_cast<int8_t>(static_cast<int16_t>(q[i] >> 16));
+            if (hq) hq[i] = static_cast<int16_t>(q[i] & 0xFFFFu);
+        }
+    }
+    return 0;
+}
+
+int bh_grow_tree(bh_context *c, bh_node *nodes, uint32_t cap, uint32_t *n_nodes, uint32_t *leaf_of_row)
+{
+    if (int r = check_ready(c)) return r;
+    HIP_TRY(hipSetDevice(c->device));
+    enqueue_grow(c);
+    launch_score_update(c->stream, c->plane0, c->Fs, c->N, c->tree, c->ctl, c->d_nbins, c->F, 0.0F, nullptr,
+                        c->leaf_rows);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    TreeCtl ctl;
+    HIP_TRY(hipMemcpy(&ctl, c->ctl, sizeof(TreeCtl), hipMemcpyDeviceToHost));
+    if (ctl.n_nodes > cap) return fail(1, "node capacity too small");
+    HIP_TRY(hipMemcpy(nodes, c->tree, ctl.n_nodes * sizeof(DevNode), hipMemcpyDeviceToHost));
+    *n_nodes = ctl.n_nodes;
+    if (leaf_of_row) HIP_TRY(hipMemcpy(leaf_of_row, c->leaf_rows, c->N * sizeof(u32), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int bh_histogram(bh_context *c, const uint32_t *rows, uint64_t n, float *out_f32, int64_t *out_i64)
+{
+    if (int r = check_ready(c)) return r;
+    if (n > c->N) return fail(1, "more rows than the plane holds");
+    HIP_TRY(hipSetDevice(c->device));
+    u32 *d_rows = nullptr;
+    HIP_TRY(dalloc(&d_rows, n));
+    HIP_TRY(hipMemcpy(d_rows, rows, n * sizeof(u32), hipMemcpyHostToDevice));
+    u32 const gh_bytes = c->quant ? 4 : 8;
+    launch_gather_rows(c->stream, c->plane0, c->Fs, gh_src(c, 2), gh_bytes, d_rows, n, c->work[0],
+                       const_cast<void *>(gh_src(c, 0)), 0);
+    // One node over [0, n) of work[0]: units + finish op, as the level loop emits them.
+    u32 const units = static_cast<u32>((n + kHistChunk - 1) / kHistChunk);
+    std::vector<HistChunk> hc(units);
+    for (u32 k = 0; k < units; ++k) {
+        hc[k] = HistChunk{};
+        hc[k].slot = 0;
+        hc[k].src = 0;
+        hc[k].begin = k * kHistChunk;
+        hc[k].end = static_cast<u32>(std::min<u64>(n, static_cast<u64>(k + 1) * kHistChunk));
+        hc[k].part = units > 1 ? k : kNone;
+    }
+    FinishOp fo{};
+    fo.small_slot = 0;
+    fo.large_slot = kNone;
+    fo.part_count = units > 1 ? units : 0;
+    fo.empty = n == 0;
+    TreeCtl ctl{};
+    ctl.n_hist_chunks = units;
+    ctl.n_finish = 1;
+    if (units) HIP_TRY(hipMemcpy(c->hch, hc.data(), units * sizeof(HistChunk), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->fops, &fo, sizeof(fo), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->ctl, &ctl, sizeof(ctl), hipMemcpyHostToDevice));
+    enqueue_build(c, c->hist[0], c->hist[0], 1, std::max(1u, std::min(units, 4096u)));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    hipFree(d_rows);
+    size_t const nc = c->node_cells();
+    if (c->quant) {
+        if (!out_i64) return fail(1, "quantized fit: pass out_i64");
+        HIP_TRY(hipMemcpy(out_i64, c->hist[0], nc * 2 * sizeof(int64_t), hipMemcpyDeviceToHost));
+    } else {
+        if (!out_f32) return fail(1, "FP32 fit: pass out_f32");
+        HIP_TRY(hipMemcpy(out_f32, c->hist[0], nc * 2 * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int bh_find_split(bh_context *c, const float *hist_f32, const int64_t *hist_i64, uint64_t n_rows, uint32_t depth,
+                  int32_t tot_given, double tot_g, double tot_h, const double *scales, bh_split *out)
+{
+    if (int r = check_ready(c)) return r;
+    HIP_TRY(hipSetDevice(c->device));
+    size_t const nc = c->node_cells();
+    if (c->quant) {
+        if (!hist_i64) return fail(1, "quantized fit: pass hist_i64");
+        HIP_TRY(hipMemcpy(c->hist[0], hist_i64, nc * 2 * sizeof(int64_t), hipMemcpyHostToDevice));
+        if (scales) HIP_TRY(hipMemcpy(c->scales, scales, 2 * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        if (!hist_f32) return fail(1, "FP32 fit: pass hist_f32");
+        HIP_TRY(hipMemcpy(c->hist[0], hist_f32, nc * 2 * sizeof(float), hipMemcpyHostToDevice));
+    }
+    NodeInfo nd{};
+    nd.cnt = static_cast<u32>(n_rows);
+    nd.depth = depth;
+    nd.slot = 0;
+    if (tot_given) {
+        nd.totals_ready = 1;
+        nd.tg = static_cast<float>(tot_g);
+        nd.th = static_cast<float>(tot_h);
+        nd.itg = static_cast<i64>(tot_g);
+        nd.ith = static_cast<i64>(tot_h);
+    }
+    HIP_TRY(hipMemcpy(c->front[0], &nd, sizeof(nd), hipMemcpyHostToDevice));
+    launch_find(c->stream, c->quant, c->front[0], c->ctl, c->hist[0], c->d_nbins, c->F, c->SB, c->gc, c->scales,
+                c->splits, 1, 0, 1);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    SplitRec rec;
+    HIP_TRY(hipMemcpy(&rec, c->splits, sizeof(rec), hipMemcpyDeviceToHost));
+    static_assert(sizeof(SplitRec) == sizeof(bh_split), "bh_split layout");
+    std::memcpy(out, &rec, sizeof(rec));
+    return 0;
+}
+
+int bh_partition(bh_context *c, const uint32_t *rows, uint64_t n, uint32_t feature, uint32_t bin,
+                 uint32_t default_left, uint32_t *out_rows, uint64_t *n_left)
+{
+    if (int r = check_ready(c)) return r;
+    if (n == 0 || n > c->N) return fail(1, "row count out of range");
+    if (feature >= c->F) return fail(1, "feature out of range");
+    HIP_TRY(hipSetDevice(c->device));
+    u32 *d_rows = nullptr;
+    HIP_TRY(dalloc(&d_rows, n));
+    HIP_TRY(hipMemcpy(d_rows, rows, n * sizeof(u32), hipMemcpyHostToDevice));
+    u32 const gh_bytes = c->quant ? 4 : 8;
+    // Rows into work[0] with their ids as the travelling gradient pair.
+    launch_gather_rows(c->stream, c->plane0, c->Fs, gh_src(c, 2), gh_bytes, d_rows, n, c->work[0],
+                       const_cast<void *>(gh_src(c, 0)), 1);
+    NodeInfo nd{};
+    nd.id = 0;
+    nd.cnt = static_cast<u32>(n);
+    nd.src = 0;
+    nd.totals_ready = 1;
+    SplitRec s{};
+    s.valid = 1;
+    s.feature = feature;
+    s.bin = bin;
+    s.default_left = default_left;
+    TreeCtl ctl{};
+    ctl.n_front = 1;
+    ctl.n_nodes = 1;
+    DevNode root{};
+    root.feature = -1;
+    HIP_TRY(hipMemcpy(c->front[0], &nd, sizeof(nd), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->splits, &s, sizeof(s), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->ctl, &ctl, sizeof(ctl), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->tree, &root, sizeof(root), hipMemcpyHostToDevice));
+    GrowCfg gc = c->gc;
+    launch_plan_level(c->stream, c->front[0], c->splits, c->ctl, c->tree, c->ops, c->pch, c->d_nbins, gc, c->scales,
+                      1);
+    launch_part_count(c->stream, c->pch, c->ops, c->ctl, c->work[0], c->work[1], c->plane0, c->Fs, c->chunk_left,
+                      part_grid(c));
+    // oblivious = 1: no demotion, so an all-one-side split still scatters.
+    launch_part_scan(c->stream, c->quant, c->ops, c->pch, c->chunk_left, c->ctl, c->front[0], c->front[1],
+                     c->splits, c->tree, c->hch, c->fops, gc, c->scales, 1, 1);
+    launch_part_scatter(c->stream, c->pch, c->ops, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
+                        gh_src(c, 1), gh_src(c, 2), c->work[0], c->work[1], const_cast<void *>(gh_src(c, 0)),
+                        const_cast<void *>(gh_src(c, 1)), c->Fs, gh_bytes, part_grid(c));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    hipFree(d_rows);
+    PartOp op;
+    HIP_TRY(hipMemcpy(&op, c->ops, sizeof(op), hipMemcpyDeviceToHost));
+    *n_left = op.n_left;
+    if (gh_bytes == 8) {
+        std::vector<uint2> v(n);
+        HIP_TRY(hipMemcpy(v.data(), gh_src(c, 1), n * sizeof(uint2), hipMemcpyDeviceToHost));
+        for (u64 i = 0; i < n; ++i) out_rows[i] = v[i].x;
+    } else {
+        HIP_TRY(hipMemcpy(out_rows, gh_src(c, 1), n * sizeof(u32), hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int bh_time_root_histogram(bh_context *c, uint32_t reps, float *ms_mean)
+{
+    if (int r = check_ready(c)) return r;
+    HIP_TRY(hipSetDevice(c->device));
+    // Root units over the pristine plane and the current gradients.
+    launch_tree_begin(c->stream, c->ctl, c->front[0], c->tree, c->hch, c->fops, static_cast<u32>(c->N));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float total = 0.0F;
+    for (u32 r = 0; r < reps; ++r) {
+        HIP_TRY(hipEventRecord(c->ev0, c->stream));
+        launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
+                    gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), hist_grid(c),
+                    c->partials, c->hist[0]);
+        HIP_TRY(hipEventRecord(c->ev1, c->stream));
+        HIP_TRY(hipEventSynchronize(c->ev1));
+        float ms = 0.0F;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        total += ms;
+    }
+    *ms_mean = reps ? total / static_cast<float>(reps) : 0.0F;
+    return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,1339 @@
+// Training kernels of the MI355X GBDT engine (gfx950, wave64).
+//
+// Every kernel is launched with a fixed grid and reads its work list from device
+// memory (TreeCtl + op/chunk arrays), so a whole boosting iteration can be replayed
+// from one hipGraph with no host round-trip.  Work lists are produced on device by the
+// single-workgroup control kernels (k_plan_level, k_part_scan, ...).
+//
+// Reference behaviour each kernel reproduces is cited per kernel.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include "bh_common.hpp"
+#include "kernels.hpp"
+
+namespace bh {
+
+// ---------------------------------------------------------------------------------
+// Shared device helpers
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ bool routes_left(u32 bin, u32 last, u32 split, u32 dl)
+{
+    // include/bonsai/dataset.hpp:234-238
+    return bin == last ? (dl != 0) : (bin <= split);
+}
+
+__device__ __forceinline__ double l1_thr(double g, double l1)
+{
+    if (g > l1) return g - l1;
+    if (g < -l1) return g + l1;
+    return 0.0;
+}
+// include/bonsai/split.hpp:135-143
+__device__ __forceinline__ double score_of(double g, double h, double l1, double l2)
+{
+    double const t = l1_thr(g, l1);
+    double const d = h + l2;
+    return d > 0.0 ? (t * t) / d : 0.0;
+}
+// include/bonsai/split.hpp:146-152 (bounds are +-inf: no monotone constraints)
+__device__ __forceinline__ float leaf_weight(double g, double h, double l1, double l2)
+{
+    double const d = h + l2;
+    double const w = d > 0.0 ? -l1_thr(g, l1) / d : 0.0;
+    return static_cast<float>(w);
+}
+
+__device__ __forceinline__ u64 splitmix64(u64 z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ float sr_uniform(u64 seed, u32 iter, u64 row, u32 channel)
+{
+    u64 const key = seed ^ (static_cast<u64>(iter) * 0xD1B54A32D192ED03ULL) ^
+                    (row * 0x9E3779B97F4A7C15ULL) ^
+                    (static_cast<u64>(channel + 1) * 0xA0761D6478BD642FULL);
+    u64 const z = splitmix64(key);
+    return static_cast<float>(static_cast<u32>(z >> 40)) * (1.0F / 16777216.0F);
+}
+
+// Block-wide exclusive scan of one u32 per thread (blockDim.x <= 1024).  Returns the
+// exclusive prefix; *total receives the block sum.  Uses `scratch` of blockDim.x u32.
+__device__ u32 block_exscan(u32 v, u32 *scratch, u32 *total)
+{
+    u32 const t = threadIdx.x;
+    scratch[t] = v;
+    __syncthreads();
+    for (u32 s = 1; s < blockDim.x; s <<= 1) {
+        u32 const add = t >= s ? scratch[t - s] : 0u;
+        __syncthreads();
+        scratch[t] += add;
+        __syncthreads();
+    }
+    u32 const incl = scratch[t];
+    *total = scratch[blockDim.x - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__device__ __forceinline__ void write_leaf(DevNode *tree, u32 id, float v)
+{
+    DevNode n = tree[id];
+    n.feature = -1;
+    n.bin = 0;
+    n.default_left = 0;
+    n.left = 0;
+    n.right = 0;
+    n.value = v;
+    tree[id] = n;
+}
+
+__device__ __forceinline__ float node_leaf_value(NodeInfo const &n, GrowCfg const &c,
+                                                 double const *scales)
+{
+    if (c.quantized)
+        return leaf_weight(static_cast<double>(n.itg) * scales[0],
+                           static_cast<double>(n.ith) * scales[1], c.l1, c.l2);
+    return leaf_weight(n.tg, n.th, c.l1, c.l2);
+}
+
+// ---------------------------------------------------------------------------------
+// Objective: src/objective.cpp:44-101 (mse, logloss); device twin of
+// gh_from_scores_kernel (src/cuda/detail/kernels.cuh:972-1009) fused with the
+// per-iteration quantizer.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ void grad_of(int objective, float s, float y, float &g, float &h)
+{
+    if (objective == 1) {
+        float const p = 1.0F / (1.0F + expf(-s));
+        g = p - y;
+        h = p * (1.0F - p);
+    } else {
+        g = s - y;
+        h = 1.0F;
+    }
+}
+
+__global__ void k_grad_f32(float const *__restrict__ score, float const *__restrict__ label,
+                           u64 n, int objective, float2 *__restrict__ gh)
+{
+    for (u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; r < n;
+         r += static_cast<u64>(gridDim.x) * blockDim.x) {
+        float g, h;
+        grad_of(objective, score[r], label[r], g, h);
+        gh[r] = make_float2(g, h);
+    }
+}
+
+__global__ void k_zero_u32(u32 *p, u32 n)
+{
+    for (u32 i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+// max |g| and max h as float bits (non-negative floats order like their bit patterns,
+// so the max is independent of the reduction order).
+__global__ void k_grad_max(float const *__restrict__ score, float const *__restrict__ label,
+                           u64 n, int objective, u32 *__restrict__ maxbits)
+{
+    float mg = 0.0F, mh = 0.0F;
+    for (u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; r < n;
+         r += static_cast<u64>(gridDim.x) * blockDim.x) {
+        float g, h;
+        grad_of(objective, score[r], label[r], g, h);
+        mg = fmaxf(mg, fabsf(g));
+        mh = fmaxf(mh, fabsf(h));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mg = fmaxf(mg, __shfl_xor(mg, o));
+        mh = fmaxf(mh, __shfl_xor(mh, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&maxbits[0], __float_as_uint(mg));
+        atomicMax(&maxbits[1], __float_as_uint(mh));
+    }
+}
+
+// Quantizer shared with the oracle (oracle/oracle.cpp orc_quantize): int8 g / int16 h,
+// stochastic rounding q = floor(x * (qmax / max) + u), u from a counter RNG keyed on
+// (seed, iteration, original row, channel).
+__global__ void k_grad_quant(float const *__restrict__ score, float const *__restrict__ label,
+                             u64 n, int objective, u32 const *__restrict__ maxbits, u64 seed,
+                             TreeCtl const *__restrict__ ctl, u32 *__restrict__ ghq,
+                             double *__restrict__ scales)
+{
+    float const mg = __uint_as_float(maxbits[0]);
+    float const mh = __uint_as_float(maxbits[1]);
+    float const ig = mg > 0.0F ? 127.0F / mg : 0.0F;
+    float const ih = mh > 0.0F ? 32767.0F / mh : 0.0F;
+    u32 const iter = ctl->iter;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scales[0] = static_cast<double>(mg) / 127.0;
+        scales[1] = static_cast<double>(mh) / 32767.0;
+    }
+    for (u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; r < n;
+         r += static_cast<u64>(gridDim.x) * blockDim.x) {
+        float g, h;
+        grad_of(objective, score[r], label[r], g, h);
+        float const tg = g * ig + sr_uniform(seed, iter, r, 0);
+        float const th = h * ih + sr_uniform(seed, iter, r, 1);
+        float qg = fminf(127.0F, fmaxf(-127.0F, floorf(tg)));
+        float qh = fminf(32767.0F, fmaxf(0.0F, floorf(th)));
+        int const ig8 = static_cast<int>(qg);
+        int const ih16 = static_cast<int>(qh);
+        ghq[r] = (static_cast<u32>(static_cast<u16>(static_cast<int16_t>(ig8))) << 16) |
+                 static_cast<u32>(static_cast<u16>(ih16));
+    }
+}
+
+// Host-supplied float gradients quantized with the same rule (parity tests).
+__global__ void k_quant_from(float2 const *__restrict__ gh, u64 n, u32 const *__restrict__ maxbits,
+                             u64 seed, u32 iter, u32 *__restrict__ ghq)
+{
+    float const mg = __uint_as_float(maxbits[0]);
+    float const mh = __uint_as_float(maxbits[1]);
+    float const ig = mg > 0.0F ? 127.0F / mg : 0.0F;
+    float const ih = mh > 0.0F ? 32767.0F / mh : 0.0F;
+    for (u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; r < n;
+         r += static_cast<u64>(gridDim.x) * blockDim.x) {
+        float2 const v = gh[r];
+        float const tg = v.x * ig + sr_uniform(seed, iter, r, 0);
+        float const th = v.y * ih + sr_uniform(seed, iter, r, 1);
+        float qg = fminf(127.0F, fmaxf(-127.0F, floorf(tg)));
+        float qh = fminf(32767.0F, fmaxf(0.0F, floorf(th)));
+        ghq[r] = (static_cast<u32>(static_cast<u16>(static_cast<int16_t>(static_cast<int>(qg))))
+                  << 16) |
+                 static_cast<u32>(static_cast<u16>(static_cast<int>(qh)));
+    }
+}
+
+__global__ void k_gh_max(float2 const *__restrict__ gh, u64 n, u32 *__restrict__ maxbits)
+{
+    float mg = 0.0F, mh = 0.0F;
+    for (u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; r < n;
+         r += static_cast<u64>(gridDim.x) * blockDim.x) {
+        float2 const v = gh[r];
+        mg = fmaxf(mg, fabsf(v.x));
+        mh = fmaxf(mh, fabsf(v.y));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mg = fmaxf(mg, __shfl_xor(mg, o));
+        mh = fmaxf(mh, __shfl_xor(mh, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&maxbits[0], __float_as_uint(mg));
+        atomicMax(&maxbits[1], __float_as_uint(mh));
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Histogram build (H6).  Device counterpart of hist_tile_kernel
+// (src/cuda/detail/kernels.cuh:186-277); CPU semantics of populate
+// (src/grower.cpp:93-139).
+//
+// One workgroup per work unit (a chunk of <= kHistChunk rows of one node's contiguous
+// segment) and feature slice.  The slice's sub-histogram lives in LDS.  Wave w owns
+// the 4-feature groups g = w, w+NW, ...: it reads one dword of every row of the chunk
+// (4 bins), in ascending row order, and is the only wave that touches those cells.
+// FP32: two ds_add_f32 per (row, feature), so every cell sums its rows in ascending
+// order (fixed order, no inter-wave races).  Quantized: one ds_add_u64 per
+// (row, feature) on the packed (g << 32) + h word, exact and order-free.
+// The unit's LDS histogram is written once to a partial (multi-unit nodes) or straight
+// into the node's slot (single-unit nodes).
+// ---------------------------------------------------------------------------------
+template <bool Q, int U>
+__global__ __launch_bounds__(1024) void k_hist(
+    HistChunk const *__restrict__ chunks, TreeCtl const *__restrict__ ctl,
+    u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
+    void const *__restrict__ gw0, void const *__restrict__ gw1, void const *__restrict__ gwp,
+    u32 Fs, u32 F, u32 SB, u32 fsl, void *__restrict__ partials, void *__restrict__ hist_next)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    u32 const f_begin = blockIdx.y * fsl;
+    if (f_begin >= F) return;
+    u32 const nf = min(F - f_begin, fsl);
+    u32 const groups = (nf + 3u) / 4u;
+    u32 const wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    u32 const NW = blockDim.x >> 6;
+    u32 const cells = nf * SB;
+    u32 const n_units = ctl->n_hist_chunks;
+    size_t const node_cells = static_cast<size_t>(F) * SB;
+
+    for (u32 c = blockIdx.x; c < n_units; c += gridDim.x) {
+        HistChunk const ch = chunks[c];
+        if (Q) {
+            u64 *lh = reinterpret_cast<u64 *>(smem);
+            for (u32 i = threadIdx.x; i < cells; i += blockDim.x) lh[i] = 0ull;
+        } else {
+            float *lg = reinterpret_cast<float *>(smem);
+            for (u32 i = threadIdx.x; i < 2u * cells; i += blockDim.x) lg[i] = 0.0F;
+        }
+        __syncthreads();
+        u8 const *plane = ch.src == 0 ? pl0 : (ch.src == 1 ? pl1 : plp);
+        void const *ghv = ch.src == 0 ? gw0 : (ch.src == 1 ? gw1 : gwp);
+        for (u32 base = ch.begin; base < ch.end; base += 64u * U) {
+            u32 rows[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                rows[u] = base + u * 64u + lane;
+                ok[u] = rows[u] < ch.end;
+            }
+            float gf[U], hf[U];
+            u64 pk[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (Q) {
+                    u32 const v = ok[u] ? reinterpret_cast<u32 const *>(ghv)[rows[u]] : 0u;
+                    i64 const g = static_cast<int16_t>(v >> 16);
+                    u64 const h = v & 0xFFFFu;
+                    pk[u] = static_cast<u64>(g * 4294967296LL) + h;
+                } else {
+                    float2 const v = ok[u] ? reinterpret_cast<float2 const *>(ghv)[rows[u]]
+                                           : make_float2(0.0F, 0.0F);
+                    gf[u] = v.x;
+                    hf[u] = v.y;
+                }
+            }
+            for (u32 g = wave; g < groups; g += NW) {
+                u32 d[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    d[u] = ok[u] ? *reinterpret_cast<u32 const *>(
+                                       plane + static_cast<size_t>(rows[u]) * Fs + f_begin + 4u * g)
+                                 : 0u;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (!ok[u]) continue;
+#pragma unroll
+                    for (u32 k = 0; k < 4; ++k) {
+                        u32 const fl = 4u * g + k;
+                        if (fl >= nf) break;
+                        u32 const cell = fl * SB + ((d[u] >> (8u * k)) & 0xFFu);
+                        if (Q) {
+                            atomicAdd(reinterpret_cast<unsigned long long *>(smem) + cell,
+                                      static_cast<unsigned long long>(pk[u]));
+                        } else {
+                            float *lg = reinterpret_cast<float *>(smem);
+                            atomicAdd(lg + cell, gf[u]);
+                            atomicAdd(lg + cells + cell, hf[u]);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // Flush: partial (packed) or the node's own slot (final cells).
+        if (Q) {
+            u64 const *lh = reinterpret_cast<u64 const *>(smem);
+            if (ch.part != kNone) {
+                u64 *dst = reinterpret_cast<u64 *>(partials) + static_cast<size_t>(ch.part) * node_cells +
+                           static_cast<size_t>(f_begin) * SB;
+                for (u32 i = threadIdx.x; i < cells; i += blockDim.x) dst[i] = lh[i];
+            } else {
+                i64 *dst = reinterpret_cast<i64 *>(hist_next) +
+                           (static_cast<size_t>(ch.slot) * node_cells + static_cast<size_t>(f_begin) * SB) * 2;
+                for (u32 i = threadIdx.x; i < cells; i += blockDim.x) {
+                    u64 const v = lh[i];
+                    dst[2 * i] = static_cast<int32_t>(static_cast<u32>(v >> 32));
+                    dst[2 * i + 1] = static_cast<i64>(v & 0xFFFFFFFFull);
+                }
+            }
+        } else {
+            float const *lg = reinterpret_cast<float const *>(smem);
+            float2 *dst = ch.part != kNone
+                              ? reinterpret_cast<float2 *>(partials) + static_cast<size_t>(ch.part) * node_cells +
+                                    static_cast<size_t>(f_begin) * SB
+                              : reinterpret_cast<float2 *>(hist_next) + static_cast<size_t>(ch.slot) * node_cells +
+                                    static_cast<size_t>(f_begin) * SB;
+            for (u32 i = threadIdx.x; i < cells; i += blockDim.x) dst[i] = make_float2(lg[i], lg[cells + i]);
+        }
+        __syncthreads();
+    }
+}
+
+// Fixed-order completion of built nodes (H6 reduce + H7 sibling subtraction,
+// src/step/primitives.hpp:156-173; device twin subtract_kernel, kernels.cuh:507-520):
+// small = p_0 + p_1 + ... in unit order, large = parent - small.
+template <bool Q>
+__global__ void k_finish(FinishOp const *__restrict__ ops, TreeCtl const *__restrict__ ctl,
+                         void const *__restrict__ partials, void const *__restrict__ hist_cur,
+                         void *__restrict__ hist_next, u32 node_cells)
+{
+    u32 const n_ops = ctl->n_finish;
+    for (u32 o = blockIdx.y; o < n_ops; o += gridDim.y) {
+        FinishOp const op = ops[o];
+        for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < node_cells; i += gridDim.x * blockDim.x) {
+            if (Q) {
+                i64 *nx = reinterpret_cast<i64 *>(hist_next);
+                i64 g = 0, h = 0;
+                if (op.empty) {
+                    nx[(static_cast<size_t>(op.small_slot) * node_cells + i) * 2] = 0;
+                    nx[(static_cast<size_t>(op.small_slot) * node_cells + i) * 2 + 1] = 0;
+                } else if (op.part_count == 0) {
+                    g = nx[(static_cast<size_t>(op.small_slot) * node_cells + i) * 2];
+                    h = nx[(static_cast<size_t>(op.small_slot) * node_cells + i) * 2 + 1];
+                } else {
+                    u64 const *pp = reinterpret_cast<u64 const *>(partials);
+                    for (u32 p = 0; p < op.part_count; ++p) {
+                        u64 const v = pp[static_cast<size_t>(op.part_first + p) * node_cells + i];
+                        g += static_cast<int32_t>(static_cast<u32>(v >> 32));
+                        h += static_cast<i64>(v & 0xFFFFFFFFull);
+                    }
+                    nx[(static_cast<size_t>(op.small_slot) * node_cells + i) * 2] = g;
+                    nx[(static_cast<size_t>(op.small_slot) * node_cells + i) * 2 + 1] = h;
+                }
+                if (op.large_slot != kNone) {
+                    i64 const *pc = reinterpret_cast<i64 const *>(hist_cur);
+                    size_t const pi = (static_cast<size_t>(op.parent_slot) * node_cells + i) * 2;
+                    size_t const li = (static_cast<size_t>(op.large_slot) * node_cells + i) * 2;
+                    nx[li] = pc[pi] - g;
+                    nx[li + 1] = pc[pi + 1] - h;
+                }
+            } else {
+                float2 *nx = reinterpret_cast<float2 *>(hist_next);
+                float2 s = make_float2(0.0F, 0.0F);
+                if (op.empty) {
+                    nx[static_cast<size_t>(op.small_slot) * node_cells + i] = s;
+                } else if (op.part_count == 0) {
+                    s = nx[static_cast<size_t>(op.small_slot) * node_cells + i];
+                } else {
+                    float2 const *pp = reinterpret_cast<float2 const *>(partials);
+                    for (u32 p = 0; p < op.part_count; ++p) {
+                        float2 const v = pp[static_cast<size_t>(op.part_first + p) * node_cells + i];
+                        s.x += v.x;
+                        s.y += v.y;
+                    }
+                    nx[static_cast<size_t>(op.small_slot) * node_cells + i] = s;
+                }
+                if (op.large_slot != kNone) {
+                    float2 const pv =
+                        reinterpret_cast<float2 const *>(hist_cur)[static_cast<size_t>(op.parent_slot) * node_cells + i];
+                    nx[static_cast<size_t>(op.large_slot) * node_cells + i] =
+                        make_float2(pv.x - s.x, pv.y - s.y);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Node split finder (H8): device twin of find_kernel + reduce_kernel
+// (src/cuda/detail/kernels.cuh:586-754) with the CPU finder's exact semantics
+// (src/split.cpp:33-162, :255-297): double running sums in bin order, dl = true before
+// false, strict > (earliest bin, then lowest feature, wins ties), an empty missing cell
+// scans one direction, rows < 2 * min_data_in_leaf cannot split.
+//
+// One workgroup per frontier node.  Phase 0 derives the node totals from its first
+// feature's cells (Histogram::totals, include/bonsai/histogram.hpp:76-87) unless a
+// parent split already set them.  Phase A: one thread per feature of a 16-feature tile
+// runs the sequential prefix into LDS.  Phase B: every (feature, bin, direction)
+// candidate is scored in parallel; phase C reduces to the winner with the serial
+// scan's tie order.
+// ---------------------------------------------------------------------------------
+constexpr u32 kFindTile = 12;
+
+template <bool Q>
+__global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, TreeCtl const *__restrict__ ctl,
+                                              void const *__restrict__ hist, u16 const *__restrict__ n_bins,
+                                              u32 F, u32 SB, GrowCfg cfg, double const *__restrict__ scales,
+                                              SplitRec *__restrict__ out, u32 front_base, u32 n_front_override)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    u32 const n_front = n_front_override ? n_front_override : ctl->n_front;
+    u32 const i = front_base + blockIdx.x;
+    if (blockIdx.x >= n_front) return;
+    __shared__ double s_best_gain[256];
+    __shared__ u32 s_best_key[256];
+    __shared__ double s_tot[2];
+    __shared__ i64 s_itot[2];
+    NodeInfo node = front[i];
+    size_t const node_cells = static_cast<size_t>(F) * SB;
+    double const sg = Q ? scales[0] : 0.0, sh = Q ? scales[1] : 0.0;
+
+    if (threadIdx.x == 0) {
+        if (!node.totals_ready) {
+            if (Q) {
+                i64 const *c = reinterpret_cast<i64 const *>(hist) + static_cast<size_t>(node.slot) * node_cells * 2;
+                i64 g = 0, h = 0;
+                for (u32 b = 0; b < n_bins[0]; ++b) {
+                    g += c[2 * b];
+                    h += c[2 * b + 1];
+                }
+                node.itg = g;
+                node.ith = h;
+            } else {
+                float2 const *c = reinterpret_cast<float2 const *>(hist) + static_cast<size_t>(node.slot) * node_cells;
+                double g = 0.0, h = 0.0;
+                for (u32 b = 0; b < n_bins[0]; ++b) {
+                    float2 const v = c[b];
+                    g += v.x;
+                    h += v.y;
+                }
+                node.tg = static_cast<float>(g);
+                node.th = static_cast<float>(h);
+            }
+            node.totals_ready = 1;
+            front[i] = node;
+        }
+        s_tot[0] = node.tg;
+        s_tot[1] = node.th;
+        s_itot[0] = node.itg;
+        s_itot[1] = node.ith;
+    }
+    __syncthreads();
+    SplitRec rec{};
+    rec.default_left = 1;
+    bool const can = node.cnt >= 2u * static_cast<u32>(cfg.min_data_in_leaf) &&
+                     static_cast<int>(node.depth) < cfg.max_depth;
+    if (!can) {
+        if (threadIdx.x == 0) out[i] = rec;
+        return;
+    }
+    double const tg = Q ? static_cast<double>(s_itot[0]) * sg : s_tot[0];
+    double const th = Q ? static_cast<double>(s_itot[1]) * sh : s_tot[1];
+    double const node_score = score_of(tg, th, cfg.l1, cfg.l2);
+
+    // LDS: prefix (g, h) per (tile feature, cut bin) + per-feature missing cell.
+    double *pg = reinterpret_cast<double *>(smem);            // [kFindTile][SB]
+    double *ph = pg + kFindTile * SB;                          // [kFindTile][SB]
+    i64 *ipg = reinterpret_cast<i64 *>(smem);                  // aliases for Q
+    i64 *iph = ipg + kFindTile * SB;
+    __shared__ double s_mg[kFindTile], s_mh[kFindTile];
+    __shared__ i64 s_img[kFindTile], s_imh[kFindTile];
+    __shared__ u32 s_ncut[kFindTile], s_dirs[kFindTile];
+
+    double best_gain = 0.0;
+    u32 best_key = kNone;
+    for (u32 f0 = 0; f0 < F; f0 += kFindTile) {
+        u32 const t = threadIdx.x;
+        if (t < kFindTile) {
+            u32 const f = f0 + t;
+            u32 ncut = 0, dirs = 1;
+            if (f < F) {
+                u32 const nb = n_bins[f];
+                ncut = nb >= 2 ? nb - 2 : 0;
+                if (Q) {
+                    i64 const *c = reinterpret_cast<i64 const *>(hist) +
+                                   (static_cast<size_t>(node.slot) * node_cells + static_cast<size_t>(f) * SB) * 2;
+                    i64 const mg = nb ? c[2 * (nb - 1)] : 0, mh = nb ? c[2 * (nb - 1) + 1] : 0;
+                    s_img[t] = mg;
+                    s_imh[t] = mh;
+                    dirs = (mg == 0 && mh == 0) ? 1 : 2;
+                    i64 g = 0, h = 0;
+                    for (u32 b = 0; b < ncut; ++b) {
+                        g += c[2 * b];
+                        h += c[2 * b + 1];
+                        ipg[t * SB + b] = g;
+                        iph[t * SB + b] = h;
+                    }
+                } else {
+                    float2 const *c = reinterpret_cast<float2 const *>(hist) +
+                                      static_cast<size_t>(node.slot) * node_cells + static_cast<size_t>(f) * SB;
+                    float2 const m = nb ? c[nb - 1] : make_float2(0.0F, 0.0F);
+                    s_mg[t] = m.x;
+                    s_mh[t] = m.y;
+                    dirs = (m.x == 0.0F && m.y == 0.0F) ? 1 : 2;
+                    double g = 0.0, h = 0.0;
+                    for (u32 b = 0; b < ncut; ++b) {
+                        float2 const v = c[b];
+                        g += v.x;
+                        h += v.y;
+                        pg[t * SB + b] = g;
+                        ph[t * SB + b] = h;
+                    }
+                }
+            }
+            s_ncut[t] = ncut;
+            s_dirs[t] = dirs;
+        }
+        __syncthreads();
+        u32 const n_cand = kFindTile * SB * 2;
+        for (u32 k = threadIdx.x; k < n_cand; k += blockDim.x) {
+            u32 const tt = k / (2 * SB);
+            u32 const b = (k >> 1) % SB;
+            u32 const d = k & 1u;
+            if (f0 + tt >= F || b >= s_ncut[tt] || d >= s_dirs[tt]) continue;
+            bool const dl = d == 0;
+            double gL, hL, gR, hR;
+            if (Q) {
+                i64 const lg = ipg[tt * SB + b], lh = iph[tt * SB + b];
+                i64 const mg = s_img[tt], mh = s_imh[tt];
+                i64 const rg = s_itot[0] - mg, rh = s_itot[1] - mh;
+                gL = static_cast<double>(lg + (dl ? mg : 0)) * sg;
+                hL = static_cast<double>(lh + (dl ? mh : 0)) * sh;
+                gR = static_cast<double>((rg - lg) + (!dl ? mg : 0)) * sg;
+                hR = static_cast<double>((rh - lh) + (!dl ? mh : 0)) * sh;
+            } else {
+                double const lg = pg[tt * SB + b], lh = ph[tt * SB + b];
+                double const mg = s_mg[tt], mh = s_mh[tt];
+                double const rg = tg - mg, rh = th - mh;
+                double const right_g = rg - lg, right_h = rh - lh;
+                gL = lg + (dl ? mg : 0.0);
+                hL = lh + (dl ? mh : 0.0);
+                gR = right_g + (!dl ? mg : 0.0);
+                hR = right_h + (!dl ? mh : 0.0);
+            }
+            if (hL < cfg.min_child_hess || hR < cfg.min_child_hess) continue;
+            double const cs = score_of(gL, hL, cfg.l1, cfg.l2) + score_of(gR, hR, cfg.l1, cfg.l2);
+            double const gain = cs - node_score;
+            if (!(gain > 0.0 && gain >= cfg.min_gain)) continue;
+            u32 const key = (f0 + tt) * (2u * SB) + b * 2u + d;
+            if (gain > best_gain || (gain == best_gain && key < best_key)) {
+                best_gain = gain;
+                best_key = key;
+            }
+        }
+        __syncthreads();
+    }
+    s_best_gain[threadIdx.x] = best_gain;
+    s_best_key[threadIdx.x] = best_key;
+    __syncthreads();
+    for (u32 s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            double const og = s_best_gain[threadIdx.x + s];
+            u32 const ok = s_best_key[threadIdx.x + s];
+            double const mg = s_best_gain[threadIdx.x];
+            u32 const mk = s_best_key[threadIdx.x];
+            if (ok != kNone && (mk == kNone || og > mg || (og == mg && ok < mk))) {
+                s_best_gain[threadIdx.x] = og;
+                s_best_key[threadIdx.x] = ok;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        u32 const key = s_best_key[0];
+        if (key != kNone) {
+            u32 const f = key / (2u * SB), b = (key >> 1) % SB, d = key & 1u;
+            bool const dl = d == 0;
+            rec.gain = s_best_gain[0];
+            rec.feature = f;
+            rec.bin = b;
+            rec.default_left = dl ? 1u : 0u;
+            rec.valid = 1;
+            u32 const nb = n_bins[f];
+            if (Q) {
+                i64 const *c = reinterpret_cast<i64 const *>(hist) +
+                               (static_cast<size_t>(node.slot) * node_cells + static_cast<size_t>(f) * SB) * 2;
+                i64 lg = 0, lh = 0;
+                for (u32 k = 0; k <= b; ++k) {
+                    lg += c[2 * k];
+                    lh += c[2 * k + 1];
+                }
+                i64 const mg = c[2 * (nb - 1)], mh = c[2 * (nb - 1) + 1];
+                rec.igL = lg + (dl ? mg : 0);
+                rec.ihL = lh + (dl ? mh : 0);
+                rec.igR = ((s_itot[0] - mg) - lg) + (!dl ? mg : 0);
+                rec.ihR = ((s_itot[1] - mh) - lh) + (!dl ? mh : 0);
+            } else {
+                float2 const *c = reinterpret_cast<float2 const *>(hist) +
+                                  static_cast<size_t>(node.slot) * node_cells + static_cast<size_t>(f) * SB;
+                double lg = 0.0, lh = 0.0;
+                for (u32 k = 0; k <= b; ++k) {
+                    lg += c[k].x;
+                    lh += c[k].y;
+                }
+                double const mg = c[nb - 1].x, mh = c[nb - 1].y;
+                double const rg = tg - mg, rh = th - mh;
+                rec.gL = lg + (dl ? mg : 0.0);
+                rec.hL = lh + (dl ? mh : 0.0);
+                rec.gR = (rg - lg) + (!dl ? mg : 0.0);
+                rec.hR = (rh - lh) + (!dl ? mh : 0.0);
+            }
+        }
+        out[i] = rec;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Level control (H13 depthwise): plan_level (src/grower_impl.hpp:212-255) and
+// commit_split_node (:188-210) on device.  One workgroup.  Splits get child ids in
+// frontier order; non-splits become leaves; the split ops and their partition units
+// are emitted for the level's partition.
+// ---------------------------------------------------------------------------------
+
+// Index of the last item whose `first` is <= c (items sorted by first).
+template <typename T, typename Get>
+__device__ __forceinline__ u32 find_owner(T const *items, u32 n, u32 c, Get get)
+{
+    u32 lo = 0, hi = n; // answer in [lo, hi)
+    while (hi - lo > 1) {
+        u32 const mid = (lo + hi) >> 1;
+        if (get(items[mid]) <= c) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(1024) void k_plan_level(NodeInfo const *__restrict__ front,
+                                                     SplitRec const *__restrict__ splits,
+                                                     TreeCtl *__restrict__ ctl, DevNode *__restrict__ tree,
+                                                     PartOp *__restrict__ ops, PartChunk *__restrict__ pch,
+                                                     u16 const *__restrict__ n_bins, GrowCfg cfg,
+                                                     double const *__restrict__ scales, u32 dst)
+{
+    __shared__ u32 scratch[1024];
+    __shared__ u32 s_base_nodes, s_ops, s_chunks;
+    u32 const P = ctl->n_front;
+    if (threadIdx.x == 0) {
+        s_base_nodes = ctl->n_nodes;
+        s_ops = 0;
+        s_chunks = 0;
+    }
+    __syncthreads();
+    for (u32 t0 = 0; t0 < P; t0 += blockDim.x) {
+        u32 const i = t0 + threadIdx.x;
+        bool const in = i < P;
+        u32 const valid = in ? splits[i].valid : 0u;
+        u32 tot;
+        u32 const rank = block_exscan(valid, scratch, &tot) + s_ops;
+        u32 nch = 0;
+        if (in && valid) nch = (front[i].cnt + kPartChunk - 1) / kPartChunk;
+        u32 ctot;
+        u32 const cfirst = block_exscan(nch, scratch, &ctot) + s_chunks;
+        if (in) {
+            NodeInfo const nd = front[i];
+            if (valid) {
+                SplitRec const s = splits[i];
+                u32 const lid = s_base_nodes + 2 * rank, rid = lid + 1;
+                DevNode pn = tree[nd.id];
+                pn.feature = static_cast<int32_t>(s.feature);
+                pn.bin = s.bin;
+                pn.default_left = s.default_left;
+                pn.left = lid;
+                pn.right = rid;
+                pn.value = 0.0F;
+                pn.gain = static_cast<float>(s.gain);
+                tree[nd.id] = pn;
+                DevNode leaf{};
+                leaf.feature = -1;
+                tree[lid] = leaf;
+                tree[rid] = leaf;
+                PartOp op{};
+                op.parent = i;
+                op.feature = s.feature;
+                op.bin = s.bin;
+                op.default_left = s.default_left;
+                op.last_bin = static_cast<u32>(n_bins[s.feature]) - 1u;
+                op.off = nd.off;
+                op.cnt = nd.cnt;
+                op.src = nd.src;
+                op.dst = dst;
+                op.chunk_first = cfirst;
+                op.chunk_count = nch;
+                op.left_id = lid;
+                op.right_id = rid;
+                op.alive = 1;
+                ops[rank] = op;
+            } else {
+                write_leaf(tree, nd.id, node_leaf_value(nd, cfg, scales));
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_ops += tot;
+            s_chunks += ctot;
+        }
+        __syncthreads();
+    }
+    // Expand the ops' unit ranges into the flat unit list.
+    u32 const n_ops = s_ops, n_ch = s_chunks;
+    for (u32 c = threadIdx.x; c < n_ch; c += blockDim.x) {
+        u32 const j = find_owner(ops, n_ops, c, [](PartOp const &o) { return o.chunk_first; });
+        PartOp const op = ops[j];
+        u32 const k = c - op.chunk_first;
+        PartChunk pc;
+        pc.op = j;
+        pc.begin = k * kPartChunk;
+        pc.end = min(op.cnt, (k + 1) * kPartChunk);
+        pc.left_before = 0;
+        pch[c] = pc;
+    }
+    if (threadIdx.x == 0) {
+        ctl->n_nodes = s_base_nodes + 2 * n_ops;
+        ctl->n_ops = n_ops;
+        ctl->n_part_chunks = n_ch;
+    }
+}
+
+// Partition count (H10, route_count_kernel, kernels.cuh:332-370): left rows per unit.
+__global__ __launch_bounds__(256) void k_part_count(PartChunk const *__restrict__ pch, PartOp const *__restrict__ ops,
+                                                    TreeCtl const *__restrict__ ctl, u8 const *__restrict__ pl0,
+                                                    u8 const *__restrict__ pl1, u8 const *__restrict__ plp, u32 Fs,
+                                                    u32 *__restrict__ chunk_left)
+{
+    __shared__ u32 red[4];
+    u32 const n = ctl->n_part_chunks;
+    for (u32 c = blockIdx.x; c < n; c += gridDim.x) {
+        PartChunk const ch = pch[c];
+        PartOp const op = ops[ch.op];
+        u8 const *plane = op.src == 0 ? pl0 : (op.src == 1 ? pl1 : plp);
+        u32 cnt = 0;
+        for (u32 k = ch.begin + threadIdx.x; k < ch.end; k += blockDim.x) {
+            u32 const b = plane[static_cast<size_t>(op.off + k) * Fs + op.feature];
+            cnt += routes_left(b, op.last_bin, op.bin, op.default_left) ? 1u : 0u;
+        }
+        for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+        __syncthreads();
+        if (threadIdx.x == 0) chunk_left[c] = red[0] + red[1] + red[2] + red[3];
+        __syncthreads();
+    }
+}
+
+// Partition scan + demotion + next frontier (H10 seg_scan_kernel, kernels.cuh:376-413;
+// demote_empty_splits, src/grower_impl.hpp:263-297; commit_children :302-324;
+// smaller_child tie -> left, src/step/primitives.hpp:105-108).  One workgroup.
+// oblivious = 1 (levelwise): no demotion, no tree nodes (the level split is broadcast).
+template <bool Q>
+__global__ __launch_bounds__(1024) void k_part_scan(
+    PartOp *__restrict__ ops, PartChunk *__restrict__ pch, u32 const *__restrict__ chunk_left,
+    TreeCtl *__restrict__ ctl, NodeInfo const *__restrict__ front, NodeInfo *__restrict__ next,
+    SplitRec const *__restrict__ splits, DevNode *__restrict__ tree, HistChunk *__restrict__ hch,
+    FinishOp *__restrict__ fops, GrowCfg cfg, double const *__restrict__ scales, u32 is_last,
+    u32 oblivious)
+{
+    __shared__ u32 scratch[1024];
+    __shared__ u32 s_run, s_alive, s_hch, s_parts;
+    u32 const n_ops = ctl->n_ops;
+    u32 const n_ch = ctl->n_part_chunks;
+    if (threadIdx.x == 0) {
+        s_run = 0;
+        s_alive = 0;
+        s_hch = 0;
+        s_parts = 0;
+    }
+    __syncthreads();
+    // 1) global exclusive prefix of left counts over all units (units of an op are
+    //    contiguous and ops are in order).
+    for (u32 t0 = 0; t0 < n_ch; t0 += blockDim.x) {
+        u32 const c = t0 + threadIdx.x;
+        u32 const v = c < n_ch ? chunk_left[c] : 0u;
+        u32 tot;
+        u32 const pre = block_exscan(v, scratch, &tot) + s_run;
+        if (c < n_ch) pch[c].left_before = pre;
+        __syncthreads();
+        if (threadIdx.x == 0) s_run += tot;
+        __syncthreads();
+    }
+    u32 const total_left = s_run;
+    // 2) per op: n_left from the prefix, demotion, covers.
+    for (u32 j = threadIdx.x; j < n_ops; j += blockDim.x) {
+        PartOp op = ops[j];
+        u32 const b0 = op.chunk_count ? pch[op.chunk_first].left_before : 0u;
+        u32 const e = op.chunk_first + op.chunk_count;
+        u32 const b1 = op.chunk_count ? (e < n_ch ? pch[e].left_before : total_left) : 0u;
+        u32 const nL = b1 - b0;
+        u32 const nR = op.cnt - nL;
+        op.base = b0;
+        op.n_left = nL;
+        bool const demote = !oblivious && ((nL == 0) != (nR == 0));
+        op.alive = demote ? 0u : 1u;
+        ops[j] = op;
+        NodeInfo const par = front[op.parent];
+        if (demote) {
+            write_leaf(tree, par.id, node_leaf_value(par, cfg, scales));
+            tree[par.id].gain = 0.0F;
+        } else if (!oblivious) {
+            tree[op.left_id].cover = static_cast<float>(nL);
+            tree[op.right_id].cover = static_cast<float>(nR);
+        }
+    }
+    __syncthreads();
+    // 3) per unit: make the prefix op-relative.
+    for (u32 c = threadIdx.x; c < n_ch; c += blockDim.x) pch[c].left_before -= ops[pch[c].op].base;
+    // 4) next frontier, finish ops, histogram units of the smaller children.
+    for (u32 t0 = 0; t0 < n_ops; t0 += blockDim.x) {
+        u32 const j = t0 + threadIdx.x;
+        bool const in = j < n_ops;
+        PartOp op{};
+        if (in) op = ops[j];
+        u32 const alive = in ? op.alive : 0u;
+        u32 atot;
+        u32 const rank = block_exscan(alive, scratch, &atot) + s_alive;
+        u32 const nL = op.n_left, nR = op.cnt - op.n_left;
+        bool const left_small = nL <= nR;
+        u32 const small_cnt = left_small ? nL : nR;
+        u32 n_units = 0, multi = 0;
+        if (in && alive && !is_last) {
+            n_units = (small_cnt + kHistChunk - 1) / kHistChunk;
+            multi = n_units > 1 ? n_units : 0;
+        }
+        u32 htot, ptot;
+        u32 const hfirst = block_exscan(n_units, scratch, &htot) + s_hch;
+        u32 const pfirst = block_exscan(multi, scratch, &ptot) + s_parts;
+        if (in && alive) {
+            NodeInfo const par = front[op.parent];
+            SplitRec const s = splits[op.parent];
+            NodeInfo L{}, R{};
+            L.id = op.left_id;
+            R.id = op.right_id;
+            L.off = op.off;
+            L.cnt = nL;
+            R.off = op.off + nL;
+            R.cnt = nR;
+            L.src = R.src = op.dst;
+            L.slot = 2 * rank;
+            R.slot = 2 * rank + 1;
+            L.depth = R.depth = par.depth + 1;
+            if (is_last) {
+                L.totals_ready = R.totals_ready = 1;
+                if (Q) {
+                    L.itg = s.igL;
+                    L.ith = s.ihL;
+                    R.itg = s.igR;
+                    R.ith = s.ihR;
+                } else {
+                    L.tg = static_cast<float>(s.gL);
+                    L.th = static_cast<float>(s.hL);
+                    R.tg = static_cast<float>(s.gR);
+                    R.th = static_cast<float>(s.hR);
+                }
+            }
+            next[2 * rank] = L;
+            next[2 * rank + 1] = R;
+            if (!is_last) {
+                FinishOp fo{};
+                fo.small_slot = left_small ? L.slot : R.slot;
+                fo.large_slot = left_small ? R.slot : L.slot;
+                fo.parent_slot = par.slot;
+                fo.part_first = multi ? pfirst : 0;
+                fo.part_count = multi;
+                fo.empty = small_cnt == 0 ? 1u : 0u;
+                fo.unit_first = hfirst;
+                fo.unit_count = n_units;
+                fo.small_off = left_small ? L.off : R.off;
+                fo.small_cnt = small_cnt;
+                fo.src = op.dst;
+                fops[rank] = fo;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_alive += atot;
+            s_hch += htot;
+            s_parts += ptot;
+        }
+        __syncthreads();
+    }
+    u32 const n_alive = s_alive, n_units_total = s_hch;
+    if (!is_last) {
+        for (u32 c = threadIdx.x; c < n_units_total; c += blockDim.x) {
+            u32 const r = find_owner(fops, n_alive, c, [](FinishOp const &o) { return o.unit_first; });
+            FinishOp const fo = fops[r];
+            u32 const k = c - fo.unit_first;
+            HistChunk hc{};
+            hc.slot = fo.small_slot;
+            hc.src = fo.src;
+            hc.begin = fo.small_off + k * kHistChunk;
+            hc.end = fo.small_off + min(fo.small_cnt, (k + 1) * kHistChunk);
+            hc.part = fo.part_count ? fo.part_first + k : kNone;
+            hch[c] = hc;
+        }
+    }
+    if (threadIdx.x == 0) {
+        ctl->next_front = 2 * n_alive;
+        ctl->n_finish = is_last ? 0 : n_alive;
+        ctl->n_hist_chunks = is_last ? 0 : n_units_total;
+        ctl->n_parts = s_parts;
+    }
+}
+
+// Stable scatter (H10 scatter_kernel, kernels.cuh:418-476; host_partition,
+// src/step/primitives.hpp:262-374): left rows keep their order at off + ..., right rows
+// follow at off + n_left + ....  Rows are contiguous per thread (16 each) so the
+// block-wide exclusive scan of per-thread left counts gives every row its stable rank;
+// the row bytes and the gradient pair are then copied with coalesced dword moves.
+__global__ __launch_bounds__(256) void k_part_scatter(
+    PartChunk const *__restrict__ pch, PartOp const *__restrict__ ops, TreeCtl const *__restrict__ ctl,
+    u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
+    void const *__restrict__ gw0, void const *__restrict__ gw1, void const *__restrict__ gwp,
+    u8 *__restrict__ out0, u8 *__restrict__ out1, void *__restrict__ go0, void *__restrict__ go1, u32 Fs,
+    u32 gh_bytes, u32 tpr_log2)
+{
+    __shared__ u32 dest[kPartChunk];
+    __shared__ u32 scratch[256];
+    constexpr u32 RPT = kPartChunk / 256; // rows per thread
+    u32 const n = ctl->n_part_chunks;
+    u32 const D = Fs / 4u;
+    u32 const tpr = 1u << tpr_log2;
+    for (u32 c = blockIdx.x; c < n; c += gridDim.x) {
+        PartChunk const ch = pch[c];
+        PartOp const op = ops[ch.op];
+        if (!op.alive) continue; // uniform across the block
+        u8 const *plane = op.src == 0 ? pl0 : (op.src == 1 ? pl1 : plp);
+        u8 const *gin = reinterpret_cast<u8 const *>(op.src == 0 ? gw0 : (op.src == 1 ? gw1 : gwp));
+        u8 *oplane = op.dst == 0 ? out0 : out1;
+        u8 *gout = reinterpret_cast<u8 *>(op.dst == 0 ? go0 : go1);
+        u32 const rows = ch.end - ch.begin;
+        u32 const r0 = threadIdx.x * RPT;
+        u32 bits = 0;
+        for (u32 k = 0; k < RPT; ++k) {
+            u32 const r = r0 + k;
+            if (r < rows) {
+                u32 const b = plane[static_cast<size_t>(op.off + ch.begin + r) * Fs + op.feature];
+                if (routes_left(b, op.last_bin, op.bin, op.default_left)) bits |= 1u << k;
+            }
+        }
+        u32 tot;
+        u32 const pre = block_exscan(__popc(bits), scratch, &tot);
+        u32 lpos = op.off + ch.left_before + pre;
+        u32 rpos = op.off + op.n_left + (ch.begin + r0 - ch.left_before - pre);
+        for (u32 k = 0; k < RPT; ++k) {
+            u32 const r = r0 + k;
+            if (r < rows) dest[r] = ((bits >> k) & 1u) ? lpos++ : rpos++;
+        }
+        __syncthreads();
+        // Row bytes: tpr threads per row, dword granularity.
+        u32 const rows_per_pass = 256u >> tpr_log2;
+        u32 const sub = threadIdx.x & (tpr - 1u);
+        for (u32 r = threadIdx.x >> tpr_log2; r < rows; r += rows_per_pass) {
+            u32 const *src = reinterpret_cast<u32 const *>(plane + static_cast<size_t>(op.off + ch.begin + r) * Fs);
+            u32 *dstp = reinterpret_cast<u32 *>(oplane + static_cast<size_t>(dest[r]) * Fs);
+            for (u32 dw = sub; dw < D; dw += tpr) dstp[dw] = src[dw];
+        }
+        // Gradient pairs.
+        for (u32 r = threadIdx.x; r < rows; r += 256u) {
+            size_t const s = static_cast<size_t>(op.off + ch.begin + r);
+            if (gh_bytes == 8)
+                reinterpret_cast<uint2 *>(gout)[dest[r]] = reinterpret_cast<uint2 const *>(gin)[s];
+            else
+                reinterpret_cast<u32 *>(gout)[dest[r]] = reinterpret_cast<u32 const *>(gin)[s];
+        }
+        __syncthreads();
+    }
+}
+
+// Frontier left at the end of the tree becomes leaves (LevelStep::end_tree,
+// src/step/level.hpp:119-128): values from the totals the parent split carried.
+__global__ void k_finalize(NodeInfo const *__restrict__ front, TreeCtl const *__restrict__ ctl,
+                           DevNode *__restrict__ tree, GrowCfg cfg, double const *__restrict__ scales)
+{
+    u32 const P = ctl->n_front;
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < P; i += gridDim.x * blockDim.x) {
+        NodeInfo const nd = front[i];
+        write_leaf(tree, nd.id, node_leaf_value(nd, cfg, scales));
+    }
+}
+
+// Frontier swap between levels.
+__global__ void k_next_level(TreeCtl *ctl)
+{
+    if (threadIdx.x == 0) {
+        ctl->n_front = ctl->next_front;
+        ctl->depth += 1;
+    }
+}
+
+// Tree start: root frontier, root histogram units over the pristine plane.
+__global__ __launch_bounds__(256) void k_tree_begin(TreeCtl *__restrict__ ctl, NodeInfo *__restrict__ front,
+                                                    DevNode *__restrict__ tree, HistChunk *__restrict__ hch,
+                                                    FinishOp *__restrict__ fops, u32 n_rows)
+{
+    u32 const n_units = (n_rows + kHistChunk - 1) / kHistChunk;
+    u32 const multi = n_units > 1;
+    for (u32 k = threadIdx.x; k < n_units; k += blockDim.x) {
+        HistChunk hc{};
+        hc.slot = 0;
+        hc.src = kSrcPlane0;
+        hc.begin = k * kHistChunk;
+        hc.end = min(n_rows, (k + 1) * kHistChunk);
+        hc.part = multi ? k : kNone;
+        hch[k] = hc;
+    }
+    if (threadIdx.x == 0) {
+        NodeInfo root{};
+        root.id = 0;
+        root.off = 0;
+        root.cnt = n_rows;
+        root.src = kSrcPlane0;
+        root.slot = 0;
+        front[0] = root;
+        DevNode r{};
+        r.feature = -1;
+        r.cover = static_cast<float>(n_rows);
+        tree[0] = r;
+        FinishOp fo{};
+        fo.small_slot = 0;
+        fo.large_slot = kNone;
+        fo.part_first = 0;
+        fo.part_count = multi ? n_units : 0;
+        fo.empty = n_rows == 0;
+        fops[0] = fo;
+        ctl->n_front = 1;
+        ctl->n_nodes = 1;
+        ctl->n_ops = 0;
+        ctl->n_part_chunks = 0;
+        ctl->n_hist_chunks = n_units;
+        ctl->n_finish = 1;
+        ctl->n_parts = multi ? n_units : 0;
+        ctl->next_front = 0;
+        ctl->depth = 0;
+        ctl->done = 0;
+        ctl->oblivious_depth = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Resident score epilogue (H12): route_add_kernel (src/cuda/detail/kernels.cuh:1140-1163)
+// and the booster's score loop (include/bonsai/booster.hpp:577-580):
+// score += lr * leaf, multiply then add, each rounded.
+// ---------------------------------------------------------------------------------
+constexpr u32 kWalkLds = 2048;
+
+__device__ __forceinline__ WalkNode to_walk(DevNode const &d)
+{
+    WalkNode w;
+    w.feature = d.feature < 0 ? 0xFFFFu : static_cast<u16>(d.feature);
+    w.bin = static_cast<u8>(d.bin);
+    w.default_left = static_cast<u8>(d.default_left);
+    w.left = static_cast<u16>(d.left);
+    w.right = static_cast<u16>(d.right);
+    w.value = d.value;
+    return w;
+}
+
+__global__ __launch_bounds__(256) void k_score_update(u8 const *__restrict__ plane, u32 Fs, u64 n,
+                                                      DevNode const *__restrict__ tree,
+                                                      TreeCtl const *__restrict__ ctl,
+                                                      u16 const *__restrict__ n_bins, u32 F, float lr,
+                                                      float *__restrict__ score, u32 *__restrict__ leaf_of_row)
+{
+    __shared__ WalkNode nodes[kWalkLds];
+    __shared__ u16 last[kWalkLds];
+    u32 const nn = ctl->n_nodes;
+    u32 const nl = min(nn, kWalkLds);
+    for (u32 i = threadIdx.x; i < nl; i += blockDim.x) nodes[i] = to_walk(tree[i]);
+    for (u32 f = threadIdx.x; f < F && f < kWalkLds; f += blockDim.x) last[f] = static_cast<u16>(n_bins[f] - 1u);
+    __syncthreads();
+    for (u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; r < n;
+         r += static_cast<u64>(gridDim.x) * blockDim.x) {
+        u8 const *row = plane + r * Fs;
+        u32 idx = 0;
+        WalkNode w = nodes[0];
+        while (w.feature != 0xFFFFu) {
+            u32 const f = w.feature;
+            u32 const b = row[f];
+            u32 const lb = f < kWalkLds ? last[f] : static_cast<u32>(n_bins[f]) - 1u;
+            idx = routes_left(b, lb, w.bin, w.default_left) ? w.left : w.right;
+            w = idx < kWalkLds ? nodes[idx] : to_walk(tree[idx]);
+        }
+        if (score) score[r] = __fadd_rn(score[r], __fmul_rn(lr, w.value));
+        if (leaf_of_row) leaf_of_row[r] = idx;
+    }
+}
+
+__global__ void k_iter_advance(TreeCtl *ctl)
+{
+    if (threadIdx.x == 0) ctl->iter += 1;
+}
+
+__global__ void k_copy_tree(DevNode const *__restrict__ tree, TreeCtl const *__restrict__ ctl,
+                            DevNode *__restrict__ out_trees, u32 *__restrict__ out_sizes, u32 max_nodes)
+{
+    u32 const it = ctl->iter;
+    u32 const nn = min(ctl->n_nodes, max_nodes);
+    for (u32 i = threadIdx.x; i < nn; i += blockDim.x)
+        out_trees[static_cast<size_t>(it) * max_nodes + i] = tree[i];
+    if (threadIdx.x == 0) out_sizes[it] = nn;
+}
+
+// Gathers listed rows of the pristine plane (and their gradient pairs, or the row id
+// itself as the pair) into a contiguous segment: the entry to the kernel-level tests.
+__global__ void k_gather_rows(u8 const *__restrict__ plane, u32 Fs, void const *__restrict__ gh, u32 gh_bytes,
+                              u32 const *__restrict__ rows, u64 n, u8 *__restrict__ out_plane,
+                              void *__restrict__ out_gh, int gh_is_row_id)
+{
+    u32 const D = Fs / 4u;
+    for (u64 i = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<u64>(gridDim.x) * blockDim.x) {
+        u32 const r = rows[i];
+        u32 const *src = reinterpret_cast<u32 const *>(plane + static_cast<size_t>(r) * Fs);
+        u32 *dst = reinterpret_cast<u32 *>(out_plane + i * Fs);
+        for (u32 d = 0; d < D; ++d) dst[d] = src[d];
+        if (gh_is_row_id) {
+            if (gh_bytes == 8) reinterpret_cast<uint2 *>(out_gh)[i] = make_uint2(r, 0u);
+            else reinterpret_cast<u32 *>(out_gh)[i] = r;
+        } else if (gh_bytes == 8) {
+            reinterpret_cast<uint2 *>(out_gh)[i] = reinterpret_cast<uint2 const *>(gh)[r];
+        } else {
+            reinterpret_cast<u32 *>(out_gh)[i] = reinterpret_cast<u32 const *>(gh)[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Launch wrappers (host side).
+// ---------------------------------------------------------------------------------
+void launch_grad_f32(hipStream_t s, float const *score, float const *label, u64 n, int obj, float2 *gh)
+{
+    hipLaunchKernelGGL(k_grad_f32, dim3(2048), dim3(256), 0, s, score, label, n, obj, gh);
+}
+void launch_grad_quant(hipStream_t s, float const *score, float const *label, u64 n, int obj, u32 *maxbits,
+                       u64 seed, TreeCtl const *ctl, u32 *ghq, double *scales)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, maxbits, 2u);
+    hipLaunchKernelGGL(k_grad_max, dim3(2048), dim3(256), 0, s, score, label, n, obj, maxbits);
+    hipLaunchKernelGGL(k_grad_quant, dim3(2048), dim3(256), 0, s, score, label, n, obj, maxbits, seed, ctl, ghq,
+                       scales);
+}
+void launch_quant_from(hipStream_t s, float2 const *gh, u64 n, u32 *maxbits, u64 seed, u32 iter, u32 *ghq,
+                       double *scales_dev)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, maxbits, 2u);
+    hipLaunchKernelGGL(k_gh_max, dim3(2048), dim3(256), 0, s, gh, n, maxbits);
+    hipLaunchKernelGGL(k_quant_from, dim3(2048), dim3(256), 0, s, gh, n, maxbits, seed, iter, ghq);
+    (void)scales_dev;
+}
+
+hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl const *ctl, u8 const *pl0,
+                       u8 const *pl1, u8 const *plp, void const *g0, void const *g1, void const *gp, u32 Fs, u32 F,
+                       u32 SB, u32 fsl, u32 n_waves, u32 grid_x, void *partials, void *hist_next)
+{
+    u32 const slices = (F + fsl - 1) / fsl;
+    size_t const lds = static_cast<size_t>(fsl) * SB * 8;
+    dim3 const grid(grid_x, slices), block(64 * n_waves);
+    if (q)
+        hipLaunchKernelGGL((k_hist<true, 4>), grid, block, lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp, Fs, F,
+                           SB, fsl, partials, hist_next);
+    else
+        hipLaunchKernelGGL((k_hist<false, 4>), grid, block, lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp, Fs, F,
+                           SB, fsl, partials, hist_next);
+    return hipGetLastError();
+}
+
+// Opt the LDS-heavy kernels into the device's full per-workgroup LDS (160 KiB on
+// gfx950).  Best effort: a runtime that does not gate dynamic LDS behind the attribute
+// rejects the call, and launches are then checked against the device limit.
+static void raise_lds(void const *fn, int max_shared)
+{
+    hipFuncAttributes a{};
+    if (hipFuncGetAttributes(&a, fn) != hipSuccess) return;
+    int const dyn = max_shared - static_cast<int>(a.sharedSizeBytes);
+    if (dyn > 0) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+    (void)hipGetLastError();
+}
+
+hipError_t set_hist_lds_limits()
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    int max_shared = 0;
+    e = hipDeviceGetAttribute(&max_shared, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return e;
+    raise_lds(reinterpret_cast<void const *>(&k_hist<true, 4>), max_shared);
+    raise_lds(reinterpret_cast<void const *>(&k_hist<false, 4>), max_shared);
+    raise_lds(reinterpret_cast<void const *>(&k_find<true>), max_shared);
+    raise_lds(reinterpret_cast<void const *>(&k_find<false>), max_shared);
+    return hipSuccess;
+}
+
+void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl const *ctl, void const *partials,
+                   void const *hist_cur, void *hist_next, u32 node_cells, u32 max_ops)
+{
+    u32 const tiles = min(64u, (node_cells + 255) / 256);
+    dim3 const grid(tiles, max(1u, min(max_ops, 1024u)));
+    if (q)
+        hipLaunchKernelGGL(k_finish<true>, grid, dim3(256), 0, s, ops, ctl, partials, hist_cur, hist_next,
+                           node_cells);
+    else
+        hipLaunchKernelGGL(k_finish<false>, grid, dim3(256), 0, s, ops, ctl, partials, hist_cur, hist_next,
+                           node_cells);
+}
+
+void launch_find(hipStream_t s, bool q, NodeInfo *front, TreeCtl const *ctl, void const *hist, u16 const *n_bins,
+                 u32 F, u32 SB, GrowCfg const &cfg, double const *scales, SplitRec *out, u32 max_front,
+                 u32 front_base, u32 n_front_override)
+{
+    size_t const lds = static_cast<size_t>(kFindTile) * SB * 16;
+    if (q)
+        hipLaunchKernelGGL(k_find<true>, dim3(max_front), dim3(256), lds, s, front, ctl, hist, n_bins, F, SB, cfg,
+                           scales, out, front_base, n_front_override);
+    else
+        hipLaunchKernelGGL(k_find<false>, dim3(max_front), dim3(256), lds, s, front, ctl, hist, n_bins, F, SB, cfg,
+                           scales, out, front_base, n_front_override);
+}
+
+void launch_plan_level(hipStream_t s, NodeInfo const *front, SplitRec const *splits, TreeCtl *ctl, DevNode *tree,
+                       PartOp *ops, PartChunk *pch, u16 const *n_bins, GrowCfg const &cfg, double const *scales,
+                       u32 dst)
+{
+    hipLaunchKernelGGL(k_plan_level, dim3(1), dim3(1024), 0, s, front, splits, ctl, tree, ops, pch, n_bins, cfg,
+                       scales, dst);
+}
+
+void launch_part_count(hipStream_t s, PartChunk const *pch, PartOp const *ops, TreeCtl const *ctl, u8 const *pl0,
+                       u8 const *pl1, u8 const *plp, u32 Fs, u32 *chunk_left, u32 grid)
+{
+    hipLaunchKernelGGL(k_part_count, dim3(grid), dim3(256), 0, s, pch, ops, ctl, pl0, pl1, plp, Fs, chunk_left);
+}
+
+void launch_part_scan(hipStream_t s, bool q, PartOp *ops, PartChunk *pch, u32 const *chunk_left, TreeCtl *ctl,
+                      NodeInfo const *front, NodeInfo *next, SplitRec const *splits, DevNode *tree, HistChunk *hch,
+                      FinishOp *fops, GrowCfg const &cfg, double const *scales, u32 is_last, u32 oblivious)
+{
+    if (q)
+        hipLaunchKernelGGL(k_part_scan<true>, dim3(1), dim3(1024), 0, s, ops, pch, chunk_left, ctl, front, next,
+                           splits, tree, hch, fops, cfg, scales, is_last, oblivious);
+    else
+        hipLaunchKernelGGL(k_part_scan<false>, dim3(1), dim3(1024), 0, s, ops, pch, chunk_left, ctl, front, next,
+                           splits, tree, hch, fops, cfg, scales, is_last, oblivious);
+}
+
+void launch_part_scatter(hipStream_t s, PartChunk const *pch, PartOp const *ops, TreeCtl const *ctl,
+                         u8 const *pl0, u8 const *pl1, u8 const *plp, void const *g0, void const *g1,
+                         void const *gp, u8 *o0, u8 *o1, void *go0, void *go1, u32 Fs, u32 gh_bytes, u32 grid)
+{
+    u32 const D = Fs / 4;
+    u32 tl = 0;
+    while ((1u << tl) < D && tl < 8) ++tl;
+    hipLaunchKernelGGL(k_part_scatter, dim3(grid), dim3(256), 0, s, pch, ops, ctl, pl0, pl1, plp, g0, g1, gp, o0,
+                       o1, go0, go1, Fs, gh_bytes, tl);
+}
+
+void launch_finalize(hipStream_t s, NodeInfo const *front, TreeCtl const *ctl, DevNode *tree, GrowCfg const &cfg,
+                     double const *scales, u32 max_front)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(max(1u, (max_front + 255) / 256)), dim3(256), 0, s, front, ctl, tree, cfg,
+                       scales);
+}
+
+void launch_next_level(hipStream_t s, TreeCtl *ctl)
+{
+    hipLaunchKernelGGL(k_next_level, dim3(1), dim3(64), 0, s, ctl);
+}
+
+void launch_tree_begin(hipStream_t s, TreeCtl *ctl, NodeInfo *front, DevNode *tree, HistChunk *hch,
+                       FinishOp *fops, u32 n_rows)
+{
+    hipLaunchKernelGGL(k_tree_begin, dim3(1), dim3(256), 0, s, ctl, front, tree, hch, fops, n_rows);
+}
+
+void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode const *tree, TreeCtl const *ctl,
+                         u16 const *n_bins, u32 F, float lr, float *score, u32 *leaf_of_row)
+{
+    hipLaunchKernelGGL(k_score_update, dim3(2048), dim3(256), 0, s, plane, Fs, n, tree, ctl, n_bins, F, lr, score,
+                       leaf_of_row);
+}
+
+void launch_iter_advance(hipStream_t s, TreeCtl *ctl)
+{
+    hipLaunchKernelGGL(k_iter_advance, dim3(1), dim3(64), 0, s, ctl);
+}
+
+void launch_gather_rows(hipStream_t s, u8 const *plane, u32 Fs, void const *gh, u32 gh_bytes, u32 const *rows,
+                        u64 n, u8 *out_plane, void *out_gh, int gh_is_row_id)
+{
+    hipLaunchKernelGGL(k_gather_rows, dim3(1024), dim3(256), 0, s, plane, Fs, gh, gh_bytes, rows, n, out_plane,
+                       out_gh, gh_is_row_id);
+}
+
+void launch_copy_tree(hipStream_t s, DevNode const *tree, TreeCtl const *ctl, DevNode *out_trees, u32 *out_sizes,
+                      u32 max_nodes)
+{
+    hipLaunchKernelGGL(k_copy_tree, dim3(1), dim3(256), 0, s, tree, ctl, out_trees, out_sizes, max_nodes);
+}
+
+} // namespace bh

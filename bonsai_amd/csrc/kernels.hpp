@@ -1,0 +1,46 @@
+// Host-side launch wrappers of the training kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "bh_common.hpp"
+
+namespace bh {
+
+void launch_grad_f32(hipStream_t s, float const *score, float const *label, u64 n, int obj, float2 *gh);
+void launch_grad_quant(hipStream_t s, float const *score, float const *label, u64 n, int obj, u32 *maxbits,
+                       u64 seed, TreeCtl const *ctl, u32 *ghq, double *scales);
+void launch_quant_from(hipStream_t s, float2 const *gh, u64 n, u32 *maxbits, u64 seed, u32 iter, u32 *ghq,
+                       double *scales_dev);
+hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl const *ctl, u8 const *pl0,
+                       u8 const *pl1, u8 const *plp, void const *g0, void const *g1, void const *gp, u32 Fs, u32 F,
+                       u32 SB, u32 fsl, u32 n_waves, u32 grid_x, void *partials, void *hist_next);
+hipError_t set_hist_lds_limits();
+void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl const *ctl, void const *partials,
+                   void const *hist_cur, void *hist_next, u32 node_cells, u32 max_ops);
+void launch_find(hipStream_t s, bool q, NodeInfo *front, TreeCtl const *ctl, void const *hist, u16 const *n_bins,
+                 u32 F, u32 SB, GrowCfg const &cfg, double const *scales, SplitRec *out, u32 max_front,
+                 u32 front_base, u32 n_front_override);
+void launch_plan_level(hipStream_t s, NodeInfo const *front, SplitRec const *splits, TreeCtl *ctl, DevNode *tree,
+                       PartOp *ops, PartChunk *pch, u16 const *n_bins, GrowCfg const &cfg, double const *scales,
+                       u32 dst);
+void launch_part_count(hipStream_t s, PartChunk const *pch, PartOp const *ops, TreeCtl const *ctl, u8 const *pl0,
+                       u8 const *pl1, u8 const *plp, u32 Fs, u32 *chunk_left, u32 grid);
+void launch_part_scan(hipStream_t s, bool q, PartOp *ops, PartChunk *pch, u32 const *chunk_left, TreeCtl *ctl,
+                      NodeInfo const *front, NodeInfo *next, SplitRec const *splits, DevNode *tree, HistChunk *hch,
+                      FinishOp *fops, GrowCfg const &cfg, double const *scales, u32 is_last, u32 oblivious);
+void launch_part_scatter(hipStream_t s, PartChunk const *pch, PartOp const *ops, TreeCtl const *ctl,
+                         u8 const *pl0, u8 const *pl1, u8 const *plp, void const *g0, void const *g1,
+                         void const *gp, u8 *o0, u8 *o1, void *go0, void *go1, u32 Fs, u32 gh_bytes, u32 grid);
+void launch_finalize(hipStream_t s, NodeInfo const *front, TreeCtl const *ctl, DevNode *tree, GrowCfg const &cfg,
+                     double const *scales, u32 max_front);
+void launch_next_level(hipStream_t s, TreeCtl *ctl);
+void launch_tree_begin(hipStream_t s, TreeCtl *ctl, NodeInfo *front, DevNode *tree, HistChunk *hch,
+                       FinishOp *fops, u32 n_rows);
+void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode const *tree, TreeCtl const *ctl,
+                         u16 const *n_bins, u32 F, float lr, float *score, u32 *leaf_of_row);
+void launch_iter_advance(hipStream_t s, TreeCtl *ctl);
+void launch_copy_tree(hipStream_t s, DevNode const *tree, TreeCtl const *ctl, DevNode *out_trees, u32 *out_sizes,
+                      u32 max_nodes);
+void launch_gather_rows(hipStream_t s, u8 const *plane, u32 Fs, void const *gh, u32 gh_bytes, u32 const *rows,
+                        u64 n, u8 *out_plane, void *out_gh, int gh_is_row_id);
+
+} // namespace bh

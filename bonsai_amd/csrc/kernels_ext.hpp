@@ -1,0 +1,64 @@
+// Leafwise and levelwise grow loops (kernels_ext.hip): their extra device state and the
+// view of the engine context they run on.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "../../include/bonsai_hip.h"
+#include "bh_common.hpp"
+
+namespace bh {
+
+// Engine buffers a grow loop reads and writes (owned by bh_context).
+struct GrowView {
+    u64 N;
+    u32 F, Fs, SB;
+    u8 *plane0, *work[2];
+    void const *gh[3];
+    u16 const *d_nbins;
+    void *hist[2];
+    void *partials;
+    TreeCtl *ctl;
+    NodeInfo *front[2];
+    SplitRec *splits;
+    DevNode *tree;
+    PartOp *ops;
+    PartChunk *pch;
+    u32 *chunk_left;
+    HistChunk *hch;
+    FinishOp *fops;
+    double *scales;
+    GrowCfg gc;
+    bh_fit_config fc;
+    u32 max_pch, max_hch, max_ops, fsl, waves;
+    bool host_sync;
+};
+
+// Leafwise candidate pool entry: a leaf with its best split, waiting in the gain heap
+// (grower_detail::Candidate, src/step/primitives.hpp:404-415).
+struct Cand {
+    NodeInfo node;
+    SplitRec split;
+};
+
+struct ExtState {
+    // leafwise
+    Cand *cands = nullptr;      // candidate pool (heap entries), capacity max_leaves + 4
+    NodeInfo *pending = nullptr;// leaves set aside (no valid split)
+    u32 *free_slots = nullptr;  // histogram slot free list
+    u32 cap = 0;
+    // levelwise
+    SplitRec *feat_best = nullptr; // per-feature best of the level scan
+    u32 *level_splits = nullptr;   // (feature, bin, default_left) per level
+    float *level_gains = nullptr;
+    u32 n_features = 0;
+};
+
+hipError_t ext_alloc(ExtState &s, bh_fit_config const &f, u32 F, u32 SB, u32 max_front, u32 hist_slots);
+void ext_free(ExtState &s);
+hipError_t ext_set_limits();
+void enqueue_levelwise(ExtState &s, hipStream_t st, bool q, GrowView const &v);
+void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v);
+
+} // namespace bh
+
+struct bh_context;
+bh::GrowView grow_view(bh_context *c);

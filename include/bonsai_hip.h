@@ -1,0 +1,180 @@
+/*
+ * bonsai_hip.h — C ABI of the MI355X-native GBDT hot path (libbonsai_hip.so).
+ *
+ * The reference has no C ABI: its CUDA engine is a same-compiler C++ class behind the
+ * engine concepts (include/bonsai/grower.hpp:78-162) with a pimpl
+ * (include/bonsai/cuda/histogram_engine.hpp:281-283).  This header is the boundary a
+ * C++ adapter satisfying those concepts (or a ctypes / cgo / JNI binding) calls.  Every
+ * declaration cites the reference interface it replaces.
+ *
+ * Conventions: plain pointers and sizes, host memory unless stated; every function
+ * returns 0 on success, > 0 for a configuration/capacity refusal (the reference's
+ * ConfigError, src/cuda/detail/device_context.cu:60-87) and < 0 for a device error
+ * (the reference's std::runtime_error("cuda: ..."), src/cuda/detail/device_buffer.cuh:96-103).
+ * bh_last_error() returns the thread-local message of the last failure.
+ * A context is not re-entrant; distinct contexts may be used from distinct threads.
+ */
+#ifndef BONSAI_HIP_H
+#define BONSAI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bh_context bh_context;
+typedef struct bh_predictor bh_predictor;
+
+/* Growth policies: the reference's cuda_depthwise / cuda_leafwise / cuda_levelwise
+ * registry names (include/bonsai/registry/names.hpp:81-94), here hip_*. */
+enum { BH_DEPTHWISE = 0, BH_LEAFWISE = 1, BH_LEVELWISE = 2 };
+/* Objectives with a device gradient (DeviceObjectiveKind, kernels.cuh:972-1009). */
+enum { BH_MSE = 0, BH_LOGLOSS = 1 };
+
+/* TreeConfig + BoosterConfig subset plus the device knobs
+ * (include/bonsai/config/tree_config.hpp:9-29, booster_config.hpp:8-24). */
+typedef struct bh_fit_config {
+    int32_t  grower;            /* BH_DEPTHWISE / BH_LEAFWISE / BH_LEVELWISE */
+    int32_t  objective;         /* BH_MSE / BH_LOGLOSS */
+    int32_t  quantized;         /* 0: FP32 fixed-order histograms; 1: int8 g / int16 h */
+    int32_t  use_graph;         /* 1: one hipGraph per boosting iteration */
+    int32_t  host_sync;         /* 1: host round-trip after every level's find and
+                                   partition (the reference's cadence, for comparison) */
+    int32_t  max_depth;
+    int32_t  min_data_in_leaf;
+    int32_t  max_leaves;        /* leafwise only */
+    float    min_child_hess;
+    float    min_gain_to_split;
+    float    lambda_l2;
+    float    lambda_l1;
+    float    learning_rate;
+    uint32_t _pad;
+    uint64_t seed;              /* stochastic-rounding key */
+} bh_fit_config;
+
+/* Tree node in bin space (DenseTree::Node, include/bonsai/tree.hpp:32-48, with the
+ * split bin instead of the float threshold, plus cover and split gain). */
+typedef struct bh_node {
+    int32_t  feature;           /* < 0: leaf */
+    uint32_t bin;               /* rows with bin <= bin go left (missing bin: default_left) */
+    uint32_t default_left;
+    uint32_t left;
+    uint32_t right;
+    float    value;             /* leaf value */
+    float    cover;
+    float    gain;
+} bh_node;
+
+/* Split decision (SplitOutput, include/bonsai/split.hpp:66-73) plus the child sums the
+ * finder derived (the reference's FeatBest, src/cuda/detail/device_buffer.cuh:84-88). */
+typedef struct bh_split {
+    double   gain;
+    uint32_t feature, bin, default_left, valid;
+    double   gL, hL, gR, hR;
+    int64_t  igL, ihL, igR, ihR;
+} bh_split;
+
+/* ---- devices and errors ---- */
+/* cuda_device_count / cuda_available (include/bonsai/cuda/histogram_engine.hpp:36-41). */
+int         bh_device_count(int *count);
+/* Message of the last failure on this thread. */
+const char *bh_last_error(void);
+
+/* ---- context (CudaDeviceContext, src/cuda/detail/device_context.cuh:40-493) ---- */
+/* cuda_select_device + engine construction (src/cuda/histogram_engine.cu:54-83). */
+int bh_create(int device, bh_context **out);
+int bh_destroy(bh_context *ctx);
+
+/* Adopt a binned plane: row-major u8 bins [n_rows][row_stride], n_bins[f] per feature
+ * (<= 256).  Replaces the reference's tiled upload stage_tiled / cuda_ingest
+ * (src/cuda/detail/device_context.cu:499-571, src/cuda/histogram_engine.cu:381). */
+int bh_upload_bins(bh_context *ctx, const uint8_t *bins, uint64_t n_rows, uint32_t n_features,
+                   uint64_t row_stride, const uint16_t *n_bins);
+/* Labels of the resident objective (resident_begin, device_context.cu:1541-1579). */
+int bh_upload_labels(bh_context *ctx, const float *labels);
+/* Configure the fit; allocates every device buffer the fit needs (the reference's
+ * capacity gates refuse_hist_budget / refuse_leaf_pool, device_context.cu:60-87). */
+int bh_configure(bh_context *ctx, const bh_fit_config *cfg);
+
+/* ---- whole-fit path (Booster::update_one_iter + resident_round,
+ *      include/bonsai/booster.hpp:500-676) ---- */
+/* Set every row's score (init_score on the first call of a fit). */
+int bh_set_scores(bh_context *ctx, const float *scores);
+int bh_get_scores(bh_context *ctx, float *scores);
+/* Run n_iters boosting iterations on the device (gradient, quantize, grow, score
+ * update).  With use_graph each iteration is one hipGraph launch. */
+int bh_train(bh_context *ctx, uint32_t n_iters);
+/* Trees grown so far: nodes[t * max_nodes + i], sizes[t]; returns via *n_trees. */
+int bh_get_trees(bh_context *ctx, bh_node *nodes, uint32_t max_nodes, uint32_t *sizes,
+                 uint32_t cap_trees, uint32_t *n_trees);
+/* Per-tree node capacity of the configured fit. */
+int bh_max_nodes(bh_context *ctx, uint32_t *max_nodes);
+
+/* ---- engine seam, one tree from host gradients (HistogramEngine::begin_tree +
+ *      the grow loop, include/bonsai/grower.hpp:54-162) ---- */
+/* Host gradients (begin_tree with host grad/hess, device_context.cu:573-600).  With a
+ * quantized fit the device quantizes them (seed, iter) and returns the quantized
+ * values and scales when gq/hq/scales are non-null. */
+int bh_set_gradients(bh_context *ctx, const float *grad, const float *hess, uint32_t iter,
+                     int8_t *gq, int16_t *hq, double *scales);
+/* Grow one tree from the current gradients without touching scores.  leaf_of_row
+ * (optional, n_rows entries) receives each row's leaf node id (stamp_kernel,
+ * kernels.cuh:479-488); for the levelwise grower the leaf-table index. */
+int bh_grow_tree(bh_context *ctx, bh_node *nodes, uint32_t cap, uint32_t *n_nodes,
+                 uint32_t *leaf_of_row);
+
+/* ---- kernel-level entry points (parity tests and profiling) ---- */
+/* Histogram of the rows listed (ascending) under the current gradients
+ * (launch_hist, device_context.cu:451-497).  FP32 fits write float pairs
+ * [n_features][bin_stride][2] into out_f32; quantized fits write int64 pairs into
+ * out_i64.  bin_stride = max n_bins. */
+int bh_histogram(bh_context *ctx, const uint32_t *rows, uint64_t n, float *out_f32,
+                 int64_t *out_i64);
+/* Node split find over a host histogram in the layout above (find_splits_many,
+ * device_context.cu:997-1048).  Totals are computed from the histogram unless
+ * tot_given != 0. */
+int bh_find_split(bh_context *ctx, const float *hist_f32, const int64_t *hist_i64,
+                  uint64_t n_rows, uint32_t depth, int32_t tot_given, double tot_g,
+                  double tot_h, const double *scales, bh_split *out);
+/* Stable partition of the listed rows by one split (partition_level,
+ * device_context.cu:790-881); writes the reordered row ids, returns n_left. */
+int bh_partition(bh_context *ctx, const uint32_t *rows, uint64_t n, uint32_t feature,
+                 uint32_t bin, uint32_t default_left, uint32_t *out_rows, uint64_t *n_left);
+/* Time the histogram kernel on the resident plane: builds the root histogram `reps`
+ * times and returns the mean kernel time (ms) measured with hipEvents on the kernel's
+ * own stream. */
+int bh_time_root_histogram(bh_context *ctx, uint32_t reps, float *ms_mean);
+/* Mean per-iteration device time split (ms) from hipEvents recorded by the last
+ * bh_train call when timing was enabled: [0] whole iteration. */
+int bh_set_timing(bh_context *ctx, int32_t enable);
+int bh_get_timing(bh_context *ctx, float *ms, uint32_t n);
+
+/* ---- predict and TreeSHAP (cuda_predict_plan / cuda_predict,
+ *      src/cuda/predict.cu:147-266; cuda_shap_plan / cuda_pred_contribs,
+ *      src/cuda/shap.cu:212-380) ---- */
+/* An immutable ensemble resident on one device.  Trees in bin space; last_bin[f] =
+ * n_bins[f] - 1.  Safe to share between serving streams. */
+int bh_predictor_create(int device, const bh_node *nodes, const uint32_t *tree_offsets,
+                        uint32_t n_trees, const uint16_t *last_bin, uint32_t n_features,
+                        float learning_rate, float init_score, bh_predictor **out);
+int bh_predictor_destroy(bh_predictor *p);
+/* init + lr * sum_t leaf_t(row), bit-equal to the host bin-space walk
+ * (include/bonsai/booster.hpp:831-846).  bins host, row-major [n][row_stride]. */
+int bh_predict(bh_predictor *p, const uint8_t *bins, uint64_t n, uint64_t row_stride,
+               float *out);
+/* TreeSHAP contributions: out[n][n_features + 1] (bias last), as pred_contribs
+ * (include/bonsai/booster.hpp:902-941). */
+int bh_shap(bh_predictor *p, const uint8_t *bins, uint64_t n, uint64_t row_stride,
+            double *out);
+/* Same two on device-resident inputs/outputs (already in HBM) on a caller stream. */
+int bh_predict_device(bh_predictor *p, const uint8_t *d_bins, uint64_t n, uint64_t row_stride,
+                      float *d_out, void *hip_stream);
+int bh_shap_device(bh_predictor *p, const uint8_t *d_bins, uint64_t n, uint64_t row_stride,
+                   double *d_out, void *hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

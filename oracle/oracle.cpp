@@ -1,0 +1,1694 @@
+// CPU oracle for the bonsai GBDT hot path — TEST INFRASTRUCTURE ONLY.
+//
+// Restates the reference's CPU semantics in plain C++17 so the HIP product can be
+// checked against it.  Loaded by tests/, __graft_entry__.smoke() and the cpu_baseline
+// leg of bench.py; never by the product.  Built with -ffp-contract=off so every
+// floating-point operation rounds exactly as written (the reference's host plane
+// compiles with contraction off, src/cuda/predict.cu:79-82 documents it).
+//
+// Parity status per function is recorded beside each definition: "pinned by" names
+// the test that pins it.
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+using u8  = uint8_t;
+using u16 = uint16_t;
+using u32 = uint32_t;
+using u64 = uint64_t;
+using i64 = int64_t;
+
+// ---------------------------------------------------------------------------------
+// Split math: include/bonsai/split.hpp:112-158.
+// ---------------------------------------------------------------------------------
+inline double l1_thresholded(double g, double l1)
+{
+    if (g > l1) return g - l1;
+    if (g < -l1) return g + l1;
+    return 0.0;
+}
+inline double score(double g, double h, double l1, double l2)
+{
+    double const t = l1_thresholded(g, l1);
+    double const d = h + l2;
+    return d > 0.0 ? (t * t) / d : 0.0;
+}
+inline double bounded_leaf_weight(double g, double h, double l1, double l2)
+{
+    double const d = h + l2;
+    double const w = d > 0.0 ? -l1_thresholded(g, l1) / d : 0.0;
+    return w; // bounds are +-inf without monotone constraints
+}
+
+// Routing truth: include/bonsai/dataset.hpp:234-238.
+inline bool routes_left(u32 bin, u32 last_bin, u32 split_bin, bool default_left)
+{
+    return bin == last_bin ? default_left : bin <= split_bin;
+}
+
+// ---------------------------------------------------------------------------------
+// Bin mapping: src/bin_mapper.cpp:57-232 (greedy, create_cuts) and :284-291.
+// ---------------------------------------------------------------------------------
+std::vector<float> greedy_weighted_cuts(std::vector<float> const &vals,
+                                        std::vector<size_t> const &counts,
+                                        size_t n_samples, size_t cut_budget,
+                                        double mean_bin)
+{
+    std::vector<float> cuts;
+    size_t const n_groups = cut_budget + 1;
+    std::vector<bool> is_big(vals.size());
+    size_t n_big = 0, big_sum = 0;
+    for (size_t i = 0; i < vals.size(); ++i) {
+        if (static_cast<double>(counts[i]) >= mean_bin) {
+            is_big[i] = true;
+            ++n_big;
+            big_sum += counts[i];
+        }
+    }
+    size_t rest_sum    = n_samples - big_sum;
+    size_t rest_groups = n_groups - n_big;
+    double bin_size    = rest_groups != 0U
+                             ? static_cast<double>(rest_sum) / static_cast<double>(rest_groups)
+                             : mean_bin;
+    size_t in_bin = 0;
+    for (size_t i = 0; i + 1 < vals.size() && cuts.size() < cut_budget; ++i) {
+        if (!is_big[i]) rest_sum -= counts[i];
+        in_bin += counts[i];
+        if (is_big[i] || static_cast<double>(in_bin) >= bin_size ||
+            (is_big[i + 1] && static_cast<double>(in_bin) >= bin_size / 2.0)) {
+            // std::midpoint for floats: a + (b - a) / 2 without overflow; for finite
+            // same-sign or small values it equals (a + b) / 2 rounded once.
+            float const a = vals[i], b = vals[i + 1];
+            float mid;
+            float const lo = std::numeric_limits<float>::min() * 2.0F;
+            float const hi = std::numeric_limits<float>::max() / 2.0F;
+            float const abs_a = std::fabs(a), abs_b = std::fabs(b);
+            if (abs_a <= hi && abs_b <= hi) mid = (a + b) / 2.0F;
+            else if (abs_a < lo) mid = a + b / 2.0F;
+            else if (abs_b < lo) mid = a / 2.0F + b;
+            else mid = a / 2.0F + b / 2.0F;
+            if (cuts.empty() || cuts.back() < mid) cuts.push_back(mid);
+            in_bin = 0;
+            if (!is_big[i] && rest_groups > 1) {
+                --rest_groups;
+                bin_size = static_cast<double>(rest_sum) / static_cast<double>(rest_groups);
+            }
+        }
+    }
+    return cuts;
+}
+
+std::vector<float> create_cuts(std::vector<float> &subsample, size_t cut_budget)
+{
+    std::sort(subsample.begin(), subsample.end());
+    std::vector<float> vals;
+    std::vector<size_t> counts;
+    for (float const v : subsample) {
+        if (vals.empty() || vals.back() < v) {
+            vals.push_back(v);
+            counts.push_back(0);
+        }
+        ++counts.back();
+    }
+    std::vector<float> cuts;
+    double const mean_bin =
+        static_cast<double>(subsample.size()) / static_cast<double>(cut_budget + 1);
+    if (vals.size() <= cut_budget) {
+        cuts = vals;
+    } else if (static_cast<double>(*std::max_element(counts.begin(), counts.end())) >=
+               mean_bin) {
+        cuts = greedy_weighted_cuts(vals, counts, subsample.size(), cut_budget, mean_bin);
+    } else {
+        size_t const step =
+            std::max<size_t>((subsample.size() + cut_budget) / (cut_budget + 1), 1UL);
+        for (size_t k = step; k < subsample.size(); k += step) {
+            float const v = subsample[k];
+            if (cuts.empty() || cuts.back() < v) cuts.push_back(v);
+        }
+    }
+    if (cuts.empty() || cuts.back() < std::numeric_limits<float>::max())
+        cuts.push_back(std::numeric_limits<float>::max());
+    cuts.push_back(std::numeric_limits<float>::infinity());
+    return cuts;
+}
+
+inline u32 transform_one(float const *cuts, int n_cuts, float x)
+{
+    if (std::isnan(x)) return static_cast<u32>(n_cuts - 1);
+    return static_cast<u32>(std::lower_bound(cuts, cuts + n_cuts, x) - cuts);
+}
+
+// ---------------------------------------------------------------------------------
+// Stochastic-rounding quantizer.  Counter-based: u = splitmix64(key) top 24 bits.
+// Shared definition with the product's gradient kernel (bonsai_amd/csrc).
+// ---------------------------------------------------------------------------------
+inline u64 splitmix64(u64 z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+inline float sr_uniform(u64 seed, u32 iter, u64 row, u32 channel)
+{
+    u64 const key = seed ^ (static_cast<u64>(iter) * 0xD1B54A32D192ED03ULL) ^
+                    (row * 0x9E3779B97F4A7C15ULL) ^
+                    (static_cast<u64>(channel + 1) * 0xA0761D6478BD642FULL);
+    u64 const z = splitmix64(key);
+    return static_cast<float>(static_cast<u32>(z >> 40)) * (1.0F / 16777216.0F);
+}
+
+// ---------------------------------------------------------------------------------
+// Histogram cells.  F32: the reference's HistCell{float,float}
+// (include/bonsai/histogram.hpp:24-36).  INT: exact int64 sums of quantized values.
+// ---------------------------------------------------------------------------------
+struct Dataset {
+    u8 const  *plane;
+    u64        stride;
+    u64        n_rows;
+    u32        n_features;
+    u16 const *n_bins;
+    u32        bin_stride; // cells per feature in a node histogram
+};
+
+struct Grad {
+    float const  *g  = nullptr;
+    float const  *h  = nullptr;
+    int8_t const *gq = nullptr;
+    int16_t const *hq = nullptr;
+    double        sg = 0.0;
+    double        sh = 0.0;
+};
+
+// Serial float fill in row order: bit-identical to the reference's single-block fill
+// (tests/unit/test_populate.cpp:160-182).
+void fill_f32_serial(Dataset const &ds, Grad const &gr, u32 const *rows, u64 n, float *out)
+{
+    u32 const F = ds.n_features, SB = ds.bin_stride;
+    for (u64 k = 0; k < n; ++k) {
+        u32 const r   = rows[k];
+        u8 const *row = ds.plane + static_cast<u64>(r) * ds.stride;
+        float const g = gr.g[r], h = gr.h[r];
+        for (u32 f = 0; f < F; ++f) {
+            float *c = out + (static_cast<u64>(f) * SB + row[f]) * 2;
+            c[0] += g;
+            c[1] += h;
+        }
+    }
+}
+
+// The device's fixed-order reduction: per `chunk` rows a sequential float partial,
+// partials folded in chunk order.
+void fill_f32_chunked(Dataset const &ds, Grad const &gr, u32 const *rows, u64 n, u32 chunk,
+                      float *out)
+{
+    size_t const cells = static_cast<size_t>(ds.n_features) * ds.bin_stride * 2;
+    std::vector<float> part(cells);
+    std::fill(out, out + cells, 0.0F);
+    for (u64 k0 = 0; k0 < n; k0 += chunk) {
+        u64 const k1 = std::min<u64>(n, k0 + chunk);
+        std::fill(part.begin(), part.end(), 0.0F);
+        fill_f32_serial(ds, gr, rows + k0, k1 - k0, part.data());
+        for (size_t i = 0; i < cells; ++i) out[i] += part[i];
+    }
+}
+
+void fill_int(Dataset const &ds, Grad const &gr, u32 const *rows, u64 n, i64 *out)
+{
+    u32 const F = ds.n_features, SB = ds.bin_stride;
+    for (u64 k = 0; k < n; ++k) {
+        u32 const r   = rows[k];
+        u8 const *row = ds.plane + static_cast<u64>(r) * ds.stride;
+        i64 const g = gr.gq[r], h = gr.hq[r];
+        for (u32 f = 0; f < F; ++f) {
+            i64 *c = out + (static_cast<u64>(f) * SB + row[f]) * 2;
+            c[0] += g;
+            c[1] += h;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Node split finder, float cells: src/split.cpp:33-162 and :255-297.
+// ---------------------------------------------------------------------------------
+struct Cfg {
+    double min_child_hess, min_gain, l2, l1;
+    int    max_depth, min_data_in_leaf, max_leaves;
+};
+Cfg to_cfg(orc_tree_cfg const &c)
+{
+    return {static_cast<double>(c.min_child_hess), static_cast<double>(c.min_gain_to_split),
+            static_cast<double>(c.lambda_l2), static_cast<double>(c.lambda_l1), c.max_depth,
+            c.min_data_in_leaf, c.max_leaves};
+}
+
+struct Split {
+    double gain = 0.0;
+    u32    feature = 0, bin = 0;
+    bool   default_left = true, valid = false;
+    // child sums the finder derived for the winner (split_sums_at, src/split.cpp:33-45)
+    double gL = 0, hL = 0, gR = 0, hR = 0;
+    i64    igL = 0, ihL = 0, igR = 0, ihR = 0;
+};
+
+inline bool update_best(Split &best, double gain, u32 f, u32 b, bool dl, Cfg const &c)
+{
+    if (gain > best.gain && gain >= c.min_gain) {
+        best.gain = gain;
+        best.feature = f;
+        best.bin = b;
+        best.default_left = dl;
+        best.valid = true;
+        return true;
+    }
+    return false;
+}
+
+// Candidate scoring in the reference's operation order (split_sums_at :33-45,
+// score_candidate :60-75).  Returns feasibility; children score through *cs.
+inline bool candidate(double lg, double lh, double mg, double mh, double rg, double rh,
+                      bool dl, Cfg const &c, double *cs)
+{
+    double const right_g = rg - lg;
+    double const right_h = rh - lh;
+    double const gL = lg + (dl ? mg : 0.0);
+    double const hL = lh + (dl ? mh : 0.0);
+    double const gR = right_g + (!dl ? mg : 0.0);
+    double const hR = right_h + (!dl ? mh : 0.0);
+    if (hL < c.min_child_hess || hR < c.min_child_hess) return false;
+    *cs = score(gL, hL, c.l1, c.l2) + score(gR, hR, c.l1, c.l2);
+    return true;
+}
+
+Split find_node_f32(float const *hist, u16 const *n_bins, u32 F, u32 SB, float tg, float th,
+                    u64 n_rows, Cfg const &c)
+{
+    Split best;
+    if (n_rows < 2 * static_cast<u64>(c.min_data_in_leaf)) return best;
+    double const node_score = score(tg, th, c.l1, c.l2);
+    for (u32 f = 0; f < F; ++f) {
+        Split cand;
+        u32 const nb = n_bins[f];
+        if (nb == 0) continue;
+        float const *cells = hist + static_cast<u64>(f) * SB * 2;
+        float const mg = cells[(nb - 1) * 2], mh = cells[(nb - 1) * 2 + 1];
+        double const real_g = static_cast<double>(tg) - mg;
+        double const real_h = static_cast<double>(th) - mh;
+        int const n_dirs = (mg == 0.0F && mh == 0.0F) ? 1 : 2;
+        double lg = 0.0, lh = 0.0;
+        u32 const n_cut = nb >= 2 ? nb - 2 : 0;
+        for (u32 b = 0; b < n_cut; ++b) {
+            lg += cells[b * 2];
+            lh += cells[b * 2 + 1];
+            for (int d = 0; d < n_dirs; ++d) {
+                bool const dl = d == 0;
+                double cs;
+                if (!candidate(lg, lh, mg, mh, real_g, real_h, dl, c, &cs)) continue;
+                if (update_best(cand, cs - node_score, f, b, dl, c)) {
+                    cand.gL = lg + (dl ? mg : 0.0);
+                    cand.hL = lh + (dl ? mh : 0.0);
+                    cand.gR = (real_g - lg) + (!dl ? mg : 0.0);
+                    cand.hR = (real_h - lh) + (!dl ? mh : 0.0);
+                }
+            }
+        }
+        if (cand.valid && cand.gain > best.gain) best = cand;
+    }
+    return best;
+}
+
+// Integer cells: the same candidate order, sums exact in int64 and converted to double
+// by the per-iteration scale before scoring.
+Split find_node_int(i64 const *hist, u16 const *n_bins, u32 F, u32 SB, i64 tg, i64 th,
+                    double sg, double sh, u64 n_rows, Cfg const &c)
+{
+    Split best;
+    if (n_rows < 2 * static_cast<u64>(c.min_data_in_leaf)) return best;
+    double const node_score =
+        score(static_cast<double>(tg) * sg, static_cast<double>(th) * sh, c.l1, c.l2);
+    for (u32 f = 0; f < F; ++f) {
+        Split cand;
+        u32 const nb = n_bins[f];
+        if (nb == 0) continue;
+        i64 const *cells = hist + static_cast<u64>(f) * SB * 2;
+        i64 const mg = cells[(nb - 1) * 2], mh = cells[(nb - 1) * 2 + 1];
+        i64 const real_g = tg - mg, real_h = th - mh;
+        int const n_dirs = (mg == 0 && mh == 0) ? 1 : 2;
+        i64 lg = 0, lh = 0;
+        u32 const n_cut = nb >= 2 ? nb - 2 : 0;
+        for (u32 b = 0; b < n_cut; ++b) {
+            lg += cells[b * 2];
+            lh += cells[b * 2 + 1];
+            for (int d = 0; d < n_dirs; ++d) {
+                bool const dl = d == 0;
+                i64 const gL = lg + (dl ? mg : 0), hL = lh + (dl ? mh : 0);
+                i64 const gR = (real_g - lg) + (!dl ? mg : 0);
+                i64 const hR = (real_h - lh) + (!dl ? mh : 0);
+                double const dgL = static_cast<double>(gL) * sg;
+                double const dhL = static_cast<double>(hL) * sh;
+                double const dgR = static_cast<double>(gR) * sg;
+                double const dhR = static_cast<double>(hR) * sh;
+                if (dhL < c.min_child_hess || dhR < c.min_child_hess) continue;
+                double const cs = score(dgL, dhL, c.l1, c.l2) + score(dgR, dhR, c.l1, c.l2);
+                if (update_best(cand, cs - node_score, f, b, dl, c)) {
+                    cand.igL = gL;
+                    cand.ihL = hL;
+                    cand.igR = gR;
+                    cand.ihR = hR;
+                }
+            }
+        }
+        if (cand.valid && cand.gain > best.gain) best = cand;
+    }
+    return best;
+}
+
+// Level (oblivious) finder: src/split.cpp:164-235 and :350-372.
+Split find_level_f32(float const *hists, float const *totals, u32 P, u16 const *n_bins,
+                     u32 F, u32 SB, Cfg const &c)
+{
+    Split best;
+    if (P == 0) return best;
+    size_t const node_cells = static_cast<size_t>(F) * SB * 2;
+    std::vector<double> pg, ph, rg(P), rh(P), ps(P);
+    for (u32 f = 0; f < F; ++f) {
+        Split cand;
+        u32 const nb = n_bins[f];
+        u32 const n_cut = nb >= 2 ? nb - 2 : 0;
+        if (n_cut == 0) continue;
+        pg.assign(static_cast<size_t>(P) * n_cut, 0.0);
+        ph.assign(static_cast<size_t>(P) * n_cut, 0.0);
+        double sum_parent = 0.0;
+        bool all_empty = true;
+        for (u32 p = 0; p < P; ++p) {
+            float const *cells = hists + p * node_cells + static_cast<u64>(f) * SB * 2;
+            // fill_prefix (include/bonsai/histogram.hpp:131-147): double running sums
+            // stored back as float cells.
+            double g = 0.0, h = 0.0;
+            for (u32 b = 0; b < n_cut; ++b) {
+                g = g + cells[b * 2];
+                h = h + cells[b * 2 + 1];
+                pg[p * n_cut + b] = static_cast<float>(g);
+                ph[p * n_cut + b] = static_cast<float>(h);
+            }
+            float const mg = cells[(nb - 1) * 2], mh = cells[(nb - 1) * 2 + 1];
+            ps[p] = score(totals[2 * p], totals[2 * p + 1], c.l1, c.l2);
+            sum_parent += ps[p];
+            rg[p] = static_cast<double>(totals[2 * p]) - mg;
+            rh[p] = static_cast<double>(totals[2 * p + 1]) - mh;
+            all_empty = all_empty && (mg == 0.0F && mh == 0.0F);
+        }
+        int const n_dirs = all_empty ? 1 : 2;
+        for (u32 b = 0; b < n_cut; ++b) {
+            for (int d = 0; d < n_dirs; ++d) {
+                bool const dl = d == 0;
+                double sum_children = 0.0;
+                for (u32 p = 0; p < P; ++p) {
+                    float const *cells = hists + p * node_cells + static_cast<u64>(f) * SB * 2;
+                    float const mg = cells[(nb - 1) * 2], mh = cells[(nb - 1) * 2 + 1];
+                    double cs;
+                    bool const ok = candidate(pg[p * n_cut + b], ph[p * n_cut + b], mg, mh,
+                                              rg[p], rh[p], dl, c, &cs);
+                    sum_children += ok ? cs : ps[p];
+                }
+                update_best(cand, sum_children - sum_parent, f, b, dl, c);
+            }
+        }
+        if (cand.valid && cand.gain > best.gain) best = cand;
+    }
+    return best;
+}
+
+Split find_level_int(i64 const *hists, i64 const *totals, u32 P, u16 const *n_bins, u32 F,
+                     u32 SB, double sg, double sh, Cfg const &c)
+{
+    Split best;
+    if (P == 0) return best;
+    size_t const node_cells = static_cast<size_t>(F) * SB * 2;
+    std::vector<i64> pg, ph;
+    std::vector<double> ps(P);
+    for (u32 f = 0; f < F; ++f) {
+        Split cand;
+        u32 const nb = n_bins[f];
+        u32 const n_cut = nb >= 2 ? nb - 2 : 0;
+        if (n_cut == 0) continue;
+        pg.assign(static_cast<size_t>(P) * n_cut, 0);
+        ph.assign(static_cast<size_t>(P) * n_cut, 0);
+        double sum_parent = 0.0;
+        bool all_empty = true;
+        for (u32 p = 0; p < P; ++p) {
+            i64 const *cells = hists + p * node_cells + static_cast<u64>(f) * SB * 2;
+            i64 g = 0, h = 0;
+            for (u32 b = 0; b < n_cut; ++b) {
+                g += cells[b * 2];
+                h += cells[b * 2 + 1];
+                pg[p * n_cut + b] = g;
+                ph[p * n_cut + b] = h;
+            }
+            ps[p] = score(static_cast<double>(totals[2 * p]) * sg,
+                          static_cast<double>(totals[2 * p + 1]) * sh, c.l1, c.l2);
+            sum_parent += ps[p];
+            all_empty = all_empty && (cells[(nb - 1) * 2] == 0 && cells[(nb - 1) * 2 + 1] == 0);
+        }
+        int const n_dirs = all_empty ? 1 : 2;
+        for (u32 b = 0; b < n_cut; ++b) {
+            for (int d = 0; d < n_dirs; ++d) {
+                bool const dl = d == 0;
+                double sum_children = 0.0;
+                for (u32 p = 0; p < P; ++p) {
+                    i64 const *cells = hists + p * node_cells + static_cast<u64>(f) * SB * 2;
+                    i64 const mg = cells[(nb - 1) * 2], mh = cells[(nb - 1) * 2 + 1];
+                    i64 const lg = pg[p * n_cut + b], lh = ph[p * n_cut + b];
+                    i64 const gL = lg + (dl ? mg : 0), hL = lh + (dl ? mh : 0);
+                    i64 const gR = ((totals[2 * p] - mg) - lg) + (!dl ? mg : 0);
+                    i64 const hR = ((totals[2 * p + 1] - mh) - lh) + (!dl ? mh : 0);
+                    double const dhL = static_cast<double>(hL) * sh;
+                    double const dhR = static_cast<double>(hR) * sh;
+                    if (dhL < c.min_child_hess || dhR < c.min_child_hess) {
+                        sum_children += ps[p];
+                        continue;
+                    }
+                    sum_children += score(static_cast<double>(gL) * sg, dhL, c.l1, c.l2) +
+                                    score(static_cast<double>(gR) * sg, dhR, c.l1, c.l2);
+                }
+                update_best(cand, sum_children - sum_parent, f, b, dl, c);
+            }
+        }
+        if (cand.valid && cand.gain > best.gain) best = cand;
+    }
+    return best;
+}
+
+// Histogram::totals (include/bonsai/histogram.hpp:76-87): double sum over the first
+// feature's cells, stored as float.
+void totals_f32(float const *hist, u32 nb0, float *tg, float *th)
+{
+    double g = 0.0, h = 0.0;
+    for (u32 b = 0; b < nb0; ++b) {
+        g += hist[b * 2];
+        h += hist[b * 2 + 1];
+    }
+    *tg = static_cast<float>(g);
+    *th = static_cast<float>(h);
+}
+void totals_int(i64 const *hist, u32 nb0, i64 *tg, i64 *th)
+{
+    i64 g = 0, h = 0;
+    for (u32 b = 0; b < nb0; ++b) {
+        g += hist[b * 2];
+        h += hist[b * 2 + 1];
+    }
+    *tg = g;
+    *th = h;
+}
+
+// ---------------------------------------------------------------------------------
+// Growers.  A node's histogram is kept in one of two cell types; the grower logic is
+// shared through this policy.
+// ---------------------------------------------------------------------------------
+struct NodeState {
+    u32 id = 0;
+    std::vector<u32> rows;
+    std::vector<float> hf; // float cells (F32 modes)
+    std::vector<i64> hi;   // int cells (INT mode)
+    float tg = 0.0F, th = 0.0F;
+    i64 itg = 0, ith = 0;
+    int depth = 0;
+};
+
+struct Engine {
+    Dataset ds;
+    Grad gr;
+    Cfg cfg;
+    int mode;
+    u32 chunk;
+    int n_threads;
+
+    size_t cells() const { return static_cast<size_t>(ds.n_features) * ds.bin_stride * 2; }
+
+    void fill(NodeState &n) const
+    {
+        if (mode == ORC_HIST_INT) {
+            n.hi.assign(cells(), 0);
+            fill_int(ds, gr, n.rows.data(), n.rows.size(), n.hi.data());
+            totals_int(n.hi.data(), ds.n_bins[0], &n.itg, &n.ith);
+        } else {
+            n.hf.assign(cells(), 0.0F);
+            if (mode == ORC_HIST_F32_CHUNKED)
+                fill_f32_chunked(ds, gr, n.rows.data(), n.rows.size(), chunk, n.hf.data());
+            else if (mode == ORC_HIST_F32_OMP)
+                fill_f32_omp(n.rows.data(), n.rows.size(), n.hf.data());
+            else
+                fill_f32_serial(ds, gr, n.rows.data(), n.rows.size(), n.hf.data());
+            totals_f32(n.hf.data(), ds.n_bins[0], &n.tg, &n.th);
+        }
+    }
+
+    // Row blocks of 16384 with per-block partials folded in block order: the result
+    // depends on the block size, not on scheduling (src/fill/rows.hpp:199-344 pattern).
+    void fill_f32_omp(u32 const *rows, u64 n, float *out) const
+    {
+        u64 const B = 16384;
+        u64 const nb = (n + B - 1) / B;
+        if (nb <= 1) {
+            fill_f32_serial(ds, gr, rows, n, out);
+            return;
+        }
+        size_t const C = cells();
+        std::vector<float> part(nb * C, 0.0F);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(n_threads)
+        for (long b = 0; b < static_cast<long>(nb); ++b) {
+            u64 const k0 = static_cast<u64>(b) * B;
+            u64 const k1 = std::min(n, k0 + B);
+            fill_f32_serial(ds, gr, rows + k0, k1 - k0, part.data() + static_cast<u64>(b) * C);
+        }
+#pragma omp parallel for schedule(static) num_threads(n_threads)
+        for (long i = 0; i < static_cast<long>(C); ++i) {
+            float acc = 0.0F;
+            for (u64 b = 0; b < nb; ++b) acc += part[b * C + static_cast<u64>(i)];
+            out[i] = acc;
+        }
+    }
+
+    // finish_split (src/step/primitives.hpp:156-173): larger = parent - smaller.
+    void subtract_into(NodeState &large, NodeState const &small, NodeState &parent) const
+    {
+        if (mode == ORC_HIST_INT) {
+            large.hi = std::move(parent.hi);
+            for (size_t i = 0; i < large.hi.size(); ++i) large.hi[i] -= small.hi[i];
+            totals_int(large.hi.data(), ds.n_bins[0], &large.itg, &large.ith);
+        } else {
+            large.hf = std::move(parent.hf);
+            for (size_t i = 0; i < large.hf.size(); ++i) large.hf[i] -= small.hf[i];
+            totals_f32(large.hf.data(), ds.n_bins[0], &large.tg, &large.th);
+        }
+    }
+
+    Split find(NodeState const &n) const
+    {
+        if (mode == ORC_HIST_INT)
+            return find_node_int(n.hi.data(), ds.n_bins, ds.n_features, ds.bin_stride, n.itg,
+                                 n.ith, gr.sg, gr.sh, n.rows.size(), cfg);
+        return find_node_f32(n.hf.data(), ds.n_bins, ds.n_features, ds.bin_stride, n.tg, n.th,
+                             n.rows.size(), cfg);
+    }
+
+    // Children that can never split (they sit at max_depth) take their totals from the
+    // parent split's sums instead of a histogram build: the device planes' rule
+    // (src/step/level.hpp:197-205 carries find's child sums).  Exact for integer cells;
+    // the float-cell device emulation follows it, the serial reference fill does not.
+    bool leaves_from_split() const { return mode == ORC_HIST_F32_CHUNKED || mode == ORC_HIST_INT; }
+    void totals_from_split(NodeState &L, NodeState &R, Split const &s) const
+    {
+        L.tg = static_cast<float>(s.gL);
+        L.th = static_cast<float>(s.hL);
+        R.tg = static_cast<float>(s.gR);
+        R.th = static_cast<float>(s.hR);
+        L.itg = s.igL;
+        L.ith = s.ihL;
+        R.itg = s.igR;
+        R.ith = s.ihR;
+    }
+
+    float leaf_value(NodeState const &n) const
+    {
+        double g, h;
+        if (mode == ORC_HIST_INT) {
+            g = static_cast<double>(n.itg) * gr.sg;
+            h = static_cast<double>(n.ith) * gr.sh;
+        } else {
+            g = n.tg;
+            h = n.th;
+        }
+        return static_cast<float>(bounded_leaf_weight(g, h, cfg.l1, cfg.l2));
+    }
+
+    // Stable partition (src/step/primitives.hpp:113-151): left rows ascending, then
+    // right rows ascending.  Blocked for large parents like host_partition (:262-374);
+    // the output order is the serial one at any decomposition.
+    void partition(std::vector<u32> const &rows, Split const &s, std::vector<u32> &L,
+                   std::vector<u32> &R) const
+    {
+        u32 const last = ds.n_bins[s.feature] - 1u;
+        auto goes_left = [&](u32 r) {
+            return routes_left(ds.plane[static_cast<u64>(r) * ds.stride + s.feature], last,
+                               s.bin, s.default_left);
+        };
+        u64 const n = rows.size();
+        u64 const B = 65536;
+        u64 const nb = (n + B - 1) / B;
+        if (nb <= 1 || n_threads <= 1) {
+            L.clear();
+            R.clear();
+            for (u32 r : rows) (goes_left(r) ? L : R).push_back(r);
+            return;
+        }
+        std::vector<u64> nl(nb);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(n_threads)
+        for (long b = 0; b < static_cast<long>(nb); ++b) {
+            u64 c = 0;
+            for (u64 k = b * B; k < std::min(n, (b + 1) * B); ++k) c += goes_left(rows[k]);
+            nl[b] = c;
+        }
+        std::vector<u64> l0(nb), r0(nb);
+        u64 li = 0, ri = 0;
+        for (u64 b = 0; b < nb; ++b) {
+            l0[b] = li;
+            r0[b] = ri;
+            li += nl[b];
+            ri += std::min(n, (b + 1) * B) - b * B - nl[b];
+        }
+        L.resize(li);
+        R.resize(ri);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(n_threads)
+        for (long b = 0; b < static_cast<long>(nb); ++b) {
+            u64 a = l0[b], c = r0[b];
+            for (u64 k = b * B; k < std::min(n, (b + 1) * B); ++k) {
+                u32 const r = rows[k];
+                if (goes_left(r)) L[a++] = r;
+                else R[c++] = r;
+            }
+        }
+    }
+};
+
+orc_node leaf_node(float v)
+{
+    orc_node n{};
+    n.feature = -1;
+    n.value = v;
+    return n;
+}
+
+// commit_split_node (src/grower_impl.hpp:188-210): two placeholder leaves appended,
+// the parent rewritten as the internal split.
+std::pair<u32, u32> commit_split(std::vector<orc_node> &nodes, u32 id, Split const &s)
+{
+    u32 const l = static_cast<u32>(nodes.size());
+    nodes.push_back(leaf_node(0.0F));
+    u32 const r = static_cast<u32>(nodes.size());
+    nodes.push_back(leaf_node(0.0F));
+    float const cover = nodes[id].cover;
+    nodes[id].feature = static_cast<int32_t>(s.feature);
+    nodes[id].bin = s.bin;
+    nodes[id].default_left = s.default_left ? 1u : 0u;
+    nodes[id].left = l;
+    nodes[id].right = r;
+    nodes[id].value = 0.0F;
+    nodes[id].gain = static_cast<float>(s.gain);
+    nodes[id].cover = cover;
+    return {l, r};
+}
+
+void write_leaf(std::vector<orc_node> &nodes, Engine const &e, NodeState const &n)
+{
+    float const cover = nodes[n.id].cover;
+    nodes[n.id] = leaf_node(e.leaf_value(n));
+    nodes[n.id].cover = cover;
+}
+
+// DepthwiseGrower::grow (src/grower_impl.hpp:501-577) with plan_level (:212-255),
+// demote_empty_splits (:263-297) and commit_children (:302-324), host plane.
+std::vector<orc_node> grow_depthwise(Engine const &e, std::vector<u32> *leaf_of_row)
+{
+    std::vector<orc_node> nodes;
+    nodes.push_back(leaf_node(0.0F));
+    nodes[0].cover = static_cast<float>(e.ds.n_rows);
+    std::vector<NodeState> current(1);
+    current[0].rows.resize(e.ds.n_rows);
+    std::iota(current[0].rows.begin(), current[0].rows.end(), 0u);
+    e.fill(current[0]);
+    int depth = 0;
+    std::vector<NodeState> leaves;
+    while (depth < e.cfg.max_depth) {
+        size_t const P = current.size();
+        std::vector<Split> splits(P);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(e.n_threads)
+        for (long i = 0; i < static_cast<long>(P); ++i) splits[i] = e.find(current[i]);
+        std::vector<NodeState> next;
+        struct Pending { size_t parent; NodeState L, R; u32 lid, rid; };
+        std::vector<Pending> pend;
+        for (size_t i = 0; i < P; ++i) {
+            if (!splits[i].valid) {
+                write_leaf(nodes, e, current[i]);
+                leaves.push_back(std::move(current[i]));
+                continue;
+            }
+            auto const [lid, rid] = commit_split(nodes, current[i].id, splits[i]);
+            pend.push_back({i, {}, {}, lid, rid});
+        }
+        for (auto &p : pend) {
+            NodeState &par = current[p.parent];
+            e.partition(par.rows, splits[p.parent], p.L.rows, p.R.rows);
+            p.L.id = p.lid;
+            p.R.id = p.rid;
+            p.L.depth = p.R.depth = par.depth + 1;
+        }
+        // demote_empty_splits: a split that left one child empty is a leaf again.
+        std::vector<Pending> kept;
+        for (auto &p : pend) {
+            NodeState &par = current[p.parent];
+            bool const both_empty = p.L.rows.empty() && p.R.rows.empty();
+            if (!both_empty && (p.L.rows.empty() || p.R.rows.empty())) {
+                write_leaf(nodes, e, par);
+                nodes[par.id].gain = 0.0F;
+                par.rows = p.L.rows.empty() ? std::move(p.R.rows) : std::move(p.L.rows);
+                leaves.push_back(std::move(par));
+                continue;
+            }
+            kept.push_back(std::move(p));
+        }
+        // build_children: smaller child filled (ties -> left), larger by subtraction.
+        bool const last = depth + 1 >= e.cfg.max_depth;
+        for (auto &p : kept) {
+            NodeState &par = current[p.parent];
+            bool const left_small = p.L.rows.size() <= p.R.rows.size();
+            NodeState &small = left_small ? p.L : p.R;
+            NodeState &large = left_small ? p.R : p.L;
+            if (last && e.leaves_from_split()) {
+                e.totals_from_split(p.L, p.R, splits[p.parent]);
+            } else {
+                e.fill(small);
+                e.subtract_into(large, small, par);
+            }
+            nodes[p.lid].cover = static_cast<float>(p.L.rows.size());
+            nodes[p.rid].cover = static_cast<float>(p.R.rows.size());
+            next.push_back(std::move(p.L));
+            next.push_back(std::move(p.R));
+        }
+        current = std::move(next);
+        if (current.empty()) break;
+        ++depth;
+    }
+    for (auto &n : current) {
+        write_leaf(nodes, e, n);
+        leaves.push_back(std::move(n));
+    }
+    if (leaf_of_row) {
+        leaf_of_row->assign(e.ds.n_rows, 0);
+        for (auto const &n : leaves)
+            for (u32 r : n.rows) (*leaf_of_row)[r] = n.id;
+    }
+    return nodes;
+}
+
+// LeafwiseGrower::grow (src/grower_impl.hpp:743-862): max-heap on gain, ties to the
+// lower node id; demotion in commit_pop (:370-400); heap + pending become leaves.
+std::vector<orc_node> grow_leafwise(Engine const &e, std::vector<u32> *leaf_of_row)
+{
+    struct Cand { NodeState node; Split split; };
+    std::vector<orc_node> nodes;
+    nodes.push_back(leaf_node(0.0F));
+    nodes[0].cover = static_cast<float>(e.ds.n_rows);
+    std::vector<Cand> heap, pending;
+    std::vector<NodeState> leaves;
+    Cand root;
+    root.node.rows.resize(e.ds.n_rows);
+    std::iota(root.node.rows.begin(), root.node.rows.end(), 0u);
+    e.fill(root.node);
+    root.split = e.find(root.node);
+    if (root.split.valid && e.cfg.max_depth > 0) heap.push_back(std::move(root));
+    else pending.push_back(std::move(root));
+    size_t live = 1;
+    auto budget = [&] { return e.cfg.max_leaves == 0 || live < static_cast<size_t>(e.cfg.max_leaves); };
+    while (!heap.empty() && budget()) {
+        size_t best = 0;
+        for (size_t i = 1; i < heap.size(); ++i) {
+            Cand const &a = heap[i], &b = heap[best];
+            if (a.split.gain > b.split.gain ||
+                (a.split.gain == b.split.gain && a.node.id < b.node.id))
+                best = i;
+        }
+        Cand c = std::move(heap[best]);
+        heap.erase(heap.begin() + static_cast<long>(best));
+        auto const [lid, rid] = commit_split(nodes, c.node.id, c.split);
+        NodeState L, R;
+        e.partition(c.node.rows, c.split, L.rows, R.rows);
+        L.id = lid;
+        R.id = rid;
+        L.depth = R.depth = c.node.depth + 1;
+        if (L.rows.empty() || R.rows.empty()) {
+            write_leaf(nodes, e, c.node);
+            nodes[c.node.id].gain = 0.0F;
+            leaves.push_back(std::move(c.node));
+            continue;
+        }
+        bool const left_small = L.rows.size() <= R.rows.size();
+        NodeState &small = left_small ? L : R;
+        NodeState &large = left_small ? R : L;
+        if (L.depth >= e.cfg.max_depth && e.leaves_from_split()) {
+            e.totals_from_split(L, R, c.split);
+        } else {
+            e.fill(small);
+            e.subtract_into(large, small, c.node);
+        }
+        nodes[lid].cover = static_cast<float>(L.rows.size());
+        nodes[rid].cover = static_cast<float>(R.rows.size());
+        ++live;
+        bool const may_split = L.depth < e.cfg.max_depth;
+        Split sl, sr;
+        if (may_split) {
+            sl = e.find(L);
+            sr = e.find(R);
+        }
+        Cand cl{std::move(L), sl}, cr{std::move(R), sr};
+        (cl.split.valid ? heap : pending).push_back(std::move(cl));
+        (cr.split.valid ? heap : pending).push_back(std::move(cr));
+    }
+    for (auto *v : {&heap, &pending})
+        for (auto &c : *v) {
+            write_leaf(nodes, e, c.node);
+            leaves.push_back(std::move(c.node));
+        }
+    if (leaf_of_row) {
+        leaf_of_row->assign(e.ds.n_rows, 0);
+        for (auto const &n : leaves)
+            for (u32 r : n.rows) (*leaf_of_row)[r] = n.id;
+    }
+    return nodes;
+}
+
+// Child sums of one frontier node under the level split, in the level finder's
+// arithmetic: fill_prefix rounds the running sum to float (include/bonsai/histogram.hpp:
+// 131-147), split_sums_at adds the missing cell (src/split.cpp:33-45).
+Split level_child_sums(Engine const &e, NodeState const &n, Split s)
+{
+    u32 const SB = e.ds.bin_stride, f = s.feature, nb = e.ds.n_bins[f];
+    bool const dl = s.default_left;
+    if (e.mode == ORC_HIST_INT) {
+        i64 const *c = n.hi.data() + static_cast<u64>(f) * SB * 2;
+        i64 lg = 0, lh = 0;
+        for (u32 b = 0; b <= s.bin; ++b) {
+            lg += c[b * 2];
+            lh += c[b * 2 + 1];
+        }
+        i64 const mg = c[(nb - 1) * 2], mh = c[(nb - 1) * 2 + 1];
+        s.igL = lg + (dl ? mg : 0);
+        s.ihL = lh + (dl ? mh : 0);
+        s.igR = ((n.itg - mg) - lg) + (!dl ? mg : 0);
+        s.ihR = ((n.ith - mh) - lh) + (!dl ? mh : 0);
+        return s;
+    }
+    float const *c = n.hf.data() + static_cast<u64>(f) * SB * 2;
+    double g = 0.0, h = 0.0;
+    for (u32 b = 0; b <= s.bin; ++b) {
+        g = g + c[b * 2];
+        h = h + c[b * 2 + 1];
+    }
+    double const lg = static_cast<float>(g), lh = static_cast<float>(h);
+    double const mg = c[(nb - 1) * 2], mh = c[(nb - 1) * 2 + 1];
+    double const rg = static_cast<double>(n.tg) - mg, rh = static_cast<double>(n.th) - mh;
+    s.gL = lg + (dl ? mg : 0.0);
+    s.hL = lh + (dl ? mh : 0.0);
+    s.gR = (rg - lg) + (!dl ? mg : 0.0);
+    s.hR = (rh - lh) + (!dl ? mh : 0.0);
+    return s;
+}
+
+Engine make_engine(uint8_t const *plane, uint64_t row_stride, uint64_t n_rows,
+                   uint32_t n_features, uint16_t const *n_bins, float const *g,
+                   float const *h, int8_t const *gq, int16_t const *hq, double sg, double sh,
+                   orc_tree_cfg const *cfg, int32_t hist_mode, uint32_t chunk)
+{
+    u32 SB = 0;
+    for (u32 f = 0; f < n_features; ++f) SB = std::max<u32>(SB, n_bins[f]);
+    Engine e{};
+    e.ds = {plane, row_stride, n_rows, n_features, n_bins, SB};
+    e.gr.g = g;
+    e.gr.h = h;
+    e.gr.gq = gq;
+    e.gr.hq = hq;
+    e.gr.sg = sg;
+    e.gr.sh = sh;
+    e.cfg = to_cfg(*cfg);
+    e.mode = hist_mode;
+    e.chunk = chunk == 0 ? 16384 : chunk;
+    e.n_threads = hist_mode == ORC_HIST_F32_OMP ? omp_get_max_threads() : 1;
+    return e;
+}
+
+// ---------------------------------------------------------------------------------
+// TreeSHAP Algorithm 2, bin space: src/shap.cpp:25-220.
+// ---------------------------------------------------------------------------------
+struct PathElement {
+    int feature = -1;
+    double zero_fraction = 0.0, one_fraction = 0.0, pweight = 0.0;
+};
+void extend(std::vector<PathElement> &path, size_t length, double zf, double of, int feature)
+{
+    path[length] = {feature, zf, of, length == 0 ? 1.0 : 0.0};
+    for (size_t i = length; i-- > 0;) {
+        path[i + 1].pweight += of * path[i].pweight * (static_cast<double>(i) + 1.0) /
+                               (static_cast<double>(length) + 1.0);
+        path[i].pweight = zf * path[i].pweight * (static_cast<double>(length - i)) /
+                          (static_cast<double>(length) + 1.0);
+    }
+}
+void unwind(std::vector<PathElement> &path, size_t length, size_t index)
+{
+    double const of = path[index].one_fraction;
+    double const zf = path[index].zero_fraction;
+    double next = path[length].pweight;
+    for (size_t i = length; i-- > 0;) {
+        if (of != 0.0) {
+            double const tmp = path[i].pweight;
+            path[i].pweight = next * (static_cast<double>(length) + 1.0) /
+                              ((static_cast<double>(i) + 1.0) * of);
+            next = tmp - path[i].pweight * zf * (static_cast<double>(length - i)) /
+                             (static_cast<double>(length) + 1.0);
+        } else {
+            path[i].pweight = path[i].pweight * (static_cast<double>(length) + 1.0) /
+                              (zf * static_cast<double>(length - i));
+        }
+    }
+    for (size_t i = index; i < length; ++i) {
+        path[i].feature = path[i + 1].feature;
+        path[i].zero_fraction = path[i + 1].zero_fraction;
+        path[i].one_fraction = path[i + 1].one_fraction;
+    }
+}
+double unwound_path_sum(std::vector<PathElement> const &path, size_t length, size_t index)
+{
+    double const of = path[index].one_fraction;
+    double const zf = path[index].zero_fraction;
+    double next = path[length].pweight, total = 0.0;
+    for (size_t i = length; i-- > 0;) {
+        if (of != 0.0) {
+            double const tmp = next * (static_cast<double>(length) + 1.0) /
+                               ((static_cast<double>(i) + 1.0) * of);
+            total += tmp;
+            next = path[i].pweight - tmp * zf * (static_cast<double>(length - i)) /
+                                         (static_cast<double>(length) + 1.0);
+        } else {
+            total += path[i].pweight * (static_cast<double>(length) + 1.0) /
+                     (zf * static_cast<double>(length - i));
+        }
+    }
+    return total;
+}
+
+struct ShapCtx {
+    orc_node const *nodes;
+    u8 const *row;
+    u16 const *last;
+    double *phi;
+};
+u32 hot_child(ShapCtx const &c, orc_node const &n)
+{
+    u32 const f = static_cast<u32>(n.feature);
+    bool const left = routes_left(c.row[f], c.last[f], n.bin, n.default_left != 0);
+    return left ? n.left : n.right;
+}
+void recurse(ShapCtx const &ctx, u32 node_id, std::vector<PathElement> &path, size_t length,
+             double pz, double po, int pf)
+{
+    extend(path, length, pz, po, pf);
+    ++length;
+    orc_node const &n = ctx.nodes[node_id];
+    if (n.feature < 0) {
+        for (size_t i = 1; i < length; ++i) {
+            double const w = unwound_path_sum(path, length - 1, i);
+            ctx.phi[path[i].feature] +=
+                w * (path[i].one_fraction - path[i].zero_fraction) * n.value;
+        }
+        return;
+    }
+    u32 const hot = hot_child(ctx, n);
+    u32 const cold = hot == n.left ? n.right : n.left;
+    double const cover = n.cover;
+    double const hot_frac = cover > 0.0 ? ctx.nodes[hot].cover / cover : 0.0;
+    double const cold_frac = cover > 0.0 ? ctx.nodes[cold].cover / cover : 0.0;
+    double iz = 1.0, io = 1.0;
+    for (size_t i = 1; i < length; ++i) {
+        if (path[i].feature == n.feature) {
+            iz = path[i].zero_fraction;
+            io = path[i].one_fraction;
+            unwind(path, length - 1, i);
+            --length;
+            break;
+        }
+    }
+    double const hot_zero = iz * hot_frac;
+    double const cold_zero = iz * cold_frac;
+    if (hot_zero > 0.0 || io > 0.0) {
+        auto hp = path;
+        recurse(ctx, hot, hp, length, hot_zero, io, n.feature);
+    }
+    if (cold_zero > 0.0) {
+        auto cp = path;
+        recurse(ctx, cold, cp, length, cold_zero, 0.0, n.feature);
+    }
+}
+
+double expected_impl(orc_node const *nodes, u8 const *row, u16 const *last,
+                     u8 const *in_subset, u32 id)
+{
+    orc_node const &n = nodes[id];
+    if (n.feature < 0) return n.value;
+    bool const conditioned = in_subset != nullptr && in_subset[n.feature] != 0;
+    if (conditioned) {
+        bool const left = routes_left(row[n.feature], last[n.feature], n.bin, n.default_left != 0);
+        return expected_impl(nodes, row, last, in_subset, left ? n.left : n.right);
+    }
+    double const cover = n.cover;
+    if (cover <= 0.0) return 0.0;
+    return (static_cast<double>(nodes[n.left].cover) *
+                expected_impl(nodes, row, last, in_subset, n.left) +
+            static_cast<double>(nodes[n.right].cover) *
+                expected_impl(nodes, row, last, in_subset, n.right)) /
+           cover;
+}
+
+// ---------------------------------------------------------------------------------
+// Packed leaf paths: src/shap_paths.cpp:33-278.  Element layout (8 B):
+// u16 feature | missing_ok<<15, u8 lo, u8 hi, f32 zero_fraction.  Head (12 B):
+// u32 first, u16 n_elems, u16 klass, f32 value.
+// ---------------------------------------------------------------------------------
+struct Elem { u16 feature; u8 lo, hi; float z; };
+struct Head { u32 first; u16 n; u16 klass; float value; };
+static_assert(sizeof(Elem) == 8, "elem");
+static_assert(sizeof(Head) == 12, "head");
+struct Active { u32 feature; int lo, hi; bool missing_ok; double zero; };
+
+void pack_walk(orc_node const *nodes, u16 const *last_bin, u32 id, std::vector<Active> &act,
+               std::vector<Elem> &elems, std::vector<Head> &heads, size_t &max_len)
+{
+    orc_node const &n = nodes[id];
+    if (n.feature < 0) {
+        heads.push_back({static_cast<u32>(elems.size()), static_cast<u16>(act.size()), 0, n.value});
+        for (auto const &a : act) {
+            int lo = a.lo, hi = a.hi;
+            if (hi < lo) { lo = 1; hi = 0; }
+            u16 f = static_cast<u16>(a.feature);
+            if (a.missing_ok) f |= 0x8000u;
+            elems.push_back({f, static_cast<u8>(lo), static_cast<u8>(hi), static_cast<float>(a.zero)});
+        }
+        max_len = std::max(max_len, act.size());
+        return;
+    }
+    u32 const f = static_cast<u32>(n.feature);
+    int const split = static_cast<int>(n.bin);
+    int const last = static_cast<int>(last_bin[f]);
+    double const cover = n.cover;
+    auto frac = [&](u32 child) { return cover > 0.0 ? nodes[child].cover / cover : 0.0; };
+    size_t idx = act.size();
+    for (size_t i = 0; i < act.size(); ++i)
+        if (act[i].feature == f) { idx = i; break; }
+    bool const fresh = idx == act.size();
+    if (fresh) act.push_back({f, 0, last - 1, true, 1.0});
+    Active const saved = act[idx];
+    act[idx].hi = std::min(saved.hi, split);
+    act[idx].missing_ok = saved.missing_ok && n.default_left;
+    act[idx].zero = saved.zero * frac(n.left);
+    pack_walk(nodes, last_bin, n.left, act, elems, heads, max_len);
+    act[idx].lo = std::max(saved.lo, split + 1);
+    act[idx].hi = saved.hi;
+    act[idx].missing_ok = saved.missing_ok && !n.default_left;
+    act[idx].zero = saved.zero * frac(n.right);
+    pack_walk(nodes, last_bin, n.right, act, elems, heads, max_len);
+    if (fresh) act.pop_back();
+    else act[idx] = saved;
+}
+
+std::vector<double> path_weights(size_t max_len)
+{
+    std::vector<double> w(max_len * (max_len + 1) / 2, 0.0);
+    if (max_len == 0) return w;
+    std::vector<double> binom(max_len, 0.0);
+    binom[0] = 1.0;
+    for (size_t n = 1; n <= max_len; ++n) {
+        size_t const r = n - 1;
+        if (r > 0) {
+            binom[r] = 0.0;
+            for (size_t i = r + 1; i-- > 1;) binom[i] += binom[i - 1];
+        }
+        size_t const base = r * n / 2;
+        for (size_t i = 0; i < n; ++i) w[base + i] = 1.0 / (static_cast<double>(n) * binom[i]);
+    }
+    return w;
+}
+
+void eval_paths(Elem const *elems, Head const *heads, int n_heads, int max_len,
+                u16 const *last_bin, u8 const *row, double *phi)
+{
+    if (max_len <= 0) return;
+    auto const w_all = path_weights(static_cast<size_t>(max_len));
+    std::vector<double> poly(max_len + 1), defl(max_len);
+    std::vector<char> sat(max_len);
+    for (int hI = 0; hI < n_heads; ++hI) {
+        Head const &head = heads[hI];
+        size_t const n = head.n;
+        if (n == 0) continue;
+        Elem const *el = elems + head.first;
+        poly[0] = 1.0;
+        for (size_t j = 0; j < n; ++j) {
+            u16 const tag = el[j].feature;
+            u32 const f = tag & 0x7FFFu;
+            u32 const bin = row[f];
+            bool const one = bin == last_bin[f] ? (tag & 0x8000u) != 0
+                                                : (bin >= el[j].lo && bin <= el[j].hi);
+            sat[j] = one;
+            double const z = el[j].z;
+            poly[j + 1] = 0.0;
+            for (size_t i = j + 2; i-- > 1;) poly[i] = (z * poly[i]) + (one ? poly[i - 1] : 0.0);
+            poly[0] *= z;
+        }
+        double const *w = w_all.data() + ((n - 1) * n / 2);
+        double unsat = 0.0;
+        for (size_t i = 0; i < n; ++i) unsat += w[i] * poly[i];
+        double const value = head.value;
+        for (size_t k = 0; k < n; ++k) {
+            u32 const f = el[k].feature & 0x7FFFu;
+            if (!sat[k]) {
+                phi[f] -= value * unsat;
+                continue;
+            }
+            double const z = el[k].z;
+            defl[n - 1] = poly[n];
+            for (size_t i = n - 1; i-- > 0;) defl[i] = poly[i + 1] - (z * defl[i + 1]);
+            double s = 0.0;
+            for (size_t i = 0; i < n; ++i) s += w[i] * defl[i];
+            phi[f] += value * (1.0 - z) * s;
+        }
+    }
+}
+
+} // namespace
+
+// =================================================================================
+// C ABI
+// =================================================================================
+extern "C" {
+
+// Pinned by tests/test_oracle_binning.py (golden cut lists of test_bin_mapper.cpp).
+int32_t orc_fit_cuts(const float *col, uint64_t n, uint64_t stride, int32_t max_bin,
+                     float *cuts_out, int32_t cap)
+{
+    if (max_bin <= 2) return -1;
+    std::vector<float> sample;
+    sample.reserve(n);
+    for (u64 i = 0; i < n; ++i) {
+        float const v = col[i * stride];
+        if (!std::isnan(v)) sample.push_back(v);
+    }
+    auto cuts = create_cuts(sample, static_cast<size_t>(max_bin - 2));
+    if (static_cast<int32_t>(cuts.size()) > cap) return -1;
+    std::copy(cuts.begin(), cuts.end(), cuts_out);
+    return static_cast<int32_t>(cuts.size());
+}
+
+void orc_transform(const float *cuts, int32_t n_cuts, const float *x, uint64_t n,
+                   uint64_t x_stride, uint8_t *out, uint64_t out_stride)
+{
+    for (u64 i = 0; i < n; ++i)
+        out[i * out_stride] = static_cast<u8>(transform_one(cuts, n_cuts, x[i * x_stride]));
+}
+
+// src/objective.cpp:86-101.  Pinned by tests/test_oracle_objective.py.
+void orc_logloss_grad(const float *score, const float *label, uint64_t n, float *g, float *h)
+{
+    for (u64 i = 0; i < n; ++i) {
+        float const p = 1.0F / (1.0F + std::exp(-score[i]));
+        g[i] = p - label[i];
+        h[i] = p * (1.0F - p);
+    }
+}
+// src/objective.cpp:44-58.
+void orc_mse_grad(const float *score, const float *target, uint64_t n, float *g, float *h)
+{
+    for (u64 i = 0; i < n; ++i) {
+        g[i] = score[i] - target[i];
+        h[i] = 1.0F;
+    }
+}
+// src/objective.cpp:77-84 (mse) and :122-130 (logloss): sequential float sums.
+float orc_init_score(int32_t objective, const float *label, uint64_t n)
+{
+    float s = 0.0F;
+    for (u64 i = 0; i < n; ++i) s += label[i];
+    float const m = s / static_cast<float>(n);
+    if (objective == 1) return std::log(m / (1.0F - m));
+    return m;
+}
+// src/objective.cpp:103-120.
+float orc_logloss_eval(const float *score, const float *label, uint64_t n)
+{
+    float s = 0.0F;
+    for (u64 i = 0; i < n; ++i) {
+        float const ax = std::fabs(score[i]);
+        s += std::max(0.0F, score[i]) + std::log1p(std::exp(-ax)) - (label[i] * score[i]);
+    }
+    return s / static_cast<float>(n);
+}
+
+// Pinned by tests/test_oracle_objective.py (unbiasedness, bounds, reconstruction error).
+void orc_quantize(const float *g, const float *h, uint64_t n, uint64_t seed, uint32_t iter,
+                  int8_t *gq, int16_t *hq, float *max_g, float *max_h)
+{
+    float mg = 0.0F, mh = 0.0F;
+    for (u64 i = 0; i < n; ++i) {
+        mg = std::max(mg, std::fabs(g[i]));
+        mh = std::max(mh, std::fabs(h[i]));
+    }
+    float const ig = mg > 0.0F ? 127.0F / mg : 0.0F;
+    float const ih = mh > 0.0F ? 32767.0F / mh : 0.0F;
+    for (u64 i = 0; i < n; ++i) {
+        float const tg = g[i] * ig + sr_uniform(seed, iter, i, 0);
+        float const th = h[i] * ih + sr_uniform(seed, iter, i, 1);
+        float qg = std::floor(tg), qh = std::floor(th);
+        qg = std::min(127.0F, std::max(-127.0F, qg));
+        qh = std::min(32767.0F, std::max(0.0F, qh));
+        gq[i] = static_cast<int8_t>(qg);
+        hq[i] = static_cast<int16_t>(qh);
+    }
+    *max_g = mg;
+    *max_h = mh;
+}
+
+// Brute-force double histogram: the FP64 reference the float paths are measured
+// against.
+void orc_hist_f64(const uint8_t *plane, uint64_t row_stride, uint32_t n_features,
+                  uint32_t bin_stride, const uint32_t *rows, uint64_t n_rows, const float *g,
+                  const float *h, double *out)
+{
+    std::fill(out, out + static_cast<size_t>(n_features) * bin_stride * 2, 0.0);
+    for (u64 k = 0; k < n_rows; ++k) {
+        u32 const r = rows[k];
+        for (u32 f = 0; f < n_features; ++f) {
+            double *c = out + (static_cast<u64>(f) * bin_stride + plane[r * row_stride + f]) * 2;
+            c[0] += g[r];
+            c[1] += h[r];
+        }
+    }
+}
+
+void orc_hist_f32(const uint8_t *plane, uint64_t row_stride, uint32_t n_features,
+                  uint32_t bin_stride, const uint32_t *rows, uint64_t n_rows, const float *g,
+                  const float *h, int32_t mode, uint32_t chunk, float *out)
+{
+    Dataset ds{plane, row_stride, 0, n_features, nullptr, bin_stride};
+    Grad gr;
+    gr.g = g;
+    gr.h = h;
+    std::fill(out, out + static_cast<size_t>(n_features) * bin_stride * 2, 0.0F);
+    if (mode == ORC_HIST_F32_CHUNKED) fill_f32_chunked(ds, gr, rows, n_rows, chunk, out);
+    else fill_f32_serial(ds, gr, rows, n_rows, out);
+}
+
+void orc_hist_int(const uint8_t *plane, uint64_t row_stride, uint32_t n_features,
+                  uint32_t bin_stride, const uint32_t *rows, uint64_t n_rows, const int8_t *gq,
+                  const int16_t *hq, int64_t *out)
+{
+    Dataset ds{plane, row_stride, 0, n_features, nullptr, bin_stride};
+    Grad gr;
+    gr.gq = gq;
+    gr.hq = hq;
+    std::fill(out, out + static_cast<size_t>(n_features) * bin_stride * 2, 0);
+    fill_int(ds, gr, rows, n_rows, out);
+}
+
+static void put(Split const &s, orc_split *o)
+{
+    o->gain = s.gain;
+    o->feature = s.feature;
+    o->bin = s.bin;
+    o->default_left = s.default_left ? 1u : 0u;
+    o->valid = s.valid ? 1u : 0u;
+}
+
+// Pinned by tests/test_oracle_split.py (closed forms of test_split_node.cpp).
+void orc_find_node_f32(const float *hist, const uint16_t *n_bins, uint32_t n_features,
+                       uint32_t bin_stride, float tot_g, float tot_h, uint64_t n_rows,
+                       const orc_tree_cfg *cfg, orc_split *out)
+{
+    put(find_node_f32(hist, n_bins, n_features, bin_stride, tot_g, tot_h, n_rows, to_cfg(*cfg)),
+        out);
+}
+void orc_find_node_int(const int64_t *hist, const uint16_t *n_bins, uint32_t n_features,
+                       uint32_t bin_stride, int64_t tot_g, int64_t tot_h, double scale_g,
+                       double scale_h, uint64_t n_rows, const orc_tree_cfg *cfg, orc_split *out)
+{
+    put(find_node_int(hist, n_bins, n_features, bin_stride, tot_g, tot_h, scale_g, scale_h,
+                      n_rows, to_cfg(*cfg)),
+        out);
+}
+// Pinned by tests/test_oracle_split.py (test_split_level.cpp cases).
+void orc_find_level_f32(const float *hists, const float *totals, uint32_t n_nodes,
+                        const uint16_t *n_bins, uint32_t n_features, uint32_t bin_stride,
+                        const orc_tree_cfg *cfg, orc_split *out)
+{
+    put(find_level_f32(hists, totals, n_nodes, n_bins, n_features, bin_stride, to_cfg(*cfg)),
+        out);
+}
+void orc_find_level_int(const int64_t *hists, const int64_t *totals, uint32_t n_nodes,
+                        const uint16_t *n_bins, uint32_t n_features, uint32_t bin_stride,
+                        double scale_g, double scale_h, const orc_tree_cfg *cfg, orc_split *out)
+{
+    put(find_level_int(hists, totals, n_nodes, n_bins, n_features, bin_stride, scale_g, scale_h,
+                       to_cfg(*cfg)),
+        out);
+}
+void orc_totals_f32(const float *hist, uint32_t n_bins0, float *tot_g, float *tot_h)
+{
+    totals_f32(hist, n_bins0, tot_g, tot_h);
+}
+float orc_leaf_value(double g, double h, const orc_tree_cfg *cfg)
+{
+    return static_cast<float>(bounded_leaf_weight(g, h, cfg->lambda_l1, cfg->lambda_l2));
+}
+
+// Pinned by tests/test_oracle_partition.py (definition + test_partition.cpp invariant).
+uint64_t orc_partition(const uint8_t *plane, uint64_t row_stride, const uint32_t *rows,
+                       uint64_t n, uint32_t feature, uint32_t split_bin, uint32_t default_left,
+                       uint32_t last_bin, uint32_t *out)
+{
+    u64 nl = 0;
+    for (u64 k = 0; k < n; ++k)
+        nl += routes_left(plane[rows[k] * row_stride + feature], last_bin, split_bin, default_left != 0);
+    u64 li = 0, ri = nl;
+    for (u64 k = 0; k < n; ++k) {
+        u32 const r = rows[k];
+        if (routes_left(plane[r * row_stride + feature], last_bin, split_bin, default_left != 0))
+            out[li++] = r;
+        else
+            out[ri++] = r;
+    }
+    return nl;
+}
+
+static int32_t emit(std::vector<orc_node> const &v, orc_node *nodes, int32_t cap)
+{
+    if (static_cast<int32_t>(v.size()) > cap) return -1;
+    std::copy(v.begin(), v.end(), nodes);
+    return static_cast<int32_t>(v.size());
+}
+
+// Pinned by tests/test_oracle_growers.py (separable trees of test_grower.cpp, brute
+// force best split on tiny data, invariants).
+int32_t orc_grow_depthwise(const uint8_t *plane, uint64_t row_stride, uint64_t n_rows,
+                           uint32_t n_features, const uint16_t *n_bins, const float *g,
+                           const float *h, const int8_t *gq, const int16_t *hq, double scale_g,
+                           double scale_h, const orc_tree_cfg *cfg, int32_t hist_mode,
+                           uint32_t chunk, orc_node *nodes, int32_t cap, uint32_t *leaf_of_row)
+{
+    Engine e = make_engine(plane, row_stride, n_rows, n_features, n_bins, g, h, gq, hq, scale_g,
+                           scale_h, cfg, hist_mode, chunk);
+    std::vector<u32> lor;
+    auto v = grow_depthwise(e, leaf_of_row ? &lor : nullptr);
+    if (leaf_of_row) std::copy(lor.begin(), lor.end(), leaf_of_row);
+    return emit(v, nodes, cap);
+}
+
+int32_t orc_grow_leafwise(const uint8_t *plane, uint64_t row_stride, uint64_t n_rows,
+                          uint32_t n_features, const uint16_t *n_bins, const float *g,
+                          const float *h, const int8_t *gq, const int16_t *hq, double scale_g,
+                          double scale_h, const orc_tree_cfg *cfg, int32_t hist_mode,
+                          uint32_t chunk, orc_node *nodes, int32_t cap, uint32_t *leaf_of_row)
+{
+    Engine e = make_engine(plane, row_stride, n_rows, n_features, n_bins, g, h, gq, hq, scale_g,
+                           scale_h, cfg, hist_mode, chunk);
+    std::vector<u32> lor;
+    auto v = grow_leafwise(e, leaf_of_row ? &lor : nullptr);
+    if (leaf_of_row) std::copy(lor.begin(), lor.end(), leaf_of_row);
+    return emit(v, nodes, cap);
+}
+
+// ObliviousGrower::grow (src/grower_impl.hpp:598-734), host plane.
+int32_t orc_grow_levelwise(const uint8_t *plane, uint64_t row_stride, uint64_t n_rows,
+                           uint32_t n_features, const uint16_t *n_bins, const float *g,
+                           const float *h, const int8_t *gq, const int16_t *hq, double scale_g,
+                           double scale_h, const orc_tree_cfg *cfg, int32_t hist_mode,
+                           uint32_t chunk, uint32_t *level_out, float *leaf_values,
+                           float *leaf_covers, float *level_gains, uint32_t *leaf_of_row)
+{
+    Engine e = make_engine(plane, row_stride, n_rows, n_features, n_bins, g, h, gq, hq, scale_g,
+                           scale_h, cfg, hist_mode, chunk);
+    u32 const F = e.ds.n_features, SB = e.ds.bin_stride;
+    size_t const C = e.cells();
+    std::vector<NodeState> frontier(1);
+    frontier[0].rows.resize(n_rows);
+    std::iota(frontier[0].rows.begin(), frontier[0].rows.end(), 0u);
+    e.fill(frontier[0]);
+    int depth = 0;
+    while (depth < e.cfg.max_depth) {
+        u32 const P = static_cast<u32>(frontier.size());
+        Split s;
+        if (hist_mode == ORC_HIST_INT) {
+            std::vector<i64> hs(P * C), tot(2 * P);
+            for (u32 p = 0; p < P; ++p) {
+                std::copy(frontier[p].hi.begin(), frontier[p].hi.end(), hs.begin() + p * C);
+                tot[2 * p] = frontier[p].itg;
+                tot[2 * p + 1] = frontier[p].ith;
+            }
+            s = find_level_int(hs.data(), tot.data(), P, n_bins, F, SB, scale_g, scale_h, e.cfg);
+        } else {
+            std::vector<float> hs(P * C), tot(2 * P);
+            for (u32 p = 0; p < P; ++p) {
+                std::copy(frontier[p].hf.begin(), frontier[p].hf.end(), hs.begin() + p * C);
+                tot[2 * p] = frontier[p].tg;
+                tot[2 * p + 1] = frontier[p].th;
+            }
+            s = find_level_f32(hs.data(), tot.data(), P, n_bins, F, SB, e.cfg);
+        }
+        if (!s.valid) break;
+        level_out[3 * depth] = s.feature;
+        level_out[3 * depth + 1] = s.bin;
+        level_out[3 * depth + 2] = s.default_left ? 1u : 0u;
+        level_gains[depth] = static_cast<float>(s.gain);
+        std::vector<NodeState> next;
+        next.reserve(2 * P);
+        bool const last = depth + 1 >= e.cfg.max_depth;
+        for (u32 p = 0; p < P; ++p) {
+            NodeState L, R;
+            e.partition(frontier[p].rows, s, L.rows, R.rows);
+            bool const left_small = L.rows.size() <= R.rows.size();
+            NodeState &small = left_small ? L : R;
+            NodeState &large = left_small ? R : L;
+            if (last && e.leaves_from_split()) {
+                e.totals_from_split(L, R, level_child_sums(e, frontier[p], s));
+            } else {
+                e.fill(small);
+                e.subtract_into(large, small, frontier[p]);
+            }
+            next.push_back(std::move(L));
+            next.push_back(std::move(R));
+        }
+        frontier = std::move(next);
+        ++depth;
+    }
+    for (size_t li = 0; li < frontier.size(); ++li) {
+        leaf_values[li] = e.leaf_value(frontier[li]);
+        leaf_covers[li] = static_cast<float>(frontier[li].rows.size());
+        if (leaf_of_row)
+            for (u32 r : frontier[li].rows) leaf_of_row[r] = static_cast<u32>(li);
+    }
+    return depth;
+}
+
+// Booster::update_one_iter (include/bonsai/booster.hpp:500-592) with the all-rows
+// sampler and the depthwise grower: the CPU baseline fit.
+int32_t orc_train_depthwise(const uint8_t *plane, uint64_t row_stride, uint64_t n_rows,
+                            uint32_t n_features, const uint16_t *n_bins, const float *label,
+                            int32_t objective, uint32_t n_iters, float learning_rate,
+                            const orc_tree_cfg *cfg, int32_t hist_mode, int32_t n_threads,
+                            orc_node *trees, int32_t max_nodes, int32_t *tree_sizes,
+                            float *init_score_out, float *scores_out)
+{
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+    std::vector<float> scores(n_rows), g(n_rows), h(n_rows);
+    float const init = orc_init_score(objective, label, n_rows);
+    std::fill(scores.begin(), scores.end(), init);
+    std::vector<u32> lor;
+    for (u32 it = 0; it < n_iters; ++it) {
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < static_cast<long>(n_rows); ++i) {
+            if (objective == 1) {
+                float const p = 1.0F / (1.0F + std::exp(-scores[i]));
+                g[i] = p - label[i];
+                h[i] = p * (1.0F - p);
+            } else {
+                g[i] = scores[i] - label[i];
+                h[i] = 1.0F;
+            }
+        }
+        Engine e = make_engine(plane, row_stride, n_rows, n_features, n_bins, g.data(), h.data(),
+                               nullptr, nullptr, 0.0, 0.0, cfg, hist_mode, 16384);
+        e.n_threads = omp_get_max_threads();
+        auto nodes = grow_depthwise(e, &lor);
+        if (static_cast<int32_t>(nodes.size()) > max_nodes) return -1;
+        std::copy(nodes.begin(), nodes.end(), trees + static_cast<size_t>(it) * max_nodes);
+        tree_sizes[it] = static_cast<int32_t>(nodes.size());
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < static_cast<long>(n_rows); ++i)
+            scores[i] += learning_rate * nodes[lor[i]].value;
+    }
+    *init_score_out = init;
+    if (scores_out) std::copy(scores.begin(), scores.end(), scores_out);
+    return static_cast<int32_t>(n_iters);
+}
+
+// Pinned by tests/test_oracle_predict.py (hand-built trees, tree-by-tree sums).
+void orc_predict(const orc_node *trees, const int32_t *tree_offsets, uint32_t n_trees,
+                 const uint8_t *plane, uint64_t row_stride, uint64_t n_rows,
+                 const uint16_t *last_bin, float learning_rate, float init_score, float *out)
+{
+#pragma omp parallel for schedule(static)
+    for (long r = 0; r < static_cast<long>(n_rows); ++r) {
+        u8 const *row = plane + static_cast<u64>(r) * row_stride;
+        float acc = 0.0F;
+        for (u32 t = 0; t < n_trees; ++t) {
+            orc_node const *nd = trees + tree_offsets[t];
+            u32 idx = 0;
+            while (nd[idx].feature >= 0) {
+                orc_node const &n = nd[idx];
+                idx = routes_left(row[n.feature], last_bin[n.feature], n.bin,
+                                  n.default_left != 0)
+                          ? n.left
+                          : n.right;
+            }
+            acc += nd[idx].value;
+        }
+        out[r] = init_score + (learning_rate * acc);
+    }
+}
+
+void orc_predict_oblivious(const uint32_t *levels, const int32_t *depths,
+                           const int32_t *level_offsets, const float *leaves,
+                           const int32_t *leaf_offsets, uint32_t n_trees, const uint8_t *plane,
+                           uint64_t row_stride, uint64_t n_rows, const uint16_t *last_bin,
+                           float learning_rate, float init_score, float *out)
+{
+#pragma omp parallel for schedule(static)
+    for (long r = 0; r < static_cast<long>(n_rows); ++r) {
+        u8 const *row = plane + static_cast<u64>(r) * row_stride;
+        float acc = 0.0F;
+        for (u32 t = 0; t < n_trees; ++t) {
+            u32 const *lv = levels + 3 * level_offsets[t];
+            u32 index = 0;
+            for (int d = 0; d < depths[t]; ++d) {
+                bool const left = routes_left(row[lv[3 * d]], last_bin[lv[3 * d]], lv[3 * d + 1],
+                                              lv[3 * d + 2] != 0);
+                index = (index << 1u) | (left ? 0u : 1u);
+            }
+            acc += leaves[leaf_offsets[t] + index];
+        }
+        out[r] = init_score + (learning_rate * acc);
+    }
+}
+
+double orc_tree_expected_value(const orc_node *nodes)
+{
+    return expected_impl(nodes, nullptr, nullptr, nullptr, 0);
+}
+
+// Pinned by tests/test_oracle_shap.py (brute-force Shapley, additivity, ice-cream tree).
+void orc_tree_shap(const orc_node *nodes, int32_t max_depth, const uint8_t *row_bins,
+                   const uint16_t *last_bin, uint32_t n_features, double *phi)
+{
+    phi[n_features] += orc_tree_expected_value(nodes);
+    std::vector<PathElement> path(static_cast<size_t>(max_depth) + 2);
+    ShapCtx ctx{nodes, row_bins, last_bin, phi};
+    recurse(ctx, 0, path, 0, 1.0, 1.0, -1);
+}
+
+double orc_cond_expectation(const orc_node *nodes, const uint8_t *row_bins,
+                            const uint16_t *last_bin, const uint8_t *in_subset)
+{
+    return expected_impl(nodes, row_bins, last_bin, in_subset, 0);
+}
+
+int32_t orc_pack_paths(const orc_node *trees, const int32_t *tree_offsets, uint32_t n_trees,
+                       const uint16_t *last_bin, uint8_t *elems, int32_t elem_cap,
+                       uint8_t *heads, int32_t head_cap, int32_t *n_heads_out,
+                       int32_t *max_len_out)
+{
+    std::vector<Elem> ev;
+    std::vector<Head> hv;
+    size_t max_len = 0;
+    std::vector<Active> act;
+    for (u32 t = 0; t < n_trees; ++t) {
+        act.clear();
+        pack_walk(trees + tree_offsets[t], last_bin, 0, act, ev, hv, max_len);
+    }
+    if (static_cast<int32_t>(ev.size()) > elem_cap || static_cast<int32_t>(hv.size()) > head_cap)
+        return -1;
+    std::memcpy(elems, ev.data(), ev.size() * sizeof(Elem));
+    std::memcpy(heads, hv.data(), hv.size() * sizeof(Head));
+    *n_heads_out = static_cast<int32_t>(hv.size());
+    *max_len_out = static_cast<int32_t>(max_len);
+    return static_cast<int32_t>(ev.size());
+}
+
+void orc_shap_path_weights(uint32_t max_len, double *out)
+{
+    auto w = path_weights(max_len);
+    std::copy(w.begin(), w.end(), out);
+}
+
+void orc_eval_paths(const uint8_t *elems, const uint8_t *heads, int32_t n_heads,
+                    int32_t max_len, const uint16_t *last_bin, const uint8_t *row_bins,
+                    double *phi)
+{
+    eval_paths(reinterpret_cast<Elem const *>(elems), reinterpret_cast<Head const *>(heads),
+               n_heads, max_len, last_bin, row_bins, phi);
+}
+
+void orc_pred_contribs(const orc_node *trees, const int32_t *tree_offsets, uint32_t n_trees,
+                       int32_t max_depth, const uint8_t *plane, uint64_t row_stride,
+                       uint64_t n_rows, const uint16_t *last_bin, uint32_t n_features,
+                       float learning_rate, float init_score, double *out)
+{
+    size_t const cols = n_features + 1;
+    std::vector<double> biases(n_trees);
+    for (u32 t = 0; t < n_trees; ++t) biases[t] = orc_tree_expected_value(trees + tree_offsets[t]);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long r = 0; r < static_cast<long>(n_rows); ++r) {
+        double *phi = out + static_cast<size_t>(r) * cols;
+        std::fill(phi, phi + cols, 0.0);
+        std::vector<PathElement> path(static_cast<size_t>(max_depth) + 2);
+        for (u32 t = 0; t < n_trees; ++t) {
+            orc_node const *nd = trees + tree_offsets[t];
+            phi[n_features] += biases[t];
+            ShapCtx ctx{nd, plane + static_cast<u64>(r) * row_stride, last_bin, phi};
+            recurse(ctx, 0, path, 0, 1.0, 1.0, -1);
+        }
+        for (size_t c = 0; c < cols; ++c) phi[c] *= learning_rate;
+        phi[n_features] += init_score;
+    }
+}
+
+// Pinned by tests/test_oracle_metric.py (brute-force O(n^2) pair count).
+float orc_auc(const float *probs, const float *labels, uint64_t n)
+{
+    std::vector<size_t> order(n);
+    std::iota(order.begin(), order.end(), size_t{0});
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return probs[a] < probs[b]; });
+    double rank_sum_pos = 0.0;
+    size_t n_pos = 0, i = 0;
+    while (i < n) {
+        size_t j = i;
+        while (j < n && probs[order[j]] == probs[order[i]]) ++j;
+        double const avg_rank = 0.5 * static_cast<double>(i + 1 + j);
+        for (size_t k = i; k < j; ++k)
+            if (labels[order[k]] >= 0.5F) {
+                rank_sum_pos += avg_rank;
+                ++n_pos;
+            }
+        i = j;
+    }
+    size_t const n_neg = n - n_pos;
+    if (n_pos == 0 || n_neg == 0) return 1.0F;
+    double const u = rank_sum_pos - (static_cast<double>(n_pos) * static_cast<double>(n_pos + 1) / 2.0);
+    return static_cast<float>(u / (static_cast<double>(n_pos) * static_cast<double>(n_neg)));
+}
+
+} // extern "C"
