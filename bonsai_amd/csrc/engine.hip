@@ -101,7 +101,9 @@ struct bh_context {
     u32 hist_waves() const
     {
         u32 const groups = (std::min(F, hist_fsl()) + 3u) / 4u;
-        return std::max(1u, std::min(groups, 16u));
+        // Quantized cells are order-free: two waves share each feature group.
+        u32 const w = quant ? 2u * groups : groups;
+        return std::max(1u, std::min(w, 16u));
     }
     size_t cell_bytes() const { return quant ? 16 : 8; }
     size_t part_bytes() const { return 8; }

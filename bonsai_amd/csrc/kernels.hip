@@ -8,6 +8,7 @@
 // Reference behaviour each kernel reproduces is cited per kernel.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <algorithm>
 #include "bh_common.hpp"
 #include "kernels.hpp"
 
@@ -133,6 +134,28 @@ __global__ void k_zero_u32(u32 *p, u32 n)
 
 // max |g| and max h as float bits (non-negative floats order like their bit patterns,
 // so the max is independent of the reduction order).
+__device__ __forceinline__ void block_max2_atomic(float mg, float mh, u32 *maxbits)
+{
+    __shared__ float s_m[2][16];
+    for (int o = 32; o > 0; o >>= 1) {
+        mg = fmaxf(mg, __shfl_xor(mg, o));
+        mh = fmaxf(mh, __shfl_xor(mh, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_m[0][threadIdx.x >> 6] = mg;
+        s_m[1][threadIdx.x >> 6] = mh;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (u32 w = 1; w < blockDim.x / 64; ++w) {
+            mg = fmaxf(mg, s_m[0][w]);
+            mh = fmaxf(mh, s_m[1][w]);
+        }
+        atomicMax(&maxbits[0], __float_as_uint(mg));
+        atomicMax(&maxbits[1], __float_as_uint(mh));
+    }
+}
+
 __global__ void k_grad_max(float const *__restrict__ score, float const *__restrict__ label,
                            u64 n, int objective, u32 *__restrict__ maxbits)
 {
@@ -144,14 +167,7 @@ __global__ void k_grad_max(float const *__restrict__ score, float const *__restr
         mg = fmaxf(mg, fabsf(g));
         mh = fmaxf(mh, fabsf(h));
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        mg = fmaxf(mg, __shfl_xor(mg, o));
-        mh = fmaxf(mh, __shfl_xor(mh, o));
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicMax(&maxbits[0], __float_as_uint(mg));
-        atomicMax(&maxbits[1], __float_as_uint(mh));
-    }
+    block_max2_atomic(mg, mh, maxbits);
 }
 
 // Quantizer shared with the oracle (oracle/oracle.cpp orc_quantize): int8 g / int16 h,
@@ -216,14 +232,7 @@ __global__ void k_gh_max(float2 const *__restrict__ gh, u64 n, u32 *__restrict__
         mg = fmaxf(mg, fabsf(v.x));
         mh = fmaxf(mh, fabsf(v.y));
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        mg = fmaxf(mg, __shfl_xor(mg, o));
-        mh = fmaxf(mh, __shfl_xor(mh, o));
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicMax(&maxbits[0], __float_as_uint(mg));
-        atomicMax(&maxbits[1], __float_as_uint(mh));
-    }
+    block_max2_atomic(mg, mh, maxbits);
 }
 
 // ---------------------------------------------------------------------------------
@@ -271,7 +280,14 @@ __global__ __launch_bounds__(1024) void k_hist(
         __syncthreads();
         u8 const *plane = ch.src == 0 ? pl0 : (ch.src == 1 ? pl1 : plp);
         void const *ghv = ch.src == 0 ? gw0 : (ch.src == 1 ? gw1 : gwp);
-        for (u32 base = ch.begin; base < ch.end; base += 64u * U) {
+        // Waves per feature group: with more waves than groups, the waves sharing a
+        // group split its row stream (quantized cells only: order-free).
+        u32 const wpg = NW >= groups ? NW / groups : 1u;
+        u32 const sub = NW >= groups ? wave / groups : 0u;
+        u32 const g_first = NW >= groups ? wave % groups : wave;
+        u32 const g_step = NW >= groups ? groups : NW;
+        bool const active = NW < groups || wave < wpg * groups;
+        for (u32 base = ch.begin + sub * 64u * U; active && base < ch.end; base += wpg * 64u * U) {
             u32 rows[U];
             bool ok[U];
 #pragma unroll
@@ -295,7 +311,7 @@ __global__ __launch_bounds__(1024) void k_hist(
                     hf[u] = v.y;
                 }
             }
-            for (u32 g = wave; g < groups; g += NW) {
+            for (u32 g = g_first; g < groups; g += g_step) {
                 u32 d[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u)
@@ -374,9 +390,20 @@ __global__ void k_finish(FinishOp const *__restrict__ ops, TreeCtl const *__rest
                     g = nx[(static_cast<size_t>(op.small_slot) * node_cells + i) * 2];
                     h = nx[(static_cast<size_t>(op.small_slot) * node_cells + i) * 2 + 1];
                 } else {
-                    u64 const *pp = reinterpret_cast<u64 const *>(partials);
-                    for (u32 p = 0; p < op.part_count; ++p) {
-                        u64 const v = pp[static_cast<size_t>(op.part_first + p) * node_cells + i];
+                    u64 const *pp = reinterpret_cast<u64 const *>(partials) + i;
+                    u32 p = 0;
+                    for (; p + 8 <= op.part_count; p += 8) {
+                        u64 v[8];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) v[k] = pp[static_cast<size_t>(op.part_first + p + k) * node_cells];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            g += static_cast<int32_t>(static_cast<u32>(v[k] >> 32));
+                            h += static_cast<i64>(v[k] & 0xFFFFFFFFull);
+                        }
+                    }
+                    for (; p < op.part_count; ++p) {
+                        u64 const v = pp[static_cast<size_t>(op.part_first + p) * node_cells];
                         g += static_cast<int32_t>(static_cast<u32>(v >> 32));
                         h += static_cast<i64>(v & 0xFFFFFFFFull);
                     }
@@ -398,9 +425,20 @@ __global__ void k_finish(FinishOp const *__restrict__ ops, TreeCtl const *__rest
                 } else if (op.part_count == 0) {
                     s = nx[static_cast<size_t>(op.small_slot) * node_cells + i];
                 } else {
-                    float2 const *pp = reinterpret_cast<float2 const *>(partials);
-                    for (u32 p = 0; p < op.part_count; ++p) {
-                        float2 const v = pp[static_cast<size_t>(op.part_first + p) * node_cells + i];
+                    float2 const *pp = reinterpret_cast<float2 const *>(partials) + i;
+                    u32 p = 0;
+                    for (; p + 8 <= op.part_count; p += 8) {
+                        float2 v[8];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) v[k] = pp[static_cast<size_t>(op.part_first + p + k) * node_cells];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            s.x += v[k].x;
+                            s.y += v[k].y;
+                        }
+                    }
+                    for (; p < op.part_count; ++p) {
+                        float2 const v = pp[static_cast<size_t>(op.part_first + p) * node_cells];
                         s.x += v.x;
                         s.y += v.y;
                     }
@@ -431,7 +469,11 @@ __global__ void k_finish(FinishOp const *__restrict__ ops, TreeCtl const *__rest
 // candidate is scored in parallel; phase C reduces to the winner with the serial
 // scan's tie order.
 // ---------------------------------------------------------------------------------
-constexpr u32 kFindTile = 12;
+constexpr u32 kFindTile = 7;
+
+template <bool Q> struct CellT;
+template <> struct CellT<false> { typedef float2 T; typedef double A; };
+template <> struct CellT<true> { struct T { i64 x, y; }; typedef i64 A; };
 
 template <bool Q>
 __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, TreeCtl const *__restrict__ ctl,
@@ -439,108 +481,203 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
                                               u32 F, u32 SB, GrowCfg cfg, double const *__restrict__ scales,
                                               SplitRec *__restrict__ out, u32 front_base, u32 n_front_override)
 {
+    typedef typename CellT<Q>::T Cell;
+    typedef typename CellT<Q>::A Acc;
     extern __shared__ __align__(16) unsigned char smem[];
-    u32 const n_front = n_front_override ? n_front_override : ctl->n_front;
-    u32 const i = front_base + blockIdx.x;
-    if (blockIdx.x >= n_front) return;
+    Cell *s_cells = reinterpret_cast<Cell *>(smem);              // [kFindTile][SB]
+    Acc *pg = reinterpret_cast<Acc *>(s_cells + kFindTile * SB); // [kFindTile][SB]
+    Acc *ph = pg + kFindTile * SB;                                // [kFindTile][SB]
     __shared__ double s_best_gain[256];
     __shared__ u32 s_best_key[256];
     __shared__ double s_tot[2];
     __shared__ i64 s_itot[2];
+    __shared__ Acc s_mg[kFindTile], s_mh[kFindTile];
+    __shared__ u32 s_ncut[kFindTile], s_dirs[kFindTile];
+
+    u32 const n_front = n_front_override ? n_front_override : ctl->n_front;
+    if (blockIdx.x >= n_front) return;
+    u32 const i = front_base + blockIdx.x;
     NodeInfo node = front[i];
     size_t const node_cells = static_cast<size_t>(F) * SB;
+    Cell const *hcells = reinterpret_cast<Cell const *>(hist) + static_cast<size_t>(node.slot) * node_cells;
     double const sg = Q ? scales[0] : 0.0, sh = Q ? scales[1] : 0.0;
-
-    if (threadIdx.x == 0) {
-        if (!node.totals_ready) {
-            if (Q) {
-                i64 const *c = reinterpret_cast<i64 const *>(hist) + static_cast<size_t>(node.slot) * node_cells * 2;
-                i64 g = 0, h = 0;
-                for (u32 b = 0; b < n_bins[0]; ++b) {
-                    g += c[2 * b];
-                    h += c[2 * b + 1];
-                }
-                node.itg = g;
-                node.ith = h;
-            } else {
-                float2 const *c = reinterpret_cast<float2 const *>(hist) + static_cast<size_t>(node.slot) * node_cells;
-                double g = 0.0, h = 0.0;
-                for (u32 b = 0; b < n_bins[0]; ++b) {
-                    float2 const v = c[b];
-                    g += v.x;
-                    h += v.y;
-                }
-                node.tg = static_cast<float>(g);
-                node.th = static_cast<float>(h);
-            }
-            node.totals_ready = 1;
-            front[i] = node;
-        }
-        s_tot[0] = node.tg;
-        s_tot[1] = node.th;
-        s_itot[0] = node.itg;
-        s_itot[1] = node.ith;
-    }
-    __syncthreads();
-    SplitRec rec{};
-    rec.default_left = 1;
     bool const can = node.cnt >= 2u * static_cast<u32>(cfg.min_data_in_leaf) &&
                      static_cast<int>(node.depth) < cfg.max_depth;
-    if (!can) {
-        if (threadIdx.x == 0) out[i] = rec;
-        return;
-    }
-    double const tg = Q ? static_cast<double>(s_itot[0]) * sg : s_tot[0];
-    double const th = Q ? static_cast<double>(s_itot[1]) * sh : s_tot[1];
-    double const node_score = score_of(tg, th, cfg.l1, cfg.l2);
-
-    // LDS: prefix (g, h) per (tile feature, cut bin) + per-feature missing cell.
-    double *pg = reinterpret_cast<double *>(smem);            // [kFindTile][SB]
-    double *ph = pg + kFindTile * SB;                          // [kFindTile][SB]
-    i64 *ipg = reinterpret_cast<i64 *>(smem);                  // aliases for Q
-    i64 *iph = ipg + kFindTile * SB;
-    __shared__ double s_mg[kFindTile], s_mh[kFindTile];
-    __shared__ i64 s_img[kFindTile], s_imh[kFindTile];
-    __shared__ u32 s_ncut[kFindTile], s_dirs[kFindTile];
 
     double best_gain = 0.0;
     u32 best_key = kNone;
+    double bgL = 0.0, bhL = 0.0, bgR = 0.0, bhR = 0.0; // winner's child sums (FP32 path)
+    i64 bigL = 0, bihL = 0, bigR = 0, bihR = 0;        // winner's child sums (quantized)
+    __shared__ SplitRec s_rec;
+    double node_score = 0.0, tg = 0.0, th = 0.0;
     for (u32 f0 = 0; f0 < F; f0 += kFindTile) {
+        u32 const nt = min(kFindTile, F - f0);
+        // Stage the tile's cells (coalesced).
+        for (u32 k = threadIdx.x; k < nt * SB; k += blockDim.x) s_cells[k] = hcells[static_cast<size_t>(f0) * SB + k];
+        __syncthreads();
+        if (f0 == 0) {
+            if (Q && !node.totals_ready) {
+                // Exact integer totals: any order (one wave reduces feature 0's cells).
+                if (threadIdx.x < 64) {
+                    i64 g = 0, h = 0;
+                    for (u32 b = threadIdx.x; b < n_bins[0]; b += 64) {
+                        g += reinterpret_cast<i64 const *>(&s_cells[b])[0];
+                        h += reinterpret_cast<i64 const *>(&s_cells[b])[1];
+                    }
+                    for (int o = 32; o > 0; o >>= 1) {
+                        g += __shfl_xor(g, o);
+                        h += __shfl_xor(h, o);
+                    }
+                    if (threadIdx.x == 0) {
+                        node.itg = g;
+                        node.ith = h;
+                    }
+                }
+            }
+            if (threadIdx.x == 0) {
+                if (!node.totals_ready) {
+                    // Histogram::totals (include/bonsai/histogram.hpp:76-87): the first
+                    // feature's cells, double in bin order, stored as float.
+                    if (!Q) {
+                        double g = 0.0, h = 0.0;
+                        float2 const *cf = reinterpret_cast<float2 const *>(s_cells);
+                        u32 const nb0 = n_bins[0];
+                        u32 b = 0;
+                        for (; b + 8 <= nb0; b += 8) {
+                            float2 v[8];
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) v[k] = cf[b + k];
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) {
+                                g += v[k].x;
+                                h += v[k].y;
+                            }
+                        }
+                        for (; b < nb0; ++b) {
+                            g += cf[b].x;
+                            h += cf[b].y;
+                        }
+                        node.tg = static_cast<float>(g);
+                        node.th = static_cast<float>(h);
+                    }
+                    node.totals_ready = 1;
+                    front[i] = node;
+                }
+                s_tot[0] = node.tg;
+                s_tot[1] = node.th;
+                s_itot[0] = node.itg;
+                s_itot[1] = node.ith;
+            }
+            __syncthreads();
+            if (!can) {
+                if (threadIdx.x == 0) {
+                    SplitRec rec{};
+                    rec.default_left = 1;
+                    out[i] = rec;
+                }
+                return;
+            }
+            tg = Q ? static_cast<double>(s_itot[0]) * sg : s_tot[0];
+            th = Q ? static_cast<double>(s_itot[1]) * sh : s_tot[1];
+            node_score = score_of(tg, th, cfg.l1, cfg.l2);
+        }
+        // Phase A: per-feature running sums in bin order (src/split.cpp:131-158).
+        // Quantized: exact integers, so a wave scans a feature in parallel.
+        if (Q) {
+            u32 const wv = threadIdx.x >> 6, ln = threadIdx.x & 63u;
+            for (u32 t = wv; t < kFindTile; t += blockDim.x >> 6) {
+                u32 ncut = 0, dirs = 1;
+                if (t < nt) {
+                    u32 const nb = n_bins[f0 + t];
+                    ncut = nb >= 2 ? nb - 2 : 0;
+                    i64 const *ci = reinterpret_cast<i64 const *>(s_cells + t * SB);
+                    i64 const mg = nb ? ci[2 * (nb - 1)] : 0, mh = nb ? ci[2 * (nb - 1) + 1] : 0;
+                    dirs = (mg == 0 && mh == 0) ? 1 : 2;
+                    if (ln == 0) {
+                        s_mg[t] = mg;
+                        s_mh[t] = mh;
+                    }
+                    u32 const per = (ncut + 63u) / 64u;
+                    u32 const b0 = ln * per;
+                    i64 lg[8], lh[8];
+                    i64 g = 0, h = 0;
+#pragma unroll
+                    for (u32 k = 0; k < 8; ++k) {
+                        u32 const b = b0 + k;
+                        if (k < per && b < ncut) {
+                            g += ci[2 * b];
+                            h += ci[2 * b + 1];
+                        }
+                        lg[k] = g;
+                        lh[k] = h;
+                    }
+                    // inclusive scan of the lane totals
+                    i64 sg2 = g, sh2 = h;
+                    for (int o = 1; o < 64; o <<= 1) {
+                        i64 const ug = __shfl_up(sg2, o), uh = __shfl_up(sh2, o);
+                        if (static_cast<int>(ln) >= o) {
+                            sg2 += ug;
+                            sh2 += uh;
+                        }
+                    }
+                    i64 const eg = sg2 - g, eh = sh2 - h;
+                    Acc *qg = pg + t * SB, *qh = ph + t * SB;
+#pragma unroll
+                    for (u32 k = 0; k < 8; ++k) {
+                        u32 const b = b0 + k;
+                        if (k < per && b < ncut) {
+                            qg[b] = eg + lg[k];
+                            qh[b] = eh + lh[k];
+                        }
+                    }
+                }
+                if (ln == 0) {
+                    s_ncut[t] = ncut;
+                    s_dirs[t] = dirs;
+                }
+            }
+        }
         u32 const t = threadIdx.x;
-        if (t < kFindTile) {
-            u32 const f = f0 + t;
+        if (!Q && t < kFindTile) {
             u32 ncut = 0, dirs = 1;
-            if (f < F) {
-                u32 const nb = n_bins[f];
+            if (t < nt) {
+                u32 const nb = n_bins[f0 + t];
                 ncut = nb >= 2 ? nb - 2 : 0;
+                Acc *qg = pg + t * SB, *qh = ph + t * SB;
                 if (Q) {
-                    i64 const *c = reinterpret_cast<i64 const *>(hist) +
-                                   (static_cast<size_t>(node.slot) * node_cells + static_cast<size_t>(f) * SB) * 2;
-                    i64 const mg = nb ? c[2 * (nb - 1)] : 0, mh = nb ? c[2 * (nb - 1) + 1] : 0;
-                    s_img[t] = mg;
-                    s_imh[t] = mh;
+                    i64 const *ci = reinterpret_cast<i64 const *>(s_cells + t * SB);
+                    i64 const mg = nb ? ci[2 * (nb - 1)] : 0, mh = nb ? ci[2 * (nb - 1) + 1] : 0;
+                    s_mg[t] = mg;
+                    s_mh[t] = mh;
                     dirs = (mg == 0 && mh == 0) ? 1 : 2;
                     i64 g = 0, h = 0;
                     for (u32 b = 0; b < ncut; ++b) {
-                        g += c[2 * b];
-                        h += c[2 * b + 1];
-                        ipg[t * SB + b] = g;
-                        iph[t * SB + b] = h;
+                        g += ci[2 * b];
+                        h += ci[2 * b + 1];
+                        qg[b] = g;
+                        qh[b] = h;
                     }
                 } else {
-                    float2 const *c = reinterpret_cast<float2 const *>(hist) +
-                                      static_cast<size_t>(node.slot) * node_cells + static_cast<size_t>(f) * SB;
-                    float2 const m = nb ? c[nb - 1] : make_float2(0.0F, 0.0F);
+                    float2 const *cf = reinterpret_cast<float2 const *>(s_cells + t * SB);
+                    float2 const m = nb ? cf[nb - 1] : make_float2(0.0F, 0.0F);
                     s_mg[t] = m.x;
                     s_mh[t] = m.y;
                     dirs = (m.x == 0.0F && m.y == 0.0F) ? 1 : 2;
                     double g = 0.0, h = 0.0;
-                    for (u32 b = 0; b < ncut; ++b) {
-                        float2 const v = c[b];
+                    u32 b = 0;
+                    for (; b + 4 <= ncut; b += 4) {
+                        float2 const v0 = cf[b], v1 = cf[b + 1], v2 = cf[b + 2], v3 = cf[b + 3];
+                        g += v0.x; h += v0.y; qg[b] = g; qh[b] = h;
+                        g += v1.x; h += v1.y; qg[b + 1] = g; qh[b + 1] = h;
+                        g += v2.x; h += v2.y; qg[b + 2] = g; qh[b + 2] = h;
+                        g += v3.x; h += v3.y; qg[b + 3] = g; qh[b + 3] = h;
+                    }
+                    for (; b < ncut; ++b) {
+                        float2 const v = cf[b];
                         g += v.x;
                         h += v.y;
-                        pg[t * SB + b] = g;
-                        ph[t * SB + b] = h;
+                        qg[b] = g;
+                        qh[b] = h;
                     }
                 }
             }
@@ -548,22 +685,28 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
             s_dirs[t] = dirs;
         }
         __syncthreads();
+        // Phase B: every (feature, cut bin, direction) candidate.
         u32 const n_cand = kFindTile * SB * 2;
         for (u32 k = threadIdx.x; k < n_cand; k += blockDim.x) {
             u32 const tt = k / (2 * SB);
             u32 const b = (k >> 1) % SB;
             u32 const d = k & 1u;
-            if (f0 + tt >= F || b >= s_ncut[tt] || d >= s_dirs[tt]) continue;
+            if (tt >= nt || b >= s_ncut[tt] || d >= s_dirs[tt]) continue;
             bool const dl = d == 0;
             double gL, hL, gR, hR;
+            i64 igL = 0, ihL = 0, igR = 0, ihR = 0;
             if (Q) {
-                i64 const lg = ipg[tt * SB + b], lh = iph[tt * SB + b];
-                i64 const mg = s_img[tt], mh = s_imh[tt];
+                i64 const lg = pg[tt * SB + b], lh = ph[tt * SB + b];
+                i64 const mg = s_mg[tt], mh = s_mh[tt];
                 i64 const rg = s_itot[0] - mg, rh = s_itot[1] - mh;
-                gL = static_cast<double>(lg + (dl ? mg : 0)) * sg;
-                hL = static_cast<double>(lh + (dl ? mh : 0)) * sh;
-                gR = static_cast<double>((rg - lg) + (!dl ? mg : 0)) * sg;
-                hR = static_cast<double>((rh - lh) + (!dl ? mh : 0)) * sh;
+                igL = lg + (dl ? mg : 0);
+                ihL = lh + (dl ? mh : 0);
+                igR = (rg - lg) + (!dl ? mg : 0);
+                ihR = (rh - lh) + (!dl ? mh : 0);
+                gL = static_cast<double>(igL) * sg;
+                hL = static_cast<double>(ihL) * sh;
+                gR = static_cast<double>(igR) * sg;
+                hR = static_cast<double>(ihR) * sh;
             } else {
                 double const lg = pg[tt * SB + b], lh = ph[tt * SB + b];
                 double const mg = s_mg[tt], mh = s_mh[tt];
@@ -582,6 +725,14 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
             if (gain > best_gain || (gain == best_gain && key < best_key)) {
                 best_gain = gain;
                 best_key = key;
+                bgL = gL;
+                bhL = hL;
+                bgR = gR;
+                bhR = hR;
+                bigL = igL;
+                bihL = ihL;
+                bigR = igR;
+                bihR = ihR;
             }
         }
         __syncthreads();
@@ -589,10 +740,10 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
     s_best_gain[threadIdx.x] = best_gain;
     s_best_key[threadIdx.x] = best_key;
     __syncthreads();
-    for (u32 s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            double const og = s_best_gain[threadIdx.x + s];
-            u32 const ok = s_best_key[threadIdx.x + s];
+    for (u32 st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            double const og = s_best_gain[threadIdx.x + st];
+            u32 const ok = s_best_key[threadIdx.x + st];
             double const mg = s_best_gain[threadIdx.x];
             u32 const mk = s_best_key[threadIdx.x];
             if (ok != kNone && (mk == kNone || og > mg || (og == mg && ok < mk))) {
@@ -602,48 +753,31 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        u32 const key = s_best_key[0];
-        if (key != kNone) {
-            u32 const f = key / (2u * SB), b = (key >> 1) % SB, d = key & 1u;
-            bool const dl = d == 0;
-            rec.gain = s_best_gain[0];
-            rec.feature = f;
-            rec.bin = b;
-            rec.default_left = dl ? 1u : 0u;
-            rec.valid = 1;
-            u32 const nb = n_bins[f];
-            if (Q) {
-                i64 const *c = reinterpret_cast<i64 const *>(hist) +
-                               (static_cast<size_t>(node.slot) * node_cells + static_cast<size_t>(f) * SB) * 2;
-                i64 lg = 0, lh = 0;
-                for (u32 k = 0; k <= b; ++k) {
-                    lg += c[2 * k];
-                    lh += c[2 * k + 1];
-                }
-                i64 const mg = c[2 * (nb - 1)], mh = c[2 * (nb - 1) + 1];
-                rec.igL = lg + (dl ? mg : 0);
-                rec.ihL = lh + (dl ? mh : 0);
-                rec.igR = ((s_itot[0] - mg) - lg) + (!dl ? mg : 0);
-                rec.ihR = ((s_itot[1] - mh) - lh) + (!dl ? mh : 0);
-            } else {
-                float2 const *c = reinterpret_cast<float2 const *>(hist) +
-                                  static_cast<size_t>(node.slot) * node_cells + static_cast<size_t>(f) * SB;
-                double lg = 0.0, lh = 0.0;
-                for (u32 k = 0; k <= b; ++k) {
-                    lg += c[k].x;
-                    lh += c[k].y;
-                }
-                double const mg = c[nb - 1].x, mh = c[nb - 1].y;
-                double const rg = tg - mg, rh = th - mh;
-                rec.gL = lg + (dl ? mg : 0.0);
-                rec.hL = lh + (dl ? mh : 0.0);
-                rec.gR = (rg - lg) + (!dl ? mg : 0.0);
-                rec.hR = (rh - lh) + (!dl ? mh : 0.0);
-            }
-        }
+    // The thread that scored the winner publishes its child sums (keys are unique).
+    u32 const key = s_best_key[0];
+    if (key != kNone && best_key == key) {
+        SplitRec rec{};
+        u32 const f = key / (2u * SB), b = (key >> 1) % SB, d = key & 1u;
+        rec.gain = s_best_gain[0];
+        rec.feature = f;
+        rec.bin = b;
+        rec.default_left = d == 0 ? 1u : 0u;
+        rec.valid = 1;
+        rec.gL = bgL;
+        rec.hL = bhL;
+        rec.gR = bgR;
+        rec.hR = bhR;
+        rec.igL = bigL;
+        rec.ihL = bihL;
+        rec.igR = bigR;
+        rec.ihR = bihR;
+        out[i] = rec;
+    } else if (key == kNone && threadIdx.x == 0) {
+        SplitRec rec{};
+        rec.default_left = 1;
         out[i] = rec;
     }
+    (void)s_rec;
 }
 
 // ---------------------------------------------------------------------------------
@@ -1005,6 +1139,175 @@ __global__ __launch_bounds__(256) void k_part_scatter(
     }
 }
 
+// Narrow rows (Fs <= 64 bytes): one lane per row, the row's dwords in registers.  The
+// route comes from the loaded row, ranks from a wave ballot plus the block's running
+// count, so the unit is read once and written once, coalesced across lanes.
+template <int DMAX>
+__global__ __launch_bounds__(256) void k_part_scatter_narrow(
+    PartChunk const *__restrict__ pch, PartOp const *__restrict__ ops, TreeCtl const *__restrict__ ctl,
+    u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
+    void const *__restrict__ gw0, void const *__restrict__ gw1, void const *__restrict__ gwp,
+    u8 *__restrict__ out0, u8 *__restrict__ out1, void *__restrict__ go0, void *__restrict__ go1, u32 Fs,
+    u32 gh_bytes)
+{
+    __shared__ u32 s_wsum[4];
+    u32 const n = ctl->n_part_chunks;
+    u32 const D = Fs / 4u;
+    u32 const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (u32 c = blockIdx.x; c < n; c += gridDim.x) {
+        PartChunk const ch = pch[c];
+        PartOp const op = ops[ch.op];
+        if (!op.alive) continue; // uniform across the block
+        u8 const *plane = op.src == 0 ? pl0 : (op.src == 1 ? pl1 : plp);
+        u8 const *gin = reinterpret_cast<u8 const *>(op.src == 0 ? gw0 : (op.src == 1 ? gw1 : gwp));
+        u8 *oplane = op.dst == 0 ? out0 : out1;
+        u8 *gout = reinterpret_cast<u8 *>(op.dst == 0 ? go0 : go1);
+        u32 const fw = op.feature >> 2, fsh = (op.feature & 3u) * 8u;
+        u32 run = 0;
+        for (u32 base = ch.begin; base < ch.end; base += 256u) {
+            u32 const r = base + threadIdx.x; // op-relative row
+            bool const ok = r < ch.end;
+            size_t const src = static_cast<size_t>(op.off) + r;
+            u32 w[DMAX];
+            u32 const *srow = reinterpret_cast<u32 const *>(plane + src * Fs);
+#pragma unroll
+            for (int d = 0; d < DMAX; ++d) w[d] = (ok && static_cast<u32>(d) < D) ? srow[d] : 0u;
+            uint2 gv = make_uint2(0u, 0u);
+            if (ok) {
+                if (gh_bytes == 8) gv = reinterpret_cast<uint2 const *>(gin)[src];
+                else gv.x = reinterpret_cast<u32 const *>(gin)[src];
+            }
+            u32 sel = 0;
+#pragma unroll
+            for (int d = 0; d < DMAX; ++d) sel = (static_cast<u32>(d) == fw) ? w[d] : sel;
+            u32 const b = (sel >> fsh) & 0xFFu;
+            bool const left = ok && routes_left(b, op.last_bin, op.bin, op.default_left);
+            u64 const bal = __ballot(left);
+            u32 const lrank = static_cast<u32>(__popcll(bal & ((1ull << lane) - 1ull)));
+            if (lane == 0) s_wsum[wave] = static_cast<u32>(__popcll(bal));
+            __syncthreads();
+            u32 pre = run;
+            for (u32 k = 0; k < wave; ++k) pre += s_wsum[k];
+            u32 const tot = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+            __syncthreads();
+            if (ok) {
+                u32 const lb = ch.left_before + pre + lrank; // left rows of the op before r
+                u32 const dest = left ? op.off + lb : op.off + op.n_left + (r - lb);
+                u32 *drow = reinterpret_cast<u32 *>(oplane + static_cast<size_t>(dest) * Fs);
+#pragma unroll
+                for (int d = 0; d < DMAX; ++d)
+                    if (static_cast<u32>(d) < D) drow[d] = w[d];
+                if (gh_bytes == 8) reinterpret_cast<uint2 *>(gout)[dest] = gv;
+                else reinterpret_cast<u32 *>(gout)[dest] = gv.x;
+            }
+            run += tot;
+        }
+    }
+}
+
+// Narrow rows, LDS-staged: the unit's rows are read as one contiguous dword stream,
+// ranked in LDS, and each child's run is written back as one contiguous dword stream,
+// so every global access of the scatter is fully coalesced.
+constexpr u32 kScatTile = 1024;
+
+__global__ __launch_bounds__(256) void k_part_scatter_lds(
+    PartChunk const *__restrict__ pch, PartOp const *__restrict__ ops, TreeCtl const *__restrict__ ctl,
+    u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
+    void const *__restrict__ gw0, void const *__restrict__ gw1, void const *__restrict__ gwp,
+    u8 *__restrict__ out0, u8 *__restrict__ out1, void *__restrict__ go0, void *__restrict__ go1, u32 Fs,
+    u32 gh_bytes, u32 tile)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    u32 *s_rows = reinterpret_cast<u32 *>(smem);             // tile * Fs bytes
+    u32 *s_gh = s_rows + tile * (Fs / 4u);                    // tile * gh_bytes
+    u16 *s_lidx = reinterpret_cast<u16 *>(s_gh + tile * (gh_bytes / 4u));
+    u16 *s_ridx = s_lidx + tile;
+    __shared__ u32 s_wsum[4];
+    u32 const n = ctl->n_part_chunks;
+    u32 const D = Fs / 4u, G = gh_bytes / 4u;
+    u32 const step_q = blockDim.x / D, step_r = blockDim.x - step_q * D;
+    u32 const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (u32 c = blockIdx.x; c < n; c += gridDim.x) {
+        PartChunk const ch = pch[c];
+        PartOp const op = ops[ch.op];
+        if (!op.alive) continue; // uniform across the block
+        u8 const *plane = op.src == 0 ? pl0 : (op.src == 1 ? pl1 : plp);
+        u32 const *gin = reinterpret_cast<u32 const *>(op.src == 0 ? gw0 : (op.src == 1 ? gw1 : gwp));
+        u8 *oplane = op.dst == 0 ? out0 : out1;
+        u32 *gout = reinterpret_cast<u32 *>(op.dst == 0 ? go0 : go1);
+        u32 left_run = ch.left_before; // left rows of the op before the current tile
+        for (u32 t0 = ch.begin; t0 < ch.end; t0 += tile) {
+            u32 const rows = min(tile, ch.end - t0);
+            size_t const srow0 = static_cast<size_t>(op.off) + t0;
+            u32 const *gsrc = reinterpret_cast<u32 const *>(plane + srow0 * Fs);
+            for (u32 k = threadIdx.x; k < rows * D; k += blockDim.x) s_rows[k] = gsrc[k];
+            for (u32 k = threadIdx.x; k < rows * G; k += blockDim.x) s_gh[k] = gin[srow0 * G + k];
+            __syncthreads();
+            u8 const *srb = reinterpret_cast<u8 const *>(s_rows);
+            u32 run = 0;
+            for (u32 p0 = 0; p0 < rows; p0 += blockDim.x) {
+                u32 const r = p0 + threadIdx.x;
+                bool const ok = r < rows;
+                bool const left = ok && routes_left(srb[r * Fs + op.feature], op.last_bin, op.bin, op.default_left);
+                u64 const bal = __ballot(left);
+                u32 const lrank = static_cast<u32>(__popcll(bal & ((1ull << lane) - 1ull)));
+                if (lane == 0) s_wsum[wave] = static_cast<u32>(__popcll(bal));
+                __syncthreads();
+                u32 pre = run;
+                for (u32 k = 0; k < wave; ++k) pre += s_wsum[k];
+                u32 tot = 0;
+                for (u32 k = 0; k < blockDim.x / 64u; ++k) tot += s_wsum[k];
+                __syncthreads();
+                if (ok) {
+                    u32 const lr = pre + lrank;
+                    if (left) s_lidx[lr] = static_cast<u16>(r);
+                    else s_ridx[r - lr] = static_cast<u16>(r);
+                }
+                run += tot;
+            }
+            __syncthreads();
+            u32 const nl = run, nr = rows - run;
+            // Left run: destination rows [off + left_run, + nl); right run: after n_left.
+            u32 const l0 = op.off + left_run;
+            u32 const r0 = op.off + op.n_left + (t0 - left_run);
+            u32 *dl = reinterpret_cast<u32 *>(oplane + static_cast<size_t>(l0) * Fs);
+            u32 *dr = reinterpret_cast<u32 *>(oplane + static_cast<size_t>(r0) * Fs);
+            // dword k of a run belongs to run row k / D; (q, rem) advance incrementally.
+            {
+                u32 q = threadIdx.x / D, rem = threadIdx.x - q * D;
+                for (u32 k = threadIdx.x; k < nl * D; k += blockDim.x) {
+                    dl[k] = s_rows[static_cast<u32>(s_lidx[q]) * D + rem];
+                    q += step_q;
+                    rem += step_r;
+                    if (rem >= D) {
+                        rem -= D;
+                        ++q;
+                    }
+                }
+            }
+            {
+                u32 q = threadIdx.x / D, rem = threadIdx.x - q * D;
+                for (u32 k = threadIdx.x; k < nr * D; k += blockDim.x) {
+                    dr[k] = s_rows[static_cast<u32>(s_ridx[q]) * D + rem];
+                    q += step_q;
+                    rem += step_r;
+                    if (rem >= D) {
+                        rem -= D;
+                        ++q;
+                    }
+                }
+            }
+            u32 const gs = G == 2 ? 1u : 0u;
+            for (u32 k = threadIdx.x; k < nl * G; k += blockDim.x)
+                gout[static_cast<size_t>(l0) * G + k] = s_gh[(static_cast<u32>(s_lidx[k >> gs]) << gs) + (k & gs)];
+            for (u32 k = threadIdx.x; k < nr * G; k += blockDim.x)
+                gout[static_cast<size_t>(r0) * G + k] = s_gh[(static_cast<u32>(s_ridx[k >> gs]) << gs) + (k & gs)];
+            left_run += nl;
+            __syncthreads();
+        }
+    }
+}
+
 // Frontier left at the end of the tree becomes leaves (LevelStep::end_tree,
 // src/step/level.hpp:119-128): values from the totals the parent split carried.
 __global__ void k_finalize(NodeInfo const *__restrict__ front, TreeCtl const *__restrict__ ctl,
@@ -1094,6 +1397,7 @@ __device__ __forceinline__ WalkNode to_walk(DevNode const &d)
     return w;
 }
 
+template <int DMAX>
 __global__ __launch_bounds__(256) void k_score_update(u8 const *__restrict__ plane, u32 Fs, u64 n,
                                                       DevNode const *__restrict__ tree,
                                                       TreeCtl const *__restrict__ ctl,
@@ -1110,11 +1414,27 @@ __global__ __launch_bounds__(256) void k_score_update(u8 const *__restrict__ pla
     for (u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; r < n;
          r += static_cast<u64>(gridDim.x) * blockDim.x) {
         u8 const *row = plane + r * Fs;
+        // Narrow rows: the whole row in registers, bins extracted by select (DMAX > 0).
+        u32 rw[DMAX > 0 ? DMAX : 1];
+        if (DMAX > 0) {
+            u32 const D = Fs / 4u;
+#pragma unroll
+            for (int d = 0; d < (DMAX > 0 ? DMAX : 1); ++d)
+                rw[d] = static_cast<u32>(d) < D ? reinterpret_cast<u32 const *>(row)[d] : 0u;
+        }
         u32 idx = 0;
         WalkNode w = nodes[0];
         while (w.feature != 0xFFFFu) {
             u32 const f = w.feature;
-            u32 const b = row[f];
+            u32 b;
+            if (DMAX > 0) {
+                u32 sel = 0;
+#pragma unroll
+                for (int d = 0; d < (DMAX > 0 ? DMAX : 1); ++d) sel = (static_cast<u32>(d) == (f >> 2)) ? rw[d] : sel;
+                b = (sel >> ((f & 3u) * 8u)) & 0xFFu;
+            } else {
+                b = row[f];
+            }
             u32 const lb = f < kWalkLds ? last[f] : static_cast<u32>(n_bins[f]) - 1u;
             idx = routes_left(b, lb, w.bin, w.default_left) ? w.left : w.right;
             w = idx < kWalkLds ? nodes[idx] : to_walk(tree[idx]);
@@ -1174,7 +1494,7 @@ void launch_grad_quant(hipStream_t s, float const *score, float const *label, u6
                        u64 seed, TreeCtl const *ctl, u32 *ghq, double *scales)
 {
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, maxbits, 2u);
-    hipLaunchKernelGGL(k_grad_max, dim3(2048), dim3(256), 0, s, score, label, n, obj, maxbits);
+    hipLaunchKernelGGL(k_grad_max, dim3(1024), dim3(256), 0, s, score, label, n, obj, maxbits);
     hipLaunchKernelGGL(k_grad_quant, dim3(2048), dim3(256), 0, s, score, label, n, obj, maxbits, seed, ctl, ghq,
                        scales);
 }
@@ -1182,7 +1502,7 @@ void launch_quant_from(hipStream_t s, float2 const *gh, u64 n, u32 *maxbits, u64
                        double *scales_dev)
 {
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, maxbits, 2u);
-    hipLaunchKernelGGL(k_gh_max, dim3(2048), dim3(256), 0, s, gh, n, maxbits);
+    hipLaunchKernelGGL(k_gh_max, dim3(1024), dim3(256), 0, s, gh, n, maxbits);
     hipLaunchKernelGGL(k_quant_from, dim3(2048), dim3(256), 0, s, gh, n, maxbits, seed, iter, ghq);
     (void)scales_dev;
 }
@@ -1195,7 +1515,7 @@ hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl c
     size_t const lds = static_cast<size_t>(fsl) * SB * 8;
     dim3 const grid(grid_x, slices), block(64 * n_waves);
     if (q)
-        hipLaunchKernelGGL((k_hist<true, 4>), grid, block, lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp, Fs, F,
+        hipLaunchKernelGGL((k_hist<true, 8>), grid, block, lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp, Fs, F,
                            SB, fsl, partials, hist_next);
     else
         hipLaunchKernelGGL((k_hist<false, 4>), grid, block, lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp, Fs, F,
@@ -1223,7 +1543,7 @@ hipError_t set_hist_lds_limits()
     int max_shared = 0;
     e = hipDeviceGetAttribute(&max_shared, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (e != hipSuccess) return e;
-    raise_lds(reinterpret_cast<void const *>(&k_hist<true, 4>), max_shared);
+    raise_lds(reinterpret_cast<void const *>(&k_hist<true, 8>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_hist<false, 4>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_find<true>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_find<false>), max_shared);
@@ -1247,7 +1567,7 @@ void launch_find(hipStream_t s, bool q, NodeInfo *front, TreeCtl const *ctl, voi
                  u32 F, u32 SB, GrowCfg const &cfg, double const *scales, SplitRec *out, u32 max_front,
                  u32 front_base, u32 n_front_override)
 {
-    size_t const lds = static_cast<size_t>(kFindTile) * SB * 16;
+    size_t const lds = static_cast<size_t>(kFindTile) * SB * (q ? 32 : 24);
     if (q)
         hipLaunchKernelGGL(k_find<true>, dim3(max_front), dim3(256), lds, s, front, ctl, hist, n_bins, F, SB, cfg,
                            scales, out, front_base, n_front_override);
@@ -1287,6 +1607,15 @@ void launch_part_scatter(hipStream_t s, PartChunk const *pch, PartOp const *ops,
                          void const *gp, u8 *o0, u8 *o1, void *go0, void *go1, u32 Fs, u32 gh_bytes, u32 grid)
 {
     u32 const D = Fs / 4;
+    if (D <= 16) {
+        // Rows per LDS tile: at most kScatTile, and the tile within 48 KiB of LDS.
+        u32 tile = (48u * 1024u) / (Fs + gh_bytes + 4u);
+        tile = std::min<u32>(kScatTile, tile & ~63u);
+        size_t const lds = static_cast<size_t>(tile) * (Fs + gh_bytes + 4);
+        hipLaunchKernelGGL(k_part_scatter_lds, dim3(grid), dim3(256), lds, s, pch, ops, ctl, pl0, pl1, plp, g0, g1,
+                           gp, o0, o1, go0, go1, Fs, gh_bytes, tile);
+        return;
+    }
     u32 tl = 0;
     while ((1u << tl) < D && tl < 8) ++tl;
     hipLaunchKernelGGL(k_part_scatter, dim3(grid), dim3(256), 0, s, pch, ops, ctl, pl0, pl1, plp, g0, g1, gp, o0,
@@ -1314,8 +1643,12 @@ void launch_tree_begin(hipStream_t s, TreeCtl *ctl, NodeInfo *front, DevNode *tr
 void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode const *tree, TreeCtl const *ctl,
                          u16 const *n_bins, u32 F, float lr, float *score, u32 *leaf_of_row)
 {
-    hipLaunchKernelGGL(k_score_update, dim3(2048), dim3(256), 0, s, plane, Fs, n, tree, ctl, n_bins, F, lr, score,
-                       leaf_of_row);
+    if (Fs <= 64)
+        hipLaunchKernelGGL(k_score_update<16>, dim3(2048), dim3(256), 0, s, plane, Fs, n, tree, ctl, n_bins, F, lr,
+                           score, leaf_of_row);
+    else
+        hipLaunchKernelGGL(k_score_update<0>, dim3(2048), dim3(256), 0, s, plane, Fs, n, tree, ctl, n_bins, F, lr,
+                           score, leaf_of_row);
 }
 
 void launch_iter_advance(hipStream_t s, TreeCtl *ctl)
