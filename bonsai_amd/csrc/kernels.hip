@@ -498,6 +498,16 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
     if (blockIdx.x >= n_front) return;
     u32 const i = front_base + blockIdx.x;
     NodeInfo node = front[i];
+    if (node.slot == kNone) {
+        // No histogram (a leafwise child at max_depth): totals came from the parent
+        // split and the node cannot split.
+        if (threadIdx.x == 0) {
+            SplitRec rec{};
+            rec.default_left = 1;
+            out[i] = rec;
+        }
+        return;
+    }
     size_t const node_cells = static_cast<size_t>(F) * SB;
     Cell const *hcells = reinterpret_cast<Cell const *>(hist) + static_cast<size_t>(node.slot) * node_cells;
     double const sg = Q ? scales[0] : 0.0, sh = Q ? scales[1] : 0.0;

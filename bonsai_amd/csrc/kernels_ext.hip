@@ -1,19 +1,1026 @@
 // Leafwise and levelwise grow loops on the device.
+//
+// Both reuse the depthwise data plane (partition count/scan/scatter, histogram units,
+// fixed-order completion, node finder) and add their own control kernels:
+//   levelwise (ObliviousGrower::grow, src/grower_impl.hpp:598-734): one split per level
+//     chosen over the whole frontier (k_find_level + k_level_pick), broadcast to every
+//     node, empty children kept;
+//   leafwise (LeafwiseGrower::grow, src/grower_impl.hpp:743-862): a device-resident
+//     candidate pool replaces the host max-heap; every round pops the best leaf (gain,
+//     then lowest node id), partitions it, builds the smaller child, subtracts in place
+//     and scores both children.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "bh_common.hpp"
 #include "kernels.hpp"
 #include "kernels_ext.hpp"
 
 namespace bh {
 
+__device__ u32 block_exscan_ext(u32 v, u32 *scratch, u32 *total);
+
+namespace {
+
+__device__ __forceinline__ double l1_thr(double g, double l1)
+{
+    if (g > l1) return g - l1;
+    if (g < -l1) return g + l1;
+    return 0.0;
+}
+__device__ __forceinline__ double score_of(double g, double h, double l1, double l2)
+{
+    double const t = l1_thr(g, l1);
+    double const d = h + l2;
+    return d > 0.0 ? (t * t) / d : 0.0;
+}
+__device__ __forceinline__ float leaf_weight(double g, double h, double l1, double l2)
+{
+    double const d = h + l2;
+    double const w = d > 0.0 ? -l1_thr(g, l1) / d : 0.0;
+    return static_cast<float>(w);
+}
+__device__ __forceinline__ float node_value(NodeInfo const &n, GrowCfg const &c, double const *scales)
+{
+    if (c.quantized)
+        return leaf_weight(static_cast<double>(n.itg) * scales[0], static_cast<double>(n.ith) * scales[1], c.l1,
+                           c.l2);
+    return leaf_weight(n.tg, n.th, c.l1, c.l2);
+}
+__device__ __forceinline__ void put_leaf(DevNode *tree, u32 id, float v)
+{
+    DevNode n = tree[id];
+    n.feature = -1;
+    n.bin = n.default_left = n.left = n.right = 0;
+    n.value = v;
+    tree[id] = n;
+}
+
+// Node totals of a frontier (Histogram::totals, include/bonsai/histogram.hpp:76-87)
+// when no parent split supplied them.  One workgroup per node.
+template <bool Q>
+__global__ __launch_bounds__(64) void k_totals(NodeInfo *__restrict__ front, TreeCtl const *__restrict__ ctl,
+                                               void const *__restrict__ hist, u16 const *__restrict__ n_bins, u32 F,
+                                               u32 SB)
+{
+    u32 const i = blockIdx.x;
+    if (i >= ctl->n_front || ctl->done) return;
+    NodeInfo nd = front[i];
+    if (nd.totals_ready) return;
+    size_t const nc = static_cast<size_t>(F) * SB;
+    u32 const nb0 = n_bins[0];
+    if (Q) {
+        i64 const *c = reinterpret_cast<i64 const *>(hist) + static_cast<size_t>(nd.slot) * nc * 2;
+        i64 g = 0, h = 0;
+        for (u32 b = threadIdx.x; b < nb0; b += 64) {
+            g += c[2 * b];
+            h += c[2 * b + 1];
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            g += __shfl_xor(g, o);
+            h += __shfl_xor(h, o);
+        }
+        if (threadIdx.x == 0) {
+            nd.itg = g;
+            nd.ith = h;
+            nd.totals_ready = 1;
+            front[i] = nd;
+        }
+    } else if (threadIdx.x == 0) {
+        float2 const *c = reinterpret_cast<float2 const *>(hist) + static_cast<size_t>(nd.slot) * nc;
+        double g = 0.0, h = 0.0;
+        u32 b = 0;
+        for (; b + 8 <= nb0; b += 8) {
+            float2 v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = c[b + k];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                g += v[k].x;
+                h += v[k].y;
+            }
+        }
+        for (; b < nb0; ++b) {
+            g += c[b].x;
+            h += c[b].y;
+        }
+        nd.tg = static_cast<float>(g);
+        nd.th = static_cast<float>(h);
+        nd.totals_ready = 1;
+        front[i] = nd;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Level (oblivious) finder: HistogramLevelSplitFinder::find (src/split.cpp:164-235,
+// :350-372).  One workgroup per feature.  Frontier nodes are taken in tiles: the tile's
+// prefix sums (fill_prefix: double running sums stored as float, or exact integers) go
+// to LDS, then each (bin, direction) candidate adds the tile's child scores to its
+// running sum in node order; an infeasible node contributes its parent score.
+// ---------------------------------------------------------------------------------
+constexpr u32 kLevelTileF32 = 16, kLevelTileQ = 8;
+
+template <bool Q>
+__global__ __launch_bounds__(256) void k_find_level(NodeInfo const *__restrict__ front,
+                                                    TreeCtl const *__restrict__ ctl, void const *__restrict__ hist,
+                                                    u16 const *__restrict__ n_bins, u32 F, u32 SB, GrowCfg cfg,
+                                                    double const *__restrict__ scales, SplitRec *__restrict__ fbest)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    u32 const f = blockIdx.x;
+    if (f >= F) return;
+    if (ctl->done) return;
+    u32 const TN = Q ? kLevelTileQ : kLevelTileF32;
+    u32 const P = ctl->n_front;
+    u32 const nb = n_bins[f];
+    u32 const ncut = nb >= 2 ? nb - 2 : 0;
+    SplitRec best{};
+    best.default_left = 1;
+    if (ncut == 0 || P == 0) {
+        if (threadIdx.x == 0) fbest[f] = best;
+        return;
+    }
+    // LDS: per tile node prefix (g, h) over the cut bins + node scalars.
+    float2 *pf = reinterpret_cast<float2 *>(smem);  // F32: [TN][ncut]
+    i64 *pq = reinterpret_cast<i64 *>(smem);        // Q:   [TN][ncut][2]
+    __shared__ double s_ps[kLevelTileF32], s_rg[kLevelTileF32], s_rh[kLevelTileF32], s_mg[kLevelTileF32],
+        s_mh[kLevelTileF32];
+    __shared__ i64 s_itg[kLevelTileF32], s_ith[kLevelTileF32], s_img[kLevelTileF32], s_imh[kLevelTileF32];
+    __shared__ double s_sum_parent;
+    __shared__ u32 s_nonempty;
+    size_t const nc = static_cast<size_t>(F) * SB;
+    double const sg = Q ? scales[0] : 0.0, sh = Q ? scales[1] : 0.0;
+    // Two candidates per thread at most (ncut <= 254): k = threadIdx.x and + 256.
+    double acc[2] = {0.0, 0.0};
+    if (threadIdx.x == 0) {
+        s_sum_parent = 0.0;
+        s_nonempty = 0;
+    }
+    __syncthreads();
+    for (u32 p0 = 0; p0 < P; p0 += TN) {
+        u32 const tn = min(TN, P - p0);
+        u32 const t = threadIdx.x;
+        if (t < tn) {
+            NodeInfo const nd = front[p0 + t];
+            if (Q) {
+                i64 const *c = reinterpret_cast<i64 const *>(hist) +
+                               (static_cast<size_t>(nd.slot) * nc + static_cast<size_t>(f) * SB) * 2;
+                i64 g = 0, h = 0;
+                for (u32 b = 0; b < ncut; ++b) {
+                    g += c[2 * b];
+                    h += c[2 * b + 1];
+                    pq[(t * ncut + b) * 2] = g;
+                    pq[(t * ncut + b) * 2 + 1] = h;
+                }
+                i64 const mg = c[2 * (nb - 1)], mh = c[2 * (nb - 1) + 1];
+                s_img[t] = mg;
+                s_imh[t] = mh;
+                s_itg[t] = nd.itg;
+                s_ith[t] = nd.ith;
+                s_ps[t] = score_of(static_cast<double>(nd.itg) * sg, static_cast<double>(nd.ith) * sh, cfg.l1,
+                                   cfg.l2);
+                if (mg != 0 || mh != 0) atomicOr(&s_nonempty, 1u);
+            } else {
+                float2 const *c = reinterpret_cast<float2 const *>(hist) + static_cast<size_t>(nd.slot) * nc +
+                                  static_cast<size_t>(f) * SB;
+                double g = 0.0, h = 0.0;
+                for (u32 b = 0; b < ncut; ++b) {
+                    g = g + c[b].x;
+                    h = h + c[b].y;
+                    pf[t * ncut + b] = make_float2(static_cast<float>(g), static_cast<float>(h));
+                }
+                float2 const m = c[nb - 1];
+                s_mg[t] = m.x;
+                s_mh[t] = m.y;
+                s_ps[t] = score_of(nd.tg, nd.th, cfg.l1, cfg.l2);
+                s_rg[t] = static_cast<double>(nd.tg) - m.x;
+                s_rh[t] = static_cast<double>(nd.th) - m.y;
+                if (m.x != 0.0F || m.y != 0.0F) atomicOr(&s_nonempty, 1u);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (u32 k = 0; k < tn; ++k) s_sum_parent += s_ps[k];
+        for (u32 s = 0; s < 2; ++s) {
+            u32 const k = threadIdx.x + s * blockDim.x;
+            if (k >= 2 * ncut) continue;
+            u32 const b = k >> 1;
+            bool const dl = (k & 1u) == 0;
+            double sum = acc[s];
+            for (u32 p = 0; p < tn; ++p) {
+                double cs;
+                bool ok;
+                if (Q) {
+                    i64 const lg = pq[(p * ncut + b) * 2], lh = pq[(p * ncut + b) * 2 + 1];
+                    i64 const mg = s_img[p], mh = s_imh[p];
+                    i64 const hL = lh + (dl ? mh : 0);
+                    i64 const hR = ((s_ith[p] - mh) - lh) + (!dl ? mh : 0);
+                    i64 const gL = lg + (dl ? mg : 0);
+                    i64 const gR = ((s_itg[p] - mg) - lg) + (!dl ? mg : 0);
+                    double const dhL = static_cast<double>(hL) * sh, dhR = static_cast<double>(hR) * sh;
+                    ok = !(dhL < cfg.min_child_hess || dhR < cfg.min_child_hess);
+                    cs = ok ? score_of(static_cast<double>(gL) * sg, dhL, cfg.l1, cfg.l2) +
+                                  score_of(static_cast<double>(gR) * sg, dhR, cfg.l1, cfg.l2)
+                            : 0.0;
+                } else {
+                    float2 const lp = pf[p * ncut + b];
+                    double const lg = lp.x, lh = lp.y;
+                    double const mg = s_mg[p], mh = s_mh[p];
+                    double const right_g = s_rg[p] - lg, right_h = s_rh[p] - lh;
+                    double const gL = lg + (dl ? mg : 0.0), hL = lh + (dl ? mh : 0.0);
+                    double const gR = right_g + (!dl ? mg : 0.0), hR = right_h + (!dl ? mh : 0.0);
+                    ok = !(hL < cfg.min_child_hess || hR < cfg.min_child_hess);
+                    cs = ok ? score_of(gL, hL, cfg.l1, cfg.l2) + score_of(gR, hR, cfg.l1, cfg.l2) : 0.0;
+                }
+                sum += ok ? cs : s_ps[p];
+            }
+            acc[s] = sum;
+        }
+        __syncthreads();
+    }
+    // Per-feature winner: max gain, earliest (bin, dl=true first); an all-empty missing
+    // cell set scans one direction (identical sums, so the first direction wins anyway).
+    __shared__ double s_g[256];
+    __shared__ u32 s_k[256];
+    double bg = 0.0;
+    u32 bk = kNone;
+    double const sp = s_sum_parent;
+    bool const one_dir = s_nonempty == 0;
+    for (u32 s = 0; s < 2; ++s) {
+        u32 const k = threadIdx.x + s * blockDim.x;
+        if (k >= 2 * ncut) continue;
+        if (one_dir && (k & 1u)) continue;
+        double const gain = acc[s] - sp;
+        if (!(gain > 0.0 && gain >= cfg.min_gain)) continue;
+        if (gain > bg || (gain == bg && k < bk)) {
+            bg = gain;
+            bk = k;
+        }
+    }
+    s_g[threadIdx.x] = bg;
+    s_k[threadIdx.x] = bk;
+    __syncthreads();
+    for (u32 st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            double const og = s_g[threadIdx.x + st];
+            u32 const ok = s_k[threadIdx.x + st];
+            if (ok != kNone &&
+                (s_k[threadIdx.x] == kNone || og > s_g[threadIdx.x] || (og == s_g[threadIdx.x] && ok < s_k[threadIdx.x]))) {
+                s_g[threadIdx.x] = og;
+                s_k[threadIdx.x] = ok;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (s_k[0] != kNone) {
+            best.gain = s_g[0];
+            best.feature = f;
+            best.bin = s_k[0] >> 1;
+            best.default_left = (s_k[0] & 1u) == 0 ? 1u : 0u;
+            best.valid = 1;
+        }
+        fbest[f] = best;
+    }
+}
+
+// reduce_in_feature_order (src/split.cpp:240-253) + the level's bookkeeping
+// (src/grower_impl.hpp:633-645): record the split or stop growth.
+__global__ __launch_bounds__(256) void k_level_pick(SplitRec const *__restrict__ fbest, u32 F,
+                                                    TreeCtl *__restrict__ ctl, u32 *__restrict__ level_splits,
+                                                    float *__restrict__ level_gains, SplitRec *__restrict__ pick)
+{
+    if (ctl->done) return;
+    __shared__ double s_g[256];
+    __shared__ u32 s_f[256];
+    double bg = 0.0;
+    u32 bf = kNone;
+    for (u32 f = threadIdx.x; f < F; f += blockDim.x) {
+        SplitRec const c = fbest[f];
+        if (c.valid && (c.gain > bg || bf == kNone || (c.gain == bg && f < bf))) {
+            if (bf == kNone || c.gain > bg || (c.gain == bg && f < bf)) {
+                bg = c.gain;
+                bf = f;
+            }
+        }
+    }
+    s_g[threadIdx.x] = bg;
+    s_f[threadIdx.x] = bf;
+    __syncthreads();
+    for (u32 st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            double const og = s_g[threadIdx.x + st];
+            u32 const of = s_f[threadIdx.x + st];
+            if (of != kNone &&
+                (s_f[threadIdx.x] == kNone || og > s_g[threadIdx.x] || (og == s_g[threadIdx.x] && of < s_f[threadIdx.x]))) {
+                s_g[threadIdx.x] = og;
+                s_f[threadIdx.x] = of;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        u32 const d = ctl->oblivious_depth;
+        if (s_f[0] == kNone) {
+            ctl->done = 1;
+            SplitRec z{};
+            pick[0] = z;
+            return;
+        }
+        SplitRec const s = fbest[s_f[0]];
+        pick[0] = s;
+        level_splits[3 * d] = s.feature;
+        level_splits[3 * d + 1] = s.bin;
+        level_splits[3 * d + 2] = s.default_left;
+        level_gains[d] = static_cast<float>(s.gain);
+        ctl->oblivious_depth = d + 1;
+    }
+}
+
+// Broadcast the level split to every frontier node: one partition op per node (empty
+// nodes included), the node's child sums under the split (for leaves made at the last
+// level), and the partition units.
+template <bool Q>
+__global__ __launch_bounds__(1024) void k_plan_oblivious(NodeInfo const *__restrict__ front,
+                                                         TreeCtl *__restrict__ ctl, SplitRec const *__restrict__ pick,
+                                                         void const *__restrict__ hist, u16 const *__restrict__ n_bins,
+                                                         u32 F, u32 SB, SplitRec *__restrict__ splits,
+                                                         PartOp *__restrict__ ops, PartChunk *__restrict__ pch, u32 dst,
+                                                         u32 is_last)
+{
+    __shared__ u32 scratch[1024];
+    __shared__ u32 s_chunks;
+    if (ctl->done) {
+        if (threadIdx.x == 0) {
+            ctl->n_ops = 0;
+            ctl->n_part_chunks = 0;
+        }
+        return;
+    }
+    SplitRec const s = pick[0];
+    u32 const P = ctl->n_front;
+    u32 const nb = n_bins[s.feature];
+    bool const dl = s.default_left != 0;
+    size_t const nc = static_cast<size_t>(F) * SB;
+    if (threadIdx.x == 0) s_chunks = 0;
+    __syncthreads();
+    for (u32 t0 = 0; t0 < P; t0 += blockDim.x) {
+        u32 const i = t0 + threadIdx.x;
+        bool const in = i < P;
+        u32 nch = 0;
+        NodeInfo nd{};
+        if (in) {
+            nd = front[i];
+            nch = (nd.cnt + kPartChunk - 1) / kPartChunk;
+        }
+        u32 ctot;
+        u32 const cfirst = block_exscan_ext(nch, scratch, &ctot) + s_chunks;
+        if (in) {
+            SplitRec r = s;
+            if (!is_last) {
+                // child sums are only read for leaves made at the last level
+            } else if (Q) {
+                i64 const *c = reinterpret_cast<i64 const *>(hist) +
+                               (static_cast<size_t>(nd.slot) * nc + static_cast<size_t>(s.feature) * SB) * 2;
+                i64 lg = 0, lh = 0;
+                for (u32 b = 0; b <= s.bin; ++b) {
+                    lg += c[2 * b];
+                    lh += c[2 * b + 1];
+                }
+                i64 const mg = c[2 * (nb - 1)], mh = c[2 * (nb - 1) + 1];
+                r.igL = lg + (dl ? mg : 0);
+                r.ihL = lh + (dl ? mh : 0);
+                r.igR = ((nd.itg - mg) - lg) + (!dl ? mg : 0);
+                r.ihR = ((nd.ith - mh) - lh) + (!dl ? mh : 0);
+            } else {
+                float2 const *c = reinterpret_cast<float2 const *>(hist) + static_cast<size_t>(nd.slot) * nc +
+                                  static_cast<size_t>(s.feature) * SB;
+                double g = 0.0, h = 0.0;
+                for (u32 b = 0; b <= s.bin; ++b) {
+                    g = g + c[b].x;
+                    h = h + c[b].y;
+                }
+                double const lg = static_cast<float>(g), lh = static_cast<float>(h);
+                double const mg = c[nb - 1].x, mh = c[nb - 1].y;
+                double const rg = static_cast<double>(nd.tg) - mg, rh = static_cast<double>(nd.th) - mh;
+                r.gL = lg + (dl ? mg : 0.0);
+                r.hL = lh + (dl ? mh : 0.0);
+                r.gR = (rg - lg) + (!dl ? mg : 0.0);
+                r.hR = (rh - lh) + (!dl ? mh : 0.0);
+            }
+            splits[i] = r;
+            PartOp op{};
+            op.parent = i;
+            op.feature = s.feature;
+            op.bin = s.bin;
+            op.default_left = s.default_left;
+            op.last_bin = nb - 1u;
+            op.off = nd.off;
+            op.cnt = nd.cnt;
+            op.src = nd.src;
+            op.dst = dst;
+            op.chunk_first = cfirst;
+            op.chunk_count = nch;
+            op.alive = 1;
+            ops[i] = op;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_chunks += ctot;
+        __syncthreads();
+    }
+    u32 const n_ch = s_chunks;
+    for (u32 c = threadIdx.x; c < n_ch; c += blockDim.x) {
+        u32 lo = 0, hi = P;
+        while (hi - lo > 1) {
+            u32 const mid = (lo + hi) >> 1;
+            if (ops[mid].chunk_first <= c) lo = mid;
+            else hi = mid;
+        }
+        PartOp const op = ops[lo];
+        u32 const k = c - op.chunk_first;
+        PartChunk pc;
+        pc.op = lo;
+        pc.begin = k * kPartChunk;
+        pc.end = min(op.cnt, (k + 1) * kPartChunk);
+        pc.left_before = 0;
+        pch[c] = pc;
+    }
+    if (threadIdx.x == 0) {
+        ctl->n_ops = P;
+        ctl->n_part_chunks = n_ch;
+    }
+}
+
+// Level swap for the oblivious loop: a stopped tree keeps its frontier (copied into
+// the buffer the next level reads).
+__global__ __launch_bounds__(256) void k_next_level_obl(TreeCtl *__restrict__ ctl, NodeInfo const *__restrict__ cur,
+                                                        NodeInfo *__restrict__ nxt)
+{
+    bool const done = ctl->done != 0;
+    u32 const P = ctl->n_front;
+    if (done)
+        for (u32 i = threadIdx.x; i < P; i += blockDim.x) nxt[i] = cur[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (!done) ctl->n_front = ctl->next_front;
+        ctl->depth += 1;
+    }
+}
+
+// The finished oblivious tree as a perfect dense tree (ObliviousTree ->
+// dense_equivalent, src/tree.cpp:21-50): internal node i at level lvl(i) carries that
+// level's split, children 2i+1 / 2i+2, leaves 2^D-1 .. in leaf-table order, covers
+// aggregated bottom-up.
+template <bool Q>
+__global__ __launch_bounds__(1024) void k_oblivious_tree(NodeInfo const *__restrict__ front,
+                                                         TreeCtl *__restrict__ ctl, u32 const *__restrict__ level_splits,
+                                                         float const *__restrict__ level_gains, DevNode *__restrict__ tree,
+                                                         GrowCfg cfg, double const *__restrict__ scales)
+{
+    u32 const D = ctl->oblivious_depth;
+    u32 const n_int = (1u << D) - 1u, n_leaf = 1u << D;
+    for (u32 j = threadIdx.x; j < n_leaf; j += blockDim.x) {
+        NodeInfo const nd = front[j];
+        DevNode n{};
+        n.feature = -1;
+        n.value = node_value(nd, cfg, scales);
+        n.cover = static_cast<float>(nd.cnt);
+        tree[n_int + j] = n;
+    }
+    __syncthreads();
+    // Internal nodes level by level from the bottom.
+    for (int lvl = static_cast<int>(D) - 1; lvl >= 0; --lvl) {
+        u32 const first = (1u << lvl) - 1u, count = 1u << lvl;
+        for (u32 k = threadIdx.x; k < count; k += blockDim.x) {
+            u32 const i = first + k;
+            DevNode n{};
+            n.feature = static_cast<int32_t>(level_splits[3 * lvl]);
+            n.bin = level_splits[3 * lvl + 1];
+            n.default_left = level_splits[3 * lvl + 2];
+            n.left = 2 * i + 1;
+            n.right = 2 * i + 2;
+            n.gain = level_gains[lvl];
+            n.cover = tree[2 * i + 1].cover + tree[2 * i + 2].cover;
+            tree[i] = n;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ctl->n_nodes = n_int + n_leaf;
+}
+
+// ---------------------------------------------------------------------------------
+// Leafwise rounds.
+// ---------------------------------------------------------------------------------
+struct LwCtl {
+    u32 n_cand;
+    u32 n_free;
+    u32 parent_slot;
+    u32 demoted;
+    u32 small_left;
+    u32 pad[11];
+};
+
+// Round start: pop the best candidate (max gain, ties to the lower node id; the
+// reference's gain_less, src/grower_impl.hpp:763-770) unless the heap is empty or the
+// leaf budget is spent, then commit its split (commit_split_node, :188-210) and emit
+// the one partition op.
+__global__ __launch_bounds__(256) void k_lw_select(TreeCtl *__restrict__ ctl, LwCtl *__restrict__ lw,
+                                                   Cand *__restrict__ cands, NodeInfo *__restrict__ parent_out,
+                                                   SplitRec *__restrict__ split_out, DevNode *__restrict__ tree,
+                                                   PartOp *__restrict__ ops, PartChunk *__restrict__ pch,
+                                                   u16 const *__restrict__ n_bins, GrowCfg cfg)
+{
+    __shared__ double s_g[256];
+    __shared__ u32 s_id[256], s_ix[256];
+    __shared__ u32 s_sel, s_go;
+    if (ctl->done) {
+        if (threadIdx.x == 0) {
+            ctl->n_ops = 0;
+            ctl->n_part_chunks = 0;
+            ctl->n_front = 0;
+            ctl->n_finish = 0;
+            ctl->n_hist_chunks = 0;
+        }
+        return;
+    }
+    u32 const n = lw->n_cand;
+    double bg = 0.0;
+    u32 bid = kNone, bix = kNone;
+    for (u32 k = threadIdx.x; k < n; k += blockDim.x) {
+        Cand const &c = cands[k];
+        double const g = c.split.gain;
+        u32 const id = c.node.id;
+        if (bix == kNone || g > bg || (g == bg && id < bid)) {
+            bg = g;
+            bid = id;
+            bix = k;
+        }
+    }
+    s_g[threadIdx.x] = bg;
+    s_id[threadIdx.x] = bid;
+    s_ix[threadIdx.x] = bix;
+    __syncthreads();
+    for (u32 st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            u32 const ox = s_ix[threadIdx.x + st];
+            if (ox != kNone) {
+                double const og = s_g[threadIdx.x + st];
+                u32 const oid = s_id[threadIdx.x + st];
+                if (s_ix[threadIdx.x] == kNone || og > s_g[threadIdx.x] ||
+                    (og == s_g[threadIdx.x] && oid < s_id[threadIdx.x])) {
+                    s_g[threadIdx.x] = og;
+                    s_id[threadIdx.x] = oid;
+                    s_ix[threadIdx.x] = ox;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        bool const budget = cfg.max_leaves == 0 || ctl->live < static_cast<u32>(cfg.max_leaves);
+        s_go = (n > 0 && budget) ? 1u : 0u;
+        s_sel = s_ix[0];
+        if (!s_go) {
+            ctl->done = 1;
+            ctl->n_ops = 0;
+            ctl->n_part_chunks = 0;
+            ctl->n_front = 0;
+            ctl->n_finish = 0;
+            ctl->n_hist_chunks = 0;
+        }
+    }
+    __syncthreads();
+    if (!s_go) return;
+    Cand const c = cands[s_sel];
+    __syncthreads();
+    u32 const nch = (c.node.cnt + kPartChunk - 1) / kPartChunk;
+    if (threadIdx.x == 0) {
+        // remove from the pool (order inside the pool is irrelevant)
+        cands[s_sel] = cands[n - 1];
+        lw->n_cand = n - 1;
+        u32 const lid = ctl->n_nodes, rid = lid + 1;
+        ctl->n_nodes = lid + 2;
+        DevNode pn = tree[c.node.id];
+        pn.feature = static_cast<int32_t>(c.split.feature);
+        pn.bin = c.split.bin;
+        pn.default_left = c.split.default_left;
+        pn.left = lid;
+        pn.right = rid;
+        pn.value = 0.0F;
+        pn.gain = static_cast<float>(c.split.gain);
+        tree[c.node.id] = pn;
+        DevNode leaf{};
+        leaf.feature = -1;
+        tree[lid] = leaf;
+        tree[rid] = leaf;
+        parent_out[0] = c.node;
+        split_out[0] = c.split;
+        PartOp op{};
+        op.parent = 0;
+        op.feature = c.split.feature;
+        op.bin = c.split.bin;
+        op.default_left = c.split.default_left;
+        op.last_bin = static_cast<u32>(n_bins[c.split.feature]) - 1u;
+        op.off = c.node.off;
+        op.cnt = c.node.cnt;
+        op.src = c.node.src;
+        op.dst = c.node.src == 0 ? 1u : 0u;
+        op.chunk_first = 0;
+        op.chunk_count = nch;
+        op.left_id = lid;
+        op.right_id = rid;
+        op.alive = 1;
+        ops[0] = op;
+        ctl->n_ops = 1;
+        ctl->n_part_chunks = nch;
+        lw->parent_slot = c.node.slot;
+    }
+    for (u32 k = threadIdx.x; k < nch; k += blockDim.x) {
+        PartChunk pc;
+        pc.op = 0;
+        pc.begin = k * kPartChunk;
+        pc.end = min(c.node.cnt, (k + 1) * kPartChunk);
+        pc.left_before = 0;
+        pch[k] = pc;
+    }
+}
+
+// Round middle: left count from the unit prefix, demotion (commit_pop,
+// src/grower_impl.hpp:370-400), children, slot handling (the larger child keeps the
+// parent's histogram slot and is completed in place), histogram units of the smaller
+// child unless the children sit at max_depth (then they only need totals, from the
+// parent split's sums).
+template <bool Q>
+__global__ __launch_bounds__(1024) void k_lw_scan(PartOp *__restrict__ ops, PartChunk *__restrict__ pch,
+                                                  u32 const *__restrict__ chunk_left, TreeCtl *__restrict__ ctl,
+                                                  LwCtl *__restrict__ lw, NodeInfo const *__restrict__ parent_in,
+                                                  SplitRec const *__restrict__ split_in, u32 *__restrict__ free_slots,
+                                                  NodeInfo *__restrict__ kids, DevNode *__restrict__ tree,
+                                                  HistChunk *__restrict__ hch, FinishOp *__restrict__ fops,
+                                                  GrowCfg cfg, double const *__restrict__ scales)
+{
+    __shared__ u32 scratch[1024];
+    __shared__ u32 s_run;
+    if (ctl->n_ops == 0) return;
+    u32 const n_ch = ctl->n_part_chunks;
+    if (threadIdx.x == 0) s_run = 0;
+    __syncthreads();
+    for (u32 t0 = 0; t0 < n_ch; t0 += blockDim.x) {
+        u32 const c = t0 + threadIdx.x;
+        u32 const v = c < n_ch ? chunk_left[c] : 0u;
+        u32 tot;
+        u32 const pre = block_exscan_ext(v, scratch, &tot) + s_run;
+        if (c < n_ch) pch[c].left_before = pre;
+        __syncthreads();
+        if (threadIdx.x == 0) s_run += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) {
+        return;
+    }
+    PartOp op = ops[0];
+    NodeInfo const par = parent_in[0];
+    SplitRec const s = split_in[0];
+    u32 const nL = s_run, nR = op.cnt - s_run;
+    op.n_left = nL;
+    bool const demote = (nL == 0) != (nR == 0);
+    op.alive = demote ? 0u : 1u;
+    ops[0] = op;
+    if (demote) {
+        put_leaf(tree, par.id, node_value(par, cfg, scales));
+        tree[par.id].gain = 0.0F;
+        free_slots[lw->n_free++] = par.slot;
+        ctl->n_front = 0;
+        ctl->n_finish = 0;
+        ctl->n_hist_chunks = 0;
+        lw->demoted = 1;
+        return;
+    }
+    lw->demoted = 0;
+    tree[op.left_id].cover = static_cast<float>(nL);
+    tree[op.right_id].cover = static_cast<float>(nR);
+    NodeInfo L{}, R{};
+    L.id = op.left_id;
+    R.id = op.right_id;
+    L.off = op.off;
+    L.cnt = nL;
+    R.off = op.off + nL;
+    R.cnt = nR;
+    L.src = R.src = op.dst;
+    L.depth = R.depth = par.depth + 1;
+    bool const left_small = nL <= nR;
+    bool const leaves = static_cast<int>(L.depth) >= cfg.max_depth;
+    ctl->live += 1;
+    ctl->max_depth_seen = max(ctl->max_depth_seen, L.depth);
+    if (leaves) {
+        // children can never split: totals from the split's sums, no histograms
+        L.totals_ready = R.totals_ready = 1;
+        if (Q) {
+            L.itg = s.igL;
+            L.ith = s.ihL;
+            R.itg = s.igR;
+            R.ith = s.ihR;
+        } else {
+            L.tg = static_cast<float>(s.gL);
+            L.th = static_cast<float>(s.hL);
+            R.tg = static_cast<float>(s.gR);
+            R.th = static_cast<float>(s.hR);
+        }
+        L.slot = R.slot = kNone;
+        free_slots[lw->n_free++] = par.slot;
+        ctl->n_finish = 0;
+        ctl->n_hist_chunks = 0;
+    } else {
+        u32 const fresh = free_slots[--lw->n_free];
+        (left_small ? L : R).slot = fresh;
+        (left_small ? R : L).slot = par.slot;
+        u32 const small_cnt = left_small ? nL : nR;
+        u32 const soff = left_small ? L.off : R.off;
+        u32 const units = (small_cnt + kHistChunk - 1) / kHistChunk;
+        FinishOp fo{};
+        fo.small_slot = fresh;
+        fo.large_slot = par.slot;
+        fo.parent_slot = par.slot;
+        fo.part_first = 0;
+        fo.part_count = units > 1 ? units : 0;
+        fo.empty = small_cnt == 0;
+        fo.unit_first = 0;
+        fo.unit_count = units;
+        fo.small_off = soff;
+        fo.small_cnt = small_cnt;
+        fo.src = op.dst;
+        fops[0] = fo;
+        ctl->n_finish = 1;
+        ctl->n_hist_chunks = units;
+    }
+    lw->small_left = left_small ? 1u : 0u;
+    kids[0] = L;
+    kids[1] = R;
+    ctl->n_front = 2;
+}
+
+// Histogram units of the smaller child (parallel expand of fops[0]).
+__global__ __launch_bounds__(256) void k_lw_units(TreeCtl const *__restrict__ ctl, FinishOp const *__restrict__ fops,
+                                                  HistChunk *__restrict__ hch)
+{
+    if (ctl->n_finish == 0) return;
+    FinishOp const fo = fops[0];
+    for (u32 k = threadIdx.x + blockIdx.x * blockDim.x; k < fo.unit_count; k += blockDim.x * gridDim.x) {
+        HistChunk hc{};
+        hc.slot = fo.small_slot;
+        hc.src = fo.src;
+        hc.begin = fo.small_off + k * kHistChunk;
+        hc.end = fo.small_off + min(fo.small_cnt, (k + 1) * kHistChunk);
+        hc.part = fo.unit_count > 1 ? k : kNone;
+        hch[k] = hc;
+    }
+}
+
+// Round end (queue_children, src/grower_impl.hpp:404-428): a child with a valid split
+// joins the pool; one without becomes a leaf now and frees its slot.
+__global__ void k_lw_push(TreeCtl *__restrict__ ctl, LwCtl *__restrict__ lw, NodeInfo const *__restrict__ kids,
+                          SplitRec const *__restrict__ splits, Cand *__restrict__ cands, u32 *__restrict__ free_slots,
+                          DevNode *__restrict__ tree, GrowCfg cfg, double const *__restrict__ scales, u32 is_root)
+{
+    if (threadIdx.x != 0) return;
+    if (ctl->done) return;
+    u32 const n = is_root ? 1u : ctl->n_front;
+    for (u32 k = 0; k < n; ++k) {
+        NodeInfo const nd = kids[k];
+        SplitRec const s = splits[k];
+        bool const ok = s.valid && (!is_root || cfg.max_depth > 0);
+        if (ok) {
+            Cand c;
+            c.node = nd;
+            c.split = s;
+            cands[lw->n_cand++] = c;
+        } else {
+            put_leaf(tree, nd.id, node_value(nd, cfg, scales));
+            if (nd.slot != kNone) free_slots[lw->n_free++] = nd.slot;
+        }
+    }
+}
+
+// Tree end: what is left in the pool becomes leaves (src/grower_impl.hpp:838-846).
+__global__ void k_lw_finalize(TreeCtl *__restrict__ ctl, LwCtl *__restrict__ lw, Cand const *__restrict__ cands,
+                              DevNode *__restrict__ tree, GrowCfg cfg, double const *__restrict__ scales)
+{
+    u32 const n = lw->n_cand;
+    for (u32 k = threadIdx.x; k < n; k += blockDim.x) put_leaf(tree, cands[k].node.id, node_value(cands[k].node, cfg, scales));
+    (void)ctl;
+}
+
+// Leafwise tree start: root frontier (slot 0), free list 1.., pool empty.
+__global__ __launch_bounds__(256) void k_lw_begin(TreeCtl *__restrict__ ctl, LwCtl *__restrict__ lw,
+                                                  NodeInfo *__restrict__ kids, DevNode *__restrict__ tree,
+                                                  HistChunk *__restrict__ hch, FinishOp *__restrict__ fops,
+                                                  u32 *__restrict__ free_slots, u32 n_slots, u32 n_rows)
+{
+    u32 const n_units = (n_rows + kHistChunk - 1) / kHistChunk;
+    u32 const multi = n_units > 1;
+    for (u32 k = threadIdx.x; k < n_units; k += blockDim.x) {
+        HistChunk hc{};
+        hc.slot = 0;
+        hc.src = kSrcPlane0;
+        hc.begin = k * kHistChunk;
+        hc.end = min(n_rows, (k + 1) * kHistChunk);
+        hc.part = multi ? k : kNone;
+        hch[k] = hc;
+    }
+    for (u32 k = threadIdx.x; k + 1 < n_slots; k += blockDim.x) free_slots[k] = n_slots - 1 - k;
+    if (threadIdx.x == 0) {
+        NodeInfo root{};
+        root.cnt = n_rows;
+        root.src = kSrcPlane0;
+        root.slot = 0;
+        kids[0] = root;
+        DevNode r{};
+        r.feature = -1;
+        r.cover = static_cast<float>(n_rows);
+        tree[0] = r;
+        FinishOp fo{};
+        fo.small_slot = 0;
+        fo.large_slot = kNone;
+        fo.part_count = multi ? n_units : 0;
+        fo.empty = n_rows == 0;
+        fops[0] = fo;
+        ctl->n_front = 1;
+        ctl->n_nodes = 1;
+        ctl->n_ops = 0;
+        ctl->n_part_chunks = 0;
+        ctl->n_hist_chunks = n_units;
+        ctl->n_finish = 1;
+        ctl->done = 0;
+        ctl->live = 1;
+        ctl->max_depth_seen = 0;
+        lw->n_cand = 0;
+        lw->n_free = n_slots - 1;
+    }
+}
+
+__global__ void k_obl_begin(TreeCtl *ctl)
+{
+    if (threadIdx.x == 0) {
+        ctl->done = 0;
+        ctl->oblivious_depth = 0;
+    }
+}
+
+} // namespace
+
+__device__ u32 block_exscan_ext(u32 v, u32 *scratch, u32 *total)
+{
+    u32 const t = threadIdx.x;
+    scratch[t] = v;
+    __syncthreads();
+    for (u32 s = 1; s < blockDim.x; s <<= 1) {
+        u32 const add = t >= s ? scratch[t - s] : 0u;
+        __syncthreads();
+        scratch[t] += add;
+        __syncthreads();
+    }
+    u32 const incl = scratch[t];
+    *total = scratch[blockDim.x - 1];
+    __syncthreads();
+    return incl - v;
+}
+
 hipError_t ext_alloc(ExtState &s, bh_fit_config const &f, u32 F, u32 SB, u32 max_front, u32 hist_slots)
 {
-    (void)s; (void)f; (void)F; (void)SB; (void)max_front; (void)hist_slots;
+    (void)SB;
+    (void)max_front;
+    s.n_features = F;
+    hipError_t e;
+    if (f.grower == BH_LEAFWISE) {
+        s.cap = static_cast<u32>(f.max_leaves) + 8u;
+        if ((e = hipMalloc(&s.cands, sizeof(Cand) * s.cap)) != hipSuccess) return e;
+        if ((e = hipMalloc(&s.free_slots, sizeof(u32) * (hist_slots + 4))) != hipSuccess) return e;
+        if ((e = hipMalloc(&s.pending, sizeof(LwCtl))) != hipSuccess) return e;
+        if ((e = hipMalloc(&s.feat_best, sizeof(SplitRec) * 4)) != hipSuccess) return e; // parent split
+    }
+    if (f.grower == BH_LEVELWISE) {
+        if ((e = hipMalloc(&s.feat_best, sizeof(SplitRec) * (F + 2))) != hipSuccess) return e;
+        if ((e = hipMalloc(&s.level_splits, sizeof(u32) * 3 * 16)) != hipSuccess) return e;
+        if ((e = hipMalloc(&s.level_gains, sizeof(float) * 16)) != hipSuccess) return e;
+    }
+    s.hist_slots = hist_slots;
     return hipSuccess;
 }
-void ext_free(ExtState &s) { (void)s; }
+
+void ext_free(ExtState &s)
+{
+    void *p[] = {s.cands, s.pending, s.free_slots, s.feat_best, s.level_splits, s.level_gains};
+    for (void *q : p)
+        if (q) (void)hipFree(q);
+    s = ExtState{};
+}
+
 hipError_t ext_set_limits() { return hipSuccess; }
-void enqueue_levelwise(ExtState &s, hipStream_t st, bool q, GrowView const &v) { (void)s; (void)st; (void)q; (void)v; }
-void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v) { (void)s; (void)st; (void)q; (void)v; }
+
+// ObliviousGrower::grow as a launch sequence (fixed shape: every level is enqueued, a
+// stopped tree turns the remaining levels into no-ops).
+void enqueue_levelwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
+{
+    u32 const D = static_cast<u32>(v.fc.max_depth);
+    u32 const gh_bytes = q ? 4 : 8;
+    SplitRec *pick = s.feat_best + v.F; // one spare record after the per-feature bests
+    launch_tree_begin(st, v.ctl, v.front[0], v.tree, v.hch, v.fops, static_cast<u32>(v.N));
+    hipLaunchKernelGGL(k_obl_begin, dim3(1), dim3(64), 0, st, v.ctl);
+    launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
+                v.fsl, v.waves, std::max(1u, std::min(v.max_hch, 4096u)), v.partials, v.hist[0]);
+    launch_finish(st, q, v.fops, v.ctl, v.partials, v.hist[1], v.hist[0], v.F * v.SB, 1);
+    u32 const TN = q ? kLevelTileQ : kLevelTileF32;
+    size_t const lds_level = static_cast<size_t>(TN) * v.SB * (q ? 16 : 8);
+    for (u32 d = 0; d < D; ++d) {
+        NodeInfo *cur = v.front[d & 1u], *nxt = v.front[(d + 1) & 1u];
+        void *hcur = v.hist[d & 1u], *hnext = v.hist[(d + 1) & 1u];
+        u32 const width = 1u << d;
+        bool const last = d + 1 == D;
+        if (q)
+            hipLaunchKernelGGL(k_totals<true>, dim3(width), dim3(64), 0, st, cur, v.ctl, hcur, v.d_nbins, v.F, v.SB);
+        else
+            hipLaunchKernelGGL(k_totals<false>, dim3(width), dim3(64), 0, st, cur, v.ctl, hcur, v.d_nbins, v.F, v.SB);
+        if (q)
+            hipLaunchKernelGGL(k_find_level<true>, dim3(v.F), dim3(256), lds_level, st, cur, v.ctl, hcur, v.d_nbins, v.F,
+                               v.SB, v.gc, v.scales, s.feat_best);
+        else
+            hipLaunchKernelGGL(k_find_level<false>, dim3(v.F), dim3(256), lds_level, st, cur, v.ctl, hcur, v.d_nbins,
+                               v.F, v.SB, v.gc, v.scales, s.feat_best);
+        hipLaunchKernelGGL(k_level_pick, dim3(1), dim3(256), 0, st, s.feat_best, v.F, v.ctl, s.level_splits,
+                           s.level_gains, pick);
+        if (q)
+            hipLaunchKernelGGL(k_plan_oblivious<true>, dim3(1), dim3(1024), 0, st, cur, v.ctl, pick, hcur, v.d_nbins,
+                               v.F, v.SB, v.splits, v.ops, v.pch, d & 1u, last ? 1u : 0u);
+        else
+            hipLaunchKernelGGL(k_plan_oblivious<false>, dim3(1), dim3(1024), 0, st, cur, v.ctl, pick, hcur,
+                               v.d_nbins, v.F, v.SB, v.splits, v.ops, v.pch, d & 1u, last ? 1u : 0u);
+        launch_part_count(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.Fs, v.chunk_left,
+                          std::max(1u, std::min(v.max_pch, 2048u)));
+        launch_part_scan(st, q, v.ops, v.pch, v.chunk_left, v.ctl, cur, nxt, v.splits, v.tree, v.hch, v.fops, v.gc,
+                         v.scales, last ? 1u : 0u, 1u);
+        if (!last) {
+            launch_part_scatter(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2],
+                                v.work[0], v.work[1], const_cast<void *>(v.gh[0]), const_cast<void *>(v.gh[1]), v.Fs,
+                                gh_bytes, std::max(1u, std::min(v.max_pch, 2048u)));
+            launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F,
+                        v.SB, v.fsl, v.waves, std::max(1u, std::min(v.max_hch, 4096u)), v.partials, hnext);
+            launch_finish(st, q, v.fops, v.ctl, v.partials, hcur, hnext, v.F * v.SB, std::min(width, v.max_ops));
+        }
+        hipLaunchKernelGGL(k_next_level_obl, dim3(1), dim3(256), 0, st, v.ctl, cur, nxt);
+        if (v.host_sync) (void)hipStreamSynchronize(st);
+    }
+    if (q)
+        hipLaunchKernelGGL(k_oblivious_tree<true>, dim3(1), dim3(1024), 0, st, v.front[D & 1u], v.ctl, s.level_splits,
+                           s.level_gains, v.tree, v.gc, v.scales);
+    else
+        hipLaunchKernelGGL(k_oblivious_tree<false>, dim3(1), dim3(1024), 0, st, v.front[D & 1u], v.ctl,
+                           s.level_splits, s.level_gains, v.tree, v.gc, v.scales);
+}
+
+// LeafwiseGrower::grow as a launch sequence: the root, then max_leaves - 1 rounds
+// (plus a margin for demoted splits, which spend a round without adding a leaf); rounds
+// after the heap empties or the budget is spent are no-ops.
+void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
+{
+    u32 const gh_bytes = q ? 4 : 8;
+    LwCtl *lw = reinterpret_cast<LwCtl *>(s.pending);
+    NodeInfo *kids = v.front[0];
+    NodeInfo *parent = v.front[1];
+    SplitRec *psplit = s.feat_best;
+    void *pool = v.hist[0];
+    u32 const pgrid = std::max(1u, std::min(v.max_pch, 2048u));
+    u32 const hgrid = std::max(1u, std::min(v.max_hch, 4096u));
+    hipLaunchKernelGGL(k_lw_begin, dim3(1), dim3(256), 0, st, v.ctl, lw, kids, v.tree, v.hch, v.fops, s.free_slots,
+                       s.hist_slots, static_cast<u32>(v.N));
+    launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
+                v.fsl, v.waves, hgrid, v.partials, pool);
+    launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1);
+    launch_find(st, q, kids, v.ctl, pool, v.d_nbins, v.F, v.SB, v.gc, v.scales, v.splits, 2, 0, 0);
+    hipLaunchKernelGGL(k_lw_push, dim3(1), dim3(64), 0, st, v.ctl, lw, kids, v.splits, s.cands, s.free_slots, v.tree,
+                       v.gc, v.scales, 1u);
+    u32 const rounds = static_cast<u32>(v.fc.max_leaves) - 1u + 8u;
+    for (u32 r = 0; r < rounds; ++r) {
+        hipLaunchKernelGGL(k_lw_select, dim3(1), dim3(256), 0, st, v.ctl, lw, s.cands, parent, psplit, v.tree, v.ops,
+                           v.pch, v.d_nbins, v.gc);
+        launch_part_count(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.Fs, v.chunk_left, pgrid);
+        if (q)
+            hipLaunchKernelGGL(k_lw_scan<true>, dim3(1), dim3(1024), 0, st, v.ops, v.pch, v.chunk_left, v.ctl, lw,
+                               parent, psplit, s.free_slots, kids, v.tree, v.hch, v.fops, v.gc, v.scales);
+        else
+            hipLaunchKernelGGL(k_lw_scan<false>, dim3(1), dim3(1024), 0, st, v.ops, v.pch, v.chunk_left, v.ctl, lw,
+                               parent, psplit, s.free_slots, kids, v.tree, v.hch, v.fops, v.gc, v.scales);
+        launch_part_scatter(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2],
+                            v.work[0], v.work[1], const_cast<void *>(v.gh[0]), const_cast<void *>(v.gh[1]), v.Fs,
+                            gh_bytes, pgrid);
+        hipLaunchKernelGGL(k_lw_units, dim3(8), dim3(256), 0, st, v.ctl, v.fops, v.hch);
+        launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
+                    v.fsl, v.waves, hgrid, v.partials, pool);
+        launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1);
+        launch_find(st, q, kids, v.ctl, pool, v.d_nbins, v.F, v.SB, v.gc, v.scales, v.splits, 2, 0, 0);
+        hipLaunchKernelGGL(k_lw_push, dim3(1), dim3(64), 0, st, v.ctl, lw, kids, v.splits, s.cands, s.free_slots,
+                           v.tree, v.gc, v.scales, 0u);
+        if (v.host_sync) (void)hipStreamSynchronize(st);
+    }
+    hipLaunchKernelGGL(k_lw_finalize, dim3(1), dim3(256), 0, st, v.ctl, lw, s.cands, v.tree, v.gc, v.scales);
+}
 
 } // namespace bh

@@ -50,6 +50,7 @@ struct ExtState {
     u32 *level_splits = nullptr;   // (feature, bin, default_left) per level
     float *level_gains = nullptr;
     u32 n_features = 0;
+    u32 hist_slots = 0;
 };
 
 hipError_t ext_alloc(ExtState &s, bh_fit_config const &f, u32 F, u32 SB, u32 max_front, u32 hist_slots);
