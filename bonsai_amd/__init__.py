@@ -52,6 +52,17 @@ class FitConfigC(C.Structure):
     ]
 
 
+class ServeConfigC(C.Structure):
+    _fields_ = [("n_slots", C.c_uint32), ("do_shap", C.c_uint32), ("max_batch_rows", C.c_uint64),
+                ("hbm_budget", C.c_uint64)]
+
+
+class ServeStatsC(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("rows_per_s", C.c_double), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("rows", C.c_uint64),
+                ("requests", C.c_uint32), ("max_inflight", C.c_uint32)]
+
+
 NODE_DTYPE = np.dtype([
     ("feature", np.int32), ("bin", np.uint32), ("default_left", np.uint32),
     ("left", np.uint32), ("right", np.uint32), ("value", np.float32),
@@ -112,10 +123,13 @@ def lib() -> C.CDLL:
     P, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32
     sig = {
         "bh_device_count": [C.POINTER(C.c_int)],
+        "bh_synchronize": [P],
         "bh_create": [C.c_int, C.POINTER(P)],
         "bh_destroy": [P],
         "bh_upload_bins": [P, P, u64, u32, u64, P],
         "bh_upload_labels": [P, P],
+        "bh_upload_features": [P, P, u64, u32, P, P],
+        "bh_download_bins": [P, P, u64],
         "bh_configure": [P, C.POINTER(FitConfigC)],
         "bh_set_scores": [P, P],
         "bh_get_scores": [P, P],
@@ -129,6 +143,8 @@ def lib() -> C.CDLL:
         "bh_partition": [P, P, u64, u32, u32, u32, P, C.POINTER(u64)],
         "bh_time_root_histogram": [P, u32, C.POINTER(C.c_float)],
         "bh_set_timing": [P, i32],
+        "bh_profile_hist": [P, u32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(u64),
+                            C.POINTER(u64)],
         "bh_get_timing": [P, P, u32],
         "bh_predictor_create": [C.c_int, P, P, u32, P, u32, C.c_float, C.c_float, C.POINTER(P)],
         "bh_predictor_destroy": [P],
@@ -136,6 +152,7 @@ def lib() -> C.CDLL:
         "bh_shap": [P, P, u64, u64, P],
         "bh_predict_device": [P, P, u64, u64, P, P],
         "bh_shap_device": [P, P, u64, u64, P, P],
+        "bh_serve": [P, P, u64, P, P, u32, C.POINTER(ServeConfigC), P, P, P, C.POINTER(ServeStatsC)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -158,6 +175,11 @@ def _check(rc: int) -> None:
 
 def _ptr(a: np.ndarray | None):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def synchronize(ctx: "HipContext") -> None:
+    """Wait for the context's stream."""
+    _check(lib().bh_synchronize(ctx._h))
 
 
 def device_count() -> int:
@@ -202,6 +224,24 @@ class HipContext:
         n, f = bins.shape
         _check(lib().bh_upload_bins(self._h, _ptr(bins), n, f, f, _ptr(n_bins)))
         self.n_rows, self.n_features, self.n_bins = n, f, n_bins.copy()
+
+    def upload_features(self, X: np.ndarray, cuts: list) -> None:
+        """Bin a row-major float matrix on the device against per-feature cut tables."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        n, f = X.shape
+        if len(cuts) != f:
+            raise ConfigError("one cut table per feature")
+        flat = np.ascontiguousarray(np.concatenate(cuts).astype(np.float32))
+        offs = np.zeros(f + 1, np.uint32)
+        offs[1:] = np.cumsum([c.size for c in cuts])
+        _check(lib().bh_upload_features(self._h, _ptr(X), n, f, _ptr(flat), _ptr(offs)))
+        self.n_rows, self.n_features = n, f
+        self.n_bins = np.diff(offs).astype(np.uint16)
+
+    def download_bins(self) -> np.ndarray:
+        out = np.empty((self.n_rows, self.n_features), np.uint8)
+        _check(lib().bh_download_bins(self._h, _ptr(out), self.n_features))
+        return out
 
     def upload_labels(self, labels: np.ndarray) -> None:
         labels = np.ascontiguousarray(labels, dtype=np.float32)
@@ -306,6 +346,14 @@ class HipContext:
                                   _ptr(out), C.byref(nl)))
         return out, nl.value
 
+    def profile_hist(self, n_iters: int) -> dict:
+        """Per-launch hipEvent timing of every histogram launch over n_iters iterations."""
+        ms, by = C.c_double(0), C.c_double(0)
+        la, ro = C.c_uint64(0), C.c_uint64(0)
+        _check(lib().bh_profile_hist(self._h, n_iters, C.byref(ms), C.byref(by), C.byref(la), C.byref(ro)))
+        self.n_trained += n_iters
+        return {"ms": ms.value, "bytes": by.value, "launches": la.value, "rows": ro.value}
+
     def time_root_histogram(self, reps: int = 10) -> float:
         ms = C.c_float(0)
         _check(lib().bh_time_root_histogram(self._h, reps, C.byref(ms)))
@@ -352,3 +400,20 @@ class Predictor:
         out = np.zeros((bins.shape[0], self.n_features + 1), np.float64)
         _check(lib().bh_shap(self._h, _ptr(bins), bins.shape[0], bins.shape[1], _ptr(out)))
         return out
+
+    def serve(self, rows: np.ndarray, req_offsets: np.ndarray, arrival_s: np.ndarray, n_slots: int = 4,
+              do_shap: bool = True, max_batch_rows: int = 65536, hbm_budget: int = 1 << 30):
+        """Continuous-batching request loop (bh_serve): returns (pred, shap, latency_ms, stats)."""
+        rows = np.ascontiguousarray(rows, np.uint8)
+        offs = np.ascontiguousarray(req_offsets, np.uint64)
+        arr = np.ascontiguousarray(arrival_s, np.float64)
+        n_req = arr.size
+        pred = np.zeros(rows.shape[0], np.float32)
+        shap = np.zeros((rows.shape[0], self.n_features + 1), np.float64) if do_shap else None
+        lat = np.zeros(n_req, np.float64)
+        cfg = ServeConfigC(n_slots, int(do_shap), max_batch_rows, hbm_budget)
+        st = ServeStatsC()
+        _check(lib().bh_serve(self._h, _ptr(rows), rows.shape[1], _ptr(offs), _ptr(arr), n_req, C.byref(cfg),
+                              _ptr(pred), _ptr(shap), _ptr(lat), C.byref(st)))
+        stats = {k: getattr(st, k) for k, _ in ServeStatsC._fields_}
+        return pred, shap, lat, stats

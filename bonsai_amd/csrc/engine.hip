@@ -48,6 +48,8 @@ u32 round_up4(u32 x) { return (x + 3u) & ~3u; }
 
 } // namespace
 
+void bh_set_error(std::string const &m) { g_err = m; }
+
 struct bh_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -92,6 +94,10 @@ struct bh_context {
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // histogram profiling (per-launch hipEvents, non-graph iterations only)
+    bool prof = false;
+    double prof_ms = 0.0, prof_bytes = 0.0;
+    u64 prof_launches = 0, prof_rows = 0;
     // timing
     bool timing = false;
     std::vector<float> iter_ms;
@@ -253,12 +259,51 @@ hipError_t host_sync_point(bh_context *c)
     return hipStreamSynchronize(c->stream);
 }
 
-// Histogram units -> partials / slots, then the fixed-order completion.
-void enqueue_build(bh_context *c, void *hist_cur, void *hist_next, u32 max_ops, u32 grid_units)
+// Profiling: time one histogram launch with hipEvents on its stream and count its
+// algorithmic bytes: rows x (F + gradient pair bytes) read + one 8-byte cell per
+// (built node, feature, bin) written.  Rows are physically compacted per node, so no
+// row-index bytes are read.
+void profiled_hist(bh_context *c, void *hist_next, u32 grid_units)
 {
+    (void)hipEventRecord(c->ev0, c->stream);
     launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
                 gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), grid_units,
                 c->partials, hist_next);
+    (void)hipEventRecord(c->ev1, c->stream);
+    (void)hipEventSynchronize(c->ev1);
+    float ms = 0.0F;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    TreeCtl ctl;
+    (void)hipMemcpy(&ctl, c->ctl, sizeof(ctl), hipMemcpyDeviceToHost);
+    if (ctl.n_hist_chunks == 0) return;
+    std::vector<HistChunk> hc(ctl.n_hist_chunks);
+    (void)hipMemcpy(hc.data(), c->hch, hc.size() * sizeof(HistChunk), hipMemcpyDeviceToHost);
+    u64 rows = 0;
+    std::vector<u32> slots;
+    for (auto const &u : hc) {
+        rows += u.end - u.begin;
+        slots.push_back(u.slot);
+    }
+    std::sort(slots.begin(), slots.end());
+    u64 const nodes = static_cast<u64>(std::unique(slots.begin(), slots.end()) - slots.begin());
+    double sum_bins = 0.0;
+    for (u16 b : c->n_bins) sum_bins += b;
+    double const cells = static_cast<double>(nodes) * sum_bins * 8.0;
+    c->prof_ms += ms;
+    c->prof_rows += rows;
+    c->prof_bytes += static_cast<double>(rows) * (c->F + (c->quant ? 4.0 : 8.0)) + cells;
+    c->prof_launches += 1;
+}
+
+// Histogram units -> partials / slots, then the fixed-order completion.
+void enqueue_build(bh_context *c, void *hist_cur, void *hist_next, u32 max_ops, u32 grid_units)
+{
+    if (c->prof)
+        profiled_hist(c, hist_next, grid_units);
+    else
+        launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
+                    gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), grid_units,
+                    c->partials, hist_next);
     launch_finish(c->stream, c->quant, c->fops, c->ctl, c->partials, hist_cur, hist_next, c->node_cells(), max_ops);
 }
 
@@ -380,6 +425,14 @@ extern "C" {
 
 const char *bh_last_error(void) { return g_err.c_str(); }
 
+int bh_synchronize(bh_context *c)
+{
+    if (!c) return fail(1, "null context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int bh_device_count(int *count)
 {
     hipError_t const e = hipGetDeviceCount(count);
@@ -433,13 +486,12 @@ int bh_destroy(bh_context *c)
     return 0;
 }
 
-int bh_upload_bins(bh_context *c, const uint8_t *bins, uint64_t n_rows, uint32_t n_features, uint64_t row_stride,
-                   const uint16_t *n_bins)
+// (Re)allocates the pristine plane, the two work planes, labels and scores for an
+// n_rows x n_features dataset; the plane is zeroed (padding columns stay 0).
+static int alloc_plane(bh_context *c, uint64_t n_rows, uint32_t n_features, const uint16_t *n_bins)
 {
-    if (!c) return fail(1, "null context");
     if (n_rows == 0 || n_features == 0) return fail(1, "empty plane");
     if (n_rows >= 0xFFFFFFF0ull) return fail(1, "n_rows exceeds the 32-bit row index");
-    if (row_stride < n_features) return fail(1, "row_stride < n_features");
     u32 SB = 0;
     for (u32 f = 0; f < n_features; ++f) {
         if (n_bins[f] < 1 || n_bins[f] > 256) return fail(1, "n_bins must be in [1, 256] (u8 bins)");
@@ -465,17 +517,79 @@ int bh_upload_bins(bh_context *c, const uint8_t *bins, uint64_t n_rows, uint32_t
     HIP_TRY(hipMalloc(&c->work[0], bytes));
     HIP_TRY(hipMalloc(&c->work[1], bytes));
     HIP_TRY(hipMemset(c->plane0, 0, bytes));
-    if (row_stride == c->Fs) {
-        HIP_TRY(hipMemcpy(c->plane0, bins, bytes, hipMemcpyHostToDevice));
-    } else {
-        HIP_TRY(hipMemcpy2D(c->plane0, c->Fs, bins, row_stride, n_features, n_rows, hipMemcpyHostToDevice));
-    }
     HIP_TRY(dalloc(&c->d_nbins, n_features));
     HIP_TRY(hipMemcpy(c->d_nbins, n_bins, n_features * sizeof(u16), hipMemcpyHostToDevice));
     HIP_TRY(dalloc(&c->label, n_rows));
     HIP_TRY(dalloc(&c->score, n_rows));
     HIP_TRY(hipMemset(c->label, 0, n_rows * sizeof(float)));
     HIP_TRY(hipMemset(c->score, 0, n_rows * sizeof(float)));
+    return 0;
+}
+
+int bh_upload_bins(bh_context *c, const uint8_t *bins, uint64_t n_rows, uint32_t n_features, uint64_t row_stride,
+                   const uint16_t *n_bins)
+{
+    if (!c) return fail(1, "null context");
+    if (row_stride < n_features) return fail(1, "row_stride < n_features");
+    if (int r = alloc_plane(c, n_rows, n_features, n_bins)) return r;
+    size_t const bytes = static_cast<size_t>(n_rows) * c->Fs;
+    if (row_stride == c->Fs) {
+        HIP_TRY(hipMemcpy(c->plane0, bins, bytes, hipMemcpyHostToDevice));
+    } else {
+        HIP_TRY(hipMemcpy2D(c->plane0, c->Fs, bins, row_stride, n_features, n_rows, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int bh_upload_features(bh_context *c, const float *X, uint64_t n_rows, uint32_t n_features, const float *cuts,
+                       const uint32_t *cut_offsets)
+{
+    if (!c) return fail(1, "null context");
+    if (n_features == 0) return fail(1, "empty plane");
+    if (cut_offsets[0] != 0) return fail(1, "cut_offsets[0] must be 0");
+    std::vector<u16> nb(n_features);
+    for (u32 f = 0; f < n_features; ++f) {
+        if (cut_offsets[f + 1] <= cut_offsets[f]) return fail(1, "every feature needs at least one cut");
+        u32 const k = cut_offsets[f + 1] - cut_offsets[f];
+        if (k > 256) return fail(1, "more than 256 cuts for a feature (u8 bins)");
+        nb[f] = static_cast<u16>(k);
+    }
+    if (int r = alloc_plane(c, n_rows, n_features, nb.data())) return r;
+    u32 const n_cuts = cut_offsets[n_features];
+    float *d_cuts = nullptr, *d_x = nullptr;
+    u32 *d_off = nullptr;
+    u64 const chunk = std::min<u64>(n_rows, std::max<u64>(1, (u64{256} << 20) / (u64{4} * n_features)));
+    auto cleanup = [&] {
+        if (d_cuts) (void)hipFree(d_cuts);
+        if (d_off) (void)hipFree(d_off);
+        if (d_x) (void)hipFree(d_x);
+    };
+    hipError_t e = dalloc(&d_cuts, n_cuts);
+    if (e == hipSuccess) e = dalloc(&d_off, n_features + 1);
+    if (e == hipSuccess) e = dalloc(&d_x, chunk * n_features);
+    if (e == hipSuccess) e = hipMemcpy(d_cuts, cuts, n_cuts * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(d_off, cut_offsets, (n_features + 1) * sizeof(u32), hipMemcpyHostToDevice);
+    for (u64 r0 = 0; e == hipSuccess && r0 < n_rows; r0 += chunk) {
+        u64 const n = std::min(chunk, n_rows - r0);
+        e = hipMemcpy(d_x, X + r0 * n_features, n * n_features * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) break;
+        launch_bin_rows(c->stream, d_x, n, n_features, c->Fs, d_cuts, d_off, n_cuts, c->plane0 + r0 * c->Fs);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    cleanup();
+    if (e != hipSuccess) return fail(-1, std::string("hip: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int bh_download_bins(bh_context *c, uint8_t *out, uint64_t out_stride)
+{
+    if (!c || !c->plane0) return fail(1, "no bins uploaded");
+    if (out_stride < c->F) return fail(1, "out_stride < n_features");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy2D(out, out_stride, c->plane0, c->Fs, c->F, c->N, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -812,6 +926,28 @@ int bh_partition(bh_context *c, const uint32_t *rows, uint64_t n, uint32_t featu
     } else {
         HIP_TRY(hipMemcpy(out_rows, gh_src(c, 1), n * sizeof(u32), hipMemcpyDeviceToHost));
     }
+    return 0;
+}
+
+int bh_profile_hist(bh_context *c, uint32_t n_iters, double *ms_total, double *bytes_total, uint64_t *launches,
+                    uint64_t *rows_total)
+{
+    if (int r = check_ready(c)) return r;
+    if (c->fc.grower != BH_DEPTHWISE) return fail(1, "histogram profiling covers the depthwise grower");
+    HIP_TRY(hipSetDevice(c->device));
+    if (int r = ensure_tree_out(c, n_iters)) return r;
+    c->prof = true;
+    c->prof_ms = c->prof_bytes = 0.0;
+    c->prof_launches = c->prof_rows = 0;
+    for (u32 it = 0; it < n_iters; ++it) enqueue_iteration(c);
+    c->prof = false;
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->trees_done += n_iters;
+    *ms_total = c->prof_ms;
+    *bytes_total = c->prof_bytes;
+    *launches = c->prof_launches;
+    *rows_total = c->prof_rows;
     return 0;
 }
 

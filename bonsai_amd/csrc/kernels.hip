@@ -1493,9 +1493,62 @@ __global__ void k_gather_rows(u8 const *__restrict__ plane, u32 Fs, void const *
     }
 }
 
+// Device bin transform (src/cuda/detail/ingest_kernels.cuh:27-50 is the reference's
+// twin of BinMapper::transform, src/bin_mapper.cpp:284-291): bin = lower_bound(cuts, x),
+// NaN -> the last bin.  One thread per (row, padded feature) cell of a chunk of rows; the
+// chunk's float rows are read coalesced, the u8 plane rows are written whole, padding
+// columns get bin 0.  Cut tables live in LDS when they fit (lds_cuts != 0).
+__global__ __launch_bounds__(256) void k_bin_rows(float const *__restrict__ X, u64 n, u32 F, u32 Fs,
+                                                  float const *__restrict__ cuts, u32 const *__restrict__ coff,
+                                                  u8 *__restrict__ plane, u32 lds_cuts)
+{
+    extern __shared__ float s_cuts[];
+    if (lds_cuts) {
+        for (u32 i = threadIdx.x; i < lds_cuts; i += blockDim.x) s_cuts[i] = cuts[i];
+        __syncthreads();
+    }
+    float const *C = lds_cuts ? s_cuts : cuts;
+    u64 const total = n * Fs;
+    for (u64 e = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; e < total;
+         e += static_cast<u64>(gridDim.x) * blockDim.x) {
+        u64 const r = e / Fs;
+        u32 const f = static_cast<u32>(e - r * Fs);
+        u32 b = 0;
+        if (f < F) {
+            float const x = X[r * F + f];
+            u32 const lo = coff[f], hi = coff[f + 1];
+            if (isnan(x)) {
+                b = hi - lo - 1u;
+            } else {
+                u32 first = lo, len = hi - lo;
+                while (len > 0) {
+                    u32 const half = len >> 1;
+                    if (C[first + half] < x) {
+                        first += half + 1u;
+                        len -= half + 1u;
+                    } else {
+                        len = half;
+                    }
+                }
+                b = first - lo;
+            }
+        }
+        plane[e] = static_cast<u8>(b);
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // Launch wrappers (host side).
 // ---------------------------------------------------------------------------------
+void launch_bin_rows(hipStream_t s, float const *X, u64 n, u32 F, u32 Fs, float const *cuts, u32 const *coff,
+                     u32 n_cuts_total, u8 *plane)
+{
+    u32 const lds_cuts = n_cuts_total * 4u <= 48u * 1024u ? n_cuts_total : 0u;
+    u64 const total = n * Fs;
+    u32 const grid = static_cast<u32>(std::min<u64>((total + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_bin_rows, dim3(grid), dim3(256), lds_cuts * 4u, s, X, n, F, Fs, cuts, coff, plane,
+                       lds_cuts);
+}
 void launch_grad_f32(hipStream_t s, float const *score, float const *label, u64 n, int obj, float2 *gh)
 {
     hipLaunchKernelGGL(k_grad_f32, dim3(2048), dim3(256), 0, s, score, label, n, obj, gh);

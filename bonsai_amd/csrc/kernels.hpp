@@ -40,6 +40,8 @@ void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode 
 void launch_iter_advance(hipStream_t s, TreeCtl *ctl);
 void launch_copy_tree(hipStream_t s, DevNode const *tree, TreeCtl const *ctl, DevNode *out_trees, u32 *out_sizes,
                       u32 max_nodes);
+void launch_bin_rows(hipStream_t s, float const *X, u64 n, u32 F, u32 Fs, float const *cuts, u32 const *coff,
+                     u32 n_cuts_total, u8 *plane);
 void launch_gather_rows(hipStream_t s, u8 const *plane, u32 Fs, void const *gh, u32 gh_bytes, u32 const *rows,
                         u64 n, u8 *out_plane, void *out_gh, int gh_is_row_id);
 

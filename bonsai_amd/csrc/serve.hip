@@ -1,12 +1,712 @@
-// Predict / TreeSHAP plans (placeholder until the serving kernels land).
+// Predict / TreeSHAP plans and the continuous-batching serving loop (H15, H16, cfg4).
+//
+// A bh_predictor holds one ensemble resident in HBM in two packed forms:
+//   walk nodes (8 B each) for the bin-space predict walk, and
+//   leaf paths (8 B elements, 12 B heads) for TreeSHAP's closed form,
+// and is immutable after creation, so serving streams share it.
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
 #include "../../include/bonsai_hip.h"
+#include "bh_common.hpp"
+
+using namespace bh;
+
+namespace {
+
+} // namespace
+
+// The engine's thread-local error slot (bh_last_error).
+void bh_set_error(std::string const &m);
+
+namespace {
+
+int sfail(int code, std::string const &m)
+{
+    bh_set_error(m);
+    return code;
+}
+#define SHIP(expr)                                                                          \
+    do {                                                                                    \
+        hipError_t const e_ = (expr);                                                       \
+        if (e_ != hipSuccess) return sfail(-1, std::string("hip: ") + #expr + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Predict walk node: internal {feature, bin, default_left, left, right}; leaf
+// {0xFFFF, value}.  Child ids are local to the tree.
+struct PNode {
+    u16 feature;
+    u8 bin;
+    u8 dl;
+    union {
+        struct {
+            u16 left, right;
+        } c;
+        float value;
+    } u;
+};
+static_assert(sizeof(PNode) == 8, "PNode");
+
+// TreeSHAP packed path element / head (src/shap_paths.cpp:33-110 layout).
+struct SElem {
+    u16 feature; // bit 15: the missing bin satisfies
+    u8 lo, hi;
+    float z;
+};
+struct SHead {
+    u32 first;
+    u16 n;
+    u16 klass;
+    float value;
+};
+static_assert(sizeof(SElem) == 8 && sizeof(SHead) == 12, "shap layout");
+
+constexpr int kMaxPath = 16;
+
+// ---------------------------------------------------------------------------------
+// Predict: init + lr * sum_t leaf_t(row), the tree sum in tree order, multiply then
+// add each rounded (predict_walk_kernel, src/cuda/predict.cu:50-85; host
+// predict_at_binned, include/bonsai/booster.hpp:831-846).  A workgroup takes kPR rows;
+// its threads walk (row, tree) pairs of one tree tile into LDS, then one thread per row
+// adds the tile in tree order.
+// ---------------------------------------------------------------------------------
+constexpr u32 kPR = 16, kPT = 256;
+
+__global__ __launch_bounds__(256) void k_predict(u8 const *__restrict__ bins, u64 n, u64 stride, u32 F,
+                                                 PNode const *__restrict__ nodes, u32 const *__restrict__ roots,
+                                                 u32 n_trees, u16 const *__restrict__ last, float lr, float init,
+                                                 float *__restrict__ out)
+{
+    __shared__ float s_val[kPR][kPT];
+    __shared__ float s_acc[kPR];
+    extern __shared__ __align__(16) unsigned char s_rows[]; // kPR * F bytes
+    u64 const r0 = static_cast<u64>(blockIdx.x) * kPR;
+    if (r0 >= n) return;
+    u32 const nr = static_cast<u32>(min<u64>(kPR, n - r0));
+    for (u32 k = threadIdx.x; k < nr * F; k += blockDim.x) {
+        u32 const rr = k / F, f = k - rr * F;
+        s_rows[k] = bins[(r0 + rr) * stride + f];
+    }
+    if (threadIdx.x < kPR) s_acc[threadIdx.x] = 0.0F;
+    __syncthreads();
+    for (u32 t0 = 0; t0 < n_trees; t0 += kPT) {
+        u32 const tt = min(kPT, n_trees - t0);
+        for (u32 k = threadIdx.x; k < nr * tt; k += blockDim.x) {
+            u32 const rr = k / tt, t = k - rr * tt;
+            PNode const *tn = nodes + roots[t0 + t];
+            u8 const *row = s_rows + rr * F;
+            PNode nd = tn[0];
+            while (nd.feature != 0xFFFFu) {
+                u32 const b = row[nd.feature];
+                bool const left = b == last[nd.feature] ? (nd.dl != 0) : (b <= nd.bin);
+                nd = tn[left ? nd.u.c.left : nd.u.c.right];
+            }
+            s_val[rr][t] = nd.u.value;
+        }
+        __syncthreads();
+        if (threadIdx.x < nr) {
+            float acc = s_acc[threadIdx.x];
+            for (u32 t = 0; t < tt; ++t) acc += s_val[threadIdx.x][t];
+            s_acc[threadIdx.x] = acc;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < nr) out[r0 + threadIdx.x] = __fadd_rn(init, __fmul_rn(lr, s_acc[threadIdx.x]));
+}
+
+// ---------------------------------------------------------------------------------
+// TreeSHAP, packed closed form (eval_shap_paths, src/shap_paths.cpp:196-278):
+//   P(t) = prod_j (z_j + o_j t);  unsatisfied element: -value * sum_i w_i [t^i] P;
+//   satisfied element: value * (1 - z) * sum_i w_i [t^i] (P / (t + z)).
+// A workgroup serves kSR rows with one wave each; lane l takes paths l, l+64, ... and
+// keeps its own phi slice in LDS; slices are then added in lane order (fixed order,
+// no atomics), so a row's result does not depend on what else is in flight.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_shap(u8 const *__restrict__ bins, u64 n, u64 stride, u32 F,
+                                              SElem const *__restrict__ elems, SHead const *__restrict__ heads,
+                                              u32 n_heads, float const *__restrict__ weights,
+                                              u16 const *__restrict__ last, float lr, double bias,
+                                              double *__restrict__ out)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    u32 const rows_per_block = blockDim.x >> 6;
+    double *s_phi = reinterpret_cast<double *>(smem);                      // [rows][64][F]
+    u8 *s_row = reinterpret_cast<u8 *>(s_phi + rows_per_block * 64 * F);   // [rows][F]
+    u32 const wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    u64 const r = static_cast<u64>(blockIdx.x) * rows_per_block + wave;
+    bool const live = r < n;
+    double *my = s_phi + (wave * 64 + lane) * F;
+    for (u32 f = 0; f < F; ++f) my[f] = 0.0;
+    if (live)
+        for (u32 f = lane; f < F; f += 64) s_row[wave * F + f] = bins[r * stride + f];
+    __syncthreads();
+    if (live) {
+        u8 const *row = s_row + wave * F;
+        for (u32 p = lane; p < n_heads; p += 64) {
+            SHead const hd = heads[p];
+            u32 const nel = hd.n;
+            if (nel == 0) continue;
+            float poly[kMaxPath + 1];
+            float z[kMaxPath];
+            bool sat[kMaxPath];
+            u32 feat[kMaxPath];
+#pragma unroll
+            for (int j = 0; j <= kMaxPath; ++j) poly[j] = 0.0F;
+            poly[0] = 1.0F;
+#pragma unroll
+            for (int j = 0; j < kMaxPath; ++j) {
+                if (static_cast<u32>(j) >= nel) break;
+                SElem const e = elems[hd.first + j];
+                u32 const f = e.feature & 0x7FFFu;
+                u32 const b = row[f];
+                bool const one = b == last[f] ? (e.feature & 0x8000u) != 0 : (b >= e.lo && b <= e.hi);
+                sat[j] = one;
+                z[j] = e.z;
+                feat[j] = f;
+                // multiply by (z + o t)
+#pragma unroll
+                for (int i = kMaxPath; i >= 1; --i) {
+                    if (i > j + 1) continue;
+                    poly[i] = z[j] * poly[i] + (one ? poly[i - 1] : 0.0F);
+                }
+                poly[0] *= z[j];
+            }
+            float const *w = weights + (nel - 1) * nel / 2;
+            float wl[kMaxPath];
+#pragma unroll
+            for (int i = 0; i < kMaxPath; ++i) wl[i] = static_cast<u32>(i) < nel ? w[i] : 0.0F;
+            float unsat = 0.0F;
+#pragma unroll
+            for (int i = 0; i < kMaxPath; ++i)
+                if (static_cast<u32>(i) < nel) unsat += wl[i] * poly[i];
+            double const v = hd.value;
+#pragma unroll
+            for (int k = 0; k < kMaxPath; ++k) {
+                if (static_cast<u32>(k) >= nel) break;
+                if (!sat[k]) {
+                    my[feat[k]] -= v * static_cast<double>(unsat);
+                    continue;
+                }
+                // synthetic division by (t + z_k): deflated[n-1] = P[n],
+                // deflated[i] = P[i+1] - z_k deflated[i+1]
+                float defl[kMaxPath];
+                float const zk = z[k];
+                float s = 0.0F;
+                float prev = 0.0F;
+#pragma unroll
+                for (int i = kMaxPath - 1; i >= 0; --i) {
+                    if (static_cast<u32>(i) < nel) {
+                        prev = (static_cast<u32>(i) + 1 == nel) ? poly[i + 1] : poly[i + 1] - zk * prev;
+                        defl[i] = prev;
+                    } else {
+                        defl[i] = 0.0F;
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < kMaxPath; ++i)
+                    if (static_cast<u32>(i) < nel) s += wl[i] * defl[i];
+                my[feat[k]] += v * static_cast<double>(1.0F - zk) * static_cast<double>(s);
+            }
+        }
+    }
+    __syncthreads();
+    if (live) {
+        double *base = s_phi + wave * 64 * F;
+        for (u32 f = lane; f < F; f += 64) {
+            double a = 0.0;
+            for (u32 l = 0; l < 64; ++l) a += base[l * F + f];
+            out[r * (F + 1) + f] = a * lr;
+        }
+        if (lane == 0) out[r * (F + 1) + F] = bias;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Host-side packing (product code): walk nodes and leaf paths from bin-space trees.
+// ---------------------------------------------------------------------------------
+struct ActiveElem {
+    u32 feature;
+    int lo, hi;
+    bool missing_ok;
+    double zero;
+};
+
+// pack_shap_paths (src/shap_paths.cpp:57-163): every root-to-leaf path, repeated
+// features merged into one interval, zero fractions multiplied along the path.
+void pack_paths(bh_node const *t, u16 const *last, u32 id, std::vector<ActiveElem> &act, std::vector<SElem> &el,
+                std::vector<SHead> &hd, size_t &max_len)
+{
+    bh_node const &nd = t[id];
+    if (nd.feature < 0) {
+        SHead h{};
+        h.first = static_cast<u32>(el.size());
+        h.n = static_cast<u16>(act.size());
+        h.value = nd.value;
+        hd.push_back(h);
+        for (auto const &a : act) {
+            int lo = a.lo, hi = a.hi;
+            if (hi < lo) {
+                lo = 1;
+                hi = 0;
+            }
+            SElem e{};
+            e.feature = static_cast<u16>(a.feature | (a.missing_ok ? 0x8000u : 0u));
+            e.lo = static_cast<u8>(lo);
+            e.hi = static_cast<u8>(hi);
+            e.z = static_cast<float>(a.zero);
+            el.push_back(e);
+        }
+        max_len = std::max(max_len, act.size());
+        return;
+    }
+    u32 const f = static_cast<u32>(nd.feature);
+    int const split = static_cast<int>(nd.bin), lb = static_cast<int>(last[f]);
+    double const cover = nd.cover;
+    auto frac = [&](u32 c) { return cover > 0.0 ? static_cast<double>(t[c].cover) / cover : 0.0; };
+    size_t idx = act.size();
+    for (size_t k = 0; k < act.size(); ++k)
+        if (act[k].feature == f) {
+            idx = k;
+            break;
+        }
+    bool const fresh = idx == act.size();
+    if (fresh) act.push_back({f, 0, lb - 1, true, 1.0});
+    ActiveElem const saved = act[idx];
+    act[idx].hi = std::min(saved.hi, split);
+    act[idx].missing_ok = saved.missing_ok && nd.default_left != 0;
+    act[idx].zero = saved.zero * frac(nd.left);
+    pack_paths(t, last, nd.left, act, el, hd, max_len);
+    act[idx].lo = std::max(saved.lo, split + 1);
+    act[idx].hi = saved.hi;
+    act[idx].missing_ok = saved.missing_ok && nd.default_left == 0;
+    act[idx].zero = saved.zero * frac(nd.right);
+    pack_paths(t, last, nd.right, act, el, hd, max_len);
+    if (fresh) act.pop_back();
+    else act[idx] = saved;
+}
+
+// tree_expected_value (src/shap.cpp:246-292): cover-weighted mean of the leaves.
+double expected_value(bh_node const *t, u32 id)
+{
+    bh_node const &nd = t[id];
+    if (nd.feature < 0) return nd.value;
+    double const cover = nd.cover;
+    if (cover <= 0.0) return 0.0;
+    return (static_cast<double>(t[nd.left].cover) * expected_value(t, nd.left) +
+            static_cast<double>(t[nd.right].cover) * expected_value(t, nd.right)) /
+           cover;
+}
+
+} // namespace
+
+struct bh_predictor {
+    int device = 0;
+    u32 n_trees = 0, F = 0;
+    float lr = 0.0F, init = 0.0F;
+    double shap_bias = 0.0;
+    u32 n_heads = 0, max_len = 0;
+    PNode *d_nodes = nullptr;
+    u32 *d_roots = nullptr;
+    u16 *d_last = nullptr;
+    SElem *d_elems = nullptr;
+    SHead *d_heads = nullptr;
+    float *d_w = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+
+int launch_predict(bh_predictor *p, u8 const *d_bins, u64 n, u64 stride, float *d_out, hipStream_t s)
+{
+    if (n == 0) return 0;
+    u32 const blocks = static_cast<u32>((n + kPR - 1) / kPR);
+    size_t const lds = static_cast<size_t>(kPR) * p->F;
+    hipLaunchKernelGGL(k_predict, dim3(blocks), dim3(256), lds, s, d_bins, n, stride, p->F, p->d_nodes, p->d_roots,
+                       p->n_trees, p->d_last, p->lr, p->init, d_out);
+    SHIP(hipGetLastError());
+    return 0;
+}
+
+int launch_shap(bh_predictor *p, u8 const *d_bins, u64 n, u64 stride, double *d_out, hipStream_t s)
+{
+    if (n == 0) return 0;
+    // One wave per row; as many rows per workgroup (<= 4) as the per-lane phi slices fit.
+    size_t const per_row = static_cast<size_t>(64) * p->F * sizeof(double) + p->F;
+    u32 const sr = static_cast<u32>(std::min<size_t>(4, (64 * 1024) / per_row));
+    if (sr == 0) return sfail(1, "TreeSHAP: more than 127 features do not fit the per-lane LDS slices");
+    size_t const lds = sr * per_row;
+    u32 const blocks = static_cast<u32>((n + sr - 1) / sr);
+    hipLaunchKernelGGL(k_shap, dim3(blocks), dim3(64 * sr), lds, s, d_bins, n, stride, p->F, p->d_elems, p->d_heads,
+                       p->n_heads, p->d_w, p->d_last, p->lr, p->shap_bias, d_out);
+    SHIP(hipGetLastError());
+    return 0;
+}
+
+} // namespace
+
 extern "C" {
-int bh_predictor_create(int, const bh_node *, const uint32_t *, uint32_t, const uint16_t *, uint32_t, float, float,
-                        bh_predictor **out) { *out = nullptr; return 1; }
-int bh_predictor_destroy(bh_predictor *) { return 0; }
-int bh_predict(bh_predictor *, const uint8_t *, uint64_t, uint64_t, float *) { return 1; }
-int bh_shap(bh_predictor *, const uint8_t *, uint64_t, uint64_t, double *) { return 1; }
-int bh_predict_device(bh_predictor *, const uint8_t *, uint64_t, uint64_t, float *, void *) { return 1; }
-int bh_shap_device(bh_predictor *, const uint8_t *, uint64_t, uint64_t, double *, void *) { return 1; }
+
+int bh_predictor_create(int device, const bh_node *nodes, const uint32_t *tree_offsets, uint32_t n_trees,
+                        const uint16_t *last_bin, uint32_t n_features, float learning_rate, float init_score,
+                        bh_predictor **out)
+{
+    *out = nullptr;
+    if (n_features == 0) return sfail(1, "no features");
+    SHIP(hipSetDevice(device));
+    auto *p = new bh_predictor();
+    p->device = device;
+    p->n_trees = n_trees;
+    p->F = n_features;
+    p->lr = learning_rate;
+    p->init = init_score;
+    std::vector<PNode> pn;
+    std::vector<u32> roots(n_trees);
+    std::vector<SElem> el;
+    std::vector<SHead> hd;
+    size_t max_len = 0;
+    double ev_sum = 0.0;
+    std::vector<ActiveElem> act;
+    for (u32 t = 0; t < n_trees; ++t) {
+        bh_node const *tn = nodes + tree_offsets[t];
+        u32 const end = t + 1 < n_trees ? tree_offsets[t + 1] : 0;
+        // node count: walk reachable ids (covers the whole array for grown trees)
+        u32 count = 0;
+        std::vector<u32> stack{0};
+        while (!stack.empty()) {
+            u32 const i = stack.back();
+            stack.pop_back();
+            count = std::max(count, i + 1);
+            if (tn[i].feature >= 0) {
+                if (static_cast<u32>(tn[i].feature) >= n_features || tn[i].bin > 255) {
+                    delete p;
+                    return sfail(1, "tree node out of range (feature id or bin)");
+                }
+                stack.push_back(tn[i].left);
+                stack.push_back(tn[i].right);
+            }
+        }
+        (void)end;
+        if (count > 65535) {
+            delete p;
+            return sfail(1, "tree larger than 65535 nodes");
+        }
+        roots[t] = static_cast<u32>(pn.size());
+        for (u32 i = 0; i < count; ++i) {
+            PNode q{};
+            if (tn[i].feature < 0) {
+                q.feature = 0xFFFFu;
+                q.u.value = tn[i].value;
+            } else {
+                q.feature = static_cast<u16>(tn[i].feature);
+                q.bin = static_cast<u8>(tn[i].bin);
+                q.dl = static_cast<u8>(tn[i].default_left);
+                q.u.c.left = static_cast<u16>(tn[i].left);
+                q.u.c.right = static_cast<u16>(tn[i].right);
+            }
+            pn.push_back(q);
+        }
+        act.clear();
+        pack_paths(tn, last_bin, 0, act, el, hd, max_len);
+        ev_sum += expected_value(tn, 0);
+    }
+    if (max_len > static_cast<size_t>(kMaxPath)) {
+        delete p;
+        return sfail(1, "TreeSHAP paths longer than 16 features are not supported");
+    }
+    // bias column: sum of per-tree expected values, times lr, plus init
+    // (include/bonsai/booster.hpp:902-925).
+    p->shap_bias = ev_sum * static_cast<double>(learning_rate) + static_cast<double>(init_score);
+    p->n_heads = static_cast<u32>(hd.size());
+    p->max_len = static_cast<u32>(max_len);
+    // Permutation weights w_i = 1 / (n C(n-1, i)) (shap_path_weights, src/shap_paths.cpp:165-194).
+    std::vector<float> w(kMaxPath * (kMaxPath + 1) / 2, 0.0F);
+    {
+        std::vector<double> binom(kMaxPath, 0.0);
+        binom[0] = 1.0;
+        for (size_t nn = 1; nn <= static_cast<size_t>(kMaxPath); ++nn) {
+            size_t const rr = nn - 1;
+            if (rr > 0) {
+                binom[rr] = 0.0;
+                for (size_t i = rr + 1; i-- > 1;) binom[i] += binom[i - 1];
+            }
+            for (size_t i = 0; i < nn; ++i)
+                w[rr * nn / 2 + i] = static_cast<float>(1.0 / (static_cast<double>(nn) * binom[i]));
+        }
+    }
+    hipError_t e = hipSuccess;
+    auto al = [&](void **ptr, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(ptr, std::max<size_t>(bytes, 16));
+    };
+    al(reinterpret_cast<void **>(&p->d_nodes), pn.size() * sizeof(PNode));
+    al(reinterpret_cast<void **>(&p->d_roots), roots.size() * sizeof(u32));
+    al(reinterpret_cast<void **>(&p->d_last), n_features * sizeof(u16));
+    al(reinterpret_cast<void **>(&p->d_elems), el.size() * sizeof(SElem));
+    al(reinterpret_cast<void **>(&p->d_heads), hd.size() * sizeof(SHead));
+    al(reinterpret_cast<void **>(&p->d_w), w.size() * sizeof(float));
+    if (e == hipSuccess && !pn.empty()) e = hipMemcpy(p->d_nodes, pn.data(), pn.size() * sizeof(PNode), hipMemcpyHostToDevice);
+    if (e == hipSuccess && n_trees) e = hipMemcpy(p->d_roots, roots.data(), roots.size() * sizeof(u32), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_last, last_bin, n_features * sizeof(u16), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !el.empty()) e = hipMemcpy(p->d_elems, el.data(), el.size() * sizeof(SElem), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !hd.empty()) e = hipMemcpy(p->d_heads, hd.data(), hd.size() * sizeof(SHead), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_w, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        bh_predictor_destroy(p);
+        return sfail(-1, std::string("hip: predictor upload: ") + hipGetErrorString(e));
+    }
+    *out = p;
+    return 0;
+}
+
+int bh_predictor_destroy(bh_predictor *p)
+{
+    if (!p) return 0;
+    (void)hipSetDevice(p->device);
+    void *ptrs[] = {p->d_nodes, p->d_roots, p->d_last, p->d_elems, p->d_heads, p->d_w};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return 0;
+}
+
+int bh_predict_device(bh_predictor *p, const uint8_t *d_bins, uint64_t n, uint64_t row_stride, float *d_out,
+                      void *hip_stream)
+{
+    if (!p) return sfail(1, "null predictor");
+    if (row_stride < p->F) return sfail(1, "row_stride < n_features");
+    return launch_predict(p, d_bins, n, row_stride, d_out, static_cast<hipStream_t>(hip_stream));
+}
+
+int bh_shap_device(bh_predictor *p, const uint8_t *d_bins, uint64_t n, uint64_t row_stride, double *d_out,
+                   void *hip_stream)
+{
+    if (!p) return sfail(1, "null predictor");
+    if (row_stride < p->F) return sfail(1, "row_stride < n_features");
+    return launch_shap(p, d_bins, n, row_stride, d_out, static_cast<hipStream_t>(hip_stream));
+}
+
+int bh_predict(bh_predictor *p, const uint8_t *bins, uint64_t n, uint64_t row_stride, float *out)
+{
+    if (!p) return sfail(1, "null predictor");
+    if (row_stride < p->F) return sfail(1, "row_stride < n_features");
+    if (n == 0) return 0;
+    SHIP(hipSetDevice(p->device));
+    u8 *d_bins = nullptr;
+    float *d_out = nullptr;
+    SHIP(hipMalloc(&d_bins, n * row_stride));
+    SHIP(hipMalloc(&d_out, n * sizeof(float)));
+    SHIP(hipMemcpy(d_bins, bins, n * row_stride, hipMemcpyHostToDevice));
+    int rc = launch_predict(p, d_bins, n, row_stride, d_out, p->stream);
+    if (rc == 0) {
+        hipError_t e = hipStreamSynchronize(p->stream);
+        if (e == hipSuccess) e = hipMemcpy(out, d_out, n * sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = sfail(-1, std::string("hip: predict: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(d_bins);
+    (void)hipFree(d_out);
+    return rc;
+}
+
+int bh_shap(bh_predictor *p, const uint8_t *bins, uint64_t n, uint64_t row_stride, double *out)
+{
+    if (!p) return sfail(1, "null predictor");
+    if (row_stride < p->F) return sfail(1, "row_stride < n_features");
+    if (n == 0) return 0;
+    SHIP(hipSetDevice(p->device));
+    u8 *d_bins = nullptr;
+    double *d_out = nullptr;
+    SHIP(hipMalloc(&d_bins, n * row_stride));
+    SHIP(hipMalloc(&d_out, n * (p->F + 1) * sizeof(double)));
+    SHIP(hipMemcpy(d_bins, bins, n * row_stride, hipMemcpyHostToDevice));
+    int rc = launch_shap(p, d_bins, n, row_stride, d_out, p->stream);
+    if (rc == 0) {
+        hipError_t e = hipStreamSynchronize(p->stream);
+        if (e == hipSuccess) e = hipMemcpy(out, d_out, n * (p->F + 1) * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = sfail(-1, std::string("hip: shap: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(d_bins);
+    (void)hipFree(d_out);
+    return rc;
+}
+
+} // extern "C"
+
+
+// ---------------------------------------------------------------------------------
+// Continuous-batching serving loop (cfg4).  Requests arrive on a clock; whenever an
+// execution slot (one HIP stream with its pinned staging and device buffers) is free,
+// every request that has arrived and still fits the slot's row capacity and the HBM
+// budget is packed into one batch: rows H2D, predict (+ TreeSHAP), results D2H, all
+// asynchronous on the slot's stream, so batches on different slots overlap on the
+// device.  Each row's result depends only on that row (per-row kernels, fixed-order
+// reductions), so a request's output is identical alone and in any batch.
+// ---------------------------------------------------------------------------------
+namespace {
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    u8 *h_rows = nullptr, *d_rows = nullptr;
+    float *h_pred = nullptr, *d_pred = nullptr;
+    double *h_shap = nullptr, *d_shap = nullptr;
+    std::vector<u32> reqs; // requests in the batch
+    u64 rows = 0;
+    u64 bytes = 0;
+    bool busy = false;
+};
+
+} // namespace
+
+extern "C" int bh_serve(bh_predictor *p, const uint8_t *rows, uint64_t row_stride, const uint64_t *req_offsets,
+                        const double *arrival_s, uint32_t n_req, const bh_serve_config *cfg, float *pred_out,
+                        double *shap_out, double *latency_ms, bh_serve_stats *stats)
+{
+    if (!p) return sfail(1, "null predictor");
+    if (row_stride < p->F) return sfail(1, "row_stride < n_features");
+    if (cfg->n_slots == 0 || cfg->max_batch_rows == 0) return sfail(1, "n_slots and max_batch_rows must be > 0");
+    SHIP(hipSetDevice(p->device));
+    u32 const F = p->F;
+    bool const shap = cfg->do_shap != 0;
+    u64 const per_row = row_stride + sizeof(float) + (shap ? (F + 1) * sizeof(double) : 0);
+    for (u32 i = 0; i < n_req; ++i)
+        if (req_offsets[i + 1] - req_offsets[i] > cfg->max_batch_rows)
+            return sfail(1, "a request is larger than max_batch_rows");
+    std::vector<Slot> slots(cfg->n_slots);
+    u64 const cap = cfg->max_batch_rows;
+    hipError_t e = hipSuccess;
+    for (auto &s : slots) {
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&s.h_rows), cap * row_stride, 0);
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&s.h_pred), cap * sizeof(float), 0);
+        if (e == hipSuccess) e = hipMalloc(&s.d_rows, cap * row_stride);
+        if (e == hipSuccess) e = hipMalloc(&s.d_pred, cap * sizeof(float));
+        if (shap) {
+            if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&s.h_shap), cap * (F + 1) * sizeof(double), 0);
+            if (e == hipSuccess) e = hipMalloc(&s.d_shap, cap * (F + 1) * sizeof(double));
+        }
+    }
+    auto cleanup = [&]() {
+        for (auto &s : slots) {
+            if (s.stream) (void)hipStreamSynchronize(s.stream);
+            if (s.h_rows) (void)hipHostFree(s.h_rows);
+            if (s.h_pred) (void)hipHostFree(s.h_pred);
+            if (s.h_shap) (void)hipHostFree(s.h_shap);
+            if (s.d_rows) (void)hipFree(s.d_rows);
+            if (s.d_pred) (void)hipFree(s.d_pred);
+            if (s.d_shap) (void)hipFree(s.d_shap);
+            if (s.done) (void)hipEventDestroy(s.done);
+            if (s.stream) (void)hipStreamDestroy(s.stream);
+        }
+    };
+    if (e != hipSuccess) {
+        cleanup();
+        return sfail(-1, std::string("hip: serve buffers: ") + hipGetErrorString(e));
+    }
+    using clk = std::chrono::steady_clock;
+    auto const t0 = clk::now();
+    auto now_s = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
+    u32 next = 0, completed = 0, inflight = 0, max_inflight = 0;
+    u64 inflight_bytes = 0, total_rows = 0;
+    int rc = 0;
+    while (completed < n_req && rc == 0) {
+        bool progressed = false;
+        // completions
+        for (auto &s : slots) {
+            if (!s.busy) continue;
+            hipError_t const q = hipEventQuery(s.done);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) {
+                rc = sfail(-1, std::string("hip: serve batch: ") + hipGetErrorString(q));
+                break;
+            }
+            double const t_done = now_s();
+            u64 pos = 0;
+            for (u32 r : s.reqs) {
+                u64 const a = req_offsets[r], nr = req_offsets[r + 1] - a;
+                std::memcpy(pred_out + a, s.h_pred + pos, nr * sizeof(float));
+                if (shap && shap_out)
+                    std::memcpy(shap_out + a * (F + 1), s.h_shap + pos * (F + 1), nr * (F + 1) * sizeof(double));
+                latency_ms[r] = (t_done - arrival_s[r]) * 1000.0;
+                pos += nr;
+                ++completed;
+            }
+            inflight -= static_cast<u32>(s.reqs.size());
+            inflight_bytes -= s.bytes;
+            s.busy = false;
+            s.reqs.clear();
+            progressed = true;
+        }
+        if (rc) break;
+        // admissions: fill free slots with arrived requests
+        double const t = now_s();
+        for (auto &s : slots) {
+            if (s.busy || next >= n_req || arrival_s[next] > t) continue;
+            u64 rows_in = 0, bytes_in = 0;
+            while (next < n_req && arrival_s[next] <= t) {
+                u64 const nr = req_offsets[next + 1] - req_offsets[next];
+                u64 const nb = nr * per_row;
+                bool const first = s.reqs.empty();
+                if (!first && (rows_in + nr > cap || inflight_bytes + bytes_in + nb > cfg->hbm_budget)) break;
+                if (first && inflight > 0 && inflight_bytes + nb > cfg->hbm_budget) break; // wait for memory
+                std::memcpy(s.h_rows + rows_in * row_stride, rows + req_offsets[next] * row_stride, nr * row_stride);
+                s.reqs.push_back(next);
+                rows_in += nr;
+                bytes_in += nb;
+                ++next;
+            }
+            if (s.reqs.empty()) continue;
+            s.rows = rows_in;
+            s.bytes = bytes_in;
+            inflight += static_cast<u32>(s.reqs.size());
+            inflight_bytes += bytes_in;
+            max_inflight = std::max(max_inflight, inflight);
+            total_rows += rows_in;
+            e = hipMemcpyAsync(s.d_rows, s.h_rows, rows_in * row_stride, hipMemcpyHostToDevice, s.stream);
+            if (e == hipSuccess) {
+                rc = launch_predict(p, s.d_rows, rows_in, row_stride, s.d_pred, s.stream);
+                if (rc == 0 && shap) rc = launch_shap(p, s.d_rows, rows_in, row_stride, s.d_shap, s.stream);
+            }
+            if (rc) break;
+            if (e == hipSuccess) e = hipMemcpyAsync(s.h_pred, s.d_pred, rows_in * sizeof(float), hipMemcpyDeviceToHost, s.stream);
+            if (e == hipSuccess && shap)
+                e = hipMemcpyAsync(s.h_shap, s.d_shap, rows_in * (F + 1) * sizeof(double), hipMemcpyDeviceToHost, s.stream);
+            if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
+            if (e != hipSuccess) {
+                rc = sfail(-1, std::string("hip: serve enqueue: ") + hipGetErrorString(e));
+                break;
+            }
+            s.busy = true;
+            progressed = true;
+        }
+        if (!progressed) std::this_thread::yield();
+    }
+    double const wall = now_s();
+    cleanup();
+    if (rc) return rc;
+    std::vector<double> lat(latency_ms, latency_ms + n_req);
+    std::sort(lat.begin(), lat.end());
+    auto pct = [&](double q) {
+        if (lat.empty()) return 0.0;
+        size_t const k = std::min(lat.size() - 1, static_cast<size_t>(std::ceil(q * lat.size())) - 1);
+        return lat[k];
+    };
+    double mean = 0.0;
+    for (double v : lat) mean += v;
+    stats->wall_s = wall;
+    stats->rows = total_rows;
+    stats->requests = n_req;
+    stats->rows_per_s = wall > 0 ? static_cast<double>(total_rows) / wall : 0.0;
+    stats->p50_ms = pct(0.50);
+    stats->p99_ms = pct(0.99);
+    stats->mean_ms = n_req ? mean / n_req : 0.0;
+    stats->max_inflight = max_inflight;
+    return 0;
 }

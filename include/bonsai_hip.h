@@ -79,6 +79,8 @@ typedef struct bh_split {
 /* ---- devices and errors ---- */
 /* cuda_device_count / cuda_available (include/bonsai/cuda/histogram_engine.hpp:36-41). */
 int         bh_device_count(int *count);
+/* Block until every operation queued on the context's stream has finished. */
+int         bh_synchronize(bh_context *ctx);
 /* Message of the last failure on this thread. */
 const char *bh_last_error(void);
 
@@ -93,6 +95,15 @@ int bh_destroy(bh_context *ctx);
 int bh_upload_bins(bh_context *ctx, const uint8_t *bins, uint64_t n_rows, uint32_t n_features,
                    uint64_t row_stride, const uint16_t *n_bins);
 /* Labels of the resident objective (resident_begin, device_context.cu:1541-1579). */
+/* Device bin transform (the reference's transform_bin, src/cuda/detail/ingest_kernels.cuh:27-50,
+ * twin of BinMapper::transform, src/bin_mapper.cpp:284-291): X is row-major float
+ * [n_rows][n_features]; feature f's ascending cuts are cuts[cut_offsets[f] .. cut_offsets[f+1]),
+ * n_bins[f] = cut count (<= 256).  bin = lower_bound(cuts, x), NaN -> last bin.  Replaces a
+ * previous plane like bh_upload_bins. */
+int bh_upload_features(bh_context *ctx, const float *X, uint64_t n_rows, uint32_t n_features,
+                       const float *cuts, const uint32_t *cut_offsets);
+/* Copy the resident pristine bin plane back (row-major, out_stride bytes per row). */
+int bh_download_bins(bh_context *ctx, uint8_t *out, uint64_t out_stride);
 int bh_upload_labels(bh_context *ctx, const float *labels);
 /* Configure the fit; allocates every device buffer the fit needs (the reference's
  * capacity gates refuse_hist_budget / refuse_leaf_pool, device_context.cu:60-87). */
@@ -148,6 +159,12 @@ int bh_partition(bh_context *ctx, const uint32_t *rows, uint64_t n, uint32_t fea
 int bh_time_root_histogram(bh_context *ctx, uint32_t reps, float *ms_mean);
 /* Mean per-iteration device time split (ms) from hipEvents recorded by the last
  * bh_train call when timing was enabled: [0] whole iteration. */
+/* Run n_iters depthwise iterations without a graph, timing every histogram launch with
+ * hipEvents on the engine stream; totals of kernel time (ms), algorithmic bytes
+ * (rows x (features + gradient-pair bytes) + 8 bytes per built (node, feature, bin) cell), launches and
+ * rows.  Trees are appended like bh_train. */
+int bh_profile_hist(bh_context *ctx, uint32_t n_iters, double *ms_total, double *bytes_total,
+                    uint64_t *launches, uint64_t *rows_total);
 int bh_set_timing(bh_context *ctx, int32_t enable);
 int bh_get_timing(bh_context *ctx, float *ms, uint32_t n);
 
@@ -173,6 +190,35 @@ int bh_predict_device(bh_predictor *p, const uint8_t *d_bins, uint64_t n, uint64
                       float *d_out, void *hip_stream);
 int bh_shap_device(bh_predictor *p, const uint8_t *d_bins, uint64_t n, uint64_t row_stride,
                    double *d_out, void *hip_stream);
+
+/* ---- serving: continuous batching over a resident ensemble (no counterpart in the
+ *      reference, whose Python binding serves one call at a time through a plan cache,
+ *      src/python/module.cpp:603-640) ---- */
+typedef struct bh_serve_config {
+    uint32_t n_slots;        /* batches in flight at once (one HIP stream each) */
+    uint32_t do_shap;        /* 1: TreeSHAP contributions as well as predictions */
+    uint64_t max_batch_rows; /* row capacity of one batch */
+    uint64_t hbm_budget;     /* bytes of request data (rows + outputs) in flight */
+} bh_serve_config;
+
+typedef struct bh_serve_stats {
+    double   wall_s;
+    double   rows_per_s;
+    double   p50_ms, p99_ms, mean_ms;
+    uint64_t rows;
+    uint32_t requests;
+    uint32_t max_inflight;
+} bh_serve_stats;
+
+/* Serve n_req requests whose rows are rows[req_offsets[i] .. req_offsets[i+1]) and
+ * which arrive at arrival_s[i] seconds after the call starts (ascending).  Requests
+ * that have arrived are packed into a batch whenever a slot frees, within the row
+ * capacity and the HBM budget.  Outputs land at each request's own rows;
+ * latency_ms[i] = completion - arrival. */
+int bh_serve(bh_predictor *p, const uint8_t *rows, uint64_t row_stride,
+             const uint64_t *req_offsets, const double *arrival_s, uint32_t n_req,
+             const bh_serve_config *cfg, float *pred_out, double *shap_out,
+             double *latency_ms, bh_serve_stats *stats);
 
 #ifdef __cplusplus
 }
