@@ -110,6 +110,7 @@ class FitConfig:
 
 
 _lib = None
+_SIGS: dict = {}     # bound C-ABI functions -> ctypes argument types
 
 
 def lib() -> C.CDLL:
@@ -160,6 +161,8 @@ def lib() -> C.CDLL:
         fn.restype = C.c_int
     L.bh_last_error.restype = C.c_char_p
     L.bh_last_error.argtypes = []
+    _SIGS.update(sig)
+    _SIGS["bh_last_error"] = []
     _lib = L
     return L
 

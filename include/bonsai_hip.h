@@ -94,7 +94,6 @@ int bh_destroy(bh_context *ctx);
  * (src/cuda/detail/device_context.cu:499-571, src/cuda/histogram_engine.cu:381). */
 int bh_upload_bins(bh_context *ctx, const uint8_t *bins, uint64_t n_rows, uint32_t n_features,
                    uint64_t row_stride, const uint16_t *n_bins);
-/* Labels of the resident objective (resident_begin, device_context.cu:1541-1579). */
 /* Device bin transform (the reference's transform_bin, src/cuda/detail/ingest_kernels.cuh:27-50,
  * twin of BinMapper::transform, src/bin_mapper.cpp:284-291): X is row-major float
  * [n_rows][n_features]; feature f's ascending cuts are cuts[cut_offsets[f] .. cut_offsets[f+1]),
@@ -104,6 +103,7 @@ int bh_upload_features(bh_context *ctx, const float *X, uint64_t n_rows, uint32_
                        const float *cuts, const uint32_t *cut_offsets);
 /* Copy the resident pristine bin plane back (row-major, out_stride bytes per row). */
 int bh_download_bins(bh_context *ctx, uint8_t *out, uint64_t out_stride);
+/* Labels of the resident objective (resident_begin, device_context.cu:1541-1579). */
 int bh_upload_labels(bh_context *ctx, const float *labels);
 /* Configure the fit; allocates every device buffer the fit needs (the reference's
  * capacity gates refuse_hist_budget / refuse_leaf_pool, device_context.cu:60-87). */
@@ -157,14 +157,15 @@ int bh_partition(bh_context *ctx, const uint32_t *rows, uint64_t n, uint32_t fea
  * times and returns the mean kernel time (ms) measured with hipEvents on the kernel's
  * own stream. */
 int bh_time_root_histogram(bh_context *ctx, uint32_t reps, float *ms_mean);
-/* Mean per-iteration device time split (ms) from hipEvents recorded by the last
- * bh_train call when timing was enabled: [0] whole iteration. */
 /* Run n_iters depthwise iterations without a graph, timing every histogram launch with
  * hipEvents on the engine stream; totals of kernel time (ms), algorithmic bytes
  * (rows x (features + gradient-pair bytes) + 8 bytes per built (node, feature, bin) cell), launches and
  * rows.  Trees are appended like bh_train. */
 int bh_profile_hist(bh_context *ctx, uint32_t n_iters, double *ms_total, double *bytes_total,
                     uint64_t *launches, uint64_t *rows_total);
+/* Per-iteration timing of bh_train (the reference's lap profile, src/cuda/detail/profile.cuh:39-40):
+ * when enabled, hipEvents bracket every iteration; bh_get_timing writes ms[i] = device
+ * time of iteration i of the last bh_train call (0 beyond its count). */
 int bh_set_timing(bh_context *ctx, int32_t enable);
 int bh_get_timing(bh_context *ctx, float *ms, uint32_t n);
 
