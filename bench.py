@@ -123,6 +123,8 @@ def main() -> int:
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=4_000_000)
     ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the 1 GiB calibration copy kernel (for HBM counter scaling) and exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -133,6 +135,13 @@ def main() -> int:
         print("bench.py needs a HIP device", file=sys.stderr)
         return 1
     from bonsai_amd import binning
+
+    copy_ms = B.copy_calibrate(1 << 30, 20)
+    copy_gbps = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
+    log(f"calibration copy of 1 GiB: {copy_ms * 1e3:.1f} us, {copy_gbps:.0f} GB/s")
+    if a.copy_calibration:
+        print(json.dumps({"copy_bytes": 1 << 30, "copy_ms": copy_ms, "copy_GBps": round(copy_gbps, 1)}))
+        return 0
 
     X, y, Xh, yh, cuts = make_data(a.rows, a.held_out)
     nb = binning.n_bins_of(cuts)
@@ -199,6 +208,7 @@ def main() -> int:
             "frac": round(hist_gbps / HBM_PEAK_GBPS, 4),
             "bytes_per_launch": round(per_launch_bytes), "avg_launch_us": round(per_launch_us, 2),
             "launches_timed": prof["launches"],
+            "copy_GBps": round(copy_gbps, 1),
             "traffic": None,
         },
         "cpu_baseline": cpu,

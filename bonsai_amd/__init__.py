@@ -125,6 +125,7 @@ def lib() -> C.CDLL:
     sig = {
         "bh_device_count": [C.POINTER(C.c_int)],
         "bh_synchronize": [P],
+        "bh_copy_calibrate": [C.c_int, u64, u32, C.POINTER(C.c_float)],
         "bh_create": [C.c_int, C.POINTER(P)],
         "bh_destroy": [P],
         "bh_upload_bins": [P, P, u64, u32, u64, P],
@@ -178,6 +179,13 @@ def _check(rc: int) -> None:
 
 def _ptr(a: np.ndarray | None):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def copy_calibrate(nbytes: int = 1 << 30, reps: int = 20, device: int = 0) -> float:
+    """Mean time (ms) of a plain device copy of nbytes; bandwidth = 2 * nbytes / time."""
+    ms = C.c_float(0)
+    _check(lib().bh_copy_calibrate(device, nbytes, reps, C.byref(ms)))
+    return ms.value
 
 
 def synchronize(ctx: "HipContext") -> None:

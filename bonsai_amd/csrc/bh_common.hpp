@@ -26,6 +26,12 @@ constexpr u32 kHistChunk  = 16384; // rows per histogram work unit (fixed: part 
                                    // FP32 summation order)
 constexpr u32 kPartChunk  = 4096;  // rows per partition work unit
 constexpr u32 kMaxTreeNodes = 8192;
+// Tail padding of planes (bytes) and gradient buffers (elements): kernels that read
+// aligned 16-byte words around a row range may touch up to 15 bytes past its end.
+constexpr size_t kTailPadBytes = 64;
+// Split find: up to this many workgroups share one node's feature tiles.
+constexpr u32 kFindMaxBlocks = 8;
+constexpr u32 kTailPad = 16;
 
 // Tree node in bin space, the layout the host reads back (feature < 0: leaf).
 struct DevNode {
@@ -125,6 +131,13 @@ struct TreeCtl {
     u32 pad[15];
 };
 static_assert(sizeof(TreeCtl) == 128, "TreeCtl");
+
+// Per-node partial split records and completion counters of a multi-block split find
+// (capacity: front indices x kFindMaxBlocks records; counters start and end at 0).
+struct FindScratch {
+    SplitRec *part;
+    u32 *done;
+};
 
 // Tree-growth settings shared by every kernel of a fit.
 struct GrowCfg {

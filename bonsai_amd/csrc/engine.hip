@@ -72,6 +72,7 @@ struct bh_context {
     TreeCtl *ctl = nullptr;
     NodeInfo *front[2] = {nullptr, nullptr};
     SplitRec *splits = nullptr;
+    FindScratch fscr{nullptr, nullptr};
     DevNode *tree = nullptr;
     PartOp *ops = nullptr;
     PartChunk *pch = nullptr;
@@ -126,7 +127,8 @@ void free_fit(bh_context *c)
     c->graph = nullptr;
     void *ptrs[] = {c->gh0f, c->ghwf[0], c->ghwf[1], c->gh0q, c->ghwq[0], c->ghwq[1], c->maxbits, c->scales,
                     c->hist[0], c->hist[1], c->partials, c->ctl, c->front[0], c->front[1], c->splits, c->tree,
-                    c->ops, c->pch, c->chunk_left, c->hch, c->fops, c->trees_out, c->tree_sizes, c->leaf_rows};
+                    c->ops, c->pch, c->chunk_left, c->hch, c->fops, c->trees_out, c->tree_sizes, c->leaf_rows,
+                    c->fscr.part, c->fscr.done};
     for (void *p : ptrs)
         if (p) hipFree(p);
     c->gh0f = c->ghwf[0] = c->ghwf[1] = nullptr;
@@ -146,6 +148,7 @@ void free_fit(bh_context *c)
     c->trees_out = nullptr;
     c->tree_sizes = nullptr;
     c->leaf_rows = nullptr;
+    c->fscr = FindScratch{nullptr, nullptr};
     ext_free(c->ext);
     c->configured = false;
     c->trees_cap = c->trees_done = 0;
@@ -175,13 +178,13 @@ int alloc_fit(bh_context *c)
     size_t const nc = c->node_cells();
     size_t const hist_bytes = static_cast<size_t>(c->hist_slots) * nc * c->cell_bytes();
     if (c->quant) {
-        HIP_TRY(dalloc(&c->gh0q, N));
-        HIP_TRY(dalloc(&c->ghwq[0], N));
-        HIP_TRY(dalloc(&c->ghwq[1], N));
+        HIP_TRY(dalloc(&c->gh0q, N + kTailPad));
+        HIP_TRY(dalloc(&c->ghwq[0], N + kTailPad));
+        HIP_TRY(dalloc(&c->ghwq[1], N + kTailPad));
     } else {
-        HIP_TRY(dalloc(&c->gh0f, N));
-        HIP_TRY(dalloc(&c->ghwf[0], N));
-        HIP_TRY(dalloc(&c->ghwf[1], N));
+        HIP_TRY(dalloc(&c->gh0f, N + kTailPad));
+        HIP_TRY(dalloc(&c->ghwf[0], N + kTailPad));
+        HIP_TRY(dalloc(&c->ghwf[1], N + kTailPad));
     }
     HIP_TRY(dalloc(&c->maxbits, 4));
     HIP_TRY(dalloc(&c->scales, 4));
@@ -195,6 +198,9 @@ int alloc_fit(bh_context *c)
     HIP_TRY(dalloc(&c->front[0], fcap));
     HIP_TRY(dalloc(&c->front[1], fcap));
     HIP_TRY(dalloc(&c->splits, fcap));
+    HIP_TRY(dalloc(&c->fscr.part, static_cast<size_t>(fcap) * kFindMaxBlocks));
+    HIP_TRY(dalloc(&c->fscr.done, fcap));
+    HIP_TRY(hipMemset(c->fscr.done, 0, fcap * sizeof(u32)));
     HIP_TRY(dalloc(&c->tree, c->max_nodes));
     HIP_TRY(dalloc(&c->ops, c->max_ops + 2));
     HIP_TRY(dalloc(&c->pch, c->max_pch));
@@ -323,7 +329,8 @@ void enqueue_depthwise(bh_context *c)
         void *hcur = c->hist[d & 1u], *hnext = c->hist[(d + 1) & 1u];
         u32 const width = 1u << std::min(d, 14u);
         bool const last = d + 1 == D;
-        launch_find(s, c->quant, cur, c->ctl, hcur, c->d_nbins, c->F, c->SB, c->gc, c->scales, c->splits, width, 0, 0);
+        launch_find(s, c->quant, cur, c->ctl, hcur, c->d_nbins, c->F, c->SB, c->gc, c->scales, c->splits, width, 0, 0,
+                    c->fscr);
         launch_plan_level(s, cur, c->splits, c->ctl, c->tree, c->ops, c->pch, c->d_nbins, c->gc, c->scales, d & 1u);
         if (c->fc.host_sync) host_sync_point(c);
         launch_part_count(s, c->pch, c->ops, c->ctl, c->work[0], c->work[1], c->plane0, c->Fs, c->chunk_left,
@@ -403,6 +410,7 @@ GrowView grow_view(bh_context *c)
     v.front[0] = c->front[0];
     v.front[1] = c->front[1];
     v.splits = c->splits;
+    v.fscr = c->fscr;
     v.tree = c->tree;
     v.ops = c->ops;
     v.pch = c->pch;
@@ -424,6 +432,39 @@ GrowView grow_view(bh_context *c)
 extern "C" {
 
 const char *bh_last_error(void) { return g_err.c_str(); }
+
+int bh_copy_calibrate(int device, uint64_t bytes, uint32_t reps, float *ms_mean)
+{
+    if (bytes < 16 || bytes % 16 != 0) return fail(1, "bytes must be a positive multiple of 16");
+    if (reps == 0) return fail(1, "reps must be positive");
+    HIP_TRY(hipSetDevice(device));
+    void *src = nullptr, *dst = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&src, bytes);
+    if (e == hipSuccess) e = hipMalloc(&dst, bytes);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(src, 1, bytes, st);
+    if (e == hipSuccess) {
+        launch_copy_calibrate(st, src, dst, bytes);   // warm
+        (void)hipEventRecord(e0, st);
+        for (u32 r = 0; r < reps; ++r) launch_copy_calibrate(st, src, dst, bytes);
+        (void)hipEventRecord(e1, st);
+        e = hipEventSynchronize(e1);
+    }
+    float ms = 0.0F;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    if (e != hipSuccess) return fail(-1, std::string("hip: ") + hipGetErrorString(e));
+    *ms_mean = ms / static_cast<float>(reps);
+    return 0;
+}
 
 int bh_synchronize(bh_context *c)
 {
@@ -513,10 +554,10 @@ static int alloc_plane(bh_context *c, uint64_t n_rows, uint32_t n_features, cons
     c->SB = SB;
     c->n_bins.assign(n_bins, n_bins + n_features);
     size_t const bytes = static_cast<size_t>(n_rows) * c->Fs;
-    HIP_TRY(hipMalloc(&c->plane0, bytes));
-    HIP_TRY(hipMalloc(&c->work[0], bytes));
-    HIP_TRY(hipMalloc(&c->work[1], bytes));
-    HIP_TRY(hipMemset(c->plane0, 0, bytes));
+    HIP_TRY(hipMalloc(&c->plane0, bytes + kTailPadBytes));
+    HIP_TRY(hipMalloc(&c->work[0], bytes + kTailPadBytes));
+    HIP_TRY(hipMalloc(&c->work[1], bytes + kTailPadBytes));
+    HIP_TRY(hipMemset(c->plane0, 0, bytes + kTailPadBytes));
     HIP_TRY(dalloc(&c->d_nbins, n_features));
     HIP_TRY(hipMemcpy(c->d_nbins, n_bins, n_features * sizeof(u16), hipMemcpyHostToDevice));
     HIP_TRY(dalloc(&c->label, n_rows));
@@ -859,7 +900,7 @@ int bh_find_split(bh_context *c, const float *hist_f32, const int64_t *hist_i64,
     }
     HIP_TRY(hipMemcpy(c->front[0], &nd, sizeof(nd), hipMemcpyHostToDevice));
     launch_find(c->stream, c->quant, c->front[0], c->ctl, c->hist[0], c->d_nbins, c->F, c->SB, c->gc, c->scales,
-                c->splits, 1, 0, 1);
+                c->splits, 1, 0, 1, c->fscr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     SplitRec rec;

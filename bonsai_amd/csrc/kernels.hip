@@ -368,6 +368,100 @@ __global__ __launch_bounds__(1024) void k_hist(
     }
 }
 
+// Quantized histogram build, row-and-dword lanes (H6, same contract as k_hist<true>).
+// Lane l of a wave takes dword d = l % Dw of row l / Dw (Dw = the slice's dwords per
+// row), so one wave load reads 64 / Dw whole rows as one contiguous span and a lane's
+// four bins feed four ds_add_u64 on the packed (g << 32) + h cell.  Each atomic
+// instruction then spreads over 64 / Dw rows x Dw features (7 features x 9 rows at 28
+// features) instead of 64 rows of one feature, which removes most same-address
+// collisions.  Integer sums are exact and order-free.  The LDS histogram is the slice's
+// [feature][SB] cells; the flush is the one k_hist<true> uses.
+constexpr u32 kHqThreads = 1024;
+constexpr int kHqUnroll = 4;
+
+__global__ __launch_bounds__(kHqThreads) void k_hist_q(
+    HistChunk const *__restrict__ chunks, TreeCtl const *__restrict__ ctl,
+    u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
+    void const *__restrict__ gw0, void const *__restrict__ gw1, void const *__restrict__ gwp,
+    u32 Fs, u32 F, u32 SB, u32 fsl, void *__restrict__ partials, void *__restrict__ hist_next)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned long long *lh = reinterpret_cast<unsigned long long *>(smem);   // fsl x SB
+    u32 const f_begin = blockIdx.y * fsl;
+    if (f_begin >= F) return;
+    u32 const nf = min(F - f_begin, fsl);
+    u32 const Dw = (nf + 3u) / 4u;              // dwords of the slice per row (<= 32)
+    u32 const R = 64u / Dw;                     // rows per wave instruction
+    u32 const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    u32 const NW = blockDim.x >> 6;
+    u32 const lr = lane / Dw, d = lane - lr * Dw;
+    bool const lane_ok = lr < R;
+    u32 const cells = nf * SB;
+    u32 const n_units = ctl->n_hist_chunks;
+    size_t const node_cells = static_cast<size_t>(F) * SB;
+
+    for (u32 c = blockIdx.x; c < n_units; c += gridDim.x) {
+        HistChunk const ch = chunks[c];
+        for (u32 i = threadIdx.x; i < cells; i += blockDim.x) lh[i] = 0ull;
+        __syncthreads();
+        u8 const *plane = ch.src == 0 ? pl0 : (ch.src == 1 ? pl1 : plp);
+        u32 const *ghv = static_cast<u32 const *>(ch.src == 0 ? gw0 : (ch.src == 1 ? gw1 : gwp));
+        u8 const *cb = plane + f_begin + 4u * d;
+        u32 const stride = NW * R;
+        u32 r = ch.begin + wave * R + lr;
+        if (lane_ok) {
+            for (; r + (kHqUnroll - 1) * stride < ch.end; r += kHqUnroll * stride) {
+                u32 dw[kHqUnroll], v[kHqUnroll];
+#pragma unroll
+                for (int u = 0; u < kHqUnroll; ++u) {
+                    u32 const rr = r + u * stride;
+                    dw[u] = *reinterpret_cast<u32 const *>(cb + static_cast<size_t>(rr) * Fs);
+                    v[u] = ghv[rr];
+                }
+#pragma unroll
+                for (int u = 0; u < kHqUnroll; ++u) {
+                    u64 const pk = static_cast<u64>(static_cast<i64>(static_cast<int16_t>(v[u] >> 16)) *
+                                                    4294967296LL) + (v[u] & 0xFFFFu);
+#pragma unroll
+                    for (u32 k = 0; k < 4; ++k) {
+                        u32 const fl = 4u * d + k;
+                        if (fl < nf)
+                            atomicAdd(lh + fl * SB + ((dw[u] >> (8u * k)) & 0xFFu),
+                                      static_cast<unsigned long long>(pk));
+                    }
+                }
+            }
+            for (; r < ch.end; r += stride) {
+                u32 const dw1 = *reinterpret_cast<u32 const *>(cb + static_cast<size_t>(r) * Fs);
+                u32 const v1 = ghv[r];
+                u64 const pk = static_cast<u64>(static_cast<i64>(static_cast<int16_t>(v1 >> 16)) * 4294967296LL) +
+                               (v1 & 0xFFFFu);
+#pragma unroll
+                for (u32 k = 0; k < 4; ++k) {
+                    u32 const fl = 4u * d + k;
+                    if (fl < nf)
+                        atomicAdd(lh + fl * SB + ((dw1 >> (8u * k)) & 0xFFu), static_cast<unsigned long long>(pk));
+                }
+            }
+        }
+        __syncthreads();
+        if (ch.part != kNone) {
+            u64 *dst = reinterpret_cast<u64 *>(partials) + static_cast<size_t>(ch.part) * node_cells +
+                       static_cast<size_t>(f_begin) * SB;
+            for (u32 i = threadIdx.x; i < cells; i += blockDim.x) dst[i] = lh[i];
+        } else {
+            i64 *dst = reinterpret_cast<i64 *>(hist_next) +
+                       (static_cast<size_t>(ch.slot) * node_cells + static_cast<size_t>(f_begin) * SB) * 2;
+            for (u32 i = threadIdx.x; i < cells; i += blockDim.x) {
+                u64 const v = lh[i];
+                dst[2 * i] = static_cast<int32_t>(static_cast<u32>(v >> 32));
+                dst[2 * i + 1] = static_cast<i64>(v & 0xFFFFFFFFull);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Fixed-order completion of built nodes (H6 reduce + H7 sibling subtraction,
 // src/step/primitives.hpp:156-173; device twin subtract_kernel, kernels.cuh:507-520):
 // small = p_0 + p_1 + ... in unit order, large = parent - small.
@@ -471,6 +565,8 @@ __global__ void k_finish(FinishOp const *__restrict__ ops, TreeCtl const *__rest
 // ---------------------------------------------------------------------------------
 constexpr u32 kFindTile = 7;
 
+u32 find_blocks_per_node(u32 F) { return std::min<u32>(kFindMaxBlocks, (F + kFindTile - 1) / kFindTile); }
+
 template <bool Q> struct CellT;
 template <> struct CellT<false> { typedef float2 T; typedef double A; };
 template <> struct CellT<true> { struct T { i64 x, y; }; typedef i64 A; };
@@ -479,7 +575,8 @@ template <bool Q>
 __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, TreeCtl const *__restrict__ ctl,
                                               void const *__restrict__ hist, u16 const *__restrict__ n_bins,
                                               u32 F, u32 SB, GrowCfg cfg, double const *__restrict__ scales,
-                                              SplitRec *__restrict__ out, u32 front_base, u32 n_front_override)
+                                              SplitRec *__restrict__ out, u32 front_base, u32 n_front_override,
+                                              SplitRec *__restrict__ part, u32 *__restrict__ done)
 {
     typedef typename CellT<Q>::T Cell;
     typedef typename CellT<Q>::A Acc;
@@ -495,13 +592,14 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
     __shared__ u32 s_ncut[kFindTile], s_dirs[kFindTile];
 
     u32 const n_front = n_front_override ? n_front_override : ctl->n_front;
-    if (blockIdx.x >= n_front) return;
-    u32 const i = front_base + blockIdx.x;
+    if (blockIdx.y >= n_front) return;
+    u32 const i = front_base + blockIdx.y;
+    u32 const G = gridDim.x, bx = blockIdx.x;
     NodeInfo node = front[i];
     if (node.slot == kNone) {
         // No histogram (a leafwise child at max_depth): totals came from the parent
         // split and the node cannot split.
-        if (threadIdx.x == 0) {
+        if (bx == 0 && threadIdx.x == 0) {
             SplitRec rec{};
             rec.default_left = 1;
             out[i] = rec;
@@ -518,79 +616,83 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
     u32 best_key = kNone;
     double bgL = 0.0, bhL = 0.0, bgR = 0.0, bhR = 0.0; // winner's child sums (FP32 path)
     i64 bigL = 0, bihL = 0, bigR = 0, bihR = 0;        // winner's child sums (quantized)
-    __shared__ SplitRec s_rec;
+    __shared__ u32 s_last;
     double node_score = 0.0, tg = 0.0, th = 0.0;
-    for (u32 f0 = 0; f0 < F; f0 += kFindTile) {
+    // Phase 0: node totals from the first feature's cells (every block of the node
+    // derives the same values) unless a parent split already set them.
+    bool const had_totals = node.totals_ready != 0;
+    if (!had_totals) {
+        u32 const nb0 = n_bins[0];
+        for (u32 k = threadIdx.x; k < nb0; k += blockDim.x) s_cells[k] = hcells[k];
+        __syncthreads();
+        if (Q) {
+            // Exact integer totals: any order (one wave reduces feature 0's cells).
+            if (threadIdx.x < 64) {
+                i64 g = 0, h = 0;
+                for (u32 b = threadIdx.x; b < nb0; b += 64) {
+                    g += reinterpret_cast<i64 const *>(&s_cells[b])[0];
+                    h += reinterpret_cast<i64 const *>(&s_cells[b])[1];
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    g += __shfl_xor(g, o);
+                    h += __shfl_xor(h, o);
+                }
+                if (threadIdx.x == 0) {
+                    node.itg = g;
+                    node.ith = h;
+                }
+            }
+        } else if (threadIdx.x == 0) {
+            // Histogram::totals (include/bonsai/histogram.hpp:76-87): the first
+            // feature's cells, double in bin order, stored as float.
+            double g = 0.0, h = 0.0;
+            float2 const *cf = reinterpret_cast<float2 const *>(s_cells);
+            u32 b = 0;
+            for (; b + 8 <= nb0; b += 8) {
+                float2 v[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) v[k] = cf[b + k];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    g += v[k].x;
+                    h += v[k].y;
+                }
+            }
+            for (; b < nb0; ++b) {
+                g += cf[b].x;
+                h += cf[b].y;
+            }
+            node.tg = static_cast<float>(g);
+            node.th = static_cast<float>(h);
+        }
+        if (threadIdx.x == 0) node.totals_ready = 1;
+    }
+    if (threadIdx.x == 0) {
+        s_tot[0] = node.tg;
+        s_tot[1] = node.th;
+        s_itot[0] = node.itg;
+        s_itot[1] = node.ith;
+    }
+    __syncthreads();
+    if (!can) {
+        if (bx == 0 && threadIdx.x == 0) {
+            if (!had_totals) front[i] = node;
+            SplitRec rec{};
+            rec.default_left = 1;
+            out[i] = rec;
+        }
+        return;
+    }
+    tg = Q ? static_cast<double>(s_itot[0]) * sg : s_tot[0];
+    th = Q ? static_cast<double>(s_itot[1]) * sh : s_tot[1];
+    node_score = score_of(tg, th, cfg.l1, cfg.l2);
+    // Block bx scans feature tiles bx, bx + G, ...
+    for (u32 f0 = bx * kFindTile; f0 < F; f0 += G * kFindTile) {
         u32 const nt = min(kFindTile, F - f0);
+        __syncthreads(); // the previous tile (or phase 0) is done with the LDS
         // Stage the tile's cells (coalesced).
         for (u32 k = threadIdx.x; k < nt * SB; k += blockDim.x) s_cells[k] = hcells[static_cast<size_t>(f0) * SB + k];
         __syncthreads();
-        if (f0 == 0) {
-            if (Q && !node.totals_ready) {
-                // Exact integer totals: any order (one wave reduces feature 0's cells).
-                if (threadIdx.x < 64) {
-                    i64 g = 0, h = 0;
-                    for (u32 b = threadIdx.x; b < n_bins[0]; b += 64) {
-                        g += reinterpret_cast<i64 const *>(&s_cells[b])[0];
-                        h += reinterpret_cast<i64 const *>(&s_cells[b])[1];
-                    }
-                    for (int o = 32; o > 0; o >>= 1) {
-                        g += __shfl_xor(g, o);
-                        h += __shfl_xor(h, o);
-                    }
-                    if (threadIdx.x == 0) {
-                        node.itg = g;
-                        node.ith = h;
-                    }
-                }
-            }
-            if (threadIdx.x == 0) {
-                if (!node.totals_ready) {
-                    // Histogram::totals (include/bonsai/histogram.hpp:76-87): the first
-                    // feature's cells, double in bin order, stored as float.
-                    if (!Q) {
-                        double g = 0.0, h = 0.0;
-                        float2 const *cf = reinterpret_cast<float2 const *>(s_cells);
-                        u32 const nb0 = n_bins[0];
-                        u32 b = 0;
-                        for (; b + 8 <= nb0; b += 8) {
-                            float2 v[8];
-#pragma unroll
-                            for (int k = 0; k < 8; ++k) v[k] = cf[b + k];
-#pragma unroll
-                            for (int k = 0; k < 8; ++k) {
-                                g += v[k].x;
-                                h += v[k].y;
-                            }
-                        }
-                        for (; b < nb0; ++b) {
-                            g += cf[b].x;
-                            h += cf[b].y;
-                        }
-                        node.tg = static_cast<float>(g);
-                        node.th = static_cast<float>(h);
-                    }
-                    node.totals_ready = 1;
-                    front[i] = node;
-                }
-                s_tot[0] = node.tg;
-                s_tot[1] = node.th;
-                s_itot[0] = node.itg;
-                s_itot[1] = node.ith;
-            }
-            __syncthreads();
-            if (!can) {
-                if (threadIdx.x == 0) {
-                    SplitRec rec{};
-                    rec.default_left = 1;
-                    out[i] = rec;
-                }
-                return;
-            }
-            tg = Q ? static_cast<double>(s_itot[0]) * sg : s_tot[0];
-            th = Q ? static_cast<double>(s_itot[1]) * sh : s_tot[1];
-            node_score = score_of(tg, th, cfg.l1, cfg.l2);
-        }
         // Phase A: per-feature running sums in bin order (src/split.cpp:131-158).
         // Quantized: exact integers, so a wave scans a feature in parallel.
         if (Q) {
@@ -763,8 +865,11 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
         }
         __syncthreads();
     }
-    // The thread that scored the winner publishes its child sums (keys are unique).
+    // The thread that scored the winner publishes its child sums (keys are unique):
+    // straight to out[i] with one block per node, else to the block's partial record;
+    // the node's last block to finish merges the partials (gain, then lowest key).
     u32 const key = s_best_key[0];
+    SplitRec *dst = G == 1 ? out + i : part + static_cast<size_t>(i) * G + bx;
     if (key != kNone && best_key == key) {
         SplitRec rec{};
         u32 const f = key / (2u * SB), b = (key >> 1) % SB, d = key & 1u;
@@ -781,13 +886,38 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
         rec.ihL = bihL;
         rec.igR = bigR;
         rec.ihR = bihR;
-        out[i] = rec;
+        *dst = rec;
+        __threadfence();
     } else if (key == kNone && threadIdx.x == 0) {
         SplitRec rec{};
         rec.default_left = 1;
-        out[i] = rec;
+        *dst = rec;
+        __threadfence();
     }
-    (void)s_rec;
+    if (G == 1) {
+        if (threadIdx.x == 0 && !had_totals) front[i] = node;
+        return;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(done + i, 1u) == G - 1u ? 1u : 0u;
+    __syncthreads();
+    if (s_last == 0u || threadIdx.x != 0) return;
+    __threadfence();
+    SplitRec best{};
+    best.default_left = 1;
+    u32 bkey = kNone;
+    for (u32 k = 0; k < G; ++k) {
+        SplitRec const r = part[static_cast<size_t>(i) * G + k];
+        if (!r.valid) continue;
+        u32 const rk = r.feature * (2u * SB) + r.bin * 2u + (r.default_left ? 0u : 1u);
+        if (bkey == kNone || r.gain > best.gain || (r.gain == best.gain && rk < bkey)) {
+            best = r;
+            bkey = rk;
+        }
+    }
+    out[i] = best;
+    if (!had_totals) front[i] = node;
+    done[i] = 0u;
 }
 
 // ---------------------------------------------------------------------------------
@@ -1149,111 +1279,91 @@ __global__ __launch_bounds__(256) void k_part_scatter(
     }
 }
 
-// Narrow rows (Fs <= 64 bytes): one lane per row, the row's dwords in registers.  The
-// route comes from the loaded row, ranks from a wave ballot plus the block's running
-// count, so the unit is read once and written once, coalesced across lanes.
-template <int DMAX>
-__global__ __launch_bounds__(256) void k_part_scatter_narrow(
-    PartChunk const *__restrict__ pch, PartOp const *__restrict__ ops, TreeCtl const *__restrict__ ctl,
-    u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
-    void const *__restrict__ gw0, void const *__restrict__ gw1, void const *__restrict__ gwp,
-    u8 *__restrict__ out0, u8 *__restrict__ out1, void *__restrict__ go0, void *__restrict__ go1, u32 Fs,
-    u32 gh_bytes)
+// Narrow rows (Fs <= 64 bytes), LDS-staged with 16-byte accesses.  A tile of the unit's
+// rows is one contiguous byte range: it is read with aligned 16-byte loads into LDS
+// (keeping its sub-16-byte offset), ranked with wave ballots, and each child's run is a
+// contiguous destination range written with aligned 16-byte stores (dword head and tail
+// at the run's edges).  Planes and gradient buffers carry 64 bytes of tail padding, so
+// the aligned over-read past a tile's last row stays inside the allocation.
+constexpr u32 kScatDivShift = 21;   // k / D as (k * ceil(2^21 / D)) >> 21: exact for k < 2^14, D <= 16
+
+__device__ __forceinline__ u32 lds_load_range(u8 const *__restrict__ src, size_t a, u32 len, uint4 *dst)
 {
-    __shared__ u32 s_wsum[4];
-    u32 const n = ctl->n_part_chunks;
-    u32 const D = Fs / 4u;
-    u32 const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    for (u32 c = blockIdx.x; c < n; c += gridDim.x) {
-        PartChunk const ch = pch[c];
-        PartOp const op = ops[ch.op];
-        if (!op.alive) continue; // uniform across the block
-        u8 const *plane = op.src == 0 ? pl0 : (op.src == 1 ? pl1 : plp);
-        u8 const *gin = reinterpret_cast<u8 const *>(op.src == 0 ? gw0 : (op.src == 1 ? gw1 : gwp));
-        u8 *oplane = op.dst == 0 ? out0 : out1;
-        u8 *gout = reinterpret_cast<u8 *>(op.dst == 0 ? go0 : go1);
-        u32 const fw = op.feature >> 2, fsh = (op.feature & 3u) * 8u;
-        u32 run = 0;
-        for (u32 base = ch.begin; base < ch.end; base += 256u) {
-            u32 const r = base + threadIdx.x; // op-relative row
-            bool const ok = r < ch.end;
-            size_t const src = static_cast<size_t>(op.off) + r;
-            u32 w[DMAX];
-            u32 const *srow = reinterpret_cast<u32 const *>(plane + src * Fs);
+    size_t const a16 = a & ~static_cast<size_t>(15);
+    u32 const sh = static_cast<u32>(a - a16);
+    u32 const nw = (sh + len + 15u) >> 4;
+    uint4 const *s = reinterpret_cast<uint4 const *>(src + a16);
+    for (u32 w = threadIdx.x; w < nw; w += blockDim.x) dst[w] = s[w];
+    return sh;
+}
+
+// dst[first + k] = src[base + idx[k / D] * D + k % D] for k < n (dword units).
+__device__ __forceinline__ void store_run(u32 *__restrict__ dst, size_t first, u32 n, u32 D, u32 magic,
+                                          u32 const *src, u32 base, u16 const *idx)
+{
+    u32 const head = min(n, static_cast<u32>((4u - (first & 3u)) & 3u));
+    if (threadIdx.x < head) {
+        u32 const k = threadIdx.x;
+        u32 const q = (k * magic) >> kScatDivShift;
+        dst[first + k] = src[base + static_cast<u32>(idx[q]) * D + (k - q * D)];
+    }
+    u32 const n_vec = (n - head) >> 2;
+    uint4 *dv = reinterpret_cast<uint4 *>(dst + first + head);
+    for (u32 m = threadIdx.x; m < n_vec; m += blockDim.x) {
+        u32 const k = head + 4u * m;
+        u32 q = (k * magic) >> kScatDivShift;
+        u32 rem = k - q * D;
+        u32 v[4];
 #pragma unroll
-            for (int d = 0; d < DMAX; ++d) w[d] = (ok && static_cast<u32>(d) < D) ? srow[d] : 0u;
-            uint2 gv = make_uint2(0u, 0u);
-            if (ok) {
-                if (gh_bytes == 8) gv = reinterpret_cast<uint2 const *>(gin)[src];
-                else gv.x = reinterpret_cast<u32 const *>(gin)[src];
+        for (int j = 0; j < 4; ++j) {
+            v[j] = src[base + static_cast<u32>(idx[q]) * D + rem];
+            if (++rem == D) {
+                rem = 0;
+                ++q;
             }
-            u32 sel = 0;
-#pragma unroll
-            for (int d = 0; d < DMAX; ++d) sel = (static_cast<u32>(d) == fw) ? w[d] : sel;
-            u32 const b = (sel >> fsh) & 0xFFu;
-            bool const left = ok && routes_left(b, op.last_bin, op.bin, op.default_left);
-            u64 const bal = __ballot(left);
-            u32 const lrank = static_cast<u32>(__popcll(bal & ((1ull << lane) - 1ull)));
-            if (lane == 0) s_wsum[wave] = static_cast<u32>(__popcll(bal));
-            __syncthreads();
-            u32 pre = run;
-            for (u32 k = 0; k < wave; ++k) pre += s_wsum[k];
-            u32 const tot = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
-            __syncthreads();
-            if (ok) {
-                u32 const lb = ch.left_before + pre + lrank; // left rows of the op before r
-                u32 const dest = left ? op.off + lb : op.off + op.n_left + (r - lb);
-                u32 *drow = reinterpret_cast<u32 *>(oplane + static_cast<size_t>(dest) * Fs);
-#pragma unroll
-                for (int d = 0; d < DMAX; ++d)
-                    if (static_cast<u32>(d) < D) drow[d] = w[d];
-                if (gh_bytes == 8) reinterpret_cast<uint2 *>(gout)[dest] = gv;
-                else reinterpret_cast<u32 *>(gout)[dest] = gv.x;
-            }
-            run += tot;
         }
+        dv[m] = make_uint4(v[0], v[1], v[2], v[3]);
+    }
+    u32 const t0 = head + 4u * n_vec;
+    if (threadIdx.x < n - t0) {
+        u32 const k = t0 + threadIdx.x;
+        u32 const q = (k * magic) >> kScatDivShift;
+        dst[first + k] = src[base + static_cast<u32>(idx[q]) * D + (k - q * D)];
     }
 }
 
-// Narrow rows, LDS-staged: the unit's rows are read as one contiguous dword stream,
-// ranked in LDS, and each child's run is written back as one contiguous dword stream,
-// so every global access of the scatter is fully coalesced.
-constexpr u32 kScatTile = 1024;
-
-__global__ __launch_bounds__(256) void k_part_scatter_lds(
+__global__ __launch_bounds__(256) void k_part_scatter_v(
     PartChunk const *__restrict__ pch, PartOp const *__restrict__ ops, TreeCtl const *__restrict__ ctl,
     u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
     void const *__restrict__ gw0, void const *__restrict__ gw1, void const *__restrict__ gwp,
     u8 *__restrict__ out0, u8 *__restrict__ out1, void *__restrict__ go0, void *__restrict__ go1, u32 Fs,
-    u32 gh_bytes, u32 tile)
+    u32 gh_bytes, u32 tile, u32 magic_d, u32 magic_g)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    u32 *s_rows = reinterpret_cast<u32 *>(smem);             // tile * Fs bytes
-    u32 *s_gh = s_rows + tile * (Fs / 4u);                    // tile * gh_bytes
-    u16 *s_lidx = reinterpret_cast<u16 *>(s_gh + tile * (gh_bytes / 4u));
+    uint4 *s_rows = reinterpret_cast<uint4 *>(smem);                        // tile * Fs + 16 bytes
+    uint4 *s_gh = reinterpret_cast<uint4 *>(smem + tile * Fs + 16u);        // tile * gh_bytes + 16
+    u16 *s_lidx = reinterpret_cast<u16 *>(smem + tile * (Fs + gh_bytes) + 32u);
     u16 *s_ridx = s_lidx + tile;
     __shared__ u32 s_wsum[4];
     u32 const n = ctl->n_part_chunks;
     u32 const D = Fs / 4u, G = gh_bytes / 4u;
-    u32 const step_q = blockDim.x / D, step_r = blockDim.x - step_q * D;
     u32 const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     for (u32 c = blockIdx.x; c < n; c += gridDim.x) {
         PartChunk const ch = pch[c];
         PartOp const op = ops[ch.op];
         if (!op.alive) continue; // uniform across the block
         u8 const *plane = op.src == 0 ? pl0 : (op.src == 1 ? pl1 : plp);
-        u32 const *gin = reinterpret_cast<u32 const *>(op.src == 0 ? gw0 : (op.src == 1 ? gw1 : gwp));
-        u8 *oplane = op.dst == 0 ? out0 : out1;
-        u32 *gout = reinterpret_cast<u32 *>(op.dst == 0 ? go0 : go1);
+        u8 const *gin = static_cast<u8 const *>(op.src == 0 ? gw0 : (op.src == 1 ? gw1 : gwp));
+        u32 *oplane = reinterpret_cast<u32 *>(op.dst == 0 ? out0 : out1);
+        u32 *gout = static_cast<u32 *>(op.dst == 0 ? go0 : go1);
         u32 left_run = ch.left_before; // left rows of the op before the current tile
         for (u32 t0 = ch.begin; t0 < ch.end; t0 += tile) {
             u32 const rows = min(tile, ch.end - t0);
             size_t const srow0 = static_cast<size_t>(op.off) + t0;
-            u32 const *gsrc = reinterpret_cast<u32 const *>(plane + srow0 * Fs);
-            for (u32 k = threadIdx.x; k < rows * D; k += blockDim.x) s_rows[k] = gsrc[k];
-            for (u32 k = threadIdx.x; k < rows * G; k += blockDim.x) s_gh[k] = gin[srow0 * G + k];
+            u32 const sh = lds_load_range(plane, srow0 * Fs, rows * Fs, s_rows);
+            u32 const shg = lds_load_range(gin, srow0 * gh_bytes, rows * gh_bytes, s_gh);
             __syncthreads();
-            u8 const *srb = reinterpret_cast<u8 const *>(s_rows);
+            u8 const *srb = reinterpret_cast<u8 const *>(s_rows) + sh;
             u32 run = 0;
             for (u32 p0 = 0; p0 < rows; p0 += blockDim.x) {
                 u32 const r = p0 + threadIdx.x;
@@ -1278,40 +1388,14 @@ __global__ __launch_bounds__(256) void k_part_scatter_lds(
             __syncthreads();
             u32 const nl = run, nr = rows - run;
             // Left run: destination rows [off + left_run, + nl); right run: after n_left.
-            u32 const l0 = op.off + left_run;
-            u32 const r0 = op.off + op.n_left + (t0 - left_run);
-            u32 *dl = reinterpret_cast<u32 *>(oplane + static_cast<size_t>(l0) * Fs);
-            u32 *dr = reinterpret_cast<u32 *>(oplane + static_cast<size_t>(r0) * Fs);
-            // dword k of a run belongs to run row k / D; (q, rem) advance incrementally.
-            {
-                u32 q = threadIdx.x / D, rem = threadIdx.x - q * D;
-                for (u32 k = threadIdx.x; k < nl * D; k += blockDim.x) {
-                    dl[k] = s_rows[static_cast<u32>(s_lidx[q]) * D + rem];
-                    q += step_q;
-                    rem += step_r;
-                    if (rem >= D) {
-                        rem -= D;
-                        ++q;
-                    }
-                }
-            }
-            {
-                u32 q = threadIdx.x / D, rem = threadIdx.x - q * D;
-                for (u32 k = threadIdx.x; k < nr * D; k += blockDim.x) {
-                    dr[k] = s_rows[static_cast<u32>(s_ridx[q]) * D + rem];
-                    q += step_q;
-                    rem += step_r;
-                    if (rem >= D) {
-                        rem -= D;
-                        ++q;
-                    }
-                }
-            }
-            u32 const gs = G == 2 ? 1u : 0u;
-            for (u32 k = threadIdx.x; k < nl * G; k += blockDim.x)
-                gout[static_cast<size_t>(l0) * G + k] = s_gh[(static_cast<u32>(s_lidx[k >> gs]) << gs) + (k & gs)];
-            for (u32 k = threadIdx.x; k < nr * G; k += blockDim.x)
-                gout[static_cast<size_t>(r0) * G + k] = s_gh[(static_cast<u32>(s_ridx[k >> gs]) << gs) + (k & gs)];
+            size_t const l0 = static_cast<size_t>(op.off) + left_run;
+            size_t const r0 = static_cast<size_t>(op.off) + op.n_left + (t0 - left_run);
+            u32 const *sr = reinterpret_cast<u32 const *>(s_rows);
+            u32 const *sg = reinterpret_cast<u32 const *>(s_gh);
+            store_run(oplane, l0 * D, nl * D, D, magic_d, sr, sh >> 2, s_lidx);
+            store_run(oplane, r0 * D, nr * D, D, magic_d, sr, sh >> 2, s_ridx);
+            store_run(gout, l0 * G, nl * G, G, magic_g, sg, shg >> 2, s_lidx);
+            store_run(gout, r0 * G, nr * G, G, magic_g, sg, shg >> 2, s_ridx);
             left_run += nl;
             __syncthreads();
         }
@@ -1537,9 +1621,31 @@ __global__ __launch_bounds__(256) void k_bin_rows(float const *__restrict__ X, u
     }
 }
 
+// Plain 16-byte-per-lane copy: the calibration kernel of known traffic (reads and
+// writes n * 16 bytes) against which HBM counters and bandwidth are read.
+__global__ __launch_bounds__(256) void k_copy_calibrate(uint4 const *__restrict__ src, uint4 *__restrict__ dst, u64 n)
+{
+    for (u64 i = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<u64>(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------------
 // Launch wrappers (host side).
 // ---------------------------------------------------------------------------------
+// Scatter tile sizing: rows per LDS tile = budget / (row + gradient + 4 index bytes),
+// a multiple of 64 in [64, kScatTileMax].  The budget is a tuning knob.
+constexpr u32 kScatTileMax = 1024;
+static u32 g_scatter_lds_budget = 20u * 1024u;
+u32 scatter_lds_budget() { return g_scatter_lds_budget; }
+void set_scatter_lds_budget(u32 bytes) { g_scatter_lds_budget = std::max(4096u, std::min(bytes, 64u * 1024u)); }
+
+void launch_copy_calibrate(hipStream_t s, void const *src, void *dst, u64 bytes)
+{
+    u64 const n = bytes / 16;
+    hipLaunchKernelGGL(k_copy_calibrate, dim3(static_cast<u32>(std::min<u64>((n + 255) / 256, 16384))), dim3(256), 0,
+                       s, static_cast<uint4 const *>(src), static_cast<uint4 *>(dst), n);
+}
 void launch_bin_rows(hipStream_t s, float const *X, u64 n, u32 F, u32 Fs, float const *cuts, u32 const *coff,
                      u32 n_cuts_total, u8 *plane)
 {
@@ -1577,10 +1683,10 @@ hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl c
     u32 const slices = (F + fsl - 1) / fsl;
     size_t const lds = static_cast<size_t>(fsl) * SB * 8;
     dim3 const grid(grid_x, slices), block(64 * n_waves);
-    if (q)
-        hipLaunchKernelGGL((k_hist<true, 8>), grid, block, lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp, Fs, F,
+    if (q) {
+        hipLaunchKernelGGL(k_hist_q, grid, dim3(kHqThreads), lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp, Fs, F,
                            SB, fsl, partials, hist_next);
-    else
+    } else
         hipLaunchKernelGGL((k_hist<false, 4>), grid, block, lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp, Fs, F,
                            SB, fsl, partials, hist_next);
     return hipGetLastError();
@@ -1607,6 +1713,7 @@ hipError_t set_hist_lds_limits()
     e = hipDeviceGetAttribute(&max_shared, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (e != hipSuccess) return e;
     raise_lds(reinterpret_cast<void const *>(&k_hist<true, 8>), max_shared);
+    raise_lds(reinterpret_cast<void const *>(&k_hist_q), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_hist<false, 4>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_find<true>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_find<false>), max_shared);
@@ -1628,15 +1735,17 @@ void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl const *ct
 
 void launch_find(hipStream_t s, bool q, NodeInfo *front, TreeCtl const *ctl, void const *hist, u16 const *n_bins,
                  u32 F, u32 SB, GrowCfg const &cfg, double const *scales, SplitRec *out, u32 max_front,
-                 u32 front_base, u32 n_front_override)
+                 u32 front_base, u32 n_front_override, FindScratch const &fs)
 {
     size_t const lds = static_cast<size_t>(kFindTile) * SB * (q ? 32 : 24);
+    u32 const G = find_blocks_per_node(F);
+    dim3 const grid(G, max_front);
     if (q)
-        hipLaunchKernelGGL(k_find<true>, dim3(max_front), dim3(256), lds, s, front, ctl, hist, n_bins, F, SB, cfg,
-                           scales, out, front_base, n_front_override);
+        hipLaunchKernelGGL(k_find<true>, grid, dim3(256), lds, s, front, ctl, hist, n_bins, F, SB, cfg, scales, out,
+                           front_base, n_front_override, fs.part, fs.done);
     else
-        hipLaunchKernelGGL(k_find<false>, dim3(max_front), dim3(256), lds, s, front, ctl, hist, n_bins, F, SB, cfg,
-                           scales, out, front_base, n_front_override);
+        hipLaunchKernelGGL(k_find<false>, grid, dim3(256), lds, s, front, ctl, hist, n_bins, F, SB, cfg, scales, out,
+                           front_base, n_front_override, fs.part, fs.done);
 }
 
 void launch_plan_level(hipStream_t s, NodeInfo const *front, SplitRec const *splits, TreeCtl *ctl, DevNode *tree,
@@ -1671,12 +1780,15 @@ void launch_part_scatter(hipStream_t s, PartChunk const *pch, PartOp const *ops,
 {
     u32 const D = Fs / 4;
     if (D <= 16) {
-        // Rows per LDS tile: at most kScatTile, and the tile within 48 KiB of LDS.
-        u32 tile = (48u * 1024u) / (Fs + gh_bytes + 4u);
-        tile = std::min<u32>(kScatTile, tile & ~63u);
-        size_t const lds = static_cast<size_t>(tile) * (Fs + gh_bytes + 4);
-        hipLaunchKernelGGL(k_part_scatter_lds, dim3(grid), dim3(256), lds, s, pch, ops, ctl, pl0, pl1, plp, g0, g1,
-                           gp, o0, o1, go0, go1, Fs, gh_bytes, tile);
+        u32 const per_row = Fs + gh_bytes + 4u;
+        u32 tile = (scatter_lds_budget() - 32u) / per_row;
+        tile = std::max(64u, std::min<u32>(kScatTileMax, tile & ~63u));
+        size_t const lds = static_cast<size_t>(tile) * per_row + 32u;
+        u32 const G = gh_bytes / 4u;
+        u32 const magic_d = ((1u << kScatDivShift) + D - 1u) / D;
+        u32 const magic_g = ((1u << kScatDivShift) + G - 1u) / G;
+        hipLaunchKernelGGL(k_part_scatter_v, dim3(grid), dim3(256), lds, s, pch, ops, ctl, pl0, pl1, plp, g0, g1,
+                           gp, o0, o1, go0, go1, Fs, gh_bytes, tile, magic_d, magic_g);
         return;
     }
     u32 tl = 0;

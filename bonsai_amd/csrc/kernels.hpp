@@ -18,7 +18,8 @@ void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl const *ct
                    void const *hist_cur, void *hist_next, u32 node_cells, u32 max_ops);
 void launch_find(hipStream_t s, bool q, NodeInfo *front, TreeCtl const *ctl, void const *hist, u16 const *n_bins,
                  u32 F, u32 SB, GrowCfg const &cfg, double const *scales, SplitRec *out, u32 max_front,
-                 u32 front_base, u32 n_front_override);
+                 u32 front_base, u32 n_front_override, FindScratch const &fs);
+u32 find_blocks_per_node(u32 F);
 void launch_plan_level(hipStream_t s, NodeInfo const *front, SplitRec const *splits, TreeCtl *ctl, DevNode *tree,
                        PartOp *ops, PartChunk *pch, u16 const *n_bins, GrowCfg const &cfg, double const *scales,
                        u32 dst);
@@ -40,6 +41,9 @@ void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode 
 void launch_iter_advance(hipStream_t s, TreeCtl *ctl);
 void launch_copy_tree(hipStream_t s, DevNode const *tree, TreeCtl const *ctl, DevNode *out_trees, u32 *out_sizes,
                       u32 max_nodes);
+u32 scatter_lds_budget();
+void set_scatter_lds_budget(u32 bytes);
+void launch_copy_calibrate(hipStream_t s, void const *src, void *dst, u64 bytes);
 void launch_bin_rows(hipStream_t s, float const *X, u64 n, u32 F, u32 Fs, float const *cuts, u32 const *coff,
                      u32 n_cuts_total, u8 *plane);
 void launch_gather_rows(hipStream_t s, u8 const *plane, u32 Fs, void const *gh, u32 gh_bytes, u32 const *rows,

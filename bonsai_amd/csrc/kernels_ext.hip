@@ -994,7 +994,7 @@ void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
     launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
                 v.fsl, v.waves, hgrid, v.partials, pool);
     launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1);
-    launch_find(st, q, kids, v.ctl, pool, v.d_nbins, v.F, v.SB, v.gc, v.scales, v.splits, 2, 0, 0);
+    launch_find(st, q, kids, v.ctl, pool, v.d_nbins, v.F, v.SB, v.gc, v.scales, v.splits, 2, 0, 0, v.fscr);
     hipLaunchKernelGGL(k_lw_push, dim3(1), dim3(64), 0, st, v.ctl, lw, kids, v.splits, s.cands, s.free_slots, v.tree,
                        v.gc, v.scales, 1u);
     u32 const rounds = static_cast<u32>(v.fc.max_leaves) - 1u + 8u;
@@ -1015,7 +1015,7 @@ void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
         launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
                     v.fsl, v.waves, hgrid, v.partials, pool);
         launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1);
-        launch_find(st, q, kids, v.ctl, pool, v.d_nbins, v.F, v.SB, v.gc, v.scales, v.splits, 2, 0, 0);
+        launch_find(st, q, kids, v.ctl, pool, v.d_nbins, v.F, v.SB, v.gc, v.scales, v.splits, 2, 0, 0, v.fscr);
         hipLaunchKernelGGL(k_lw_push, dim3(1), dim3(64), 0, st, v.ctl, lw, kids, v.splits, s.cands, s.free_slots,
                            v.tree, v.gc, v.scales, 0u);
         if (v.host_sync) (void)hipStreamSynchronize(st);

@@ -19,6 +19,7 @@ struct GrowView {
     TreeCtl *ctl;
     NodeInfo *front[2];
     SplitRec *splits;
+    FindScratch fscr;
     DevNode *tree;
     PartOp *ops;
     PartChunk *pch;

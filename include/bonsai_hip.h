@@ -81,6 +81,11 @@ typedef struct bh_split {
 int         bh_device_count(int *count);
 /* Block until every operation queued on the context's stream has finished. */
 int         bh_synchronize(bh_context *ctx);
+/* Calibration (no counterpart in the reference): copy `bytes` (a multiple of 16) between
+ * two fresh device buffers `reps` times with a plain vector copy kernel and return the
+ * mean kernel time (ms, hipEvents).  Its traffic is exactly 2 x bytes, which calibrates
+ * HBM counters and gives the achievable copy bandwidth beside the histogram's. */
+int         bh_copy_calibrate(int device, uint64_t bytes, uint32_t reps, float *ms_mean);
 /* Message of the last failure on this thread. */
 const char *bh_last_error(void);
 
