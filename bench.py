@@ -68,10 +68,17 @@ def make_data(n_rows: int, n_held: int):
     return X, y, Xh, yh, cuts
 
 
-def fit(B, ctx, quantized: bool, steps: int, warmup: int, init: float, n_rows: int):
+def fit(B, ctx, quantized: bool, steps: int, warmup: int, init: float, n_rows: int, autotune: int):
     ctx.configure(B.FitConfig(grower="hip_depthwise", objective="logloss", quantized=quantized,
                               use_graph=True, max_depth=8, learning_rate=0.1, lambda_l2=1.0))
     start = np.full(n_rows, init, np.float32)
+    tuned = None
+    if autotune:
+        ctx.set_scores(start)
+        t0 = time.perf_counter()
+        shapes, ms = ctx.autotune(autotune)
+        tuned = {"shapes": shapes, "ms_per_iter": round(ms, 4), "seconds": round(time.perf_counter() - t0, 2)}
+        log(f"autotune ({'quantized' if quantized else 'fp32'}): {shapes} -> {ms:.3f} ms/iter")
     if warmup:
         ctx.set_scores(start)
         ctx.train(warmup)
@@ -81,7 +88,7 @@ def fit(B, ctx, quantized: bool, steps: int, warmup: int, init: float, n_rows: i
     ctx.train(steps)            # returns after the engine stream has drained
     B.synchronize(ctx)
     dt = time.perf_counter() - t0
-    return dt, ctx.trees()
+    return dt, ctx.trees(), tuned
 
 
 def held_out_auc(B, trees, n_bins, init, bins_h, yh):
@@ -119,6 +126,7 @@ def main() -> int:
     ap.add_argument("--rows", type=int, default=16_000_000)
     ap.add_argument("--held-out", type=int, default=1_600_000)
     ap.add_argument("--prof-iters", type=int, default=3, help="eager iterations timed per histogram launch")
+    ap.add_argument("--autotune", type=int, default=5, help="iterations per autotuner trial (0: default shapes)")
     ap.add_argument("--skip-fp32", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=4_000_000)
@@ -155,7 +163,7 @@ def main() -> int:
     ctx.upload_labels(y)
     log(f"plane binned on the device in {time.perf_counter() - t:.2f}s")
 
-    dt_q, trees_q = fit(B, ctx, True, a.steps, a.warmup, init, a.rows)
+    dt_q, trees_q, tuned_q = fit(B, ctx, True, a.steps, a.warmup, init, a.rows, a.autotune)
     auc_q = held_out_auc(B, trees_q, nb, init, bins_h, yh)
     log(f"quantized: {a.steps} iters in {dt_q:.3f}s, held-out AUC {auc_q:.5f}")
     prof = ctx.profile_hist(a.prof_iters)
@@ -166,12 +174,12 @@ def main() -> int:
 
     fp32 = None
     if not a.skip_fp32:
-        dt_f, trees_f = fit(B, ctx, False, a.steps, a.warmup, init, a.rows)
+        dt_f, trees_f, tuned_f = fit(B, ctx, False, a.steps, a.warmup, init, a.rows, a.autotune)
         auc_f = held_out_auc(B, trees_f, nb, init, bins_h, yh)
         log(f"fp32: {a.steps} iters in {dt_f:.3f}s, held-out AUC {auc_f:.5f}")
         fp32 = {"fit_s_500": round(dt_f / a.steps * FULL_ITERS, 3), "ms_per_step": round(dt_f / a.steps * 1e3, 3),
                 "held_out_auc": round(auc_f, 6), "speedup_quantized": round(dt_f / dt_q, 3),
-                "auc_delta_quantized": round(auc_q - auc_f, 6)}
+                "auc_delta_quantized": round(auc_q - auc_f, 6), "tuning": tuned_f}
     ctx.close()
 
     cpu = None
@@ -201,6 +209,8 @@ def main() -> int:
             "held_out_auc": round(auc_q, 6),
             "fp32": fp32,
             "hist_GBps": round(hist_gbps, 1),
+            "tuning": tuned_q,
+            "rows_per_s": round(a.rows * a.steps / dt_q, 1),
         },
         "roofline": {
             "bound": "hbm", "kernel": "k_hist (quantized)",

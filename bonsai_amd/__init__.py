@@ -52,6 +52,11 @@ class FitConfigC(C.Structure):
     ]
 
 
+class TuningC(C.Structure):
+    _fields_ = [("scatter_lds_bytes", C.c_uint32), ("find_blocks", C.c_uint32), ("hist_threads", C.c_uint32),
+                ("part_grid", C.c_uint32)]
+
+
 class ServeConfigC(C.Structure):
     _fields_ = [("n_slots", C.c_uint32), ("do_shap", C.c_uint32), ("max_batch_rows", C.c_uint64),
                 ("hbm_budget", C.c_uint64)]
@@ -148,6 +153,9 @@ def lib() -> C.CDLL:
         "bh_profile_hist": [P, u32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(u64),
                             C.POINTER(u64)],
         "bh_get_timing": [P, P, u32],
+        "bh_set_tuning": [P, C.POINTER(TuningC)],
+        "bh_get_tuning": [P, C.POINTER(TuningC)],
+        "bh_autotune": [P, u32, C.POINTER(TuningC), C.POINTER(C.c_float)],
         "bh_predictor_create": [C.c_int, P, P, u32, P, u32, C.c_float, C.c_float, C.POINTER(P)],
         "bh_predictor_destroy": [P],
         "bh_predict": [P, P, u64, u64, P],
@@ -356,6 +364,24 @@ class HipContext:
         _check(lib().bh_partition(self._h, _ptr(rows), rows.size, feature, bin_, int(default_left),
                                   _ptr(out), C.byref(nl)))
         return out, nl.value
+
+    def tuning(self) -> dict:
+        t = TuningC()
+        _check(lib().bh_get_tuning(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in TuningC._fields_}
+
+    def set_tuning(self, **kw) -> None:
+        cur = self.tuning()
+        cur.update(kw)
+        t = TuningC(**cur)
+        _check(lib().bh_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, iters: int = 5) -> tuple[dict, float]:
+        """Pick launch shapes by timing the configured fit; model state is unchanged."""
+        t = TuningC()
+        ms = C.c_float(0)
+        _check(lib().bh_autotune(self._h, iters, C.byref(t), C.byref(ms)))
+        return {k: getattr(t, k) for k, _ in TuningC._fields_}, ms.value
 
     def profile_hist(self, n_iters: int) -> dict:
         """Per-launch hipEvent timing of every histogram launch over n_iters iterations."""

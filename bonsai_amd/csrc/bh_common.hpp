@@ -132,6 +132,14 @@ struct TreeCtl {
 };
 static_assert(sizeof(TreeCtl) == 128, "TreeCtl");
 
+// Launch-shape knobs of one context, picked by the autotuner (bh_autotune).
+struct Tuning {
+    u32 scatter_lds = 20u * 1024u; // LDS bytes per partition-scatter workgroup (sets its tile rows)
+    u32 find_blocks = kFindMaxBlocks; // workgroups sharing one node's split find (1..kFindMaxBlocks)
+    u32 hist_threads = 1024;     // quantized histogram workgroup size (256, 512 or 1024)
+    u32 part_grid = 2048;        // partition count / scatter workgroups
+};
+
 // Per-node partial split records and completion counters of a multi-block split find
 // (capacity: front indices x kFindMaxBlocks records; counters start and end at 0).
 struct FindScratch {

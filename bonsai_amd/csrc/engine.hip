@@ -73,6 +73,7 @@ struct bh_context {
     NodeInfo *front[2] = {nullptr, nullptr};
     SplitRec *splits = nullptr;
     FindScratch fscr{nullptr, nullptr};
+    Tuning tune;
     DevNode *tree = nullptr;
     PartOp *ops = nullptr;
     PartChunk *pch = nullptr;
@@ -314,7 +315,7 @@ void enqueue_build(bh_context *c, void *hist_cur, void *hist_next, u32 max_ops, 
 }
 
 u32 hist_grid(bh_context *c) { return std::max(1u, std::min(c->max_hch, 4096u)); }
-u32 part_grid(bh_context *c) { return std::max(1u, std::min(c->max_pch, 2048u)); }
+u32 part_grid(bh_context *c) { return std::max(1u, std::min(c->max_pch, c->tune.part_grid)); }
 
 // DepthwiseGrower::grow (src/grower_impl.hpp:501-577) as a device launch sequence.
 void enqueue_depthwise(bh_context *c)
@@ -383,6 +384,7 @@ int check_ready(bh_context *c)
     if (!c) return fail(1, "null context");
     if (!c->plane0) return fail(1, "no bins uploaded");
     if (!c->configured) return fail(1, "not configured");
+    set_tuning(c->tune);
     return 0;
 }
 
@@ -737,6 +739,123 @@ int bh_train(bh_context *c, uint32_t n_iters)
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->trees_done += n_iters;
+    return 0;
+}
+
+static void drop_graph(bh_context *c)
+{
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+}
+
+static int validate_tuning(bh_tuning const *t)
+{
+    if (t->scatter_lds_bytes < 4096u || t->scatter_lds_bytes > 65536u)
+        return fail(1, "scatter_lds_bytes must be in [4096, 65536]");
+    if (t->find_blocks < 1u || t->find_blocks > kFindMaxBlocks) return fail(1, "find_blocks must be in [1, 8]");
+    if (t->hist_threads != 256u && t->hist_threads != 512u && t->hist_threads != 1024u)
+        return fail(1, "hist_threads must be 256, 512 or 1024");
+    if (t->part_grid < 64u || t->part_grid > 65536u) return fail(1, "part_grid must be in [64, 65536]");
+    return 0;
+}
+
+int bh_set_tuning(bh_context *c, const bh_tuning *t)
+{
+    if (!c) return fail(1, "null context");
+    if (int r = validate_tuning(t)) return r;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->tune = Tuning{t->scatter_lds_bytes, t->find_blocks, t->hist_threads, t->part_grid};
+    drop_graph(c); // the iteration graph is re-captured with the new shapes
+    return 0;
+}
+
+int bh_get_tuning(bh_context *c, bh_tuning *t)
+{
+    if (!c) return fail(1, "null context");
+    *t = bh_tuning{c->tune.scatter_lds, c->tune.find_blocks, c->tune.hist_threads, c->tune.part_grid};
+    return 0;
+}
+
+// Coordinate descent over the launch-shape knobs on the resident data and the
+// configured fit: each trial restores the scores and the tree state, runs one warm
+// iteration (graph capture) and times `iters` iterations with hipEvents.  The model
+// state after the call is the one before it.
+int bh_autotune(bh_context *c, uint32_t iters, bh_tuning *chosen, float *ms_per_iter)
+{
+    if (int r = check_ready(c)) return r;
+    if (!c->have_labels) return fail(1, "no labels uploaded");
+    if (iters == 0) return fail(1, "iters must be positive");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float *saved = nullptr;
+    HIP_TRY(dalloc(&saved, c->N));
+    TreeCtl saved_ctl;
+    hipError_t e = hipMemcpy(saved, c->score, c->N * sizeof(float), hipMemcpyDeviceToDevice);
+    if (e == hipSuccess) e = hipMemcpy(&saved_ctl, c->ctl, sizeof(TreeCtl), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        (void)hipFree(saved);
+        return fail(-1, std::string("hip: ") + hipGetErrorString(e));
+    }
+    u32 const saved_trees = c->trees_done;
+    bool const saved_timing = c->timing;
+    c->timing = false;
+    auto restore = [&]() -> int {
+        HIP_TRY(hipMemcpy(c->score, saved, c->N * sizeof(float), hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(c->ctl, &saved_ctl, sizeof(TreeCtl), hipMemcpyHostToDevice));
+        c->trees_done = saved_trees;
+        return 0;
+    };
+    auto trial = [&](Tuning const &t, float *ms) -> int {
+        c->tune = t;
+        drop_graph(c);
+        if (int r = restore()) return r;
+        if (int r = bh_train(c, 1)) return r;
+        if (int r = restore()) return r;
+        HIP_TRY(hipEventRecord(c->ev0, c->stream));
+        if (int r = bh_train(c, iters)) return r;
+        HIP_TRY(hipEventRecord(c->ev1, c->stream));
+        HIP_TRY(hipEventSynchronize(c->ev1));
+        float total = 0.0F;
+        HIP_TRY(hipEventElapsedTime(&total, c->ev0, c->ev1));
+        *ms = total / static_cast<float>(iters);
+        return 0;
+    };
+    Tuning best = c->tune;
+    float best_ms = 0.0F;
+    int rc = trial(best, &best_ms);
+    static const u32 lds_c[] = {8192u, 12288u, 16384u, 20480u, 32768u, 49152u};
+    static const u32 thr_c[] = {256u, 512u, 1024u};
+    static const u32 fb_c[] = {1u, 2u, 4u, 8u};
+    static const u32 pg_c[] = {512u, 1024u, 2048u, 4096u};
+    for (int knob = 0; rc == 0 && knob < 4; ++knob) {
+        u32 const *cand = knob == 0 ? lds_c : knob == 1 ? thr_c : knob == 2 ? fb_c : pg_c;
+        int const nc = knob == 0 ? 6 : knob == 1 ? 3 : 4;
+        for (int k = 0; rc == 0 && k < nc; ++k) {
+            Tuning t = best;
+            u32 &slot = knob == 0 ? t.scatter_lds : knob == 1 ? t.hist_threads : knob == 2 ? t.find_blocks : t.part_grid;
+            if (slot == cand[k]) continue;
+            slot = cand[k];
+            float ms = 0.0F;
+            rc = trial(t, &ms);
+            if (rc == 0 && ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        }
+    }
+    c->tune = best;
+    drop_graph(c);
+    int const rr = restore();
+    (void)hipFree(saved);
+    c->timing = saved_timing;
+    if (rc) return rc;
+    if (rr) return rr;
+    set_tuning(c->tune);
+    if (chosen) *chosen = bh_tuning{best.scatter_lds, best.find_blocks, best.hist_threads, best.part_grid};
+    if (ms_per_iter) *ms_per_iter = best_ms;
     return 0;
 }
 

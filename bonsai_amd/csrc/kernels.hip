@@ -565,7 +565,10 @@ __global__ void k_finish(FinishOp const *__restrict__ ops, TreeCtl const *__rest
 // ---------------------------------------------------------------------------------
 constexpr u32 kFindTile = 7;
 
-u32 find_blocks_per_node(u32 F) { return std::min<u32>(kFindMaxBlocks, (F + kFindTile - 1) / kFindTile); }
+u32 find_blocks_per_node(u32 F)
+{
+    return std::max(1u, std::min<u32>(std::min(tuning().find_blocks, kFindMaxBlocks), (F + kFindTile - 1) / kFindTile));
+}
 
 template <bool Q> struct CellT;
 template <> struct CellT<false> { typedef float2 T; typedef double A; };
@@ -1491,7 +1494,12 @@ __device__ __forceinline__ WalkNode to_walk(DevNode const &d)
     return w;
 }
 
-template <int DMAX>
+// Score epilogue (H12, route_add_kernel, kernels.cuh:1140-1163): every row walks the
+// tree from the pristine plane, score += lr * leaf (one rounding each), optionally the
+// leaf id.  STAGED (rows of <= 64 bytes): a block's 256 rows are read as one contiguous
+// byte range with 16-byte loads into LDS and each lane walks its row from there;
+// otherwise (wide rows) the lane reads only the bins its path touches.
+template <bool STAGED>
 __global__ __launch_bounds__(256) void k_score_update(u8 const *__restrict__ plane, u32 Fs, u64 n,
                                                       DevNode const *__restrict__ tree,
                                                       TreeCtl const *__restrict__ ctl,
@@ -1500,35 +1508,30 @@ __global__ __launch_bounds__(256) void k_score_update(u8 const *__restrict__ pla
 {
     __shared__ WalkNode nodes[kWalkLds];
     __shared__ u16 last[kWalkLds];
+    __shared__ __align__(16) uint4 s_rows[STAGED ? (256u * 64u + 16u) / 16u : 1u];
     u32 const nn = ctl->n_nodes;
     u32 const nl = min(nn, kWalkLds);
     for (u32 i = threadIdx.x; i < nl; i += blockDim.x) nodes[i] = to_walk(tree[i]);
     for (u32 f = threadIdx.x; f < F && f < kWalkLds; f += blockDim.x) last[f] = static_cast<u16>(n_bins[f] - 1u);
     __syncthreads();
-    for (u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; r < n;
-         r += static_cast<u64>(gridDim.x) * blockDim.x) {
-        u8 const *row = plane + r * Fs;
-        // Narrow rows: the whole row in registers, bins extracted by select (DMAX > 0).
-        u32 rw[DMAX > 0 ? DMAX : 1];
-        if (DMAX > 0) {
-            u32 const D = Fs / 4u;
-#pragma unroll
-            for (int d = 0; d < (DMAX > 0 ? DMAX : 1); ++d)
-                rw[d] = static_cast<u32>(d) < D ? reinterpret_cast<u32 const *>(row)[d] : 0u;
+    for (u64 r0 = blockIdx.x * static_cast<u64>(blockDim.x); r0 < n; r0 += static_cast<u64>(gridDim.x) * blockDim.x) {
+        u64 const r = r0 + threadIdx.x;
+        u8 const *row;
+        if (STAGED) {
+            u32 const rows = static_cast<u32>(min<u64>(blockDim.x, n - r0));
+            __syncthreads(); // previous tile walked
+            u32 const sh = lds_load_range(plane, r0 * Fs, rows * Fs, s_rows);
+            __syncthreads();
+            row = reinterpret_cast<u8 const *>(s_rows) + sh + threadIdx.x * Fs;
+        } else {
+            row = plane + r * Fs;
         }
+        if (r >= n) continue;
         u32 idx = 0;
         WalkNode w = nodes[0];
         while (w.feature != 0xFFFFu) {
             u32 const f = w.feature;
-            u32 b;
-            if (DMAX > 0) {
-                u32 sel = 0;
-#pragma unroll
-                for (int d = 0; d < (DMAX > 0 ? DMAX : 1); ++d) sel = (static_cast<u32>(d) == (f >> 2)) ? rw[d] : sel;
-                b = (sel >> ((f & 3u) * 8u)) & 0xFFu;
-            } else {
-                b = row[f];
-            }
+            u32 const b = row[f];
             u32 const lb = f < kWalkLds ? last[f] : static_cast<u32>(n_bins[f]) - 1u;
             idx = routes_left(b, lb, w.bin, w.default_left) ? w.left : w.right;
             w = idx < kWalkLds ? nodes[idx] : to_walk(tree[idx]);
@@ -1634,11 +1637,14 @@ __global__ __launch_bounds__(256) void k_copy_calibrate(uint4 const *__restrict_
 // Launch wrappers (host side).
 // ---------------------------------------------------------------------------------
 // Scatter tile sizing: rows per LDS tile = budget / (row + gradient + 4 index bytes),
-// a multiple of 64 in [64, kScatTileMax].  The budget is a tuning knob.
+// a multiple of 64 in [64, kScatTileMax].
 constexpr u32 kScatTileMax = 1024;
-static u32 g_scatter_lds_budget = 20u * 1024u;
-u32 scatter_lds_budget() { return g_scatter_lds_budget; }
-void set_scatter_lds_budget(u32 bytes) { g_scatter_lds_budget = std::max(4096u, std::min(bytes, 64u * 1024u)); }
+
+// Launch-shape knobs in force for the work the calling thread enqueues (each context
+// activates its own before enqueueing; graphs keep the shapes they were captured with).
+static thread_local Tuning t_tune;
+void set_tuning(Tuning const &t) { t_tune = t; }
+Tuning const &tuning() { return t_tune; }
 
 void launch_copy_calibrate(hipStream_t s, void const *src, void *dst, u64 bytes)
 {
@@ -1684,8 +1690,8 @@ hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl c
     size_t const lds = static_cast<size_t>(fsl) * SB * 8;
     dim3 const grid(grid_x, slices), block(64 * n_waves);
     if (q) {
-        hipLaunchKernelGGL(k_hist_q, grid, dim3(kHqThreads), lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp, Fs, F,
-                           SB, fsl, partials, hist_next);
+        hipLaunchKernelGGL(k_hist_q, grid, dim3(t_tune.hist_threads), lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp,
+                           Fs, F, SB, fsl, partials, hist_next);
     } else
         hipLaunchKernelGGL((k_hist<false, 4>), grid, block, lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp, Fs, F,
                            SB, fsl, partials, hist_next);
@@ -1781,7 +1787,7 @@ void launch_part_scatter(hipStream_t s, PartChunk const *pch, PartOp const *ops,
     u32 const D = Fs / 4;
     if (D <= 16) {
         u32 const per_row = Fs + gh_bytes + 4u;
-        u32 tile = (scatter_lds_budget() - 32u) / per_row;
+        u32 tile = (t_tune.scatter_lds - 32u) / per_row;
         tile = std::max(64u, std::min<u32>(kScatTileMax, tile & ~63u));
         size_t const lds = static_cast<size_t>(tile) * per_row + 32u;
         u32 const G = gh_bytes / 4u;
@@ -1819,11 +1825,11 @@ void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode 
                          u16 const *n_bins, u32 F, float lr, float *score, u32 *leaf_of_row)
 {
     if (Fs <= 64)
-        hipLaunchKernelGGL(k_score_update<16>, dim3(2048), dim3(256), 0, s, plane, Fs, n, tree, ctl, n_bins, F, lr,
+        hipLaunchKernelGGL(k_score_update<true>, dim3(2048), dim3(256), 0, s, plane, Fs, n, tree, ctl, n_bins, F, lr,
                            score, leaf_of_row);
     else
-        hipLaunchKernelGGL(k_score_update<0>, dim3(2048), dim3(256), 0, s, plane, Fs, n, tree, ctl, n_bins, F, lr,
-                           score, leaf_of_row);
+        hipLaunchKernelGGL(k_score_update<false>, dim3(2048), dim3(256), 0, s, plane, Fs, n, tree, ctl, n_bins, F,
+                           lr, score, leaf_of_row);
 }
 
 void launch_iter_advance(hipStream_t s, TreeCtl *ctl)

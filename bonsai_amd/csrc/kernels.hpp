@@ -41,8 +41,8 @@ void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode 
 void launch_iter_advance(hipStream_t s, TreeCtl *ctl);
 void launch_copy_tree(hipStream_t s, DevNode const *tree, TreeCtl const *ctl, DevNode *out_trees, u32 *out_sizes,
                       u32 max_nodes);
-u32 scatter_lds_budget();
-void set_scatter_lds_budget(u32 bytes);
+void set_tuning(Tuning const &t);
+Tuning const &tuning();
 void launch_copy_calibrate(hipStream_t s, void const *src, void *dst, u64 bytes);
 void launch_bin_rows(hipStream_t s, float const *X, u64 n, u32 F, u32 Fs, float const *cuts, u32 const *coff,
                      u32 n_cuts_total, u8 *plane);

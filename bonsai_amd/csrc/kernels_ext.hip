@@ -954,13 +954,13 @@ void enqueue_levelwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
             hipLaunchKernelGGL(k_plan_oblivious<false>, dim3(1), dim3(1024), 0, st, cur, v.ctl, pick, hcur,
                                v.d_nbins, v.F, v.SB, v.splits, v.ops, v.pch, d & 1u, last ? 1u : 0u);
         launch_part_count(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.Fs, v.chunk_left,
-                          std::max(1u, std::min(v.max_pch, 2048u)));
+                          std::max(1u, std::min(v.max_pch, tuning().part_grid)));
         launch_part_scan(st, q, v.ops, v.pch, v.chunk_left, v.ctl, cur, nxt, v.splits, v.tree, v.hch, v.fops, v.gc,
                          v.scales, last ? 1u : 0u, 1u);
         if (!last) {
             launch_part_scatter(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2],
                                 v.work[0], v.work[1], const_cast<void *>(v.gh[0]), const_cast<void *>(v.gh[1]), v.Fs,
-                                gh_bytes, std::max(1u, std::min(v.max_pch, 2048u)));
+                                gh_bytes, std::max(1u, std::min(v.max_pch, tuning().part_grid)));
             launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F,
                         v.SB, v.fsl, v.waves, std::max(1u, std::min(v.max_hch, 4096u)), v.partials, hnext);
             launch_finish(st, q, v.fops, v.ctl, v.partials, hcur, hnext, v.F * v.SB, std::min(width, v.max_ops));
@@ -987,7 +987,7 @@ void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
     NodeInfo *parent = v.front[1];
     SplitRec *psplit = s.feat_best;
     void *pool = v.hist[0];
-    u32 const pgrid = std::max(1u, std::min(v.max_pch, 2048u));
+    u32 const pgrid = std::max(1u, std::min(v.max_pch, tuning().part_grid));
     u32 const hgrid = std::max(1u, std::min(v.max_hch, 4096u));
     hipLaunchKernelGGL(k_lw_begin, dim3(1), dim3(256), 0, st, v.ctl, lw, kids, v.tree, v.hch, v.fops, s.free_slots,
                        s.hist_slots, static_cast<u32>(v.N));

@@ -174,6 +174,22 @@ int bh_profile_hist(bh_context *ctx, uint32_t n_iters, double *ms_total, double 
 int bh_set_timing(bh_context *ctx, int32_t enable);
 int bh_get_timing(bh_context *ctx, float *ms, uint32_t n);
 
+/* ---- launch-shape tuning (no counterpart in the reference, whose launch shapes are
+ *      fixed constants, src/cuda/detail/device_context.cu:451-497) ---- */
+typedef struct bh_tuning {
+    uint32_t scatter_lds_bytes; /* LDS per partition-scatter workgroup: sets its tile rows */
+    uint32_t find_blocks;       /* workgroups sharing one node's split find (1..8) */
+    uint32_t hist_threads;      /* quantized histogram workgroup size: 256, 512 or 1024 */
+    uint32_t part_grid;         /* partition workgroups */
+} bh_tuning;
+/* Set / read the context's launch shapes (the iteration graph is re-captured). */
+int bh_set_tuning(bh_context *ctx, const bh_tuning *t);
+int bh_get_tuning(bh_context *ctx, bh_tuning *t);
+/* Autotuner: coordinate descent over the knobs above on the resident data and the
+ * configured fit, `iters` timed boosting iterations per trial (hipEvents).  Scores,
+ * trees and the iteration counter are restored afterwards; the best shapes stay set. */
+int bh_autotune(bh_context *ctx, uint32_t iters, bh_tuning *chosen, float *ms_per_iter);
+
 /* ---- predict and TreeSHAP (cuda_predict_plan / cuda_predict,
  *      src/cuda/predict.cu:147-266; cuda_shap_plan / cuda_pred_contribs,
  *      src/cuda/shap.cu:212-380) ---- */
