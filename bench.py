@@ -34,6 +34,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "fit s/500 iters @16M×28 d8 (FP32 vs quantized, equal AUC); hist HBM GB/s"
+# Histogram HBM traffic per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs
+# (calibrated against a copy of known size), with the algorithmic bytes of the same launches.
+TRAFFIC = ROOT / "profiles" / "hist_quantized_16Mx28_traffic.json"
 HBM_PEAK_GBPS = 8000.0
 FULL_ITERS = 500
 
@@ -188,6 +191,12 @@ def main() -> int:
         cpu = cpu_baseline(X, y, cuts, a.rows, rows, a.cpu_iters)
         log(f"cpu baseline: {cpu['value']} s extrapolated on {cpu['cores']} threads")
 
+    traffic = traffic_ratio = None
+    if TRAFFIC.exists() and a.rows == 16_000_000:
+        t = json.loads(TRAFFIC.read_text())
+        traffic = t["traffic_bytes_per_launch"]
+        traffic_ratio = round(t["traffic_bytes_per_launch"] / t["algorithmic_bytes_per_launch"], 3)
+
     out = {
         "metric": METRIC,
         "value": round(dt_q / a.steps * FULL_ITERS, 4),
@@ -219,7 +228,9 @@ def main() -> int:
             "bytes_per_launch": round(per_launch_bytes), "avg_launch_us": round(per_launch_us, 2),
             "launches_timed": prof["launches"],
             "copy_GBps": round(copy_gbps, 1),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_over_algorithmic": traffic_ratio,
+            "traffic_source": "profiles/hist_quantized_16Mx28_counters.md" if traffic else None,
         },
         "cpu_baseline": cpu,
     }

@@ -374,7 +374,7 @@ void enqueue_iteration(bh_context *c)
     enqueue_gradients(c);
     enqueue_grow(c);
     launch_score_update(c->stream, c->plane0, c->Fs, c->N, c->tree, c->ctl, c->d_nbins, c->F, c->fc.learning_rate,
-                        c->score, nullptr);
+                        c->score, nullptr, c->max_nodes);
     launch_copy_tree(c->stream, c->tree, c->ctl, c->trees_out, c->tree_sizes, c->max_nodes);
     launch_iter_advance(c->stream, c->ctl);
 }
@@ -932,7 +932,7 @@ int bh_grow_tree(bh_context *c, bh_node *nodes, uint32_t cap, uint32_t *n_nodes,
     HIP_TRY(hipSetDevice(c->device));
     enqueue_grow(c);
     launch_score_update(c->stream, c->plane0, c->Fs, c->N, c->tree, c->ctl, c->d_nbins, c->F, 0.0F, nullptr,
-                        c->leaf_rows);
+                        c->leaf_rows, c->max_nodes);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     TreeCtl ctl;

@@ -1504,15 +1504,19 @@ __global__ __launch_bounds__(256) void k_score_update(u8 const *__restrict__ pla
                                                       DevNode const *__restrict__ tree,
                                                       TreeCtl const *__restrict__ ctl,
                                                       u16 const *__restrict__ n_bins, u32 F, float lr,
-                                                      float *__restrict__ score, u32 *__restrict__ leaf_of_row)
+                                                      float *__restrict__ score, u32 *__restrict__ leaf_of_row,
+                                                      u32 node_cap, u32 last_cap)
 {
-    __shared__ WalkNode nodes[kWalkLds];
-    __shared__ u16 last[kWalkLds];
-    __shared__ __align__(16) uint4 s_rows[STAGED ? (256u * 64u + 16u) / 16u : 1u];
+    // LDS: [node_cap WalkNode][last_cap u16][rows: 256 * Fs + 16, 16-byte aligned]
+    extern __shared__ __align__(16) unsigned char smem[];
+    WalkNode *nodes = reinterpret_cast<WalkNode *>(smem);
+    u16 *last = reinterpret_cast<u16 *>(smem + node_cap * sizeof(WalkNode));
+    uint4 *s_rows = reinterpret_cast<uint4 *>(
+        smem + ((node_cap * sizeof(WalkNode) + last_cap * sizeof(u16) + 15u) & ~static_cast<size_t>(15)));
     u32 const nn = ctl->n_nodes;
-    u32 const nl = min(nn, kWalkLds);
+    u32 const nl = min(nn, node_cap);
     for (u32 i = threadIdx.x; i < nl; i += blockDim.x) nodes[i] = to_walk(tree[i]);
-    for (u32 f = threadIdx.x; f < F && f < kWalkLds; f += blockDim.x) last[f] = static_cast<u16>(n_bins[f] - 1u);
+    for (u32 f = threadIdx.x; f < F && f < last_cap; f += blockDim.x) last[f] = static_cast<u16>(n_bins[f] - 1u);
     __syncthreads();
     for (u64 r0 = blockIdx.x * static_cast<u64>(blockDim.x); r0 < n; r0 += static_cast<u64>(gridDim.x) * blockDim.x) {
         u64 const r = r0 + threadIdx.x;
@@ -1528,13 +1532,13 @@ __global__ __launch_bounds__(256) void k_score_update(u8 const *__restrict__ pla
         }
         if (r >= n) continue;
         u32 idx = 0;
-        WalkNode w = nodes[0];
+        WalkNode w = nl ? nodes[0] : to_walk(tree[0]);
         while (w.feature != 0xFFFFu) {
             u32 const f = w.feature;
             u32 const b = row[f];
-            u32 const lb = f < kWalkLds ? last[f] : static_cast<u32>(n_bins[f]) - 1u;
+            u32 const lb = f < last_cap ? last[f] : static_cast<u32>(n_bins[f]) - 1u;
             idx = routes_left(b, lb, w.bin, w.default_left) ? w.left : w.right;
-            w = idx < kWalkLds ? nodes[idx] : to_walk(tree[idx]);
+            w = idx < nl ? nodes[idx] : to_walk(tree[idx]);
         }
         if (score) score[r] = __fadd_rn(score[r], __fmul_rn(lr, w.value));
         if (leaf_of_row) leaf_of_row[r] = idx;
@@ -1822,14 +1826,17 @@ void launch_tree_begin(hipStream_t s, TreeCtl *ctl, NodeInfo *front, DevNode *tr
 }
 
 void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode const *tree, TreeCtl const *ctl,
-                         u16 const *n_bins, u32 F, float lr, float *score, u32 *leaf_of_row)
+                         u16 const *n_bins, u32 F, float lr, float *score, u32 *leaf_of_row, u32 max_nodes)
 {
+    u32 const node_cap = std::min(max_nodes, kWalkLds);
+    u32 const last_cap = std::min(F, kWalkLds);
+    size_t const head = (node_cap * sizeof(WalkNode) + last_cap * sizeof(u16) + 15u) & ~static_cast<size_t>(15);
     if (Fs <= 64)
-        hipLaunchKernelGGL(k_score_update<true>, dim3(2048), dim3(256), 0, s, plane, Fs, n, tree, ctl, n_bins, F, lr,
-                           score, leaf_of_row);
+        hipLaunchKernelGGL(k_score_update<true>, dim3(2048), dim3(256), head + 256u * Fs + 16u, s, plane, Fs, n, tree,
+                           ctl, n_bins, F, lr, score, leaf_of_row, node_cap, last_cap);
     else
-        hipLaunchKernelGGL(k_score_update<false>, dim3(2048), dim3(256), 0, s, plane, Fs, n, tree, ctl, n_bins, F,
-                           lr, score, leaf_of_row);
+        hipLaunchKernelGGL(k_score_update<false>, dim3(2048), dim3(256), head, s, plane, Fs, n, tree, ctl, n_bins, F,
+                           lr, score, leaf_of_row, node_cap, last_cap);
 }
 
 void launch_iter_advance(hipStream_t s, TreeCtl *ctl)

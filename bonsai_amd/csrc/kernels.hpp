@@ -37,7 +37,7 @@ void launch_next_level(hipStream_t s, TreeCtl *ctl);
 void launch_tree_begin(hipStream_t s, TreeCtl *ctl, NodeInfo *front, DevNode *tree, HistChunk *hch,
                        FinishOp *fops, u32 n_rows);
 void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode const *tree, TreeCtl const *ctl,
-                         u16 const *n_bins, u32 F, float lr, float *score, u32 *leaf_of_row);
+                         u16 const *n_bins, u32 F, float lr, float *score, u32 *leaf_of_row, u32 max_nodes);
 void launch_iter_advance(hipStream_t s, TreeCtl *ctl);
 void launch_copy_tree(hipStream_t s, DevNode const *tree, TreeCtl const *ctl, DevNode *out_trees, u32 *out_sizes,
                       u32 max_nodes);
