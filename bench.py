@@ -71,11 +71,15 @@ def make_data(n_rows: int, n_held: int):
     return X, y, Xh, yh, cuts
 
 
-def fit(B, ctx, quantized: bool, steps: int, warmup: int, init: float, n_rows: int, autotune: int):
+def fit(B, ctx, quantized: bool, steps: int, warmup: int, init: float, n_rows: int, autotune: int,
+        shapes: dict | None = None):
     ctx.configure(B.FitConfig(grower="hip_depthwise", objective="logloss", quantized=quantized,
                               use_graph=True, max_depth=8, learning_rate=0.1, lambda_l2=1.0))
     start = np.full(n_rows, init, np.float32)
     tuned = None
+    if shapes:
+        ctx.set_tuning(**shapes)
+        tuned = {"shapes": ctx.tuning(), "fixed": True}
     if autotune:
         ctx.set_scores(start)
         t0 = time.perf_counter()
@@ -130,6 +134,7 @@ def main() -> int:
     ap.add_argument("--held-out", type=int, default=1_600_000)
     ap.add_argument("--prof-iters", type=int, default=3, help="eager iterations timed per histogram launch")
     ap.add_argument("--autotune", type=int, default=5, help="iterations per autotuner trial (0: default shapes)")
+    ap.add_argument("--tuning", default="", help="fixed launch shapes, e.g. scatter_lds_bytes=49152,find_blocks=4")
     ap.add_argument("--skip-fp32", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=4_000_000)
@@ -166,7 +171,8 @@ def main() -> int:
     ctx.upload_labels(y)
     log(f"plane binned on the device in {time.perf_counter() - t:.2f}s")
 
-    dt_q, trees_q, tuned_q = fit(B, ctx, True, a.steps, a.warmup, init, a.rows, a.autotune)
+    shapes = {k: int(v) for k, v in (kv.split("=") for kv in a.tuning.split(",") if kv)} or None
+    dt_q, trees_q, tuned_q = fit(B, ctx, True, a.steps, a.warmup, init, a.rows, 0 if shapes else a.autotune, shapes)
     auc_q = held_out_auc(B, trees_q, nb, init, bins_h, yh)
     log(f"quantized: {a.steps} iters in {dt_q:.3f}s, held-out AUC {auc_q:.5f}")
     prof = ctx.profile_hist(a.prof_iters)
@@ -177,7 +183,8 @@ def main() -> int:
 
     fp32 = None
     if not a.skip_fp32:
-        dt_f, trees_f, tuned_f = fit(B, ctx, False, a.steps, a.warmup, init, a.rows, a.autotune)
+        dt_f, trees_f, tuned_f = fit(B, ctx, False, a.steps, a.warmup, init, a.rows, 0 if shapes else a.autotune,
+                                     shapes)
         auc_f = held_out_auc(B, trees_f, nb, init, bins_h, yh)
         log(f"fp32: {a.steps} iters in {dt_f:.3f}s, held-out AUC {auc_f:.5f}")
         fp32 = {"fit_s_500": round(dt_f / a.steps * FULL_ITERS, 3), "ms_per_step": round(dt_f / a.steps * 1e3, 3),

@@ -78,8 +78,9 @@ SPLIT_DTYPE = np.dtype([
     ("default_left", np.uint32), ("valid", np.uint32),
     ("gL", np.float64), ("hL", np.float64), ("gR", np.float64), ("hR", np.float64),
     ("igL", np.int64), ("ihL", np.int64), ("igR", np.int64), ("ihR", np.int64),
+    ("n_left", np.uint32), ("n_right", np.uint32),
 ])
-assert NODE_DTYPE.itemsize == 32 and SPLIT_DTYPE.itemsize == 88
+assert NODE_DTYPE.itemsize == 32 and SPLIT_DTYPE.itemsize == 96
 
 
 @dataclass

@@ -9,6 +9,7 @@
 //   ghw0/1   gradient pairs travelling with the partitioned rows
 //   hist[2]  node histograms of the current / next level, [slot][F][SB][2] cells
 //            (f32 pairs, or i64 pairs when quantized)
+//   cnt[2]   quantized fits: row count per (slot, feature, bin), u32 [slot][F][SB]
 #pragma once
 #include <stdint.h>
 
@@ -22,8 +23,16 @@ typedef int64_t  i64;
 
 constexpr u32 kNone       = 0xFFFFFFFFu;
 constexpr u32 kSrcPlane0  = 2; // segment lives in the pristine plane
-constexpr u32 kHistChunk  = 16384; // rows per histogram work unit (fixed: part of the
-                                   // FP32 summation order)
+// Rows per histogram work unit.  Fixed: it is part of the FP32 summation order (the
+// oracle's chunked restatement uses the same size), and it bounds the packed quantized
+// cells, which hold one unit's sums in 64 bits: h in bits [0, 28), the row count in
+// [28, 42), signed g in [42, 64) (h <= 32767 * 8192 < 2^28, count <= 2^13,
+// |g| <= 127 * 8192 < 2^21).
+constexpr u32 kHistChunk  = 8192;
+constexpr u32 kPackCntShift = 28;
+constexpr u32 kPackGShift = 42;
+constexpr u64 kPackHMask = (1ull << 28) - 1ull;
+constexpr u64 kPackCntMask = (1ull << 14) - 1ull;
 constexpr u32 kPartChunk  = 4096;  // rows per partition work unit
 constexpr u32 kMaxTreeNodes = 8192;
 // Tail padding of planes (bytes) and gradient buffers (elements): kernels that read
@@ -32,6 +41,7 @@ constexpr size_t kTailPadBytes = 64;
 // Split find: up to this many workgroups share one node's feature tiles.
 constexpr u32 kFindMaxBlocks = 8;
 constexpr u32 kTailPad = 16;
+
 
 // Tree node in bin space, the layout the host reads back (feature < 0: leaf).
 struct DevNode {
@@ -67,8 +77,9 @@ struct SplitRec {
     u32    feature, bin, default_left, valid;
     double gL, hL, gR, hR;     // FP32 path child sums (double)
     i64    igL, ihL, igR, ihR; // quantized path child sums (exact)
+    u32    n_left, n_right;    // child row counts (quantized finder, from the count cells)
 };
-static_assert(sizeof(SplitRec) == 88, "SplitRec");
+static_assert(sizeof(SplitRec) == 96, "SplitRec");
 
 // A histogram work unit: rows [begin, end) of one node's segment.
 struct HistChunk {
@@ -128,7 +139,8 @@ struct TreeCtl {
     u32 sel;            // leafwise: selected candidate index
     u32 max_depth_seen;
     u32 oblivious_depth; // levelwise: levels split so far
-    u32 pad[15];
+    u32 part_error;     // a partition chunk's prefix disagreed with its op's row counts
+    u32 pad[14];
 };
 static_assert(sizeof(TreeCtl) == 128, "TreeCtl");
 

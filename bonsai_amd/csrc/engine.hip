@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -73,6 +74,7 @@ struct bh_context {
     NodeInfo *front[2] = {nullptr, nullptr};
     SplitRec *splits = nullptr;
     FindScratch fscr{nullptr, nullptr};
+    u32 *cnt[2] = {nullptr, nullptr}; // quantized depthwise: row count per (slot, feature, bin)
     Tuning tune;
     DevNode *tree = nullptr;
     PartOp *ops = nullptr;
@@ -129,7 +131,7 @@ void free_fit(bh_context *c)
     void *ptrs[] = {c->gh0f, c->ghwf[0], c->ghwf[1], c->gh0q, c->ghwq[0], c->ghwq[1], c->maxbits, c->scales,
                     c->hist[0], c->hist[1], c->partials, c->ctl, c->front[0], c->front[1], c->splits, c->tree,
                     c->ops, c->pch, c->chunk_left, c->hch, c->fops, c->trees_out, c->tree_sizes, c->leaf_rows,
-                    c->fscr.part, c->fscr.done};
+                    c->fscr.part, c->fscr.done, c->cnt[0], c->cnt[1]};
     for (void *p : ptrs)
         if (p) hipFree(p);
     c->gh0f = c->ghwf[0] = c->ghwf[1] = nullptr;
@@ -150,6 +152,7 @@ void free_fit(bh_context *c)
     c->tree_sizes = nullptr;
     c->leaf_rows = nullptr;
     c->fscr = FindScratch{nullptr, nullptr};
+    c->cnt[0] = c->cnt[1] = nullptr;
     ext_free(c->ext);
     c->configured = false;
     c->trees_cap = c->trees_done = 0;
@@ -193,6 +196,11 @@ int alloc_fit(bh_context *c)
     HIP_TRY(hipMalloc(&c->hist[0], hist_bytes));
     if (f.grower != BH_LEAFWISE) HIP_TRY(hipMalloc(&c->hist[1], hist_bytes));
     HIP_TRY(hipMalloc(&c->partials, static_cast<size_t>(c->max_parts) * nc * c->part_bytes()));
+    if (c->quant && f.grower == BH_DEPTHWISE) {
+        size_t const cells = static_cast<size_t>(c->hist_slots) * nc;
+        HIP_TRY(dalloc(&c->cnt[0], cells));
+        HIP_TRY(dalloc(&c->cnt[1], cells));
+    }
     HIP_TRY(dalloc(&c->ctl, 1));
     HIP_TRY(hipMemset(c->ctl, 0, sizeof(TreeCtl)));
     u32 const fcap = std::max(c->max_front * 2u, 4u);
@@ -270,12 +278,12 @@ hipError_t host_sync_point(bh_context *c)
 // algorithmic bytes: rows x (F + gradient pair bytes) read + one 8-byte cell per
 // (built node, feature, bin) written.  Rows are physically compacted per node, so no
 // row-index bytes are read.
-void profiled_hist(bh_context *c, void *hist_next, u32 grid_units)
+void profiled_hist(bh_context *c, void *hist_next, u32 grid_units, u32 *cnt_next)
 {
     (void)hipEventRecord(c->ev0, c->stream);
     launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
                 gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), grid_units,
-                c->partials, hist_next);
+                c->partials, hist_next, cnt_next);
     (void)hipEventRecord(c->ev1, c->stream);
     (void)hipEventSynchronize(c->ev1);
     float ms = 0.0F;
@@ -303,23 +311,66 @@ void profiled_hist(bh_context *c, void *hist_next, u32 grid_units)
 }
 
 // Histogram units -> partials / slots, then the fixed-order completion.
-void enqueue_build(bh_context *c, void *hist_cur, void *hist_next, u32 max_ops, u32 grid_units)
+void enqueue_build(bh_context *c, void *hist_cur, void *hist_next, u32 max_ops, u32 grid_units,
+                   u32 const *cnt_cur = nullptr, u32 *cnt_next = nullptr)
 {
     if (c->prof)
-        profiled_hist(c, hist_next, grid_units);
+        profiled_hist(c, hist_next, grid_units, cnt_next);
     else
         launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
                     gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), grid_units,
-                    c->partials, hist_next);
-    launch_finish(c->stream, c->quant, c->fops, c->ctl, c->partials, hist_cur, hist_next, c->node_cells(), max_ops);
+                    c->partials, hist_next, cnt_next);
+    launch_finish(c->stream, c->quant, c->fops, c->ctl, c->partials, hist_cur, hist_next, c->node_cells(), max_ops,
+                  cnt_cur, cnt_next);
 }
 
 u32 hist_grid(bh_context *c) { return std::max(1u, std::min(c->max_hch, 4096u)); }
 u32 part_grid(bh_context *c) { return std::max(1u, std::min(c->max_pch, c->tune.part_grid)); }
 
+// Quantized depthwise: the histograms carry row counts, so each level is find (child
+// counts from the count cells) -> plan (one workgroup: tree, demotion, next frontier,
+// work lists) -> per-chunk left counts -> scatter (chunk prefixes summed in-kernel) ->
+// histograms of the smaller children.  The last level moves no rows and reads none.
+void enqueue_depthwise_counted(bh_context *c)
+{
+    hipStream_t s = c->stream;
+    u32 const D = static_cast<u32>(c->fc.max_depth);
+    u32 const gh_bytes = 4;
+    launch_tree_begin(s, c->ctl, c->front[0], c->tree, c->hch, c->fops, static_cast<u32>(c->N));
+    enqueue_build(c, c->hist[1], c->hist[0], 1, hist_grid(c), c->cnt[1], c->cnt[0]);
+    for (u32 d = 0; d < D; ++d) {
+        NodeInfo *cur = c->front[d & 1u], *nxt = c->front[(d + 1) & 1u];
+        void *hcur = c->hist[d & 1u], *hnext = c->hist[(d + 1) & 1u];
+        u32 *ccur = c->cnt[d & 1u], *cnext = c->cnt[(d + 1) & 1u];
+        u32 const width = 1u << std::min(d, 14u);
+        bool const last = d + 1 == D;
+        launch_find(s, true, cur, c->ctl, hcur, c->d_nbins, c->F, c->SB, c->gc, c->scales, c->splits, width, 0, 0,
+                    c->fscr, ccur);
+        launch_plan_counted(s, cur, c->splits, c->ctl, c->tree, c->ops, c->pch, nxt, c->hch, c->fops, c->d_nbins,
+                            c->gc, c->scales, d & 1u, last ? 1u : 0u, kPartChunk);
+        if (c->fc.host_sync) host_sync_point(c);
+        if (!last) {
+            launch_part_count(s, c->pch, c->ops, c->ctl, c->work[0], c->work[1], c->plane0, c->Fs, c->chunk_left,
+                              part_grid(c));
+            launch_part_scatter(s, c->pch, c->ops, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
+                                gh_src(c, 1), gh_src(c, 2), c->work[0], c->work[1],
+                                const_cast<void *>(gh_src(c, 0)), const_cast<void *>(gh_src(c, 1)), c->Fs,
+                                gh_bytes, part_grid(c), c->chunk_left);
+            enqueue_build(c, hcur, hnext, std::min(width, c->max_ops), hist_grid(c), ccur, cnext);
+        }
+        launch_next_level(s, c->ctl);
+        if (c->fc.host_sync) host_sync_point(c);
+    }
+    launch_finalize(s, c->front[D & 1u], c->ctl, c->tree, c->gc, c->scales, 1u << std::min(D, 14u));
+}
+
 // DepthwiseGrower::grow (src/grower_impl.hpp:501-577) as a device launch sequence.
 void enqueue_depthwise(bh_context *c)
 {
+    if (c->quant) {
+        enqueue_depthwise_counted(c);
+        return;
+    }
     hipStream_t s = c->stream;
     u32 const D = static_cast<u32>(c->fc.max_depth);
     u32 const gh_bytes = c->quant ? 4 : 8;
@@ -377,6 +428,21 @@ void enqueue_iteration(bh_context *c)
                         c->score, nullptr, c->max_nodes);
     launch_copy_tree(c->stream, c->tree, c->ctl, c->trees_out, c->tree_sizes, c->max_nodes);
     launch_iter_advance(c->stream, c->ctl);
+}
+
+// The counted scatter reports a chunk whose prefix disagrees with its op's row counts
+// through the control block instead of writing outside the segment; surface it as a
+// device error.
+int check_partition(bh_context *c)
+{
+    if (!c->cnt[0]) return 0;
+    TreeCtl t;
+    HIP_TRY(hipMemcpy(&t, c->ctl, sizeof(TreeCtl), hipMemcpyDeviceToHost));
+    if (t.part_error == 0) return 0;
+    u32 const zero = 0;
+    HIP_TRY(hipMemcpy(reinterpret_cast<unsigned char *>(c->ctl) + offsetof(TreeCtl, part_error), &zero, sizeof(u32),
+                      hipMemcpyHostToDevice));
+    return fail(-2, "partition: chunk prefix inconsistent with the split's row counts");
 }
 
 int check_ready(bh_context *c)
@@ -738,6 +804,7 @@ int bh_train(bh_context *c, uint32_t n_iters)
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (int r = check_partition(c)) return r;
     c->trees_done += n_iters;
     return 0;
 }
@@ -752,8 +819,8 @@ static void drop_graph(bh_context *c)
 
 static int validate_tuning(bh_tuning const *t)
 {
-    if (t->scatter_lds_bytes < 4096u || t->scatter_lds_bytes > 65536u)
-        return fail(1, "scatter_lds_bytes must be in [4096, 65536]");
+    if (t->scatter_lds_bytes < 4096u || t->scatter_lds_bytes > 162816u)
+        return fail(1, "scatter_lds_bytes must be in [4096, 162816]");
     if (t->find_blocks < 1u || t->find_blocks > kFindMaxBlocks) return fail(1, "find_blocks must be in [1, 8]");
     if (t->hist_threads != 256u && t->hist_threads != 512u && t->hist_threads != 1024u)
         return fail(1, "hist_threads must be 256, 512 or 1024");
@@ -826,13 +893,13 @@ int bh_autotune(bh_context *c, uint32_t iters, bh_tuning *chosen, float *ms_per_
     Tuning best = c->tune;
     float best_ms = 0.0F;
     int rc = trial(best, &best_ms);
-    static const u32 lds_c[] = {8192u, 12288u, 16384u, 20480u, 32768u, 49152u};
+    static const u32 lds_c[] = {8192u, 12288u, 20480u, 32768u, 49152u, 98304u, 147456u};
     static const u32 thr_c[] = {256u, 512u, 1024u};
     static const u32 fb_c[] = {1u, 2u, 4u, 8u};
     static const u32 pg_c[] = {512u, 1024u, 2048u, 4096u};
     for (int knob = 0; rc == 0 && knob < 4; ++knob) {
         u32 const *cand = knob == 0 ? lds_c : knob == 1 ? thr_c : knob == 2 ? fb_c : pg_c;
-        int const nc = knob == 0 ? 6 : knob == 1 ? 3 : 4;
+        int const nc = knob == 0 ? 7 : knob == 1 ? 3 : 4;
         for (int k = 0; rc == 0 && k < nc; ++k) {
             Tuning t = best;
             u32 &slot = knob == 0 ? t.scatter_lds : knob == 1 ? t.hist_threads : knob == 2 ? t.find_blocks : t.part_grid;
@@ -935,6 +1002,7 @@ int bh_grow_tree(bh_context *c, bh_node *nodes, uint32_t cap, uint32_t *n_nodes,
                         c->leaf_rows, c->max_nodes);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (int r = check_partition(c)) return r;
     TreeCtl ctl;
     HIP_TRY(hipMemcpy(&ctl, c->ctl, sizeof(TreeCtl), hipMemcpyDeviceToHost));
     if (ctl.n_nodes > cap) return fail(1, "node capacity too small");
@@ -1103,6 +1171,7 @@ int bh_profile_hist(bh_context *c, uint32_t n_iters, double *ms_total, double *b
     c->prof = false;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (int r = check_partition(c)) return r;
     c->trees_done += n_iters;
     *ms_total = c->prof_ms;
     *bytes_total = c->prof_bytes;

@@ -379,11 +379,23 @@ __global__ __launch_bounds__(1024) void k_hist(
 constexpr u32 kHqThreads = 1024;
 constexpr int kHqUnroll = 4;
 
+// One row's packed contribution (see kPackGShift): the gradient word carries int16 g in
+// its high half and uint16 h in its low half.
+__device__ __forceinline__ u64 pack_q(u32 v)
+{
+    return static_cast<u64>(static_cast<i64>(static_cast<int16_t>(v >> 16)) * (1ll << kPackGShift)) +
+           (1ull << kPackCntShift) + (v & 0xFFFFu);
+}
+__device__ __forceinline__ i64 unpack_g(u64 v) { return static_cast<i64>(v) >> kPackGShift; }
+__device__ __forceinline__ i64 unpack_h(u64 v) { return static_cast<i64>(v & kPackHMask); }
+__device__ __forceinline__ u32 unpack_cnt(u64 v) { return static_cast<u32>((v >> kPackCntShift) & kPackCntMask); }
+
 __global__ __launch_bounds__(kHqThreads) void k_hist_q(
     HistChunk const *__restrict__ chunks, TreeCtl const *__restrict__ ctl,
     u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
     void const *__restrict__ gw0, void const *__restrict__ gw1, void const *__restrict__ gwp,
-    u32 Fs, u32 F, u32 SB, u32 fsl, void *__restrict__ partials, void *__restrict__ hist_next)
+    u32 Fs, u32 F, u32 SB, u32 fsl, void *__restrict__ partials, void *__restrict__ hist_next,
+    u32 *__restrict__ cnt_next)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned long long *lh = reinterpret_cast<unsigned long long *>(smem);   // fsl x SB
@@ -420,8 +432,7 @@ __global__ __launch_bounds__(kHqThreads) void k_hist_q(
                 }
 #pragma unroll
                 for (int u = 0; u < kHqUnroll; ++u) {
-                    u64 const pk = static_cast<u64>(static_cast<i64>(static_cast<int16_t>(v[u] >> 16)) *
-                                                    4294967296LL) + (v[u] & 0xFFFFu);
+                    u64 const pk = pack_q(v[u]);
 #pragma unroll
                     for (u32 k = 0; k < 4; ++k) {
                         u32 const fl = 4u * d + k;
@@ -434,8 +445,7 @@ __global__ __launch_bounds__(kHqThreads) void k_hist_q(
             for (; r < ch.end; r += stride) {
                 u32 const dw1 = *reinterpret_cast<u32 const *>(cb + static_cast<size_t>(r) * Fs);
                 u32 const v1 = ghv[r];
-                u64 const pk = static_cast<u64>(static_cast<i64>(static_cast<int16_t>(v1 >> 16)) * 4294967296LL) +
-                               (v1 & 0xFFFFu);
+                u64 const pk = pack_q(v1);
 #pragma unroll
                 for (u32 k = 0; k < 4; ++k) {
                     u32 const fl = 4u * d + k;
@@ -450,12 +460,13 @@ __global__ __launch_bounds__(kHqThreads) void k_hist_q(
                        static_cast<size_t>(f_begin) * SB;
             for (u32 i = threadIdx.x; i < cells; i += blockDim.x) dst[i] = lh[i];
         } else {
-            i64 *dst = reinterpret_cast<i64 *>(hist_next) +
-                       (static_cast<size_t>(ch.slot) * node_cells + static_cast<size_t>(f_begin) * SB) * 2;
+            size_t const base = static_cast<size_t>(ch.slot) * node_cells + static_cast<size_t>(f_begin) * SB;
+            i64 *dst = reinterpret_cast<i64 *>(hist_next) + base * 2;
             for (u32 i = threadIdx.x; i < cells; i += blockDim.x) {
                 u64 const v = lh[i];
-                dst[2 * i] = static_cast<int32_t>(static_cast<u32>(v >> 32));
-                dst[2 * i + 1] = static_cast<i64>(v & 0xFFFFFFFFull);
+                dst[2 * i] = unpack_g(v);
+                dst[2 * i + 1] = unpack_h(v);
+                if (cnt_next) cnt_next[base + i] = unpack_cnt(v);
             }
         }
         __syncthreads();
@@ -468,7 +479,8 @@ __global__ __launch_bounds__(kHqThreads) void k_hist_q(
 template <bool Q>
 __global__ void k_finish(FinishOp const *__restrict__ ops, TreeCtl const *__restrict__ ctl,
                          void const *__restrict__ partials, void const *__restrict__ hist_cur,
-                         void *__restrict__ hist_next, u32 node_cells)
+                         void *__restrict__ hist_next, u32 node_cells, u32 const *__restrict__ cnt_cur,
+                         u32 *__restrict__ cnt_next)
 {
     u32 const n_ops = ctl->n_finish;
     for (u32 o = blockIdx.y; o < n_ops; o += gridDim.y) {
@@ -476,14 +488,19 @@ __global__ void k_finish(FinishOp const *__restrict__ ops, TreeCtl const *__rest
         for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < node_cells; i += gridDim.x * blockDim.x) {
             if (Q) {
                 i64 *nx = reinterpret_cast<i64 *>(hist_next);
+                size_t const si = static_cast<size_t>(op.small_slot) * node_cells + i;
                 i64 g = 0, h = 0;
+                u32 n = 0;
                 if (op.empty) {
-                    nx[(static_cast<size_t>(op.small_slot) * node_cells + i) * 2] = 0;
-                    nx[(static_cast<size_t>(op.small_slot) * node_cells + i) * 2 + 1] = 0;
+                    nx[si * 2] = 0;
+                    nx[si * 2 + 1] = 0;
+                    if (cnt_next) cnt_next[si] = 0;
                 } else if (op.part_count == 0) {
-                    g = nx[(static_cast<size_t>(op.small_slot) * node_cells + i) * 2];
-                    h = nx[(static_cast<size_t>(op.small_slot) * node_cells + i) * 2 + 1];
+                    g = nx[si * 2];
+                    h = nx[si * 2 + 1];
+                    if (cnt_next) n = cnt_next[si];
                 } else {
+                    // Partials hold packed unit sums: unpack each before adding.
                     u64 const *pp = reinterpret_cast<u64 const *>(partials) + i;
                     u32 p = 0;
                     for (; p + 8 <= op.part_count; p += 8) {
@@ -492,24 +509,28 @@ __global__ void k_finish(FinishOp const *__restrict__ ops, TreeCtl const *__rest
                         for (int k = 0; k < 8; ++k) v[k] = pp[static_cast<size_t>(op.part_first + p + k) * node_cells];
 #pragma unroll
                         for (int k = 0; k < 8; ++k) {
-                            g += static_cast<int32_t>(static_cast<u32>(v[k] >> 32));
-                            h += static_cast<i64>(v[k] & 0xFFFFFFFFull);
+                            g += unpack_g(v[k]);
+                            h += unpack_h(v[k]);
+                            n += unpack_cnt(v[k]);
                         }
                     }
                     for (; p < op.part_count; ++p) {
                         u64 const v = pp[static_cast<size_t>(op.part_first + p) * node_cells];
-                        g += static_cast<int32_t>(static_cast<u32>(v >> 32));
-                        h += static_cast<i64>(v & 0xFFFFFFFFull);
+                        g += unpack_g(v);
+                        h += unpack_h(v);
+                        n += unpack_cnt(v);
                     }
-                    nx[(static_cast<size_t>(op.small_slot) * node_cells + i) * 2] = g;
-                    nx[(static_cast<size_t>(op.small_slot) * node_cells + i) * 2 + 1] = h;
+                    nx[si * 2] = g;
+                    nx[si * 2 + 1] = h;
+                    if (cnt_next) cnt_next[si] = n;
                 }
                 if (op.large_slot != kNone) {
                     i64 const *pc = reinterpret_cast<i64 const *>(hist_cur);
-                    size_t const pi = (static_cast<size_t>(op.parent_slot) * node_cells + i) * 2;
-                    size_t const li = (static_cast<size_t>(op.large_slot) * node_cells + i) * 2;
-                    nx[li] = pc[pi] - g;
-                    nx[li + 1] = pc[pi + 1] - h;
+                    size_t const pi = static_cast<size_t>(op.parent_slot) * node_cells + i;
+                    size_t const li = static_cast<size_t>(op.large_slot) * node_cells + i;
+                    nx[li * 2] = pc[pi * 2] - g;
+                    nx[li * 2 + 1] = pc[pi * 2 + 1] - h;
+                    if (cnt_next) cnt_next[li] = cnt_cur[pi] - n;
                 }
             } else {
                 float2 *nx = reinterpret_cast<float2 *>(hist_next);
@@ -579,7 +600,8 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
                                               void const *__restrict__ hist, u16 const *__restrict__ n_bins,
                                               u32 F, u32 SB, GrowCfg cfg, double const *__restrict__ scales,
                                               SplitRec *__restrict__ out, u32 front_base, u32 n_front_override,
-                                              SplitRec *__restrict__ part, u32 *__restrict__ done)
+                                              SplitRec *__restrict__ part, u32 *__restrict__ done,
+                                              u32 const *__restrict__ cnt)
 {
     typedef typename CellT<Q>::T Cell;
     typedef typename CellT<Q>::A Acc;
@@ -868,11 +890,11 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
         }
         __syncthreads();
     }
-    // The thread that scored the winner publishes its child sums (keys are unique):
-    // straight to out[i] with one block per node, else to the block's partial record;
-    // the node's last block to finish merges the partials (gain, then lowest key).
+    // The thread that scored the winner publishes its child sums (keys are unique) to
+    // the block's partial record; the node's last block to finish merges the partials
+    // (gain, then lowest key) and, with count cells, derives the child row counts.
     u32 const key = s_best_key[0];
-    SplitRec *dst = G == 1 ? out + i : part + static_cast<size_t>(i) * G + bx;
+    SplitRec *dst = part + static_cast<size_t>(i) * G + bx;
     if (key != kNone && best_key == key) {
         SplitRec rec{};
         u32 const f = key / (2u * SB), b = (key >> 1) % SB, d = key & 1u;
@@ -897,30 +919,44 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
         *dst = rec;
         __threadfence();
     }
-    if (G == 1) {
-        if (threadIdx.x == 0 && !had_totals) front[i] = node;
-        return;
-    }
     __syncthreads();
     if (threadIdx.x == 0) s_last = atomicAdd(done + i, 1u) == G - 1u ? 1u : 0u;
     __syncthreads();
-    if (s_last == 0u || threadIdx.x != 0) return;
-    __threadfence();
-    SplitRec best{};
-    best.default_left = 1;
-    u32 bkey = kNone;
-    for (u32 k = 0; k < G; ++k) {
-        SplitRec const r = part[static_cast<size_t>(i) * G + k];
-        if (!r.valid) continue;
-        u32 const rk = r.feature * (2u * SB) + r.bin * 2u + (r.default_left ? 0u : 1u);
-        if (bkey == kNone || r.gain > best.gain || (r.gain == best.gain && rk < bkey)) {
-            best = r;
-            bkey = rk;
+    if (s_last == 0u) return;
+    __shared__ SplitRec s_win;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        SplitRec best{};
+        best.default_left = 1;
+        u32 bkey = kNone;
+        for (u32 k = 0; k < G; ++k) {
+            SplitRec const r = part[static_cast<size_t>(i) * G + k];
+            if (!r.valid) continue;
+            u32 const rk = r.feature * (2u * SB) + r.bin * 2u + (r.default_left ? 0u : 1u);
+            if (bkey == kNone || r.gain > best.gain || (r.gain == best.gain && rk < bkey)) {
+                best = r;
+                bkey = rk;
+            }
         }
+        s_win = best;
+        done[i] = 0u;
+        if (!had_totals) front[i] = node;
     }
-    out[i] = best;
-    if (!had_totals) front[i] = node;
-    done[i] = 0u;
+    __syncthreads();
+    if (threadIdx.x >= 64) return;
+    SplitRec w = s_win;
+    if (cnt && w.valid) {
+        // Left rows: bins <= split bin, plus the missing bin when it routes left.
+        u32 const *cc = cnt + static_cast<size_t>(node.slot) * node_cells + static_cast<size_t>(w.feature) * SB;
+        u32 const last = n_bins[w.feature] - 1u;
+        u32 nl = 0;
+        for (u32 b = threadIdx.x; b <= w.bin; b += 64) nl += cc[b];
+        if (threadIdx.x == 0 && w.default_left) nl += cc[last];
+        for (int o = 32; o > 0; o >>= 1) nl += __shfl_xor(nl, o);
+        w.n_left = nl;
+        w.n_right = node.cnt - nl;
+    }
+    if (threadIdx.x == 0) out[i] = w;
 }
 
 // ---------------------------------------------------------------------------------
@@ -1222,12 +1258,35 @@ __global__ __launch_bounds__(1024) void k_part_scan(
 // follow at off + n_left + ....  Rows are contiguous per thread (16 each) so the
 // block-wide exclusive scan of per-thread left counts gives every row its stable rank;
 // the row bytes and the gradient pair are then copied with coalesced dword moves.
+// Left rows of an op's chunks first..c-1, summed by the whole block from their counts
+// (uses wsum[0 .. waves)).  Every thread receives the total.
+__device__ u32 chunk_prefix(u32 const *__restrict__ chunk_left, u32 first, u32 c, u32 *wsum)
+{
+    u32 part = 0;
+    for (u32 j = first + threadIdx.x; j < c; j += blockDim.x) part += chunk_left[j];
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+    if ((threadIdx.x & 63u) == 0) wsum[threadIdx.x >> 6] = part;
+    __syncthreads();
+    u32 total = 0;
+    for (u32 k = 0; k < blockDim.x / 64u; ++k) total += wsum[k];
+    __syncthreads();
+    return total;
+}
+
+// A chunk's left and right runs both inside its op's segment.
+__device__ __forceinline__ bool chunk_fits(PartOp const &op, PartChunk const &ch, u32 left_before, u32 n_left_chunk)
+{
+    u32 const rows = ch.end - ch.begin;
+    return left_before <= ch.begin && left_before + n_left_chunk <= op.n_left && op.n_left <= op.cnt &&
+           (ch.begin - left_before) + (rows - n_left_chunk) <= op.cnt - op.n_left;
+}
+
 __global__ __launch_bounds__(256) void k_part_scatter(
     PartChunk const *__restrict__ pch, PartOp const *__restrict__ ops, TreeCtl const *__restrict__ ctl,
     u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
     void const *__restrict__ gw0, void const *__restrict__ gw1, void const *__restrict__ gwp,
     u8 *__restrict__ out0, u8 *__restrict__ out1, void *__restrict__ go0, void *__restrict__ go1, u32 Fs,
-    u32 gh_bytes, u32 tpr_log2)
+    u32 gh_bytes, u32 tpr_log2, u32 const *__restrict__ chunk_left, TreeCtl *ctl_err)
 {
     __shared__ u32 dest[kPartChunk];
     __shared__ u32 scratch[256];
@@ -1255,8 +1314,16 @@ __global__ __launch_bounds__(256) void k_part_scatter(
         }
         u32 tot;
         u32 const pre = block_exscan(__popc(bits), scratch, &tot);
-        u32 lpos = op.off + ch.left_before + pre;
-        u32 rpos = op.off + op.n_left + (ch.begin + r0 - ch.left_before - pre);
+        u32 left_before = ch.left_before;
+        if (chunk_left) {
+            left_before = chunk_prefix(chunk_left, op.chunk_first, c, scratch + 64);
+            if (!chunk_fits(op, ch, left_before, tot)) {
+                if (threadIdx.x == 0) atomicExch(&ctl_err->part_error, 1u);
+                continue; // uniform across the block
+            }
+        }
+        u32 lpos = op.off + left_before + pre;
+        u32 rpos = op.off + op.n_left + (ch.begin + r0 - left_before - pre);
         for (u32 k = 0; k < RPT; ++k) {
             u32 const r = r0 + k;
             if (r < rows) dest[r] = ((bits >> k) & 1u) ? lpos++ : rpos++;
@@ -1288,7 +1355,7 @@ __global__ __launch_bounds__(256) void k_part_scatter(
 // contiguous destination range written with aligned 16-byte stores (dword head and tail
 // at the run's edges).  Planes and gradient buffers carry 64 bytes of tail padding, so
 // the aligned over-read past a tile's last row stays inside the allocation.
-constexpr u32 kScatDivShift = 21;   // k / D as (k * ceil(2^21 / D)) >> 21: exact for k < 2^14, D <= 16
+// k / D as (k * ceil(2^32 / D)) >> 32 in 64-bit arithmetic: exact for k < 2^16, D <= 16.
 
 __device__ __forceinline__ u32 lds_load_range(u8 const *__restrict__ src, size_t a, u32 len, uint4 *dst)
 {
@@ -1301,20 +1368,25 @@ __device__ __forceinline__ u32 lds_load_range(u8 const *__restrict__ src, size_t
 }
 
 // dst[first + k] = src[base + idx[k / D] * D + k % D] for k < n (dword units).
-__device__ __forceinline__ void store_run(u32 *__restrict__ dst, size_t first, u32 n, u32 D, u32 magic,
+__device__ __forceinline__ u32 div_magic(u32 k, u64 magic)
+{
+    return static_cast<u32>((static_cast<u64>(k) * magic) >> 32);
+}
+
+__device__ __forceinline__ void store_run(u32 *__restrict__ dst, size_t first, u32 n, u32 D, u64 magic,
                                           u32 const *src, u32 base, u16 const *idx)
 {
     u32 const head = min(n, static_cast<u32>((4u - (first & 3u)) & 3u));
     if (threadIdx.x < head) {
         u32 const k = threadIdx.x;
-        u32 const q = (k * magic) >> kScatDivShift;
+        u32 const q = div_magic(k, magic);
         dst[first + k] = src[base + static_cast<u32>(idx[q]) * D + (k - q * D)];
     }
     u32 const n_vec = (n - head) >> 2;
     uint4 *dv = reinterpret_cast<uint4 *>(dst + first + head);
     for (u32 m = threadIdx.x; m < n_vec; m += blockDim.x) {
         u32 const k = head + 4u * m;
-        u32 q = (k * magic) >> kScatDivShift;
+        u32 q = div_magic(k, magic);
         u32 rem = k - q * D;
         u32 v[4];
 #pragma unroll
@@ -1330,7 +1402,7 @@ __device__ __forceinline__ void store_run(u32 *__restrict__ dst, size_t first, u
     u32 const t0 = head + 4u * n_vec;
     if (threadIdx.x < n - t0) {
         u32 const k = t0 + threadIdx.x;
-        u32 const q = (k * magic) >> kScatDivShift;
+        u32 const q = div_magic(k, magic);
         dst[first + k] = src[base + static_cast<u32>(idx[q]) * D + (k - q * D)];
     }
 }
@@ -1340,7 +1412,7 @@ __global__ __launch_bounds__(256) void k_part_scatter_v(
     u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
     void const *__restrict__ gw0, void const *__restrict__ gw1, void const *__restrict__ gwp,
     u8 *__restrict__ out0, u8 *__restrict__ out1, void *__restrict__ go0, void *__restrict__ go1, u32 Fs,
-    u32 gh_bytes, u32 tile, u32 magic_d, u32 magic_g)
+    u32 gh_bytes, u32 tile, u64 magic_d, u64 magic_g, u32 const *__restrict__ chunk_left, TreeCtl *ctl_err)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     uint4 *s_rows = reinterpret_cast<uint4 *>(smem);                        // tile * Fs + 16 bytes
@@ -1360,6 +1432,15 @@ __global__ __launch_bounds__(256) void k_part_scatter_v(
         u32 *oplane = reinterpret_cast<u32 *>(op.dst == 0 ? out0 : out1);
         u32 *gout = static_cast<u32 *>(op.dst == 0 ? go0 : go1);
         u32 left_run = ch.left_before; // left rows of the op before the current tile
+        if (chunk_left) {
+            // Prefix over the op's earlier chunks straight from their counts; every run
+            // of this chunk must land inside the op's segment.
+            left_run = chunk_prefix(chunk_left, op.chunk_first, c, s_wsum);
+            if (!chunk_fits(op, ch, left_run, chunk_left[c])) {
+                if (threadIdx.x == 0) atomicExch(&ctl_err->part_error, 1u);
+                continue; // uniform across the block
+            }
+        }
         for (u32 t0 = ch.begin; t0 < ch.end; t0 += tile) {
             u32 const rows = min(tile, ch.end - t0);
             size_t const srow0 = static_cast<size_t>(op.off) + t0;
@@ -1402,6 +1483,182 @@ __global__ __launch_bounds__(256) void k_part_scatter_v(
             left_run += nl;
             __syncthreads();
         }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Counted quantized level (depthwise): the split finder already derived each split's
+// child row counts from the histogram count cells, so one workgroup plans the whole
+// level -- tree nodes (commit_split_node, src/grower_impl.hpp:188-210), empty-split
+// demotion (demote_empty_splits, :263-297: ids are committed first, as the reference
+// does), covers, partition ops with their chunks, the next frontier, the finish ops and
+// the histogram units of the smaller children.  Only the partition's per-chunk left
+// counts still need a pass over the rows (none at the last level, where nothing moves).
+// chunk_rows = rows per partition chunk.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_plan_counted(
+    NodeInfo const *__restrict__ front, SplitRec const *__restrict__ splits, TreeCtl *__restrict__ ctl,
+    DevNode *__restrict__ tree, PartOp *__restrict__ ops, PartChunk *__restrict__ pch, NodeInfo *__restrict__ next,
+    HistChunk *__restrict__ hch, FinishOp *__restrict__ fops, u16 const *__restrict__ n_bins, GrowCfg cfg,
+    double const *__restrict__ scales, u32 dst, u32 is_last, u32 chunk_rows)
+{
+    __shared__ u32 scratch[1024];
+    __shared__ u32 s_base_nodes, s_valid, s_alive, s_chunks, s_units, s_parts;
+    u32 const P = ctl->n_front;
+    if (threadIdx.x == 0) {
+        s_base_nodes = ctl->n_nodes;
+        s_valid = s_alive = s_chunks = s_units = s_parts = 0;
+    }
+    __syncthreads();
+    for (u32 t0 = 0; t0 < P; t0 += blockDim.x) {
+        u32 const i = t0 + threadIdx.x;
+        bool const in = i < P;
+        NodeInfo nd{};
+        SplitRec sp{};
+        if (in) {
+            nd = front[i];
+            sp = splits[i];
+        }
+        u32 const valid = in ? sp.valid : 0u;
+        u32 const nL = sp.n_left, nR = nd.cnt - sp.n_left;
+        bool const demote = valid && ((nL == 0) != (nR == 0));
+        u32 const alive = valid && !demote ? 1u : 0u;
+        u32 nch = 0, n_units = 0, multi = 0;
+        bool const left_small = nL <= nR;
+        u32 const small_cnt = left_small ? nL : nR;
+        if (alive && !is_last) {
+            nch = (nd.cnt + chunk_rows - 1) / chunk_rows;
+            n_units = (small_cnt + kHistChunk - 1) / kHistChunk;
+            multi = n_units > 1 ? n_units : 0;
+        }
+        u32 vtot, atot, ctot, htot, ptot;
+        u32 const vrank = block_exscan(valid, scratch, &vtot) + s_valid;
+        u32 const arank = block_exscan(alive, scratch, &atot) + s_alive;
+        u32 const cfirst = block_exscan(nch, scratch, &ctot) + s_chunks;
+        u32 const hfirst = block_exscan(n_units, scratch, &htot) + s_units;
+        u32 const pfirst = block_exscan(multi, scratch, &ptot) + s_parts;
+        if (in) {
+            if (valid) {
+                u32 const lid = s_base_nodes + 2 * vrank, rid = lid + 1;
+                DevNode leaf{};
+                leaf.feature = -1;
+                tree[lid] = leaf;
+                tree[rid] = leaf;
+                if (demote) {
+                    write_leaf(tree, nd.id, node_leaf_value(nd, cfg, scales));
+                    tree[nd.id].gain = 0.0F;
+                } else {
+                    DevNode pn = tree[nd.id];
+                    pn.feature = static_cast<int32_t>(sp.feature);
+                    pn.bin = sp.bin;
+                    pn.default_left = sp.default_left;
+                    pn.left = lid;
+                    pn.right = rid;
+                    pn.value = 0.0F;
+                    pn.gain = static_cast<float>(sp.gain);
+                    tree[nd.id] = pn;
+                    tree[lid].cover = static_cast<float>(nL);
+                    tree[rid].cover = static_cast<float>(nR);
+                    PartOp op{};
+                    op.parent = i;
+                    op.feature = sp.feature;
+                    op.bin = sp.bin;
+                    op.default_left = sp.default_left;
+                    op.last_bin = static_cast<u32>(n_bins[sp.feature]) - 1u;
+                    op.off = nd.off;
+                    op.cnt = nd.cnt;
+                    op.src = nd.src;
+                    op.dst = dst;
+                    op.chunk_first = cfirst;
+                    op.chunk_count = nch;
+                    op.n_left = nL;
+                    op.left_id = lid;
+                    op.right_id = rid;
+                    op.alive = 1;
+                    ops[arank] = op;
+                    NodeInfo L{}, R{};
+                    L.id = lid;
+                    R.id = rid;
+                    L.off = nd.off;
+                    L.cnt = nL;
+                    R.off = nd.off + nL;
+                    R.cnt = nR;
+                    L.src = R.src = dst;
+                    L.slot = 2 * arank;
+                    R.slot = 2 * arank + 1;
+                    L.depth = R.depth = nd.depth + 1;
+                    if (is_last) {
+                        L.totals_ready = R.totals_ready = 1;
+                        L.itg = sp.igL;
+                        L.ith = sp.ihL;
+                        R.itg = sp.igR;
+                        R.ith = sp.ihR;
+                    }
+                    next[2 * arank] = L;
+                    next[2 * arank + 1] = R;
+                    if (!is_last) {
+                        FinishOp fo{};
+                        fo.small_slot = left_small ? L.slot : R.slot;
+                        fo.large_slot = left_small ? R.slot : L.slot;
+                        fo.parent_slot = nd.slot;
+                        fo.part_first = multi ? pfirst : 0;
+                        fo.part_count = multi;
+                        fo.empty = small_cnt == 0 ? 1u : 0u;
+                        fo.unit_first = hfirst;
+                        fo.unit_count = n_units;
+                        fo.small_off = left_small ? L.off : R.off;
+                        fo.small_cnt = small_cnt;
+                        fo.src = dst;
+                        fops[arank] = fo;
+                    }
+                }
+            } else {
+                write_leaf(tree, nd.id, node_leaf_value(nd, cfg, scales));
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_valid += vtot;
+            s_alive += atot;
+            s_chunks += ctot;
+            s_units += htot;
+            s_parts += ptot;
+        }
+        __syncthreads();
+    }
+    u32 const n_alive = s_alive, n_ch = s_chunks, n_units_total = s_units;
+    for (u32 c = threadIdx.x; c < n_ch; c += blockDim.x) {
+        u32 const j = find_owner(ops, n_alive, c, [](PartOp const &o) { return o.chunk_first; });
+        PartOp const op = ops[j];
+        u32 const k = c - op.chunk_first;
+        PartChunk pc;
+        pc.op = j;
+        pc.begin = k * chunk_rows;
+        pc.end = min(op.cnt, (k + 1) * chunk_rows);
+        pc.left_before = 0;
+        pch[c] = pc;
+    }
+    for (u32 c = threadIdx.x; c < n_units_total; c += blockDim.x) {
+        u32 const r = find_owner(fops, n_alive, c, [](FinishOp const &o) { return o.unit_first; });
+        FinishOp const fo = fops[r];
+        u32 const k = c - fo.unit_first;
+        HistChunk hc{};
+        hc.slot = fo.small_slot;
+        hc.src = fo.src;
+        hc.begin = fo.small_off + k * kHistChunk;
+        hc.end = fo.small_off + min(fo.small_cnt, (k + 1) * kHistChunk);
+        hc.part = fo.part_count ? fo.part_first + k : kNone;
+        hch[c] = hc;
+    }
+    if (threadIdx.x == 0) {
+        ctl->n_nodes = s_base_nodes + 2 * s_valid;
+        ctl->n_ops = n_alive;
+        ctl->n_part_chunks = n_ch;
+        ctl->next_front = 2 * n_alive;
+        ctl->n_finish = is_last ? 0 : n_alive;
+        ctl->n_hist_chunks = is_last ? 0 : n_units_total;
+        ctl->n_parts = s_parts;
+        ctl->part_error = 0;
     }
 }
 
@@ -1642,7 +1899,7 @@ __global__ __launch_bounds__(256) void k_copy_calibrate(uint4 const *__restrict_
 // ---------------------------------------------------------------------------------
 // Scatter tile sizing: rows per LDS tile = budget / (row + gradient + 4 index bytes),
 // a multiple of 64 in [64, kScatTileMax].
-constexpr u32 kScatTileMax = 1024;
+constexpr u32 kScatTileMax = 4096;
 
 // Launch-shape knobs in force for the work the calling thread enqueues (each context
 // activates its own before enqueueing; graphs keep the shapes they were captured with).
@@ -1688,14 +1945,14 @@ void launch_quant_from(hipStream_t s, float2 const *gh, u64 n, u32 *maxbits, u64
 
 hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl const *ctl, u8 const *pl0,
                        u8 const *pl1, u8 const *plp, void const *g0, void const *g1, void const *gp, u32 Fs, u32 F,
-                       u32 SB, u32 fsl, u32 n_waves, u32 grid_x, void *partials, void *hist_next)
+                       u32 SB, u32 fsl, u32 n_waves, u32 grid_x, void *partials, void *hist_next, u32 *cnt_next)
 {
     u32 const slices = (F + fsl - 1) / fsl;
     size_t const lds = static_cast<size_t>(fsl) * SB * 8;
     dim3 const grid(grid_x, slices), block(64 * n_waves);
     if (q) {
         hipLaunchKernelGGL(k_hist_q, grid, dim3(t_tune.hist_threads), lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp,
-                           Fs, F, SB, fsl, partials, hist_next);
+                           Fs, F, SB, fsl, partials, hist_next, cnt_next);
     } else
         hipLaunchKernelGGL((k_hist<false, 4>), grid, block, lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp, Fs, F,
                            SB, fsl, partials, hist_next);
@@ -1731,31 +1988,32 @@ hipError_t set_hist_lds_limits()
 }
 
 void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl const *ctl, void const *partials,
-                   void const *hist_cur, void *hist_next, u32 node_cells, u32 max_ops)
+                   void const *hist_cur, void *hist_next, u32 node_cells, u32 max_ops, u32 const *cnt_cur,
+                   u32 *cnt_next)
 {
     u32 const tiles = min(64u, (node_cells + 255) / 256);
     dim3 const grid(tiles, max(1u, min(max_ops, 1024u)));
     if (q)
         hipLaunchKernelGGL(k_finish<true>, grid, dim3(256), 0, s, ops, ctl, partials, hist_cur, hist_next,
-                           node_cells);
+                           node_cells, cnt_cur, cnt_next);
     else
         hipLaunchKernelGGL(k_finish<false>, grid, dim3(256), 0, s, ops, ctl, partials, hist_cur, hist_next,
-                           node_cells);
+                           node_cells, nullptr, nullptr);
 }
 
 void launch_find(hipStream_t s, bool q, NodeInfo *front, TreeCtl const *ctl, void const *hist, u16 const *n_bins,
                  u32 F, u32 SB, GrowCfg const &cfg, double const *scales, SplitRec *out, u32 max_front,
-                 u32 front_base, u32 n_front_override, FindScratch const &fs)
+                 u32 front_base, u32 n_front_override, FindScratch const &fs, u32 const *cnt)
 {
     size_t const lds = static_cast<size_t>(kFindTile) * SB * (q ? 32 : 24);
     u32 const G = find_blocks_per_node(F);
     dim3 const grid(G, max_front);
     if (q)
         hipLaunchKernelGGL(k_find<true>, grid, dim3(256), lds, s, front, ctl, hist, n_bins, F, SB, cfg, scales, out,
-                           front_base, n_front_override, fs.part, fs.done);
+                           front_base, n_front_override, fs.part, fs.done, cnt);
     else
         hipLaunchKernelGGL(k_find<false>, grid, dim3(256), lds, s, front, ctl, hist, n_bins, F, SB, cfg, scales, out,
-                           front_base, n_front_override, fs.part, fs.done);
+                           front_base, n_front_override, fs.part, fs.done, nullptr);
 }
 
 void launch_plan_level(hipStream_t s, NodeInfo const *front, SplitRec const *splits, TreeCtl *ctl, DevNode *tree,
@@ -1786,7 +2044,8 @@ void launch_part_scan(hipStream_t s, bool q, PartOp *ops, PartChunk *pch, u32 co
 
 void launch_part_scatter(hipStream_t s, PartChunk const *pch, PartOp const *ops, TreeCtl const *ctl,
                          u8 const *pl0, u8 const *pl1, u8 const *plp, void const *g0, void const *g1,
-                         void const *gp, u8 *o0, u8 *o1, void *go0, void *go1, u32 Fs, u32 gh_bytes, u32 grid)
+                         void const *gp, u8 *o0, u8 *o1, void *go0, void *go1, u32 Fs, u32 gh_bytes, u32 grid,
+                         u32 const *chunk_left)
 {
     u32 const D = Fs / 4;
     if (D <= 16) {
@@ -1795,17 +2054,35 @@ void launch_part_scatter(hipStream_t s, PartChunk const *pch, PartOp const *ops,
         tile = std::max(64u, std::min<u32>(kScatTileMax, tile & ~63u));
         size_t const lds = static_cast<size_t>(tile) * per_row + 32u;
         u32 const G = gh_bytes / 4u;
-        u32 const magic_d = ((1u << kScatDivShift) + D - 1u) / D;
-        u32 const magic_g = ((1u << kScatDivShift) + G - 1u) / G;
+        u64 const magic_d = ((1ull << 32) + D - 1u) / D;
+        u64 const magic_g = ((1ull << 32) + G - 1u) / G;
         hipLaunchKernelGGL(k_part_scatter_v, dim3(grid), dim3(256), lds, s, pch, ops, ctl, pl0, pl1, plp, g0, g1,
-                           gp, o0, o1, go0, go1, Fs, gh_bytes, tile, magic_d, magic_g);
+                           gp, o0, o1, go0, go1, Fs, gh_bytes, tile, magic_d, magic_g, chunk_left,
+                           const_cast<TreeCtl *>(ctl));
         return;
     }
     u32 tl = 0;
     while ((1u << tl) < D && tl < 8) ++tl;
     hipLaunchKernelGGL(k_part_scatter, dim3(grid), dim3(256), 0, s, pch, ops, ctl, pl0, pl1, plp, g0, g1, gp, o0,
-                       o1, go0, go1, Fs, gh_bytes, tl);
+                       o1, go0, go1, Fs, gh_bytes, tl, chunk_left, const_cast<TreeCtl *>(ctl));
 }
+
+u32 scatter_tile_rows(u32 Fs, u32 gh_bytes)
+{
+    u32 const per_row = Fs + gh_bytes + 4u;
+    u32 tile = (t_tune.scatter_lds - 32u) / per_row;
+    return std::max(64u, std::min<u32>(kScatTileMax, tile & ~63u));
+}
+
+void launch_plan_counted(hipStream_t s, NodeInfo const *front, SplitRec const *splits, TreeCtl *ctl, DevNode *tree,
+                         PartOp *ops, PartChunk *pch, NodeInfo *next, HistChunk *hch, FinishOp *fops,
+                         u16 const *n_bins, GrowCfg const &cfg, double const *scales, u32 dst, u32 is_last,
+                         u32 chunk_rows)
+{
+    hipLaunchKernelGGL(k_plan_counted, dim3(1), dim3(1024), 0, s, front, splits, ctl, tree, ops, pch, next, hch, fops,
+                       n_bins, cfg, scales, dst, is_last, chunk_rows);
+}
+
 
 void launch_finalize(hipStream_t s, NodeInfo const *front, TreeCtl const *ctl, DevNode *tree, GrowCfg const &cfg,
                      double const *scales, u32 max_front)

@@ -12,13 +12,15 @@ void launch_quant_from(hipStream_t s, float2 const *gh, u64 n, u32 *maxbits, u64
                        double *scales_dev);
 hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl const *ctl, u8 const *pl0,
                        u8 const *pl1, u8 const *plp, void const *g0, void const *g1, void const *gp, u32 Fs, u32 F,
-                       u32 SB, u32 fsl, u32 n_waves, u32 grid_x, void *partials, void *hist_next);
+                       u32 SB, u32 fsl, u32 n_waves, u32 grid_x, void *partials, void *hist_next,
+                       u32 *cnt_next = nullptr);
 hipError_t set_hist_lds_limits();
 void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl const *ctl, void const *partials,
-                   void const *hist_cur, void *hist_next, u32 node_cells, u32 max_ops);
+                   void const *hist_cur, void *hist_next, u32 node_cells, u32 max_ops,
+                   u32 const *cnt_cur = nullptr, u32 *cnt_next = nullptr);
 void launch_find(hipStream_t s, bool q, NodeInfo *front, TreeCtl const *ctl, void const *hist, u16 const *n_bins,
                  u32 F, u32 SB, GrowCfg const &cfg, double const *scales, SplitRec *out, u32 max_front,
-                 u32 front_base, u32 n_front_override, FindScratch const &fs);
+                 u32 front_base, u32 n_front_override, FindScratch const &fs, u32 const *cnt = nullptr);
 u32 find_blocks_per_node(u32 F);
 void launch_plan_level(hipStream_t s, NodeInfo const *front, SplitRec const *splits, TreeCtl *ctl, DevNode *tree,
                        PartOp *ops, PartChunk *pch, u16 const *n_bins, GrowCfg const &cfg, double const *scales,
@@ -30,7 +32,13 @@ void launch_part_scan(hipStream_t s, bool q, PartOp *ops, PartChunk *pch, u32 co
                       FinishOp *fops, GrowCfg const &cfg, double const *scales, u32 is_last, u32 oblivious);
 void launch_part_scatter(hipStream_t s, PartChunk const *pch, PartOp const *ops, TreeCtl const *ctl,
                          u8 const *pl0, u8 const *pl1, u8 const *plp, void const *g0, void const *g1,
-                         void const *gp, u8 *o0, u8 *o1, void *go0, void *go1, u32 Fs, u32 gh_bytes, u32 grid);
+                         void const *gp, u8 *o0, u8 *o1, void *go0, void *go1, u32 Fs, u32 gh_bytes, u32 grid,
+                         u32 const *chunk_left = nullptr);
+u32 scatter_tile_rows(u32 Fs, u32 gh_bytes);
+void launch_plan_counted(hipStream_t s, NodeInfo const *front, SplitRec const *splits, TreeCtl *ctl, DevNode *tree,
+                         PartOp *ops, PartChunk *pch, NodeInfo *next, HistChunk *hch, FinishOp *fops,
+                         u16 const *n_bins, GrowCfg const &cfg, double const *scales, u32 dst, u32 is_last,
+                         u32 chunk_rows);
 void launch_finalize(hipStream_t s, NodeInfo const *front, TreeCtl const *ctl, DevNode *tree, GrowCfg const &cfg,
                      double const *scales, u32 max_front);
 void launch_next_level(hipStream_t s, TreeCtl *ctl);

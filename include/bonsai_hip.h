@@ -74,6 +74,7 @@ typedef struct bh_split {
     uint32_t feature, bin, default_left, valid;
     double   gL, hL, gR, hR;
     int64_t  igL, ihL, igR, ihR;
+    uint32_t n_left, n_right;   /* child row counts (quantized fits) */
 } bh_split;
 
 /* ---- devices and errors ---- */
@@ -177,7 +178,7 @@ int bh_get_timing(bh_context *ctx, float *ms, uint32_t n);
 /* ---- launch-shape tuning (no counterpart in the reference, whose launch shapes are
  *      fixed constants, src/cuda/detail/device_context.cu:451-497) ---- */
 typedef struct bh_tuning {
-    uint32_t scatter_lds_bytes; /* LDS per partition-scatter workgroup: sets its tile rows */
+    uint32_t scatter_lds_bytes; /* LDS per partition-scatter workgroup (4,096..162,816 bytes): sets its tile rows */
     uint32_t find_blocks;       /* workgroups sharing one node's split find (1..8) */
     uint32_t hist_threads;      /* quantized histogram workgroup size: 256, 512 or 1024 */
     uint32_t part_grid;         /* partition workgroups */

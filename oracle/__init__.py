@@ -25,6 +25,8 @@ NODE_DTYPE = np.dtype([
 ])
 
 HIST_F32_SERIAL, HIST_F32_CHUNKED, HIST_INT, HIST_F32_OMP = 0, 1, 2, 3
+# Rows per device histogram work unit: the FP32 chunked order the device sums in.
+DEVICE_CHUNK = 8192
 
 
 class TreeCfg(C.Structure):
@@ -128,7 +130,7 @@ class Oracle:
                             P(np.ascontiguousarray(h, np.float32)), P(out))
         return out
 
-    def hist_f32(self, bins, rows, g, h, bin_stride, mode=HIST_F32_SERIAL, chunk=16384):
+    def hist_f32(self, bins, rows, g, h, bin_stride, mode=HIST_F32_SERIAL, chunk=DEVICE_CHUNK):
         n, f = bins.shape
         out = np.zeros((f, bin_stride, 2), np.float32)
         rows = np.ascontiguousarray(rows, np.uint32)
@@ -204,7 +206,7 @@ class Oracle:
         return out, nl
 
     # -- growers --
-    def _grow(self, fn, bins, n_bins, g, h, gq, hq, scales, c, mode, chunk=16384, cap=1 << 14):
+    def _grow(self, fn, bins, n_bins, g, h, gq, hq, scales, c, mode, chunk=DEVICE_CHUNK, cap=1 << 14):
         bins = np.ascontiguousarray(bins, np.uint8)
         nb = np.ascontiguousarray(n_bins, np.uint16)
         nodes = np.zeros(cap, NODE_DTYPE)
@@ -217,17 +219,17 @@ class Oracle:
         return nodes[:n].copy(), lor
 
     def grow_depthwise(self, bins, n_bins, g=None, h=None, gq=None, hq=None, scales=None, c=None,
-                       mode=HIST_F32_SERIAL, chunk=16384):
+                       mode=HIST_F32_SERIAL, chunk=DEVICE_CHUNK):
         return self._grow(self.L.orc_grow_depthwise, bins, n_bins, _f(g), _f(h), _i8(gq), _i16(hq), scales, c,
                           mode, chunk)
 
     def grow_leafwise(self, bins, n_bins, g=None, h=None, gq=None, hq=None, scales=None, c=None,
-                      mode=HIST_F32_SERIAL, chunk=16384):
+                      mode=HIST_F32_SERIAL, chunk=DEVICE_CHUNK):
         return self._grow(self.L.orc_grow_leafwise, bins, n_bins, _f(g), _f(h), _i8(gq), _i16(hq), scales, c,
                           mode, chunk)
 
     def grow_levelwise(self, bins, n_bins, g=None, h=None, gq=None, hq=None, scales=None, c=None,
-                       mode=HIST_F32_SERIAL, chunk=16384):
+                       mode=HIST_F32_SERIAL, chunk=DEVICE_CHUNK):
         bins = np.ascontiguousarray(bins, np.uint8)
         nb = np.ascontiguousarray(n_bins, np.uint16)
         D = c.max_depth

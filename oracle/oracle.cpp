@@ -22,6 +22,10 @@
 
 namespace {
 
+// Rows per device histogram work unit: the order the device's FP32 histograms sum in
+// (HIST_F32_CHUNKED restates it).
+constexpr uint64_t kDeviceChunk = 8192;
+
 using u8  = uint8_t;
 using u16 = uint16_t;
 using u32 = uint32_t;
@@ -931,7 +935,7 @@ Engine make_engine(uint8_t const *plane, uint64_t row_stride, uint64_t n_rows,
     e.gr.sh = sh;
     e.cfg = to_cfg(*cfg);
     e.mode = hist_mode;
-    e.chunk = chunk == 0 ? 16384 : chunk;
+    e.chunk = chunk == 0 ? kDeviceChunk : chunk;
     e.n_threads = hist_mode == ORC_HIST_F32_OMP ? omp_get_max_threads() : 1;
     return e;
 }
@@ -1521,7 +1525,7 @@ int32_t orc_train_depthwise(const uint8_t *plane, uint64_t row_stride, uint64_t 
             }
         }
         Engine e = make_engine(plane, row_stride, n_rows, n_features, n_bins, g.data(), h.data(),
-                               nullptr, nullptr, 0.0, 0.0, cfg, hist_mode, 16384);
+                               nullptr, nullptr, 0.0, 0.0, cfg, hist_mode, kDeviceChunk);
         e.n_threads = omp_get_max_threads();
         auto nodes = grow_depthwise(e, &lor);
         if (static_cast<int32_t>(nodes.size()) > max_nodes) return -1;

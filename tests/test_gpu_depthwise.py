@@ -8,7 +8,7 @@ histograms.
 import numpy as np
 import pytest
 
-from oracle_lib import HIST_F32_CHUNKED, HIST_F32_SERIAL, HIST_INT, cfg as orc_cfg
+from oracle_lib import DEVICE_CHUNK, HIST_F32_CHUNKED, HIST_F32_SERIAL, HIST_INT, cfg as orc_cfg
 
 pytestmark = pytest.mark.gpu
 
@@ -87,9 +87,9 @@ def test_histogram_f32_fixed_order(hip, orc, tall):
     # error bound of a float sum of n terms: n * eps * sum|x| per cell
     absg = orc.hist_f64(bins, rows, np.abs(g), np.abs(h), int(nb.max()))
     err = np.abs(d1.astype(np.float64) - ref64)
-    assert np.all(err <= 16384 * 2 ** -24 * absg + 1e-30)
-    chunked = orc.hist_f32(bins, rows, g, h, int(nb.max()), HIST_F32_CHUNKED, 16384)
-    # The device sums each cell in ascending row order per 16384-row unit, units in
+    assert np.all(err <= rows.size * 2 ** -24 * absg + 1e-30)
+    chunked = orc.hist_f32(bins, rows, g, h, int(nb.max()), HIST_F32_CHUNKED, DEVICE_CHUNK)
+    # The device sums each cell in ascending row order per DEVICE_CHUNK-row unit, units in
     # order: the chunked oracle restates that order.
     assert np.array_equal(d1.view(np.uint32), chunked.view(np.uint32))
 
