@@ -150,6 +150,7 @@ struct Tuning {
     u32 find_blocks = kFindMaxBlocks; // workgroups sharing one node's split find (1..kFindMaxBlocks)
     u32 hist_threads = 1024;     // quantized histogram workgroup size (256, 512 or 1024)
     u32 part_grid = 2048;        // partition count / scatter workgroups
+    u32 hist_layout = 0;         // quantized histogram, rows <= 32 bytes: 1 feature-in-lane, 0 row-dword lanes
 };
 
 // Per-node partial split records and completion counters of a multi-block split find

@@ -825,6 +825,7 @@ static int validate_tuning(bh_tuning const *t)
     if (t->hist_threads != 256u && t->hist_threads != 512u && t->hist_threads != 1024u)
         return fail(1, "hist_threads must be 256, 512 or 1024");
     if (t->part_grid < 64u || t->part_grid > 65536u) return fail(1, "part_grid must be in [64, 65536]");
+    if (t->hist_layout > 1u) return fail(1, "hist_layout must be 0 or 1");
     return 0;
 }
 
@@ -834,7 +835,7 @@ int bh_set_tuning(bh_context *c, const bh_tuning *t)
     if (int r = validate_tuning(t)) return r;
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    c->tune = Tuning{t->scatter_lds_bytes, t->find_blocks, t->hist_threads, t->part_grid};
+    c->tune = Tuning{t->scatter_lds_bytes, t->find_blocks, t->hist_threads, t->part_grid, t->hist_layout};
     drop_graph(c); // the iteration graph is re-captured with the new shapes
     return 0;
 }
@@ -842,7 +843,8 @@ int bh_set_tuning(bh_context *c, const bh_tuning *t)
 int bh_get_tuning(bh_context *c, bh_tuning *t)
 {
     if (!c) return fail(1, "null context");
-    *t = bh_tuning{c->tune.scatter_lds, c->tune.find_blocks, c->tune.hist_threads, c->tune.part_grid};
+    *t = bh_tuning{c->tune.scatter_lds, c->tune.find_blocks, c->tune.hist_threads, c->tune.part_grid,
+                   c->tune.hist_layout};
     return 0;
 }
 
@@ -897,12 +899,17 @@ int bh_autotune(bh_context *c, uint32_t iters, bh_tuning *chosen, float *ms_per_
     static const u32 thr_c[] = {256u, 512u, 1024u};
     static const u32 fb_c[] = {1u, 2u, 4u, 8u};
     static const u32 pg_c[] = {512u, 1024u, 2048u, 4096u};
-    for (int knob = 0; rc == 0 && knob < 4; ++knob) {
-        u32 const *cand = knob == 0 ? lds_c : knob == 1 ? thr_c : knob == 2 ? fb_c : pg_c;
-        int const nc = knob == 0 ? 7 : knob == 1 ? 3 : 4;
+    static const u32 hl_c[] = {0u, 1u};
+    for (int knob = 0; rc == 0 && knob < 5; ++knob) {
+        u32 const *cand = knob == 0 ? lds_c : knob == 1 ? thr_c : knob == 2 ? fb_c : knob == 3 ? pg_c : hl_c;
+        int const nc = knob == 0 ? 7 : knob == 1 ? 3 : knob == 4 ? 2 : 4;
         for (int k = 0; rc == 0 && k < nc; ++k) {
             Tuning t = best;
-            u32 &slot = knob == 0 ? t.scatter_lds : knob == 1 ? t.hist_threads : knob == 2 ? t.find_blocks : t.part_grid;
+            u32 &slot = knob == 0   ? t.scatter_lds
+                        : knob == 1 ? t.hist_threads
+                        : knob == 2 ? t.find_blocks
+                        : knob == 3 ? t.part_grid
+                                    : t.hist_layout;
             if (slot == cand[k]) continue;
             slot = cand[k];
             float ms = 0.0F;
@@ -921,7 +928,8 @@ int bh_autotune(bh_context *c, uint32_t iters, bh_tuning *chosen, float *ms_per_
     if (rc) return rc;
     if (rr) return rr;
     set_tuning(c->tune);
-    if (chosen) *chosen = bh_tuning{best.scatter_lds, best.find_blocks, best.hist_threads, best.part_grid};
+    if (chosen)
+        *chosen = bh_tuning{best.scatter_lds, best.find_blocks, best.hist_threads, best.part_grid, best.hist_layout};
     if (ms_per_iter) *ms_per_iter = best_ms;
     return 0;
 }

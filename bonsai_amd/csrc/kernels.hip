@@ -59,6 +59,19 @@ __device__ __forceinline__ float sr_uniform(u64 seed, u32 iter, u64 row, u32 cha
     return static_cast<float>(static_cast<u32>(z >> 40)) * (1.0F / 16777216.0F);
 }
 
+// Stages bytes [a, a + len) of src in LDS with aligned 16-byte loads from the enclosing
+// aligned range; returns the byte offset of `a` inside dst.  Planes and gradient buffers
+// carry 64 bytes of tail padding, so the over-read past the range stays allocated.
+__device__ __forceinline__ u32 lds_load_range(u8 const *__restrict__ src, size_t a, u32 len, uint4 *dst)
+{
+    size_t const a16 = a & ~static_cast<size_t>(15);
+    u32 const sh = static_cast<u32>(a - a16);
+    u32 const nw = (sh + len + 15u) >> 4;
+    uint4 const *s = reinterpret_cast<uint4 const *>(src + a16);
+    for (u32 w = threadIdx.x; w < nw; w += blockDim.x) dst[w] = s[w];
+    return sh;
+}
+
 // Block-wide exclusive scan of one u32 per thread (blockDim.x <= 1024).  Returns the
 // exclusive prefix; *total receives the block sum.  Uses `scratch` of blockDim.x u32.
 __device__ u32 block_exscan(u32 v, u32 *scratch, u32 *total)
@@ -473,6 +486,95 @@ __global__ __launch_bounds__(kHqThreads) void k_hist_q(
     }
 }
 
+// Quantized histogram for narrow rows (F <= 32), feature-in-lane layout (H6, same
+// contract as k_hist_q).  A tile of the unit's rows is staged in LDS with 16-byte loads;
+// one ds_add_u64 then covers two rows x the row's features (lane = row parity x
+// feature column) on an LDS histogram laid out [bin][32]: column f always maps to bank
+// pair f, so the only possible bank conflict is between the two rows of one feature.
+// The random bins therefore no longer serialise the LDS atomics.
+constexpr u32 kHq2Cols = 32;
+constexpr int kHq2Unroll = 4;
+// LDS per workgroup kept under half of the CU's 160 KiB so two workgroups share a CU
+// and overlap one another's staging and accumulation.
+constexpr u32 kHq2LdsBudget = 81408;
+
+__global__ __launch_bounds__(512) void k_hist_q2(
+    HistChunk const *__restrict__ chunks, TreeCtl const *__restrict__ ctl,
+    u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
+    void const *__restrict__ gw0, void const *__restrict__ gw1, void const *__restrict__ gwp,
+    u32 Fs, u32 F, u32 SB, u32 tile, void *__restrict__ partials, void *__restrict__ hist_next,
+    u32 *__restrict__ cnt_next)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned long long *lh = reinterpret_cast<unsigned long long *>(smem);            // SB x 32
+    uint4 *s_rows = reinterpret_cast<uint4 *>(smem + static_cast<size_t>(SB) * kHq2Cols * 8u); // tile*Fs+16
+    uint4 *s_gh = reinterpret_cast<uint4 *>(reinterpret_cast<unsigned char *>(s_rows) + tile * Fs + 16u);
+    u32 const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    u32 const NW = blockDim.x >> 6;
+    u32 const col = lane & 31u, half = lane >> 5;
+    bool const col_ok = col < F;
+    u32 const lds_cells = SB * kHq2Cols;
+    u32 const n_units = ctl->n_hist_chunks;
+    size_t const node_cells = static_cast<size_t>(F) * SB;
+
+    for (u32 c = blockIdx.x; c < n_units; c += gridDim.x) {
+        HistChunk const ch = chunks[c];
+        for (u32 i = threadIdx.x; i < lds_cells; i += blockDim.x) lh[i] = 0ull;
+        u8 const *plane = ch.src == 0 ? pl0 : (ch.src == 1 ? pl1 : plp);
+        u8 const *ghb = static_cast<u8 const *>(ch.src == 0 ? gw0 : (ch.src == 1 ? gw1 : gwp));
+        for (u32 t0 = ch.begin; t0 < ch.end; t0 += tile) {
+            u32 const rows = min(tile, ch.end - t0);
+            __syncthreads(); // zeroing done / previous tile consumed
+            u32 const sh = lds_load_range(plane, static_cast<size_t>(t0) * Fs, rows * Fs, s_rows);
+            u32 const shg = lds_load_range(ghb, static_cast<size_t>(t0) * 4u, rows * 4u, s_gh);
+            __syncthreads();
+            u8 const *rb = reinterpret_cast<u8 const *>(s_rows) + sh + col;
+            u32 const *gb = reinterpret_cast<u32 const *>(reinterpret_cast<u8 const *>(s_gh) + shg);
+            if (col_ok) {
+                u32 const stride = 2u * NW;
+                u32 r = 2u * wave + half;
+                for (; r + (kHq2Unroll - 1) * stride < rows; r += kHq2Unroll * stride) {
+                    u32 bin[kHq2Unroll], v[kHq2Unroll];
+#pragma unroll
+                    for (int u = 0; u < kHq2Unroll; ++u) {
+                        u32 const rr = r + u * stride;
+                        bin[u] = rb[rr * Fs];
+                        v[u] = gb[rr];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kHq2Unroll; ++u)
+                        atomicAdd(lh + bin[u] * kHq2Cols + col, static_cast<unsigned long long>(pack_q(v[u])));
+                }
+                for (; r < rows; r += stride)
+                    atomicAdd(lh + static_cast<u32>(rb[r * Fs]) * kHq2Cols + col,
+                              static_cast<unsigned long long>(pack_q(gb[r])));
+            }
+        }
+        __syncthreads();
+        // Flush in (bin, feature) order: LDS reads stay on distinct bank pairs.
+        u32 const out_cells = F * SB;
+        if (ch.part != kNone) {
+            u64 *dst = reinterpret_cast<u64 *>(partials) + static_cast<size_t>(ch.part) * node_cells;
+            for (u32 i = threadIdx.x; i < out_cells; i += blockDim.x) {
+                u32 const b = i / F, fl = i - b * F;
+                dst[static_cast<size_t>(fl) * SB + b] = lh[b * kHq2Cols + fl];
+            }
+        } else {
+            size_t const base = static_cast<size_t>(ch.slot) * node_cells;
+            i64 *dst = reinterpret_cast<i64 *>(hist_next) + base * 2;
+            for (u32 i = threadIdx.x; i < out_cells; i += blockDim.x) {
+                u32 const b = i / F, fl = i - b * F;
+                u64 const v = lh[b * kHq2Cols + fl];
+                size_t const o = static_cast<size_t>(fl) * SB + b;
+                dst[2 * o] = unpack_g(v);
+                dst[2 * o + 1] = unpack_h(v);
+                if (cnt_next) cnt_next[base + o] = unpack_cnt(v);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Fixed-order completion of built nodes (H6 reduce + H7 sibling subtraction,
 // src/step/primitives.hpp:156-173; device twin subtract_kernel, kernels.cuh:507-520):
 // small = p_0 + p_1 + ... in unit order, large = parent - small.
@@ -566,6 +668,94 @@ __global__ void k_finish(FinishOp const *__restrict__ ops, TreeCtl const *__rest
                         make_float2(pv.x - s.x, pv.y - s.y);
                 }
             }
+        }
+    }
+}
+
+// Quantized completion with the partial range spread over threads (integer sums are
+// exact in any order).  A 256-thread workgroup owns kFqCells consecutive cells of one
+// op: thread (g, j) sums partials g, g + kFqGroups, ... of cell j (coalesced across the
+// cells), the kFqGroups group sums meet in LDS, and one thread per cell writes the small
+// child and large = parent - small (with counts when cnt_next is given).
+constexpr u32 kFqCells = 16;
+constexpr u32 kFqGroups = 16;
+
+__global__ __launch_bounds__(256) void k_finish_q(FinishOp const *__restrict__ ops, TreeCtl const *__restrict__ ctl,
+                                                  void const *__restrict__ partials, void const *__restrict__ hist_cur,
+                                                  void *__restrict__ hist_next, u32 node_cells,
+                                                  u32 const *__restrict__ cnt_cur, u32 *__restrict__ cnt_next)
+{
+    __shared__ i64 s_g[kFqGroups][kFqCells], s_h[kFqGroups][kFqCells];
+    __shared__ u32 s_n[kFqGroups][kFqCells];
+    u32 const n_ops = ctl->n_finish;
+    u32 const j = threadIdx.x % kFqCells, grp = threadIdx.x / kFqCells;
+    u32 const n_tiles = (node_cells + kFqCells - 1) / kFqCells;
+    for (u32 o = blockIdx.y; o < n_ops; o += gridDim.y) {
+        FinishOp const op = ops[o];
+        i64 *nx = reinterpret_cast<i64 *>(hist_next);
+        for (u32 tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+            u32 const i = tile * kFqCells + j;
+            bool const in = i < node_cells;
+            i64 g = 0, h = 0;
+            u32 n = 0;
+            if (in && op.part_count > 0) {
+                u64 const *pp = reinterpret_cast<u64 const *>(partials) + i;
+                u32 p = grp;
+                for (; p + 3 * kFqGroups < op.part_count; p += 4 * kFqGroups) {
+                    u64 v[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        v[k] = pp[static_cast<size_t>(op.part_first + p + k * kFqGroups) * node_cells];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        g += unpack_g(v[k]);
+                        h += unpack_h(v[k]);
+                        n += unpack_cnt(v[k]);
+                    }
+                }
+                for (; p < op.part_count; p += kFqGroups) {
+                    u64 const v = pp[static_cast<size_t>(op.part_first + p) * node_cells];
+                    g += unpack_g(v);
+                    h += unpack_h(v);
+                    n += unpack_cnt(v);
+                }
+            }
+            s_g[grp][j] = g;
+            s_h[grp][j] = h;
+            s_n[grp][j] = n;
+            __syncthreads();
+            if (grp == 0 && in) {
+                size_t const si = static_cast<size_t>(op.small_slot) * node_cells + i;
+                if (op.empty) {
+                    g = h = 0;
+                    n = 0;
+                    nx[si * 2] = 0;
+                    nx[si * 2 + 1] = 0;
+                    if (cnt_next) cnt_next[si] = 0;
+                } else if (op.part_count == 0) {
+                    g = nx[si * 2];
+                    h = nx[si * 2 + 1];
+                    n = cnt_next ? cnt_next[si] : 0u;
+                } else {
+                    for (u32 k = 1; k < kFqGroups; ++k) {
+                        g += s_g[k][j];
+                        h += s_h[k][j];
+                        n += s_n[k][j];
+                    }
+                    nx[si * 2] = g;
+                    nx[si * 2 + 1] = h;
+                    if (cnt_next) cnt_next[si] = n;
+                }
+                if (op.large_slot != kNone) {
+                    i64 const *pc = reinterpret_cast<i64 const *>(hist_cur);
+                    size_t const pi = static_cast<size_t>(op.parent_slot) * node_cells + i;
+                    size_t const li = static_cast<size_t>(op.large_slot) * node_cells + i;
+                    nx[li * 2] = pc[pi * 2] - g;
+                    nx[li * 2 + 1] = pc[pi * 2 + 1] - h;
+                    if (cnt_next) cnt_next[li] = cnt_cur[pi] - n;
+                }
+            }
+            __syncthreads();
         }
     }
 }
@@ -1357,15 +1547,6 @@ __global__ __launch_bounds__(256) void k_part_scatter(
 // the aligned over-read past a tile's last row stays inside the allocation.
 // k / D as (k * ceil(2^32 / D)) >> 32 in 64-bit arithmetic: exact for k < 2^16, D <= 16.
 
-__device__ __forceinline__ u32 lds_load_range(u8 const *__restrict__ src, size_t a, u32 len, uint4 *dst)
-{
-    size_t const a16 = a & ~static_cast<size_t>(15);
-    u32 const sh = static_cast<u32>(a - a16);
-    u32 const nw = (sh + len + 15u) >> 4;
-    uint4 const *s = reinterpret_cast<uint4 const *>(src + a16);
-    for (u32 w = threadIdx.x; w < nw; w += blockDim.x) dst[w] = s[w];
-    return sh;
-}
 
 // dst[first + k] = src[base + idx[k / D] * D + k % D] for k < n (dword units).
 __device__ __forceinline__ u32 div_magic(u32 k, u64 magic)
@@ -1950,7 +2131,16 @@ hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl c
     u32 const slices = (F + fsl - 1) / fsl;
     size_t const lds = static_cast<size_t>(fsl) * SB * 8;
     dim3 const grid(grid_x, slices), block(64 * n_waves);
-    if (q) {
+    if (q && Fs <= kHq2Cols && t_tune.hist_layout == 1u) {
+        u32 const lh_bytes = SB * kHq2Cols * 8u;
+        u32 tile = lh_bytes + 32u + 64u * (Fs + 4u) <= kHq2LdsBudget
+                       ? ((kHq2LdsBudget - lh_bytes - 32u) / (Fs + 4u)) & ~63u
+                       : 64u;
+        tile = std::max(64u, std::min(tile, 1024u));
+        size_t const lds2 = static_cast<size_t>(lh_bytes) + tile * (Fs + 4u) + 32u;
+        hipLaunchKernelGGL(k_hist_q2, dim3(grid_x), dim3(512), lds2, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp, Fs, F,
+                           SB, tile, partials, hist_next, cnt_next);
+    } else if (q) {
         hipLaunchKernelGGL(k_hist_q, grid, dim3(t_tune.hist_threads), lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp,
                            Fs, F, SB, fsl, partials, hist_next, cnt_next);
     } else
@@ -1981,6 +2171,7 @@ hipError_t set_hist_lds_limits()
     if (e != hipSuccess) return e;
     raise_lds(reinterpret_cast<void const *>(&k_hist<true, 8>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_hist_q), max_shared);
+    raise_lds(reinterpret_cast<void const *>(&k_hist_q2), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_hist<false, 4>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_find<true>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_find<false>), max_shared);
@@ -1993,10 +2184,15 @@ void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl const *ct
 {
     u32 const tiles = min(64u, (node_cells + 255) / 256);
     dim3 const grid(tiles, max(1u, min(max_ops, 1024u)));
-    if (q)
-        hipLaunchKernelGGL(k_finish<true>, grid, dim3(256), 0, s, ops, ctl, partials, hist_cur, hist_next,
+    if (q) {
+        // Enough workgroups to cover the cells of one op several times over; ops beyond
+        // the grid's y extent loop.
+        u32 const ops_y = max(1u, min(max_ops, 1024u));
+        u32 const cell_tiles = (node_cells + kFqCells - 1) / kFqCells;
+        u32 const tx = max(1u, min(cell_tiles, max(1u, 4096u / ops_y)));
+        hipLaunchKernelGGL(k_finish_q, dim3(tx, ops_y), dim3(256), 0, s, ops, ctl, partials, hist_cur, hist_next,
                            node_cells, cnt_cur, cnt_next);
-    else
+    } else
         hipLaunchKernelGGL(k_finish<false>, grid, dim3(256), 0, s, ops, ctl, partials, hist_cur, hist_next,
                            node_cells, nullptr, nullptr);
 }

@@ -182,6 +182,7 @@ typedef struct bh_tuning {
     uint32_t find_blocks;       /* workgroups sharing one node's split find (1..8) */
     uint32_t hist_threads;      /* quantized histogram workgroup size: 256, 512 or 1024 */
     uint32_t part_grid;         /* partition workgroups */
+    uint32_t hist_layout;       /* quantized histogram of rows <= 32 bytes: 1 feature-in-lane, 0 row-dword */
 } bh_tuning;
 /* Set / read the context's launch shapes (the iteration graph is re-captured). */
 int bh_set_tuning(bh_context *ctx, const bh_tuning *t);

@@ -35,9 +35,9 @@ def fit(hip, orc, bins, nb, y, quantized, tuning=None, autotune=False, iters=4):
 
 
 SHAPES = [
-    dict(scatter_lds_bytes=4096, find_blocks=1, hist_threads=256, part_grid=64),
-    dict(scatter_lds_bytes=49152, find_blocks=3, hist_threads=512, part_grid=4096),
-    dict(scatter_lds_bytes=147456, find_blocks=8, hist_threads=1024, part_grid=2048),
+    dict(scatter_lds_bytes=4096, find_blocks=1, hist_threads=256, part_grid=64, hist_layout=0),
+    dict(scatter_lds_bytes=49152, find_blocks=3, hist_threads=512, part_grid=4096, hist_layout=1),
+    dict(scatter_lds_bytes=147456, find_blocks=8, hist_threads=1024, part_grid=2048, hist_layout=0),
 ]
 
 
