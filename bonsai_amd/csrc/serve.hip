@@ -124,112 +124,125 @@ __global__ __launch_bounds__(256) void k_predict(u8 const *__restrict__ bins, u6
 // TreeSHAP, packed closed form (eval_shap_paths, src/shap_paths.cpp:196-278):
 //   P(t) = prod_j (z_j + o_j t);  unsatisfied element: -value * sum_i w_i [t^i] P;
 //   satisfied element: value * (1 - z) * sum_i w_i [t^i] (P / (t + z)).
-// A workgroup serves kSR rows with one wave each; lane l takes paths l, l+64, ... and
-// keeps its own phi slice in LDS; slices are then added in lane order (fixed order,
-// no atomics), so a row's result does not depend on what else is in flight.
+// Paths are packed in groups of 64, element j of the group's paths stored contiguously
+// ([group][j][lane]), so one coalesced load gives every lane its path's element.  A
+// workgroup serves kShapRows rows: each lane loads a path once and applies it to all of
+// them.  Contributions are accumulated as 64-bit fixed point (value x scale, scale from
+// the ensemble's leaf magnitudes) with LDS integer atomics: exact and order-free, so a
+// row's result does not depend on the order of the adds or on what shares its batch.
 // ---------------------------------------------------------------------------------
+constexpr u32 kShapRows = 32;
+
+// KM: compile-time bound on the path length (8 covers depth-8 trees), so every per-path
+// array lives in registers.
+template <int KM>
 __global__ __launch_bounds__(256) void k_shap(u8 const *__restrict__ bins, u64 n, u64 stride, u32 F,
                                               SElem const *__restrict__ elems, SHead const *__restrict__ heads,
-                                              u32 n_heads, float const *__restrict__ weights,
-                                              u16 const *__restrict__ last, float lr, double bias,
+                                              u32 n_groups, u32 K, float const *__restrict__ weights,
+                                              u16 const *__restrict__ last, float lr, double bias, double scale,
                                               double *__restrict__ out)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    u32 const rows_per_block = blockDim.x >> 6;
-    double *s_phi = reinterpret_cast<double *>(smem);                      // [rows][64][F]
-    u8 *s_row = reinterpret_cast<u8 *>(s_phi + rows_per_block * 64 * F);   // [rows][F]
-    u32 const wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    u64 const r = static_cast<u64>(blockIdx.x) * rows_per_block + wave;
-    bool const live = r < n;
-    double *my = s_phi + (wave * 64 + lane) * F;
-    for (u32 f = 0; f < F; ++f) my[f] = 0.0;
-    if (live)
-        for (u32 f = lane; f < F; f += 64) s_row[wave * F + f] = bins[r * stride + f];
+    unsigned long long *s_phi = reinterpret_cast<unsigned long long *>(smem); // [kShapRows][F] fixed point
+    u8 *s_row = reinterpret_cast<u8 *>(s_phi + kShapRows * F);                 // [kShapRows][F]
+    u64 const r0 = static_cast<u64>(blockIdx.x) * kShapRows;
+    if (r0 >= n) return;
+    u32 const nr = static_cast<u32>(min<u64>(kShapRows, n - r0));
+    for (u32 k = threadIdx.x; k < kShapRows * F; k += blockDim.x) s_phi[k] = 0ull;
+    for (u32 k = threadIdx.x; k < nr * F; k += blockDim.x) {
+        u32 const rr = k / F, f = k - rr * F;
+        s_row[k] = bins[(r0 + rr) * stride + f];
+    }
     __syncthreads();
-    if (live) {
-        u8 const *row = s_row + wave * F;
-        for (u32 p = lane; p < n_heads; p += 64) {
-            SHead const hd = heads[p];
-            u32 const nel = hd.n;
-            if (nel == 0) continue;
-            float poly[kMaxPath + 1];
-            float z[kMaxPath];
-            bool sat[kMaxPath];
-            u32 feat[kMaxPath];
+    u32 const wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, NW = blockDim.x >> 6;
+    for (u32 g = wave; g < n_groups; g += NW) {
+        SHead const hd = heads[static_cast<size_t>(g) * 64u + lane];
+        u32 const nel = hd.n;
+        if (nel == 0) continue;
+        u32 feat[KM], lo[KM], hi[KM];
+        bool miss[KM];
+        float z[KM], wl[KM];
+        float const *w = weights + (nel - 1) * nel / 2;
 #pragma unroll
-            for (int j = 0; j <= kMaxPath; ++j) poly[j] = 0.0F;
-            poly[0] = 1.0F;
-#pragma unroll
-            for (int j = 0; j < kMaxPath; ++j) {
-                if (static_cast<u32>(j) >= nel) break;
-                SElem const e = elems[hd.first + j];
-                u32 const f = e.feature & 0x7FFFu;
-                u32 const b = row[f];
-                bool const one = b == last[f] ? (e.feature & 0x8000u) != 0 : (b >= e.lo && b <= e.hi);
-                sat[j] = one;
+        for (int j = 0; j < KM; ++j) {
+            feat[j] = 0;
+            lo[j] = 1;
+            hi[j] = 0;
+            miss[j] = false;
+            z[j] = 1.0F;
+            wl[j] = 0.0F;
+            if (static_cast<u32>(j) < nel) {
+                SElem const e = elems[(static_cast<size_t>(g) * K + j) * 64u + lane];
+                feat[j] = e.feature & 0x7FFFu;
+                miss[j] = (e.feature & 0x8000u) != 0;
+                lo[j] = e.lo;
+                hi[j] = e.hi;
                 z[j] = e.z;
-                feat[j] = f;
-                // multiply by (z + o t)
-#pragma unroll
-                for (int i = kMaxPath; i >= 1; --i) {
-                    if (i > j + 1) continue;
-                    poly[i] = z[j] * poly[i] + (one ? poly[i - 1] : 0.0F);
-                }
-                poly[0] *= z[j];
+                wl[j] = w[j];
             }
-            float const *w = weights + (nel - 1) * nel / 2;
-            float wl[kMaxPath];
+        }
+        double const v = hd.value;
+        // Lanes start on different rows, so one atomic instruction spreads over the
+        // rows' accumulators instead of piling onto one row's few features.
+        u32 rr = lane % nr;
+        for (u32 r = 0; r < nr; ++r, rr = rr + 1 == nr ? 0u : rr + 1) {
+            u8 const *row = s_row + rr * F;
+            unsigned long long *phi = s_phi + rr * F;
+            float poly[KM + 1];
+            bool sat[KM];
 #pragma unroll
-            for (int i = 0; i < kMaxPath; ++i) wl[i] = static_cast<u32>(i) < nel ? w[i] : 0.0F;
+            for (int j = 0; j <= KM; ++j) poly[j] = j == 0 ? 1.0F : 0.0F;
+#pragma unroll
+            for (int j = 0; j < KM; ++j) {
+                sat[j] = false;
+                if (static_cast<u32>(j) < nel) {
+                    u32 const b = row[feat[j]];
+                    bool const one = b == last[feat[j]] ? miss[j] : (b >= lo[j] && b <= hi[j]);
+                    sat[j] = one;
+                    // multiply by (z + o t)
+#pragma unroll
+                    for (int i = KM; i >= 1; --i)
+                        if (i <= j + 1) poly[i] = z[j] * poly[i] + (one ? poly[i - 1] : 0.0F);
+                    poly[0] *= z[j];
+                }
+            }
             float unsat = 0.0F;
 #pragma unroll
-            for (int i = 0; i < kMaxPath; ++i)
-                if (static_cast<u32>(i) < nel) unsat += wl[i] * poly[i];
-            double const v = hd.value;
+            for (int i = 0; i < KM; ++i) unsat += wl[i] * poly[i]; // wl[i] = 0 beyond the path
 #pragma unroll
-            for (int k = 0; k < kMaxPath; ++k) {
-                if (static_cast<u32>(k) >= nel) break;
-                if (!sat[k]) {
-                    my[feat[k]] -= v * static_cast<double>(unsat);
-                    continue;
-                }
-                // synthetic division by (t + z_k): deflated[n-1] = P[n],
-                // deflated[i] = P[i+1] - z_k deflated[i+1]
-                float defl[kMaxPath];
-                float const zk = z[k];
-                float s = 0.0F;
-                float prev = 0.0F;
-#pragma unroll
-                for (int i = kMaxPath - 1; i >= 0; --i) {
-                    if (static_cast<u32>(i) < nel) {
-                        prev = (static_cast<u32>(i) + 1 == nel) ? poly[i + 1] : poly[i + 1] - zk * prev;
-                        defl[i] = prev;
+            for (int k = 0; k < KM; ++k) {
+                if (static_cast<u32>(k) < nel) {
+                    double c;
+                    if (!sat[k]) {
+                        c = -v * static_cast<double>(unsat);
                     } else {
-                        defl[i] = 0.0F;
-                    }
-                }
+                        // synthetic division by (t + z_k): deflated[n-1] = P[n],
+                        // deflated[i] = P[i+1] - z_k deflated[i+1]
+                        float const zk = z[k];
+                        float sdef = 0.0F, prev = 0.0F;
 #pragma unroll
-                for (int i = 0; i < kMaxPath; ++i)
-                    if (static_cast<u32>(i) < nel) s += wl[i] * defl[i];
-                my[feat[k]] += v * static_cast<double>(1.0F - zk) * static_cast<double>(s);
+                        for (int i = KM - 1; i >= 0; --i) {
+                            if (static_cast<u32>(i) < nel) {
+                                prev = (static_cast<u32>(i) + 1 == nel) ? poly[i + 1] : poly[i + 1] - zk * prev;
+                                sdef += wl[i] * prev;
+                            }
+                        }
+                        c = v * static_cast<double>(1.0F - zk) * static_cast<double>(sdef);
+                    }
+                    atomicAdd(phi + feat[k], static_cast<unsigned long long>(__double2ll_rn(c * scale)));
+                }
             }
         }
     }
     __syncthreads();
-    if (live) {
-        double *base = s_phi + wave * 64 * F;
-        for (u32 f = lane; f < F; f += 64) {
-            double a = 0.0;
-            for (u32 l = 0; l < 64; ++l) a += base[l * F + f];
-            out[r * (F + 1) + f] = a * lr;
-        }
-        if (lane == 0) out[r * (F + 1) + F] = bias;
+    for (u32 k = threadIdx.x; k < nr * F; k += blockDim.x) {
+        u32 const rr = k / F, f = k - rr * F;
+        out[(r0 + rr) * (F + 1) + f] =
+            static_cast<double>(static_cast<long long>(s_phi[k])) / scale * static_cast<double>(lr);
     }
+    for (u32 rr = threadIdx.x; rr < nr; rr += blockDim.x) out[(r0 + rr) * (F + 1) + F] = bias;
 }
 
-// ---------------------------------------------------------------------------------
-// Host-side packing (product code): walk nodes and leaf paths from bin-space trees.
-// ---------------------------------------------------------------------------------
 struct ActiveElem {
     u32 feature;
     int lo, hi;
@@ -311,6 +324,8 @@ struct bh_predictor {
     float lr = 0.0F, init = 0.0F;
     double shap_bias = 0.0;
     u32 n_heads = 0, max_len = 0;
+    u32 n_groups = 0;          // TreeSHAP path groups of 64
+    double shap_scale = 1.0;   // fixed-point scale of the TreeSHAP accumulators
     PNode *d_nodes = nullptr;
     u32 *d_roots = nullptr;
     u16 *d_last = nullptr;
@@ -336,14 +351,17 @@ int launch_predict(bh_predictor *p, u8 const *d_bins, u64 n, u64 stride, float *
 int launch_shap(bh_predictor *p, u8 const *d_bins, u64 n, u64 stride, double *d_out, hipStream_t s)
 {
     if (n == 0) return 0;
-    // One wave per row; as many rows per workgroup (<= 4) as the per-lane phi slices fit.
-    size_t const per_row = static_cast<size_t>(64) * p->F * sizeof(double) + p->F;
-    u32 const sr = static_cast<u32>(std::min<size_t>(4, (64 * 1024) / per_row));
-    if (sr == 0) return sfail(1, "TreeSHAP: more than 127 features do not fit the per-lane LDS slices");
-    size_t const lds = sr * per_row;
-    u32 const blocks = static_cast<u32>((n + sr - 1) / sr);
-    hipLaunchKernelGGL(k_shap, dim3(blocks), dim3(64 * sr), lds, s, d_bins, n, stride, p->F, p->d_elems, p->d_heads,
-                       p->n_heads, p->d_w, p->d_last, p->lr, p->shap_bias, d_out);
+    size_t const lds = static_cast<size_t>(kShapRows) * p->F * (sizeof(unsigned long long) + 1u);
+    if (lds > 160u * 1024u) return sfail(1, "TreeSHAP: too many features for the LDS accumulators");
+    u32 const blocks = static_cast<u32>((n + kShapRows - 1) / kShapRows);
+    if (p->max_len <= 8)
+        hipLaunchKernelGGL(k_shap<8>, dim3(blocks), dim3(256), lds, s, d_bins, n, stride, p->F, p->d_elems,
+                           p->d_heads, p->n_groups, p->max_len, p->d_w, p->d_last, p->lr, p->shap_bias,
+                           p->shap_scale, d_out);
+    else
+        hipLaunchKernelGGL(k_shap<kMaxPath>, dim3(blocks), dim3(256), lds, s, d_bins, n, stride, p->F, p->d_elems,
+                           p->d_heads, p->n_groups, p->max_len, p->d_w, p->d_last, p->lr, p->shap_bias,
+                           p->shap_scale, d_out);
     SHIP(hipGetLastError());
     return 0;
 }
@@ -423,7 +441,25 @@ int bh_predictor_create(int device, const bh_node *nodes, const uint32_t *tree_o
     // (include/bonsai/booster.hpp:902-925).
     p->shap_bias = ev_sum * static_cast<double>(learning_rate) + static_cast<double>(init_score);
     p->n_heads = static_cast<u32>(hd.size());
-    p->max_len = static_cast<u32>(max_len);
+    p->max_len = static_cast<u32>(std::max<size_t>(max_len, 1));
+    // Interleave the paths in groups of 64: element j of the group's paths contiguous.
+    u32 const K = p->max_len;
+    p->n_groups = (p->n_heads + 63u) / 64u;
+    std::vector<SElem> el2(static_cast<size_t>(p->n_groups) * K * 64u);
+    std::vector<SHead> hd2(static_cast<size_t>(p->n_groups) * 64u);
+    double leaf_abs = 0.0;
+    for (size_t q = 0; q < hd.size(); ++q) {
+        size_t const g = q / 64u, l = q % 64u;
+        hd2[g * 64u + l] = hd[q];
+        hd2[g * 64u + l].first = 0;
+        for (u32 j = 0; j < hd[q].n; ++j) el2[(g * K + j) * 64u + l] = el[hd[q].first + j];
+        leaf_abs += std::fabs(static_cast<double>(hd[q].value));
+    }
+    // |one path's contribution| <= |value| * sum_i w_i C(n, i) < 4 |value| for n <= 16,
+    // so every accumulator stays below 2^61 in magnitude.
+    p->shap_scale = std::ldexp(1.0, 61) / std::max(4.0 * leaf_abs, 1e-300);
+    el.swap(el2);
+    hd.swap(hd2);
     // Permutation weights w_i = 1 / (n C(n-1, i)) (shap_path_weights, src/shap_paths.cpp:165-194).
     std::vector<float> w(kMaxPath * (kMaxPath + 1) / 2, 0.0F);
     {
