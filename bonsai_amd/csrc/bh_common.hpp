@@ -81,13 +81,17 @@ struct SplitRec {
 };
 static_assert(sizeof(SplitRec) == 96, "SplitRec");
 
-// A histogram work unit: rows [begin, end) of one node's segment.
+// A histogram work unit: rows [begin, end) of one node's segment, or of its parent's
+// segment when the node's rows have not been moved yet (route != kNone): then only the
+// rows that the parent's split (ops[route]) sends to `side` (0 left, 1 right) count.
 struct HistChunk {
     u32 slot;  // destination node histogram slot (next-level buffer)
     u32 src;   // which plane/gh buffer holds the rows
     u32 begin, end;
     u32 part;  // partial index, or kNone when the chunk is the node's only one
-    u32 pad[3];
+    u32 route = kNone; // PartOp index of the parent's split, kNone: every row counts
+    u32 side = 0;
+    u32 pad = 0;
 };
 
 // Completion of one built node: small = sum of partials (fixed order), large = parent -
@@ -97,8 +101,9 @@ struct FinishOp {
     u32 part_first, part_count; // part_count == 0: written directly (or empty)
     u32 empty;                  // small child has no rows: zero its histogram
     u32 unit_first, unit_count; // histogram units of the small child
-    u32 small_off, small_cnt, src;
-    u32 pad[5];
+    u32 small_off, small_cnt, src; // rows the units scan (the parent's segment when routed)
+    u32 route = kNone, side = 0; // HistChunk::route / side of the units
+    u32 pad[3];
 };
 static_assert(sizeof(FinishOp) == 64, "FinishOp");
 
@@ -113,6 +118,23 @@ struct PartOp {
     u32 alive;        // 0 after demotion
     u32 base;         // scan scratch: global prefix at the op's first unit
 };
+
+// Two levels of splits applied in one pass (quantized depthwise): the rows of a
+// grandparent segment [off, off + cnt) go to its four grandchildren, each a contiguous
+// run of the same range: AL at off, AR at off + n_al, BL at off + n_a, BR at
+// off + n_a + n_bl (A = left child, B = right child).  Rows of a child that did not
+// split are not moved (no later level reads them).
+struct PartOp4 {
+    u32 off, cnt, src, dst;
+    u32 feature, bin, default_left, last_bin;  // the grandparent's split
+    u32 fa, ba, da, la;                        // A's split (feature, bin, default_left, last_bin)
+    u32 fb, bb, db, lb;                        // B's split
+    u32 n_a, n_al, n_bl;
+    u32 alive;                                 // bit 0: A splits, bit 1: B splits
+    u32 chunk_first, chunk_count;
+    u32 pad[2];
+};
+static_assert(sizeof(PartOp4) == 96, "PartOp4");
 
 struct PartChunk {
     u32 op;
@@ -151,6 +173,7 @@ struct Tuning {
     u32 hist_threads = 1024;     // quantized histogram workgroup size (256, 512 or 1024)
     u32 part_grid = 2048;        // partition count / scatter workgroups
     u32 hist_layout = 0;         // quantized histogram, rows <= 32 bytes: 1 feature-in-lane, 0 row-dword lanes
+    u32 pair_levels = 1;         // quantized depthwise, rows <= 64 bytes: move rows every second level
 };
 
 // Per-node partial split records and completion counters of a multi-block split find

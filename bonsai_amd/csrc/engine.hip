@@ -75,6 +75,8 @@ struct bh_context {
     SplitRec *splits = nullptr;
     FindScratch fscr{nullptr, nullptr};
     u32 *cnt[2] = {nullptr, nullptr}; // quantized depthwise: row count per (slot, feature, bin)
+    PartOp4 *ops4 = nullptr;          // quantized depthwise: four-way partition ops
+    uint4 *chunk_cls = nullptr;       // and their per-chunk class counts
     Tuning tune;
     DevNode *tree = nullptr;
     PartOp *ops = nullptr;
@@ -131,7 +133,7 @@ void free_fit(bh_context *c)
     void *ptrs[] = {c->gh0f, c->ghwf[0], c->ghwf[1], c->gh0q, c->ghwq[0], c->ghwq[1], c->maxbits, c->scales,
                     c->hist[0], c->hist[1], c->partials, c->ctl, c->front[0], c->front[1], c->splits, c->tree,
                     c->ops, c->pch, c->chunk_left, c->hch, c->fops, c->trees_out, c->tree_sizes, c->leaf_rows,
-                    c->fscr.part, c->fscr.done, c->cnt[0], c->cnt[1]};
+                    c->fscr.part, c->fscr.done, c->cnt[0], c->cnt[1], c->ops4, c->chunk_cls};
     for (void *p : ptrs)
         if (p) hipFree(p);
     c->gh0f = c->ghwf[0] = c->ghwf[1] = nullptr;
@@ -153,6 +155,8 @@ void free_fit(bh_context *c)
     c->leaf_rows = nullptr;
     c->fscr = FindScratch{nullptr, nullptr};
     c->cnt[0] = c->cnt[1] = nullptr;
+    c->ops4 = nullptr;
+    c->chunk_cls = nullptr;
     ext_free(c->ext);
     c->configured = false;
     c->trees_cap = c->trees_done = 0;
@@ -200,6 +204,8 @@ int alloc_fit(bh_context *c)
         size_t const cells = static_cast<size_t>(c->hist_slots) * nc;
         HIP_TRY(dalloc(&c->cnt[0], cells));
         HIP_TRY(dalloc(&c->cnt[1], cells));
+        HIP_TRY(dalloc(&c->ops4, c->max_ops / 2 + 2));
+        HIP_TRY(dalloc(&c->chunk_cls, c->max_pch));
     }
     HIP_TRY(dalloc(&c->ctl, 1));
     HIP_TRY(hipMemset(c->ctl, 0, sizeof(TreeCtl)));
@@ -275,15 +281,16 @@ hipError_t host_sync_point(bh_context *c)
 }
 
 // Profiling: time one histogram launch with hipEvents on its stream and count its
-// algorithmic bytes: rows x (F + gradient pair bytes) read + one 8-byte cell per
-// (built node, feature, bin) written.  Rows are physically compacted per node, so no
-// row-index bytes are read.
+// algorithmic bytes: rows scanned x (F + gradient pair bytes) read + one 8-byte cell per
+// (built node, feature, bin) written.  Rows are physically compacted per node (or, for
+// a routed unit, per parent: it scans the parent's whole segment), so no row-index
+// bytes are read.
 void profiled_hist(bh_context *c, void *hist_next, u32 grid_units, u32 *cnt_next)
 {
     (void)hipEventRecord(c->ev0, c->stream);
     launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
                 gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), grid_units,
-                c->partials, hist_next, cnt_next);
+                c->partials, hist_next, cnt_next, c->ops);
     (void)hipEventRecord(c->ev1, c->stream);
     (void)hipEventSynchronize(c->ev1);
     float ms = 0.0F;
@@ -319,7 +326,7 @@ void enqueue_build(bh_context *c, void *hist_cur, void *hist_next, u32 max_ops, 
     else
         launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
                     gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), grid_units,
-                    c->partials, hist_next, cnt_next);
+                    c->partials, hist_next, cnt_next, c->ops);
     launch_finish(c->stream, c->quant, c->fops, c->ctl, c->partials, hist_cur, hist_next, c->node_cells(), max_ops,
                   cnt_cur, cnt_next);
 }
@@ -329,13 +336,19 @@ u32 part_grid(bh_context *c) { return std::max(1u, std::min(c->max_pch, c->tune.
 
 // Quantized depthwise: the histograms carry row counts, so each level is find (child
 // counts from the count cells) -> plan (one workgroup: tree, demotion, next frontier,
-// work lists) -> per-chunk left counts -> scatter (chunk prefixes summed in-kernel) ->
+// work lists) -> per-chunk counts -> scatter (chunk prefixes summed in-kernel) ->
 // histograms of the smaller children.  The last level moves no rows and reads none.
+// With paired levels (rows of at most 64 bytes) the rows move after every second level
+// only: a scan of the parent segment is cheaper than a partition pass.
 void enqueue_depthwise_counted(bh_context *c)
 {
     hipStream_t s = c->stream;
     u32 const D = static_cast<u32>(c->fc.max_depth);
     u32 const gh_bytes = 4;
+    // Paired levels: even levels leave the rows where they are (their children's units
+    // scan the parent segments, routed), odd levels move them two splits at once.
+    bool const paired = c->tune.pair_levels != 0 && c->Fs <= 64;
+    u32 moves = 0; // partitions so far: they alternate between the two work planes
     launch_tree_begin(s, c->ctl, c->front[0], c->tree, c->hch, c->fops, static_cast<u32>(c->N));
     enqueue_build(c, c->hist[1], c->hist[0], 1, hist_grid(c), c->cnt[1], c->cnt[0]);
     for (u32 d = 0; d < D; ++d) {
@@ -344,18 +357,33 @@ void enqueue_depthwise_counted(bh_context *c)
         u32 *ccur = c->cnt[d & 1u], *cnext = c->cnt[(d + 1) & 1u];
         u32 const width = 1u << std::min(d, 14u);
         bool const last = d + 1 == D;
+        u32 const mode = !paired || last ? 0u : ((d & 1u) ? 2u : 1u);
+        u32 const dst = paired ? (moves & 1u) : (d & 1u);
         launch_find(s, true, cur, c->ctl, hcur, c->d_nbins, c->F, c->SB, c->gc, c->scales, c->splits, width, 0, 0,
                     c->fscr, ccur);
         launch_plan_counted(s, cur, c->splits, c->ctl, c->tree, c->ops, c->pch, nxt, c->hch, c->fops, c->d_nbins,
-                            c->gc, c->scales, d & 1u, last ? 1u : 0u, kPartChunk);
+                            c->gc, c->scales, dst, last ? 1u : 0u, kPartChunk, mode, c->ops4);
         if (c->fc.host_sync) host_sync_point(c);
         if (!last) {
-            launch_part_count(s, c->pch, c->ops, c->ctl, c->work[0], c->work[1], c->plane0, c->Fs, c->chunk_left,
-                              part_grid(c));
-            launch_part_scatter(s, c->pch, c->ops, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
-                                gh_src(c, 1), gh_src(c, 2), c->work[0], c->work[1],
-                                const_cast<void *>(gh_src(c, 0)), const_cast<void *>(gh_src(c, 1)), c->Fs,
-                                gh_bytes, part_grid(c), c->chunk_left);
+            if (mode == 0) {
+                launch_part_count(s, c->pch, c->ops, c->ctl, c->work[0], c->work[1], c->plane0, c->Fs,
+                                  c->chunk_left, part_grid(c));
+                launch_part_scatter(s, c->pch, c->ops, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
+                                    gh_src(c, 1), gh_src(c, 2), c->work[0], c->work[1],
+                                    const_cast<void *>(gh_src(c, 0)), const_cast<void *>(gh_src(c, 1)), c->Fs,
+                                    gh_bytes, part_grid(c), c->chunk_left);
+                ++moves;
+            } else if (mode == 2) {
+                launch_part_count4(s, c->pch, c->ops4, c->ctl, c->work[0], c->work[1], c->plane0, c->Fs,
+                                   c->chunk_cls, part_grid(c));
+                launch_part_scatter4(s, c->pch, c->ops4, c->ctl, c->work[0], c->work[1], c->plane0,
+                                     static_cast<u32 const *>(gh_src(c, 0)), static_cast<u32 const *>(gh_src(c, 1)),
+                                     static_cast<u32 const *>(gh_src(c, 2)), c->work[0], c->work[1],
+                                     static_cast<u32 *>(const_cast<void *>(gh_src(c, 0))),
+                                     static_cast<u32 *>(const_cast<void *>(gh_src(c, 1))), c->Fs, part_grid(c),
+                                     c->chunk_cls);
+                ++moves;
+            }
             enqueue_build(c, hcur, hnext, std::min(width, c->max_ops), hist_grid(c), ccur, cnext);
         }
         launch_next_level(s, c->ctl);
@@ -826,6 +854,7 @@ static int validate_tuning(bh_tuning const *t)
         return fail(1, "hist_threads must be 256, 512 or 1024");
     if (t->part_grid < 64u || t->part_grid > 65536u) return fail(1, "part_grid must be in [64, 65536]");
     if (t->hist_layout > 1u) return fail(1, "hist_layout must be 0 or 1");
+    if (t->pair_levels > 1u) return fail(1, "pair_levels must be 0 or 1");
     return 0;
 }
 
@@ -835,7 +864,8 @@ int bh_set_tuning(bh_context *c, const bh_tuning *t)
     if (int r = validate_tuning(t)) return r;
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    c->tune = Tuning{t->scatter_lds_bytes, t->find_blocks, t->hist_threads, t->part_grid, t->hist_layout};
+    c->tune = Tuning{t->scatter_lds_bytes, t->find_blocks, t->hist_threads, t->part_grid, t->hist_layout,
+                     t->pair_levels};
     drop_graph(c); // the iteration graph is re-captured with the new shapes
     return 0;
 }
@@ -844,7 +874,7 @@ int bh_get_tuning(bh_context *c, bh_tuning *t)
 {
     if (!c) return fail(1, "null context");
     *t = bh_tuning{c->tune.scatter_lds, c->tune.find_blocks, c->tune.hist_threads, c->tune.part_grid,
-                   c->tune.hist_layout};
+                   c->tune.hist_layout, c->tune.pair_levels};
     return 0;
 }
 
@@ -900,16 +930,17 @@ int bh_autotune(bh_context *c, uint32_t iters, bh_tuning *chosen, float *ms_per_
     static const u32 fb_c[] = {1u, 2u, 4u, 8u};
     static const u32 pg_c[] = {512u, 1024u, 2048u, 4096u};
     static const u32 hl_c[] = {0u, 1u};
-    for (int knob = 0; rc == 0 && knob < 5; ++knob) {
+    for (int knob = 0; rc == 0 && knob < 6; ++knob) {
         u32 const *cand = knob == 0 ? lds_c : knob == 1 ? thr_c : knob == 2 ? fb_c : knob == 3 ? pg_c : hl_c;
-        int const nc = knob == 0 ? 7 : knob == 1 ? 3 : knob == 4 ? 2 : 4;
+        int const nc = knob == 0 ? 7 : knob == 1 ? 3 : knob >= 4 ? 2 : 4;
         for (int k = 0; rc == 0 && k < nc; ++k) {
             Tuning t = best;
             u32 &slot = knob == 0   ? t.scatter_lds
                         : knob == 1 ? t.hist_threads
                         : knob == 2 ? t.find_blocks
                         : knob == 3 ? t.part_grid
-                                    : t.hist_layout;
+                        : knob == 4 ? t.hist_layout
+                                    : t.pair_levels;
             if (slot == cand[k]) continue;
             slot = cand[k];
             float ms = 0.0F;
@@ -929,7 +960,8 @@ int bh_autotune(bh_context *c, uint32_t iters, bh_tuning *chosen, float *ms_per_
     if (rr) return rr;
     set_tuning(c->tune);
     if (chosen)
-        *chosen = bh_tuning{best.scatter_lds, best.find_blocks, best.hist_threads, best.part_grid, best.hist_layout};
+        *chosen = bh_tuning{best.scatter_lds, best.find_blocks, best.hist_threads, best.part_grid, best.hist_layout,
+                            best.pair_levels};
     if (ms_per_iter) *ms_per_iter = best_ms;
     return 0;
 }

@@ -403,12 +403,71 @@ __device__ __forceinline__ i64 unpack_g(u64 v) { return static_cast<i64>(v) >> k
 __device__ __forceinline__ i64 unpack_h(u64 v) { return static_cast<i64>(v & kPackHMask); }
 __device__ __forceinline__ u32 unpack_cnt(u64 v) { return static_cast<u32>((v >> kPackCntShift) & kPackCntMask); }
 
+// Rows of a unit whose node has not been moved out of its parent's segment count only
+// when the parent's split sends them to the node's side.  The routing byte of a row
+// sits in one lane of the row's lane group; a wave ballot hands its verdict to the
+// group (a feature outside the slice is read directly).
+struct HqRoute {
+    u32 feature, bin, dl, last, want_left, in_slice, lane_d, shift;
+};
+
+template <bool ROUTED>
+__device__ __forceinline__ void hist_q_rows(unsigned long long *__restrict__ lh, u8 const *__restrict__ plane,
+                                            u32 const *__restrict__ ghv, u8 const *__restrict__ cb, u32 Fs,
+                                            u32 SB, u32 nf, u32 d, u32 lr, u32 Dw, u32 r, u32 end, u32 stride,
+                                            HqRoute const &rt)
+{
+    auto keep_row = [&](u32 dwv, u32 row) -> bool {
+        if (!ROUTED) return true;
+        if (rt.in_slice) {
+            bool const mine = d == rt.lane_d &&
+                              routes_left((dwv >> rt.shift) & 0xFFu, rt.last, rt.bin, rt.dl) == (rt.want_left != 0);
+            u64 const m = __ballot(mine);
+            return ((m >> (lr * Dw + rt.lane_d)) & 1ull) != 0;
+        }
+        u32 const b = plane[static_cast<size_t>(row) * Fs + rt.feature];
+        return routes_left(b, rt.last, rt.bin, rt.dl) == (rt.want_left != 0);
+    };
+    for (; r + (kHqUnroll - 1) * stride < end; r += kHqUnroll * stride) {
+        u32 dw[kHqUnroll], v[kHqUnroll];
+#pragma unroll
+        for (int u = 0; u < kHqUnroll; ++u) {
+            u32 const rr = r + u * stride;
+            dw[u] = *reinterpret_cast<u32 const *>(cb + static_cast<size_t>(rr) * Fs);
+            v[u] = ghv[rr];
+        }
+#pragma unroll
+        for (int u = 0; u < kHqUnroll; ++u) {
+            if (!keep_row(dw[u], r + u * stride)) continue;
+            u64 const pk = pack_q(v[u]);
+#pragma unroll
+            for (u32 k = 0; k < 4; ++k) {
+                u32 const fl = 4u * d + k;
+                if (fl < nf)
+                    atomicAdd(lh + fl * SB + ((dw[u] >> (8u * k)) & 0xFFu), static_cast<unsigned long long>(pk));
+            }
+        }
+    }
+    for (; r < end; r += stride) {
+        u32 const dw1 = *reinterpret_cast<u32 const *>(cb + static_cast<size_t>(r) * Fs);
+        u32 const v1 = ghv[r];
+        if (!keep_row(dw1, r)) continue;
+        u64 const pk = pack_q(v1);
+#pragma unroll
+        for (u32 k = 0; k < 4; ++k) {
+            u32 const fl = 4u * d + k;
+            if (fl < nf)
+                atomicAdd(lh + fl * SB + ((dw1 >> (8u * k)) & 0xFFu), static_cast<unsigned long long>(pk));
+        }
+    }
+}
+
 __global__ __launch_bounds__(kHqThreads) void k_hist_q(
     HistChunk const *__restrict__ chunks, TreeCtl const *__restrict__ ctl,
     u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
     void const *__restrict__ gw0, void const *__restrict__ gw1, void const *__restrict__ gwp,
     u32 Fs, u32 F, u32 SB, u32 fsl, void *__restrict__ partials, void *__restrict__ hist_next,
-    u32 *__restrict__ cnt_next)
+    u32 *__restrict__ cnt_next, PartOp const *__restrict__ routes)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned long long *lh = reinterpret_cast<unsigned long long *>(smem);   // fsl x SB
@@ -433,38 +492,22 @@ __global__ __launch_bounds__(kHqThreads) void k_hist_q(
         u32 const *ghv = static_cast<u32 const *>(ch.src == 0 ? gw0 : (ch.src == 1 ? gw1 : gwp));
         u8 const *cb = plane + f_begin + 4u * d;
         u32 const stride = NW * R;
-        u32 r = ch.begin + wave * R + lr;
+        u32 const r = ch.begin + wave * R + lr;
         if (lane_ok) {
-            for (; r + (kHqUnroll - 1) * stride < ch.end; r += kHqUnroll * stride) {
-                u32 dw[kHqUnroll], v[kHqUnroll];
-#pragma unroll
-                for (int u = 0; u < kHqUnroll; ++u) {
-                    u32 const rr = r + u * stride;
-                    dw[u] = *reinterpret_cast<u32 const *>(cb + static_cast<size_t>(rr) * Fs);
-                    v[u] = ghv[rr];
-                }
-#pragma unroll
-                for (int u = 0; u < kHqUnroll; ++u) {
-                    u64 const pk = pack_q(v[u]);
-#pragma unroll
-                    for (u32 k = 0; k < 4; ++k) {
-                        u32 const fl = 4u * d + k;
-                        if (fl < nf)
-                            atomicAdd(lh + fl * SB + ((dw[u] >> (8u * k)) & 0xFFu),
-                                      static_cast<unsigned long long>(pk));
-                    }
-                }
-            }
-            for (; r < ch.end; r += stride) {
-                u32 const dw1 = *reinterpret_cast<u32 const *>(cb + static_cast<size_t>(r) * Fs);
-                u32 const v1 = ghv[r];
-                u64 const pk = pack_q(v1);
-#pragma unroll
-                for (u32 k = 0; k < 4; ++k) {
-                    u32 const fl = 4u * d + k;
-                    if (fl < nf)
-                        atomicAdd(lh + fl * SB + ((dw1 >> (8u * k)) & 0xFFu), static_cast<unsigned long long>(pk));
-                }
+            HqRoute rt{};
+            if (ch.route != kNone) {
+                PartOp const &op = routes[ch.route];
+                rt.feature = op.feature;
+                rt.bin = op.bin;
+                rt.dl = op.default_left;
+                rt.last = op.last_bin;
+                rt.want_left = ch.side == 0 ? 1u : 0u;
+                rt.in_slice = op.feature >= f_begin && op.feature < f_begin + nf ? 1u : 0u;
+                rt.lane_d = (op.feature - f_begin) >> 2;
+                rt.shift = 8u * ((op.feature - f_begin) & 3u);
+                hist_q_rows<true>(lh, plane, ghv, cb, Fs, SB, nf, d, lr, Dw, r, ch.end, stride, rt);
+            } else {
+                hist_q_rows<false>(lh, plane, ghv, cb, Fs, SB, nf, d, lr, Dw, r, ch.end, stride, rt);
             }
         }
         __syncthreads();
@@ -503,7 +546,7 @@ __global__ __launch_bounds__(512) void k_hist_q2(
     u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
     void const *__restrict__ gw0, void const *__restrict__ gw1, void const *__restrict__ gwp,
     u32 Fs, u32 F, u32 SB, u32 tile, void *__restrict__ partials, void *__restrict__ hist_next,
-    u32 *__restrict__ cnt_next)
+    u32 *__restrict__ cnt_next, PartOp const *__restrict__ routes)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned long long *lh = reinterpret_cast<unsigned long long *>(smem);            // SB x 32
@@ -522,6 +565,11 @@ __global__ __launch_bounds__(512) void k_hist_q2(
         for (u32 i = threadIdx.x; i < lds_cells; i += blockDim.x) lh[i] = 0ull;
         u8 const *plane = ch.src == 0 ? pl0 : (ch.src == 1 ? pl1 : plp);
         u8 const *ghb = static_cast<u8 const *>(ch.src == 0 ? gw0 : (ch.src == 1 ? gw1 : gwp));
+        // Routed unit: a row counts only when the parent's split sends it to ch.side.
+        bool const routed = ch.route != kNone;
+        PartOp rop{};
+        if (routed) rop = routes[ch.route];
+        bool const want_left = ch.side == 0;
         for (u32 t0 = ch.begin; t0 < ch.end; t0 += tile) {
             u32 const rows = min(tile, ch.end - t0);
             __syncthreads(); // zeroing done / previous tile consumed
@@ -529,7 +577,12 @@ __global__ __launch_bounds__(512) void k_hist_q2(
             u32 const shg = lds_load_range(ghb, static_cast<size_t>(t0) * 4u, rows * 4u, s_gh);
             __syncthreads();
             u8 const *rb = reinterpret_cast<u8 const *>(s_rows) + sh + col;
+            u8 const *rtb = reinterpret_cast<u8 const *>(s_rows) + sh + rop.feature;
             u32 const *gb = reinterpret_cast<u32 const *>(reinterpret_cast<u8 const *>(s_gh) + shg);
+            auto keep = [&](u32 rr) -> bool {
+                return !routed ||
+                       routes_left(rtb[rr * Fs], rop.last_bin, rop.bin, rop.default_left) == want_left;
+            };
             if (col_ok) {
                 u32 const stride = 2u * NW;
                 u32 r = 2u * wave + half;
@@ -543,11 +596,14 @@ __global__ __launch_bounds__(512) void k_hist_q2(
                     }
 #pragma unroll
                     for (int u = 0; u < kHq2Unroll; ++u)
-                        atomicAdd(lh + bin[u] * kHq2Cols + col, static_cast<unsigned long long>(pack_q(v[u])));
+                        if (keep(r + u * stride))
+                            atomicAdd(lh + bin[u] * kHq2Cols + col,
+                                      static_cast<unsigned long long>(pack_q(v[u])));
                 }
                 for (; r < rows; r += stride)
-                    atomicAdd(lh + static_cast<u32>(rb[r * Fs]) * kHq2Cols + col,
-                              static_cast<unsigned long long>(pack_q(gb[r])));
+                    if (keep(r))
+                        atomicAdd(lh + static_cast<u32>(rb[r * Fs]) * kHq2Cols + col,
+                                  static_cast<unsigned long long>(pack_q(gb[r])));
             }
         }
         __syncthreads();
@@ -1668,6 +1724,195 @@ __global__ __launch_bounds__(256) void k_part_scatter_v(
 }
 
 // ---------------------------------------------------------------------------------
+// Four-way partition (two levels of splits in one pass, PartOp4).  Row classes: 0 AL,
+// 1 AR, 2 BL, 3 BR, 4 = the row's child did not split (the row is not moved).
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ u32 route4(u8 const *row, PartOp4 const &q)
+{
+    if (routes_left(row[q.feature], q.last_bin, q.bin, q.default_left)) {
+        if (!(q.alive & 1u)) return 4u;
+        return routes_left(row[q.fa], q.la, q.ba, q.da) ? 0u : 1u;
+    }
+    if (!(q.alive & 2u)) return 4u;
+    return routes_left(row[q.fb], q.lb, q.bb, q.db) ? 2u : 3u;
+}
+
+// Rows per class of every partition chunk.
+__global__ __launch_bounds__(256) void k_part_count4(PartChunk const *__restrict__ pch,
+                                                     PartOp4 const *__restrict__ ops4,
+                                                     TreeCtl const *__restrict__ ctl, u8 const *__restrict__ pl0,
+                                                     u8 const *__restrict__ pl1, u8 const *__restrict__ plp, u32 Fs,
+                                                     uint4 *__restrict__ chunk_cls)
+{
+    __shared__ u32 red[4][4];
+    u32 const n = ctl->n_part_chunks;
+    u32 const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (u32 c = blockIdx.x; c < n; c += gridDim.x) {
+        PartChunk const ch = pch[c];
+        PartOp4 const q = ops4[ch.op];
+        u8 const *plane = q.src == 0 ? pl0 : (q.src == 1 ? pl1 : plp);
+        u32 k0 = 0, k1 = 0, k2 = 0, k3 = 0;
+        for (u32 k = ch.begin + threadIdx.x; k < ch.end; k += blockDim.x) {
+            u32 const cls = route4(plane + static_cast<size_t>(q.off + k) * Fs, q);
+            k0 += cls == 0u;
+            k1 += cls == 1u;
+            k2 += cls == 2u;
+            k3 += cls == 3u;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            k0 += __shfl_xor(k0, o);
+            k1 += __shfl_xor(k1, o);
+            k2 += __shfl_xor(k2, o);
+            k3 += __shfl_xor(k3, o);
+        }
+        if (lane == 0) {
+            red[0][wave] = k0;
+            red[1][wave] = k1;
+            red[2][wave] = k2;
+            red[3][wave] = k3;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint4 t;
+            t.x = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+            t.y = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+            t.z = red[2][0] + red[2][1] + red[2][2] + red[2][3];
+            t.w = red[3][0] + red[3][1] + red[3][2] + red[3][3];
+            chunk_cls[c] = t;
+        }
+        __syncthreads();
+    }
+}
+
+// Per-class rows of an op's chunks first..c-1, summed by the whole block (uses
+// wsum[0 .. 4 * waves)).  Every thread receives the totals.
+__device__ uint4 chunk_prefix4(uint4 const *__restrict__ chunk_cls, u32 first, u32 c, u32 *wsum)
+{
+    uint4 p = make_uint4(0, 0, 0, 0);
+    for (u32 j = first + threadIdx.x; j < c; j += blockDim.x) {
+        uint4 const v = chunk_cls[j];
+        p.x += v.x;
+        p.y += v.y;
+        p.z += v.z;
+        p.w += v.w;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        p.x += __shfl_xor(p.x, o);
+        p.y += __shfl_xor(p.y, o);
+        p.z += __shfl_xor(p.z, o);
+        p.w += __shfl_xor(p.w, o);
+    }
+    u32 const waves = blockDim.x / 64u, wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0) {
+        wsum[0 * waves + wave] = p.x;
+        wsum[1 * waves + wave] = p.y;
+        wsum[2 * waves + wave] = p.z;
+        wsum[3 * waves + wave] = p.w;
+    }
+    __syncthreads();
+    uint4 t = make_uint4(0, 0, 0, 0);
+    for (u32 k = 0; k < waves; ++k) {
+        t.x += wsum[0 * waves + k];
+        t.y += wsum[1 * waves + k];
+        t.z += wsum[2 * waves + k];
+        t.w += wsum[3 * waves + k];
+    }
+    __syncthreads();
+    return t;
+}
+
+// Narrow rows (Fs <= 64), LDS-staged like k_part_scatter_v: a tile is read with
+// aligned 16-byte loads, its rows are ranked per class with wave ballots (stable within
+// the class), and each class run is written as one contiguous destination range.
+__global__ __launch_bounds__(256) void k_part_scatter4(
+    PartChunk const *__restrict__ pch, PartOp4 const *__restrict__ ops4, TreeCtl *__restrict__ ctl,
+    u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
+    u32 const *__restrict__ gw0, u32 const *__restrict__ gw1, u32 const *__restrict__ gwp,
+    u8 *__restrict__ out0, u8 *__restrict__ out1, u32 *__restrict__ go0, u32 *__restrict__ go1, u32 Fs,
+    u32 tile, u64 magic_d, uint4 const *__restrict__ chunk_cls)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint4 *s_rows = reinterpret_cast<uint4 *>(smem);                  // tile * Fs + 16 bytes
+    uint4 *s_gh = reinterpret_cast<uint4 *>(smem + tile * Fs + 16u);  // tile * 4 + 16
+    u16 *s_idx = reinterpret_cast<u16 *>(smem + tile * (Fs + 4u) + 32u); // 4 x tile
+    __shared__ u32 s_wsum[16];
+    u32 const n = ctl->n_part_chunks;
+    u32 const D = Fs / 4u;
+    u32 const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    u32 const waves = blockDim.x / 64u;
+    for (u32 c = blockIdx.x; c < n; c += gridDim.x) {
+        PartChunk const ch = pch[c];
+        PartOp4 const q = ops4[ch.op];
+        u8 const *plane = q.src == 0 ? pl0 : (q.src == 1 ? pl1 : plp);
+        u32 const *gin = q.src == 0 ? gw0 : (q.src == 1 ? gw1 : gwp);
+        u32 *oplane = reinterpret_cast<u32 *>(q.dst == 0 ? out0 : out1);
+        u32 *gout = q.dst == 0 ? go0 : go1;
+        // Class run starts inside the segment and the room each class has.
+        u32 const n_ar = q.n_a - q.n_al, n_b = q.cnt - q.n_a, n_br = n_b - q.n_bl;
+        uint4 run = chunk_prefix4(chunk_cls, q.chunk_first, c, s_wsum);
+        uint4 const mine = chunk_cls[c];
+        bool const fits = q.n_a <= q.cnt && q.n_al <= q.n_a && q.n_bl <= n_b && run.x + mine.x <= q.n_al &&
+                          run.y + mine.y <= n_ar && run.z + mine.z <= q.n_bl && run.w + mine.w <= n_br;
+        if (!fits) {
+            if (threadIdx.x == 0) atomicExch(&ctl->part_error, 1u);
+            continue; // uniform across the block
+        }
+        size_t const base[4] = {static_cast<size_t>(q.off), static_cast<size_t>(q.off) + q.n_al,
+                                static_cast<size_t>(q.off) + q.n_a, static_cast<size_t>(q.off) + q.n_a + q.n_bl};
+        for (u32 t0 = ch.begin; t0 < ch.end; t0 += tile) {
+            u32 const rows = min(tile, ch.end - t0);
+            size_t const srow0 = static_cast<size_t>(q.off) + t0;
+            u32 const sh = lds_load_range(plane, srow0 * Fs, rows * Fs, s_rows);
+            u32 const shg = lds_load_range(reinterpret_cast<u8 const *>(gin), srow0 * 4u, rows * 4u, s_gh);
+            __syncthreads();
+            u8 const *srb = reinterpret_cast<u8 const *>(s_rows) + sh;
+            u32 tr[4] = {0, 0, 0, 0}; // rows of each class ranked so far in this tile
+            for (u32 p0 = 0; p0 < rows; p0 += blockDim.x) {
+                u32 const r = p0 + threadIdx.x;
+                u32 const cls = r < rows ? route4(srb + r * Fs, q) : 4u;
+                u32 rank = 0;
+#pragma unroll
+                for (u32 m = 0; m < 4; ++m) {
+                    u64 const bal = __ballot(cls == m);
+                    if (cls == m) rank = static_cast<u32>(__popcll(bal & ((1ull << lane) - 1ull)));
+                    if (lane == 0) s_wsum[m * 4u + wave] = static_cast<u32>(__popcll(bal));
+                }
+                __syncthreads();
+                u32 pre = 0;
+                if (cls < 4u) {
+                    pre = cls == 0u ? tr[0] : (cls == 1u ? tr[1] : (cls == 2u ? tr[2] : tr[3]));
+                    for (u32 k = 0; k < wave; ++k) pre += s_wsum[cls * 4u + k];
+                    s_idx[cls * tile + pre + rank] = static_cast<u16>(r);
+                }
+#pragma unroll
+                for (u32 m = 0; m < 4; ++m)
+                    for (u32 k = 0; k < waves; ++k) tr[m] += s_wsum[m * 4u + k];
+                __syncthreads();
+            }
+            u32 const *sr = reinterpret_cast<u32 const *>(s_rows);
+            u32 const *sg = reinterpret_cast<u32 const *>(s_gh);
+            u32 const done[4] = {run.x, run.y, run.z, run.w};
+            // The tile's runs stay inside their class ranges (they do whenever the
+            // chunk's counts matched its rows; checked again before any store).
+            bool const tile_fits = run.x + tr[0] <= q.n_al && run.y + tr[1] <= n_ar && run.z + tr[2] <= q.n_bl &&
+                                   run.w + tr[3] <= n_br;
+            if (!tile_fits && threadIdx.x == 0) atomicExch(&ctl->part_error, 1u);
+#pragma unroll
+            for (u32 m = 0; m < 4 && tile_fits; ++m) {
+                size_t const d0 = base[m] + done[m];
+                store_run(oplane, d0 * D, tr[m] * D, D, magic_d, sr, sh >> 2, s_idx + m * tile);
+                store_run(gout, d0, tr[m], 1u, 1ull << 32, sg, shg >> 2, s_idx + m * tile);
+            }
+            run.x += tr[0];
+            run.y += tr[1];
+            run.z += tr[2];
+            run.w += tr[3];
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
 // Counted quantized level (depthwise): the split finder already derived each split's
 // child row counts from the histogram count cells, so one workgroup plans the whole
 // level -- tree nodes (commit_split_node, src/grower_impl.hpp:188-210), empty-split
@@ -1676,12 +1921,29 @@ __global__ __launch_bounds__(256) void k_part_scatter_v(
 // the histogram units of the smaller children.  Only the partition's per-chunk left
 // counts still need a pass over the rows (none at the last level, where nothing moves).
 // chunk_rows = rows per partition chunk.
+//
+// mode 0: this level's splits move the rows (two-way partition ops, ops[]).
+// mode 1: rows stay put; the children's histogram units scan this level's segments and
+//         keep the rows routed to the small child (ops[] holds the routing splits).
+// mode 2: this level's nodes still share their parent's segment; one four-way
+//         partition per parent applies the parent's split (ops[] from mode 1) and this
+//         level's splits together (ops4[]).  Node offsets set in mode 1 are the ones
+//         the nodes get from this partition, so the next frontier is the same as mode 0's.
+// ---------------------------------------------------------------------------------
+
+// A counted split that moves rows: valid and not demoted (both children non-empty, or
+// both empty).
+__device__ __forceinline__ bool counted_alive(SplitRec const &sp, NodeInfo const &nd)
+{
+    u32 const nL = sp.n_left, nR = nd.cnt - sp.n_left;
+    return sp.valid && ((nL == 0) == (nR == 0));
+}
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_plan_counted(
     NodeInfo const *__restrict__ front, SplitRec const *__restrict__ splits, TreeCtl *__restrict__ ctl,
     DevNode *__restrict__ tree, PartOp *__restrict__ ops, PartChunk *__restrict__ pch, NodeInfo *__restrict__ next,
     HistChunk *__restrict__ hch, FinishOp *__restrict__ fops, u16 const *__restrict__ n_bins, GrowCfg cfg,
-    double const *__restrict__ scales, u32 dst, u32 is_last, u32 chunk_rows)
+    double const *__restrict__ scales, u32 dst, u32 is_last, u32 chunk_rows, u32 mode, PartOp4 *__restrict__ ops4)
 {
     __shared__ u32 scratch[1024];
     __shared__ u32 s_base_nodes, s_valid, s_alive, s_chunks, s_units, s_parts;
@@ -1708,8 +1970,9 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
         bool const left_small = nL <= nR;
         u32 const small_cnt = left_small ? nL : nR;
         if (alive && !is_last) {
-            nch = (nd.cnt + chunk_rows - 1) / chunk_rows;
-            n_units = (small_cnt + kHistChunk - 1) / kHistChunk;
+            nch = mode == 0 ? (nd.cnt + chunk_rows - 1) / chunk_rows : 0u;
+            u32 const scan = mode == 1 ? (small_cnt ? nd.cnt : 0u) : small_cnt;
+            n_units = (scan + kHistChunk - 1) / kHistChunk;
             multi = n_units > 1 ? n_units : 0;
         }
         u32 vtot, atot, ctot, htot, ptot;
@@ -1756,7 +2019,7 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
                     op.left_id = lid;
                     op.right_id = rid;
                     op.alive = 1;
-                    ops[arank] = op;
+                    if (mode != 2) ops[arank] = op;
                     NodeInfo L{}, R{};
                     L.id = lid;
                     R.id = rid;
@@ -1764,7 +2027,7 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
                     L.cnt = nL;
                     R.off = nd.off + nL;
                     R.cnt = nR;
-                    L.src = R.src = dst;
+                    L.src = R.src = mode == 1 ? nd.src : dst;
                     L.slot = 2 * arank;
                     R.slot = 2 * arank + 1;
                     L.depth = R.depth = nd.depth + 1;
@@ -1787,9 +2050,11 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
                         fo.empty = small_cnt == 0 ? 1u : 0u;
                         fo.unit_first = hfirst;
                         fo.unit_count = n_units;
-                        fo.small_off = left_small ? L.off : R.off;
-                        fo.small_cnt = small_cnt;
-                        fo.src = dst;
+                        fo.small_off = mode == 1 ? nd.off : (left_small ? L.off : R.off);
+                        fo.small_cnt = mode == 1 ? nd.cnt : small_cnt;
+                        fo.src = L.src;
+                        fo.route = mode == 1 ? arank : kNone;
+                        fo.side = left_small ? 0u : 1u;
                         fops[arank] = fo;
                     }
                 }
@@ -1807,8 +2072,68 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
         }
         __syncthreads();
     }
+    u32 const pairs = mode == 2 && !is_last ? P / 2u : 0u;
+    for (u32 t0 = 0; t0 < pairs; t0 += blockDim.x) {
+        // Four-way ops, one per parent segment (parent j of frontier nodes 2j, 2j+1).
+        u32 const j = t0 + threadIdx.x;
+        bool const in = j < pairs;
+        PartOp4 q{};
+        u32 nch = 0;
+        if (in) {
+            PartOp const g = ops[j];
+            NodeInfo const a = front[2 * j], b = front[2 * j + 1];
+            SplitRec const sa = splits[2 * j], sb = splits[2 * j + 1];
+            bool const aa = counted_alive(sa, a), ab = counted_alive(sb, b);
+            q.off = g.off;
+            q.cnt = g.cnt;
+            q.src = g.src;
+            q.dst = dst;
+            q.feature = g.feature;
+            q.bin = g.bin;
+            q.default_left = g.default_left;
+            q.last_bin = g.last_bin;
+            if (aa) {
+                q.fa = sa.feature;
+                q.ba = sa.bin;
+                q.da = sa.default_left;
+                q.la = static_cast<u32>(n_bins[sa.feature]) - 1u;
+            }
+            if (ab) {
+                q.fb = sb.feature;
+                q.bb = sb.bin;
+                q.db = sb.default_left;
+                q.lb = static_cast<u32>(n_bins[sb.feature]) - 1u;
+            }
+            q.n_a = a.cnt;
+            q.n_al = aa ? sa.n_left : 0u;
+            q.n_bl = ab ? sb.n_left : 0u;
+            q.alive = (aa ? 1u : 0u) | (ab ? 2u : 0u);
+            nch = q.alive ? (g.cnt + chunk_rows - 1) / chunk_rows : 0u;
+        }
+        u32 ctot;
+        u32 const cfirst = block_exscan(nch, scratch, &ctot) + s_chunks;
+        if (in) {
+            q.chunk_first = cfirst;
+            q.chunk_count = nch;
+            ops4[j] = q;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_chunks += ctot;
+        __syncthreads();
+    }
     u32 const n_alive = s_alive, n_ch = s_chunks, n_units_total = s_units;
-    for (u32 c = threadIdx.x; c < n_ch; c += blockDim.x) {
+    for (u32 c = threadIdx.x; c < n_ch && pairs; c += blockDim.x) {
+        u32 const j = find_owner(ops4, pairs, c, [](PartOp4 const &o) { return o.chunk_first; });
+        PartOp4 const q = ops4[j];
+        u32 const k = c - q.chunk_first;
+        PartChunk pc;
+        pc.op = j;
+        pc.begin = k * chunk_rows;
+        pc.end = min(q.cnt, (k + 1) * chunk_rows);
+        pc.left_before = 0;
+        pch[c] = pc;
+    }
+    for (u32 c = threadIdx.x; c < n_ch && !pairs; c += blockDim.x) {
         u32 const j = find_owner(ops, n_alive, c, [](PartOp const &o) { return o.chunk_first; });
         PartOp const op = ops[j];
         u32 const k = c - op.chunk_first;
@@ -1829,6 +2154,8 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
         hc.begin = fo.small_off + k * kHistChunk;
         hc.end = fo.small_off + min(fo.small_cnt, (k + 1) * kHistChunk);
         hc.part = fo.part_count ? fo.part_first + k : kNone;
+        hc.route = fo.route;
+        hc.side = fo.side;
         hch[c] = hc;
     }
     if (threadIdx.x == 0) {
@@ -1839,7 +2166,6 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
         ctl->n_finish = is_last ? 0 : n_alive;
         ctl->n_hist_chunks = is_last ? 0 : n_units_total;
         ctl->n_parts = s_parts;
-        ctl->part_error = 0;
     }
 }
 
@@ -2126,7 +2452,8 @@ void launch_quant_from(hipStream_t s, float2 const *gh, u64 n, u32 *maxbits, u64
 
 hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl const *ctl, u8 const *pl0,
                        u8 const *pl1, u8 const *plp, void const *g0, void const *g1, void const *gp, u32 Fs, u32 F,
-                       u32 SB, u32 fsl, u32 n_waves, u32 grid_x, void *partials, void *hist_next, u32 *cnt_next)
+                       u32 SB, u32 fsl, u32 n_waves, u32 grid_x, void *partials, void *hist_next, u32 *cnt_next,
+                       PartOp const *routes)
 {
     u32 const slices = (F + fsl - 1) / fsl;
     size_t const lds = static_cast<size_t>(fsl) * SB * 8;
@@ -2139,10 +2466,10 @@ hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl c
         tile = std::max(64u, std::min(tile, 1024u));
         size_t const lds2 = static_cast<size_t>(lh_bytes) + tile * (Fs + 4u) + 32u;
         hipLaunchKernelGGL(k_hist_q2, dim3(grid_x), dim3(512), lds2, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp, Fs, F,
-                           SB, tile, partials, hist_next, cnt_next);
+                           SB, tile, partials, hist_next, cnt_next, routes);
     } else if (q) {
         hipLaunchKernelGGL(k_hist_q, grid, dim3(t_tune.hist_threads), lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp,
-                           Fs, F, SB, fsl, partials, hist_next, cnt_next);
+                           Fs, F, SB, fsl, partials, hist_next, cnt_next, routes);
     } else
         hipLaunchKernelGGL((k_hist<false, 4>), grid, block, lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp, Fs, F,
                            SB, fsl, partials, hist_next);
@@ -2273,10 +2600,35 @@ u32 scatter_tile_rows(u32 Fs, u32 gh_bytes)
 void launch_plan_counted(hipStream_t s, NodeInfo const *front, SplitRec const *splits, TreeCtl *ctl, DevNode *tree,
                          PartOp *ops, PartChunk *pch, NodeInfo *next, HistChunk *hch, FinishOp *fops,
                          u16 const *n_bins, GrowCfg const &cfg, double const *scales, u32 dst, u32 is_last,
-                         u32 chunk_rows)
+                         u32 chunk_rows, u32 mode, PartOp4 *ops4)
 {
     hipLaunchKernelGGL(k_plan_counted, dim3(1), dim3(1024), 0, s, front, splits, ctl, tree, ops, pch, next, hch, fops,
-                       n_bins, cfg, scales, dst, is_last, chunk_rows);
+                       n_bins, cfg, scales, dst, is_last, chunk_rows, mode, ops4);
+}
+
+void launch_part_count4(hipStream_t s, PartChunk const *pch, PartOp4 const *ops4, TreeCtl const *ctl, u8 const *pl0,
+                        u8 const *pl1, u8 const *plp, u32 Fs, uint4 *chunk_cls, u32 grid)
+{
+    hipLaunchKernelGGL(k_part_count4, dim3(grid), dim3(256), 0, s, pch, ops4, ctl, pl0, pl1, plp, Fs, chunk_cls);
+}
+
+u32 scatter4_tile_rows(u32 Fs)
+{
+    u32 const per_row = Fs + 4u + 8u;
+    u32 tile = (t_tune.scatter_lds - 32u) / per_row;
+    return std::max(64u, std::min<u32>(kScatTileMax, tile & ~63u));
+}
+
+void launch_part_scatter4(hipStream_t s, PartChunk const *pch, PartOp4 const *ops4, TreeCtl *ctl, u8 const *pl0,
+                          u8 const *pl1, u8 const *plp, u32 const *g0, u32 const *g1, u32 const *gp, u8 *o0,
+                          u8 *o1, u32 *go0, u32 *go1, u32 Fs, u32 grid, uint4 const *chunk_cls)
+{
+    u32 const D = Fs / 4u;
+    u32 const tile = scatter4_tile_rows(Fs);
+    size_t const lds = static_cast<size_t>(tile) * (Fs + 12u) + 32u;
+    u64 const magic_d = ((1ull << 32) + D - 1u) / D;
+    hipLaunchKernelGGL(k_part_scatter4, dim3(grid), dim3(256), lds, s, pch, ops4, ctl, pl0, pl1, plp, g0, g1, gp, o0,
+                       o1, go0, go1, Fs, tile, magic_d, chunk_cls);
 }
 
 

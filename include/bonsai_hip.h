@@ -183,6 +183,8 @@ typedef struct bh_tuning {
     uint32_t hist_threads;      /* quantized histogram workgroup size: 256, 512 or 1024 */
     uint32_t part_grid;         /* partition workgroups */
     uint32_t hist_layout;       /* quantized histogram of rows <= 32 bytes: 1 feature-in-lane, 0 row-dword */
+    uint32_t pair_levels;       /* quantized depthwise, rows <= 64 bytes: 1 = move rows every second level
+                                   (four-way partitions; the levels between scan parent segments) */
 } bh_tuning;
 /* Set / read the context's launch shapes (the iteration graph is re-captured). */
 int bh_set_tuning(bh_context *ctx, const bh_tuning *t);

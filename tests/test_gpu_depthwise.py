@@ -159,11 +159,15 @@ def assert_same_tree(dev, ref):
         assert np.array_equal(a, b), f"field {f} differs"
 
 
-@pytest.mark.parametrize("depth", [1, 4, 7])
-def test_grow_depthwise_int_bit_exact(hip, orc, tall, depth):
+@pytest.mark.parametrize("pair", [1, 0])
+@pytest.mark.parametrize("depth", [1, 2, 4, 7])
+def test_grow_depthwise_int_bit_exact(hip, orc, tall, depth, pair):
+    """pair=1 moves rows every second level (four-way partitions, routed histogram
+    units in between); pair=0 moves them after every split."""
     bins, nb, X, y = tall
     g, h = grad_of(orc, y, seed=depth)
     ctx = ctx_for(hip, bins, nb, quantized=True, max_depth=depth, min_data_in_leaf=20)
+    ctx.set_tuning(pair_levels=pair)
     gq, hq, sc = ctx.set_gradients(g, h)
     nodes, lor = ctx.grow_tree()
     ref, rlor = orc.grow_depthwise(bins, nb, gq=gq, hq=hq, scales=sc, c=orc_cfg(max_depth=depth), mode=HIST_INT)

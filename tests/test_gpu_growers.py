@@ -129,6 +129,22 @@ def test_grow_depthwise_wide_rows(hip, orc, wide, quantized):
     assert np.array_equal(lor, rlor)
 
 
+@pytest.mark.parametrize("pair", [1, 0])
+def test_grow_depthwise_int_50_features(hip, orc, pair):
+    """52-byte rows: paired levels with 13 dwords per row in the histogram lanes."""
+    bins, nb, X, y = wide_data(n=30000, f=50)
+    rng = np.random.default_rng(5)
+    g, h = orc.mse_grad(rng.normal(0, 1, y.size).astype(np.float32), y)
+    ctx = ctx_for(hip, bins, nb, quantized=True, max_depth=6, min_data_in_leaf=5)
+    ctx.set_tuning(pair_levels=pair)
+    gq, hq, sc = ctx.set_gradients(g, h)
+    ref, rlor = orc.grow_depthwise(bins, nb, gq=gq, hq=hq, scales=sc, c=orc_cfg(max_depth=6, min_data=5),
+                                   mode=HIST_INT)
+    nodes, lor = ctx.grow_tree()
+    assert_same_tree(nodes, ref)
+    assert np.array_equal(lor, rlor)
+
+
 @pytest.mark.parametrize("grower", ["hip_leafwise", "hip_levelwise"])
 def test_train_graph_equals_eager(hip, orc, tall, grower):
     bins, nb, X, y = tall

@@ -35,9 +35,11 @@ def fit(hip, orc, bins, nb, y, quantized, tuning=None, autotune=False, iters=4):
 
 
 SHAPES = [
-    dict(scatter_lds_bytes=4096, find_blocks=1, hist_threads=256, part_grid=64, hist_layout=0),
-    dict(scatter_lds_bytes=49152, find_blocks=3, hist_threads=512, part_grid=4096, hist_layout=1),
-    dict(scatter_lds_bytes=147456, find_blocks=8, hist_threads=1024, part_grid=2048, hist_layout=0),
+    dict(scatter_lds_bytes=4096, find_blocks=1, hist_threads=256, part_grid=64, hist_layout=0, pair_levels=1),
+    dict(scatter_lds_bytes=49152, find_blocks=3, hist_threads=512, part_grid=4096, hist_layout=1, pair_levels=1),
+    dict(scatter_lds_bytes=147456, find_blocks=8, hist_threads=1024, part_grid=2048, hist_layout=0,
+         pair_levels=0),
+    dict(scatter_lds_bytes=20480, find_blocks=8, hist_threads=1024, part_grid=2048, hist_layout=1, pair_levels=0),
 ]
 
 
@@ -70,4 +72,6 @@ def test_tuning_validation(hip, tall):
         ctx.set_tuning(hist_threads=300)
     with pytest.raises(hip.ConfigError):
         ctx.set_tuning(find_blocks=9)
+    with pytest.raises(hip.ConfigError):
+        ctx.set_tuning(pair_levels=2)
     ctx.close()
