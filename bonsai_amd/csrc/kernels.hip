@@ -390,7 +390,7 @@ __global__ __launch_bounds__(1024) void k_hist(
 // collisions.  Integer sums are exact and order-free.  The LDS histogram is the slice's
 // [feature][SB] cells; the flush is the one k_hist<true> uses.
 constexpr u32 kHqThreads = 1024;
-constexpr int kHqUnroll = 4;
+constexpr int kHqUnroll = 8;
 
 // One row's packed contribution (see kPackGShift): the gradient word carries int16 g in
 // its high half and uint16 h in its low half.
