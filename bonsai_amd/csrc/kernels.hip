@@ -1601,46 +1601,62 @@ __global__ __launch_bounds__(256) void k_part_scatter(
 // contiguous destination range written with aligned 16-byte stores (dword head and tail
 // at the run's edges).  Planes and gradient buffers carry 64 bytes of tail padding, so
 // the aligned over-read past a tile's last row stays inside the allocation.
+
 // k / D as (k * ceil(2^32 / D)) >> 32 in 64-bit arithmetic: exact for k < 2^16, D <= 16.
-
-
-// dst[first + k] = src[base + idx[k / D] * D + k % D] for k < n (dword units).
 __device__ __forceinline__ u32 div_magic(u32 k, u64 magic)
 {
     return static_cast<u32>((static_cast<u64>(k) * magic) >> 32);
 }
 
-__device__ __forceinline__ void store_run(u32 *__restrict__ dst, size_t first, u32 n, u32 D, u64 magic,
-                                          u32 const *src, u32 base, u16 const *idx)
+// NR runs in one pass: run m writes n[m] dwords to dst[first[m] ...], dword k reading
+// src[base + idx_m[k / D] * D + k % D] with idx_m = idx + m * idx_stride.  The aligned
+// 16-byte stores of all runs share one index space dealt round-robin over the threads
+// (short runs leave no threads idle); the unaligned head and tail dwords of every run
+// are one more step.  Run indices stay wave-uniform: no indexed register arrays.
+template <int NR> struct Runs {
+    size_t first[NR]; // first destination dword of each run
+    u32 n[NR];        // dwords of each run
+};
+
+template <int NR>
+__device__ __forceinline__ void store_runs(u32 *__restrict__ dst, Runs<NR> const rs, u32 D, u64 magic,
+                                           u32 const *src, u32 base, u16 const *idx, u32 idx_stride)
 {
-    u32 const head = min(n, static_cast<u32>((4u - (first & 3u)) & 3u));
-    if (threadIdx.x < head) {
-        u32 const k = threadIdx.x;
-        u32 const q = div_magic(k, magic);
-        dst[first + k] = src[base + static_cast<u32>(idx[q]) * D + (k - q * D)];
-    }
-    u32 const n_vec = (n - head) >> 2;
-    uint4 *dv = reinterpret_cast<uint4 *>(dst + first + head);
-    for (u32 m = threadIdx.x; m < n_vec; m += blockDim.x) {
-        u32 const k = head + 4u * m;
-        u32 q = div_magic(k, magic);
-        u32 rem = k - q * D;
-        u32 v[4];
+    u32 const T = blockDim.x, t = threadIdx.x;
+    u32 start = 0; // first vector of run m in the shared index space
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            v[j] = src[base + static_cast<u32>(idx[q]) * D + rem];
-            if (++rem == D) {
-                rem = 0;
-                ++q;
+    for (int m = 0; m < NR; ++m) {
+        u32 const n = rs.n[m];
+        size_t const first = rs.first[m];
+        u32 const head = min(n, static_cast<u32>((4u - (first & 3u)) & 3u));
+        u32 const nvec = (n - head) >> 2;
+        u16 const *im = idx + m * idx_stride;
+        if ((t >> 3) == static_cast<u32>(m)) {
+            // e < 4: head dword e; e >= 4: tail dword e - 4
+            u32 const e = t & 7u;
+            u32 const k = e < 4u ? e : head + 4u * nvec + (e - 4u);
+            if (e < 4u ? e < head : k < n) {
+                u32 const q = div_magic(k, magic);
+                dst[first + k] = src[base + static_cast<u32>(im[q]) * D + (k - q * D)];
             }
         }
-        dv[m] = make_uint4(v[0], v[1], v[2], v[3]);
-    }
-    u32 const t0 = head + 4u * n_vec;
-    if (threadIdx.x < n - t0) {
-        u32 const k = t0 + threadIdx.x;
-        u32 const q = div_magic(k, magic);
-        dst[first + k] = src[base + static_cast<u32>(idx[q]) * D + (k - q * D)];
+        u32 const lag = (t + T - start % T) % T; // first vector of this thread in the run
+        for (u32 v = lag; v < nvec; v += T) {
+            u32 const k0 = head + 4u * v;
+            u32 q = div_magic(k0, magic);
+            u32 rem = k0 - q * D;
+            u32 w[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                w[j] = src[base + static_cast<u32>(im[q]) * D + rem];
+                if (++rem == D) {
+                    rem = 0;
+                    ++q;
+                }
+            }
+            *reinterpret_cast<uint4 *>(dst + first + k0) = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+        start += nvec;
     }
 }
 
@@ -1713,10 +1729,8 @@ __global__ __launch_bounds__(256) void k_part_scatter_v(
             size_t const r0 = static_cast<size_t>(op.off) + op.n_left + (t0 - left_run);
             u32 const *sr = reinterpret_cast<u32 const *>(s_rows);
             u32 const *sg = reinterpret_cast<u32 const *>(s_gh);
-            store_run(oplane, l0 * D, nl * D, D, magic_d, sr, sh >> 2, s_lidx);
-            store_run(oplane, r0 * D, nr * D, D, magic_d, sr, sh >> 2, s_ridx);
-            store_run(gout, l0 * G, nl * G, G, magic_g, sg, shg >> 2, s_lidx);
-            store_run(gout, r0 * G, nr * G, G, magic_g, sg, shg >> 2, s_ridx);
+            store_runs<2>(oplane, Runs<2>{{l0 * D, r0 * D}, {nl * D, nr * D}}, D, magic_d, sr, sh >> 2, s_lidx, tile);
+            store_runs<2>(gout, Runs<2>{{l0 * G, r0 * G}, {nl * G, nr * G}}, G, magic_g, sg, shg >> 2, s_lidx, tile);
             left_run += nl;
             __syncthreads();
         }
@@ -1897,11 +1911,13 @@ __global__ __launch_bounds__(256) void k_part_scatter4(
             bool const tile_fits = run.x + tr[0] <= q.n_al && run.y + tr[1] <= n_ar && run.z + tr[2] <= q.n_bl &&
                                    run.w + tr[3] <= n_br;
             if (!tile_fits && threadIdx.x == 0) atomicExch(&ctl->part_error, 1u);
-#pragma unroll
-            for (u32 m = 0; m < 4 && tile_fits; ++m) {
-                size_t const d0 = base[m] + done[m];
-                store_run(oplane, d0 * D, tr[m] * D, D, magic_d, sr, sh >> 2, s_idx + m * tile);
-                store_run(gout, d0, tr[m], 1u, 1ull << 32, sg, shg >> 2, s_idx + m * tile);
+            if (tile_fits) { // uniform across the block
+                size_t const a0 = base[0] + done[0], a1 = base[1] + done[1], a2 = base[2] + done[2],
+                             a3 = base[3] + done[3];
+                store_runs<4>(oplane, Runs<4>{{a0 * D, a1 * D, a2 * D, a3 * D}, {tr[0] * D, tr[1] * D, tr[2] * D, tr[3] * D}},
+                              D, magic_d, sr, sh >> 2, s_idx, tile);
+                store_runs<4>(gout, Runs<4>{{a0, a1, a2, a3}, {tr[0], tr[1], tr[2], tr[3]}}, 1u, 1ull << 32, sg, shg >> 2,
+                              s_idx, tile);
             }
             run.x += tr[0];
             run.y += tr[1];
