@@ -319,7 +319,7 @@ void profiled_hist(bh_context *c, void *hist_next, u32 grid_units, u32 *cnt_next
 
 // Histogram units -> partials / slots, then the fixed-order completion.
 void enqueue_build(bh_context *c, void *hist_cur, void *hist_next, u32 max_ops, u32 grid_units,
-                   u32 const *cnt_cur = nullptr, u32 *cnt_next = nullptr)
+                   u32 const *cnt_cur = nullptr, u32 *cnt_next = nullptr, bool advance = false)
 {
     if (c->prof)
         profiled_hist(c, hist_next, grid_units, cnt_next);
@@ -328,7 +328,7 @@ void enqueue_build(bh_context *c, void *hist_cur, void *hist_next, u32 max_ops, 
                     gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), grid_units,
                     c->partials, hist_next, cnt_next, c->ops);
     launch_finish(c->stream, c->quant, c->fops, c->ctl, c->partials, hist_cur, hist_next, c->node_cells(), max_ops,
-                  cnt_cur, cnt_next);
+                  cnt_cur, cnt_next, advance);
 }
 
 u32 hist_grid(bh_context *c) { return std::max(1u, std::min(c->max_hch, 4096u)); }
@@ -384,9 +384,10 @@ void enqueue_depthwise_counted(bh_context *c)
                                      c->chunk_cls);
                 ++moves;
             }
-            enqueue_build(c, hcur, hnext, std::min(width, c->max_ops), hist_grid(c), ccur, cnext);
+            enqueue_build(c, hcur, hnext, std::min(width, c->max_ops), hist_grid(c), ccur, cnext, true);
+        } else {
+            launch_next_level(s, c->ctl);
         }
-        launch_next_level(s, c->ctl);
         if (c->fc.host_sync) host_sync_point(c);
     }
     launch_finalize(s, c->front[D & 1u], c->ctl, c->tree, c->gc, c->scales, 1u << std::min(D, 14u));

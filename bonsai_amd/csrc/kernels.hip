@@ -736,14 +736,21 @@ __global__ void k_finish(FinishOp const *__restrict__ ops, TreeCtl const *__rest
 constexpr u32 kFqCells = 16;
 constexpr u32 kFqGroups = 16;
 
-__global__ __launch_bounds__(256) void k_finish_q(FinishOp const *__restrict__ ops, TreeCtl const *__restrict__ ctl,
+// advance != 0: the level ends with this launch, and workgroup (0, 0) also moves the
+// control block to the next frontier (no workgroup of the launch reads it).
+__global__ __launch_bounds__(256) void k_finish_q(FinishOp const *__restrict__ ops, TreeCtl *__restrict__ ctl,
                                                   void const *__restrict__ partials, void const *__restrict__ hist_cur,
                                                   void *__restrict__ hist_next, u32 node_cells,
-                                                  u32 const *__restrict__ cnt_cur, u32 *__restrict__ cnt_next)
+                                                  u32 const *__restrict__ cnt_cur, u32 *__restrict__ cnt_next,
+                                                  u32 advance)
 {
     __shared__ i64 s_g[kFqGroups][kFqCells], s_h[kFqGroups][kFqCells];
     __shared__ u32 s_n[kFqGroups][kFqCells];
     u32 const n_ops = ctl->n_finish;
+    if (advance && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        ctl->n_front = ctl->next_front;
+        ctl->depth += 1;
+    }
     u32 const j = threadIdx.x % kFqCells, grp = threadIdx.x / kFqCells;
     u32 const n_tiles = (node_cells + kFqCells - 1) / kFqCells;
     for (u32 o = blockIdx.y; o < n_ops; o += gridDim.y) {
@@ -1170,13 +1177,20 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
     __syncthreads();
     if (s_last == 0u) return;
     __shared__ SplitRec s_win;
-    if (threadIdx.x == 0) {
+    __shared__ SplitRec s_parts[kFindMaxBlocks];
+    // The G partial records are loaded in parallel by lanes of wave 0 (G <= 64), after
+    // one fence in that wave, then compared.
+    if (threadIdx.x < 64u) {
         __threadfence();
+        if (threadIdx.x < G) s_parts[threadIdx.x] = part[static_cast<size_t>(i) * G + threadIdx.x];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
         SplitRec best{};
         best.default_left = 1;
         u32 bkey = kNone;
         for (u32 k = 0; k < G; ++k) {
-            SplitRec const r = part[static_cast<size_t>(i) * G + k];
+            SplitRec const r = s_parts[k];
             if (!r.valid) continue;
             u32 const rk = r.feature * (2u * SB) + r.bin * 2u + (r.default_left ? 0u : 1u);
             if (bkey == kNone || r.gain > best.gain || (r.gain == best.gain && rk < bkey)) {
@@ -1955,6 +1969,8 @@ __device__ __forceinline__ bool counted_alive(SplitRec const &sp, NodeInfo const
     return sp.valid && ((nL == 0) == (nR == 0));
 }
 // ---------------------------------------------------------------------------------
+constexpr u32 kPlanLdsOps = 4096;
+
 __global__ __launch_bounds__(1024) void k_plan_counted(
     NodeInfo const *__restrict__ front, SplitRec const *__restrict__ splits, TreeCtl *__restrict__ ctl,
     DevNode *__restrict__ tree, PartOp *__restrict__ ops, PartChunk *__restrict__ pch, NodeInfo *__restrict__ next,
@@ -1963,6 +1979,9 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
 {
     __shared__ u32 scratch[1024];
     __shared__ u32 s_base_nodes, s_valid, s_alive, s_chunks, s_units, s_parts;
+    // First chunk / unit of each op for the owner searches below (levels of at most
+    // kPlanLdsOps ops; larger ones search the op arrays in global memory).
+    __shared__ u32 s_cfirst[kPlanLdsOps], s_ufirst[kPlanLdsOps];
     u32 const P = ctl->n_front;
     if (threadIdx.x == 0) {
         s_base_nodes = ctl->n_nodes;
@@ -2008,7 +2027,8 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
                     write_leaf(tree, nd.id, node_leaf_value(nd, cfg, scales));
                     tree[nd.id].gain = 0.0F;
                 } else {
-                    DevNode pn = tree[nd.id];
+                    DevNode pn{}; // cover: the node's rows, as its parent set it
+                    pn.cover = static_cast<float>(nd.cnt);
                     pn.feature = static_cast<int32_t>(sp.feature);
                     pn.bin = sp.bin;
                     pn.default_left = sp.default_left;
@@ -2036,6 +2056,10 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
                     op.right_id = rid;
                     op.alive = 1;
                     if (mode != 2) ops[arank] = op;
+                    if (arank < kPlanLdsOps) {
+                        if (mode == 0) s_cfirst[arank] = cfirst;
+                        s_ufirst[arank] = hfirst;
+                    }
                     NodeInfo L{}, R{};
                     L.id = lid;
                     R.id = rid;
@@ -2132,6 +2156,7 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
             q.chunk_first = cfirst;
             q.chunk_count = nch;
             ops4[j] = q;
+            if (j < kPlanLdsOps) s_cfirst[j] = cfirst;
         }
         __syncthreads();
         if (threadIdx.x == 0) s_chunks += ctot;
@@ -2139,7 +2164,9 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
     }
     u32 const n_alive = s_alive, n_ch = s_chunks, n_units_total = s_units;
     for (u32 c = threadIdx.x; c < n_ch && pairs; c += blockDim.x) {
-        u32 const j = find_owner(ops4, pairs, c, [](PartOp4 const &o) { return o.chunk_first; });
+        u32 const j = pairs <= kPlanLdsOps
+                          ? find_owner(s_cfirst, pairs, c, [](u32 v) { return v; })
+                          : find_owner(ops4, pairs, c, [](PartOp4 const &o) { return o.chunk_first; });
         PartOp4 const q = ops4[j];
         u32 const k = c - q.chunk_first;
         PartChunk pc;
@@ -2150,7 +2177,9 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
         pch[c] = pc;
     }
     for (u32 c = threadIdx.x; c < n_ch && !pairs; c += blockDim.x) {
-        u32 const j = find_owner(ops, n_alive, c, [](PartOp const &o) { return o.chunk_first; });
+        u32 const j = n_alive <= kPlanLdsOps
+                          ? find_owner(s_cfirst, n_alive, c, [](u32 v) { return v; })
+                          : find_owner(ops, n_alive, c, [](PartOp const &o) { return o.chunk_first; });
         PartOp const op = ops[j];
         u32 const k = c - op.chunk_first;
         PartChunk pc;
@@ -2161,7 +2190,9 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
         pch[c] = pc;
     }
     for (u32 c = threadIdx.x; c < n_units_total; c += blockDim.x) {
-        u32 const r = find_owner(fops, n_alive, c, [](FinishOp const &o) { return o.unit_first; });
+        u32 const r = n_alive <= kPlanLdsOps
+                          ? find_owner(s_ufirst, n_alive, c, [](u32 v) { return v; })
+                          : find_owner(fops, n_alive, c, [](FinishOp const &o) { return o.unit_first; });
         FinishOp const fo = fops[r];
         u32 const k = c - fo.unit_first;
         HistChunk hc{};
@@ -2521,9 +2552,9 @@ hipError_t set_hist_lds_limits()
     return hipSuccess;
 }
 
-void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl const *ctl, void const *partials,
+void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl *ctl, void const *partials,
                    void const *hist_cur, void *hist_next, u32 node_cells, u32 max_ops, u32 const *cnt_cur,
-                   u32 *cnt_next)
+                   u32 *cnt_next, bool advance)
 {
     u32 const tiles = min(64u, (node_cells + 255) / 256);
     dim3 const grid(tiles, max(1u, min(max_ops, 1024u)));
@@ -2534,10 +2565,12 @@ void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl const *ct
         u32 const cell_tiles = (node_cells + kFqCells - 1) / kFqCells;
         u32 const tx = max(1u, min(cell_tiles, max(1u, 4096u / ops_y)));
         hipLaunchKernelGGL(k_finish_q, dim3(tx, ops_y), dim3(256), 0, s, ops, ctl, partials, hist_cur, hist_next,
-                           node_cells, cnt_cur, cnt_next);
-    } else
+                           node_cells, cnt_cur, cnt_next, advance ? 1u : 0u);
+    } else {
         hipLaunchKernelGGL(k_finish<false>, grid, dim3(256), 0, s, ops, ctl, partials, hist_cur, hist_next,
                            node_cells, nullptr, nullptr);
+        if (advance) launch_next_level(s, ctl);
+    }
 }
 
 void launch_find(hipStream_t s, bool q, NodeInfo *front, TreeCtl const *ctl, void const *hist, u16 const *n_bins,

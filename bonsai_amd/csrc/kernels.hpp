@@ -15,9 +15,10 @@ hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl c
                        u32 SB, u32 fsl, u32 n_waves, u32 grid_x, void *partials, void *hist_next,
                        u32 *cnt_next = nullptr, PartOp const *routes = nullptr);
 hipError_t set_hist_lds_limits();
-void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl const *ctl, void const *partials,
+// advance: the level ends with this completion; the frontier moves on with it.
+void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl *ctl, void const *partials,
                    void const *hist_cur, void *hist_next, u32 node_cells, u32 max_ops,
-                   u32 const *cnt_cur = nullptr, u32 *cnt_next = nullptr);
+                   u32 const *cnt_cur = nullptr, u32 *cnt_next = nullptr, bool advance = false);
 void launch_find(hipStream_t s, bool q, NodeInfo *front, TreeCtl const *ctl, void const *hist, u16 const *n_bins,
                  u32 F, u32 SB, GrowCfg const &cfg, double const *scales, SplitRec *out, u32 max_front,
                  u32 front_base, u32 n_front_override, FindScratch const &fs, u32 const *cnt = nullptr);
