@@ -72,6 +72,33 @@ __device__ __forceinline__ u32 lds_load_range(u8 const *__restrict__ src, size_t
     return sh;
 }
 
+// lds_load_range in two halves, so a tile's loads can be in flight while the block
+// works on the previous tile: pf_load reads the aligned 16-byte words into registers
+// (at most K per thread, w[] with constant indices only), pf_store writes them to LDS.
+template <u32 K>
+__device__ __forceinline__ u32 pf_load(uint4 (&w)[K], u32 &nw, u8 const *__restrict__ src, size_t a, u32 len)
+{
+    size_t const a16 = a & ~static_cast<size_t>(15);
+    u32 const sh = static_cast<u32>(a - a16);
+    nw = (sh + len + 15u) >> 4;
+    uint4 const *s = reinterpret_cast<uint4 const *>(src + a16);
+#pragma unroll
+    for (u32 k = 0; k < K; ++k) {
+        u32 const i = threadIdx.x + k * blockDim.x;
+        w[k] = i < nw ? s[i] : make_uint4(0, 0, 0, 0);
+    }
+    return sh;
+}
+
+template <u32 K> __device__ __forceinline__ void pf_store(uint4 const (&w)[K], u32 nw, uint4 *dst)
+{
+#pragma unroll
+    for (u32 k = 0; k < K; ++k) {
+        u32 const i = threadIdx.x + k * blockDim.x;
+        if (i < nw) dst[i] = w[k];
+    }
+}
+
 // Block-wide exclusive scan of one u32 per thread (blockDim.x <= 1024).  Returns the
 // exclusive prefix; *total receives the block sum.  Uses `scratch` of blockDim.x u32.
 __device__ u32 block_exscan(u32 v, u32 *scratch, u32 *total)
@@ -1674,6 +1701,9 @@ __device__ __forceinline__ void store_runs(u32 *__restrict__ dst, Runs<NR> const
     }
 }
 
+// Register prefetch words per thread (256 threads) of the narrow scatters.
+constexpr u32 kPfRows = 8, kPfGh = 2;
+
 __global__ __launch_bounds__(256) void k_part_scatter_v(
     PartChunk const *__restrict__ pch, PartOp const *__restrict__ ops, TreeCtl const *__restrict__ ctl,
     u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
@@ -1708,12 +1738,28 @@ __global__ __launch_bounds__(256) void k_part_scatter_v(
                 continue; // uniform across the block
             }
         }
+        // Register prefetch (the launcher sizes tiles to fit it): the next tile's loads
+        // fly while this one is ranked and stored.
+        uint4 pr[kPfRows], pg[kPfGh];
+        u32 pr_n, pg_n, pr_sh, pg_sh;
+        {
+            size_t const srow0 = static_cast<size_t>(op.off) + ch.begin;
+            u32 const rows0 = min(tile, ch.end - ch.begin);
+            pr_sh = pf_load(pr, pr_n, plane, srow0 * Fs, rows0 * Fs);
+            pg_sh = pf_load(pg, pg_n, gin, srow0 * gh_bytes, rows0 * gh_bytes);
+        }
         for (u32 t0 = ch.begin; t0 < ch.end; t0 += tile) {
             u32 const rows = min(tile, ch.end - t0);
-            size_t const srow0 = static_cast<size_t>(op.off) + t0;
-            u32 const sh = lds_load_range(plane, srow0 * Fs, rows * Fs, s_rows);
-            u32 const shg = lds_load_range(gin, srow0 * gh_bytes, rows * gh_bytes, s_gh);
+            pf_store(pr, pr_n, s_rows);
+            pf_store(pg, pg_n, s_gh);
+            u32 const sh = pr_sh, shg = pg_sh;
             __syncthreads();
+            if (t0 + tile < ch.end) {
+                size_t const srow1 = static_cast<size_t>(op.off) + t0 + tile;
+                u32 const rows1 = min(tile, ch.end - t0 - tile);
+                pr_sh = pf_load(pr, pr_n, plane, srow1 * Fs, rows1 * Fs);
+                pg_sh = pf_load(pg, pg_n, gin, srow1 * gh_bytes, rows1 * gh_bytes);
+            }
             u8 const *srb = reinterpret_cast<u8 const *>(s_rows) + sh;
             u32 run = 0;
             for (u32 p0 = 0; p0 < rows; p0 += blockDim.x) {
@@ -1859,6 +1905,8 @@ __global__ __launch_bounds__(256) void k_part_scatter4(
     u8 *__restrict__ out0, u8 *__restrict__ out1, u32 *__restrict__ go0, u32 *__restrict__ go1, u32 Fs,
     u32 tile, u64 magic_d, uint4 const *__restrict__ chunk_cls)
 {
+    // Tiles fit the register prefetch: tile * Fs + 16 <= kPfRows * 16 * 256 bytes and
+    // tile * 4 + 16 <= kPfGh * 16 * 256 (checked by the launcher).
     extern __shared__ __align__(16) unsigned char smem[];
     uint4 *s_rows = reinterpret_cast<uint4 *>(smem);                  // tile * Fs + 16 bytes
     uint4 *s_gh = reinterpret_cast<uint4 *>(smem + tile * Fs + 16u);  // tile * 4 + 16
@@ -1887,12 +1935,27 @@ __global__ __launch_bounds__(256) void k_part_scatter4(
         }
         size_t const base[4] = {static_cast<size_t>(q.off), static_cast<size_t>(q.off) + q.n_al,
                                 static_cast<size_t>(q.off) + q.n_a, static_cast<size_t>(q.off) + q.n_a + q.n_bl};
+        uint4 pr[kPfRows], pg[kPfGh];
+        u32 pr_n, pg_n, pr_sh, pg_sh;
+        {
+            size_t const srow0 = static_cast<size_t>(q.off) + ch.begin;
+            u32 const rows0 = min(tile, ch.end - ch.begin);
+            pr_sh = pf_load(pr, pr_n, plane, srow0 * Fs, rows0 * Fs);
+            pg_sh = pf_load(pg, pg_n, reinterpret_cast<u8 const *>(gin), srow0 * 4u, rows0 * 4u);
+        }
         for (u32 t0 = ch.begin; t0 < ch.end; t0 += tile) {
             u32 const rows = min(tile, ch.end - t0);
-            size_t const srow0 = static_cast<size_t>(q.off) + t0;
-            u32 const sh = lds_load_range(plane, srow0 * Fs, rows * Fs, s_rows);
-            u32 const shg = lds_load_range(reinterpret_cast<u8 const *>(gin), srow0 * 4u, rows * 4u, s_gh);
+            pf_store(pr, pr_n, s_rows);
+            pf_store(pg, pg_n, s_gh);
+            u32 const sh = pr_sh, shg = pg_sh;
             __syncthreads();
+            // Next tile's loads fly while this one is ranked and stored.
+            if (t0 + tile < ch.end) {
+                size_t const srow1 = static_cast<size_t>(q.off) + t0 + tile;
+                u32 const rows1 = min(tile, ch.end - t0 - tile);
+                pr_sh = pf_load(pr, pr_n, plane, srow1 * Fs, rows1 * Fs);
+                pg_sh = pf_load(pg, pg_n, reinterpret_cast<u8 const *>(gin), srow1 * 4u, rows1 * 4u);
+            }
             u8 const *srb = reinterpret_cast<u8 const *>(s_rows) + sh;
             u32 tr[4] = {0, 0, 0, 0}; // rows of each class ranked so far in this tile
             for (u32 p0 = 0; p0 < rows; p0 += blockDim.x) {
@@ -2621,10 +2684,8 @@ void launch_part_scatter(hipStream_t s, PartChunk const *pch, PartOp const *ops,
 {
     u32 const D = Fs / 4;
     if (D <= 16) {
-        u32 const per_row = Fs + gh_bytes + 4u;
-        u32 tile = (t_tune.scatter_lds - 32u) / per_row;
-        tile = std::max(64u, std::min<u32>(kScatTileMax, tile & ~63u));
-        size_t const lds = static_cast<size_t>(tile) * per_row + 32u;
+        u32 tile = scatter_tile_rows(Fs, gh_bytes);
+        size_t const lds = static_cast<size_t>(tile) * (Fs + gh_bytes + 4u) + 32u;
         u32 const G = gh_bytes / 4u;
         u64 const magic_d = ((1ull << 32) + D - 1u) / D;
         u64 const magic_g = ((1ull << 32) + G - 1u) / G;
@@ -2643,6 +2704,9 @@ u32 scatter_tile_rows(u32 Fs, u32 gh_bytes)
 {
     u32 const per_row = Fs + gh_bytes + 4u;
     u32 tile = (t_tune.scatter_lds - 32u) / per_row;
+    // the register prefetch of k_part_scatter_v holds a whole tile
+    u32 const pf_rows = (kPfRows * 16u * 256u - 32u) / Fs, pf_gh = (kPfGh * 16u * 256u - 32u) / gh_bytes;
+    tile = std::min(tile, std::min(pf_rows, pf_gh));
     return std::max(64u, std::min<u32>(kScatTileMax, tile & ~63u));
 }
 
@@ -2665,6 +2729,9 @@ u32 scatter4_tile_rows(u32 Fs)
 {
     u32 const per_row = Fs + 4u + 8u;
     u32 tile = (t_tune.scatter_lds - 32u) / per_row;
+    // the register prefetch holds a whole tile (rows and gradient words)
+    u32 const pf_rows = (kPfRows * 16u * 256u - 32u) / Fs, pf_gh = (kPfGh * 16u * 256u - 32u) / 4u;
+    tile = std::min(tile, std::min(pf_rows, pf_gh));
     return std::max(64u, std::min<u32>(kScatTileMax, tile & ~63u));
 }
 
