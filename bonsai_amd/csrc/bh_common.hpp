@@ -170,9 +170,10 @@ static_assert(sizeof(TreeCtl) == 128, "TreeCtl");
 struct Tuning {
     u32 scatter_lds = 20u * 1024u; // LDS bytes per partition-scatter workgroup (sets its tile rows)
     u32 find_blocks = kFindMaxBlocks; // workgroups sharing one node's split find (1..kFindMaxBlocks)
-    u32 hist_threads = 1024;     // quantized histogram workgroup size (256, 512 or 1024)
+    u32 hist_threads = 512;      // quantized histogram workgroup size (256, 512 or 1024)
     u32 part_grid = 2048;        // partition count / scatter workgroups
-    u32 hist_layout = 0;         // quantized histogram, rows <= 32 bytes: 1 feature-in-lane, 0 row-dword lanes
+    u32 hist_layout = 2;         // quantized histogram: 0 row-dword lanes, 1 feature-in-lane (rows <= 32 B),
+                                 // 2 row-in-lane (rows <= 64 B)
     u32 pair_levels = 1;         // quantized depthwise, rows <= 64 bytes: move rows every second level
 };
 

@@ -854,7 +854,7 @@ static int validate_tuning(bh_tuning const *t)
     if (t->hist_threads != 256u && t->hist_threads != 512u && t->hist_threads != 1024u)
         return fail(1, "hist_threads must be 256, 512 or 1024");
     if (t->part_grid < 64u || t->part_grid > 65536u) return fail(1, "part_grid must be in [64, 65536]");
-    if (t->hist_layout > 1u) return fail(1, "hist_layout must be 0 or 1");
+    if (t->hist_layout > 2u) return fail(1, "hist_layout must be 0, 1 or 2");
     if (t->pair_levels > 1u) return fail(1, "pair_levels must be 0 or 1");
     return 0;
 }
@@ -930,10 +930,16 @@ int bh_autotune(bh_context *c, uint32_t iters, bh_tuning *chosen, float *ms_per_
     static const u32 thr_c[] = {256u, 512u, 1024u};
     static const u32 fb_c[] = {1u, 2u, 4u, 8u};
     static const u32 pg_c[] = {512u, 1024u, 2048u, 4096u};
-    static const u32 hl_c[] = {0u, 1u};
+    static const u32 hl_c[] = {0u, 1u, 2u};
+    static const u32 pl_c[] = {0u, 1u};
     for (int knob = 0; rc == 0 && knob < 6; ++knob) {
-        u32 const *cand = knob == 0 ? lds_c : knob == 1 ? thr_c : knob == 2 ? fb_c : knob == 3 ? pg_c : hl_c;
-        int const nc = knob == 0 ? 7 : knob == 1 ? 3 : knob >= 4 ? 2 : 4;
+        u32 const *cand = knob == 0   ? lds_c
+                          : knob == 1 ? thr_c
+                          : knob == 2 ? fb_c
+                          : knob == 3 ? pg_c
+                          : knob == 4 ? hl_c
+                                      : pl_c;
+        int const nc = knob == 0 ? 7 : knob == 1 || knob == 4 ? 3 : knob == 5 ? 2 : 4;
         for (int k = 0; rc == 0 && k < nc; ++k) {
             Tuning t = best;
             u32 &slot = knob == 0   ? t.scatter_lds

@@ -556,6 +556,93 @@ __global__ __launch_bounds__(kHqThreads) void k_hist_q(
     }
 }
 
+// Quantized histogram, row-in-lane layout (rows of at most 64 bytes, one feature slice;
+// same contract as k_hist_q).  Lane l of a wave owns whole rows: dword k of 64
+// consecutive rows is one load instruction spanning 64 rows (the lines of a 64-row
+// block are fetched once and re-read from cache for the other dwords), the gradient word
+// is loaded once per row and packed once, and U rows per lane are in flight.  Atomic
+// instruction (row, k, j) puts 64 rows' bins of one feature into the LDS cells.  A
+// routed unit's filtered rows skip their atomics.
+template <u32 DW, u32 U>
+__global__ __launch_bounds__(kHqThreads) void k_hist_q3(
+    HistChunk const *__restrict__ chunks, TreeCtl const *__restrict__ ctl,
+    u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
+    void const *__restrict__ gw0, void const *__restrict__ gw1, void const *__restrict__ gwp,
+    u32 Fs, u32 F, u32 SB, void *__restrict__ partials, void *__restrict__ hist_next,
+    u32 *__restrict__ cnt_next, PartOp const *__restrict__ routes)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned long long *lh = reinterpret_cast<unsigned long long *>(smem); // F x SB
+    u32 const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    u32 const NW = blockDim.x >> 6;
+    u32 const cells = F * SB;
+    u32 const n_units = ctl->n_hist_chunks;
+    size_t const node_cells = static_cast<size_t>(F) * SB;
+    for (u32 c = blockIdx.x; c < n_units; c += gridDim.x) {
+        HistChunk const ch = chunks[c];
+        for (u32 i = threadIdx.x; i < cells; i += blockDim.x) lh[i] = 0ull;
+        __syncthreads();
+        u8 const *plane = ch.src == 0 ? pl0 : (ch.src == 1 ? pl1 : plp);
+        u32 const *ghv = static_cast<u32 const *>(ch.src == 0 ? gw0 : (ch.src == 1 ? gw1 : gwp));
+        bool const routed = ch.route != kNone;
+        PartOp rop{};
+        if (routed) rop = routes[ch.route];
+        bool const want_left = ch.side == 0;
+        u32 const rf_dw = rop.feature >> 2, rf_sh = 8u * (rop.feature & 3u);
+        for (u32 r0 = ch.begin + wave * (U * 64u); r0 < ch.end; r0 += NW * U * 64u) {
+            u32 dw[U][DW], v[U];
+            bool ok[U];
+#pragma unroll
+            for (u32 u = 0; u < U; ++u) {
+                u32 const row = r0 + u * 64u + lane;
+                ok[u] = row < ch.end;
+                u32 const *rp = reinterpret_cast<u32 const *>(plane + static_cast<size_t>(ok[u] ? row : ch.begin) * Fs);
+#pragma unroll
+                for (u32 k = 0; k < DW; ++k) dw[u][k] = rp[k];
+                v[u] = ghv[ok[u] ? row : ch.begin];
+            }
+#pragma unroll
+            for (u32 u = 0; u < U; ++u) {
+                bool keep = ok[u];
+                if (routed) {
+                    u32 w = 0;
+#pragma unroll
+                    for (u32 k = 0; k < DW; ++k)
+                        if (k == rf_dw) w = dw[u][k];
+                    keep = keep && routes_left((w >> rf_sh) & 0xFFu, rop.last_bin, rop.bin, rop.default_left) ==
+                                       want_left;
+                }
+                if (!keep) continue;
+                u64 const pk = pack_q(v[u]);
+#pragma unroll
+                for (u32 k = 0; k < DW; ++k)
+#pragma unroll
+                    for (u32 j = 0; j < 4; ++j) {
+                        u32 const f = 4u * k + j;
+                        if (f < F)
+                            atomicAdd(lh + f * SB + ((dw[u][k] >> (8u * j)) & 0xFFu),
+                                      static_cast<unsigned long long>(pk));
+                    }
+            }
+        }
+        __syncthreads();
+        if (ch.part != kNone) {
+            u64 *dst = reinterpret_cast<u64 *>(partials) + static_cast<size_t>(ch.part) * node_cells;
+            for (u32 i = threadIdx.x; i < cells; i += blockDim.x) dst[i] = lh[i];
+        } else {
+            size_t const base = static_cast<size_t>(ch.slot) * node_cells;
+            i64 *dst = reinterpret_cast<i64 *>(hist_next) + base * 2;
+            for (u32 i = threadIdx.x; i < cells; i += blockDim.x) {
+                u64 const v = lh[i];
+                dst[2 * i] = unpack_g(v);
+                dst[2 * i + 1] = unpack_h(v);
+                if (cnt_next) cnt_next[base + i] = unpack_cnt(v);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Quantized histogram for narrow rows (F <= 32), feature-in-lane layout (H6, same
 // contract as k_hist_q).  A tile of the unit's rows is staged in LDS with 16-byte loads;
 // one ds_add_u64 then covers two rows x the row's features (lane = row parity x
@@ -2568,7 +2655,24 @@ hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl c
     u32 const slices = (F + fsl - 1) / fsl;
     size_t const lds = static_cast<size_t>(fsl) * SB * 8;
     dim3 const grid(grid_x, slices), block(64 * n_waves);
-    if (q && Fs <= kHq2Cols && t_tune.hist_layout == 1u) {
+    if (q && Fs <= 64u && t_tune.hist_layout == 2u) {
+        // row-in-lane: whole rows per lane, instantiated per row width in dwords
+        u32 const Dw = Fs / 4u;
+        dim3 const gq3(grid_x);
+        size_t const lds1 = static_cast<size_t>(F) * SB * 8u;
+#define BH_HQ3(DWV, UV)                                                                                          \
+    case DWV:                                                                                                     \
+        hipLaunchKernelGGL((k_hist_q3<DWV, UV>), gq3, dim3(t_tune.hist_threads), lds1, s, chunks, ctl, pl0, pl1, \
+                           plp, g0, g1, gp, Fs, F, SB, partials, hist_next, cnt_next, routes);                  \
+        break;
+        switch (Dw) {
+            BH_HQ3(1, 4) BH_HQ3(2, 4) BH_HQ3(3, 4) BH_HQ3(4, 4) BH_HQ3(5, 4) BH_HQ3(6, 4) BH_HQ3(7, 4)
+            BH_HQ3(8, 3) BH_HQ3(9, 3) BH_HQ3(10, 2) BH_HQ3(11, 2) BH_HQ3(12, 2) BH_HQ3(13, 2) BH_HQ3(14, 2)
+            BH_HQ3(15, 2) BH_HQ3(16, 2)
+        default: break;
+        }
+#undef BH_HQ3
+    } else if (q && Fs <= kHq2Cols && t_tune.hist_layout == 1u) {
         u32 const lh_bytes = SB * kHq2Cols * 8u;
         u32 tile = lh_bytes + 32u + 64u * (Fs + 4u) <= kHq2LdsBudget
                        ? ((kHq2LdsBudget - lh_bytes - 32u) / (Fs + 4u)) & ~63u
