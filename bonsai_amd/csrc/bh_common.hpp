@@ -172,8 +172,8 @@ struct Tuning {
     u32 find_blocks = kFindMaxBlocks; // workgroups sharing one node's split find (1..kFindMaxBlocks)
     u32 hist_threads = 512;      // quantized histogram workgroup size (256, 512 or 1024)
     u32 part_grid = 2048;        // partition count / scatter workgroups
-    u32 hist_layout = 2;         // quantized histogram: 0 row-dword lanes, 1 feature-in-lane (rows <= 32 B),
-                                 // 2 row-in-lane (rows <= 64 B)
+    u32 hist_layout = 0;         // quantized histogram: 0 row-dword lanes, 1 feature-in-lane (rows <= 32 B),
+                                 // 2 row-in-lane (rows <= 64 B; routed levels always use it)
     u32 pair_levels = 1;         // quantized depthwise, rows <= 64 bytes: move rows every second level
 };
 

@@ -285,12 +285,12 @@ hipError_t host_sync_point(bh_context *c)
 // (built node, feature, bin) written.  Rows are physically compacted per node (or, for
 // a routed unit, per parent: it scans the parent's whole segment), so no row-index
 // bytes are read.
-void profiled_hist(bh_context *c, void *hist_next, u32 grid_units, u32 *cnt_next)
+void profiled_hist(bh_context *c, void *hist_next, u32 grid_units, u32 *cnt_next, PartOp const *routes)
 {
     (void)hipEventRecord(c->ev0, c->stream);
     launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
                 gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), grid_units,
-                c->partials, hist_next, cnt_next, c->ops);
+                c->partials, hist_next, cnt_next, routes);
     (void)hipEventRecord(c->ev1, c->stream);
     (void)hipEventSynchronize(c->ev1);
     float ms = 0.0F;
@@ -318,15 +318,18 @@ void profiled_hist(bh_context *c, void *hist_next, u32 grid_units, u32 *cnt_next
 }
 
 // Histogram units -> partials / slots, then the fixed-order completion.
+// routes: the split rules of the level's routed units (levels whose rows were not
+// moved), else null.
 void enqueue_build(bh_context *c, void *hist_cur, void *hist_next, u32 max_ops, u32 grid_units,
-                   u32 const *cnt_cur = nullptr, u32 *cnt_next = nullptr, bool advance = false)
+                   u32 const *cnt_cur = nullptr, u32 *cnt_next = nullptr, bool advance = false,
+                   PartOp const *routes = nullptr)
 {
     if (c->prof)
-        profiled_hist(c, hist_next, grid_units, cnt_next);
+        profiled_hist(c, hist_next, grid_units, cnt_next, routes);
     else
         launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
                     gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), grid_units,
-                    c->partials, hist_next, cnt_next, c->ops);
+                    c->partials, hist_next, cnt_next, routes);
     launch_finish(c->stream, c->quant, c->fops, c->ctl, c->partials, hist_cur, hist_next, c->node_cells(), max_ops,
                   cnt_cur, cnt_next, advance);
 }
@@ -384,7 +387,8 @@ void enqueue_depthwise_counted(bh_context *c)
                                      c->chunk_cls);
                 ++moves;
             }
-            enqueue_build(c, hcur, hnext, std::min(width, c->max_ops), hist_grid(c), ccur, cnext, true);
+            enqueue_build(c, hcur, hnext, std::min(width, c->max_ops), hist_grid(c), ccur, cnext, true,
+                          mode == 1 ? c->ops : nullptr);
         } else {
             launch_next_level(s, c->ctl);
         }

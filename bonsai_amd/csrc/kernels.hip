@@ -2655,8 +2655,9 @@ hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl c
     u32 const slices = (F + fsl - 1) / fsl;
     size_t const lds = static_cast<size_t>(fsl) * SB * 8;
     dim3 const grid(grid_x, slices), block(64 * n_waves);
-    if (q && Fs <= 64u && t_tune.hist_layout == 2u) {
-        // row-in-lane: whole rows per lane, instantiated per row width in dwords
+    if (q && Fs <= 64u && (routes || t_tune.hist_layout == 2u)) {
+        // row-in-lane: whole rows per lane, instantiated per row width in dwords.  Routed
+        // levels always use it (a filtered row skips all its atomics at once).
         u32 const Dw = Fs / 4u;
         dim3 const gq3(grid_x);
         size_t const lds1 = static_cast<size_t>(F) * SB * 8u;

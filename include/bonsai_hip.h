@@ -182,8 +182,9 @@ typedef struct bh_tuning {
     uint32_t find_blocks;       /* workgroups sharing one node's split find (1..8) */
     uint32_t hist_threads;      /* quantized histogram workgroup size: 256, 512 or 1024 */
     uint32_t part_grid;         /* partition workgroups */
-    uint32_t hist_layout;       /* quantized histogram: 0 row-dword lanes, 1 feature-in-lane (rows <= 32 bytes),
-                                   2 row-in-lane (rows <= 64 bytes; default); other rows use 0 */
+    uint32_t hist_layout;       /* quantized histogram: 0 row-dword lanes (default), 1 feature-in-lane (rows <= 32
+                                   bytes), 2 row-in-lane (rows <= 64 bytes); levels whose units filter their
+                                   parent's rows use 2 whenever rows are <= 64 bytes */
     uint32_t pair_levels;       /* quantized depthwise, rows <= 64 bytes: 1 = move rows every second level
                                    (four-way partitions; the levels between scan parent segments) */
 } bh_tuning;
