@@ -54,7 +54,8 @@ class FitConfigC(C.Structure):
 
 class TuningC(C.Structure):
     _fields_ = [("scatter_lds_bytes", C.c_uint32), ("find_blocks", C.c_uint32), ("hist_threads", C.c_uint32),
-                ("part_grid", C.c_uint32), ("hist_layout", C.c_uint32), ("pair_levels", C.c_uint32)]
+                ("part_grid", C.c_uint32), ("hist_layout", C.c_uint32), ("pair_levels", C.c_uint32),
+                ("hist_group", C.c_uint32)]
 
 
 class ServeConfigC(C.Structure):

@@ -33,6 +33,10 @@ constexpr u32 kPackCntShift = 28;
 constexpr u32 kPackGShift = 42;
 constexpr u64 kPackHMask = (1ull << 28) - 1ull;
 constexpr u64 kPackCntMask = (1ull << 14) - 1ull;
+// Quantized depthwise fits may group up to this many kHistChunk sub-units in one
+// histogram work unit (the kernels add each sub-unit's packed cells into the unit's wide
+// cells); the group size is a tuning knob (1..kHistGroupMax).
+constexpr u32 kHistGroupMax = 4;
 constexpr u32 kPartChunk  = 4096;  // rows per partition work unit
 constexpr u32 kMaxTreeNodes = 8192;
 // Tail padding of planes (bytes) and gradient buffers (elements): kernels that read
@@ -175,6 +179,7 @@ struct Tuning {
     u32 hist_layout = 0;         // quantized histogram: 0 row-dword lanes, 1 feature-in-lane (rows <= 32 B),
                                  // 2 row-in-lane (rows <= 64 B; routed levels always use it)
     u32 pair_levels = 1;         // quantized depthwise, rows <= 64 bytes: move rows every second level
+    u32 hist_group = 2;          // quantized depthwise: sub-units per histogram unit of full-scan levels
 };
 
 // Per-node partial split records and completion counters of a multi-block split find

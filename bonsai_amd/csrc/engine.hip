@@ -118,7 +118,14 @@ struct bh_context {
         return std::max(1u, std::min(w, 16u));
     }
     size_t cell_bytes() const { return quant ? 16 : 8; }
-    size_t part_bytes() const { return 8; }
+    // unit partials: float2 cells (FP32), three u32 planes g / h / count (quantized)
+    size_t part_bytes() const { return quant ? 12 : 8; }
+    // Rows per histogram work unit of the counted (quantized depthwise) path on levels
+    // that scan every live row (the root and routed levels): groups of hist_group
+    // sub-units, unless the feature-in-lane layout (single sub-units) is set.  Levels that
+    // build the small children only keep single sub-units: their fewer units would leave
+    // workgroup slots idle.
+    u32 unit_rows() const { return quant && tune.hist_layout != 1u ? kHistChunk * tune.hist_group : kHistChunk; }
     u32 node_cells() const { return F * SB; }
 };
 
@@ -285,12 +292,12 @@ hipError_t host_sync_point(bh_context *c)
 // (built node, feature, bin) written.  Rows are physically compacted per node (or, for
 // a routed unit, per parent: it scans the parent's whole segment), so no row-index
 // bytes are read.
-void profiled_hist(bh_context *c, void *hist_next, u32 grid_units, u32 *cnt_next, PartOp const *routes)
+void profiled_hist(bh_context *c, void *hist_next, u32 grid_units, u32 *cnt_next, PartOp const *routes, bool group)
 {
     (void)hipEventRecord(c->ev0, c->stream);
     launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
                 gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), grid_units,
-                c->partials, hist_next, cnt_next, routes);
+                c->partials, hist_next, cnt_next, routes, group);
     (void)hipEventRecord(c->ev1, c->stream);
     (void)hipEventSynchronize(c->ev1);
     float ms = 0.0F;
@@ -322,14 +329,14 @@ void profiled_hist(bh_context *c, void *hist_next, u32 grid_units, u32 *cnt_next
 // moved), else null.
 void enqueue_build(bh_context *c, void *hist_cur, void *hist_next, u32 max_ops, u32 grid_units,
                    u32 const *cnt_cur = nullptr, u32 *cnt_next = nullptr, bool advance = false,
-                   PartOp const *routes = nullptr)
+                   PartOp const *routes = nullptr, bool group = false)
 {
     if (c->prof)
-        profiled_hist(c, hist_next, grid_units, cnt_next, routes);
+        profiled_hist(c, hist_next, grid_units, cnt_next, routes, group);
     else
         launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
                     gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), grid_units,
-                    c->partials, hist_next, cnt_next, routes);
+                    c->partials, hist_next, cnt_next, routes, group);
     launch_finish(c->stream, c->quant, c->fops, c->ctl, c->partials, hist_cur, hist_next, c->node_cells(), max_ops,
                   cnt_cur, cnt_next, advance);
 }
@@ -352,8 +359,10 @@ void enqueue_depthwise_counted(bh_context *c)
     // scan the parent segments, routed), odd levels move them two splits at once.
     bool const paired = c->tune.pair_levels != 0 && c->Fs <= 64;
     u32 moves = 0; // partitions so far: they alternate between the two work planes
-    launch_tree_begin(s, c->ctl, c->front[0], c->tree, c->hch, c->fops, static_cast<u32>(c->N));
-    enqueue_build(c, c->hist[1], c->hist[0], 1, hist_grid(c), c->cnt[1], c->cnt[0]);
+    u32 const unit_rows = c->unit_rows();
+    bool const group = unit_rows > kHistChunk;
+    launch_tree_begin(s, c->ctl, c->front[0], c->tree, c->hch, c->fops, static_cast<u32>(c->N), unit_rows);
+    enqueue_build(c, c->hist[1], c->hist[0], 1, hist_grid(c), c->cnt[1], c->cnt[0], false, nullptr, group);
     for (u32 d = 0; d < D; ++d) {
         NodeInfo *cur = c->front[d & 1u], *nxt = c->front[(d + 1) & 1u];
         void *hcur = c->hist[d & 1u], *hnext = c->hist[(d + 1) & 1u];
@@ -365,7 +374,8 @@ void enqueue_depthwise_counted(bh_context *c)
         launch_find(s, true, cur, c->ctl, hcur, c->d_nbins, c->F, c->SB, c->gc, c->scales, c->splits, width, 0, 0,
                     c->fscr, ccur);
         launch_plan_counted(s, cur, c->splits, c->ctl, c->tree, c->ops, c->pch, nxt, c->hch, c->fops, c->d_nbins,
-                            c->gc, c->scales, dst, last ? 1u : 0u, kPartChunk, mode, c->ops4);
+                            c->gc, c->scales, dst, last ? 1u : 0u, kPartChunk, mode, c->ops4,
+                            mode == 1 ? unit_rows : kHistChunk);
         if (c->fc.host_sync) host_sync_point(c);
         if (!last) {
             if (mode == 0) {
@@ -388,7 +398,7 @@ void enqueue_depthwise_counted(bh_context *c)
                 ++moves;
             }
             enqueue_build(c, hcur, hnext, std::min(width, c->max_ops), hist_grid(c), ccur, cnext, true,
-                          mode == 1 ? c->ops : nullptr);
+                          mode == 1 ? c->ops : nullptr, group && mode == 1);
         } else {
             launch_next_level(s, c->ctl);
         }
@@ -860,6 +870,7 @@ static int validate_tuning(bh_tuning const *t)
     if (t->part_grid < 64u || t->part_grid > 65536u) return fail(1, "part_grid must be in [64, 65536]");
     if (t->hist_layout > 2u) return fail(1, "hist_layout must be 0, 1 or 2");
     if (t->pair_levels > 1u) return fail(1, "pair_levels must be 0 or 1");
+    if (t->hist_group < 1u || t->hist_group > kHistGroupMax) return fail(1, "hist_group must be in [1, 4]");
     return 0;
 }
 
@@ -870,7 +881,7 @@ int bh_set_tuning(bh_context *c, const bh_tuning *t)
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->tune = Tuning{t->scatter_lds_bytes, t->find_blocks, t->hist_threads, t->part_grid, t->hist_layout,
-                     t->pair_levels};
+                     t->pair_levels, t->hist_group};
     drop_graph(c); // the iteration graph is re-captured with the new shapes
     return 0;
 }
@@ -879,7 +890,7 @@ int bh_get_tuning(bh_context *c, bh_tuning *t)
 {
     if (!c) return fail(1, "null context");
     *t = bh_tuning{c->tune.scatter_lds, c->tune.find_blocks, c->tune.hist_threads, c->tune.part_grid,
-                   c->tune.hist_layout, c->tune.pair_levels};
+                   c->tune.hist_layout, c->tune.pair_levels, c->tune.hist_group};
     return 0;
 }
 
@@ -936,14 +947,16 @@ int bh_autotune(bh_context *c, uint32_t iters, bh_tuning *chosen, float *ms_per_
     static const u32 pg_c[] = {512u, 1024u, 2048u, 4096u};
     static const u32 hl_c[] = {0u, 1u, 2u};
     static const u32 pl_c[] = {0u, 1u};
-    for (int knob = 0; rc == 0 && knob < 6; ++knob) {
+    static const u32 hg_c[] = {1u, 2u, 4u};
+    for (int knob = 0; rc == 0 && knob < 7; ++knob) {
         u32 const *cand = knob == 0   ? lds_c
                           : knob == 1 ? thr_c
                           : knob == 2 ? fb_c
                           : knob == 3 ? pg_c
                           : knob == 4 ? hl_c
-                                      : pl_c;
-        int const nc = knob == 0 ? 7 : knob == 1 || knob == 4 ? 3 : knob == 5 ? 2 : 4;
+                          : knob == 5 ? pl_c
+                                      : hg_c;
+        int const nc = knob == 0 ? 7 : knob == 1 || knob == 4 || knob == 6 ? 3 : knob == 5 ? 2 : 4;
         for (int k = 0; rc == 0 && k < nc; ++k) {
             Tuning t = best;
             u32 &slot = knob == 0   ? t.scatter_lds
@@ -951,7 +964,8 @@ int bh_autotune(bh_context *c, uint32_t iters, bh_tuning *chosen, float *ms_per_
                         : knob == 2 ? t.find_blocks
                         : knob == 3 ? t.part_grid
                         : knob == 4 ? t.hist_layout
-                                    : t.pair_levels;
+                        : knob == 5 ? t.pair_levels
+                                    : t.hist_group;
             if (slot == cand[k]) continue;
             slot = cand[k];
             float ms = 0.0F;
@@ -972,7 +986,7 @@ int bh_autotune(bh_context *c, uint32_t iters, bh_tuning *chosen, float *ms_per_
     set_tuning(c->tune);
     if (chosen)
         *chosen = bh_tuning{best.scatter_lds, best.find_blocks, best.hist_threads, best.part_grid, best.hist_layout,
-                            best.pair_levels};
+                            best.pair_levels, best.hist_group};
     if (ms_per_iter) *ms_per_iter = best_ms;
     return 0;
 }

@@ -489,6 +489,58 @@ __device__ __forceinline__ void hist_q_rows(unsigned long long *__restrict__ lh,
     }
 }
 
+// Quantized unit partials are wide: per partial, three u32 planes of node_cells each
+// (g as int32, h, row count).  A grouped work unit (up to kHistGroup 8192-row sub-units
+// of one node) adds every sub-unit's packed LDS cells into its own partial (or its
+// node's slot) in global memory and clears the LDS for the next sub-unit: only this
+// workgroup touches those cells during the launch, and the group writes one partial
+// instead of one per sub-unit (sums of 4 sub-units: |g| <= 127 * 32768,
+// h <= 32767 * 32768 < 2^32, count <= 2^15).
+//
+// Flush of the unit's cells [cell0, cell0 + cells) after one sub-unit: `first` writes,
+// later sub-units add.  `clear` zeroes the LDS cells for the next sub-unit.
+__device__ __forceinline__ void hq_flush(unsigned long long *__restrict__ lh, u32 cells, HistChunk const &ch,
+                                         size_t node_cells, size_t cell0, bool first, bool clear,
+                                         void *__restrict__ partials, void *__restrict__ hist_next,
+                                         u32 *__restrict__ cnt_next)
+{
+    if (ch.part != kNone) {
+        u32 *P = static_cast<u32 *>(partials) + static_cast<size_t>(ch.part) * node_cells * 3u + cell0;
+        for (u32 i = threadIdx.x; i < cells; i += blockDim.x) {
+            u64 const v = lh[i];
+            if (clear) lh[i] = 0ull;
+            u32 g = static_cast<u32>(static_cast<int32_t>(unpack_g(v)));
+            u32 h = static_cast<u32>(unpack_h(v));
+            u32 n = unpack_cnt(v);
+            if (!first) {
+                g += P[i];
+                h += P[node_cells + i];
+                n += P[2 * node_cells + i];
+            }
+            P[i] = g;
+            P[node_cells + i] = h;
+            P[2 * node_cells + i] = n;
+        }
+    } else {
+        size_t const base = static_cast<size_t>(ch.slot) * node_cells + cell0;
+        i64 *dst = static_cast<i64 *>(hist_next) + base * 2;
+        for (u32 i = threadIdx.x; i < cells; i += blockDim.x) {
+            u64 const v = lh[i];
+            if (clear) lh[i] = 0ull;
+            i64 g = unpack_g(v), h = unpack_h(v);
+            u32 n = unpack_cnt(v);
+            if (!first) {
+                g += dst[2 * i];
+                h += dst[2 * i + 1];
+                if (cnt_next) n += cnt_next[base + i];
+            }
+            dst[2 * i] = g;
+            dst[2 * i + 1] = h;
+            if (cnt_next) cnt_next[base + i] = n;
+        }
+    }
+}
+
 __global__ __launch_bounds__(kHqThreads) void k_hist_q(
     HistChunk const *__restrict__ chunks, TreeCtl const *__restrict__ ctl,
     u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
@@ -519,40 +571,32 @@ __global__ __launch_bounds__(kHqThreads) void k_hist_q(
         u32 const *ghv = static_cast<u32 const *>(ch.src == 0 ? gw0 : (ch.src == 1 ? gw1 : gwp));
         u8 const *cb = plane + f_begin + 4u * d;
         u32 const stride = NW * R;
-        u32 const r = ch.begin + wave * R + lr;
-        if (lane_ok) {
-            HqRoute rt{};
-            if (ch.route != kNone) {
-                PartOp const &op = routes[ch.route];
-                rt.feature = op.feature;
-                rt.bin = op.bin;
-                rt.dl = op.default_left;
-                rt.last = op.last_bin;
-                rt.want_left = ch.side == 0 ? 1u : 0u;
-                rt.in_slice = op.feature >= f_begin && op.feature < f_begin + nf ? 1u : 0u;
-                rt.lane_d = (op.feature - f_begin) >> 2;
-                rt.shift = 8u * ((op.feature - f_begin) & 3u);
-                hist_q_rows<true>(lh, plane, ghv, cb, Fs, SB, nf, d, lr, Dw, r, ch.end, stride, rt);
-            } else {
-                hist_q_rows<false>(lh, plane, ghv, cb, Fs, SB, nf, d, lr, Dw, r, ch.end, stride, rt);
-            }
+        HqRoute rt{};
+        if (ch.route != kNone) {
+            PartOp const &op = routes[ch.route];
+            rt.feature = op.feature;
+            rt.bin = op.bin;
+            rt.dl = op.default_left;
+            rt.last = op.last_bin;
+            rt.want_left = ch.side == 0 ? 1u : 0u;
+            rt.in_slice = op.feature >= f_begin && op.feature < f_begin + nf ? 1u : 0u;
+            rt.lane_d = (op.feature - f_begin) >> 2;
+            rt.shift = 8u * ((op.feature - f_begin) & 3u);
         }
-        __syncthreads();
-        if (ch.part != kNone) {
-            u64 *dst = reinterpret_cast<u64 *>(partials) + static_cast<size_t>(ch.part) * node_cells +
-                       static_cast<size_t>(f_begin) * SB;
-            for (u32 i = threadIdx.x; i < cells; i += blockDim.x) dst[i] = lh[i];
-        } else {
-            size_t const base = static_cast<size_t>(ch.slot) * node_cells + static_cast<size_t>(f_begin) * SB;
-            i64 *dst = reinterpret_cast<i64 *>(hist_next) + base * 2;
-            for (u32 i = threadIdx.x; i < cells; i += blockDim.x) {
-                u64 const v = lh[i];
-                dst[2 * i] = unpack_g(v);
-                dst[2 * i + 1] = unpack_h(v);
-                if (cnt_next) cnt_next[base + i] = unpack_cnt(v);
+        for (u32 s0 = ch.begin; s0 < ch.end; s0 += kHistChunk) {
+            u32 const s1 = min(ch.end, s0 + kHistChunk);
+            u32 const r = s0 + wave * R + lr;
+            if (lane_ok) {
+                if (ch.route != kNone)
+                    hist_q_rows<true>(lh, plane, ghv, cb, Fs, SB, nf, d, lr, Dw, r, s1, stride, rt);
+                else
+                    hist_q_rows<false>(lh, plane, ghv, cb, Fs, SB, nf, d, lr, Dw, r, s1, stride, rt);
             }
+            __syncthreads();
+            hq_flush(lh, cells, ch, node_cells, static_cast<size_t>(f_begin) * SB, s0 == ch.begin, s1 < ch.end,
+                     partials, hist_next, cnt_next);
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -589,13 +633,15 @@ __global__ __launch_bounds__(kHqThreads) void k_hist_q3(
         if (routed) rop = routes[ch.route];
         bool const want_left = ch.side == 0;
         u32 const rf_dw = rop.feature >> 2, rf_sh = 8u * (rop.feature & 3u);
-        for (u32 r0 = ch.begin + wave * (U * 64u); r0 < ch.end; r0 += NW * U * 64u) {
+        for (u32 s0 = ch.begin; s0 < ch.end; s0 += kHistChunk) {
+        u32 const s1 = min(ch.end, s0 + kHistChunk);
+        for (u32 r0 = s0 + wave * (U * 64u); r0 < s1; r0 += NW * U * 64u) {
             u32 dw[U][DW], v[U];
             bool ok[U];
 #pragma unroll
             for (u32 u = 0; u < U; ++u) {
                 u32 const row = r0 + u * 64u + lane;
-                ok[u] = row < ch.end;
+                ok[u] = row < s1;
                 u32 const *rp = reinterpret_cast<u32 const *>(plane + static_cast<size_t>(ok[u] ? row : ch.begin) * Fs);
 #pragma unroll
                 for (u32 k = 0; k < DW; ++k) dw[u][k] = rp[k];
@@ -626,20 +672,9 @@ __global__ __launch_bounds__(kHqThreads) void k_hist_q3(
             }
         }
         __syncthreads();
-        if (ch.part != kNone) {
-            u64 *dst = reinterpret_cast<u64 *>(partials) + static_cast<size_t>(ch.part) * node_cells;
-            for (u32 i = threadIdx.x; i < cells; i += blockDim.x) dst[i] = lh[i];
-        } else {
-            size_t const base = static_cast<size_t>(ch.slot) * node_cells;
-            i64 *dst = reinterpret_cast<i64 *>(hist_next) + base * 2;
-            for (u32 i = threadIdx.x; i < cells; i += blockDim.x) {
-                u64 const v = lh[i];
-                dst[2 * i] = unpack_g(v);
-                dst[2 * i + 1] = unpack_h(v);
-                if (cnt_next) cnt_next[base + i] = unpack_cnt(v);
-            }
-        }
+        hq_flush(lh, cells, ch, node_cells, 0, s0 == ch.begin, s1 < ch.end, partials, hist_next, cnt_next);
         __syncthreads();
+        } // sub-units
     }
 }
 
@@ -723,11 +758,15 @@ __global__ __launch_bounds__(512) void k_hist_q2(
         __syncthreads();
         // Flush in (bin, feature) order: LDS reads stay on distinct bank pairs.
         u32 const out_cells = F * SB;
-        if (ch.part != kNone) {
-            u64 *dst = reinterpret_cast<u64 *>(partials) + static_cast<size_t>(ch.part) * node_cells;
+        if (ch.part != kNone) { // wide partial (units of one 8192-row sub-unit)
+            u32 *P = static_cast<u32 *>(partials) + static_cast<size_t>(ch.part) * node_cells * 3u;
             for (u32 i = threadIdx.x; i < out_cells; i += blockDim.x) {
                 u32 const b = i / F, fl = i - b * F;
-                dst[static_cast<size_t>(fl) * SB + b] = lh[b * kHq2Cols + fl];
+                u64 const v = lh[b * kHq2Cols + fl];
+                size_t const o = static_cast<size_t>(fl) * SB + b;
+                P[o] = static_cast<u32>(static_cast<int32_t>(unpack_g(v)));
+                P[node_cells + o] = static_cast<u32>(unpack_h(v));
+                P[2 * node_cells + o] = unpack_cnt(v);
             }
         } else {
             size_t const base = static_cast<size_t>(ch.slot) * node_cells;
@@ -876,25 +915,31 @@ __global__ __launch_bounds__(256) void k_finish_q(FinishOp const *__restrict__ o
             i64 g = 0, h = 0;
             u32 n = 0;
             if (in && op.part_count > 0) {
-                u64 const *pp = reinterpret_cast<u64 const *>(partials) + i;
+                // wide partials: three u32 planes (g as int32, h, count) per partial
+                u32 const *pp = static_cast<u32 const *>(partials) + i;
+                size_t const pst = static_cast<size_t>(node_cells) * 3u;
                 u32 p = grp;
                 for (; p + 3 * kFqGroups < op.part_count; p += 4 * kFqGroups) {
-                    u64 v[4];
-#pragma unroll
-                    for (int k = 0; k < 4; ++k)
-                        v[k] = pp[static_cast<size_t>(op.part_first + p + k * kFqGroups) * node_cells];
+                    u32 vg[4], vh[4], vn[4];
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
-                        g += unpack_g(v[k]);
-                        h += unpack_h(v[k]);
-                        n += unpack_cnt(v[k]);
+                        u32 const *q = pp + static_cast<size_t>(op.part_first + p + k * kFqGroups) * pst;
+                        vg[k] = q[0];
+                        vh[k] = q[node_cells];
+                        vn[k] = q[2 * node_cells];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        g += static_cast<int32_t>(vg[k]);
+                        h += vh[k];
+                        n += vn[k];
                     }
                 }
                 for (; p < op.part_count; p += kFqGroups) {
-                    u64 const v = pp[static_cast<size_t>(op.part_first + p) * node_cells];
-                    g += unpack_g(v);
-                    h += unpack_h(v);
-                    n += unpack_cnt(v);
+                    u32 const *q = pp + static_cast<size_t>(op.part_first + p) * pst;
+                    g += static_cast<int32_t>(q[0]);
+                    h += q[node_cells];
+                    n += q[2 * node_cells];
                 }
             }
             s_g[grp][j] = g;
@@ -2125,7 +2170,8 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
     NodeInfo const *__restrict__ front, SplitRec const *__restrict__ splits, TreeCtl *__restrict__ ctl,
     DevNode *__restrict__ tree, PartOp *__restrict__ ops, PartChunk *__restrict__ pch, NodeInfo *__restrict__ next,
     HistChunk *__restrict__ hch, FinishOp *__restrict__ fops, u16 const *__restrict__ n_bins, GrowCfg cfg,
-    double const *__restrict__ scales, u32 dst, u32 is_last, u32 chunk_rows, u32 mode, PartOp4 *__restrict__ ops4)
+    double const *__restrict__ scales, u32 dst, u32 is_last, u32 chunk_rows, u32 mode, PartOp4 *__restrict__ ops4,
+    u32 unit_rows)
 {
     __shared__ u32 scratch[1024];
     __shared__ u32 s_base_nodes, s_valid, s_alive, s_chunks, s_units, s_parts;
@@ -2157,7 +2203,7 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
         if (alive && !is_last) {
             nch = mode == 0 ? (nd.cnt + chunk_rows - 1) / chunk_rows : 0u;
             u32 const scan = mode == 1 ? (small_cnt ? nd.cnt : 0u) : small_cnt;
-            n_units = (scan + kHistChunk - 1) / kHistChunk;
+            n_units = (scan + unit_rows - 1) / unit_rows;
             multi = n_units > 1 ? n_units : 0;
         }
         u32 vtot, atot, ctot, htot, ptot;
@@ -2348,8 +2394,8 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
         HistChunk hc{};
         hc.slot = fo.small_slot;
         hc.src = fo.src;
-        hc.begin = fo.small_off + k * kHistChunk;
-        hc.end = fo.small_off + min(fo.small_cnt, (k + 1) * kHistChunk);
+        hc.begin = fo.small_off + k * unit_rows;
+        hc.end = fo.small_off + min(fo.small_cnt, (k + 1) * unit_rows);
         hc.part = fo.part_count ? fo.part_first + k : kNone;
         hc.route = fo.route;
         hc.side = fo.side;
@@ -2390,16 +2436,16 @@ __global__ void k_next_level(TreeCtl *ctl)
 // Tree start: root frontier, root histogram units over the pristine plane.
 __global__ __launch_bounds__(256) void k_tree_begin(TreeCtl *__restrict__ ctl, NodeInfo *__restrict__ front,
                                                     DevNode *__restrict__ tree, HistChunk *__restrict__ hch,
-                                                    FinishOp *__restrict__ fops, u32 n_rows)
+                                                    FinishOp *__restrict__ fops, u32 n_rows, u32 unit_rows)
 {
-    u32 const n_units = (n_rows + kHistChunk - 1) / kHistChunk;
+    u32 const n_units = (n_rows + unit_rows - 1) / unit_rows;
     u32 const multi = n_units > 1;
     for (u32 k = threadIdx.x; k < n_units; k += blockDim.x) {
         HistChunk hc{};
         hc.slot = 0;
         hc.src = kSrcPlane0;
-        hc.begin = k * kHistChunk;
-        hc.end = min(n_rows, (k + 1) * kHistChunk);
+        hc.begin = k * unit_rows;
+        hc.end = min(n_rows, (k + 1) * unit_rows);
         hc.part = multi ? k : kNone;
         hch[k] = hc;
     }
@@ -2650,8 +2696,11 @@ void launch_quant_from(hipStream_t s, float2 const *gh, u64 n, u32 *maxbits, u64
 hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl const *ctl, u8 const *pl0,
                        u8 const *pl1, u8 const *plp, void const *g0, void const *g1, void const *gp, u32 Fs, u32 F,
                        u32 SB, u32 fsl, u32 n_waves, u32 grid_x, void *partials, void *hist_next, u32 *cnt_next,
-                       PartOp const *routes)
+                       PartOp const *routes, bool group)
 {
+    // group: units may span several 8192-row sub-units (the row-dword and row-in-lane
+    // quantized kernels add each sub-unit into the unit's cells; the feature-in-lane one
+    // takes single sub-units only).
     u32 const slices = (F + fsl - 1) / fsl;
     size_t const lds = static_cast<size_t>(fsl) * SB * 8;
     dim3 const grid(grid_x, slices), block(64 * n_waves);
@@ -2673,7 +2722,7 @@ hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl c
         default: break;
         }
 #undef BH_HQ3
-    } else if (q && Fs <= kHq2Cols && t_tune.hist_layout == 1u) {
+    } else if (q && !group && Fs <= kHq2Cols && t_tune.hist_layout == 1u) {
         u32 const lh_bytes = SB * kHq2Cols * 8u;
         u32 tile = lh_bytes + 32u + 64u * (Fs + 4u) <= kHq2LdsBudget
                        ? ((kHq2LdsBudget - lh_bytes - 32u) / (Fs + 4u)) & ~63u
@@ -2711,7 +2760,6 @@ hipError_t set_hist_lds_limits()
     int max_shared = 0;
     e = hipDeviceGetAttribute(&max_shared, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (e != hipSuccess) return e;
-    raise_lds(reinterpret_cast<void const *>(&k_hist<true, 8>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_hist_q), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_hist_q2), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_hist<false, 4>), max_shared);
@@ -2818,10 +2866,10 @@ u32 scatter_tile_rows(u32 Fs, u32 gh_bytes)
 void launch_plan_counted(hipStream_t s, NodeInfo const *front, SplitRec const *splits, TreeCtl *ctl, DevNode *tree,
                          PartOp *ops, PartChunk *pch, NodeInfo *next, HistChunk *hch, FinishOp *fops,
                          u16 const *n_bins, GrowCfg const &cfg, double const *scales, u32 dst, u32 is_last,
-                         u32 chunk_rows, u32 mode, PartOp4 *ops4)
+                         u32 chunk_rows, u32 mode, PartOp4 *ops4, u32 unit_rows)
 {
     hipLaunchKernelGGL(k_plan_counted, dim3(1), dim3(1024), 0, s, front, splits, ctl, tree, ops, pch, next, hch, fops,
-                       n_bins, cfg, scales, dst, is_last, chunk_rows, mode, ops4);
+                       n_bins, cfg, scales, dst, is_last, chunk_rows, mode, ops4, unit_rows);
 }
 
 void launch_part_count4(hipStream_t s, PartChunk const *pch, PartOp4 const *ops4, TreeCtl const *ctl, u8 const *pl0,
@@ -2866,9 +2914,9 @@ void launch_next_level(hipStream_t s, TreeCtl *ctl)
 }
 
 void launch_tree_begin(hipStream_t s, TreeCtl *ctl, NodeInfo *front, DevNode *tree, HistChunk *hch,
-                       FinishOp *fops, u32 n_rows)
+                       FinishOp *fops, u32 n_rows, u32 unit_rows)
 {
-    hipLaunchKernelGGL(k_tree_begin, dim3(1), dim3(256), 0, s, ctl, front, tree, hch, fops, n_rows);
+    hipLaunchKernelGGL(k_tree_begin, dim3(1), dim3(256), 0, s, ctl, front, tree, hch, fops, n_rows, unit_rows);
 }
 
 void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode const *tree, TreeCtl const *ctl,

@@ -13,7 +13,7 @@ void launch_quant_from(hipStream_t s, float2 const *gh, u64 n, u32 *maxbits, u64
 hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl const *ctl, u8 const *pl0,
                        u8 const *pl1, u8 const *plp, void const *g0, void const *g1, void const *gp, u32 Fs, u32 F,
                        u32 SB, u32 fsl, u32 n_waves, u32 grid_x, void *partials, void *hist_next,
-                       u32 *cnt_next = nullptr, PartOp const *routes = nullptr);
+                       u32 *cnt_next = nullptr, PartOp const *routes = nullptr, bool group = false);
 hipError_t set_hist_lds_limits();
 // advance: the level ends with this completion; the frontier moves on with it.
 void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl *ctl, void const *partials,
@@ -39,7 +39,7 @@ u32 scatter_tile_rows(u32 Fs, u32 gh_bytes);
 void launch_plan_counted(hipStream_t s, NodeInfo const *front, SplitRec const *splits, TreeCtl *ctl, DevNode *tree,
                          PartOp *ops, PartChunk *pch, NodeInfo *next, HistChunk *hch, FinishOp *fops,
                          u16 const *n_bins, GrowCfg const &cfg, double const *scales, u32 dst, u32 is_last,
-                         u32 chunk_rows, u32 mode = 0, PartOp4 *ops4 = nullptr);
+                         u32 chunk_rows, u32 mode = 0, PartOp4 *ops4 = nullptr, u32 unit_rows = kHistChunk);
 // Four-way partition of grandparent segments (quantized depthwise, rows of at most 64
 // bytes): per-chunk class counts, then the LDS-staged scatter.
 void launch_part_count4(hipStream_t s, PartChunk const *pch, PartOp4 const *ops4, TreeCtl const *ctl, u8 const *pl0,
@@ -50,8 +50,9 @@ void launch_part_scatter4(hipStream_t s, PartChunk const *pch, PartOp4 const *op
 void launch_finalize(hipStream_t s, NodeInfo const *front, TreeCtl const *ctl, DevNode *tree, GrowCfg const &cfg,
                      double const *scales, u32 max_front);
 void launch_next_level(hipStream_t s, TreeCtl *ctl);
+// unit_rows: rows per root histogram unit (kHistChunk, or a group of sub-units).
 void launch_tree_begin(hipStream_t s, TreeCtl *ctl, NodeInfo *front, DevNode *tree, HistChunk *hch,
-                       FinishOp *fops, u32 n_rows);
+                       FinishOp *fops, u32 n_rows, u32 unit_rows = kHistChunk);
 void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode const *tree, TreeCtl const *ctl,
                          u16 const *n_bins, u32 F, float lr, float *score, u32 *leaf_of_row, u32 max_nodes);
 void launch_iter_advance(hipStream_t s, TreeCtl *ctl);

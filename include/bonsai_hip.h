@@ -187,6 +187,8 @@ typedef struct bh_tuning {
                                    parent's rows use 2 whenever rows are <= 64 bytes */
     uint32_t pair_levels;       /* quantized depthwise, rows <= 64 bytes: 1 = move rows every second level
                                    (four-way partitions; the levels between scan parent segments) */
+    uint32_t hist_group;        /* quantized depthwise: 8192-row sub-units per histogram work unit on levels
+                                   that scan every row (1..4; fewer, larger unit partials) */
 } bh_tuning;
 /* Set / read the context's launch shapes (the iteration graph is re-captured). */
 int bh_set_tuning(bh_context *ctx, const bh_tuning *t);

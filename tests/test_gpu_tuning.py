@@ -40,8 +40,12 @@ SHAPES = [
     dict(scatter_lds_bytes=147456, find_blocks=8, hist_threads=1024, part_grid=2048, hist_layout=0,
          pair_levels=0),
     dict(scatter_lds_bytes=20480, find_blocks=8, hist_threads=1024, part_grid=2048, hist_layout=1, pair_levels=0),
-    dict(scatter_lds_bytes=32768, find_blocks=4, hist_threads=512, part_grid=1024, hist_layout=2, pair_levels=1),
-    dict(scatter_lds_bytes=32768, find_blocks=4, hist_threads=1024, part_grid=1024, hist_layout=2, pair_levels=0),
+    dict(scatter_lds_bytes=32768, find_blocks=4, hist_threads=512, part_grid=1024, hist_layout=2, pair_levels=1,
+         hist_group=4),
+    dict(scatter_lds_bytes=32768, find_blocks=4, hist_threads=1024, part_grid=1024, hist_layout=2, pair_levels=0,
+         hist_group=3),
+    dict(scatter_lds_bytes=20480, find_blocks=2, hist_threads=256, part_grid=2048, hist_layout=0, pair_levels=1,
+         hist_group=1),
 ]
 
 
@@ -78,4 +82,6 @@ def test_tuning_validation(hip, tall):
         ctx.set_tuning(pair_levels=2)
     with pytest.raises(hip.ConfigError):
         ctx.set_tuning(hist_layout=3)
+    with pytest.raises(hip.ConfigError):
+        ctx.set_tuning(hist_group=5)
     ctx.close()
