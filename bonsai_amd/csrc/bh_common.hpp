@@ -95,7 +95,7 @@ struct HistChunk {
     u32 part;  // partial index, or kNone when the chunk is the node's only one
     u32 route = kNone; // PartOp index of the parent's split, kNone: every row counts
     u32 side = 0;
-    u32 pad = 0;
+    u32 wide = 0;      // quantized partial format: 1 wide (g, h, count planes), 0 packed 64-bit cells
 };
 
 // Completion of one built node: small = sum of partials (fixed order), large = parent -
@@ -107,7 +107,8 @@ struct FinishOp {
     u32 unit_first, unit_count; // histogram units of the small child
     u32 small_off, small_cnt, src; // rows the units scan (the parent's segment when routed)
     u32 route = kNone, side = 0; // HistChunk::route / side of the units
-    u32 pad[3];
+    u32 wide = 0;                // HistChunk::wide of the units
+    u32 pad[2];
 };
 static_assert(sizeof(FinishOp) == 64, "FinishOp");
 
@@ -179,7 +180,7 @@ struct Tuning {
     u32 hist_layout = 0;         // quantized histogram: 0 row-dword lanes, 1 feature-in-lane (rows <= 32 B),
                                  // 2 row-in-lane (rows <= 64 B; routed levels always use it)
     u32 pair_levels = 1;         // quantized depthwise, rows <= 64 bytes: move rows every second level
-    u32 hist_group = 2;          // quantized depthwise: sub-units per histogram unit of full-scan levels
+    u32 hist_group = 1;          // quantized depthwise: sub-units per histogram unit of full-scan levels
 };
 
 // Per-node partial split records and completion counters of a multi-block split find

@@ -10,6 +10,7 @@
 #include <math.h>
 #include <algorithm>
 #include "bh_common.hpp"
+#include "bh_device.hpp"
 #include "kernels.hpp"
 
 namespace bh {
@@ -97,25 +98,6 @@ template <u32 K> __device__ __forceinline__ void pf_store(uint4 const (&w)[K], u
         u32 const i = threadIdx.x + k * blockDim.x;
         if (i < nw) dst[i] = w[k];
     }
-}
-
-// Block-wide exclusive scan of one u32 per thread (blockDim.x <= 1024).  Returns the
-// exclusive prefix; *total receives the block sum.  Uses `scratch` of blockDim.x u32.
-__device__ u32 block_exscan(u32 v, u32 *scratch, u32 *total)
-{
-    u32 const t = threadIdx.x;
-    scratch[t] = v;
-    __syncthreads();
-    for (u32 s = 1; s < blockDim.x; s <<= 1) {
-        u32 const add = t >= s ? scratch[t - s] : 0u;
-        __syncthreads();
-        scratch[t] += add;
-        __syncthreads();
-    }
-    u32 const incl = scratch[t];
-    *total = scratch[blockDim.x - 1];
-    __syncthreads();
-    return incl - v;
 }
 
 __device__ __forceinline__ void write_leaf(DevNode *tree, u32 id, float v)
@@ -489,13 +471,14 @@ __device__ __forceinline__ void hist_q_rows(unsigned long long *__restrict__ lh,
     }
 }
 
-// Quantized unit partials are wide: per partial, three u32 planes of node_cells each
-// (g as int32, h, row count).  A grouped work unit (up to kHistGroup 8192-row sub-units
-// of one node) adds every sub-unit's packed LDS cells into its own partial (or its
-// node's slot) in global memory and clears the LDS for the next sub-unit: only this
-// workgroup touches those cells during the launch, and the group writes one partial
-// instead of one per sub-unit (sums of 4 sub-units: |g| <= 127 * 32768,
-// h <= 32767 * 32768 < 2^32, count <= 2^15).
+// Quantized unit partials: packed 64-bit cells (one 8192-row sub-unit per unit), or wide
+// -- three u32 planes of node_cells each (g as int32, h, row count) -- on levels whose
+// units group several sub-units (HistChunk::wide).  A grouped work unit (up to
+// kHistGroupMax sub-units of one node) adds every sub-unit's packed LDS cells into its
+// own partial (or its node's slot) in global memory and clears the LDS for the next
+// sub-unit: only this workgroup touches those cells during the launch, and the group
+// writes one partial instead of one per sub-unit (sums of 4 sub-units:
+// |g| <= 127 * 32768, h <= 32767 * 32768 < 2^32, count <= 2^15).
 //
 // Flush of the unit's cells [cell0, cell0 + cells) after one sub-unit: `first` writes,
 // later sub-units add.  `clear` zeroes the LDS cells for the next sub-unit.
@@ -504,7 +487,10 @@ __device__ __forceinline__ void hq_flush(unsigned long long *__restrict__ lh, u3
                                          void *__restrict__ partials, void *__restrict__ hist_next,
                                          u32 *__restrict__ cnt_next)
 {
-    if (ch.part != kNone) {
+    if (ch.part != kNone && !ch.wide) {
+        u64 *P = static_cast<u64 *>(partials) + static_cast<size_t>(ch.part) * node_cells + cell0;
+        for (u32 i = threadIdx.x; i < cells; i += blockDim.x) P[i] = lh[i];
+    } else if (ch.part != kNone) {
         u32 *P = static_cast<u32 *>(partials) + static_cast<size_t>(ch.part) * node_cells * 3u + cell0;
         for (u32 i = threadIdx.x; i < cells; i += blockDim.x) {
             u64 const v = lh[i];
@@ -758,15 +744,11 @@ __global__ __launch_bounds__(512) void k_hist_q2(
         __syncthreads();
         // Flush in (bin, feature) order: LDS reads stay on distinct bank pairs.
         u32 const out_cells = F * SB;
-        if (ch.part != kNone) { // wide partial (units of one 8192-row sub-unit)
-            u32 *P = static_cast<u32 *>(partials) + static_cast<size_t>(ch.part) * node_cells * 3u;
+        if (ch.part != kNone) { // packed partial (units of one 8192-row sub-unit)
+            u64 *dst = static_cast<u64 *>(partials) + static_cast<size_t>(ch.part) * node_cells;
             for (u32 i = threadIdx.x; i < out_cells; i += blockDim.x) {
                 u32 const b = i / F, fl = i - b * F;
-                u64 const v = lh[b * kHq2Cols + fl];
-                size_t const o = static_cast<size_t>(fl) * SB + b;
-                P[o] = static_cast<u32>(static_cast<int32_t>(unpack_g(v)));
-                P[node_cells + o] = static_cast<u32>(unpack_h(v));
-                P[2 * node_cells + o] = unpack_cnt(v);
+                dst[static_cast<size_t>(fl) * SB + b] = lh[b * kHq2Cols + fl];
             }
         } else {
             size_t const base = static_cast<size_t>(ch.slot) * node_cells;
@@ -914,7 +896,28 @@ __global__ __launch_bounds__(256) void k_finish_q(FinishOp const *__restrict__ o
             bool const in = i < node_cells;
             i64 g = 0, h = 0;
             u32 n = 0;
-            if (in && op.part_count > 0) {
+            if (in && op.part_count > 0 && !op.wide) {
+                u64 const *pp = static_cast<u64 const *>(partials) + i;
+                u32 p = grp;
+                for (; p + 3 * kFqGroups < op.part_count; p += 4 * kFqGroups) {
+                    u64 v[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        v[k] = pp[static_cast<size_t>(op.part_first + p + k * kFqGroups) * node_cells];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        g += unpack_g(v[k]);
+                        h += unpack_h(v[k]);
+                        n += unpack_cnt(v[k]);
+                    }
+                }
+                for (; p < op.part_count; p += kFqGroups) {
+                    u64 const v = pp[static_cast<size_t>(op.part_first + p) * node_cells];
+                    g += unpack_g(v);
+                    h += unpack_h(v);
+                    n += unpack_cnt(v);
+                }
+            } else if (in && op.part_count > 0) {
                 // wide partials: three u32 planes (g as int32, h, count) per partial
                 u32 const *pp = static_cast<u32 const *>(partials) + i;
                 size_t const pst = static_cast<size_t>(node_cells) * 3u;
@@ -2291,6 +2294,7 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
                         fo.src = L.src;
                         fo.route = mode == 1 ? arank : kNone;
                         fo.side = left_small ? 0u : 1u;
+                        fo.wide = unit_rows > kHistChunk ? 1u : 0u;
                         fops[arank] = fo;
                     }
                 }
@@ -2399,6 +2403,7 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
         hc.part = fo.part_count ? fo.part_first + k : kNone;
         hc.route = fo.route;
         hc.side = fo.side;
+        hc.wide = fo.wide;
         hch[c] = hc;
     }
     if (threadIdx.x == 0) {
@@ -2447,6 +2452,7 @@ __global__ __launch_bounds__(256) void k_tree_begin(TreeCtl *__restrict__ ctl, N
         hc.begin = k * unit_rows;
         hc.end = min(n_rows, (k + 1) * unit_rows);
         hc.part = multi ? k : kNone;
+        hc.wide = unit_rows > kHistChunk ? 1u : 0u;
         hch[k] = hc;
     }
     if (threadIdx.x == 0) {
@@ -2467,6 +2473,7 @@ __global__ __launch_bounds__(256) void k_tree_begin(TreeCtl *__restrict__ ctl, N
         fo.part_first = 0;
         fo.part_count = multi ? n_units : 0;
         fo.empty = n_rows == 0;
+        fo.wide = unit_rows > kHistChunk ? 1u : 0u;
         fops[0] = fo;
         ctl->n_front = 1;
         ctl->n_nodes = 1;

@@ -14,10 +14,9 @@
 #include "bh_common.hpp"
 #include "kernels.hpp"
 #include "kernels_ext.hpp"
+#include "bh_device.hpp"
 
 namespace bh {
-
-__device__ u32 block_exscan_ext(u32 v, u32 *scratch, u32 *total);
 
 namespace {
 
@@ -373,7 +372,7 @@ __global__ __launch_bounds__(1024) void k_plan_oblivious(NodeInfo const *__restr
             nch = (nd.cnt + kPartChunk - 1) / kPartChunk;
         }
         u32 ctot;
-        u32 const cfirst = block_exscan_ext(nch, scratch, &ctot) + s_chunks;
+        u32 const cfirst = block_exscan(nch, scratch, &ctot) + s_chunks;
         if (in) {
             SplitRec r = s;
             if (!is_last) {
@@ -668,7 +667,7 @@ __global__ __launch_bounds__(1024) void k_lw_scan(PartOp *__restrict__ ops, Part
         u32 const c = t0 + threadIdx.x;
         u32 const v = c < n_ch ? chunk_left[c] : 0u;
         u32 tot;
-        u32 const pre = block_exscan_ext(v, scratch, &tot) + s_run;
+        u32 const pre = block_exscan(v, scratch, &tot) + s_run;
         if (c < n_ch) pch[c].left_before = pre;
         __syncthreads();
         if (threadIdx.x == 0) s_run += tot;
@@ -866,23 +865,6 @@ __global__ void k_obl_begin(TreeCtl *ctl)
 }
 
 } // namespace
-
-__device__ u32 block_exscan_ext(u32 v, u32 *scratch, u32 *total)
-{
-    u32 const t = threadIdx.x;
-    scratch[t] = v;
-    __syncthreads();
-    for (u32 s = 1; s < blockDim.x; s <<= 1) {
-        u32 const add = t >= s ? scratch[t - s] : 0u;
-        __syncthreads();
-        scratch[t] += add;
-        __syncthreads();
-    }
-    u32 const incl = scratch[t];
-    *total = scratch[blockDim.x - 1];
-    __syncthreads();
-    return incl - v;
-}
 
 hipError_t ext_alloc(ExtState &s, bh_fit_config const &f, u32 F, u32 SB, u32 max_front, u32 hist_slots)
 {
