@@ -143,6 +143,8 @@ def lib() -> C.CDLL:
         "bh_set_scores": [P, P],
         "bh_get_scores": [P, P],
         "bh_train": [P, u32],
+        "bh_set_validation": [P, P, u64, u64, P, C.c_float, u32],
+        "bh_get_eval": [P, P, u32, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_int32)],
         "bh_get_trees": [P, P, u32, P, u32, C.POINTER(u32)],
         "bh_max_nodes": [P, C.POINTER(u32)],
         "bh_set_gradients": [P, P, P, u32, P, P, P],
@@ -371,6 +373,28 @@ class HipContext:
         t = TuningC()
         _check(lib().bh_get_tuning(self._h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in TuningC._fields_}
+
+    def set_validation(self, bins: np.ndarray, labels: np.ndarray, init_score: float,
+                       early_stopping_rounds: int = 0) -> None:
+        """Arm the validation plane: every training round records its mean loss;
+        early_stopping_rounds > 0 stops training after that many rounds without a new
+        best and keeps the trees up to the best round."""
+        bins = np.ascontiguousarray(bins, np.uint8)
+        labels = np.ascontiguousarray(labels, np.float32)
+        if bins.ndim != 2 or labels.shape != (bins.shape[0],):
+            raise ConfigError("validation bins must be [n, F] with one label per row")
+        _check(lib().bh_set_validation(self._h, _ptr(bins), bins.shape[0], bins.shape[1], _ptr(labels),
+                                       float(init_score), int(early_stopping_rounds)))
+
+    def eval_history(self) -> tuple[np.ndarray, int, bool]:
+        """(losses so far, best round, stopped)."""
+        n = C.c_uint32(0)
+        best = C.c_uint32(0)
+        stopped = C.c_int32(0)
+        _check(lib().bh_get_eval(self._h, None, 0, C.byref(n), C.byref(best), C.byref(stopped)))
+        out = np.zeros(n.value, np.float32)
+        _check(lib().bh_get_eval(self._h, _ptr(out), n.value, C.byref(n), C.byref(best), C.byref(stopped)))
+        return out, best.value, bool(stopped.value)
 
     def set_tuning(self, **kw) -> None:
         cur = self.tuning()

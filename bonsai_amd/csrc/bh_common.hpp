@@ -167,7 +167,12 @@ struct TreeCtl {
     u32 max_depth_seen;
     u32 oblivious_depth; // levelwise: levels split so far
     u32 part_error;     // a partition chunk's prefix disagreed with its op's row counts
-    u32 pad[14];
+    // validation / early stopping (the fit loop's rule, src/cli/pipeline.cpp:443-476)
+    u32 es_n;           // validation losses recorded since the validation set was armed
+    u32 es_best_iter;   // index of the best (lowest) loss
+    u32 es_stopped;     // 1 once es_n - 1 - es_best_iter reached the patience
+    float es_best;      // best loss
+    u32 pad[10];
 };
 static_assert(sizeof(TreeCtl) == 128, "TreeCtl");
 

@@ -62,6 +62,13 @@ struct bh_context {
     u16 *d_nbins = nullptr;
     float *label = nullptr, *score = nullptr;
     bool have_labels = false;
+    // validation plane (eval_begin / eval_accumulate, device_context.cu:1666-1760)
+    u8 *vplane = nullptr;
+    float *vlabel = nullptr, *vscore = nullptr, *vhist = nullptr;
+    double *vpartial = nullptr;
+    u64 vN = 0;
+    u32 vhist_cap = 0, es_rounds = 0, es_base = 0;
+    bool have_valid = false, es_done = false;
     // gradients
     float2 *gh0f = nullptr, *ghwf[2] = {nullptr, nullptr};
     u32 *gh0q = nullptr, *ghwq[2] = {nullptr, nullptr};
@@ -469,6 +476,14 @@ void enqueue_iteration(bh_context *c)
     enqueue_grow(c);
     launch_score_update(c->stream, c->plane0, c->Fs, c->N, c->tree, c->ctl, c->d_nbins, c->F, c->fc.learning_rate,
                         c->score, nullptr, c->max_nodes);
+    if (c->have_valid) {
+        // the new tree on the validation plane, then the round's loss and the
+        // early-stopping rule (eval_accumulate + the fit loop's check)
+        launch_score_update(c->stream, c->vplane, c->Fs, c->vN, c->tree, c->ctl, c->d_nbins, c->F,
+                            c->fc.learning_rate, c->vscore, nullptr, c->max_nodes);
+        launch_eval(c->stream, c->fc.objective, c->vscore, c->vlabel, static_cast<u32>(c->vN), c->vpartial, c->ctl,
+                    c->vhist, c->vhist_cap, c->es_rounds);
+    }
     launch_copy_tree(c->stream, c->tree, c->ctl, c->trees_out, c->tree_sizes, c->max_nodes);
     launch_iter_advance(c->stream, c->ctl);
 }
@@ -628,7 +643,8 @@ int bh_destroy(bh_context *c)
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     free_fit(c);
-    void *ptrs[] = {c->plane0, c->work[0], c->work[1], c->d_nbins, c->label, c->score};
+    void *ptrs[] = {c->plane0, c->work[0], c->work[1], c->d_nbins, c->label, c->score,
+                    c->vplane, c->vlabel, c->vscore, c->vhist, c->vpartial};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -771,15 +787,82 @@ int bh_configure(bh_context *c, const bh_fit_config *cfg)
     return alloc_fit(c);
 }
 
+static void drop_graph(bh_context *c);
+
 int bh_set_scores(bh_context *c, const float *scores)
 {
     if (int r = check_ready(c)) return r;
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipMemcpy(c->score, scores, c->N * sizeof(float), hipMemcpyHostToDevice));
-    // A new fit starts: iteration counter and tree store reset.
+    // A new fit starts: iteration counter and tree store reset; a validation set armed
+    // for the previous fit is disarmed (its scores track that fit).
     TreeCtl z{};
     HIP_TRY(hipMemcpy(c->ctl, &z, sizeof(TreeCtl), hipMemcpyHostToDevice));
     c->trees_done = 0;
+    if (c->have_valid) {
+        c->have_valid = false;
+        c->es_done = false;
+        drop_graph(c);
+    }
+    return 0;
+}
+
+int bh_set_validation(bh_context *c, const uint8_t *bins, uint64_t n_rows, uint64_t row_stride, const float *labels,
+                      float init_score, uint32_t early_stopping_rounds)
+{
+    if (int r = check_ready(c)) return r;
+    if (n_rows == 0) return fail(1, "empty validation set");
+    if (n_rows > 0xFFFFFFFFull) return fail(1, "validation set larger than 2^32 rows");
+    if (row_stride < c->F) return fail(1, "row_stride < n_features");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    void *old[] = {c->vplane, c->vlabel, c->vscore, c->vhist, c->vpartial};
+    for (void *p : old)
+        if (p) (void)hipFree(p);
+    c->vplane = nullptr;
+    c->vlabel = c->vscore = c->vhist = nullptr;
+    c->vpartial = nullptr;
+    c->have_valid = false;
+    size_t const bytes = static_cast<size_t>(n_rows) * c->Fs;
+    HIP_TRY(dalloc(&c->vplane, bytes + kTailPadBytes));
+    HIP_TRY(hipMemset(c->vplane, 0, bytes + kTailPadBytes));
+    HIP_TRY(hipMemcpy2D(c->vplane, c->Fs, bins, row_stride, c->F, n_rows, hipMemcpyHostToDevice));
+    HIP_TRY(dalloc(&c->vlabel, n_rows));
+    HIP_TRY(hipMemcpy(c->vlabel, labels, n_rows * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<float> seed(n_rows, init_score);
+    HIP_TRY(dalloc(&c->vscore, n_rows));
+    HIP_TRY(hipMemcpy(c->vscore, seed.data(), n_rows * sizeof(float), hipMemcpyHostToDevice));
+    c->vhist_cap = 1u << 16;
+    HIP_TRY(dalloc(&c->vhist, c->vhist_cap));
+    HIP_TRY(dalloc(&c->vpartial, 1024));
+    // early-stopping state starts empty; the rounds count from here (warm start)
+    TreeCtl t;
+    HIP_TRY(hipMemcpy(&t, c->ctl, sizeof(TreeCtl), hipMemcpyDeviceToHost));
+    t.es_n = t.es_best_iter = t.es_stopped = 0;
+    t.es_best = 0.0F;
+    HIP_TRY(hipMemcpy(c->ctl, &t, sizeof(TreeCtl), hipMemcpyHostToDevice));
+    c->vN = n_rows;
+    c->es_rounds = early_stopping_rounds;
+    c->es_base = c->trees_done;
+    c->es_done = false;
+    c->have_valid = true;
+    drop_graph(c); // the iteration graph gains the validation launches
+    return 0;
+}
+
+int bh_get_eval(bh_context *c, float *history, uint32_t cap, uint32_t *n, uint32_t *best_iter, int32_t *stopped)
+{
+    if (!c) return fail(1, "null context");
+    if (!c->have_valid) return fail(1, "no validation set armed");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    TreeCtl t;
+    HIP_TRY(hipMemcpy(&t, c->ctl, sizeof(TreeCtl), hipMemcpyDeviceToHost));
+    u32 const k = std::min(t.es_n, c->vhist_cap);
+    if (history && cap) HIP_TRY(hipMemcpy(history, c->vhist, std::min(k, cap) * sizeof(float), hipMemcpyDeviceToHost));
+    if (n) *n = k;
+    if (best_iter) *best_iter = t.es_best_iter;
+    if (stopped) *stopped = static_cast<int32_t>(t.es_stopped);
     return 0;
 }
 
@@ -810,20 +893,30 @@ int bh_train(bh_context *c, uint32_t n_iters)
 {
     if (int r = check_ready(c)) return r;
     if (!c->have_labels) return fail(1, "no labels uploaded");
+    if (c->es_done) return 0; // the fit stopped early: its model is final
     HIP_TRY(hipSetDevice(c->device));
     if (int r = ensure_tree_out(c, n_iters)) return r;
     c->iter_ms.clear();
-    if (c->fc.use_graph && !c->fc.host_sync) {
-        if (!c->graph_exec) {
-            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            enqueue_iteration(c);
-            hipError_t const e = hipStreamEndCapture(c->stream, &c->graph);
-            if (e != hipSuccess) return fail(-1, std::string("hip: capture: ") + hipGetErrorString(e));
-            HIP_TRY(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
-        }
-        for (u32 it = 0; it < n_iters; ++it) {
+    bool const graph = c->fc.use_graph && !c->fc.host_sync;
+    if (graph && !c->graph_exec) {
+        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_iteration(c);
+        hipError_t const e = hipStreamEndCapture(c->stream, &c->graph);
+        if (e != hipSuccess) return fail(-1, std::string("hip: capture: ") + hipGetErrorString(e));
+        HIP_TRY(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+    }
+    // With early stopping the device records the rule's verdict every round; the host
+    // looks at it every kEsCheck rounds, so at most kEsCheck - 1 rounds past the stop
+    // are grown and dropped.
+    constexpr u32 kEsCheck = 8;
+    bool const es = c->have_valid && c->es_rounds > 0;
+    u32 const step = es ? kEsCheck : std::max(1u, n_iters);
+    for (u32 done = 0; done < n_iters; done += step) {
+        u32 const k = std::min(step, n_iters - done);
+        for (u32 it = 0; it < k; ++it) {
             if (c->timing) HIP_TRY(hipEventRecord(c->ev0, c->stream));
-            HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+            if (graph) HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+            else enqueue_iteration(c);
             if (c->timing) {
                 HIP_TRY(hipEventRecord(c->ev1, c->stream));
                 HIP_TRY(hipEventSynchronize(c->ev1));
@@ -832,23 +925,22 @@ int bh_train(bh_context *c, uint32_t n_iters)
                 c->iter_ms.push_back(ms);
             }
         }
-    } else {
-        for (u32 it = 0; it < n_iters; ++it) {
-            if (c->timing) HIP_TRY(hipEventRecord(c->ev0, c->stream));
-            enqueue_iteration(c);
-            if (c->timing) {
-                HIP_TRY(hipEventRecord(c->ev1, c->stream));
-                HIP_TRY(hipEventSynchronize(c->ev1));
-                float ms = 0.0F;
-                hipEventElapsedTime(&ms, c->ev0, c->ev1);
-                c->iter_ms.push_back(ms);
+        if (es) {
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            TreeCtl t;
+            HIP_TRY(hipMemcpy(&t, c->ctl, sizeof(TreeCtl), hipMemcpyDeviceToHost));
+            if (t.es_stopped) {
+                if (int r = check_partition(c)) return r;
+                c->trees_done = c->es_base + t.es_best_iter + 1; // Booster::truncate(best + 1)
+                c->es_done = true;
+                return 0;
             }
         }
+        c->trees_done += k;
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (int r = check_partition(c)) return r;
-    c->trees_done += n_iters;
     return 0;
 }
 
@@ -903,6 +995,7 @@ int bh_autotune(bh_context *c, uint32_t iters, bh_tuning *chosen, float *ms_per_
     if (int r = check_ready(c)) return r;
     if (!c->have_labels) return fail(1, "no labels uploaded");
     if (iters == 0) return fail(1, "iters must be positive");
+    if (c->have_valid) return fail(1, "autotune before arming a validation set (its scores would move)");
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     float *saved = nullptr;

@@ -2559,6 +2559,62 @@ __global__ __launch_bounds__(256) void k_score_update(u8 const *__restrict__ pla
     }
 }
 
+// Validation loss pass (eval_loss_pass1_kernel, src/cuda/detail/kernels.cuh:1061-1103):
+// per-row loss terms in float, summed in double by grid-stride loops, then a tree
+// reduction per block; one partial per block.  Objective terms as the reference's
+// objectives evaluate them (src/objective.cpp:60-120): squared error, and binary
+// cross-entropy from the raw score, max(0, s) + log1p(exp(-|s|)) - y * s.
+template <int OBJ>
+__global__ __launch_bounds__(256) void k_eval_loss(float const *__restrict__ scores, float const *__restrict__ labels,
+                                                   u32 n, double *__restrict__ partial)
+{
+    __shared__ double sl[256];
+    double acc = 0.0;
+    for (u32 r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        float const s = scores[r], y = labels[r];
+        if (OBJ == 0) {
+            double const d = static_cast<double>(s) - static_cast<double>(y);
+            acc += d * d;
+        } else {
+            float const ax = fabsf(s);
+            acc += fmaxf(0.0F, s) + log1pf(expf(-ax)) - y * s;
+        }
+    }
+    sl[threadIdx.x] = acc;
+    __syncthreads();
+    for (u32 st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) sl[threadIdx.x] += sl[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sl[0];
+}
+
+// Block partials summed in block order (fixed, so the loss is run-to-run stable), the
+// mean loss recorded, and the early-stopping rule applied: a new best (or the first
+// loss) resets the patience; once `patience` losses follow the best without beating it
+// the fit stops and keeps the trees up to the best one.  After the stop nothing more is
+// recorded.  patience == 0: record only.
+__global__ void k_eval_finish(double const *__restrict__ partial, u32 blocks, u32 n, TreeCtl *__restrict__ ctl,
+                              float *__restrict__ hist, u32 hist_cap, u32 patience)
+{
+    __shared__ double s_p[1024];
+    for (u32 b = threadIdx.x; b < blocks; b += blockDim.x) s_p[b] = partial[b];
+    __syncthreads();
+    if (threadIdx.x != 0 || ctl->es_stopped) return;
+    double total = 0.0;
+    for (u32 b = 0; b < blocks; ++b) total += s_p[b];
+    float const loss = static_cast<float>(total / static_cast<double>(n));
+    u32 const i = ctl->es_n;
+    if (i < hist_cap) hist[i] = loss;
+    ctl->es_n = i + 1;
+    if (i == 0 || loss < ctl->es_best) {
+        ctl->es_best = loss;
+        ctl->es_best_iter = i;
+    } else if (patience && i - ctl->es_best_iter >= patience) {
+        ctl->es_stopped = 1;
+    }
+}
+
 __global__ void k_iter_advance(TreeCtl *ctl)
 {
     if (threadIdx.x == 0) ctl->iter += 1;
@@ -2938,6 +2994,19 @@ void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode 
     else
         hipLaunchKernelGGL(k_score_update<false>, dim3(2048), dim3(256), head, s, plane, Fs, n, tree, ctl, n_bins, F,
                            lr, score, leaf_of_row, node_cap, last_cap);
+}
+
+u32 eval_blocks(u32 n) { return std::max(1u, std::min(n / 256u, 1024u)); }
+
+void launch_eval(hipStream_t s, int objective, float const *scores, float const *labels, u32 n, double *partial,
+                 TreeCtl *ctl, float *hist, u32 hist_cap, u32 patience)
+{
+    u32 const blocks = eval_blocks(n);
+    if (objective == 1)
+        hipLaunchKernelGGL(k_eval_loss<1>, dim3(blocks), dim3(256), 0, s, scores, labels, n, partial);
+    else
+        hipLaunchKernelGGL(k_eval_loss<0>, dim3(blocks), dim3(256), 0, s, scores, labels, n, partial);
+    hipLaunchKernelGGL(k_eval_finish, dim3(1), dim3(256), 0, s, partial, blocks, n, ctl, hist, hist_cap, patience);
 }
 
 void launch_iter_advance(hipStream_t s, TreeCtl *ctl)

@@ -50,6 +50,11 @@ void launch_part_scatter4(hipStream_t s, PartChunk const *pch, PartOp4 const *op
 void launch_finalize(hipStream_t s, NodeInfo const *front, TreeCtl const *ctl, DevNode *tree, GrowCfg const &cfg,
                      double const *scales, u32 max_front);
 void launch_next_level(hipStream_t s, TreeCtl *ctl);
+// Validation loss of one round (objective 0 MSE, 1 logloss) and the early-stopping rule
+// in the control block; eval_blocks(n) partials of scratch.
+u32 eval_blocks(u32 n);
+void launch_eval(hipStream_t s, int objective, float const *scores, float const *labels, u32 n, double *partial,
+                 TreeCtl *ctl, float *hist, u32 hist_cap, u32 patience);
 // unit_rows: rows per root histogram unit (kHistChunk, or a group of sub-units).
 void launch_tree_begin(hipStream_t s, TreeCtl *ctl, NodeInfo *front, DevNode *tree, HistChunk *hch,
                        FinishOp *fops, u32 n_rows, u32 unit_rows = kHistChunk);

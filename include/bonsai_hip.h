@@ -123,6 +123,18 @@ int bh_get_scores(bh_context *ctx, float *scores);
 /* Run n_iters boosting iterations on the device (gradient, quantize, grow, score
  * update).  With use_graph each iteration is one hipGraph launch. */
 int bh_train(bh_context *ctx, uint32_t n_iters);
+/* Validation plane (eval_begin / eval_accumulate, src/cuda/detail/device_context.cu:1666-1760,
+ * driven by the fit loop's early-stopping rule, src/cli/pipeline.cpp:443-476): binned rows
+ * (row-major u8, same features and bins as the training plane), labels, the seed score.
+ * Every bh_train round then walks the new tree on these rows and records the mean loss of
+ * the configured objective (MSE or logloss).  early_stopping_rounds > 0: once that many
+ * rounds follow the best loss without beating it, training stops and keeps the trees up
+ * to the best round (later bh_train calls return at once).  bh_set_scores disarms it. */
+int bh_set_validation(bh_context *ctx, const uint8_t *bins, uint64_t n_rows, uint64_t row_stride,
+                      const float *labels, float init_score, uint32_t early_stopping_rounds);
+/* Losses recorded so far (up to cap), their count, the best round and the stop flag. */
+int bh_get_eval(bh_context *ctx, float *history, uint32_t cap, uint32_t *n, uint32_t *best_iter,
+                int32_t *stopped);
 /* Trees grown so far: nodes[t * max_nodes + i], sizes[t]; returns via *n_trees. */
 int bh_get_trees(bh_context *ctx, bh_node *nodes, uint32_t max_nodes, uint32_t *sizes,
                  uint32_t cap_trees, uint32_t *n_trees);
