@@ -229,7 +229,7 @@ def main() -> int:
             "rows_per_s": round(a.rows * a.steps / dt_q, 1),
         },
         "roofline": {
-            "bound": "hbm", "kernel": "k_hist_q",
+            "bound": "hbm", "kernel": "k_hist_q / k_hist_q3",
             "achieved": round(hist_gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(hist_gbps / HBM_PEAK_GBPS, 4),
             "bytes_per_launch": round(per_launch_bytes), "avg_launch_us": round(per_launch_us, 2),
