@@ -38,6 +38,15 @@ constexpr u64 kPackCntMask = (1ull << 14) - 1ull;
 // cells); the group size is a tuning knob (1..kHistGroupMax).
 constexpr u32 kHistGroupMax = 4;
 constexpr u32 kPartChunk  = 4096;  // rows per partition work unit
+// Leafwise rounds partition one leaf and build one child's histogram; the work units are
+// sized to the leaf so that a small leaf is still spread over many workgroups: partition
+// chunks of kLwMinChunk..kPartChunk rows, about kLwChunkTarget of them, and (quantized:
+// exact integer sums, any unit size gives the same histogram) histogram units of
+// kLwMinUnit..kHistChunk rows, at most the context's unit target.
+constexpr u32 kLwChunkTarget = 1024;
+constexpr u32 kLwMinChunk = 256;
+constexpr u32 kLwUnitTarget = 256;
+constexpr u32 kLwMinUnit = 1024;
 constexpr u32 kMaxTreeNodes = 8192;
 // Tail padding of planes (bytes) and gradient buffers (elements): kernels that read
 // aligned 16-byte words around a row range may touch up to 15 bytes past its end.

@@ -99,6 +99,7 @@ struct bh_context {
     u32 trees_cap = 0, trees_done = 0, max_nodes = 0;
     // capacities
     u32 max_front = 0, max_ops = 0, max_pch = 0, max_hch = 0, max_parts = 0, hist_slots = 0;
+    u32 lw_units = 0; // quantized leafwise: histogram units per child at most
     // configuration
     bh_fit_config fc{};
     GrowCfg gc{};
@@ -198,6 +199,19 @@ int alloc_fit(bh_context *c)
     c->hist_slots = f.grower == BH_LEAFWISE ? static_cast<u32>(f.max_leaves) + 4u : std::max(1u, c->max_front / 2u);
     c->max_nodes = tree_capacity(f);
     size_t const nc = c->node_cells();
+    c->lw_units = 0;
+    if (f.grower == BH_LEAFWISE) {
+        // Leaf-sized partition chunks (at most kLwChunkTarget beyond the full-size ones)
+        // and, quantized, up to lw_units histogram units per child, their partials within
+        // about 256 MiB.
+        c->max_pch += kLwChunkTarget;
+        if (c->quant) {
+            size_t const per = std::max<size_t>(1, nc * c->part_bytes());
+            c->lw_units = static_cast<u32>(std::min<size_t>(kLwUnitTarget, std::max<size_t>(16, (256ull << 20) / per)));
+            c->max_hch = std::max(c->max_hch, c->lw_units + 2u);
+            c->max_parts = std::max(c->max_parts, c->lw_units + 2u);
+        }
+    }
     size_t const hist_bytes = static_cast<size_t>(c->hist_slots) * nc * c->cell_bytes();
     if (c->quant) {
         HIP_TRY(dalloc(&c->gh0q, N + kTailPad));
@@ -221,6 +235,7 @@ int alloc_fit(bh_context *c)
         HIP_TRY(dalloc(&c->ops4, c->max_ops / 2 + 2));
         HIP_TRY(dalloc(&c->chunk_cls, c->max_pch));
     }
+    if (c->quant && f.grower == BH_LEAFWISE) HIP_TRY(dalloc(&c->cnt[0], static_cast<size_t>(c->hist_slots) * nc));
     HIP_TRY(dalloc(&c->ctl, 1));
     HIP_TRY(hipMemset(c->ctl, 0, sizeof(TreeCtl)));
     u32 const fcap = std::max(c->max_front * 2u, 4u);
@@ -549,6 +564,8 @@ GrowView grow_view(bh_context *c)
     v.max_pch = c->max_pch;
     v.max_hch = c->max_hch;
     v.max_ops = c->max_ops;
+    v.cnt = c->cnt[0];
+    v.lw_units = c->lw_units;
     v.fsl = c->hist_fsl();
     v.waves = c->hist_waves();
     v.host_sync = c->fc.host_sync != 0;

@@ -992,14 +992,21 @@ __global__ __launch_bounds__(256) void k_finish_q(FinishOp const *__restrict__ o
 // false, strict > (earliest bin, then lowest feature, wins ties), an empty missing cell
 // scans one direction, rows < 2 * min_data_in_leaf cannot split.
 //
-// One workgroup per frontier node.  Phase 0 derives the node totals from its first
-// feature's cells (Histogram::totals, include/bonsai/histogram.hpp:76-87) unless a
-// parent split already set them.  Phase A: one thread per feature of a 16-feature tile
-// runs the sequential prefix into LDS.  Phase B: every (feature, bin, direction)
-// candidate is scored in parallel; phase C reduces to the winner with the serial
-// scan's tie order.
+// Up to kFindMaxBlocks workgroups per frontier node, each scanning kFindTile-feature
+// tiles.  Phase 0 derives the node totals from its first feature's cells
+// (Histogram::totals, include/bonsai/histogram.hpp:76-87) unless a parent split already
+// set them (quantized: every feature's cells sum to the same exact totals, so a block
+// takes them from its first staged feature).  Phase A: the per-feature prefix in bin
+// order into LDS (FP32: one thread per feature, sequential double sums; quantized: one
+// wave per feature, and the count cells' prefix beside it).  Phase B: every (feature,
+// bin, direction) candidate is scored in parallel; phase C reduces to the block's winner
+// with the serial scan's tie order, and the node's last block merges the blocks'
+// winners the same way.
 // ---------------------------------------------------------------------------------
 constexpr u32 kFindTile = 7;
+// Threads per split-find workgroup: 16 waves, so each SIMD interleaves four waves of the
+// FP64 candidate scoring.
+constexpr u32 kFindThreads = 1024;
 
 u32 find_blocks_per_node(u32 F)
 {
@@ -1011,7 +1018,7 @@ template <> struct CellT<false> { typedef float2 T; typedef double A; };
 template <> struct CellT<true> { struct T { i64 x, y; }; typedef i64 A; };
 
 template <bool Q>
-__global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, TreeCtl const *__restrict__ ctl,
+__global__ __launch_bounds__(kFindThreads) void k_find(NodeInfo *__restrict__ front, TreeCtl const *__restrict__ ctl,
                                               void const *__restrict__ hist, u16 const *__restrict__ n_bins,
                                               u32 F, u32 SB, GrowCfg cfg, double const *__restrict__ scales,
                                               SplitRec *__restrict__ out, u32 front_base, u32 n_front_override,
@@ -1024,18 +1031,21 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
     Cell *s_cells = reinterpret_cast<Cell *>(smem);              // [kFindTile][SB]
     Acc *pg = reinterpret_cast<Acc *>(s_cells + kFindTile * SB); // [kFindTile][SB]
     Acc *ph = pg + kFindTile * SB;                                // [kFindTile][SB]
-    __shared__ double s_best_gain[256];
-    __shared__ u32 s_best_key[256];
+    u32 *s_cn = reinterpret_cast<u32 *>(ph + kFindTile * SB);    // [kFindTile][SB] counts, then their prefix
+    __shared__ double s_best_gain[kFindThreads / 64];
+    __shared__ u32 s_best_key[kFindThreads / 64];
     __shared__ double s_tot[2];
     __shared__ i64 s_itot[2];
     __shared__ Acc s_mg[kFindTile], s_mh[kFindTile];
-    __shared__ u32 s_ncut[kFindTile], s_dirs[kFindTile];
+    __shared__ u32 s_ncut[kFindTile], s_dirs[kFindTile], s_nb[kFindTile], s_mc[kFindTile];
 
+    // Every launch's frontier buffer holds gridDim.y entries past front_base, so the
+    // node loads beside the frontier size.
+    u32 const i = front_base + blockIdx.y;
+    NodeInfo node = front[i];
     u32 const n_front = n_front_override ? n_front_override : ctl->n_front;
     if (blockIdx.y >= n_front) return;
-    u32 const i = front_base + blockIdx.y;
     u32 const G = gridDim.x, bx = blockIdx.x;
-    NodeInfo node = front[i];
     if (node.slot == kNone) {
         // No histogram (a leafwise child at max_depth): totals came from the parent
         // split and the node cannot split.
@@ -1048,6 +1058,7 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
     }
     size_t const node_cells = static_cast<size_t>(F) * SB;
     Cell const *hcells = reinterpret_cast<Cell const *>(hist) + static_cast<size_t>(node.slot) * node_cells;
+    u32 const *hcnt = cnt ? cnt + static_cast<size_t>(node.slot) * node_cells : nullptr;
     double const sg = Q ? scales[0] : 0.0, sh = Q ? scales[1] : 0.0;
     bool const can = node.cnt >= 2u * static_cast<u32>(cfg.min_data_in_leaf) &&
                      static_cast<int>(node.depth) < cfg.max_depth;
@@ -1056,12 +1067,16 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
     u32 best_key = kNone;
     double bgL = 0.0, bhL = 0.0, bgR = 0.0, bhR = 0.0; // winner's child sums (FP32 path)
     i64 bigL = 0, bihL = 0, bigR = 0, bihR = 0;        // winner's child sums (quantized)
+    u32 bnl = 0;                                       // winner's left rows (count cells)
     __shared__ u32 s_last;
     double node_score = 0.0, tg = 0.0, th = 0.0;
-    // Phase 0: node totals from the first feature's cells (every block of the node
-    // derives the same values) unless a parent split already set them.
     bool const had_totals = node.totals_ready != 0;
-    if (!had_totals) {
+    // Quantized totals are exact integers, equal over every feature's cells: a block that
+    // scans takes them from the first feature of its first tile (no extra round trip).
+    bool const tot_from_tile = Q && can && !had_totals;
+    // Phase 0: node totals from the first feature's cells (Histogram::totals,
+    // include/bonsai/histogram.hpp:76-87) unless a parent split already set them.
+    if (!had_totals && !tot_from_tile) {
         u32 const nb0 = n_bins[0];
         for (u32 k = threadIdx.x; k < nb0; k += blockDim.x) s_cells[k] = hcells[k];
         __syncthreads();
@@ -1083,8 +1098,8 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
                 }
             }
         } else if (threadIdx.x == 0) {
-            // Histogram::totals (include/bonsai/histogram.hpp:76-87): the first
-            // feature's cells, double in bin order, stored as float.
+            // Histogram::totals: the first feature's cells, double in bin order, stored
+            // as float.
             double g = 0.0, h = 0.0;
             float2 const *cf = reinterpret_cast<float2 const *>(s_cells);
             u32 b = 0;
@@ -1123,56 +1138,93 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
         }
         return;
     }
-    tg = Q ? static_cast<double>(s_itot[0]) * sg : s_tot[0];
-    th = Q ? static_cast<double>(s_itot[1]) * sh : s_tot[1];
-    node_score = score_of(tg, th, cfg.l1, cfg.l2);
     // Block bx scans feature tiles bx, bx + G, ...
     for (u32 f0 = bx * kFindTile; f0 < F; f0 += G * kFindTile) {
         u32 const nt = min(kFindTile, F - f0);
         __syncthreads(); // the previous tile (or phase 0) is done with the LDS
-        // Stage the tile's cells (coalesced).
+        // Stage the tile's cells, bin counts and count cells (coalesced, one round trip).
+        if (threadIdx.x < nt) s_nb[threadIdx.x] = n_bins[f0 + threadIdx.x];
         for (u32 k = threadIdx.x; k < nt * SB; k += blockDim.x) s_cells[k] = hcells[static_cast<size_t>(f0) * SB + k];
+        if (hcnt)
+            for (u32 k = threadIdx.x; k < nt * SB; k += blockDim.x) s_cn[k] = hcnt[static_cast<size_t>(f0) * SB + k];
         __syncthreads();
+        if (tot_from_tile && f0 == bx * kFindTile) {
+            if (threadIdx.x < 64) {
+                i64 g = 0, h = 0;
+                for (u32 b = threadIdx.x; b < SB; b += 64) {
+                    g += reinterpret_cast<i64 const *>(&s_cells[b])[0];
+                    h += reinterpret_cast<i64 const *>(&s_cells[b])[1];
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    g += __shfl_xor(g, o);
+                    h += __shfl_xor(h, o);
+                }
+                if (threadIdx.x == 0) {
+                    node.itg = g;
+                    node.ith = h;
+                    node.totals_ready = 1;
+                    s_itot[0] = g;
+                    s_itot[1] = h;
+                }
+            }
+            __syncthreads();
+        }
+        if (f0 == bx * kFindTile) {
+            tg = Q ? static_cast<double>(s_itot[0]) * sg : s_tot[0];
+            th = Q ? static_cast<double>(s_itot[1]) * sh : s_tot[1];
+            node_score = score_of(tg, th, cfg.l1, cfg.l2);
+        }
         // Phase A: per-feature running sums in bin order (src/split.cpp:131-158).
-        // Quantized: exact integers, so a wave scans a feature in parallel.
+        // Quantized: exact integers, so a wave scans a feature in parallel (the count
+        // cells alongside, in place).
         if (Q) {
             u32 const wv = threadIdx.x >> 6, ln = threadIdx.x & 63u;
             for (u32 t = wv; t < kFindTile; t += blockDim.x >> 6) {
                 u32 ncut = 0, dirs = 1;
                 if (t < nt) {
-                    u32 const nb = n_bins[f0 + t];
+                    u32 const nb = s_nb[t];
                     ncut = nb >= 2 ? nb - 2 : 0;
                     i64 const *ci = reinterpret_cast<i64 const *>(s_cells + t * SB);
+                    u32 *cn = s_cn + t * SB;
                     i64 const mg = nb ? ci[2 * (nb - 1)] : 0, mh = nb ? ci[2 * (nb - 1) + 1] : 0;
                     dirs = (mg == 0 && mh == 0) ? 1 : 2;
                     if (ln == 0) {
                         s_mg[t] = mg;
                         s_mh[t] = mh;
+                        s_mc[t] = hcnt && nb ? cn[nb - 1] : 0u;
                     }
                     u32 const per = (ncut + 63u) / 64u;
                     u32 const b0 = ln * per;
                     i64 lg[8], lh[8];
+                    u32 lc[8];
                     i64 g = 0, h = 0;
+                    u32 cc = 0;
 #pragma unroll
                     for (u32 k = 0; k < 8; ++k) {
                         u32 const b = b0 + k;
                         if (k < per && b < ncut) {
                             g += ci[2 * b];
                             h += ci[2 * b + 1];
+                            if (hcnt) cc += cn[b];
                         }
                         lg[k] = g;
                         lh[k] = h;
+                        lc[k] = cc;
                     }
                     // inclusive scan of the lane totals
                     i64 sg2 = g, sh2 = h;
+                    u32 sc2 = cc;
                     for (int o = 1; o < 64; o <<= 1) {
                         i64 const ug = __shfl_up(sg2, o), uh = __shfl_up(sh2, o);
+                        u32 const uc = __shfl_up(sc2, o);
                         if (static_cast<int>(ln) >= o) {
                             sg2 += ug;
                             sh2 += uh;
+                            sc2 += uc;
                         }
                     }
                     i64 const eg = sg2 - g, eh = sh2 - h;
+                    u32 const ec = sc2 - cc;
                     Acc *qg = pg + t * SB, *qh = ph + t * SB;
 #pragma unroll
                     for (u32 k = 0; k < 8; ++k) {
@@ -1180,6 +1232,7 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
                         if (k < per && b < ncut) {
                             qg[b] = eg + lg[k];
                             qh[b] = eh + lh[k];
+                            if (hcnt) cn[b] = ec + lc[k];
                         }
                     }
                 }
@@ -1193,121 +1246,135 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
         if (!Q && t < kFindTile) {
             u32 ncut = 0, dirs = 1;
             if (t < nt) {
-                u32 const nb = n_bins[f0 + t];
+                u32 const nb = s_nb[t];
                 ncut = nb >= 2 ? nb - 2 : 0;
                 Acc *qg = pg + t * SB, *qh = ph + t * SB;
-                if (Q) {
-                    i64 const *ci = reinterpret_cast<i64 const *>(s_cells + t * SB);
-                    i64 const mg = nb ? ci[2 * (nb - 1)] : 0, mh = nb ? ci[2 * (nb - 1) + 1] : 0;
-                    s_mg[t] = mg;
-                    s_mh[t] = mh;
-                    dirs = (mg == 0 && mh == 0) ? 1 : 2;
-                    i64 g = 0, h = 0;
-                    for (u32 b = 0; b < ncut; ++b) {
-                        g += ci[2 * b];
-                        h += ci[2 * b + 1];
-                        qg[b] = g;
-                        qh[b] = h;
-                    }
-                } else {
-                    float2 const *cf = reinterpret_cast<float2 const *>(s_cells + t * SB);
-                    float2 const m = nb ? cf[nb - 1] : make_float2(0.0F, 0.0F);
-                    s_mg[t] = m.x;
-                    s_mh[t] = m.y;
-                    dirs = (m.x == 0.0F && m.y == 0.0F) ? 1 : 2;
-                    double g = 0.0, h = 0.0;
-                    u32 b = 0;
-                    for (; b + 4 <= ncut; b += 4) {
-                        float2 const v0 = cf[b], v1 = cf[b + 1], v2 = cf[b + 2], v3 = cf[b + 3];
-                        g += v0.x; h += v0.y; qg[b] = g; qh[b] = h;
-                        g += v1.x; h += v1.y; qg[b + 1] = g; qh[b + 1] = h;
-                        g += v2.x; h += v2.y; qg[b + 2] = g; qh[b + 2] = h;
-                        g += v3.x; h += v3.y; qg[b + 3] = g; qh[b + 3] = h;
-                    }
-                    for (; b < ncut; ++b) {
-                        float2 const v = cf[b];
-                        g += v.x;
-                        h += v.y;
-                        qg[b] = g;
-                        qh[b] = h;
-                    }
+                float2 const *cf = reinterpret_cast<float2 const *>(s_cells + t * SB);
+                float2 const m = nb ? cf[nb - 1] : make_float2(0.0F, 0.0F);
+                s_mg[t] = m.x;
+                s_mh[t] = m.y;
+                dirs = (m.x == 0.0F && m.y == 0.0F) ? 1 : 2;
+                double g = 0.0, h = 0.0;
+                u32 b = 0;
+                for (; b + 4 <= ncut; b += 4) {
+                    float2 const v0 = cf[b], v1 = cf[b + 1], v2 = cf[b + 2], v3 = cf[b + 3];
+                    g += v0.x; h += v0.y; qg[b] = g; qh[b] = h;
+                    g += v1.x; h += v1.y; qg[b + 1] = g; qh[b + 1] = h;
+                    g += v2.x; h += v2.y; qg[b + 2] = g; qh[b + 2] = h;
+                    g += v3.x; h += v3.y; qg[b + 3] = g; qh[b + 3] = h;
+                }
+                for (; b < ncut; ++b) {
+                    float2 const v = cf[b];
+                    g += v.x;
+                    h += v.y;
+                    qg[b] = g;
+                    qh[b] = h;
                 }
             }
             s_ncut[t] = ncut;
             s_dirs[t] = dirs;
         }
         __syncthreads();
-        // Phase B: every (feature, cut bin, direction) candidate.
-        u32 const n_cand = kFindTile * SB * 2;
-        for (u32 k = threadIdx.x; k < n_cand; k += blockDim.x) {
-            u32 const tt = k / (2 * SB);
-            u32 const b = (k >> 1) % SB;
-            u32 const d = k & 1u;
-            if (tt >= nt || b >= s_ncut[tt] || d >= s_dirs[tt]) continue;
-            bool const dl = d == 0;
-            double gL, hL, gR, hR;
-            i64 igL = 0, ihL = 0, igR = 0, ihR = 0;
-            if (Q) {
-                i64 const lg = pg[tt * SB + b], lh = ph[tt * SB + b];
-                i64 const mg = s_mg[tt], mh = s_mh[tt];
-                i64 const rg = s_itot[0] - mg, rh = s_itot[1] - mh;
-                igL = lg + (dl ? mg : 0);
-                ihL = lh + (dl ? mh : 0);
-                igR = (rg - lg) + (!dl ? mg : 0);
-                ihR = (rh - lh) + (!dl ? mh : 0);
-                gL = static_cast<double>(igL) * sg;
-                hL = static_cast<double>(ihL) * sh;
-                gR = static_cast<double>(igR) * sg;
-                hR = static_cast<double>(ihR) * sh;
-            } else {
-                double const lg = pg[tt * SB + b], lh = ph[tt * SB + b];
-                double const mg = s_mg[tt], mh = s_mh[tt];
-                double const rg = tg - mg, rh = th - mh;
-                double const right_g = rg - lg, right_h = rh - lh;
-                gL = lg + (dl ? mg : 0.0);
-                hL = lh + (dl ? mh : 0.0);
-                gR = right_g + (!dl ? mg : 0.0);
-                hR = right_h + (!dl ? mh : 0.0);
-            }
-            if (hL < cfg.min_child_hess || hR < cfg.min_child_hess) continue;
-            double const cs = score_of(gL, hL, cfg.l1, cfg.l2) + score_of(gR, hR, cfg.l1, cfg.l2);
-            double const gain = cs - node_score;
-            if (!(gain > 0.0 && gain >= cfg.min_gain)) continue;
-            u32 const key = (f0 + tt) * (2u * SB) + b * 2u + d;
-            if (gain > best_gain || (gain == best_gain && key < best_key)) {
-                best_gain = gain;
-                best_key = key;
-                bgL = gL;
-                bhL = hL;
-                bgR = gR;
-                bhR = hR;
-                bigL = igL;
-                bihL = ihL;
-                bigR = igR;
-                bihR = ihR;
-            }
-        }
-        __syncthreads();
-    }
-    s_best_gain[threadIdx.x] = best_gain;
-    s_best_key[threadIdx.x] = best_key;
-    __syncthreads();
-    for (u32 st = blockDim.x / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st) {
-            double const og = s_best_gain[threadIdx.x + st];
-            u32 const ok = s_best_key[threadIdx.x + st];
-            double const mg = s_best_gain[threadIdx.x];
-            u32 const mk = s_best_key[threadIdx.x];
-            if (ok != kNone && (mk == kNone || og > mg || (og == mg && ok < mk))) {
-                s_best_gain[threadIdx.x] = og;
-                s_best_key[threadIdx.x] = ok;
+        // Phase B: every (feature, cut bin, direction) candidate; a thread takes one
+        // (feature, bin) pair and scores its directions from one prefix load.
+        u32 const n_pairs = nt * SB;
+        for (u32 pi = threadIdx.x; pi < n_pairs; pi += blockDim.x) {
+            u32 const tt = pi / SB;
+            u32 const b = pi - tt * SB;
+            if (b >= s_ncut[tt]) continue;
+            u32 const dirs = s_dirs[tt];
+            Acc const plg = pg[tt * SB + b], plh = ph[tt * SB + b];
+            u32 const pnl = hcnt ? s_cn[tt * SB + b] : 0u;
+            for (u32 d = 0; d < dirs; ++d) {
+                bool const dl = d == 0;
+                double gL, hL, gR, hR;
+                i64 igL = 0, ihL = 0, igR = 0, ihR = 0;
+                if (Q) {
+                    i64 const lg = plg, lh = plh;
+                    i64 const mg = s_mg[tt], mh = s_mh[tt];
+                    i64 const rg = s_itot[0] - mg, rh = s_itot[1] - mh;
+                    igL = lg + (dl ? mg : 0);
+                    ihL = lh + (dl ? mh : 0);
+                    igR = (rg - lg) + (!dl ? mg : 0);
+                    ihR = (rh - lh) + (!dl ? mh : 0);
+                    gL = static_cast<double>(igL) * sg;
+                    hL = static_cast<double>(ihL) * sh;
+                    gR = static_cast<double>(igR) * sg;
+                    hR = static_cast<double>(ihR) * sh;
+                } else {
+                    double const lg = plg, lh = plh;
+                    double const mg = s_mg[tt], mh = s_mh[tt];
+                    double const rg = tg - mg, rh = th - mh;
+                    double const right_g = rg - lg, right_h = rh - lh;
+                    gL = lg + (dl ? mg : 0.0);
+                    hL = lh + (dl ? mh : 0.0);
+                    gR = right_g + (!dl ? mg : 0.0);
+                    hR = right_h + (!dl ? mh : 0.0);
+                }
+                if (hL < cfg.min_child_hess || hR < cfg.min_child_hess) continue;
+                double const cs = score_of(gL, hL, cfg.l1, cfg.l2) + score_of(gR, hR, cfg.l1, cfg.l2);
+                double const gain = cs - node_score;
+                if (!(gain > 0.0 && gain >= cfg.min_gain)) continue;
+                u32 const key = (f0 + tt) * (2u * SB) + b * 2u + d;
+                if (gain > best_gain || (gain == best_gain && key < best_key)) {
+                    best_gain = gain;
+                    best_key = key;
+                    bgL = gL;
+                    bhL = hL;
+                    bgR = gR;
+                    bhR = hR;
+                    bigL = igL;
+                    bihL = ihL;
+                    bigR = igR;
+                    bihR = ihR;
+                    // Left rows: bins <= the cut bin, plus the missing bin when it routes left.
+                    if (hcnt) bnl = pnl + (dl ? s_mc[tt] : 0u);
+                }
             }
         }
         __syncthreads();
     }
-    // The thread that scored the winner publishes its child sums (keys are unique) to
-    // the block's partial record; the node's last block to finish merges the partials
-    // (gain, then lowest key) and, with count cells, derives the child row counts.
+    // Block winner (gain, then lowest key; a total order, so the butterfly is exact):
+    // within each wave by shuffles, then over the wave winners in wave 0.
+    {
+        double g = best_gain;
+        u32 k = best_key;
+        for (int o = 32; o > 0; o >>= 1) {
+            double const og = __shfl_xor(g, o);
+            u32 const ok = __shfl_xor(k, o);
+            if (ok != kNone && (k == kNone || og > g || (og == g && ok < k))) {
+                g = og;
+                k = ok;
+            }
+        }
+        if ((threadIdx.x & 63u) == 0) {
+            s_best_gain[threadIdx.x >> 6] = g;
+            s_best_key[threadIdx.x >> 6] = k;
+        }
+        __syncthreads();
+        if (threadIdx.x < 64u) {
+            u32 const nw = blockDim.x >> 6;
+            g = threadIdx.x < nw ? s_best_gain[threadIdx.x] : 0.0;
+            k = threadIdx.x < nw ? s_best_key[threadIdx.x] : kNone;
+            for (int o = 32; o > 0; o >>= 1) {
+                double const og = __shfl_xor(g, o);
+                u32 const ok = __shfl_xor(k, o);
+                if (ok != kNone && (k == kNone || og > g || (og == g && ok < k))) {
+                    g = og;
+                    k = ok;
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_best_gain[0] = g;
+            s_best_key[0] = k;
+        }
+        __syncthreads();
+    }
+    // The thread that scored the winner publishes its child sums and left rows (keys
+    // are unique) to the block's partial record; the node's last block to finish merges
+    // the partials (gain, then lowest key).
     u32 const key = s_best_key[0];
     SplitRec *dst = part + static_cast<size_t>(i) * G + bx;
     if (key != kNone && best_key == key) {
@@ -1326,6 +1393,10 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
         rec.ihL = bihL;
         rec.igR = bigR;
         rec.ihR = bihR;
+        if (hcnt) {
+            rec.n_left = bnl;
+            rec.n_right = node.cnt - bnl;
+        }
         *dst = rec;
         __threadfence();
     } else if (key == kNone && threadIdx.x == 0) {
@@ -1338,7 +1409,6 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
     if (threadIdx.x == 0) s_last = atomicAdd(done + i, 1u) == G - 1u ? 1u : 0u;
     __syncthreads();
     if (s_last == 0u) return;
-    __shared__ SplitRec s_win;
     __shared__ SplitRec s_parts[kFindMaxBlocks];
     // The G partial records are loaded in parallel by lanes of wave 0 (G <= 64), after
     // one fence in that wave, then compared.
@@ -1348,37 +1418,28 @@ __global__ __launch_bounds__(256) void k_find(NodeInfo *__restrict__ front, Tree
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        SplitRec best{};
-        best.default_left = 1;
-        u32 bkey = kNone;
+        u32 bi = kNone, bkey = kNone;
+        double bg = 0.0;
         for (u32 k = 0; k < G; ++k) {
-            SplitRec const r = s_parts[k];
+            SplitRec const &r = s_parts[k];
             if (!r.valid) continue;
             u32 const rk = r.feature * (2u * SB) + r.bin * 2u + (r.default_left ? 0u : 1u);
-            if (bkey == kNone || r.gain > best.gain || (r.gain == best.gain && rk < bkey)) {
-                best = r;
+            if (bkey == kNone || r.gain > bg || (r.gain == bg && rk < bkey)) {
+                bi = k;
                 bkey = rk;
+                bg = r.gain;
             }
         }
-        s_win = best;
         done[i] = 0u;
         if (!had_totals) front[i] = node;
+        if (bi == kNone) {
+            SplitRec rec{};
+            rec.default_left = 1;
+            out[i] = rec;
+        } else {
+            out[i] = s_parts[bi];
+        }
     }
-    __syncthreads();
-    if (threadIdx.x >= 64) return;
-    SplitRec w = s_win;
-    if (cnt && w.valid) {
-        // Left rows: bins <= split bin, plus the missing bin when it routes left.
-        u32 const *cc = cnt + static_cast<size_t>(node.slot) * node_cells + static_cast<size_t>(w.feature) * SB;
-        u32 const last = n_bins[w.feature] - 1u;
-        u32 nl = 0;
-        for (u32 b = threadIdx.x; b <= w.bin; b += 64) nl += cc[b];
-        if (threadIdx.x == 0 && w.default_left) nl += cc[last];
-        for (int o = 32; o > 0; o >>= 1) nl += __shfl_xor(nl, o);
-        w.n_left = nl;
-        w.n_right = node.cnt - nl;
-    }
-    if (threadIdx.x == 0) out[i] = w;
 }
 
 // ---------------------------------------------------------------------------------
@@ -2856,14 +2917,15 @@ void launch_find(hipStream_t s, bool q, NodeInfo *front, TreeCtl const *ctl, voi
                  u32 F, u32 SB, GrowCfg const &cfg, double const *scales, SplitRec *out, u32 max_front,
                  u32 front_base, u32 n_front_override, FindScratch const &fs, u32 const *cnt)
 {
-    size_t const lds = static_cast<size_t>(kFindTile) * SB * (q ? 32 : 24);
+    // cells + two prefix planes, and (quantized with count cells) the counts' prefix
+    size_t const lds = static_cast<size_t>(kFindTile) * SB * (q ? (cnt ? 36 : 32) : 24);
     u32 const G = find_blocks_per_node(F);
     dim3 const grid(G, max_front);
     if (q)
-        hipLaunchKernelGGL(k_find<true>, grid, dim3(256), lds, s, front, ctl, hist, n_bins, F, SB, cfg, scales, out,
+        hipLaunchKernelGGL(k_find<true>, grid, dim3(kFindThreads), lds, s, front, ctl, hist, n_bins, F, SB, cfg, scales, out,
                            front_base, n_front_override, fs.part, fs.done, cnt);
     else
-        hipLaunchKernelGGL(k_find<false>, grid, dim3(256), lds, s, front, ctl, hist, n_bins, F, SB, cfg, scales, out,
+        hipLaunchKernelGGL(k_find<false>, grid, dim3(kFindThreads), lds, s, front, ctl, hist, n_bins, F, SB, cfg, scales, out,
                            front_base, n_front_override, fs.part, fs.done, nullptr);
 }
 

@@ -515,22 +515,188 @@ struct LwCtl {
     u32 parent_slot;
     u32 demoted;
     u32 small_left;
-    u32 pad[11];
+    u32 n_push;      // children of the last round that still have to join the pool
+    u32 pad[10];
 };
 
-// Round start: pop the best candidate (max gain, ties to the lower node id; the
-// reference's gain_less, src/grower_impl.hpp:763-770) unless the heap is empty or the
-// leaf budget is spent, then commit its split (commit_split_node, :188-210) and emit
-// the one partition op.
+// Rows per partition chunk of a leaf of `rows` rows: about kLwChunkTarget chunks,
+// multiples of 256 rows, kLwMinChunk..kPartChunk.
+__device__ __forceinline__ u32 lw_chunk_rows(u32 rows)
+{
+    u32 const per = ((rows + kLwChunkTarget - 1u) / kLwChunkTarget + 255u) & ~255u;
+    return min(kPartChunk, max(kLwMinChunk, per));
+}
+
+// Rows per histogram unit of a child of `rows` rows: at most `target` units of
+// kLwMinUnit..kHistChunk rows (multiples of 256); target 0 keeps kHistChunk (the FP32
+// summation order is defined on kHistChunk units).
+__device__ __forceinline__ u32 lw_unit_rows(u32 rows, u32 target)
+{
+    if (target == 0) return kHistChunk;
+    u32 const per = ((rows + target - 1u) / target + 255u) & ~255u;
+    return min(kHistChunk, max(kLwMinUnit, per));
+}
+
+// queue_children (src/grower_impl.hpp:404-428): a child with a valid split joins the
+// pool; one without becomes a leaf now and frees its slot.  One thread.
+__device__ void lw_push(LwCtl *lw, NodeInfo const *kids, SplitRec const *splits, u32 n, bool is_root, Cand *cands,
+                        u32 *free_slots, DevNode *tree, GrowCfg const &cfg, double const *scales)
+{
+    for (u32 k = 0; k < n; ++k) {
+        NodeInfo const nd = kids[k];
+        SplitRec const s = splits[k];
+        bool const ok = s.valid && (!is_root || cfg.max_depth > 0);
+        if (ok) {
+            Cand c;
+            c.node = nd;
+            c.split = s;
+            cands[lw->n_cand++] = c;
+        } else {
+            put_leaf(tree, nd.id, node_value(nd, cfg, scales));
+            if (nd.slot != kNone) free_slots[lw->n_free++] = nd.slot;
+        }
+    }
+}
+
+// The partitioned leaf's children (commit_pop, src/grower_impl.hpp:370-400), one
+// thread: demotion when every row went one way; else two children, the larger keeping
+// the parent's histogram slot (completed in place) and the smaller getting a fresh slot
+// and the histogram units in *fo, unless the children sit at max_depth (then they only
+// need totals, from the parent split's sums).  split_totals: the children's totals are
+// the split's exact integer sums (quantized), so the finder need not derive them.
+// Returns the smaller child's unit count (0: no histogram this round).
+template <bool Q>
+__device__ u32 lw_children(PartOp &op, u32 nL, TreeCtl *ctl, LwCtl *lw, NodeInfo const &par, SplitRec const &s,
+                           u32 *free_slots, NodeInfo *kids, DevNode *tree, GrowCfg const &cfg,
+                           double const *scales, bool split_totals, u32 unit_target, FinishOp &fo,
+                           u32 &unit_rows)
+{
+    u32 const nR = op.cnt - nL;
+    op.n_left = nL;
+    bool const demote = (nL == 0) != (nR == 0);
+    op.alive = demote ? 0u : 1u;
+    lw->n_push = 0;
+    if (demote) {
+        put_leaf(tree, par.id, node_value(par, cfg, scales));
+        tree[par.id].gain = 0.0F;
+        free_slots[lw->n_free++] = par.slot;
+        ctl->n_front = 0;
+        ctl->n_finish = 0;
+        ctl->n_hist_chunks = 0;
+        lw->demoted = 1;
+        return 0;
+    }
+    lw->demoted = 0;
+    tree[op.left_id].cover = static_cast<float>(nL);
+    tree[op.right_id].cover = static_cast<float>(nR);
+    NodeInfo L{}, R{};
+    L.id = op.left_id;
+    R.id = op.right_id;
+    L.off = op.off;
+    L.cnt = nL;
+    R.off = op.off + nL;
+    R.cnt = nR;
+    L.src = R.src = op.dst;
+    L.depth = R.depth = par.depth + 1;
+    bool const left_small = nL <= nR;
+    bool const leaves = static_cast<int>(L.depth) >= cfg.max_depth;
+    ctl->live += 1;
+    ctl->max_depth_seen = max(ctl->max_depth_seen, L.depth);
+    if (leaves || split_totals) {
+        L.totals_ready = R.totals_ready = 1;
+        if (Q) {
+            L.itg = s.igL;
+            L.ith = s.ihL;
+            R.itg = s.igR;
+            R.ith = s.ihR;
+        } else {
+            L.tg = static_cast<float>(s.gL);
+            L.th = static_cast<float>(s.hL);
+            R.tg = static_cast<float>(s.gR);
+            R.th = static_cast<float>(s.hR);
+        }
+    }
+    u32 units = 0;
+    if (leaves) {
+        L.slot = R.slot = kNone;
+        free_slots[lw->n_free++] = par.slot;
+        ctl->n_finish = 0;
+        ctl->n_hist_chunks = 0;
+    } else {
+        u32 const fresh = free_slots[--lw->n_free];
+        (left_small ? L : R).slot = fresh;
+        (left_small ? R : L).slot = par.slot;
+        u32 const small_cnt = left_small ? nL : nR;
+        unit_rows = lw_unit_rows(small_cnt, unit_target);
+        units = (small_cnt + unit_rows - 1) / unit_rows;
+        fo = FinishOp{};
+        fo.small_slot = fresh;
+        fo.large_slot = par.slot;
+        fo.parent_slot = par.slot;
+        fo.part_first = 0;
+        fo.part_count = units > 1 ? units : 0;
+        fo.empty = small_cnt == 0;
+        fo.unit_first = 0;
+        fo.unit_count = units;
+        fo.small_off = left_small ? L.off : R.off;
+        fo.small_cnt = small_cnt;
+        fo.src = op.dst;
+        ctl->n_finish = 1;
+        ctl->n_hist_chunks = units;
+    }
+    lw->small_left = left_small ? 1u : 0u;
+    lw->n_push = 2;
+    kids[0] = L;
+    kids[1] = R;
+    ctl->n_front = 2;
+    return units;
+}
+
+// Histogram units of the smaller child: rows [off, off + cnt) of plane `src` in
+// unit_rows pieces, each with its own partial when there is more than one.  u[] holds
+// (slot, src, off, cnt) in LDS.
+__device__ __forceinline__ void lw_expand_units(u32 const *u, u32 units, u32 unit_rows, HistChunk *hch)
+{
+    for (u32 k = threadIdx.x; k < units; k += blockDim.x) {
+        HistChunk hc{};
+        hc.slot = u[0];
+        hc.src = u[1];
+        hc.begin = u[2] + k * unit_rows;
+        hc.end = u[2] + min(u[3], (k + 1) * unit_rows);
+        hc.part = units > 1 ? k : kNone;
+        hch[k] = hc;
+    }
+}
+
+__device__ __forceinline__ void lw_stash_units(u32 *u, FinishOp const &fo)
+{
+    u[0] = fo.small_slot;
+    u[1] = fo.src;
+    u[2] = fo.small_off;
+    u[3] = fo.small_cnt;
+}
+
+// Round start: the last round's children join the pool, then the best candidate is
+// popped (max gain, ties to the lower node id; the reference's gain_less,
+// src/grower_impl.hpp:763-770) unless the pool is empty or the leaf budget is spent,
+// its split committed (commit_split_node, :188-210) and the one partition op emitted,
+// in chunks sized to the leaf.  COUNTED (quantized): the split carries its child row
+// counts, so the children, their slots and the histogram units are set here and the
+// scatter takes each chunk's prefix from the chunk counts.
+template <bool COUNTED>
 __global__ __launch_bounds__(256) void k_lw_select(TreeCtl *__restrict__ ctl, LwCtl *__restrict__ lw,
-                                                   Cand *__restrict__ cands, NodeInfo *__restrict__ parent_out,
+                                                   Cand *__restrict__ cands, NodeInfo *__restrict__ kids,
+                                                   SplitRec const *__restrict__ kid_splits,
+                                                   u32 *__restrict__ free_slots, NodeInfo *__restrict__ parent_out,
                                                    SplitRec *__restrict__ split_out, DevNode *__restrict__ tree,
                                                    PartOp *__restrict__ ops, PartChunk *__restrict__ pch,
-                                                   u16 const *__restrict__ n_bins, GrowCfg cfg)
+                                                   HistChunk *__restrict__ hch, u16 const *__restrict__ n_bins,
+                                                   FinishOp *__restrict__ fops, GrowCfg cfg,
+                                                   double const *__restrict__ scales, u32 unit_target)
 {
     __shared__ double s_g[256];
     __shared__ u32 s_id[256], s_ix[256];
-    __shared__ u32 s_sel, s_go;
+    __shared__ u32 s_sel, s_go, s_n, s_units, s_urows, s_fo[4];
     if (ctl->done) {
         if (threadIdx.x == 0) {
             ctl->n_ops = 0;
@@ -541,7 +707,14 @@ __global__ __launch_bounds__(256) void k_lw_select(TreeCtl *__restrict__ ctl, Lw
         }
         return;
     }
-    u32 const n = lw->n_cand;
+    if (threadIdx.x == 0) {
+        u32 const np = lw->n_push;
+        if (np) lw_push(lw, kids, kid_splits, np, false, cands, free_slots, tree, cfg, scales);
+        lw->n_push = 0;
+        s_n = lw->n_cand;
+    }
+    __syncthreads();
+    u32 const n = s_n;
     double bg = 0.0;
     u32 bid = kNone, bix = kNone;
     for (u32 k = threadIdx.x; k < n; k += blockDim.x) {
@@ -591,7 +764,8 @@ __global__ __launch_bounds__(256) void k_lw_select(TreeCtl *__restrict__ ctl, Lw
     if (!s_go) return;
     Cand const c = cands[s_sel];
     __syncthreads();
-    u32 const nch = (c.node.cnt + kPartChunk - 1) / kPartChunk;
+    u32 const crow = lw_chunk_rows(c.node.cnt);
+    u32 const nch = (c.node.cnt + crow - 1) / crow;
     if (threadIdx.x == 0) {
         // remove from the pool (order inside the pool is irrelevant)
         cands[s_sel] = cands[n - 1];
@@ -628,26 +802,36 @@ __global__ __launch_bounds__(256) void k_lw_select(TreeCtl *__restrict__ ctl, Lw
         op.left_id = lid;
         op.right_id = rid;
         op.alive = 1;
+        lw->parent_slot = c.node.slot;
+        if (COUNTED) {
+            FinishOp fo{};
+            u32 urows = kHistChunk;
+            s_units = lw_children<true>(op, c.split.n_left, ctl, lw, c.node, c.split, free_slots, kids, tree, cfg,
+                                        scales, true, unit_target, fo, urows);
+            lw_stash_units(s_fo, fo);
+            s_urows = urows;
+            if (s_units) fops[0] = fo;
+        }
         ops[0] = op;
         ctl->n_ops = 1;
         ctl->n_part_chunks = nch;
-        lw->parent_slot = c.node.slot;
     }
     for (u32 k = threadIdx.x; k < nch; k += blockDim.x) {
         PartChunk pc;
         pc.op = 0;
-        pc.begin = k * kPartChunk;
-        pc.end = min(c.node.cnt, (k + 1) * kPartChunk);
+        pc.begin = k * crow;
+        pc.end = min(c.node.cnt, (k + 1) * crow);
         pc.left_before = 0;
         pch[k] = pc;
     }
+    if (COUNTED) {
+        __syncthreads();
+        lw_expand_units(s_fo, s_units, s_urows, hch);
+    }
 }
 
-// Round middle: left count from the unit prefix, demotion (commit_pop,
-// src/grower_impl.hpp:370-400), children, slot handling (the larger child keeps the
-// parent's histogram slot and is completed in place), histogram units of the smaller
-// child unless the children sit at max_depth (then they only need totals, from the
-// parent split's sums).
+// Round middle of the FP32 grower: left count from the chunk prefix, then the children
+// (lw_children) and the smaller child's histogram units.
 template <bool Q>
 __global__ __launch_bounds__(1024) void k_lw_scan(PartOp *__restrict__ ops, PartChunk *__restrict__ pch,
                                                   u32 const *__restrict__ chunk_left, TreeCtl *__restrict__ ctl,
@@ -658,7 +842,7 @@ __global__ __launch_bounds__(1024) void k_lw_scan(PartOp *__restrict__ ops, Part
                                                   GrowCfg cfg, double const *__restrict__ scales)
 {
     __shared__ u32 scratch[1024];
-    __shared__ u32 s_run;
+    __shared__ u32 s_run, s_units, s_urows, s_fo[4];
     if (ctl->n_ops == 0) return;
     u32 const n_ch = ctl->n_part_chunks;
     if (threadIdx.x == 0) s_run = 0;
@@ -673,137 +857,47 @@ __global__ __launch_bounds__(1024) void k_lw_scan(PartOp *__restrict__ ops, Part
         if (threadIdx.x == 0) s_run += tot;
         __syncthreads();
     }
-    if (threadIdx.x != 0) {
-        return;
-    }
-    PartOp op = ops[0];
-    NodeInfo const par = parent_in[0];
-    SplitRec const s = split_in[0];
-    u32 const nL = s_run, nR = op.cnt - s_run;
-    op.n_left = nL;
-    bool const demote = (nL == 0) != (nR == 0);
-    op.alive = demote ? 0u : 1u;
-    ops[0] = op;
-    if (demote) {
-        put_leaf(tree, par.id, node_value(par, cfg, scales));
-        tree[par.id].gain = 0.0F;
-        free_slots[lw->n_free++] = par.slot;
-        ctl->n_front = 0;
-        ctl->n_finish = 0;
-        ctl->n_hist_chunks = 0;
-        lw->demoted = 1;
-        return;
-    }
-    lw->demoted = 0;
-    tree[op.left_id].cover = static_cast<float>(nL);
-    tree[op.right_id].cover = static_cast<float>(nR);
-    NodeInfo L{}, R{};
-    L.id = op.left_id;
-    R.id = op.right_id;
-    L.off = op.off;
-    L.cnt = nL;
-    R.off = op.off + nL;
-    R.cnt = nR;
-    L.src = R.src = op.dst;
-    L.depth = R.depth = par.depth + 1;
-    bool const left_small = nL <= nR;
-    bool const leaves = static_cast<int>(L.depth) >= cfg.max_depth;
-    ctl->live += 1;
-    ctl->max_depth_seen = max(ctl->max_depth_seen, L.depth);
-    if (leaves) {
-        // children can never split: totals from the split's sums, no histograms
-        L.totals_ready = R.totals_ready = 1;
-        if (Q) {
-            L.itg = s.igL;
-            L.ith = s.ihL;
-            R.itg = s.igR;
-            R.ith = s.ihR;
-        } else {
-            L.tg = static_cast<float>(s.gL);
-            L.th = static_cast<float>(s.hL);
-            R.tg = static_cast<float>(s.gR);
-            R.th = static_cast<float>(s.hR);
-        }
-        L.slot = R.slot = kNone;
-        free_slots[lw->n_free++] = par.slot;
-        ctl->n_finish = 0;
-        ctl->n_hist_chunks = 0;
-    } else {
-        u32 const fresh = free_slots[--lw->n_free];
-        (left_small ? L : R).slot = fresh;
-        (left_small ? R : L).slot = par.slot;
-        u32 const small_cnt = left_small ? nL : nR;
-        u32 const soff = left_small ? L.off : R.off;
-        u32 const units = (small_cnt + kHistChunk - 1) / kHistChunk;
+    if (threadIdx.x == 0) {
+        PartOp op = ops[0];
         FinishOp fo{};
-        fo.small_slot = fresh;
-        fo.large_slot = par.slot;
-        fo.parent_slot = par.slot;
-        fo.part_first = 0;
-        fo.part_count = units > 1 ? units : 0;
-        fo.empty = small_cnt == 0;
-        fo.unit_first = 0;
-        fo.unit_count = units;
-        fo.small_off = soff;
-        fo.small_cnt = small_cnt;
-        fo.src = op.dst;
-        fops[0] = fo;
-        ctl->n_finish = 1;
-        ctl->n_hist_chunks = units;
+        u32 urows = kHistChunk;
+        s_units = lw_children<Q>(op, s_run, ctl, lw, parent_in[0], split_in[0], free_slots, kids, tree, cfg, scales,
+                                 false, 0u, fo, urows);
+        ops[0] = op;
+        lw_stash_units(s_fo, fo);
+        s_urows = urows;
+        if (s_units) fops[0] = fo;
     }
-    lw->small_left = left_small ? 1u : 0u;
-    kids[0] = L;
-    kids[1] = R;
-    ctl->n_front = 2;
+    __syncthreads();
+    lw_expand_units(s_fo, s_units, s_urows, hch);
 }
 
-// Histogram units of the smaller child (parallel expand of fops[0]).
-__global__ __launch_bounds__(256) void k_lw_units(TreeCtl const *__restrict__ ctl, FinishOp const *__restrict__ fops,
-                                                  HistChunk *__restrict__ hch)
-{
-    if (ctl->n_finish == 0) return;
-    FinishOp const fo = fops[0];
-    for (u32 k = threadIdx.x + blockIdx.x * blockDim.x; k < fo.unit_count; k += blockDim.x * gridDim.x) {
-        HistChunk hc{};
-        hc.slot = fo.small_slot;
-        hc.src = fo.src;
-        hc.begin = fo.small_off + k * kHistChunk;
-        hc.end = fo.small_off + min(fo.small_cnt, (k + 1) * kHistChunk);
-        hc.part = fo.unit_count > 1 ? k : kNone;
-        hch[k] = hc;
-    }
-}
-
-// Round end (queue_children, src/grower_impl.hpp:404-428): a child with a valid split
-// joins the pool; one without becomes a leaf now and frees its slot.
-__global__ void k_lw_push(TreeCtl *__restrict__ ctl, LwCtl *__restrict__ lw, NodeInfo const *__restrict__ kids,
-                          SplitRec const *__restrict__ splits, Cand *__restrict__ cands, u32 *__restrict__ free_slots,
-                          DevNode *__restrict__ tree, GrowCfg cfg, double const *__restrict__ scales, u32 is_root)
+// The root's split joins the pool (or the root becomes a leaf).
+__global__ void k_lw_push_root(LwCtl *__restrict__ lw, NodeInfo const *__restrict__ kids,
+                               SplitRec const *__restrict__ splits, Cand *__restrict__ cands,
+                               u32 *__restrict__ free_slots, DevNode *__restrict__ tree, GrowCfg cfg,
+                               double const *__restrict__ scales)
 {
     if (threadIdx.x != 0) return;
-    if (ctl->done) return;
-    u32 const n = is_root ? 1u : ctl->n_front;
-    for (u32 k = 0; k < n; ++k) {
-        NodeInfo const nd = kids[k];
-        SplitRec const s = splits[k];
-        bool const ok = s.valid && (!is_root || cfg.max_depth > 0);
-        if (ok) {
-            Cand c;
-            c.node = nd;
-            c.split = s;
-            cands[lw->n_cand++] = c;
-        } else {
-            put_leaf(tree, nd.id, node_value(nd, cfg, scales));
-            if (nd.slot != kNone) free_slots[lw->n_free++] = nd.slot;
-        }
-    }
+    lw_push(lw, kids, splits, 1, true, cands, free_slots, tree, cfg, scales);
 }
 
-// Tree end: what is left in the pool becomes leaves (src/grower_impl.hpp:838-846).
-__global__ void k_lw_finalize(TreeCtl *__restrict__ ctl, LwCtl *__restrict__ lw, Cand const *__restrict__ cands,
-                              DevNode *__restrict__ tree, GrowCfg cfg, double const *__restrict__ scales)
+// Tree end: the last round's children join the pool, then what is left in the pool
+// becomes leaves (src/grower_impl.hpp:838-846).
+__global__ void k_lw_finalize(TreeCtl *__restrict__ ctl, LwCtl *__restrict__ lw, Cand *__restrict__ cands,
+                              NodeInfo const *__restrict__ kids, SplitRec const *__restrict__ kid_splits,
+                              u32 *__restrict__ free_slots, DevNode *__restrict__ tree, GrowCfg cfg,
+                              double const *__restrict__ scales)
 {
-    u32 const n = lw->n_cand;
+    __shared__ u32 s_n;
+    if (threadIdx.x == 0) {
+        u32 const np = lw->n_push;
+        if (np) lw_push(lw, kids, kid_splits, np, false, cands, free_slots, tree, cfg, scales);
+        lw->n_push = 0;
+        s_n = lw->n_cand;
+    }
+    __syncthreads();
+    u32 const n = s_n;
     for (u32 k = threadIdx.x; k < n; k += blockDim.x) put_leaf(tree, cands[k].node.id, node_value(cands[k].node, cfg, scales));
     (void)ctl;
 }
@@ -853,6 +947,7 @@ __global__ __launch_bounds__(256) void k_lw_begin(TreeCtl *__restrict__ ctl, LwC
         ctl->max_depth_seen = 0;
         lw->n_cand = 0;
         lw->n_free = n_slots - 1;
+        lw->n_push = 0;
     }
 }
 
@@ -969,40 +1064,48 @@ void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
     NodeInfo *parent = v.front[1];
     SplitRec *psplit = s.feat_best;
     void *pool = v.hist[0];
+    // Quantized: count cells beside the histogram pool give every split its child row
+    // counts (the counted round); FP32: the chunk counts are scanned on the device.
+    u32 *cnt = q ? v.cnt : nullptr;
     u32 const pgrid = std::max(1u, std::min(v.max_pch, tuning().part_grid));
     u32 const hgrid = std::max(1u, std::min(v.max_hch, 4096u));
     hipLaunchKernelGGL(k_lw_begin, dim3(1), dim3(256), 0, st, v.ctl, lw, kids, v.tree, v.hch, v.fops, s.free_slots,
                        s.hist_slots, static_cast<u32>(v.N));
     launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
-                v.fsl, v.waves, hgrid, v.partials, pool);
-    launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1);
-    launch_find(st, q, kids, v.ctl, pool, v.d_nbins, v.F, v.SB, v.gc, v.scales, v.splits, 2, 0, 0, v.fscr);
-    hipLaunchKernelGGL(k_lw_push, dim3(1), dim3(64), 0, st, v.ctl, lw, kids, v.splits, s.cands, s.free_slots, v.tree,
-                       v.gc, v.scales, 1u);
+                v.fsl, v.waves, hgrid, v.partials, pool, cnt);
+    launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1, cnt, cnt);
+    launch_find(st, q, kids, v.ctl, pool, v.d_nbins, v.F, v.SB, v.gc, v.scales, v.splits, 2, 0, 0, v.fscr, cnt);
+    hipLaunchKernelGGL(k_lw_push_root, dim3(1), dim3(64), 0, st, lw, kids, v.splits, s.cands, s.free_slots, v.tree,
+                       v.gc, v.scales);
     u32 const rounds = static_cast<u32>(v.fc.max_leaves) - 1u + 8u;
     for (u32 r = 0; r < rounds; ++r) {
-        hipLaunchKernelGGL(k_lw_select, dim3(1), dim3(256), 0, st, v.ctl, lw, s.cands, parent, psplit, v.tree, v.ops,
-                           v.pch, v.d_nbins, v.gc);
-        launch_part_count(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.Fs, v.chunk_left, pgrid);
-        if (q)
-            hipLaunchKernelGGL(k_lw_scan<true>, dim3(1), dim3(1024), 0, st, v.ops, v.pch, v.chunk_left, v.ctl, lw,
-                               parent, psplit, s.free_slots, kids, v.tree, v.hch, v.fops, v.gc, v.scales);
-        else
+        if (q) {
+            hipLaunchKernelGGL(k_lw_select<true>, dim3(1), dim3(256), 0, st, v.ctl, lw, s.cands, kids, v.splits,
+                               s.free_slots, parent, psplit, v.tree, v.ops, v.pch, v.hch, v.d_nbins, v.fops, v.gc,
+                               v.scales, v.lw_units);
+            launch_part_count(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.Fs, v.chunk_left, pgrid);
+            launch_part_scatter(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1],
+                                v.gh[2], v.work[0], v.work[1], const_cast<void *>(v.gh[0]),
+                                const_cast<void *>(v.gh[1]), v.Fs, gh_bytes, pgrid, v.chunk_left);
+        } else {
+            hipLaunchKernelGGL(k_lw_select<false>, dim3(1), dim3(256), 0, st, v.ctl, lw, s.cands, kids, v.splits,
+                               s.free_slots, parent, psplit, v.tree, v.ops, v.pch, v.hch, v.d_nbins, v.fops, v.gc,
+                               v.scales, 0u);
+            launch_part_count(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.Fs, v.chunk_left, pgrid);
             hipLaunchKernelGGL(k_lw_scan<false>, dim3(1), dim3(1024), 0, st, v.ops, v.pch, v.chunk_left, v.ctl, lw,
                                parent, psplit, s.free_slots, kids, v.tree, v.hch, v.fops, v.gc, v.scales);
-        launch_part_scatter(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2],
-                            v.work[0], v.work[1], const_cast<void *>(v.gh[0]), const_cast<void *>(v.gh[1]), v.Fs,
-                            gh_bytes, pgrid);
-        hipLaunchKernelGGL(k_lw_units, dim3(8), dim3(256), 0, st, v.ctl, v.fops, v.hch);
+            launch_part_scatter(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1],
+                                v.gh[2], v.work[0], v.work[1], const_cast<void *>(v.gh[0]),
+                                const_cast<void *>(v.gh[1]), v.Fs, gh_bytes, pgrid);
+        }
         launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
-                    v.fsl, v.waves, hgrid, v.partials, pool);
-        launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1);
-        launch_find(st, q, kids, v.ctl, pool, v.d_nbins, v.F, v.SB, v.gc, v.scales, v.splits, 2, 0, 0, v.fscr);
-        hipLaunchKernelGGL(k_lw_push, dim3(1), dim3(64), 0, st, v.ctl, lw, kids, v.splits, s.cands, s.free_slots,
-                           v.tree, v.gc, v.scales, 0u);
+                    v.fsl, v.waves, hgrid, v.partials, pool, cnt);
+        launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1, cnt, cnt);
+        launch_find(st, q, kids, v.ctl, pool, v.d_nbins, v.F, v.SB, v.gc, v.scales, v.splits, 2, 0, 0, v.fscr, cnt);
         if (v.host_sync) (void)hipStreamSynchronize(st);
     }
-    hipLaunchKernelGGL(k_lw_finalize, dim3(1), dim3(256), 0, st, v.ctl, lw, s.cands, v.tree, v.gc, v.scales);
+    hipLaunchKernelGGL(k_lw_finalize, dim3(1), dim3(256), 0, st, v.ctl, lw, s.cands, kids, v.splits, s.free_slots,
+                       v.tree, v.gc, v.scales);
 }
 
 } // namespace bh

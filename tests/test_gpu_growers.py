@@ -59,6 +59,33 @@ def test_grow_leafwise_f32_matches_device_order_oracle(hip, orc, tall, leaves, d
     assert np.array_equal(lor, rlor)
 
 
+@pytest.fixture(scope="module")
+def tall600k():
+    return tall_data(600000, seed_cfg=3)
+
+
+# Large enough that the leaf-sized work units leave their minimum sizes: the root's
+# partition chunks are 768 rows, the first children's 512, and a quantized child of
+# more than 262,144 rows gets histogram units above 1024 rows, with many partials.
+@pytest.mark.parametrize("quantized", [True, False])
+@pytest.mark.parametrize("leaves,depth", [(15, 10), (63, 12)])
+def test_grow_leafwise_600k_rows(hip, orc, tall600k, quantized, leaves, depth):
+    bins, nb, X, y = tall600k
+    g, h = grad_of(orc, y, seed=7 + leaves)
+    ctx = ctx_for(hip, bins, nb, grower="hip_leafwise", quantized=quantized, max_depth=depth, max_leaves=leaves)
+    c = orc_cfg(max_depth=depth, max_leaves=leaves)
+    if quantized:
+        gq, hq, sc = ctx.set_gradients(g, h)
+        ref, rlor = orc.grow_leafwise(bins, nb, gq=gq, hq=hq, scales=sc, c=c, mode=HIST_INT)
+    else:
+        ctx.set_gradients(g, h)
+        ref, rlor = orc.grow_leafwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_CHUNKED)
+    nodes, lor = ctx.grow_tree()
+    ctx.close()
+    assert_same_tree(nodes, ref)
+    assert np.array_equal(lor, rlor)
+
+
 def levelwise_view(nodes, D):
     """Perfect dense tree -> (level splits, leaf values, leaf covers)."""
     levels = np.array([[nodes[(1 << k) - 1]["feature"], nodes[(1 << k) - 1]["bin"],
