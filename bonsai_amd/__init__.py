@@ -145,6 +145,8 @@ def lib() -> C.CDLL:
         "bh_train": [P, u32],
         "bh_set_validation": [P, P, u64, u64, P, C.c_float, u32],
         "bh_get_eval": [P, P, u32, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_int32)],
+        "bh_set_row_subset": [P, P, u64],
+        "bh_set_bernoulli": [P, C.c_float, u64],
         "bh_get_trees": [P, P, u32, P, u32, C.POINTER(u32)],
         "bh_max_nodes": [P, C.POINTER(u32)],
         "bh_set_gradients": [P, P, P, u32, P, P, P],
@@ -385,6 +387,21 @@ class HipContext:
             raise ConfigError("validation bins must be [n, F] with one label per row")
         _check(lib().bh_set_validation(self._h, _ptr(bins), bins.shape[0], bins.shape[1], _ptr(labels),
                                        float(init_score), int(early_stopping_rounds)))
+
+    def set_row_subset(self, rows) -> None:
+        """Grow every following tree on these rows only (ascending, unique, below n_rows;
+        the row_indices a booster's sampler hands to grow); scores still cover every
+        row.  None or an empty array: every row."""
+        if rows is None or len(rows) == 0:
+            _check(lib().bh_set_row_subset(self._h, None, 0))
+            return
+        rows = np.ascontiguousarray(rows, np.uint32)
+        _check(lib().bh_set_row_subset(self._h, _ptr(rows), rows.size))
+
+    def set_bernoulli(self, subsample: float, seed: int = 0) -> None:
+        """Device Bernoulli row sampler: every iteration keeps each row with probability
+        `subsample` (> 0; >= 1 keeps every row), drawn from (seed, iteration, row)."""
+        _check(lib().bh_set_bernoulli(self._h, float(subsample), int(seed) & ((1 << 64) - 1)))
 
     def eval_history(self) -> tuple[np.ndarray, int, bool]:
         """(losses so far, best round, stopped)."""

@@ -181,9 +181,19 @@ struct TreeCtl {
     u32 es_best_iter;   // index of the best (lowest) loss
     u32 es_stopped;     // 1 once es_n - 1 - es_best_iter reached the patience
     float es_best;      // best loss
-    u32 pad[10];
+    u32 root_n;         // rows of the root segment when a device sampler chose them
+    u32 pad[9];
 };
 static_assert(sizeof(TreeCtl) == 128, "TreeCtl");
+
+// Where a tree's root rows are: every row in the pristine plane, or a row subset (a
+// sampler's kept rows, ascending) gathered into work plane 1, with its count known on
+// the host or left in TreeCtl::root_n by the device sampler.
+struct RootSel {
+    u32 n;          // rows (when known on the host)
+    u32 src;        // kSrcPlane0 or 1
+    u32 n_from_ctl; // 1: the count is TreeCtl::root_n
+};
 
 // Launch-shape knobs of one context, picked by the autotuner (bh_autotune).
 struct Tuning {

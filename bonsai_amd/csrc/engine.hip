@@ -100,6 +100,13 @@ struct bh_context {
     // capacities
     u32 max_front = 0, max_ops = 0, max_pch = 0, max_hch = 0, max_parts = 0, hist_slots = 0;
     u32 lw_units = 0; // quantized leafwise: histogram units per child at most
+    // row sampling of the tree's root (bh_set_row_subset / bh_set_bernoulli)
+    u32 samp_kind = 0;           // 0 every row, 1 a fixed subset, 2 a Bernoulli draw per iteration
+    u32 *samp_rows = nullptr;    // kind 1: the subset, ascending
+    u32 samp_n = 0;
+    u32 samp_thr = 0;            // kind 2: keep when 32 random bits < thr
+    u64 samp_seed = 0;
+    u32 *samp_cnt = nullptr;     // kind 2: per-chunk kept counts, then their prefix
     // configuration
     bh_fit_config fc{};
     GrowCfg gc{};
@@ -364,6 +371,29 @@ void enqueue_build(bh_context *c, void *hist_cur, void *hist_next, u32 max_ops, 
 }
 
 u32 hist_grid(bh_context *c) { return std::max(1u, std::min(c->max_hch, 4096u)); }
+
+// The tree's root rows: the pristine plane, or the sampled rows in work plane 1
+// (SplitInput make_root over row_indices, src/step/level.hpp:171-187; the rows the
+// sampler dropped still get their scores from the tree walk over every row, the
+// reference's route_unsampled, src/grower_impl.hpp:430-482).
+RootSel root_sel(bh_context *c)
+{
+    if (c->samp_kind == 1) return RootSel{c->samp_n, 1u, 0u};
+    if (c->samp_kind == 2) return RootSel{0u, 1u, 1u};
+    return RootSel{static_cast<u32>(c->N), kSrcPlane0, 0u};
+}
+
+// Gathers this iteration's sampled rows (and their gradient words) into work plane 1.
+void enqueue_sample(bh_context *c)
+{
+    u32 const gh_bytes = c->quant ? 4 : 8;
+    if (c->samp_kind == 1)
+        launch_gather_rows(c->stream, c->plane0, c->Fs, gh_src(c, 2), gh_bytes, c->samp_rows, c->samp_n, c->work[1],
+                           const_cast<void *>(gh_src(c, 1)), 0);
+    else if (c->samp_kind == 2)
+        launch_bernoulli(c->stream, c->ctl, static_cast<u32>(c->N), c->samp_seed, c->samp_thr, c->samp_cnt, c->plane0,
+                         c->Fs, gh_src(c, 2), gh_bytes, c->work[1], const_cast<void *>(gh_src(c, 1)));
+}
 u32 part_grid(bh_context *c) { return std::max(1u, std::min(c->max_pch, c->tune.part_grid)); }
 
 // Quantized depthwise: the histograms carry row counts, so each level is find (child
@@ -383,7 +413,7 @@ void enqueue_depthwise_counted(bh_context *c)
     u32 moves = 0; // partitions so far: they alternate between the two work planes
     u32 const unit_rows = c->unit_rows();
     bool const group = unit_rows > kHistChunk;
-    launch_tree_begin(s, c->ctl, c->front[0], c->tree, c->hch, c->fops, static_cast<u32>(c->N), unit_rows);
+    launch_tree_begin(s, c->ctl, c->front[0], c->tree, c->hch, c->fops, root_sel(c), unit_rows);
     enqueue_build(c, c->hist[1], c->hist[0], 1, hist_grid(c), c->cnt[1], c->cnt[0], false, nullptr, group);
     for (u32 d = 0; d < D; ++d) {
         NodeInfo *cur = c->front[d & 1u], *nxt = c->front[(d + 1) & 1u];
@@ -439,7 +469,7 @@ void enqueue_depthwise(bh_context *c)
     hipStream_t s = c->stream;
     u32 const D = static_cast<u32>(c->fc.max_depth);
     u32 const gh_bytes = c->quant ? 4 : 8;
-    launch_tree_begin(s, c->ctl, c->front[0], c->tree, c->hch, c->fops, static_cast<u32>(c->N));
+    launch_tree_begin(s, c->ctl, c->front[0], c->tree, c->hch, c->fops, root_sel(c));
     enqueue_build(c, c->hist[1], c->hist[0], 1, hist_grid(c));
     for (u32 d = 0; d < D; ++d) {
         NodeInfo *cur = c->front[d & 1u], *nxt = c->front[(d + 1) & 1u];
@@ -470,6 +500,7 @@ void enqueue_depthwise(bh_context *c)
 
 void enqueue_grow(bh_context *c)
 {
+    enqueue_sample(c);
     if (c->fc.grower == BH_DEPTHWISE) enqueue_depthwise(c);
     else if (c->fc.grower == BH_LEVELWISE) enqueue_levelwise(c->ext, c->stream, c->quant, grow_view(c));
     else enqueue_leafwise(c->ext, c->stream, c->quant, grow_view(c));
@@ -566,6 +597,7 @@ GrowView grow_view(bh_context *c)
     v.max_ops = c->max_ops;
     v.cnt = c->cnt[0];
     v.lw_units = c->lw_units;
+    v.root = root_sel(c);
     v.fsl = c->hist_fsl();
     v.waves = c->hist_waves();
     v.host_sync = c->fc.host_sync != 0;
@@ -661,7 +693,7 @@ int bh_destroy(bh_context *c)
     hipStreamSynchronize(c->stream);
     free_fit(c);
     void *ptrs[] = {c->plane0, c->work[0], c->work[1], c->d_nbins, c->label, c->score,
-                    c->vplane, c->vlabel, c->vscore, c->vhist, c->vpartial};
+                    c->vplane, c->vlabel, c->vscore, c->vhist, c->vpartial, c->samp_rows, c->samp_cnt};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -685,10 +717,12 @@ static int alloc_plane(bh_context *c, uint64_t n_rows, uint32_t n_features, cons
     HIP_TRY(hipSetDevice(c->device));
     hipStreamSynchronize(c->stream);
     free_fit(c);
-    void *ptrs[] = {c->plane0, c->work[0], c->work[1], c->d_nbins, c->label, c->score};
+    void *ptrs[] = {c->plane0, c->work[0], c->work[1], c->d_nbins, c->label, c->score, c->samp_rows, c->samp_cnt};
     for (void *p : ptrs)
         if (p) hipFree(p);
     c->plane0 = c->work[0] = c->work[1] = nullptr;
+    c->samp_rows = c->samp_cnt = nullptr;
+    c->samp_kind = c->samp_n = 0;
     c->d_nbins = nullptr;
     c->label = c->score = nullptr;
     c->have_labels = false;
@@ -880,6 +914,53 @@ int bh_get_eval(bh_context *c, float *history, uint32_t cap, uint32_t *n, uint32
     if (n) *n = k;
     if (best_iter) *best_iter = t.es_best_iter;
     if (stopped) *stopped = static_cast<int32_t>(t.es_stopped);
+    return 0;
+}
+
+int bh_set_row_subset(bh_context *c, const uint32_t *rows, uint64_t n)
+{
+    if (!c) return fail(1, "null context");
+    if (!c->plane0) return fail(1, "no bins uploaded");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->samp_rows) (void)hipFree(c->samp_rows);
+    c->samp_rows = nullptr;
+    c->samp_kind = c->samp_n = 0;
+    drop_graph(c); // the iteration graph gains or loses the gather
+    if (!rows || n == 0 || n == c->N) {
+        if (rows && n == c->N)
+            for (u64 i = 0; i < n; ++i)
+                if (rows[i] != i) return fail(1, "row subset must be ascending, unique and below n_rows");
+        return 0; // every row: the plain root
+    }
+    if (n > c->N) return fail(1, "row subset larger than the plane");
+    for (u64 i = 0; i < n; ++i)
+        if (rows[i] >= c->N || (i > 0 && rows[i] <= rows[i - 1]))
+            return fail(1, "row subset must be ascending, unique and below n_rows");
+    HIP_TRY(dalloc(&c->samp_rows, n));
+    HIP_TRY(hipMemcpy(c->samp_rows, rows, n * sizeof(u32), hipMemcpyHostToDevice));
+    c->samp_n = static_cast<u32>(n);
+    c->samp_kind = 1;
+    return 0;
+}
+
+int bh_set_bernoulli(bh_context *c, float subsample, uint64_t seed)
+{
+    if (!c) return fail(1, "null context");
+    if (!c->plane0) return fail(1, "no bins uploaded");
+    if (!(subsample > 0.0F)) return fail(1, "subsample must be > 0");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->samp_rows) (void)hipFree(c->samp_rows);
+    c->samp_rows = nullptr;
+    c->samp_kind = c->samp_n = 0;
+    drop_graph(c);
+    if (subsample >= 1.0F) return 0; // the reference keeps every row for p >= 1
+    if (!c->samp_cnt) HIP_TRY(dalloc(&c->samp_cnt, c->N / kPartChunk + 2));
+    double const t = std::floor(static_cast<double>(subsample) * 4294967296.0);
+    c->samp_thr = static_cast<u32>(std::min(t, 4294967295.0));
+    c->samp_seed = seed;
+    c->samp_kind = 2;
     return 0;
 }
 
@@ -1360,7 +1441,8 @@ int bh_time_root_histogram(bh_context *c, uint32_t reps, float *ms_mean)
     if (int r = check_ready(c)) return r;
     HIP_TRY(hipSetDevice(c->device));
     // Root units over the pristine plane and the current gradients.
-    launch_tree_begin(c->stream, c->ctl, c->front[0], c->tree, c->hch, c->fops, static_cast<u32>(c->N));
+    launch_tree_begin(c->stream, c->ctl, c->front[0], c->tree, c->hch, c->fops,
+                      RootSel{static_cast<u32>(c->N), kSrcPlane0, 0u});
     HIP_TRY(hipStreamSynchronize(c->stream));
     float total = 0.0F;
     for (u32 r = 0; r < reps; ++r) {

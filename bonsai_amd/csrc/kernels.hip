@@ -2502,14 +2502,16 @@ __global__ void k_next_level(TreeCtl *ctl)
 // Tree start: root frontier, root histogram units over the pristine plane.
 __global__ __launch_bounds__(256) void k_tree_begin(TreeCtl *__restrict__ ctl, NodeInfo *__restrict__ front,
                                                     DevNode *__restrict__ tree, HistChunk *__restrict__ hch,
-                                                    FinishOp *__restrict__ fops, u32 n_rows, u32 unit_rows)
+                                                    FinishOp *__restrict__ fops, u32 n_rows_host, u32 unit_rows,
+                                                    u32 src, u32 n_from_ctl)
 {
+    u32 const n_rows = n_from_ctl ? ctl->root_n : n_rows_host;
     u32 const n_units = (n_rows + unit_rows - 1) / unit_rows;
     u32 const multi = n_units > 1;
     for (u32 k = threadIdx.x; k < n_units; k += blockDim.x) {
         HistChunk hc{};
         hc.slot = 0;
-        hc.src = kSrcPlane0;
+        hc.src = src;
         hc.begin = k * unit_rows;
         hc.end = min(n_rows, (k + 1) * unit_rows);
         hc.part = multi ? k : kNone;
@@ -2521,7 +2523,7 @@ __global__ __launch_bounds__(256) void k_tree_begin(TreeCtl *__restrict__ ctl, N
         root.id = 0;
         root.off = 0;
         root.cnt = n_rows;
-        root.src = kSrcPlane0;
+        root.src = src;
         root.slot = 0;
         front[0] = root;
         DevNode r{};
@@ -2693,6 +2695,110 @@ __global__ void k_copy_tree(DevNode const *__restrict__ tree, TreeCtl const *__r
 
 // Gathers listed rows of the pristine plane (and their gradient pairs, or the row id
 // itself as the pair) into a contiguous segment: the entry to the kernel-level tests.
+// ---------------------------------------------------------------------------------
+// Device Bernoulli row sampler on the resident path (BernoulliSampler::sample,
+// src/sampler.cpp:38-55; eligible there, include/bonsai/booster.hpp:632-635): row r of
+// iteration t is kept when the top 32 bits of a counter-based draw (splitmix64 of the
+// seed, t, r and channel 2; channels 0 and 1 are the gradient roundings) fall below
+// thr = floor(p * 2^32).  The kept rows, ascending, are gathered into work plane 1 as
+// the tree's root segment; the score update still walks every row.  (The reference
+// draws with std::mt19937; its exact stream is reproduced by handing the indices over
+// instead, bh_set_row_subset.)
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ bool bern_keep(u64 seed, u32 iter, u32 row, u32 thr)
+{
+    u64 const key = seed ^ (static_cast<u64>(iter) * 0xD1B54A32D192ED03ULL) ^
+                    (static_cast<u64>(row) * 0x9E3779B97F4A7C15ULL) ^ (3ULL * 0xA0761D6478BD642FULL);
+    return static_cast<u32>(splitmix64(key) >> 32) < thr;
+}
+
+// Kept rows per kPartChunk-row chunk.
+__global__ __launch_bounds__(256) void k_bern_count(TreeCtl const *__restrict__ ctl, u32 n, u64 seed, u32 thr,
+                                                    u32 *__restrict__ chunk_cnt)
+{
+    __shared__ u32 red[4];
+    u32 const iter = ctl->iter;
+    u32 const n_ch = (n + kPartChunk - 1) / kPartChunk;
+    for (u32 c = blockIdx.x; c < n_ch; c += gridDim.x) {
+        u32 const r0 = c * kPartChunk, r1 = min(n, r0 + kPartChunk);
+        u32 k = 0;
+        for (u32 r = r0 + threadIdx.x; r < r1; r += blockDim.x) k += bern_keep(seed, iter, r, thr) ? 1u : 0u;
+        for (int o = 32; o > 0; o >>= 1) k += __shfl_xor(k, o);
+        if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = k;
+        __syncthreads();
+        if (threadIdx.x == 0) chunk_cnt[c] = red[0] + red[1] + red[2] + red[3];
+        __syncthreads();
+    }
+}
+
+// Exclusive prefix of the chunk counts in place; the total is the root's row count.
+__global__ __launch_bounds__(1024) void k_bern_scan(u32 *__restrict__ chunk_cnt, u32 n_ch, TreeCtl *__restrict__ ctl)
+{
+    __shared__ u32 scratch[1024];
+    __shared__ u32 s_run;
+    if (threadIdx.x == 0) s_run = 0;
+    __syncthreads();
+    for (u32 t0 = 0; t0 < n_ch; t0 += blockDim.x) {
+        u32 const c = t0 + threadIdx.x;
+        u32 const v = c < n_ch ? chunk_cnt[c] : 0u;
+        u32 tot;
+        u32 const pre = block_exscan(v, scratch, &tot) + s_run;
+        if (c < n_ch) chunk_cnt[c] = pre;
+        __syncthreads();
+        if (threadIdx.x == 0) s_run += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ctl->root_n = s_run;
+}
+
+// Stable compaction of the kept rows (and their gradient words) into out_plane/out_gh.
+__global__ __launch_bounds__(256) void k_bern_gather(TreeCtl const *__restrict__ ctl, u32 n, u64 seed, u32 thr,
+                                                     u32 const *__restrict__ chunk_off, u8 const *__restrict__ plane,
+                                                     u32 Fs, void const *__restrict__ gh, u32 gh_bytes,
+                                                     u8 *__restrict__ out_plane, void *__restrict__ out_gh)
+{
+    __shared__ u32 s_w[4];
+    u32 const iter = ctl->iter;
+    u32 const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    u32 const D = Fs / 4u;
+    u32 const n_ch = (n + kPartChunk - 1) / kPartChunk;
+    for (u32 c = blockIdx.x; c < n_ch; c += gridDim.x) {
+        u32 const r0 = c * kPartChunk, r1 = min(n, r0 + kPartChunk);
+        u32 run = chunk_off[c];
+        for (u32 p0 = r0; p0 < r1; p0 += blockDim.x) {
+            u32 const r = p0 + threadIdx.x;
+            bool const keep = r < r1 && bern_keep(seed, iter, r, thr);
+            u64 const bal = __ballot(keep);
+            if (lane == 0) s_w[wave] = static_cast<u32>(__popcll(bal));
+            __syncthreads();
+            u32 pre = run;
+            for (u32 k = 0; k < wave; ++k) pre += s_w[k];
+            u32 const tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+            __syncthreads();
+            if (keep) {
+                size_t const dst = pre + static_cast<u32>(__popcll(bal & ((1ull << lane) - 1ull)));
+                u32 const *src = reinterpret_cast<u32 const *>(plane + static_cast<size_t>(r) * Fs);
+                u32 *o = reinterpret_cast<u32 *>(out_plane + dst * Fs);
+                for (u32 d = 0; d < D; ++d) o[d] = src[d];
+                if (gh_bytes == 8) reinterpret_cast<uint2 *>(out_gh)[dst] = reinterpret_cast<uint2 const *>(gh)[r];
+                else reinterpret_cast<u32 *>(out_gh)[dst] = reinterpret_cast<u32 const *>(gh)[r];
+            }
+            run += tot;
+        }
+    }
+}
+
+void launch_bernoulli(hipStream_t s, TreeCtl *ctl, u32 n, u64 seed, u32 thr, u32 *chunk_cnt, u8 const *plane,
+                      u32 Fs, void const *gh, u32 gh_bytes, u8 *out_plane, void *out_gh)
+{
+    u32 const n_ch = (n + kPartChunk - 1) / kPartChunk;
+    u32 const grid = std::max(1u, std::min(n_ch, 2048u));
+    hipLaunchKernelGGL(k_bern_count, dim3(grid), dim3(256), 0, s, ctl, n, seed, thr, chunk_cnt);
+    hipLaunchKernelGGL(k_bern_scan, dim3(1), dim3(1024), 0, s, chunk_cnt, n_ch, ctl);
+    hipLaunchKernelGGL(k_bern_gather, dim3(grid), dim3(256), 0, s, ctl, n, seed, thr, chunk_cnt, plane, Fs, gh,
+                       gh_bytes, out_plane, out_gh);
+}
+
 __global__ void k_gather_rows(u8 const *__restrict__ plane, u32 Fs, void const *__restrict__ gh, u32 gh_bytes,
                               u32 const *__restrict__ rows, u64 n, u8 *__restrict__ out_plane,
                               void *__restrict__ out_gh, int gh_is_row_id)
@@ -2889,6 +2995,8 @@ hipError_t set_hist_lds_limits()
     raise_lds(reinterpret_cast<void const *>(&k_hist<false, 4>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_find<true>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_find<false>), max_shared);
+    raise_lds(reinterpret_cast<void const *>(&k_part_scatter4), max_shared);
+    raise_lds(reinterpret_cast<void const *>(&k_part_scatter_v), max_shared);
     return hipSuccess;
 }
 
@@ -3039,9 +3147,10 @@ void launch_next_level(hipStream_t s, TreeCtl *ctl)
 }
 
 void launch_tree_begin(hipStream_t s, TreeCtl *ctl, NodeInfo *front, DevNode *tree, HistChunk *hch,
-                       FinishOp *fops, u32 n_rows, u32 unit_rows)
+                       FinishOp *fops, RootSel const &root, u32 unit_rows)
 {
-    hipLaunchKernelGGL(k_tree_begin, dim3(1), dim3(256), 0, s, ctl, front, tree, hch, fops, n_rows, unit_rows);
+    hipLaunchKernelGGL(k_tree_begin, dim3(1), dim3(256), 0, s, ctl, front, tree, hch, fops, root.n, unit_rows,
+                       root.src, root.n_from_ctl);
 }
 
 void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode const *tree, TreeCtl const *ctl,

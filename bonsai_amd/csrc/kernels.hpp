@@ -57,7 +57,12 @@ void launch_eval(hipStream_t s, int objective, float const *scores, float const 
                  TreeCtl *ctl, float *hist, u32 hist_cap, u32 patience);
 // unit_rows: rows per root histogram unit (kHistChunk, or a group of sub-units).
 void launch_tree_begin(hipStream_t s, TreeCtl *ctl, NodeInfo *front, DevNode *tree, HistChunk *hch,
-                       FinishOp *fops, u32 n_rows, u32 unit_rows = kHistChunk);
+                       FinishOp *fops, RootSel const &root, u32 unit_rows = kHistChunk);
+// Device Bernoulli row sampler: the rows of [0, n) kept in this iteration (threshold
+// thr on 32 random bits), ascending, with their gradient words, into out_plane/out_gh;
+// their count into ctl->root_n.  chunk_cnt: n / kPartChunk + 1 words of scratch.
+void launch_bernoulli(hipStream_t s, TreeCtl *ctl, u32 n, u64 seed, u32 thr, u32 *chunk_cnt, u8 const *plane,
+                      u32 Fs, void const *gh, u32 gh_bytes, u8 *out_plane, void *out_gh);
 void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode const *tree, TreeCtl const *ctl,
                          u16 const *n_bins, u32 F, float lr, float *score, u32 *leaf_of_row, u32 max_nodes);
 void launch_iter_advance(hipStream_t s, TreeCtl *ctl);

@@ -906,14 +906,15 @@ __global__ void k_lw_finalize(TreeCtl *__restrict__ ctl, LwCtl *__restrict__ lw,
 __global__ __launch_bounds__(256) void k_lw_begin(TreeCtl *__restrict__ ctl, LwCtl *__restrict__ lw,
                                                   NodeInfo *__restrict__ kids, DevNode *__restrict__ tree,
                                                   HistChunk *__restrict__ hch, FinishOp *__restrict__ fops,
-                                                  u32 *__restrict__ free_slots, u32 n_slots, u32 n_rows)
+                                                  u32 *__restrict__ free_slots, u32 n_slots, RootSel root)
 {
+    u32 const n_rows = root.n_from_ctl ? ctl->root_n : root.n;
     u32 const n_units = (n_rows + kHistChunk - 1) / kHistChunk;
     u32 const multi = n_units > 1;
     for (u32 k = threadIdx.x; k < n_units; k += blockDim.x) {
         HistChunk hc{};
         hc.slot = 0;
-        hc.src = kSrcPlane0;
+        hc.src = root.src;
         hc.begin = k * kHistChunk;
         hc.end = min(n_rows, (k + 1) * kHistChunk);
         hc.part = multi ? k : kNone;
@@ -921,11 +922,11 @@ __global__ __launch_bounds__(256) void k_lw_begin(TreeCtl *__restrict__ ctl, LwC
     }
     for (u32 k = threadIdx.x; k + 1 < n_slots; k += blockDim.x) free_slots[k] = n_slots - 1 - k;
     if (threadIdx.x == 0) {
-        NodeInfo root{};
-        root.cnt = n_rows;
-        root.src = kSrcPlane0;
-        root.slot = 0;
-        kids[0] = root;
+        NodeInfo rn{};
+        rn.cnt = n_rows;
+        rn.src = root.src;
+        rn.slot = 0;
+        kids[0] = rn;
         DevNode r{};
         r.feature = -1;
         r.cover = static_cast<float>(n_rows);
@@ -1000,7 +1001,7 @@ void enqueue_levelwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
     u32 const D = static_cast<u32>(v.fc.max_depth);
     u32 const gh_bytes = q ? 4 : 8;
     SplitRec *pick = s.feat_best + v.F; // one spare record after the per-feature bests
-    launch_tree_begin(st, v.ctl, v.front[0], v.tree, v.hch, v.fops, static_cast<u32>(v.N));
+    launch_tree_begin(st, v.ctl, v.front[0], v.tree, v.hch, v.fops, v.root);
     hipLaunchKernelGGL(k_obl_begin, dim3(1), dim3(64), 0, st, v.ctl);
     launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
                 v.fsl, v.waves, std::max(1u, std::min(v.max_hch, 4096u)), v.partials, v.hist[0]);
@@ -1070,7 +1071,7 @@ void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
     u32 const pgrid = std::max(1u, std::min(v.max_pch, tuning().part_grid));
     u32 const hgrid = std::max(1u, std::min(v.max_hch, 4096u));
     hipLaunchKernelGGL(k_lw_begin, dim3(1), dim3(256), 0, st, v.ctl, lw, kids, v.tree, v.hch, v.fops, s.free_slots,
-                       s.hist_slots, static_cast<u32>(v.N));
+                       s.hist_slots, v.root);
     launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
                 v.fsl, v.waves, hgrid, v.partials, pool, cnt);
     launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1, cnt, cnt);

@@ -135,6 +135,21 @@ int bh_set_validation(bh_context *ctx, const uint8_t *bins, uint64_t n_rows, uin
 /* Losses recorded so far (up to cap), their count, the best round and the stop flag. */
 int bh_get_eval(bh_context *ctx, float *history, uint32_t cap, uint32_t *n, uint32_t *best_iter,
                 int32_t *stopped);
+/* Row subset of every following tree (the row_indices the booster's sampler hands to
+ * grow, include/bonsai/booster.hpp:666-676; the root over them, SplitInput make_root,
+ * src/step/level.hpp:171-187): rows ascending, unique, below n_rows.  The tree is grown
+ * on those rows only; every row's score (and bh_grow_tree's leaf_of_row) still comes
+ * from the finished tree (route_unsampled, src/grower_impl.hpp:430-482).  rows NULL,
+ * n 0 or all n_rows rows: no subset.  Replaces a Bernoulli sampler. */
+int bh_set_row_subset(bh_context *ctx, const uint32_t *rows, uint64_t n);
+/* Bernoulli row sampler on the device, drawn afresh every iteration (BernoulliSampler,
+ * src/sampler.cpp:27-55; allowed on the resident path, include/bonsai/booster.hpp:632-635):
+ * row r of iteration t is kept when 32 counter-based random bits (seed, t, r) fall below
+ * floor(subsample * 2^32).  subsample must be > 0 (the reference's ConfigError);
+ * subsample >= 1 keeps every row.  The draw is this library's own stream, not the
+ * reference's std::mt19937 one (bh_set_row_subset takes that one's indices).  Replaces
+ * a row subset. */
+int bh_set_bernoulli(bh_context *ctx, float subsample, uint64_t seed);
 /* Trees grown so far: nodes[t * max_nodes + i], sizes[t]; returns via *n_trees. */
 int bh_get_trees(bh_context *ctx, bh_node *nodes, uint32_t max_nodes, uint32_t *sizes,
                  uint32_t cap_trees, uint32_t *n_trees);

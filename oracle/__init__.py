@@ -59,6 +59,7 @@ class Oracle:
         L.orc_init_score.restype = C.c_float
         L.orc_logloss_eval.restype = C.c_float
         L.orc_leaf_value.restype = C.c_float
+        L.orc_bernoulli_rows.restype = C.c_uint64
         L.orc_leaf_value.argtypes = [C.c_double, C.c_double, C.c_void_p]
         L.orc_partition.restype = C.c_uint64
         L.orc_grow_depthwise.restype = C.c_int32
@@ -121,6 +122,16 @@ class Oracle:
         return gq, hq, np.array([np.float64(mg.value) / 127.0, np.float64(mh.value) / 32767.0])
 
     # -- histograms --
+    def bernoulli_rows(self, n, subsample, seed, it):
+        """Rows kept by the Bernoulli sampler at iteration `it` (ascending)."""
+        if subsample >= 1.0:
+            return np.arange(n, dtype=np.uint32)
+        thr = min(int(np.floor(np.float64(np.float32(subsample)) * 4294967296.0)), 4294967295)
+        out = np.zeros(max(n, 1), np.uint32)
+        k = self.L.orc_bernoulli_rows(C.c_uint64(n), C.c_uint32(thr), C.c_uint64(seed & ((1 << 64) - 1)),
+                                      C.c_uint32(it), P(out))
+        return out[:k].copy()
+
     def hist_f64(self, bins, rows, g, h, bin_stride):
         n, f = bins.shape
         out = np.zeros((f, bin_stride, 2), np.float64)

@@ -1278,6 +1278,23 @@ void orc_quantize(const float *g, const float *h, uint64_t n, uint64_t seed, uin
     *max_h = mh;
 }
 
+// BernoulliSampler::sample (src/sampler.cpp:38-55): keep each row with probability p,
+// ascending indices out; p >= 1 keeps every row.  The draw here is the counter-based one
+// the product's device sampler uses (32 bits of splitmix64 over seed, iteration, row and
+// channel 2, kept when below thr = floor(p * 2^32)), not the reference's std::mt19937
+// stream.  Pinned by tests/test_oracle_sampler.py (p >= 1 identity, kept fraction
+// within binomial bounds, independence of iterations and seeds).
+uint64_t orc_bernoulli_rows(uint64_t n, uint32_t thr, uint64_t seed, uint32_t iter, uint32_t *out)
+{
+    u64 k = 0;
+    for (u64 r = 0; r < n; ++r) {
+        u64 const key = seed ^ (static_cast<u64>(iter) * 0xD1B54A32D192ED03ULL) ^ (r * 0x9E3779B97F4A7C15ULL) ^
+                        (3ULL * 0xA0761D6478BD642FULL);
+        if (static_cast<u32>(splitmix64(key) >> 32) < thr) out[k++] = static_cast<u32>(r);
+    }
+    return k;
+}
+
 // Brute-force double histogram: the FP64 reference the float paths are measured
 // against.
 void orc_hist_f64(const uint8_t *plane, uint64_t row_stride, uint32_t n_features,

@@ -87,6 +87,10 @@ float orc_logloss_eval(const float *score, const float *label, uint64_t n);
 /* Stochastic-rounding quantizer (int8 g, int16 h, per-call max-abs scale). */
 void orc_quantize(const float *g, const float *h, uint64_t n, uint64_t seed,
                   uint32_t iter, int8_t *gq, int16_t *hq, float *max_g, float *max_h);
+/* Bernoulli row sampler (BernoulliSampler::sample, src/sampler.cpp:38-55) with the
+ * product's counter-based draw: rows r of [0, n) whose 32 random bits of (seed, iter, r)
+ * are below thr, ascending, into out; returns their count. */
+uint64_t orc_bernoulli_rows(uint64_t n, uint32_t thr, uint64_t seed, uint32_t iter, uint32_t *out);
 
 /* ---- histograms over a row list ---- */
 void orc_hist_f64(const uint8_t *plane, uint64_t row_stride, uint32_t n_features,

@@ -1,0 +1,171 @@
+"""Sampled roots on the GPU: a tree grown on a row subset (the row_indices a booster's
+sampler hands to grow, include/bonsai/booster.hpp:666-676; the root over them,
+src/step/level.hpp:171-187) equals the oracle's tree grown on exactly those rows, and
+every row, sampled or not, gets its leaf from the finished tree (route_unsampled,
+src/grower_impl.hpp:430-482).  The device Bernoulli sampler keeps the rows the oracle's
+restatement keeps.
+"""
+import numpy as np
+import pytest
+
+from oracle_lib import HIST_F32_CHUNKED, HIST_INT, cfg as orc_cfg
+from test_gpu_depthwise import assert_same_tree, ctx_for, grad_of, tall_data
+
+pytestmark = pytest.mark.gpu
+
+GROWERS = {"hip_depthwise": "grow_depthwise", "hip_leafwise": "grow_leafwise", "hip_levelwise": "grow_levelwise"}
+
+
+@pytest.fixture(scope="module")
+def tall():
+    return tall_data(50000, seed_cfg=4)
+
+
+def walk(nodes, bins, nb):
+    """Leaf of every row by an independent bin-space walk (bin <= split goes left, the
+    missing bin goes the default way)."""
+    idx = np.zeros(bins.shape[0], np.int64)
+    for _ in range(64):
+        feat = nodes["feature"][idx]
+        inner = feat >= 0
+        if not inner.any():
+            break
+        r = np.nonzero(inner)[0]
+        f = feat[r]
+        b = bins[r, f].astype(np.int64)
+        last = nb[f].astype(np.int64) - 1
+        left = np.where(b == last, nodes["default_left"][idx[r]] != 0, b <= nodes["bin"][idx[r]])
+        idx[r] = np.where(left, nodes["left"][idx[r]], nodes["right"][idx[r]])
+    return idx
+
+
+def oracle_tree(orc, grower, bins, nb, rows, g, h, gq, hq, sc, quantized, c):
+    fn = getattr(orc, GROWERS[grower])
+    if quantized:
+        return fn(bins[rows], nb, gq=gq[rows], hq=hq[rows], scales=sc, c=c, mode=HIST_INT)
+    return fn(bins[rows], nb, g=g[rows], h=h[rows], c=c, mode=HIST_F32_CHUNKED)
+
+
+def grow_cfg(grower):
+    if grower == "hip_leafwise":
+        return dict(max_depth=10, max_leaves=31)
+    return dict(max_depth=6)
+
+
+def check_levelwise(nodes, ref):
+    # the oblivious grower returns levels, not a node array: compare the dense tree
+    levels, lv, lc, gains, rlor = ref
+    D = levels.shape[0]
+    assert len(nodes) == (2 << D) - 1
+    for k in range(D):
+        n0 = nodes[(1 << k) - 1]
+        assert (n0["feature"], n0["bin"], n0["default_left"]) == tuple(int(x) for x in levels[k])
+    leaves = nodes[(1 << D) - 1:]
+    assert np.array_equal(leaves["value"].view(np.uint32), lv.view(np.uint32))
+    assert np.array_equal(leaves["cover"], lc)
+
+
+@pytest.mark.parametrize("quantized", [True, False])
+@pytest.mark.parametrize("grower", list(GROWERS))
+def test_row_subset_tree_and_leaves(hip, orc, tall, grower, quantized):
+    bins, nb, X, y = tall
+    n = bins.shape[0]
+    rng = np.random.default_rng(21)
+    rows = np.sort(rng.choice(n, size=int(0.55 * n), replace=False)).astype(np.uint32)
+    g, h = grad_of(orc, y, seed=3)
+    kw = grow_cfg(grower)
+    ctx = ctx_for(hip, bins, nb, grower=grower, quantized=quantized, **kw)
+    gq = hq = sc = None
+    if quantized:
+        gq, hq, sc = ctx.set_gradients(g, h)
+    else:
+        ctx.set_gradients(g, h)
+    ctx.set_row_subset(rows)
+    nodes, lor = ctx.grow_tree()
+    ctx.close()
+    ref = oracle_tree(orc, grower, bins, nb, rows, g, h, gq, hq, sc, quantized, orc_cfg(**kw))
+    if grower == "hip_levelwise":
+        check_levelwise(nodes, ref)
+    else:
+        assert_same_tree(nodes, ref[0])
+        assert np.array_equal(lor[rows], ref[1])
+    assert nodes["cover"][0] == rows.size
+    assert np.array_equal(lor, walk(nodes, bins, nb))
+
+
+def test_row_subset_of_every_row_is_the_plain_root(hip, orc, tall):
+    bins, nb, X, y = tall
+    g, h = grad_of(orc, y, seed=5)
+    out = []
+    for subset in (None, np.arange(bins.shape[0], dtype=np.uint32)):
+        ctx = ctx_for(hip, bins, nb, grower="hip_depthwise", quantized=True, max_depth=5)
+        ctx.set_gradients(g, h)
+        ctx.set_row_subset(subset)
+        out.append(ctx.grow_tree())
+        ctx.close()
+    assert np.array_equal(out[0][0].view(np.uint8), out[1][0].view(np.uint8))
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_row_subset_rejects_bad_rows(hip, tall):
+    bins, nb, X, y = tall
+    ctx = ctx_for(hip, bins, nb, grower="hip_depthwise", quantized=True, max_depth=3)
+    for bad in ([5, 3], [1, 1], [bins.shape[0]]):
+        with pytest.raises(hip.ConfigError):
+            ctx.set_row_subset(np.array(bad, np.uint32))
+    with pytest.raises(hip.ConfigError):
+        ctx.set_bernoulli(0.0)
+    ctx.close()
+
+
+@pytest.mark.parametrize("quantized", [True, False])
+@pytest.mark.parametrize("grower", list(GROWERS))
+def test_bernoulli_keeps_the_oracles_rows(hip, orc, tall, grower, quantized):
+    bins, nb, X, y = tall
+    n = bins.shape[0]
+    g, h = grad_of(orc, y, seed=9)
+    kw = grow_cfg(grower)
+    ctx = ctx_for(hip, bins, nb, grower=grower, quantized=quantized, **kw)
+    gq = hq = sc = None
+    if quantized:
+        gq, hq, sc = ctx.set_gradients(g, h)
+    else:
+        ctx.set_gradients(g, h)
+    ctx.set_bernoulli(0.35, seed=77)
+    nodes, lor = ctx.grow_tree()  # a fresh fit: iteration 0 keys the draw
+    ctx.close()
+    rows = orc.bernoulli_rows(n, 0.35, 77, 0)
+    ref = oracle_tree(orc, grower, bins, nb, rows, g, h, gq, hq, sc, quantized, orc_cfg(**kw))
+    if grower == "hip_levelwise":
+        check_levelwise(nodes, ref)
+    else:
+        assert_same_tree(nodes, ref[0])
+        assert np.array_equal(lor[rows], ref[1])
+    assert np.array_equal(lor, walk(nodes, bins, nb))
+
+
+@pytest.mark.parametrize("sampler", ["rows", "bernoulli"])
+def test_sampled_training_graph_equals_eager(hip, orc, tall, sampler):
+    bins, nb, X, y = tall
+    res = []
+    for use_graph in (True, False):
+        ctx = ctx_for(hip, bins, nb, grower="hip_depthwise", quantized=True, max_depth=6, use_graph=use_graph,
+                      learning_rate=0.1)
+        ctx.upload_labels(y)
+        ctx.set_scores(np.full(y.size, orc.init_score(1, y), np.float32))
+        if sampler == "rows":
+            ctx.set_row_subset(np.arange(0, y.size, 3, dtype=np.uint32))
+        else:
+            ctx.set_bernoulli(0.5, seed=4)
+        ctx.train(4)
+        res.append((ctx.trees(), ctx.get_scores()))
+        ctx.close()
+    (ta, sa), (tb, sb) = res
+    assert len(ta) == len(tb) == 4
+    for a, b in zip(ta, tb):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    assert np.array_equal(sa.view(np.uint32), sb.view(np.uint32))
+    if sampler == "bernoulli":
+        # a fresh draw every iteration: the root covers differ
+        covers = [t["cover"][0] for t in ta]
+        assert len(set(covers)) > 1
