@@ -169,3 +169,45 @@ def test_sampled_training_graph_equals_eager(hip, orc, tall, sampler):
         # a fresh draw every iteration: the root covers differ
         covers = [t["cover"][0] for t in ta]
         assert len(set(covers)) > 1
+
+
+def goss_host(g, h, top_rate, other_rate, seed):
+    """GossSampler::sample (src/sampler.cpp:73-125) on the host: the top_k rows by |grad|
+    kept, other_k drawn uniformly from the rest and their grad/hess amplified by
+    (1 - top_rate) / other_rate, rows emitted ascending.  (Ties in |grad| at the cut
+    and the uniform draw use numpy's order and generator, not nth_element / std::sample
+    with std::mt19937: the seam under test takes whatever rows the sampler chose.)"""
+    n = g.size
+    top_k, other_k = int(round(top_rate * n)), int(round(other_rate * n))
+    order = np.argsort(-np.abs(g), kind="stable")
+    rest = order[top_k:]
+    picked = np.random.default_rng(seed).choice(rest, size=other_k, replace=False)
+    amp = np.float32((np.float32(1.0) - np.float32(top_rate)) / np.float32(other_rate))
+    g2, h2 = g.copy(), h.copy()
+    g2[picked] *= amp
+    h2[picked] *= amp
+    rows = np.sort(np.concatenate([order[:top_k], picked])).astype(np.uint32)
+    return rows, g2, h2
+
+
+@pytest.mark.parametrize("quantized", [True, False])
+def test_goss_through_the_engine_seam(hip, orc, tall, quantized):
+    # GOSS reads gradients, so the reference keeps it off the resident path
+    # (include/bonsai/booster.hpp:632-635): the host samples and amplifies, the engine
+    # grows on the kept rows with the amplified gradients.
+    bins, nb, X, y = tall
+    g, h = grad_of(orc, y, seed=13)
+    rows, g2, h2 = goss_host(g, h, 0.2, 0.1, seed=6)
+    ctx = ctx_for(hip, bins, nb, grower="hip_depthwise", quantized=quantized, max_depth=6)
+    gq = hq = sc = None
+    if quantized:
+        gq, hq, sc = ctx.set_gradients(g2, h2)
+    else:
+        ctx.set_gradients(g2, h2)
+    ctx.set_row_subset(rows)
+    nodes, lor = ctx.grow_tree()
+    ctx.close()
+    ref = oracle_tree(orc, "hip_depthwise", bins, nb, rows, g2, h2, gq, hq, sc, quantized, orc_cfg(max_depth=6))
+    assert_same_tree(nodes, ref[0])
+    assert np.array_equal(lor[rows], ref[1])
+    assert np.array_equal(lor, walk(nodes, bins, nb))
