@@ -62,6 +62,9 @@ struct bh_context {
     u16 *d_nbins = nullptr;
     float *label = nullptr, *score = nullptr;
     bool have_labels = false;
+    // quantized fits: the gradient maxima of the current scores are not in maxbits yet
+    // (scores, labels or the fit changed outside the iteration loop)
+    bool max_stale = true;
     // validation plane (eval_begin / eval_accumulate, device_context.cu:1666-1760)
     u8 *vplane = nullptr;
     float *vlabel = nullptr, *vscore = nullptr, *vhist = nullptr;
@@ -196,6 +199,7 @@ int alloc_fit(bh_context *c)
     bh_fit_config const &f = c->fc;
     u32 const N = static_cast<u32>(c->N);
     c->quant = f.quantized != 0;
+    c->max_stale = true;
     // Frontier capacity: 2^max_depth for level growth, 2 for leafwise rounds.
     u32 const depth = static_cast<u32>(f.max_depth);
     c->max_front = f.grower == BH_LEAFWISE ? 2u : (1u << std::min(depth, 14u));
@@ -506,6 +510,15 @@ void enqueue_grow(bh_context *c)
     else enqueue_leafwise(c->ext, c->stream, c->quant, grow_view(c));
 }
 
+// Quantized fits: the first iteration after scores, labels or the fit were set takes
+// its gradient maxima from a pass of its own (later ones from the score update).
+void prime_gradients(bh_context *c)
+{
+    if (!c->quant || !c->max_stale) return;
+    launch_grad_prime(c->stream, c->score, c->label, c->N, c->fc.objective, c->maxbits, c->ctl);
+    c->max_stale = false;
+}
+
 void enqueue_gradients(bh_context *c)
 {
     if (c->quant)
@@ -521,7 +534,7 @@ void enqueue_iteration(bh_context *c)
     enqueue_gradients(c);
     enqueue_grow(c);
     launch_score_update(c->stream, c->plane0, c->Fs, c->N, c->tree, c->ctl, c->d_nbins, c->F, c->fc.learning_rate,
-                        c->score, nullptr, c->max_nodes);
+                        c->score, nullptr, c->max_nodes, c->label, c->fc.objective, c->quant ? c->maxbits : nullptr);
     if (c->have_valid) {
         // the new tree on the validation plane, then the round's loss and the
         // early-stopping rule (eval_accumulate + the fit loop's check)
@@ -726,6 +739,7 @@ static int alloc_plane(bh_context *c, uint64_t n_rows, uint32_t n_features, cons
     c->d_nbins = nullptr;
     c->label = c->score = nullptr;
     c->have_labels = false;
+    c->max_stale = true;
     c->N = n_rows;
     c->F = n_features;
     c->Fs = round_up4(n_features);
@@ -818,6 +832,7 @@ int bh_upload_labels(bh_context *c, const float *labels)
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipMemcpy(c->label, labels, c->N * sizeof(float), hipMemcpyHostToDevice));
     c->have_labels = true;
+    c->max_stale = true;
     return 0;
 }
 
@@ -845,6 +860,7 @@ int bh_set_scores(bh_context *c, const float *scores)
     if (int r = check_ready(c)) return r;
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipMemcpy(c->score, scores, c->N * sizeof(float), hipMemcpyHostToDevice));
+    c->max_stale = true;
     // A new fit starts: iteration counter and tree store reset; a validation set armed
     // for the previous fit is disarmed (its scores track that fit).
     TreeCtl z{};
@@ -996,6 +1012,7 @@ int bh_train(bh_context *c, uint32_t n_iters)
     if (int r = ensure_tree_out(c, n_iters)) return r;
     c->iter_ms.clear();
     bool const graph = c->fc.use_graph && !c->fc.host_sync;
+    prime_gradients(c);
     if (graph && !c->graph_exec) {
         HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         enqueue_iteration(c);
@@ -1112,6 +1129,7 @@ int bh_autotune(bh_context *c, uint32_t iters, bh_tuning *chosen, float *ms_per_
         HIP_TRY(hipMemcpy(c->score, saved, c->N * sizeof(float), hipMemcpyDeviceToDevice));
         HIP_TRY(hipMemcpy(c->ctl, &saved_ctl, sizeof(TreeCtl), hipMemcpyHostToDevice));
         c->trees_done = saved_trees;
+        c->max_stale = true; // the restored scores' gradient maxima are not in maxbits
         return 0;
     };
     auto trial = [&](Tuning const &t, float *ms) -> int {
@@ -1217,6 +1235,7 @@ int bh_set_gradients(bh_context *c, const float *grad, const float *hess, uint32
     HIP_TRY(hipSetDevice(c->device));
     std::vector<float2> gh(c->N);
     for (u64 i = 0; i < c->N; ++i) gh[i] = make_float2(grad[i], hess[i]);
+    c->max_stale = true; // the host gradients' maxima overwrite slot 0
     if (!c->quant) {
         HIP_TRY(hipMemcpy(c->gh0f, gh.data(), c->N * sizeof(float2), hipMemcpyHostToDevice));
         return 0;
@@ -1423,6 +1442,7 @@ int bh_profile_hist(bh_context *c, uint32_t n_iters, double *ms_total, double *b
     c->prof = true;
     c->prof_ms = c->prof_bytes = 0.0;
     c->prof_launches = c->prof_rows = 0;
+    prime_gradients(c);
     for (u32 it = 0; it < n_iters; ++it) enqueue_iteration(c);
     c->prof = false;
     HIP_TRY(hipGetLastError());

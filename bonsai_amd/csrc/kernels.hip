@@ -178,9 +178,14 @@ __device__ __forceinline__ void block_max2_atomic(float mg, float mh, u32 *maxbi
     }
 }
 
+// Max-abs of g and h over every row, for the quantizer of the current iteration.  The
+// maxima live in two slots of maxbits by iteration parity (2 words each): the training
+// score update of iteration t fills slot (t + 1) & 1 for the next iteration, so this
+// pass only primes slot t & 1 when scores or labels changed outside the loop.
 __global__ void k_grad_max(float const *__restrict__ score, float const *__restrict__ label,
-                           u64 n, int objective, u32 *__restrict__ maxbits)
+                           u64 n, int objective, u32 *__restrict__ maxbits, TreeCtl const *__restrict__ ctl)
 {
+    maxbits += 2u * (ctl->iter & 1u);
     float mg = 0.0F, mh = 0.0F;
     for (u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; r < n;
          r += static_cast<u64>(gridDim.x) * blockDim.x) {
@@ -196,15 +201,18 @@ __global__ void k_grad_max(float const *__restrict__ score, float const *__restr
 // stochastic rounding q = floor(x * (qmax / max) + u), u from a counter RNG keyed on
 // (seed, iteration, original row, channel).
 __global__ void k_grad_quant(float const *__restrict__ score, float const *__restrict__ label,
-                             u64 n, int objective, u32 const *__restrict__ maxbits, u64 seed,
+                             u64 n, int objective, u32 *__restrict__ maxbits, u64 seed,
                              TreeCtl const *__restrict__ ctl, u32 *__restrict__ ghq,
                              double *__restrict__ scales)
 {
-    float const mg = __uint_as_float(maxbits[0]);
-    float const mh = __uint_as_float(maxbits[1]);
+    u32 const iter = ctl->iter;
+    u32 const par = iter & 1u;
+    float const mg = __uint_as_float(maxbits[2u * par]);
+    float const mh = __uint_as_float(maxbits[2u * par + 1u]);
     float const ig = mg > 0.0F ? 127.0F / mg : 0.0F;
     float const ih = mh > 0.0F ? 32767.0F / mh : 0.0F;
-    u32 const iter = ctl->iter;
+    // the other slot collects this iteration's score update maxima (nothing reads it now)
+    if (blockIdx.x == 0 && threadIdx.x < 2u) maxbits[2u * (par ^ 1u) + threadIdx.x] = 0u;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         scales[0] = static_cast<double>(mg) / 127.0;
         scales[1] = static_cast<double>(mh) / 32767.0;
@@ -2582,7 +2590,8 @@ __global__ __launch_bounds__(256) void k_score_update(u8 const *__restrict__ pla
                                                       TreeCtl const *__restrict__ ctl,
                                                       u16 const *__restrict__ n_bins, u32 F, float lr,
                                                       float *__restrict__ score, u32 *__restrict__ leaf_of_row,
-                                                      u32 node_cap, u32 last_cap)
+                                                      u32 node_cap, u32 last_cap, float const *__restrict__ label,
+                                                      int objective, u32 *__restrict__ maxbits)
 {
     // LDS: [node_cap WalkNode][last_cap u16][rows: 256 * Fs + 16, 16-byte aligned]
     extern __shared__ __align__(16) unsigned char smem[];
@@ -2592,6 +2601,7 @@ __global__ __launch_bounds__(256) void k_score_update(u8 const *__restrict__ pla
         smem + ((node_cap * sizeof(WalkNode) + last_cap * sizeof(u16) + 15u) & ~static_cast<size_t>(15)));
     u32 const nn = ctl->n_nodes;
     u32 const nl = min(nn, node_cap);
+    float mg = 0.0F, mh = 0.0F;
     for (u32 i = threadIdx.x; i < nl; i += blockDim.x) nodes[i] = to_walk(tree[i]);
     for (u32 f = threadIdx.x; f < F && f < last_cap; f += blockDim.x) last[f] = static_cast<u16>(n_bins[f] - 1u);
     __syncthreads();
@@ -2608,6 +2618,9 @@ __global__ __launch_bounds__(256) void k_score_update(u8 const *__restrict__ pla
             row = plane + r * Fs;
         }
         if (r >= n) continue;
+        // the row's score and label loads fly during the walk
+        float const s0 = score ? score[r] : 0.0F;
+        float const y = maxbits ? label[r] : 0.0F;
         u32 idx = 0;
         WalkNode w = nl ? nodes[0] : to_walk(tree[0]);
         while (w.feature != 0xFFFFu) {
@@ -2617,9 +2630,20 @@ __global__ __launch_bounds__(256) void k_score_update(u8 const *__restrict__ pla
             idx = routes_left(b, lb, w.bin, w.default_left) ? w.left : w.right;
             w = idx < nl ? nodes[idx] : to_walk(tree[idx]);
         }
-        if (score) score[r] = __fadd_rn(score[r], __fmul_rn(lr, w.value));
+        if (score) {
+            float const s = __fadd_rn(s0, __fmul_rn(lr, w.value));
+            score[r] = s;
+            if (maxbits) {
+                float g, h;
+                grad_of(objective, s, y, g, h);
+                mg = fmaxf(mg, fabsf(g));
+                mh = fmaxf(mh, fabsf(h));
+            }
+        }
         if (leaf_of_row) leaf_of_row[r] = idx;
     }
+    // the next iteration's gradient maxima (k_grad_max's pass, fused)
+    if (maxbits) block_max2_atomic(mg, mh, maxbits + 2u * ((ctl->iter + 1u) & 1u));
 }
 
 // Validation loss pass (eval_loss_pass1_kernel, src/cuda/detail/kernels.cuh:1061-1103):
@@ -2909,10 +2933,14 @@ void launch_grad_f32(hipStream_t s, float const *score, float const *label, u64 
 void launch_grad_quant(hipStream_t s, float const *score, float const *label, u64 n, int obj, u32 *maxbits,
                        u64 seed, TreeCtl const *ctl, u32 *ghq, double *scales)
 {
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, maxbits, 2u);
-    hipLaunchKernelGGL(k_grad_max, dim3(1024), dim3(256), 0, s, score, label, n, obj, maxbits);
     hipLaunchKernelGGL(k_grad_quant, dim3(2048), dim3(256), 0, s, score, label, n, obj, maxbits, seed, ctl, ghq,
                        scales);
+}
+void launch_grad_prime(hipStream_t s, float const *score, float const *label, u64 n, int obj, u32 *maxbits,
+                       TreeCtl const *ctl)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, maxbits, 4u);
+    hipLaunchKernelGGL(k_grad_max, dim3(1024), dim3(256), 0, s, score, label, n, obj, maxbits, ctl);
 }
 void launch_quant_from(hipStream_t s, float2 const *gh, u64 n, u32 *maxbits, u64 seed, u32 iter, u32 *ghq,
                        double *scales_dev)
@@ -3154,17 +3182,18 @@ void launch_tree_begin(hipStream_t s, TreeCtl *ctl, NodeInfo *front, DevNode *tr
 }
 
 void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode const *tree, TreeCtl const *ctl,
-                         u16 const *n_bins, u32 F, float lr, float *score, u32 *leaf_of_row, u32 max_nodes)
+                         u16 const *n_bins, u32 F, float lr, float *score, u32 *leaf_of_row, u32 max_nodes,
+                         float const *label, int objective, u32 *maxbits)
 {
     u32 const node_cap = std::min(max_nodes, kWalkLds);
     u32 const last_cap = std::min(F, kWalkLds);
     size_t const head = (node_cap * sizeof(WalkNode) + last_cap * sizeof(u16) + 15u) & ~static_cast<size_t>(15);
     if (Fs <= 64)
         hipLaunchKernelGGL(k_score_update<true>, dim3(2048), dim3(256), head + 256u * Fs + 16u, s, plane, Fs, n, tree,
-                           ctl, n_bins, F, lr, score, leaf_of_row, node_cap, last_cap);
+                           ctl, n_bins, F, lr, score, leaf_of_row, node_cap, last_cap, label, objective, maxbits);
     else
         hipLaunchKernelGGL(k_score_update<false>, dim3(2048), dim3(256), head, s, plane, Fs, n, tree, ctl, n_bins, F,
-                           lr, score, leaf_of_row, node_cap, last_cap);
+                           lr, score, leaf_of_row, node_cap, last_cap, label, objective, maxbits);
 }
 
 u32 eval_blocks(u32 n) { return std::max(1u, std::min(n / 256u, 1024u)); }

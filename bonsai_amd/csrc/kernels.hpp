@@ -6,8 +6,14 @@
 namespace bh {
 
 void launch_grad_f32(hipStream_t s, float const *score, float const *label, u64 n, int obj, float2 *gh);
+// Quantized gradients of iteration ctl->iter with the max-abs in maxbits slot iter & 1
+// (two slots of two words); clears the other slot for the score update to fill.
 void launch_grad_quant(hipStream_t s, float const *score, float const *label, u64 n, int obj, u32 *maxbits,
                        u64 seed, TreeCtl const *ctl, u32 *ghq, double *scales);
+// Fills maxbits slot ctl->iter & 1 from the current scores and labels (when they were
+// set outside the iteration loop).
+void launch_grad_prime(hipStream_t s, float const *score, float const *label, u64 n, int obj, u32 *maxbits,
+                       TreeCtl const *ctl);
 void launch_quant_from(hipStream_t s, float2 const *gh, u64 n, u32 *maxbits, u64 seed, u32 iter, u32 *ghq,
                        double *scales_dev);
 hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl const *ctl, u8 const *pl0,
@@ -63,8 +69,11 @@ void launch_tree_begin(hipStream_t s, TreeCtl *ctl, NodeInfo *front, DevNode *tr
 // their count into ctl->root_n.  chunk_cnt: n / kPartChunk + 1 words of scratch.
 void launch_bernoulli(hipStream_t s, TreeCtl *ctl, u32 n, u64 seed, u32 thr, u32 *chunk_cnt, u8 const *plane,
                       u32 Fs, void const *gh, u32 gh_bytes, u8 *out_plane, void *out_gh);
+// maxbits (training, quantized): also the next iteration's gradient maxima, from the
+// updated scores and the labels, into slot (ctl->iter + 1) & 1.
 void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode const *tree, TreeCtl const *ctl,
-                         u16 const *n_bins, u32 F, float lr, float *score, u32 *leaf_of_row, u32 max_nodes);
+                         u16 const *n_bins, u32 F, float lr, float *score, u32 *leaf_of_row, u32 max_nodes,
+                         float const *label = nullptr, int objective = 0, u32 *maxbits = nullptr);
 void launch_iter_advance(hipStream_t s, TreeCtl *ctl);
 void launch_copy_tree(hipStream_t s, DevNode const *tree, TreeCtl const *ctl, DevNode *out_trees, u32 *out_sizes,
                       u32 max_nodes);

@@ -211,3 +211,21 @@ def test_goss_through_the_engine_seam(hip, orc, tall, quantized):
     assert_same_tree(nodes, ref[0])
     assert np.array_equal(lor[rows], ref[1])
     assert np.array_equal(lor, walk(nodes, bins, nb))
+
+
+@pytest.mark.parametrize("grower", list(GROWERS))
+def test_bernoulli_keeping_no_rows_gives_a_single_leaf(hip, orc, tall, grower):
+    # BernoulliSampler can select no row at all; the tree is then one leaf over no rows
+    # (zero sums: value 0) and every row lands on it.
+    bins, nb, X, y = tall
+    n = bins.shape[0]
+    assert orc.bernoulli_rows(n, 1e-9, 5, 0).size == 0
+    g, h = grad_of(orc, y, seed=2)
+    ctx = ctx_for(hip, bins, nb, grower=grower, quantized=True, **grow_cfg(grower))
+    ctx.set_gradients(g, h)
+    ctx.set_bernoulli(1e-9, seed=5)
+    nodes, lor = ctx.grow_tree()
+    ctx.close()
+    assert nodes[0]["feature"] < 0
+    assert nodes[0]["value"] == 0.0 and nodes[0]["cover"] == 0.0
+    assert np.all(lor == 0)
