@@ -1,0 +1,455 @@
+// Node split find (the body of k_find, shared with the leafwise finder that also starts
+// the next round).  The algorithm and its reference are described at k_find.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "bh_common.hpp"
+
+namespace bh {
+
+constexpr u32 kFindTile = 7;
+// Threads per split-find workgroup: 16 waves, so each SIMD interleaves four waves of the
+// FP64 candidate scoring.
+constexpr u32 kFindThreads = 1024;
+
+template <bool Q> struct CellT;
+template <> struct CellT<false> { typedef float2 T; typedef double A; };
+template <> struct CellT<true> { struct T { i64 x, y; }; typedef i64 A; };
+
+__device__ __forceinline__ double fd_l1_thr(double g, double l1)
+{
+    if (g > l1) return g - l1;
+    if (g < -l1) return g + l1;
+    return 0.0;
+}
+// include/bonsai/split.hpp:135-143
+__device__ __forceinline__ double fd_score_of(double g, double h, double l1, double l2)
+{
+    double const t = fd_l1_thr(g, l1);
+    double const d = h + l2;
+    return d > 0.0 ? (t * t) / d : 0.0;
+}
+
+template <bool Q>
+__device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restrict__ ctl, void const *__restrict__ hist,
+                          u16 const *__restrict__ n_bins, u32 F, u32 SB, GrowCfg cfg, double const *__restrict__ scales,
+                          SplitRec *__restrict__ out, u32 front_base, u32 n_front_override,
+                          SplitRec *__restrict__ part, u32 *__restrict__ done, u32 const *__restrict__ cnt)
+{
+    typedef typename CellT<Q>::T Cell;
+    typedef typename CellT<Q>::A Acc;
+    extern __shared__ __align__(16) unsigned char smem[];
+    Cell *s_cells = reinterpret_cast<Cell *>(smem);              // [kFindTile][SB]
+    Acc *pg = reinterpret_cast<Acc *>(s_cells + kFindTile * SB); // [kFindTile][SB]
+    Acc *ph = pg + kFindTile * SB;                                // [kFindTile][SB]
+    u32 *s_cn = reinterpret_cast<u32 *>(ph + kFindTile * SB);    // [kFindTile][SB] counts, then their prefix
+    __shared__ double s_best_gain[kFindThreads / 64];
+    __shared__ u32 s_best_key[kFindThreads / 64];
+    __shared__ double s_tot[2];
+    __shared__ i64 s_itot[2];
+    __shared__ Acc s_mg[kFindTile], s_mh[kFindTile];
+    __shared__ u32 s_ncut[kFindTile], s_dirs[kFindTile], s_nb[kFindTile], s_mc[kFindTile];
+
+    // Every launch's frontier buffer holds gridDim.y entries past front_base, so the
+    // node loads beside the frontier size.
+    u32 const i = front_base + blockIdx.y;
+    NodeInfo node = front[i];
+    u32 const n_front = n_front_override ? n_front_override : ctl->n_front;
+    if (blockIdx.y >= n_front) return;
+    u32 const G = gridDim.x, bx = blockIdx.x;
+    if (node.slot == kNone) {
+        // No histogram (a leafwise child at max_depth): totals came from the parent
+        // split and the node cannot split.
+        if (bx == 0 && threadIdx.x == 0) {
+            SplitRec rec{};
+            rec.default_left = 1;
+            out[i] = rec;
+        }
+        return;
+    }
+    size_t const node_cells = static_cast<size_t>(F) * SB;
+    Cell const *hcells = reinterpret_cast<Cell const *>(hist) + static_cast<size_t>(node.slot) * node_cells;
+    u32 const *hcnt = cnt ? cnt + static_cast<size_t>(node.slot) * node_cells : nullptr;
+    double const sg = Q ? scales[0] : 0.0, sh = Q ? scales[1] : 0.0;
+    bool const can = node.cnt >= 2u * static_cast<u32>(cfg.min_data_in_leaf) &&
+                     static_cast<int>(node.depth) < cfg.max_depth;
+
+    double best_gain = 0.0;
+    u32 best_key = kNone;
+    double bgL = 0.0, bhL = 0.0, bgR = 0.0, bhR = 0.0; // winner's child sums (FP32 path)
+    i64 bigL = 0, bihL = 0, bigR = 0, bihR = 0;        // winner's child sums (quantized)
+    u32 bnl = 0;                                       // winner's left rows (count cells)
+    __shared__ u32 s_last;
+    double node_score = 0.0, tg = 0.0, th = 0.0;
+    bool const had_totals = node.totals_ready != 0;
+    // Quantized totals are exact integers, equal over every feature's cells: a block that
+    // scans takes them from the first feature of its first tile (no extra round trip).
+    bool const tot_from_tile = Q && can && !had_totals;
+    // Phase 0: node totals from the first feature's cells (Histogram::totals,
+    // include/bonsai/histogram.hpp:76-87) unless a parent split already set them.
+    if (!had_totals && !tot_from_tile) {
+        u32 const nb0 = n_bins[0];
+        for (u32 k = threadIdx.x; k < nb0; k += blockDim.x) s_cells[k] = hcells[k];
+        __syncthreads();
+        if (Q) {
+            // Exact integer totals: any order (one wave reduces feature 0's cells).
+            if (threadIdx.x < 64) {
+                i64 g = 0, h = 0;
+                for (u32 b = threadIdx.x; b < nb0; b += 64) {
+                    g += reinterpret_cast<i64 const *>(&s_cells[b])[0];
+                    h += reinterpret_cast<i64 const *>(&s_cells[b])[1];
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    g += __shfl_xor(g, o);
+                    h += __shfl_xor(h, o);
+                }
+                if (threadIdx.x == 0) {
+                    node.itg = g;
+                    node.ith = h;
+                }
+            }
+        } else if (threadIdx.x == 0) {
+            // Histogram::totals: the first feature's cells, double in bin order, stored
+            // as float.
+            double g = 0.0, h = 0.0;
+            float2 const *cf = reinterpret_cast<float2 const *>(s_cells);
+            u32 b = 0;
+            for (; b + 8 <= nb0; b += 8) {
+                float2 v[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) v[k] = cf[b + k];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    g += v[k].x;
+                    h += v[k].y;
+                }
+            }
+            for (; b < nb0; ++b) {
+                g += cf[b].x;
+                h += cf[b].y;
+            }
+            node.tg = static_cast<float>(g);
+            node.th = static_cast<float>(h);
+        }
+        if (threadIdx.x == 0) node.totals_ready = 1;
+    }
+    if (threadIdx.x == 0) {
+        s_tot[0] = node.tg;
+        s_tot[1] = node.th;
+        s_itot[0] = node.itg;
+        s_itot[1] = node.ith;
+    }
+    __syncthreads();
+    if (!can) {
+        if (bx == 0 && threadIdx.x == 0) {
+            if (!had_totals) front[i] = node;
+            SplitRec rec{};
+            rec.default_left = 1;
+            out[i] = rec;
+        }
+        return;
+    }
+    // Block bx scans feature tiles bx, bx + G, ...
+    for (u32 f0 = bx * kFindTile; f0 < F; f0 += G * kFindTile) {
+        u32 const nt = min(kFindTile, F - f0);
+        __syncthreads(); // the previous tile (or phase 0) is done with the LDS
+        // Stage the tile's cells, bin counts and count cells (coalesced, one round trip).
+        if (threadIdx.x < nt) s_nb[threadIdx.x] = n_bins[f0 + threadIdx.x];
+        for (u32 k = threadIdx.x; k < nt * SB; k += blockDim.x) s_cells[k] = hcells[static_cast<size_t>(f0) * SB + k];
+        if (hcnt)
+            for (u32 k = threadIdx.x; k < nt * SB; k += blockDim.x) s_cn[k] = hcnt[static_cast<size_t>(f0) * SB + k];
+        __syncthreads();
+        if (tot_from_tile && f0 == bx * kFindTile) {
+            if (threadIdx.x < 64) {
+                i64 g = 0, h = 0;
+                for (u32 b = threadIdx.x; b < SB; b += 64) {
+                    g += reinterpret_cast<i64 const *>(&s_cells[b])[0];
+                    h += reinterpret_cast<i64 const *>(&s_cells[b])[1];
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    g += __shfl_xor(g, o);
+                    h += __shfl_xor(h, o);
+                }
+                if (threadIdx.x == 0) {
+                    node.itg = g;
+                    node.ith = h;
+                    node.totals_ready = 1;
+                    s_itot[0] = g;
+                    s_itot[1] = h;
+                }
+            }
+            __syncthreads();
+        }
+        if (f0 == bx * kFindTile) {
+            tg = Q ? static_cast<double>(s_itot[0]) * sg : s_tot[0];
+            th = Q ? static_cast<double>(s_itot[1]) * sh : s_tot[1];
+            node_score = fd_score_of(tg, th, cfg.l1, cfg.l2);
+        }
+        // Phase A: per-feature running sums in bin order (src/split.cpp:131-158).
+        // Quantized: exact integers, so a wave scans a feature in parallel (the count
+        // cells alongside, in place).
+        if (Q) {
+            u32 const wv = threadIdx.x >> 6, ln = threadIdx.x & 63u;
+            for (u32 t = wv; t < kFindTile; t += blockDim.x >> 6) {
+                u32 ncut = 0, dirs = 1;
+                if (t < nt) {
+                    u32 const nb = s_nb[t];
+                    ncut = nb >= 2 ? nb - 2 : 0;
+                    i64 const *ci = reinterpret_cast<i64 const *>(s_cells + t * SB);
+                    u32 *cn = s_cn + t * SB;
+                    i64 const mg = nb ? ci[2 * (nb - 1)] : 0, mh = nb ? ci[2 * (nb - 1) + 1] : 0;
+                    dirs = (mg == 0 && mh == 0) ? 1 : 2;
+                    if (ln == 0) {
+                        s_mg[t] = mg;
+                        s_mh[t] = mh;
+                        s_mc[t] = hcnt && nb ? cn[nb - 1] : 0u;
+                    }
+                    u32 const per = (ncut + 63u) / 64u;
+                    u32 const b0 = ln * per;
+                    i64 lg[8], lh[8];
+                    u32 lc[8];
+                    i64 g = 0, h = 0;
+                    u32 cc = 0;
+#pragma unroll
+                    for (u32 k = 0; k < 8; ++k) {
+                        u32 const b = b0 + k;
+                        if (k < per && b < ncut) {
+                            g += ci[2 * b];
+                            h += ci[2 * b + 1];
+                            if (hcnt) cc += cn[b];
+                        }
+                        lg[k] = g;
+                        lh[k] = h;
+                        lc[k] = cc;
+                    }
+                    // inclusive scan of the lane totals
+                    i64 sg2 = g, sh2 = h;
+                    u32 sc2 = cc;
+                    for (int o = 1; o < 64; o <<= 1) {
+                        i64 const ug = __shfl_up(sg2, o), uh = __shfl_up(sh2, o);
+                        u32 const uc = __shfl_up(sc2, o);
+                        if (static_cast<int>(ln) >= o) {
+                            sg2 += ug;
+                            sh2 += uh;
+                            sc2 += uc;
+                        }
+                    }
+                    i64 const eg = sg2 - g, eh = sh2 - h;
+                    u32 const ec = sc2 - cc;
+                    Acc *qg = pg + t * SB, *qh = ph + t * SB;
+#pragma unroll
+                    for (u32 k = 0; k < 8; ++k) {
+                        u32 const b = b0 + k;
+                        if (k < per && b < ncut) {
+                            qg[b] = eg + lg[k];
+                            qh[b] = eh + lh[k];
+                            if (hcnt) cn[b] = ec + lc[k];
+                        }
+                    }
+                }
+                if (ln == 0) {
+                    s_ncut[t] = ncut;
+                    s_dirs[t] = dirs;
+                }
+            }
+        }
+        u32 const t = threadIdx.x;
+        if (!Q && t < kFindTile) {
+            u32 ncut = 0, dirs = 1;
+            if (t < nt) {
+                u32 const nb = s_nb[t];
+                ncut = nb >= 2 ? nb - 2 : 0;
+                Acc *qg = pg + t * SB, *qh = ph + t * SB;
+                float2 const *cf = reinterpret_cast<float2 const *>(s_cells + t * SB);
+                float2 const m = nb ? cf[nb - 1] : make_float2(0.0F, 0.0F);
+                s_mg[t] = m.x;
+                s_mh[t] = m.y;
+                dirs = (m.x == 0.0F && m.y == 0.0F) ? 1 : 2;
+                double g = 0.0, h = 0.0;
+                u32 b = 0;
+                for (; b + 4 <= ncut; b += 4) {
+                    float2 const v0 = cf[b], v1 = cf[b + 1], v2 = cf[b + 2], v3 = cf[b + 3];
+                    g += v0.x; h += v0.y; qg[b] = g; qh[b] = h;
+                    g += v1.x; h += v1.y; qg[b + 1] = g; qh[b + 1] = h;
+                    g += v2.x; h += v2.y; qg[b + 2] = g; qh[b + 2] = h;
+                    g += v3.x; h += v3.y; qg[b + 3] = g; qh[b + 3] = h;
+                }
+                for (; b < ncut; ++b) {
+                    float2 const v = cf[b];
+                    g += v.x;
+                    h += v.y;
+                    qg[b] = g;
+                    qh[b] = h;
+                }
+            }
+            s_ncut[t] = ncut;
+            s_dirs[t] = dirs;
+        }
+        __syncthreads();
+        // Phase B: every (feature, cut bin, direction) candidate; a thread takes one
+        // (feature, bin) pair and scores its directions from one prefix load.
+        u32 const n_pairs = nt * SB;
+        for (u32 pi = threadIdx.x; pi < n_pairs; pi += blockDim.x) {
+            u32 const tt = pi / SB;
+            u32 const b = pi - tt * SB;
+            if (b >= s_ncut[tt]) continue;
+            u32 const dirs = s_dirs[tt];
+            Acc const plg = pg[tt * SB + b], plh = ph[tt * SB + b];
+            u32 const pnl = hcnt ? s_cn[tt * SB + b] : 0u;
+            for (u32 d = 0; d < dirs; ++d) {
+                bool const dl = d == 0;
+                double gL, hL, gR, hR;
+                i64 igL = 0, ihL = 0, igR = 0, ihR = 0;
+                if (Q) {
+                    i64 const lg = plg, lh = plh;
+                    i64 const mg = s_mg[tt], mh = s_mh[tt];
+                    i64 const rg = s_itot[0] - mg, rh = s_itot[1] - mh;
+                    igL = lg + (dl ? mg : 0);
+                    ihL = lh + (dl ? mh : 0);
+                    igR = (rg - lg) + (!dl ? mg : 0);
+                    ihR = (rh - lh) + (!dl ? mh : 0);
+                    gL = static_cast<double>(igL) * sg;
+                    hL = static_cast<double>(ihL) * sh;
+                    gR = static_cast<double>(igR) * sg;
+                    hR = static_cast<double>(ihR) * sh;
+                } else {
+                    double const lg = plg, lh = plh;
+                    double const mg = s_mg[tt], mh = s_mh[tt];
+                    double const rg = tg - mg, rh = th - mh;
+                    double const right_g = rg - lg, right_h = rh - lh;
+                    gL = lg + (dl ? mg : 0.0);
+                    hL = lh + (dl ? mh : 0.0);
+                    gR = right_g + (!dl ? mg : 0.0);
+                    hR = right_h + (!dl ? mh : 0.0);
+                }
+                if (hL < cfg.min_child_hess || hR < cfg.min_child_hess) continue;
+                double const cs = fd_score_of(gL, hL, cfg.l1, cfg.l2) + fd_score_of(gR, hR, cfg.l1, cfg.l2);
+                double const gain = cs - node_score;
+                if (!(gain > 0.0 && gain >= cfg.min_gain)) continue;
+                u32 const key = (f0 + tt) * (2u * SB) + b * 2u + d;
+                if (gain > best_gain || (gain == best_gain && key < best_key)) {
+                    best_gain = gain;
+                    best_key = key;
+                    bgL = gL;
+                    bhL = hL;
+                    bgR = gR;
+                    bhR = hR;
+                    bigL = igL;
+                    bihL = ihL;
+                    bigR = igR;
+                    bihR = ihR;
+                    // Left rows: bins <= the cut bin, plus the missing bin when it routes left.
+                    if (hcnt) bnl = pnl + (dl ? s_mc[tt] : 0u);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // Block winner (gain, then lowest key; a total order, so the butterfly is exact):
+    // within each wave by shuffles, then over the wave winners in wave 0.
+    {
+        double g = best_gain;
+        u32 k = best_key;
+        for (int o = 32; o > 0; o >>= 1) {
+            double const og = __shfl_xor(g, o);
+            u32 const ok = __shfl_xor(k, o);
+            if (ok != kNone && (k == kNone || og > g || (og == g && ok < k))) {
+                g = og;
+                k = ok;
+            }
+        }
+        if ((threadIdx.x & 63u) == 0) {
+            s_best_gain[threadIdx.x >> 6] = g;
+            s_best_key[threadIdx.x >> 6] = k;
+        }
+        __syncthreads();
+        if (threadIdx.x < 64u) {
+            u32 const nw = blockDim.x >> 6;
+            g = threadIdx.x < nw ? s_best_gain[threadIdx.x] : 0.0;
+            k = threadIdx.x < nw ? s_best_key[threadIdx.x] : kNone;
+            for (int o = 32; o > 0; o >>= 1) {
+                double const og = __shfl_xor(g, o);
+                u32 const ok = __shfl_xor(k, o);
+                if (ok != kNone && (k == kNone || og > g || (og == g && ok < k))) {
+                    g = og;
+                    k = ok;
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_best_gain[0] = g;
+            s_best_key[0] = k;
+        }
+        __syncthreads();
+    }
+    // The thread that scored the winner publishes its child sums and left rows (keys
+    // are unique) to the block's partial record; the node's last block to finish merges
+    // the partials (gain, then lowest key).
+    u32 const key = s_best_key[0];
+    SplitRec *dst = part + static_cast<size_t>(i) * G + bx;
+    if (key != kNone && best_key == key) {
+        SplitRec rec{};
+        u32 const f = key / (2u * SB), b = (key >> 1) % SB, d = key & 1u;
+        rec.gain = s_best_gain[0];
+        rec.feature = f;
+        rec.bin = b;
+        rec.default_left = d == 0 ? 1u : 0u;
+        rec.valid = 1;
+        rec.gL = bgL;
+        rec.hL = bhL;
+        rec.gR = bgR;
+        rec.hR = bhR;
+        rec.igL = bigL;
+        rec.ihL = bihL;
+        rec.igR = bigR;
+        rec.ihR = bihR;
+        if (hcnt) {
+            rec.n_left = bnl;
+            rec.n_right = node.cnt - bnl;
+        }
+        *dst = rec;
+        __threadfence();
+    } else if (key == kNone && threadIdx.x == 0) {
+        SplitRec rec{};
+        rec.default_left = 1;
+        *dst = rec;
+        __threadfence();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(done + i, 1u) == G - 1u ? 1u : 0u;
+    __syncthreads();
+    if (s_last == 0u) return;
+    __shared__ SplitRec s_parts[kFindMaxBlocks];
+    // The G partial records are loaded in parallel by lanes of wave 0 (G <= 64), after
+    // one fence in that wave, then compared.
+    if (threadIdx.x < 64u) {
+        __threadfence();
+        if (threadIdx.x < G) s_parts[threadIdx.x] = part[static_cast<size_t>(i) * G + threadIdx.x];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u32 bi = kNone, bkey = kNone;
+        double bg = 0.0;
+        for (u32 k = 0; k < G; ++k) {
+            SplitRec const &r = s_parts[k];
+            if (!r.valid) continue;
+            u32 const rk = r.feature * (2u * SB) + r.bin * 2u + (r.default_left ? 0u : 1u);
+            if (bkey == kNone || r.gain > bg || (r.gain == bg && rk < bkey)) {
+                bi = k;
+                bkey = rk;
+                bg = r.gain;
+            }
+        }
+        done[i] = 0u;
+        if (!had_totals) front[i] = node;
+        if (bi == kNone) {
+            SplitRec rec{};
+            rec.default_left = 1;
+            out[i] = rec;
+        } else {
+            out[i] = s_parts[bi];
+        }
+    }
+}
+
+} // namespace bh

@@ -15,6 +15,7 @@
 #include "kernels.hpp"
 #include "kernels_ext.hpp"
 #include "bh_device.hpp"
+#include "bh_find.hpp"
 
 namespace bh {
 
@@ -516,7 +517,8 @@ struct LwCtl {
     u32 demoted;
     u32 small_left;
     u32 n_push;      // children of the last round that still have to join the pool
-    u32 pad[10];
+    u32 grid_done;   // workgroups of the running split find that have finished
+    u32 pad[9];
 };
 
 // Rows per partition chunk of a leaf of `rows` rows: about kLwChunkTarget chunks,
@@ -539,7 +541,7 @@ __device__ __forceinline__ u32 lw_unit_rows(u32 rows, u32 target)
 
 // queue_children (src/grower_impl.hpp:404-428): a child with a valid split joins the
 // pool; one without becomes a leaf now and frees its slot.  One thread.
-__device__ void lw_push(LwCtl *lw, NodeInfo const *kids, SplitRec const *splits, u32 n, bool is_root, Cand *cands,
+__device__ __forceinline__ void lw_push(LwCtl *lw, NodeInfo const *kids, SplitRec const *splits, u32 n, bool is_root, Cand *cands,
                         u32 *free_slots, DevNode *tree, GrowCfg const &cfg, double const *scales)
 {
     for (u32 k = 0; k < n; ++k) {
@@ -566,7 +568,7 @@ __device__ void lw_push(LwCtl *lw, NodeInfo const *kids, SplitRec const *splits,
 // the split's exact integer sums (quantized), so the finder need not derive them.
 // Returns the smaller child's unit count (0: no histogram this round).
 template <bool Q>
-__device__ u32 lw_children(PartOp &op, u32 nL, TreeCtl *ctl, LwCtl *lw, NodeInfo const &par, SplitRec const &s,
+__device__ __forceinline__ u32 lw_children(PartOp &op, u32 nL, TreeCtl *ctl, LwCtl *lw, NodeInfo const &par, SplitRec const &s,
                            u32 *free_slots, NodeInfo *kids, DevNode *tree, GrowCfg const &cfg,
                            double const *scales, bool split_totals, u32 unit_target, FinishOp &fo,
                            u32 &unit_rows)
@@ -624,8 +626,8 @@ __device__ u32 lw_children(PartOp &op, u32 nL, TreeCtl *ctl, LwCtl *lw, NodeInfo
         ctl->n_hist_chunks = 0;
     } else {
         u32 const fresh = free_slots[--lw->n_free];
-        (left_small ? L : R).slot = fresh;
-        (left_small ? R : L).slot = par.slot;
+        L.slot = left_small ? fresh : par.slot;
+        R.slot = left_small ? par.slot : fresh;
         u32 const small_cnt = left_small ? nL : nR;
         unit_rows = lw_unit_rows(small_cnt, unit_target);
         units = (small_cnt + unit_rows - 1) / unit_rows;
@@ -682,21 +684,21 @@ __device__ __forceinline__ void lw_stash_units(u32 *u, FinishOp const &fo)
 // its split committed (commit_split_node, :188-210) and the one partition op emitted,
 // in chunks sized to the leaf.  COUNTED (quantized): the split carries its child row
 // counts, so the children, their slots and the histogram units are set here and the
-// scatter takes each chunk's prefix from the chunk counts.
+// scatter takes each chunk's prefix from the chunk counts.  Runs as its own one-block
+// kernel for the first round and in the last workgroup of every round's split find for
+// the next; any workgroup size that is a multiple of 64.
 template <bool COUNTED>
-__global__ __launch_bounds__(256) void k_lw_select(TreeCtl *__restrict__ ctl, LwCtl *__restrict__ lw,
-                                                   Cand *__restrict__ cands, NodeInfo *__restrict__ kids,
-                                                   SplitRec const *__restrict__ kid_splits,
-                                                   u32 *__restrict__ free_slots, NodeInfo *__restrict__ parent_out,
-                                                   SplitRec *__restrict__ split_out, DevNode *__restrict__ tree,
-                                                   PartOp *__restrict__ ops, PartChunk *__restrict__ pch,
-                                                   HistChunk *__restrict__ hch, u16 const *__restrict__ n_bins,
-                                                   FinishOp *__restrict__ fops, GrowCfg cfg,
-                                                   double const *__restrict__ scales, u32 unit_target)
+__device__ __forceinline__ void lw_select_round(TreeCtl *__restrict__ ctl, LwCtl *__restrict__ lw, Cand *__restrict__ cands,
+                                NodeInfo *__restrict__ kids, SplitRec const *__restrict__ kid_splits,
+                                u32 *__restrict__ free_slots, NodeInfo *__restrict__ parent_out,
+                                SplitRec *__restrict__ split_out, DevNode *__restrict__ tree, PartOp *__restrict__ ops,
+                                PartChunk *__restrict__ pch, HistChunk *__restrict__ hch,
+                                u16 const *__restrict__ n_bins, FinishOp *__restrict__ fops, GrowCfg const &cfg,
+                                double const *__restrict__ scales, u32 unit_target)
 {
-    __shared__ double s_g[256];
-    __shared__ u32 s_id[256], s_ix[256];
-    __shared__ u32 s_sel, s_go, s_n, s_units, s_urows, s_fo[4];
+    __shared__ double s_wg[16];
+    __shared__ u32 s_wid[16], s_wix[16];
+    __shared__ u32 s_sel, s_go, s_n, s_units, s_urows, s_fo[4], s_ccnt, s_crow, s_nch;
     if (ctl->done) {
         if (threadIdx.x == 0) {
             ctl->n_ops = 0;
@@ -727,30 +729,42 @@ __global__ __launch_bounds__(256) void k_lw_select(TreeCtl *__restrict__ ctl, Lw
             bix = k;
         }
     }
-    s_g[threadIdx.x] = bg;
-    s_id[threadIdx.x] = bid;
-    s_ix[threadIdx.x] = bix;
+    // argmax (gain, then lower id: a total order, so the butterfly is exact) in each wave,
+    // then over the wave winners in wave 0
+    for (int o = 32; o > 0; o >>= 1) {
+        double const og = __shfl_xor(bg, o);
+        u32 const oid = __shfl_xor(bid, o), oix = __shfl_xor(bix, o);
+        if (oix != kNone && (bix == kNone || og > bg || (og == bg && oid < bid))) {
+            bg = og;
+            bid = oid;
+            bix = oix;
+        }
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        s_wg[threadIdx.x >> 6] = bg;
+        s_wid[threadIdx.x >> 6] = bid;
+        s_wix[threadIdx.x >> 6] = bix;
+    }
     __syncthreads();
-    for (u32 st = blockDim.x / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st) {
-            u32 const ox = s_ix[threadIdx.x + st];
-            if (ox != kNone) {
-                double const og = s_g[threadIdx.x + st];
-                u32 const oid = s_id[threadIdx.x + st];
-                if (s_ix[threadIdx.x] == kNone || og > s_g[threadIdx.x] ||
-                    (og == s_g[threadIdx.x] && oid < s_id[threadIdx.x])) {
-                    s_g[threadIdx.x] = og;
-                    s_id[threadIdx.x] = oid;
-                    s_ix[threadIdx.x] = ox;
-                }
+    if (threadIdx.x < 64u) {
+        u32 const nw = blockDim.x >> 6;
+        bg = threadIdx.x < nw ? s_wg[threadIdx.x] : 0.0;
+        bid = threadIdx.x < nw ? s_wid[threadIdx.x] : kNone;
+        bix = threadIdx.x < nw ? s_wix[threadIdx.x] : kNone;
+        for (int o = 32; o > 0; o >>= 1) {
+            double const og = __shfl_xor(bg, o);
+            u32 const oid = __shfl_xor(bid, o), oix = __shfl_xor(bix, o);
+            if (oix != kNone && (bix == kNone || og > bg || (og == bg && oid < bid))) {
+                bg = og;
+                bid = oid;
+                bix = oix;
             }
         }
-        __syncthreads();
     }
     if (threadIdx.x == 0) {
         bool const budget = cfg.max_leaves == 0 || ctl->live < static_cast<u32>(cfg.max_leaves);
         s_go = (n > 0 && budget) ? 1u : 0u;
-        s_sel = s_ix[0];
+        s_sel = bix;
         if (!s_go) {
             ctl->done = 1;
             ctl->n_ops = 0;
@@ -762,11 +776,13 @@ __global__ __launch_bounds__(256) void k_lw_select(TreeCtl *__restrict__ ctl, Lw
     }
     __syncthreads();
     if (!s_go) return;
-    Cand const c = cands[s_sel];
-    __syncthreads();
-    u32 const crow = lw_chunk_rows(c.node.cnt);
-    u32 const nch = (c.node.cnt + crow - 1) / crow;
     if (threadIdx.x == 0) {
+        Cand const c = cands[s_sel];
+        u32 const crow = lw_chunk_rows(c.node.cnt);
+        u32 const nch = (c.node.cnt + crow - 1) / crow;
+        s_ccnt = c.node.cnt;
+        s_crow = crow;
+        s_nch = nch;
         // remove from the pool (order inside the pool is irrelevant)
         cands[s_sel] = cands[n - 1];
         lw->n_cand = n - 1;
@@ -816,18 +832,76 @@ __global__ __launch_bounds__(256) void k_lw_select(TreeCtl *__restrict__ ctl, Lw
         ctl->n_ops = 1;
         ctl->n_part_chunks = nch;
     }
+    __syncthreads();
+    u32 const crow = s_crow, nch = s_nch, ccnt = s_ccnt;
     for (u32 k = threadIdx.x; k < nch; k += blockDim.x) {
         PartChunk pc;
         pc.op = 0;
         pc.begin = k * crow;
-        pc.end = min(c.node.cnt, (k + 1) * crow);
+        pc.end = min(ccnt, (k + 1) * crow);
         pc.left_before = 0;
         pch[k] = pc;
     }
-    if (COUNTED) {
-        __syncthreads();
-        lw_expand_units(s_fo, s_units, s_urows, hch);
+    if (COUNTED) lw_expand_units(s_fo, s_units, s_urows, hch);
+}
+
+template <bool COUNTED>
+__global__ __launch_bounds__(256) void k_lw_select(TreeCtl *__restrict__ ctl, LwCtl *__restrict__ lw,
+                                                   Cand *__restrict__ cands, NodeInfo *__restrict__ kids,
+                                                   SplitRec const *__restrict__ kid_splits,
+                                                   u32 *__restrict__ free_slots, NodeInfo *__restrict__ parent_out,
+                                                   SplitRec *__restrict__ split_out, DevNode *__restrict__ tree,
+                                                   PartOp *__restrict__ ops, PartChunk *__restrict__ pch,
+                                                   HistChunk *__restrict__ hch, u16 const *__restrict__ n_bins,
+                                                   FinishOp *__restrict__ fops, GrowCfg cfg,
+                                                   double const *__restrict__ scales, u32 unit_target)
+{
+    lw_select_round<COUNTED>(ctl, lw, cands, kids, kid_splits, free_slots, parent_out, split_out, tree, ops, pch, hch,
+                             n_bins, fops, cfg, scales, unit_target);
+}
+
+// Split find of a round's children (k_find's body) whose last workgroup to finish -- all
+// others have stopped reading anything the select writes -- starts the next round
+// (next != 0).
+struct LwNext {
+    Cand *cands;
+    u32 *free_slots;
+    NodeInfo *parent_out;
+    SplitRec *split_out;
+    DevNode *tree;
+    PartOp *ops;
+    PartChunk *pch;
+    HistChunk *hch;
+    FinishOp *fops;
+    u32 unit_target;
+    u32 next;
+};
+
+template <bool Q, bool COUNTED>
+__global__ __launch_bounds__(kFindThreads) void k_lw_find(NodeInfo *__restrict__ kids, TreeCtl *__restrict__ ctl,
+                                                        LwCtl *__restrict__ lw, void const *__restrict__ hist,
+                                                        u16 const *__restrict__ n_bins, u32 F, u32 SB, GrowCfg cfg,
+                                                        double const *__restrict__ scales,
+                                                        SplitRec *__restrict__ splits, SplitRec *__restrict__ part,
+                                                        u32 *__restrict__ done, u32 const *__restrict__ cnt, LwNext nx)
+{
+    __shared__ u32 s_last_wg;
+    find_node<Q>(kids, ctl, hist, n_bins, F, SB, cfg, scales, splits, 0u, 0u, part, done, cnt);
+    if (!nx.next) return;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        s_last_wg = atomicAdd(&lw->grid_done, 1u) == gridDim.x * gridDim.y - 1u ? 1u : 0u;
     }
+    __syncthreads();
+    if (!s_last_wg) return;
+    if (threadIdx.x == 0) {
+        lw->grid_done = 0u;
+        __threadfence();
+    }
+    __syncthreads();
+    lw_select_round<COUNTED>(ctl, lw, nx.cands, kids, splits, nx.free_slots, nx.parent_out, nx.split_out, nx.tree,
+                             nx.ops, nx.pch, nx.hch, n_bins, nx.fops, cfg, scales, nx.unit_target);
 }
 
 // Round middle of the FP32 grower: left count from the chunk prefix, then the children
@@ -949,6 +1023,7 @@ __global__ __launch_bounds__(256) void k_lw_begin(TreeCtl *__restrict__ ctl, LwC
         lw->n_cand = 0;
         lw->n_free = n_slots - 1;
         lw->n_push = 0;
+        lw->grid_done = 0;
     }
 }
 
@@ -992,7 +1067,24 @@ void ext_free(ExtState &s)
     s = ExtState{};
 }
 
-hipError_t ext_set_limits() { return hipSuccess; }
+// The leafwise finders use the device's full per-workgroup LDS like k_find.
+hipError_t ext_set_limits()
+{
+    int dev = 0, max_shared = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&max_shared, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return e;
+    void const *fns[] = {reinterpret_cast<void const *>(&k_lw_find<true, true>),
+                         reinterpret_cast<void const *>(&k_lw_find<false, false>)};
+    for (void const *fn : fns) {
+        hipFuncAttributes a{};
+        if (hipFuncGetAttributes(&a, fn) != hipSuccess) continue;
+        int const dyn = max_shared - static_cast<int>(a.sharedSizeBytes);
+        if (dyn > 0) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+    }
+    (void)hipGetLastError();
+    return hipSuccess;
+}
 
 // ObliviousGrower::grow as a launch sequence (fixed shape: every level is enqueued, a
 // stopped tree turns the remaining levels into no-ops).
@@ -1057,6 +1149,15 @@ void enqueue_levelwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
 // LeafwiseGrower::grow as a launch sequence: the root, then max_leaves - 1 rounds
 // (plus a margin for demoted splits, which spend a round without adding a leaf); rounds
 // after the heap empties or the budget is spent are no-ops.
+template <bool Q, bool COUNTED>
+void launch_lw_find(hipStream_t st, GrowView const &v, LwCtl *lw, u32 *cnt, LwNext const &nx)
+{
+    size_t const lds = static_cast<size_t>(kFindTile) * v.SB * (Q ? (cnt ? 36 : 32) : 24);
+    dim3 const grid(find_blocks_per_node(v.F), 2);
+    hipLaunchKernelGGL((k_lw_find<Q, COUNTED>), grid, dim3(kFindThreads), lds, st, v.front[0], v.ctl, lw, v.hist[0],
+                       v.d_nbins, v.F, v.SB, v.gc, v.scales, v.splits, v.fscr.part, v.fscr.done, cnt, nx);
+}
+
 void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
 {
     u32 const gh_bytes = q ? 4 : 8;
@@ -1078,21 +1179,24 @@ void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
     launch_find(st, q, kids, v.ctl, pool, v.d_nbins, v.F, v.SB, v.gc, v.scales, v.splits, 2, 0, 0, v.fscr, cnt);
     hipLaunchKernelGGL(k_lw_push_root, dim3(1), dim3(64), 0, st, lw, kids, v.splits, s.cands, s.free_slots, v.tree,
                        v.gc, v.scales);
+    // The first round's select is a kernel of its own; every later one runs in the last
+    // workgroup of the previous round's split find.
+    if (q)
+        hipLaunchKernelGGL(k_lw_select<true>, dim3(1), dim3(256), 0, st, v.ctl, lw, s.cands, kids, v.splits,
+                           s.free_slots, parent, psplit, v.tree, v.ops, v.pch, v.hch, v.d_nbins, v.fops, v.gc, v.scales,
+                           v.lw_units);
+    else
+        hipLaunchKernelGGL(k_lw_select<false>, dim3(1), dim3(256), 0, st, v.ctl, lw, s.cands, kids, v.splits,
+                           s.free_slots, parent, psplit, v.tree, v.ops, v.pch, v.hch, v.d_nbins, v.fops, v.gc, v.scales,
+                           0u);
     u32 const rounds = static_cast<u32>(v.fc.max_leaves) - 1u + 8u;
     for (u32 r = 0; r < rounds; ++r) {
+        launch_part_count(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.Fs, v.chunk_left, pgrid);
         if (q) {
-            hipLaunchKernelGGL(k_lw_select<true>, dim3(1), dim3(256), 0, st, v.ctl, lw, s.cands, kids, v.splits,
-                               s.free_slots, parent, psplit, v.tree, v.ops, v.pch, v.hch, v.d_nbins, v.fops, v.gc,
-                               v.scales, v.lw_units);
-            launch_part_count(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.Fs, v.chunk_left, pgrid);
             launch_part_scatter(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1],
                                 v.gh[2], v.work[0], v.work[1], const_cast<void *>(v.gh[0]),
                                 const_cast<void *>(v.gh[1]), v.Fs, gh_bytes, pgrid, v.chunk_left);
         } else {
-            hipLaunchKernelGGL(k_lw_select<false>, dim3(1), dim3(256), 0, st, v.ctl, lw, s.cands, kids, v.splits,
-                               s.free_slots, parent, psplit, v.tree, v.ops, v.pch, v.hch, v.d_nbins, v.fops, v.gc,
-                               v.scales, 0u);
-            launch_part_count(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.Fs, v.chunk_left, pgrid);
             hipLaunchKernelGGL(k_lw_scan<false>, dim3(1), dim3(1024), 0, st, v.ops, v.pch, v.chunk_left, v.ctl, lw,
                                parent, psplit, s.free_slots, kids, v.tree, v.hch, v.fops, v.gc, v.scales);
             launch_part_scatter(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1],
@@ -1102,7 +1206,10 @@ void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
         launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
                     v.fsl, v.waves, hgrid, v.partials, pool, cnt);
         launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1, cnt, cnt);
-        launch_find(st, q, kids, v.ctl, pool, v.d_nbins, v.F, v.SB, v.gc, v.scales, v.splits, 2, 0, 0, v.fscr, cnt);
+        LwNext const nx{s.cands, s.free_slots, parent, psplit, v.tree, v.ops, v.pch, v.hch, v.fops,
+                        q ? v.lw_units : 0u, r + 1 < rounds ? 1u : 0u};
+        if (q) launch_lw_find<true, true>(st, v, lw, cnt, nx);
+        else launch_lw_find<false, false>(st, v, lw, cnt, nx);
         if (v.host_sync) (void)hipStreamSynchronize(st);
     }
     hipLaunchKernelGGL(k_lw_finalize, dim3(1), dim3(256), 0, st, v.ctl, lw, s.cands, kids, v.splits, s.free_slots,
