@@ -40,9 +40,9 @@ constexpr u32 kHistGroupMax = 4;
 constexpr u32 kPartChunk  = 4096;  // rows per partition work unit
 // Leafwise rounds partition one leaf and build one child's histogram; the work units are
 // sized to the leaf so that a small leaf is still spread over many workgroups: partition
-// chunks of kLwMinChunk..kPartChunk rows, about kLwChunkTarget of them, and (quantized:
-// exact integer sums, any unit size gives the same histogram) histogram units of
-// kLwMinUnit..kHistChunk rows, at most the context's unit target.
+// chunks of kLwMinChunk..kPartChunk rows, about kLwChunkTarget of them, and histogram
+// units of kLwMinUnit..kHistChunk rows, at most the context's unit target (quantized sums
+// are exact in any grouping; FP32 sums follow the units, a rule the oracle restates).
 constexpr u32 kLwChunkTarget = 1024;
 constexpr u32 kLwMinChunk = 256;
 constexpr u32 kLwUnitTarget = 256;

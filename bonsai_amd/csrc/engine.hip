@@ -213,15 +213,13 @@ int alloc_fit(bh_context *c)
     c->lw_units = 0;
     if (f.grower == BH_LEAFWISE) {
         // Leaf-sized partition chunks (at most kLwChunkTarget beyond the full-size ones)
-        // and, quantized, up to lw_units histogram units per child, their partials within
-        // about 256 MiB.
+        // and up to lw_units histogram units per built node, their partials within about
+        // 256 MiB.
         c->max_pch += kLwChunkTarget;
-        if (c->quant) {
-            size_t const per = std::max<size_t>(1, nc * c->part_bytes());
-            c->lw_units = static_cast<u32>(std::min<size_t>(kLwUnitTarget, std::max<size_t>(16, (256ull << 20) / per)));
-            c->max_hch = std::max(c->max_hch, c->lw_units + 2u);
-            c->max_parts = std::max(c->max_parts, c->lw_units + 2u);
-        }
+        size_t const per = std::max<size_t>(1, nc * c->part_bytes());
+        c->lw_units = static_cast<u32>(std::min<size_t>(kLwUnitTarget, std::max<size_t>(16, (256ull << 20) / per)));
+        c->max_hch = std::max(c->max_hch, c->lw_units + 2u);
+        c->max_parts = std::max(c->max_parts, c->lw_units + 2u);
     }
     size_t const hist_bytes = static_cast<size_t>(c->hist_slots) * nc * c->cell_bytes();
     if (c->quant) {

@@ -529,9 +529,10 @@ __device__ __forceinline__ u32 lw_chunk_rows(u32 rows)
     return min(kPartChunk, max(kLwMinChunk, per));
 }
 
-// Rows per histogram unit of a child of `rows` rows: at most `target` units of
-// kLwMinUnit..kHistChunk rows (multiples of 256); target 0 keeps kHistChunk (the FP32
-// summation order is defined on kHistChunk units).
+// Rows per histogram unit of a built node of `rows` rows: at most `target` units of
+// kLwMinUnit..kHistChunk rows (multiples of 256); target 0 keeps kHistChunk.  FP32 sums
+// run in ascending rows within a unit and over units in order, so this rule is part of
+// the FP32 leafwise summation order (the oracle restates it).
 __device__ __forceinline__ u32 lw_unit_rows(u32 rows, u32 target)
 {
     if (target == 0) return kHistChunk;
@@ -913,7 +914,7 @@ __global__ __launch_bounds__(1024) void k_lw_scan(PartOp *__restrict__ ops, Part
                                                   SplitRec const *__restrict__ split_in, u32 *__restrict__ free_slots,
                                                   NodeInfo *__restrict__ kids, DevNode *__restrict__ tree,
                                                   HistChunk *__restrict__ hch, FinishOp *__restrict__ fops,
-                                                  GrowCfg cfg, double const *__restrict__ scales)
+                                                  GrowCfg cfg, double const *__restrict__ scales, u32 unit_target)
 {
     __shared__ u32 scratch[1024];
     __shared__ u32 s_run, s_units, s_urows, s_fo[4];
@@ -936,7 +937,7 @@ __global__ __launch_bounds__(1024) void k_lw_scan(PartOp *__restrict__ ops, Part
         FinishOp fo{};
         u32 urows = kHistChunk;
         s_units = lw_children<Q>(op, s_run, ctl, lw, parent_in[0], split_in[0], free_slots, kids, tree, cfg, scales,
-                                 false, 0u, fo, urows);
+                                 false, unit_target, fo, urows);
         ops[0] = op;
         lw_stash_units(s_fo, fo);
         s_urows = urows;
@@ -980,17 +981,19 @@ __global__ void k_lw_finalize(TreeCtl *__restrict__ ctl, LwCtl *__restrict__ lw,
 __global__ __launch_bounds__(256) void k_lw_begin(TreeCtl *__restrict__ ctl, LwCtl *__restrict__ lw,
                                                   NodeInfo *__restrict__ kids, DevNode *__restrict__ tree,
                                                   HistChunk *__restrict__ hch, FinishOp *__restrict__ fops,
-                                                  u32 *__restrict__ free_slots, u32 n_slots, RootSel root)
+                                                  u32 *__restrict__ free_slots, u32 n_slots, RootSel root,
+                                                  u32 unit_target)
 {
     u32 const n_rows = root.n_from_ctl ? ctl->root_n : root.n;
-    u32 const n_units = (n_rows + kHistChunk - 1) / kHistChunk;
+    u32 const urows = lw_unit_rows(n_rows, unit_target);
+    u32 const n_units = (n_rows + urows - 1) / urows;
     u32 const multi = n_units > 1;
     for (u32 k = threadIdx.x; k < n_units; k += blockDim.x) {
         HistChunk hc{};
         hc.slot = 0;
         hc.src = root.src;
-        hc.begin = k * kHistChunk;
-        hc.end = min(n_rows, (k + 1) * kHistChunk);
+        hc.begin = k * urows;
+        hc.end = min(n_rows, (k + 1) * urows);
         hc.part = multi ? k : kNone;
         hch[k] = hc;
     }
@@ -1172,7 +1175,7 @@ void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
     u32 const pgrid = std::max(1u, std::min(v.max_pch, tuning().part_grid));
     u32 const hgrid = std::max(1u, std::min(v.max_hch, 4096u));
     hipLaunchKernelGGL(k_lw_begin, dim3(1), dim3(256), 0, st, v.ctl, lw, kids, v.tree, v.hch, v.fops, s.free_slots,
-                       s.hist_slots, v.root);
+                       s.hist_slots, v.root, v.lw_units);
     launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
                 v.fsl, v.waves, hgrid, v.partials, pool, cnt);
     launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1, cnt, cnt);
@@ -1198,7 +1201,7 @@ void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
                                 const_cast<void *>(v.gh[1]), v.Fs, gh_bytes, pgrid, v.chunk_left);
         } else {
             hipLaunchKernelGGL(k_lw_scan<false>, dim3(1), dim3(1024), 0, st, v.ops, v.pch, v.chunk_left, v.ctl, lw,
-                               parent, psplit, s.free_slots, kids, v.tree, v.hch, v.fops, v.gc, v.scales);
+                               parent, psplit, s.free_slots, kids, v.tree, v.hch, v.fops, v.gc, v.scales, v.lw_units);
             launch_part_scatter(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1],
                                 v.gh[2], v.work[0], v.work[1], const_cast<void *>(v.gh[0]),
                                 const_cast<void *>(v.gh[1]), v.Fs, gh_bytes, pgrid);
@@ -1206,8 +1209,8 @@ void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
         launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
                     v.fsl, v.waves, hgrid, v.partials, pool, cnt);
         launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1, cnt, cnt);
-        LwNext const nx{s.cands, s.free_slots, parent, psplit, v.tree, v.ops, v.pch, v.hch, v.fops,
-                        q ? v.lw_units : 0u, r + 1 < rounds ? 1u : 0u};
+        LwNext const nx{s.cands, s.free_slots, parent, psplit, v.tree, v.ops, v.pch, v.hch, v.fops, v.lw_units,
+                        r + 1 < rounds ? 1u : 0u};
         if (q) launch_lw_find<true, true>(st, v, lw, cnt, nx);
         else launch_lw_find<false, false>(st, v, lw, cnt, nx);
         if (v.host_sync) (void)hipStreamSynchronize(st);

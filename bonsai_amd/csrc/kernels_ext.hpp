@@ -32,7 +32,7 @@ struct GrowView {
     u32 max_pch, max_hch, max_ops, fsl, waves;
     RootSel root;  // the tree's root rows
     u32 *cnt;      // quantized leafwise: row count per (slot, feature, bin) beside the pool
-    u32 lw_units;  // quantized leafwise: histogram units per child at most (0: kHistChunk units)
+    u32 lw_units;  // leafwise: histogram units per built node at most
     bool host_sync;
 };
 

@@ -27,6 +27,7 @@ NODE_DTYPE = np.dtype([
 HIST_F32_SERIAL, HIST_F32_CHUNKED, HIST_INT, HIST_F32_OMP = 0, 1, 2, 3
 # Rows per device histogram work unit: the FP32 chunked order the device sums in.
 DEVICE_CHUNK = 8192
+CHUNK_ADAPTIVE = 0x80000000  # | unit target T: leaf-sized FP32 partials (oracle.h)
 
 
 class TreeCfg(C.Structure):

@@ -539,6 +539,16 @@ struct Engine {
 
     size_t cells() const { return static_cast<size_t>(ds.n_features) * ds.bin_stride * 2; }
 
+    // Rows per partial for a node of n rows: the fixed chunk, or (ORC_CHUNK_ADAPTIVE) the
+    // leaf-sized unit rule.
+    u32 chunk_for(u64 n) const
+    {
+        if (!(chunk & ORC_CHUNK_ADAPTIVE)) return chunk;
+        u64 const target = chunk & ~ORC_CHUNK_ADAPTIVE;
+        u64 const per = ((n + target - 1) / target + 255) / 256 * 256;
+        return static_cast<u32>(std::min<u64>(kDeviceChunk, std::max<u64>(1024, per)));
+    }
+
     void fill(NodeState &n) const
     {
         if (mode == ORC_HIST_INT) {
@@ -548,7 +558,7 @@ struct Engine {
         } else {
             n.hf.assign(cells(), 0.0F);
             if (mode == ORC_HIST_F32_CHUNKED)
-                fill_f32_chunked(ds, gr, n.rows.data(), n.rows.size(), chunk, n.hf.data());
+                fill_f32_chunked(ds, gr, n.rows.data(), n.rows.size(), chunk_for(n.rows.size()), n.hf.data());
             else if (mode == ORC_HIST_F32_OMP)
                 fill_f32_omp(n.rows.data(), n.rows.size(), n.hf.data());
             else

@@ -131,7 +131,11 @@ uint64_t orc_partition(const uint8_t *plane, uint64_t row_stride, const uint32_t
 
 /* ---- growers: src/grower_impl.hpp ---- */
 /* Returns #nodes written (<= cap) or -1 on error.  hist_mode picks the cell type; for
- * ORC_HIST_INT gq/hq and the scales are read, otherwise g/h. */
+ * ORC_HIST_INT gq/hq and the scales are read, otherwise g/h.  chunk: rows per
+ * ORC_HIST_F32_CHUNKED partial (0: 8192), or ORC_CHUNK_ADAPTIVE | T: each built node of
+ * n rows uses ceil(ceil(n / T) / 256) * 256 rows per partial, clamped to [1024, 8192]
+ * (the device's leafwise work units with unit target T). */
+#define ORC_CHUNK_ADAPTIVE 0x80000000u
 int32_t orc_grow_depthwise(const uint8_t *plane, uint64_t row_stride, uint64_t n_rows,
                            uint32_t n_features, const uint16_t *n_bins,
                            const float *g, const float *h, const int8_t *gq,

@@ -6,10 +6,14 @@ against the oracle restating the device's fixed summation order.
 import numpy as np
 import pytest
 
-from oracle_lib import HIST_F32_CHUNKED, HIST_INT, cfg as orc_cfg
+from oracle_lib import CHUNK_ADAPTIVE, HIST_F32_CHUNKED, HIST_INT, cfg as orc_cfg
 from test_gpu_depthwise import assert_same_tree, ctx_for, grad_of, tall_data
 
 pytestmark = pytest.mark.gpu
+
+# Leafwise FP32 histograms sum over leaf-sized units (at most 256 per built node at these
+# shapes); the oracle follows the same rule.
+LW_F32_CHUNK = CHUNK_ADAPTIVE | 256
 
 
 @pytest.fixture(scope="module")
@@ -54,7 +58,7 @@ def test_grow_leafwise_f32_matches_device_order_oracle(hip, orc, tall, leaves, d
     ctx.set_gradients(g, h)
     nodes, lor = ctx.grow_tree()
     ref, rlor = orc.grow_leafwise(bins, nb, g=g, h=h, c=orc_cfg(max_depth=depth, max_leaves=leaves),
-                                  mode=HIST_F32_CHUNKED)
+                                  mode=HIST_F32_CHUNKED, chunk=LW_F32_CHUNK)
     assert_same_tree(nodes, ref)
     assert np.array_equal(lor, rlor)
 
@@ -79,7 +83,7 @@ def test_grow_leafwise_600k_rows(hip, orc, tall600k, quantized, leaves, depth):
         ref, rlor = orc.grow_leafwise(bins, nb, gq=gq, hq=hq, scales=sc, c=c, mode=HIST_INT)
     else:
         ctx.set_gradients(g, h)
-        ref, rlor = orc.grow_leafwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_CHUNKED)
+        ref, rlor = orc.grow_leafwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_CHUNKED, chunk=LW_F32_CHUNK)
     nodes, lor = ctx.grow_tree()
     ctx.close()
     assert_same_tree(nodes, ref)

@@ -10,6 +10,7 @@ import pytest
 
 from oracle_lib import HIST_F32_CHUNKED, HIST_INT, cfg as orc_cfg
 from test_gpu_depthwise import assert_same_tree, ctx_for, grad_of, tall_data
+from test_gpu_growers import LW_F32_CHUNK
 
 pytestmark = pytest.mark.gpu
 
@@ -43,6 +44,8 @@ def oracle_tree(orc, grower, bins, nb, rows, g, h, gq, hq, sc, quantized, c):
     fn = getattr(orc, GROWERS[grower])
     if quantized:
         return fn(bins[rows], nb, gq=gq[rows], hq=hq[rows], scales=sc, c=c, mode=HIST_INT)
+    if grower == "hip_leafwise":
+        return fn(bins[rows], nb, g=g[rows], h=h[rows], c=c, mode=HIST_F32_CHUNKED, chunk=LW_F32_CHUNK)
     return fn(bins[rows], nb, g=g[rows], h=h[rows], c=c, mode=HIST_F32_CHUNKED)
 
 

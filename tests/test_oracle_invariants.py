@@ -9,7 +9,8 @@ import math
 import numpy as np
 import pytest
 
-from oracle_lib import (HIST_F32_CHUNKED, HIST_F32_SERIAL, HIST_INT, NODE_DTYPE, P, TreeCfg, cfg as orc_cfg)
+from oracle_lib import (CHUNK_ADAPTIVE, HIST_F32_CHUNKED, HIST_F32_SERIAL, HIST_INT, NODE_DTYPE, P, TreeCfg,
+                        cfg as orc_cfg)
 
 
 def small_plane(n=3000, f=5, nb=12, seed=0, miss=0.05):
@@ -158,6 +159,25 @@ def test_depthwise_int_equals_f64_when_exact(orc):
         assert np.array_equal(a["bin"][internal], t["bin"][internal])
         assert np.array_equal(a["value"], t["value"])
     assert np.array_equal(la, lb) and np.array_equal(la, lq)
+
+
+def test_leafwise_adaptive_chunks_keep_every_row(orc):
+    """The leaf-sized FP32 units (ORC_CHUNK_ADAPTIVE: per built node, ceil(n / T) rows
+    rounded up to 256, clamped to [1024, 8192]) regroup the float sums only: with integer
+    gradients every sum is exact, so any target must give the serial grower's tree (a row
+    dropped or counted twice by the chunking would change a sum or a cover)."""
+    bins, nb = small_plane(n=30000, seed=18)
+    rng = np.random.default_rng(19)
+    gq = rng.integers(-127, 128, bins.shape[0]).astype(np.int8)
+    hq = rng.integers(1, 200, bins.shape[0]).astype(np.int16)
+    g = gq.astype(np.float32)
+    h = hq.astype(np.float32)
+    c = orc_cfg(max_depth=8, min_data=5, max_leaves=40)
+    a, la = orc.grow_leafwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_SERIAL)
+    for target in (1, 4, 256):
+        b, lb = orc.grow_leafwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_CHUNKED, chunk=CHUNK_ADAPTIVE | target)
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+        assert np.array_equal(la, lb)
 
 
 def leaf_groups(lor):
