@@ -38,6 +38,9 @@ constexpr u64 kPackCntMask = (1ull << 14) - 1ull;
 // cells); the group size is a tuning knob (1..kHistGroupMax).
 constexpr u32 kHistGroupMax = 4;
 constexpr u32 kPartChunk  = 4096;  // rows per partition work unit
+// Rows wider than 64 bytes: partition chunks shrink (powers of two, at least 64 rows) to
+// about this many bytes, so a level of wide rows still has many chunks.
+constexpr u64 kPartChunkBytes = 256u * 1024u;
 // Leafwise rounds partition one leaf and build one child's histogram; the work units are
 // sized to the leaf so that a small leaf is still spread over many workgroups: partition
 // chunks of kLwMinChunk..kPartChunk rows, about kLwChunkTarget of them, and histogram

@@ -204,7 +204,7 @@ int alloc_fit(bh_context *c)
     u32 const depth = static_cast<u32>(f.max_depth);
     c->max_front = f.grower == BH_LEAFWISE ? 2u : (1u << std::min(depth, 14u));
     c->max_ops = std::max(1u, c->max_front);
-    c->max_pch = N / kPartChunk + c->max_ops + 2;
+    c->max_pch = N / part_chunk_rows(c->Fs) + c->max_ops + 2;
     c->max_hch = N / kHistChunk + c->max_ops + 2;
     c->max_parts = N / kHistChunk + c->max_ops + 2;
     c->hist_slots = f.grower == BH_LEAFWISE ? static_cast<u32>(f.max_leaves) + 4u : std::max(1u, c->max_front / 2u);
@@ -428,7 +428,7 @@ void enqueue_depthwise_counted(bh_context *c)
         launch_find(s, true, cur, c->ctl, hcur, c->d_nbins, c->F, c->SB, c->gc, c->scales, c->splits, width, 0, 0,
                     c->fscr, ccur);
         launch_plan_counted(s, cur, c->splits, c->ctl, c->tree, c->ops, c->pch, nxt, c->hch, c->fops, c->d_nbins,
-                            c->gc, c->scales, dst, last ? 1u : 0u, kPartChunk, mode, c->ops4,
+                            c->gc, c->scales, dst, last ? 1u : 0u, part_chunk_rows(c->Fs), mode, c->ops4,
                             mode == 1 ? unit_rows : kHistChunk);
         if (c->fc.host_sync) host_sync_point(c);
         if (!last) {
@@ -480,7 +480,8 @@ void enqueue_depthwise(bh_context *c)
         bool const last = d + 1 == D;
         launch_find(s, c->quant, cur, c->ctl, hcur, c->d_nbins, c->F, c->SB, c->gc, c->scales, c->splits, width, 0, 0,
                     c->fscr);
-        launch_plan_level(s, cur, c->splits, c->ctl, c->tree, c->ops, c->pch, c->d_nbins, c->gc, c->scales, d & 1u);
+        launch_plan_level(s, cur, c->splits, c->ctl, c->tree, c->ops, c->pch, c->d_nbins, c->gc, c->scales, d & 1u,
+                          part_chunk_rows(c->Fs));
         if (c->fc.host_sync) host_sync_point(c);
         launch_part_count(s, c->pch, c->ops, c->ctl, c->work[0], c->work[1], c->plane0, c->Fs, c->chunk_left,
                           part_grid(c));
@@ -1405,7 +1406,7 @@ int bh_partition(bh_context *c, const uint32_t *rows, uint64_t n, uint32_t featu
     HIP_TRY(hipMemcpy(c->tree, &root, sizeof(root), hipMemcpyHostToDevice));
     GrowCfg gc = c->gc;
     launch_plan_level(c->stream, c->front[0], c->splits, c->ctl, c->tree, c->ops, c->pch, c->d_nbins, gc, c->scales,
-                      1);
+                      1, part_chunk_rows(c->Fs));
     launch_part_count(c->stream, c->pch, c->ops, c->ctl, c->work[0], c->work[1], c->plane0, c->Fs, c->chunk_left,
                       part_grid(c));
     // oblivious = 1: no demotion, so an all-one-side split still scatters.

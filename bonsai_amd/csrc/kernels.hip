@@ -1053,7 +1053,8 @@ __global__ __launch_bounds__(1024) void k_plan_level(NodeInfo const *__restrict_
                                                      TreeCtl *__restrict__ ctl, DevNode *__restrict__ tree,
                                                      PartOp *__restrict__ ops, PartChunk *__restrict__ pch,
                                                      u16 const *__restrict__ n_bins, GrowCfg cfg,
-                                                     double const *__restrict__ scales, u32 dst)
+                                                     double const *__restrict__ scales, u32 dst,
+                                                     u32 chunk_rows)
 {
     __shared__ u32 scratch[1024];
     __shared__ u32 s_base_nodes, s_ops, s_chunks;
@@ -1071,7 +1072,7 @@ __global__ __launch_bounds__(1024) void k_plan_level(NodeInfo const *__restrict_
         u32 tot;
         u32 const rank = block_exscan(valid, scratch, &tot) + s_ops;
         u32 nch = 0;
-        if (in && valid) nch = (front[i].cnt + kPartChunk - 1) / kPartChunk;
+        if (in && valid) nch = (front[i].cnt + chunk_rows - 1) / chunk_rows;
         u32 ctot;
         u32 const cfirst = block_exscan(nch, scratch, &ctot) + s_chunks;
         if (in) {
@@ -1127,8 +1128,8 @@ __global__ __launch_bounds__(1024) void k_plan_level(NodeInfo const *__restrict_
         u32 const k = c - op.chunk_first;
         PartChunk pc;
         pc.op = j;
-        pc.begin = k * kPartChunk;
-        pc.end = min(op.cnt, (k + 1) * kPartChunk);
+        pc.begin = k * chunk_rows;
+        pc.end = min(op.cnt, (k + 1) * chunk_rows);
         pc.left_before = 0;
         pch[c] = pc;
     }
@@ -1350,6 +1351,8 @@ __device__ __forceinline__ bool chunk_fits(PartOp const &op, PartChunk const &ch
            (ch.begin - left_before) + (rows - n_left_chunk) <= op.cnt - op.n_left;
 }
 
+constexpr u32 kCopyRows = 8;
+
 __global__ __launch_bounds__(256) void k_part_scatter(
     PartChunk const *__restrict__ pch, PartOp const *__restrict__ ops, TreeCtl const *__restrict__ ctl,
     u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
@@ -1398,13 +1401,25 @@ __global__ __launch_bounds__(256) void k_part_scatter(
             if (r < rows) dest[r] = ((bits >> k) & 1u) ? lpos++ : rpos++;
         }
         __syncthreads();
-        // Row bytes: tpr threads per row, dword granularity.
+        // Row bytes: tpr threads per row, dword granularity, kCopyRows rows' loads in
+        // flight per thread (a wide row is a long latency chain otherwise).
         u32 const rows_per_pass = 256u >> tpr_log2;
-        u32 const sub = threadIdx.x & (tpr - 1u);
-        for (u32 r = threadIdx.x >> tpr_log2; r < rows; r += rows_per_pass) {
-            u32 const *src = reinterpret_cast<u32 const *>(plane + static_cast<size_t>(op.off + ch.begin + r) * Fs);
-            u32 *dstp = reinterpret_cast<u32 *>(oplane + static_cast<size_t>(dest[r]) * Fs);
-            for (u32 dw = sub; dw < D; dw += tpr) dstp[dw] = src[dw];
+        u32 const sub = threadIdx.x & (tpr - 1u), rsub = threadIdx.x >> tpr_log2;
+        size_t const sbase = static_cast<size_t>(op.off) + ch.begin;
+        for (u32 r0 = 0; r0 < rows; r0 += rows_per_pass * kCopyRows) {
+            for (u32 dw = sub; dw < D; dw += tpr) {
+                u32 v[kCopyRows];
+#pragma unroll
+                for (u32 u = 0; u < kCopyRows; ++u) {
+                    u32 const r = r0 + rsub + u * rows_per_pass;
+                    v[u] = r < rows ? reinterpret_cast<u32 const *>(plane + (sbase + r) * Fs)[dw] : 0u;
+                }
+#pragma unroll
+                for (u32 u = 0; u < kCopyRows; ++u) {
+                    u32 const r = r0 + rsub + u * rows_per_pass;
+                    if (r < rows) reinterpret_cast<u32 *>(oplane + static_cast<size_t>(dest[r]) * Fs)[dw] = v[u];
+                }
+            }
         }
         // Gradient pairs.
         for (u32 r = threadIdx.x; r < rows; r += 256u) {
@@ -2652,10 +2667,18 @@ void launch_find(hipStream_t s, bool q, NodeInfo *front, TreeCtl const *ctl, voi
 
 void launch_plan_level(hipStream_t s, NodeInfo const *front, SplitRec const *splits, TreeCtl *ctl, DevNode *tree,
                        PartOp *ops, PartChunk *pch, u16 const *n_bins, GrowCfg const &cfg, double const *scales,
-                       u32 dst)
+                       u32 dst, u32 chunk_rows)
 {
     hipLaunchKernelGGL(k_plan_level, dim3(1), dim3(1024), 0, s, front, splits, ctl, tree, ops, pch, n_bins, cfg,
-                       scales, dst);
+                       scales, dst, chunk_rows);
+}
+
+u32 part_chunk_rows(u32 Fs)
+{
+    if (Fs <= 64) return kPartChunk;
+    u32 r = kPartChunk;
+    while (r > 64u && static_cast<u64>(r) * Fs > kPartChunkBytes) r >>= 1;
+    return r;
 }
 
 void launch_part_count(hipStream_t s, PartChunk const *pch, PartOp const *ops, TreeCtl const *ctl, u8 const *pl0,

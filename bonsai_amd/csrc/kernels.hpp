@@ -31,7 +31,9 @@ void launch_find(hipStream_t s, bool q, NodeInfo *front, TreeCtl const *ctl, voi
 u32 find_blocks_per_node(u32 F);
 void launch_plan_level(hipStream_t s, NodeInfo const *front, SplitRec const *splits, TreeCtl *ctl, DevNode *tree,
                        PartOp *ops, PartChunk *pch, u16 const *n_bins, GrowCfg const &cfg, double const *scales,
-                       u32 dst);
+                       u32 dst, u32 chunk_rows);
+// Rows per partition chunk for rows of Fs bytes (kPartChunk, smaller for wide rows).
+u32 part_chunk_rows(u32 Fs);
 void launch_part_count(hipStream_t s, PartChunk const *pch, PartOp const *ops, TreeCtl const *ctl, u8 const *pl0,
                        u8 const *pl1, u8 const *plp, u32 Fs, u32 *chunk_left, u32 grid);
 void launch_part_scan(hipStream_t s, bool q, PartOp *ops, PartChunk *pch, u32 const *chunk_left, TreeCtl *ctl,

@@ -345,7 +345,7 @@ __global__ __launch_bounds__(1024) void k_plan_oblivious(NodeInfo const *__restr
                                                          void const *__restrict__ hist, u16 const *__restrict__ n_bins,
                                                          u32 F, u32 SB, SplitRec *__restrict__ splits,
                                                          PartOp *__restrict__ ops, PartChunk *__restrict__ pch, u32 dst,
-                                                         u32 is_last)
+                                                         u32 is_last, u32 chunk_rows)
 {
     __shared__ u32 scratch[1024];
     __shared__ u32 s_chunks;
@@ -370,7 +370,7 @@ __global__ __launch_bounds__(1024) void k_plan_oblivious(NodeInfo const *__restr
         NodeInfo nd{};
         if (in) {
             nd = front[i];
-            nch = (nd.cnt + kPartChunk - 1) / kPartChunk;
+            nch = (nd.cnt + chunk_rows - 1) / chunk_rows;
         }
         u32 ctot;
         u32 const cfirst = block_exscan(nch, scratch, &ctot) + s_chunks;
@@ -439,8 +439,8 @@ __global__ __launch_bounds__(1024) void k_plan_oblivious(NodeInfo const *__restr
         u32 const k = c - op.chunk_first;
         PartChunk pc;
         pc.op = lo;
-        pc.begin = k * kPartChunk;
-        pc.end = min(op.cnt, (k + 1) * kPartChunk);
+        pc.begin = k * chunk_rows;
+        pc.end = min(op.cnt, (k + 1) * chunk_rows);
         pc.left_before = 0;
         pch[c] = pc;
     }
@@ -522,11 +522,12 @@ struct LwCtl {
 };
 
 // Rows per partition chunk of a leaf of `rows` rows: about kLwChunkTarget chunks,
-// multiples of 256 rows, kLwMinChunk..kPartChunk.
-__device__ __forceinline__ u32 lw_chunk_rows(u32 rows)
+// multiples of 256 rows, kLwMinChunk..kPartChunk, and at most cap (the byte-sized chunk
+// of wide rows).
+__device__ __forceinline__ u32 lw_chunk_rows(u32 rows, u32 cap)
 {
     u32 const per = ((rows + kLwChunkTarget - 1u) / kLwChunkTarget + 255u) & ~255u;
-    return min(kPartChunk, max(kLwMinChunk, per));
+    return min(cap, min(kPartChunk, max(kLwMinChunk, per)));
 }
 
 // Rows per histogram unit of a built node of `rows` rows: at most `target` units of
@@ -695,7 +696,7 @@ __device__ __forceinline__ void lw_select_round(TreeCtl *__restrict__ ctl, LwCtl
                                 SplitRec *__restrict__ split_out, DevNode *__restrict__ tree, PartOp *__restrict__ ops,
                                 PartChunk *__restrict__ pch, HistChunk *__restrict__ hch,
                                 u16 const *__restrict__ n_bins, FinishOp *__restrict__ fops, GrowCfg const &cfg,
-                                double const *__restrict__ scales, u32 unit_target)
+                                double const *__restrict__ scales, u32 unit_target, u32 chunk_cap)
 {
     __shared__ double s_wg[16];
     __shared__ u32 s_wid[16], s_wix[16];
@@ -779,7 +780,7 @@ __device__ __forceinline__ void lw_select_round(TreeCtl *__restrict__ ctl, LwCtl
     if (!s_go) return;
     if (threadIdx.x == 0) {
         Cand const c = cands[s_sel];
-        u32 const crow = lw_chunk_rows(c.node.cnt);
+        u32 const crow = lw_chunk_rows(c.node.cnt, chunk_cap);
         u32 const nch = (c.node.cnt + crow - 1) / crow;
         s_ccnt = c.node.cnt;
         s_crow = crow;
@@ -855,10 +856,10 @@ __global__ __launch_bounds__(256) void k_lw_select(TreeCtl *__restrict__ ctl, Lw
                                                    PartOp *__restrict__ ops, PartChunk *__restrict__ pch,
                                                    HistChunk *__restrict__ hch, u16 const *__restrict__ n_bins,
                                                    FinishOp *__restrict__ fops, GrowCfg cfg,
-                                                   double const *__restrict__ scales, u32 unit_target)
+                                                   double const *__restrict__ scales, u32 unit_target, u32 chunk_cap)
 {
     lw_select_round<COUNTED>(ctl, lw, cands, kids, kid_splits, free_slots, parent_out, split_out, tree, ops, pch, hch,
-                             n_bins, fops, cfg, scales, unit_target);
+                             n_bins, fops, cfg, scales, unit_target, chunk_cap);
 }
 
 // Split find of a round's children (k_find's body) whose last workgroup to finish -- all
@@ -875,6 +876,7 @@ struct LwNext {
     HistChunk *hch;
     FinishOp *fops;
     u32 unit_target;
+    u32 chunk_cap;
     u32 next;
 };
 
@@ -902,7 +904,7 @@ __global__ __launch_bounds__(kFindThreads) void k_lw_find(NodeInfo *__restrict__
     }
     __syncthreads();
     lw_select_round<COUNTED>(ctl, lw, nx.cands, kids, splits, nx.free_slots, nx.parent_out, nx.split_out, nx.tree,
-                             nx.ops, nx.pch, nx.hch, n_bins, nx.fops, cfg, scales, nx.unit_target);
+                             nx.ops, nx.pch, nx.hch, n_bins, nx.fops, cfg, scales, nx.unit_target, nx.chunk_cap);
 }
 
 // Round middle of the FP32 grower: left count from the chunk prefix, then the children
@@ -1122,10 +1124,11 @@ void enqueue_levelwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
                            s.level_gains, pick);
         if (q)
             hipLaunchKernelGGL(k_plan_oblivious<true>, dim3(1), dim3(1024), 0, st, cur, v.ctl, pick, hcur, v.d_nbins,
-                               v.F, v.SB, v.splits, v.ops, v.pch, d & 1u, last ? 1u : 0u);
+                               v.F, v.SB, v.splits, v.ops, v.pch, d & 1u, last ? 1u : 0u, part_chunk_rows(v.Fs));
         else
             hipLaunchKernelGGL(k_plan_oblivious<false>, dim3(1), dim3(1024), 0, st, cur, v.ctl, pick, hcur,
-                               v.d_nbins, v.F, v.SB, v.splits, v.ops, v.pch, d & 1u, last ? 1u : 0u);
+                               v.d_nbins, v.F, v.SB, v.splits, v.ops, v.pch, d & 1u, last ? 1u : 0u,
+                               part_chunk_rows(v.Fs));
         launch_part_count(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.Fs, v.chunk_left,
                           std::max(1u, std::min(v.max_pch, tuning().part_grid)));
         launch_part_scan(st, q, v.ops, v.pch, v.chunk_left, v.ctl, cur, nxt, v.splits, v.tree, v.hch, v.fops, v.gc,
@@ -1187,11 +1190,11 @@ void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
     if (q)
         hipLaunchKernelGGL(k_lw_select<true>, dim3(1), dim3(256), 0, st, v.ctl, lw, s.cands, kids, v.splits,
                            s.free_slots, parent, psplit, v.tree, v.ops, v.pch, v.hch, v.d_nbins, v.fops, v.gc, v.scales,
-                           v.lw_units);
+                           v.lw_units, part_chunk_rows(v.Fs));
     else
         hipLaunchKernelGGL(k_lw_select<false>, dim3(1), dim3(256), 0, st, v.ctl, lw, s.cands, kids, v.splits,
                            s.free_slots, parent, psplit, v.tree, v.ops, v.pch, v.hch, v.d_nbins, v.fops, v.gc, v.scales,
-                           0u);
+                           0u, part_chunk_rows(v.Fs));
     u32 const rounds = static_cast<u32>(v.fc.max_leaves) - 1u + 8u;
     for (u32 r = 0; r < rounds; ++r) {
         launch_part_count(st, v.pch, v.ops, v.ctl, v.work[0], v.work[1], v.plane0, v.Fs, v.chunk_left, pgrid);
@@ -1210,7 +1213,7 @@ void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
                     v.fsl, v.waves, hgrid, v.partials, pool, cnt);
         launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1, cnt, cnt);
         LwNext const nx{s.cands, s.free_slots, parent, psplit, v.tree, v.ops, v.pch, v.hch, v.fops, v.lw_units,
-                        r + 1 < rounds ? 1u : 0u};
+                        part_chunk_rows(v.Fs), r + 1 < rounds ? 1u : 0u};
         if (q) launch_lw_find<true, true>(st, v, lw, cnt, nx);
         else launch_lw_find<false, false>(st, v, lw, cnt, nx);
         if (v.host_sync) (void)hipStreamSynchronize(st);

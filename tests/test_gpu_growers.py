@@ -194,3 +194,25 @@ def test_train_graph_equals_eager(hip, orc, tall, grower):
     for a, b in zip(ta, tb):
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
     assert np.array_equal(sa.view(np.uint32), sb.view(np.uint32))
+
+
+@pytest.mark.parametrize("quantized", [True, False])
+def test_grow_leafwise_wide_rows(hip, orc, quantized):
+    """1,500-byte rows: partition chunks are capped at 128 rows by bytes, below the
+    leaf-sized rule's 256-row minimum."""
+    bins, nb, X, y = wide_data(n=4000, f=1500)
+    rng = np.random.default_rng(6)
+    g, h = orc.mse_grad(rng.normal(0, 1, y.size).astype(np.float32), y)
+    ctx = ctx_for(hip, bins, nb, grower="hip_leafwise", quantized=quantized, max_depth=8, max_leaves=20,
+                  min_data_in_leaf=5)
+    c = orc_cfg(max_depth=8, max_leaves=20, min_data=5)
+    if quantized:
+        gq, hq, sc = ctx.set_gradients(g, h)
+        ref, rlor = orc.grow_leafwise(bins, nb, gq=gq, hq=hq, scales=sc, c=c, mode=HIST_INT)
+    else:
+        ctx.set_gradients(g, h)
+        ref, rlor = orc.grow_leafwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_CHUNKED, chunk=LW_F32_CHUNK)
+    nodes, lor = ctx.grow_tree()
+    ctx.close()
+    assert_same_tree(nodes, ref)
+    assert np.array_equal(lor, rlor)
