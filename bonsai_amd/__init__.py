@@ -147,6 +147,8 @@ def lib() -> C.CDLL:
         "bh_get_eval": [P, P, u32, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_int32)],
         "bh_set_row_subset": [P, P, u64],
         "bh_set_bernoulli": [P, C.c_float, u64],
+        "bh_set_feature_mask": [P, P, u32],
+        "bh_set_feature_fraction": [P, C.c_float, u64],
         "bh_get_trees": [P, P, u32, P, u32, C.POINTER(u32)],
         "bh_max_nodes": [P, C.POINTER(u32)],
         "bh_set_gradients": [P, P, P, u32, P, P, P],
@@ -402,6 +404,21 @@ class HipContext:
         """Device Bernoulli row sampler: every iteration keeps each row with probability
         `subsample` (> 0; >= 1 keeps every row), drawn from (seed, iteration, row)."""
         _check(lib().bh_set_bernoulli(self._h, float(subsample), int(seed) & ((1 << 64) - 1)))
+
+    def set_feature_mask(self, mask) -> None:
+        """Grow every following tree on the features whose mask byte is non-zero (the
+        `selected` list of a per-tree feature draw); a mask selecting nothing is refused.
+        None: every feature."""
+        if mask is None:
+            _check(lib().bh_set_feature_mask(self._h, None, 0))
+            return
+        mask = np.ascontiguousarray(np.asarray(mask) != 0, np.uint8)
+        _check(lib().bh_set_feature_mask(self._h, _ptr(mask), mask.size))
+
+    def set_feature_fraction(self, fraction: float, seed: int = 0) -> None:
+        """Device per-tree feature draw: tree t selects max(1, ceil(fraction * F))
+        features, keyed by (seed, t, feature); fraction >= 1 selects every feature."""
+        _check(lib().bh_set_feature_fraction(self._h, float(fraction), int(seed) & ((1 << 64) - 1)))
 
     def eval_history(self) -> tuple[np.ndarray, int, bool]:
         """(losses so far, best round, stopped)."""

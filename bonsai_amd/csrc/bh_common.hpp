@@ -221,7 +221,9 @@ struct FindScratch {
 struct GrowCfg {
     double min_child_hess, min_gain, l2, l1;
     int32_t max_depth, min_data_in_leaf, max_leaves, quantized;
-    double scale_g, scale_h; // unused on device (read from scales buffer)
+    // Per-tree feature selection (sample_features, src/grower_impl.hpp:165-180): the
+    // finders skip a feature whose byte is 0 (src/split.cpp:106-110); nullptr selects all.
+    u8 const *fmask;
 };
 
 // Compact node for the bin-space walks (score update, predict).

@@ -153,7 +153,9 @@ __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restric
         u32 const nt = min(kFindTile, F - f0);
         __syncthreads(); // the previous tile (or phase 0) is done with the LDS
         // Stage the tile's cells, bin counts and count cells (coalesced, one round trip).
-        if (threadIdx.x < nt) s_nb[threadIdx.x] = n_bins[f0 + threadIdx.x];
+        // An unselected feature offers no cut (zero bins).
+        if (threadIdx.x < nt)
+            s_nb[threadIdx.x] = (cfg.fmask && !cfg.fmask[f0 + threadIdx.x]) ? 0u : n_bins[f0 + threadIdx.x];
         for (u32 k = threadIdx.x; k < nt * SB; k += blockDim.x) s_cells[k] = hcells[static_cast<size_t>(f0) * SB + k];
         if (hcnt)
             for (u32 k = threadIdx.x; k < nt * SB; k += blockDim.x) s_cn[k] = hcnt[static_cast<size_t>(f0) * SB + k];

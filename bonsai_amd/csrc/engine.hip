@@ -110,6 +110,11 @@ struct bh_context {
     u32 samp_thr = 0;            // kind 2: keep when 32 random bits < thr
     u64 samp_seed = 0;
     u32 *samp_cnt = nullptr;     // kind 2: per-chunk kept counts, then their prefix
+    // feature sampling (bh_set_feature_mask / bh_set_feature_fraction)
+    u32 fsamp_kind = 0;          // 0 every feature, 1 a fixed mask, 2 a draw per iteration
+    u8 *fmask = nullptr;         // [F] 1 = selected
+    u32 fsamp_k = 0;             // kind 2: features selected per tree
+    u64 fsamp_seed = 0;
     // configuration
     bh_fit_config fc{};
     GrowCfg gc{};
@@ -388,6 +393,8 @@ RootSel root_sel(bh_context *c)
 // Gathers this iteration's sampled rows (and their gradient words) into work plane 1.
 void enqueue_sample(bh_context *c)
 {
+    if (c->fsamp_kind == 2)
+        launch_feature_draw(c->stream, c->ctl, c->F, c->fsamp_k, c->fsamp_seed, c->fmask);
     u32 const gh_bytes = c->quant ? 4 : 8;
     if (c->samp_kind == 1)
         launch_gather_rows(c->stream, c->plane0, c->Fs, gh_src(c, 2), gh_bytes, c->samp_rows, c->samp_n, c->work[1],
@@ -705,7 +712,7 @@ int bh_destroy(bh_context *c)
     hipStreamSynchronize(c->stream);
     free_fit(c);
     void *ptrs[] = {c->plane0, c->work[0], c->work[1], c->d_nbins, c->label, c->score,
-                    c->vplane, c->vlabel, c->vscore, c->vhist, c->vpartial, c->samp_rows, c->samp_cnt};
+                    c->vplane, c->vlabel, c->vscore, c->vhist, c->vpartial, c->samp_rows, c->samp_cnt, c->fmask};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -729,12 +736,16 @@ static int alloc_plane(bh_context *c, uint64_t n_rows, uint32_t n_features, cons
     HIP_TRY(hipSetDevice(c->device));
     hipStreamSynchronize(c->stream);
     free_fit(c);
-    void *ptrs[] = {c->plane0, c->work[0], c->work[1], c->d_nbins, c->label, c->score, c->samp_rows, c->samp_cnt};
+    void *ptrs[] = {c->plane0, c->work[0], c->work[1], c->d_nbins, c->label, c->score, c->samp_rows, c->samp_cnt,
+                    c->fmask};
     for (void *p : ptrs)
         if (p) hipFree(p);
     c->plane0 = c->work[0] = c->work[1] = nullptr;
     c->samp_rows = c->samp_cnt = nullptr;
     c->samp_kind = c->samp_n = 0;
+    c->fmask = nullptr;
+    c->fsamp_kind = 0;
+    c->gc.fmask = nullptr;
     c->d_nbins = nullptr;
     c->label = c->score = nullptr;
     c->have_labels = false;
@@ -976,6 +987,53 @@ int bh_set_bernoulli(bh_context *c, float subsample, uint64_t seed)
     c->samp_thr = static_cast<u32>(std::min(t, 4294967295.0));
     c->samp_seed = seed;
     c->samp_kind = 2;
+    return 0;
+}
+
+// Selection state shared by both feature setters: clears it, dropping the graph (the
+// finders' configuration and the draw launch change).
+int reset_feature_sampling(bh_context *c)
+{
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->fsamp_kind = 0;
+    c->gc.fmask = nullptr;
+    drop_graph(c);
+    if (!c->fmask) HIP_TRY(dalloc(&c->fmask, c->F));
+    return 0;
+}
+
+int bh_set_feature_mask(bh_context *c, const uint8_t *mask, uint32_t n_features)
+{
+    if (!c) return fail(1, "null context");
+    if (!c->plane0) return fail(1, "no bins uploaded");
+    if (mask && n_features != c->F) return fail(1, "feature mask length differs from n_features");
+    u32 n_sel = 0;
+    if (mask)
+        for (u32 f = 0; f < n_features; ++f) n_sel += mask[f] ? 1u : 0u;
+    if (mask && n_sel == 0) return fail(1, "the feature mask selects no features");
+    if (int r = reset_feature_sampling(c)) return r;
+    if (!mask || n_sel == c->F) return 0; // every feature
+    std::vector<u8> m(mask, mask + n_features);
+    for (u8 &v : m) v = v ? 1 : 0;
+    HIP_TRY(hipMemcpy(c->fmask, m.data(), c->F, hipMemcpyHostToDevice));
+    c->fsamp_kind = 1;
+    c->gc.fmask = c->fmask;
+    return 0;
+}
+
+int bh_set_feature_fraction(bh_context *c, float fraction, uint64_t seed)
+{
+    if (!c) return fail(1, "null context");
+    if (!c->plane0) return fail(1, "no bins uploaded");
+    if (std::isnan(fraction)) return fail(1, "feature fraction is NaN");
+    if (int r = reset_feature_sampling(c)) return r;
+    if (fraction >= 1.0F || c->F <= 1) return 0; // sample_features returns every feature
+    double const k = std::ceil(static_cast<double>(fraction * static_cast<float>(c->F)));
+    c->fsamp_k = static_cast<u32>(std::min(static_cast<double>(c->F), std::max(1.0, k)));
+    c->fsamp_seed = seed;
+    c->fsamp_kind = 2;
+    c->gc.fmask = c->fmask;
     return 0;
 }
 

@@ -2416,6 +2416,47 @@ void launch_bernoulli(hipStream_t s, TreeCtl *ctl, u32 n, u64 seed, u32 thr, u32
                        gh_bytes, out_plane, out_gh);
 }
 
+// ---------------------------------------------------------------------------------
+// Per-tree feature draw (sample_features, src/grower_impl.hpp:165-180): the
+// k = max(1, ceil(fraction * F)) features with the smallest keys are selected, a key
+// being the 64-bit splitmix64 draw of (seed, t, f, channel 5) with ties broken by the
+// feature id.  Every thread ranks one feature against all F keys (F is at most a few
+// thousand: one pass over them through LDS per workgroup).  (The reference draws with
+// std::sample over std::mt19937; its exact selection is reproduced by handing the
+// mask over instead, bh_set_feature_mask.)
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ u64 feature_key(u64 seed, u32 iter, u32 f)
+{
+    return splitmix64(seed ^ (static_cast<u64>(iter) * 0xD1B54A32D192ED03ULL) ^
+                      (static_cast<u64>(f) * 0x9E3779B97F4A7C15ULL) ^ (5ULL * 0xA0761D6478BD642FULL));
+}
+
+__global__ __launch_bounds__(256) void k_feature_draw(TreeCtl const *__restrict__ ctl, u32 F, u32 k, u64 seed,
+                                                      u8 *__restrict__ mask)
+{
+    __shared__ u64 s_key[256];
+    u32 const iter = ctl->iter;
+    u32 const f = blockIdx.x * 256u + threadIdx.x;
+    u64 const mine = f < F ? feature_key(seed, iter, f) : 0ULL;
+    u32 rank = 0;
+    for (u32 g0 = 0; g0 < F; g0 += 256u) {
+        __syncthreads();
+        s_key[threadIdx.x] = g0 + threadIdx.x < F ? feature_key(seed, iter, g0 + threadIdx.x) : 0ULL;
+        __syncthreads();
+        u32 const n = min(256u, F - g0);
+        for (u32 j = 0; j < n; ++j) {
+            u64 const kj = s_key[j];
+            rank += (kj < mine || (kj == mine && g0 + j < f)) ? 1u : 0u;
+        }
+    }
+    if (f < F) mask[f] = rank < k ? 1 : 0;
+}
+
+void launch_feature_draw(hipStream_t s, TreeCtl const *ctl, u32 F, u32 k, u64 seed, u8 *mask)
+{
+    hipLaunchKernelGGL(k_feature_draw, dim3((F + 255u) / 256u), dim3(256), 0, s, ctl, F, k, seed, mask);
+}
+
 __global__ void k_gather_rows(u8 const *__restrict__ plane, u32 Fs, void const *__restrict__ gh, u32 gh_bytes,
                               u32 const *__restrict__ rows, u64 n, u8 *__restrict__ out_plane,
                               void *__restrict__ out_gh, int gh_is_row_id)

@@ -71,6 +71,9 @@ void launch_tree_begin(hipStream_t s, TreeCtl *ctl, NodeInfo *front, DevNode *tr
 // their count into ctl->root_n.  chunk_cnt: n / kPartChunk + 1 words of scratch.
 void launch_bernoulli(hipStream_t s, TreeCtl *ctl, u32 n, u64 seed, u32 thr, u32 *chunk_cnt, u8 const *plane,
                       u32 Fs, void const *gh, u32 gh_bytes, u8 *out_plane, void *out_gh);
+// Per-tree feature selection for iteration ctl->iter: mask[f] = 1 for the k smallest
+// (key, f) pairs.
+void launch_feature_draw(hipStream_t s, TreeCtl const *ctl, u32 F, u32 k, u64 seed, u8 *mask);
 // maxbits (training, quantized): also the next iteration's gradient maxima, from the
 // updated scores and the labels, into slot (ctl->iter + 1) & 1.
 void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode const *tree, TreeCtl const *ctl,

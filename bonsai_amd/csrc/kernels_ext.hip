@@ -131,7 +131,7 @@ __global__ __launch_bounds__(256) void k_find_level(NodeInfo const *__restrict__
     if (ctl->done) return;
     u32 const TN = Q ? kLevelTileQ : kLevelTileF32;
     u32 const P = ctl->n_front;
-    u32 const nb = n_bins[f];
+    u32 const nb = (cfg.fmask && !cfg.fmask[f]) ? 0u : n_bins[f]; // unselected: no cut
     u32 const ncut = nb >= 2 ? nb - 2 : 0;
     SplitRec best{};
     best.default_left = 1;

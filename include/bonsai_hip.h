@@ -150,6 +150,21 @@ int bh_set_row_subset(bh_context *ctx, const uint32_t *rows, uint64_t n);
  * reference's std::mt19937 one (bh_set_row_subset takes that one's indices).  Replaces
  * a row subset. */
 int bh_set_bernoulli(bh_context *ctx, float subsample, uint64_t seed);
+/* Per-tree feature selection handed over (the `selected` list of sample_features,
+ * src/grower_impl.hpp:165-180 and :527-528): mask[f] != 0 selects feature f for every
+ * following tree; the split finders skip the others (src/split.cpp:106-110).  n_features
+ * must equal the uploaded plane's; a mask that selects nothing is refused (the
+ * reference's refuse_empty_selection, src/cuda/detail/device_context.cu:81-86).  mask
+ * NULL or all selected: every feature.  Replaces a feature fraction. */
+int bh_set_feature_mask(bh_context *ctx, const uint8_t *mask, uint32_t n_features);
+/* Per-tree feature draw on the device (tree.feature_fraction / feature_seed,
+ * include/bonsai/config/tree_config.hpp:15-19): tree t selects
+ * k = max(1, ceil(fraction * n_features)) features, those with the k smallest
+ * counter-based 64-bit keys (seed, t, f); fraction >= 1 (or one feature) selects all.
+ * NaN is refused.  The draw is this library's own stream, not std::sample over the
+ * reference's std::mt19937 (bh_set_feature_mask takes that one's selection).  Replaces
+ * a feature mask. */
+int bh_set_feature_fraction(bh_context *ctx, float fraction, uint64_t seed);
 /* Trees grown so far: nodes[t * max_nodes + i], sizes[t]; returns via *n_trees. */
 int bh_get_trees(bh_context *ctx, bh_node *nodes, uint32_t max_nodes, uint32_t *sizes,
                  uint32_t cap_trees, uint32_t *n_trees);

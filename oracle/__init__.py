@@ -133,6 +133,21 @@ class Oracle:
                                       C.c_uint32(it), P(out))
         return out[:k].copy()
 
+    def set_feature_mask(self, mask):
+        """Restrict the following finders and growers to mask[f] != 0 (None: all)."""
+        if mask is None:
+            self.L.orc_set_feature_mask(None, C.c_uint32(0))
+            return
+        m = np.ascontiguousarray(mask, np.uint8)
+        self.L.orc_set_feature_mask(P(m), C.c_uint32(m.size))
+
+    def feature_draw(self, n_features, fraction, seed, it):
+        """The device's per-iteration feature subsample as a 0/1 mask."""
+        out = np.zeros(n_features, np.uint8)
+        self.L.orc_feature_draw(C.c_uint32(n_features), C.c_float(fraction), C.c_uint64(seed & ((1 << 64) - 1)),
+                                C.c_uint32(it), P(out))
+        return out
+
     def hist_f64(self, bins, rows, g, h, bin_stride):
         n, f = bins.shape
         out = np.zeros((f, bin_stride, 2), np.float64)

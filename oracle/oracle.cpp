@@ -248,6 +248,19 @@ struct Cfg {
     double min_child_hess, min_gain, l2, l1;
     int    max_depth, min_data_in_leaf, max_leaves;
 };
+
+// Per-tree feature subsample (sample_features, src/grower_impl.hpp:163-180): the finders
+// skip features outside it, as the reference's finder skips features whose histogram
+// was not built (src/split.cpp:106-110).  Empty: every feature.  Set by
+// orc_set_feature_mask for the growers and finders that follow.  Node totals still come
+// from feature 0's cells, built whether selected or not: every feature's cells sum the
+// same rows (SplitInput::totals takes the first selected one's, include/bonsai/split.hpp:
+// 39-53; quantized sums are equal integers, FP32 ones differ in summation order only).
+std::vector<u8> g_feature_mask;
+inline bool feature_selected(u32 f)
+{
+    return g_feature_mask.empty() || (f < g_feature_mask.size() && g_feature_mask[f] != 0);
+}
 Cfg to_cfg(orc_tree_cfg const &c)
 {
     return {static_cast<double>(c.min_child_hess), static_cast<double>(c.min_gain_to_split),
@@ -300,6 +313,7 @@ Split find_node_f32(float const *hist, u16 const *n_bins, u32 F, u32 SB, float t
     if (n_rows < 2 * static_cast<u64>(c.min_data_in_leaf)) return best;
     double const node_score = score(tg, th, c.l1, c.l2);
     for (u32 f = 0; f < F; ++f) {
+        if (!feature_selected(f)) continue;
         Split cand;
         u32 const nb = n_bins[f];
         if (nb == 0) continue;
@@ -340,6 +354,7 @@ Split find_node_int(i64 const *hist, u16 const *n_bins, u32 F, u32 SB, i64 tg, i
     double const node_score =
         score(static_cast<double>(tg) * sg, static_cast<double>(th) * sh, c.l1, c.l2);
     for (u32 f = 0; f < F; ++f) {
+        if (!feature_selected(f)) continue;
         Split cand;
         u32 const nb = n_bins[f];
         if (nb == 0) continue;
@@ -385,6 +400,7 @@ Split find_level_f32(float const *hists, float const *totals, u32 P, u16 const *
     size_t const node_cells = static_cast<size_t>(F) * SB * 2;
     std::vector<double> pg, ph, rg(P), rh(P), ps(P);
     for (u32 f = 0; f < F; ++f) {
+        if (!feature_selected(f)) continue;
         Split cand;
         u32 const nb = n_bins[f];
         u32 const n_cut = nb >= 2 ? nb - 2 : 0;
@@ -441,6 +457,7 @@ Split find_level_int(i64 const *hists, i64 const *totals, u32 P, u16 const *n_bi
     std::vector<i64> pg, ph;
     std::vector<double> ps(P);
     for (u32 f = 0; f < F; ++f) {
+        if (!feature_selected(f)) continue;
         Split cand;
         u32 const nb = n_bins[f];
         u32 const n_cut = nb >= 2 ? nb - 2 : 0;
@@ -1303,6 +1320,36 @@ uint64_t orc_bernoulli_rows(uint64_t n, uint32_t thr, uint64_t seed, uint32_t it
         if (static_cast<u32>(splitmix64(key) >> 32) < thr) out[k++] = static_cast<u32>(r);
     }
     return k;
+}
+
+void orc_set_feature_mask(const uint8_t *mask, uint32_t n)
+{
+    if (!mask || n == 0) g_feature_mask.clear();
+    else g_feature_mask.assign(mask, mask + n);
+}
+
+// The product's per-iteration device draw for feature_fraction < 1 (the reference draws
+// with std::sample over std::mt19937, src/grower_impl.hpp:163-180): k = max(1,
+// ceil(fraction * F)) in float as the reference computes it, and the k features with the
+// smallest 64-bit keys splitmix64(seed, iter, f, channel 4), ties to the lower id.
+// fraction >= 1 or F <= 1: every feature.  Pinned by tests/test_oracle_sampler.py (k
+// selected, each feature about equally often, independent draws per iteration).
+void orc_feature_draw(uint32_t n_features, float fraction, uint64_t seed, uint32_t iter, uint8_t *mask)
+{
+    if (fraction >= 1.0F || n_features <= 1) {
+        for (u32 f = 0; f < n_features; ++f) mask[f] = 1;
+        return;
+    }
+    u64 const k = std::max<u64>(1, static_cast<u64>(std::ceil(fraction * static_cast<float>(n_features))));
+    std::vector<std::pair<u64, u32>> keys(n_features);
+    for (u32 f = 0; f < n_features; ++f) {
+        u64 const key = seed ^ (static_cast<u64>(iter) * 0xD1B54A32D192ED03ULL) ^
+                        (static_cast<u64>(f) * 0x9E3779B97F4A7C15ULL) ^ (5ULL * 0xA0761D6478BD642FULL);
+        keys[f] = {splitmix64(key), f};
+    }
+    std::sort(keys.begin(), keys.end());
+    for (u32 f = 0; f < n_features; ++f) mask[f] = 0;
+    for (u64 i = 0; i < k && i < n_features; ++i) mask[keys[i].second] = 1;
 }
 
 // Brute-force double histogram: the FP64 reference the float paths are measured

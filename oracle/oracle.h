@@ -91,6 +91,12 @@ void orc_quantize(const float *g, const float *h, uint64_t n, uint64_t seed,
  * product's counter-based draw: rows r of [0, n) whose 32 random bits of (seed, iter, r)
  * are below thr, ascending, into out; returns their count. */
 uint64_t orc_bernoulli_rows(uint64_t n, uint32_t thr, uint64_t seed, uint32_t iter, uint32_t *out);
+/* Per-tree feature subsample for the finders and growers that follow (mask[f] != 0:
+ * selected; NULL or n 0: every feature). */
+void orc_set_feature_mask(const uint8_t *mask, uint32_t n);
+/* The product's device draw for feature_fraction < 1: k = max(1, ceil(fraction * F))
+ * features with the smallest counter-based keys of (seed, iter, f); mask out [F]. */
+void orc_feature_draw(uint32_t n_features, float fraction, uint64_t seed, uint32_t iter, uint8_t *mask);
 
 /* ---- histograms over a row list ---- */
 void orc_hist_f64(const uint8_t *plane, uint64_t row_stride, uint32_t n_features,

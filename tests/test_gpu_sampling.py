@@ -232,3 +232,128 @@ def test_bernoulli_keeping_no_rows_gives_a_single_leaf(hip, orc, tall, grower):
     assert nodes[0]["feature"] < 0
     assert nodes[0]["value"] == 0.0 and nodes[0]["cover"] == 0.0
     assert np.all(lor == 0)
+
+
+def oracle_tree_masked(orc, mask, *a):
+    orc.set_feature_mask(mask)
+    try:
+        return oracle_tree(orc, *a)
+    finally:
+        orc.set_feature_mask(None)
+
+
+def check_tree(grower, nodes, lor, ref, rows=None):
+    if grower == "hip_levelwise":
+        check_levelwise(nodes, ref)
+    else:
+        assert_same_tree(nodes, ref[0])
+        assert np.array_equal(lor if rows is None else lor[rows], ref[1])
+
+
+@pytest.mark.parametrize("quantized", [True, False])
+@pytest.mark.parametrize("grower", list(GROWERS))
+def test_feature_mask_matches_oracle(hip, orc, tall, grower, quantized):
+    """A handed-over per-tree selection (the `selected` list of sample_features): the
+    finders skip the rest, feature 0 included, and the tree equals the oracle's."""
+    bins, nb, X, y = tall
+    n, F = bins.shape
+    mask = np.zeros(F, np.uint8)
+    mask[[2, 5, 6, 11, 17, 20, 27]] = 1
+    g, h = grad_of(orc, y, seed=13)
+    kw = grow_cfg(grower)
+    ctx = ctx_for(hip, bins, nb, grower=grower, quantized=quantized, **kw)
+    gq = hq = sc = None
+    if quantized:
+        gq, hq, sc = ctx.set_gradients(g, h)
+    else:
+        ctx.set_gradients(g, h)
+    ctx.set_feature_mask(mask)
+    nodes, lor = ctx.grow_tree()
+    ctx.close()
+    rows = np.arange(n, dtype=np.uint32)
+    ref = oracle_tree_masked(orc, mask, grower, bins, nb, rows, g, h, gq, hq, sc, quantized, orc_cfg(**kw))
+    check_tree(grower, nodes, lor, ref)
+    used = nodes["feature"][nodes["feature"] >= 0]
+    assert used.size > 0 and np.all(mask[used] == 1)
+
+
+@pytest.mark.parametrize("grower", list(GROWERS))
+def test_feature_fraction_first_tree_matches_oracle(hip, orc, tall, grower):
+    bins, nb, X, y = tall
+    n, F = bins.shape
+    g, h = grad_of(orc, y, seed=17)
+    kw = grow_cfg(grower)
+    ctx = ctx_for(hip, bins, nb, grower=grower, quantized=True, **kw)
+    gq, hq, sc = ctx.set_gradients(g, h)
+    ctx.set_feature_fraction(0.3, seed=41)
+    nodes, lor = ctx.grow_tree()  # a fresh fit: iteration 0 keys the draw
+    ctx.close()
+    mask = orc.feature_draw(F, 0.3, 41, 0)
+    assert int(mask.sum()) == 9
+    rows = np.arange(n, dtype=np.uint32)
+    ref = oracle_tree_masked(orc, mask, grower, bins, nb, rows, g, h, gq, hq, sc, True, orc_cfg(**kw))
+    check_tree(grower, nodes, lor, ref)
+
+
+def test_feature_fraction_and_rows_together(hip, orc, tall):
+    """Both samplers at once: the Bernoulli rows and the drawn features of iteration 0."""
+    bins, nb, X, y = tall
+    n, F = bins.shape
+    g, h = grad_of(orc, y, seed=19)
+    kw = grow_cfg("hip_leafwise")
+    ctx = ctx_for(hip, bins, nb, grower="hip_leafwise", quantized=False, **kw)
+    ctx.set_gradients(g, h)
+    ctx.set_bernoulli(0.6, seed=8)
+    ctx.set_feature_fraction(0.5, seed=9)
+    nodes, lor = ctx.grow_tree()
+    ctx.close()
+    rows = orc.bernoulli_rows(n, 0.6, 8, 0)
+    mask = orc.feature_draw(F, 0.5, 9, 0)
+    ref = oracle_tree_masked(orc, mask, "hip_leafwise", bins, nb, rows, g, h, None, None, None, False, orc_cfg(**kw))
+    check_tree("hip_leafwise", nodes, lor, ref, rows)
+    assert np.array_equal(lor, walk(nodes, bins, nb))
+
+
+@pytest.mark.parametrize("grower", ["hip_depthwise", "hip_levelwise"])
+def test_feature_fraction_training_draws_per_tree(hip, orc, tall, grower):
+    """A fresh draw every iteration: tree t splits only on the oracle's draw for t, and a
+    captured iteration graph trains the same trees as eager launches."""
+    bins, nb, X, y = tall
+    F = bins.shape[1]
+    res = []
+    for use_graph in (True, False):
+        ctx = ctx_for(hip, bins, nb, grower=grower, quantized=True, max_depth=6, use_graph=use_graph,
+                      learning_rate=0.1)
+        ctx.upload_labels(y)
+        ctx.set_scores(np.full(y.size, orc.init_score(1, y), np.float32))
+        ctx.set_feature_fraction(0.25, seed=5)
+        ctx.train(3)
+        ctx.train(2)
+        res.append((ctx.trees(), ctx.get_scores()))
+        ctx.close()
+    (ta, sa), (tb, sb) = res
+    assert len(ta) == len(tb) == 5
+    for a, b in zip(ta, tb):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    assert np.array_equal(sa.view(np.uint32), sb.view(np.uint32))
+    sets = []
+    for t, tree in enumerate(ta):
+        mask = orc.feature_draw(F, 0.25, 5, t)
+        used = np.unique(tree["feature"][tree["feature"] >= 0])
+        assert used.size > 0 and np.all(mask[used] == 1), t
+        sets.append(tuple(used))
+    assert len(set(sets)) > 1
+
+
+def test_feature_sampling_rejects_bad_input(hip, tall):
+    bins, nb, X, y = tall
+    ctx = ctx_for(hip, bins, nb, grower="hip_depthwise", quantized=True, max_depth=3)
+    with pytest.raises(hip.ConfigError):
+        ctx.set_feature_mask(np.zeros(bins.shape[1], np.uint8))
+    with pytest.raises(hip.ConfigError):
+        ctx.set_feature_mask(np.ones(bins.shape[1] + 1, np.uint8))
+    with pytest.raises(hip.ConfigError):
+        ctx.set_feature_fraction(float("nan"))
+    ctx.set_feature_mask(None)
+    ctx.set_feature_fraction(1.0)
+    ctx.close()
