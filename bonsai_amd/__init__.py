@@ -149,6 +149,7 @@ def lib() -> C.CDLL:
         "bh_set_bernoulli": [P, C.c_float, u64],
         "bh_set_feature_mask": [P, P, u32],
         "bh_set_feature_fraction": [P, C.c_float, u64],
+        "bh_set_monotone": [P, P, u32],
         "bh_get_trees": [P, P, u32, P, u32, C.POINTER(u32)],
         "bh_max_nodes": [P, C.POINTER(u32)],
         "bh_set_gradients": [P, P, P, u32, P, P, P],
@@ -414,6 +415,15 @@ class HipContext:
             return
         mask = np.ascontiguousarray(np.asarray(mask) != 0, np.uint8)
         _check(lib().bh_set_feature_mask(self._h, _ptr(mask), mask.size))
+
+    def set_monotone(self, dirs) -> None:
+        """Monotone constraints: dirs[f] > 0 increasing, < 0 decreasing, 0 free (features
+        past the list are free).  None: no constraints.  Refused with the levelwise grower."""
+        if dirs is None or len(dirs) == 0:
+            _check(lib().bh_set_monotone(self._h, None, 0))
+            return
+        d = np.ascontiguousarray(np.sign(np.asarray(dirs)), np.int8)
+        _check(lib().bh_set_monotone(self._h, _ptr(d), d.size))
 
     def set_feature_fraction(self, fraction: float, seed: int = 0) -> None:
         """Device per-tree feature draw: tree t selects max(1, ceil(fraction * F))

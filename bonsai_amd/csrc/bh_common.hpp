@@ -224,6 +224,11 @@ struct GrowCfg {
     // Per-tree feature selection (sample_features, src/grower_impl.hpp:165-180): the
     // finders skip a feature whose byte is 0 (src/split.cpp:106-110); nullptr selects all.
     u8 const *fmask;
+    // Monotone constraints (tree.monotone_constraints, include/bonsai/config/tree_config.hpp:
+    // 20-22): per-feature direction (+1, -1, 0), nullptr for none; then bounds holds the
+    // [lo, hi] leaf-weight box of every tree node id (2 doubles per node).
+    int8_t const *mono;
+    double *bounds;
 };
 
 // Compact node for the bin-space walks (score update, predict).

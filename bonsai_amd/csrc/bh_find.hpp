@@ -29,6 +29,62 @@ __device__ __forceinline__ double fd_score_of(double g, double h, double l1, dou
     return d > 0.0 ? (t * t) / d : 0.0;
 }
 
+// bounded_leaf_weight (include/bonsai/split.hpp:141-150): the Newton weight clamped to
+// [lo, hi] in std::clamp's comparison order.
+__device__ __forceinline__ double fd_bounded_weight(double g, double h, double l1, double l2, double lo, double hi)
+{
+    double const d = h + l2;
+    double const w = d > 0.0 ? -fd_l1_thr(g, l1) / d : 0.0;
+    return w < lo ? lo : (hi < w ? hi : w);
+}
+
+// propagate_monotone_bounds (src/grower_impl.hpp:47-75): both children inherit the
+// parent's box; a split on a monotone feature fences them at the midpoint of the two
+// child weights, each bounded by the parent's box.  The child sums are the ones the
+// finder scored (exact integers times the scales when quantized).
+__device__ __forceinline__ void fd_propagate_bounds(GrowCfg const &c, double const *__restrict__ scales, u32 pid,
+                                                    SplitRec const &s, u32 lid, u32 rid)
+{
+    if (!c.mono) return;
+    double const plo = c.bounds[2 * pid], phi = c.bounds[2 * pid + 1];
+    double llo = plo, lhi = phi, rlo = plo, rhi = phi;
+    int const mc = c.mono[s.feature];
+    if (mc != 0) {
+        double gL = s.gL, hL = s.hL, gR = s.gR, hR = s.hR;
+        if (c.quantized) {
+            gL = static_cast<double>(s.igL) * scales[0];
+            hL = static_cast<double>(s.ihL) * scales[1];
+            gR = static_cast<double>(s.igR) * scales[0];
+            hR = static_cast<double>(s.ihR) * scales[1];
+        }
+        double const wL = fd_bounded_weight(gL, hL, c.l1, c.l2, plo, phi);
+        double const wR = fd_bounded_weight(gR, hR, c.l1, c.l2, plo, phi);
+        double const mid = 0.5 * (wL + wR);
+        if (mc > 0) {
+            lhi = mid < lhi ? mid : lhi;
+            rlo = rlo < mid ? mid : rlo;
+        } else {
+            llo = llo < mid ? mid : llo;
+            rhi = mid < rhi ? mid : rhi;
+        }
+    }
+    c.bounds[2 * lid] = llo;
+    c.bounds[2 * lid + 1] = lhi;
+    c.bounds[2 * rid] = rlo;
+    c.bounds[2 * rid + 1] = rhi;
+}
+
+// A node's leaf weight as float, clamped to its box when constraints are on.
+__device__ __forceinline__ float fd_leaf_value(u32 id, double g, double h, GrowCfg const &c)
+{
+    double lo = -__builtin_huge_val(), hi = __builtin_huge_val();
+    if (c.mono) {
+        lo = c.bounds[2 * id];
+        hi = c.bounds[2 * id + 1];
+    }
+    return static_cast<float>(fd_bounded_weight(g, h, c.l1, c.l2, lo, hi));
+}
+
 template <bool Q>
 __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restrict__ ctl, void const *__restrict__ hist,
                           u16 const *__restrict__ n_bins, u32 F, u32 SB, GrowCfg cfg, double const *__restrict__ scales,
@@ -80,6 +136,11 @@ __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restric
     u32 bnl = 0;                                       // winner's left rows (count cells)
     __shared__ u32 s_last;
     double node_score = 0.0, tg = 0.0, th = 0.0;
+    double node_lo = -__builtin_huge_val(), node_hi = __builtin_huge_val(); // monotone box
+    if (cfg.mono) {
+        node_lo = cfg.bounds[2 * node.id];
+        node_hi = cfg.bounds[2 * node.id + 1];
+    }
     bool const had_totals = node.totals_ready != 0;
     // Quantized totals are exact integers, equal over every feature's cells: a block that
     // scans takes them from the first feature of its first tile (no extra round trip).
@@ -324,6 +385,16 @@ __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restric
                     hR = right_h + (!dl ? mh : 0.0);
                 }
                 if (hL < cfg.min_child_hess || hR < cfg.min_child_hess) continue;
+                if (cfg.mono) {
+                    // src/split.cpp:143-153: a candidate whose bounded child weights
+                    // break the feature's direction is skipped.
+                    int const mc = cfg.mono[f0 + tt];
+                    if (mc != 0) {
+                        double const wl = fd_bounded_weight(gL, hL, cfg.l1, cfg.l2, node_lo, node_hi);
+                        double const wr = fd_bounded_weight(gR, hR, cfg.l1, cfg.l2, node_lo, node_hi);
+                        if (static_cast<double>(mc) * (wr - wl) < 0.0) continue;
+                    }
+                }
                 double const cs = fd_score_of(gL, hL, cfg.l1, cfg.l2) + fd_score_of(gR, hR, cfg.l1, cfg.l2);
                 double const gain = cs - node_score;
                 if (!(gain > 0.0 && gain >= cfg.min_gain)) continue;

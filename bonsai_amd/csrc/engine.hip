@@ -115,6 +115,10 @@ struct bh_context {
     u8 *fmask = nullptr;         // [F] 1 = selected
     u32 fsamp_k = 0;             // kind 2: features selected per tree
     u64 fsamp_seed = 0;
+    // monotone constraints (bh_set_monotone): directions by sign, per-node boxes
+    std::vector<int8_t> mono_host;
+    int8_t *mono_dev = nullptr;  // [F]
+    double *bounds = nullptr;    // [max_nodes][2]
     // configuration
     bh_fit_config fc{};
     GrowCfg gc{};
@@ -163,9 +167,14 @@ void free_fit(bh_context *c)
     void *ptrs[] = {c->gh0f, c->ghwf[0], c->ghwf[1], c->gh0q, c->ghwq[0], c->ghwq[1], c->maxbits, c->scales,
                     c->hist[0], c->hist[1], c->partials, c->ctl, c->front[0], c->front[1], c->splits, c->tree,
                     c->ops, c->pch, c->chunk_left, c->hch, c->fops, c->trees_out, c->tree_sizes, c->leaf_rows,
-                    c->fscr.part, c->fscr.done, c->cnt[0], c->cnt[1], c->ops4, c->chunk_cls};
+                    c->fscr.part, c->fscr.done, c->cnt[0], c->cnt[1], c->ops4, c->chunk_cls, c->mono_dev,
+                    c->bounds};
     for (void *p : ptrs)
         if (p) hipFree(p);
+    c->mono_dev = nullptr;
+    c->bounds = nullptr;
+    c->gc.mono = nullptr;
+    c->gc.bounds = nullptr;
     c->gh0f = c->ghwf[0] = c->ghwf[1] = nullptr;
     c->gh0q = c->ghwq[0] = c->ghwq[1] = nullptr;
     c->maxbits = nullptr;
@@ -197,6 +206,31 @@ u32 tree_capacity(bh_fit_config const &f)
 {
     if (f.grower == BH_LEAFWISE) return std::min<u32>(kMaxTreeNodes, 4u * static_cast<u32>(f.max_leaves) + 8u);
     return std::min<u32>(kMaxTreeNodes, (2u << static_cast<u32>(f.max_depth)) + 2u);
+}
+
+// Device copies of the monotone directions and the per-node boxes for the configured
+// fit; none when every direction is 0.  The levelwise grower refuses constraints
+// (src/grower_impl.hpp:583-588).
+int arm_monotone(bh_context *c)
+{
+    if (c->mono_dev) (void)hipFree(c->mono_dev);
+    if (c->bounds) (void)hipFree(c->bounds);
+    c->mono_dev = nullptr;
+    c->bounds = nullptr;
+    c->gc.mono = nullptr;
+    c->gc.bounds = nullptr;
+    bool any = false;
+    for (int8_t v : c->mono_host) any = any || v != 0;
+    if (!any || !c->configured) return 0;
+    if (c->fc.grower == BH_LEVELWISE) return fail(1, "monotone_constraints are not supported by the levelwise grower");
+    std::vector<int8_t> d(c->F, 0);
+    for (size_t f = 0; f < c->mono_host.size() && f < c->F; ++f) d[f] = c->mono_host[f];
+    HIP_TRY(dalloc(&c->mono_dev, c->F));
+    HIP_TRY(dalloc(&c->bounds, 2 * static_cast<size_t>(c->max_nodes)));
+    HIP_TRY(hipMemcpy(c->mono_dev, d.data(), c->F, hipMemcpyHostToDevice));
+    c->gc.mono = c->mono_dev;
+    c->gc.bounds = c->bounds;
+    return 0;
 }
 
 int alloc_fit(bh_context *c)
@@ -277,7 +311,7 @@ int alloc_fit(bh_context *c)
     c->gc.max_leaves = f.max_leaves;
     c->gc.quantized = f.quantized;
     c->configured = true;
-    return 0;
+    return arm_monotone(c);
 }
 
 int ensure_tree_out(bh_context *c, u32 n_more)
@@ -511,6 +545,7 @@ void enqueue_depthwise(bh_context *c)
 void enqueue_grow(bh_context *c)
 {
     enqueue_sample(c);
+    if (c->gc.mono) launch_bounds_root(c->stream, c->bounds);
     if (c->fc.grower == BH_DEPTHWISE) enqueue_depthwise(c);
     else if (c->fc.grower == BH_LEVELWISE) enqueue_levelwise(c->ext, c->stream, c->quant, grow_view(c));
     else enqueue_leafwise(c->ext, c->stream, c->quant, grow_view(c));
@@ -987,6 +1022,24 @@ int bh_set_bernoulli(bh_context *c, float subsample, uint64_t seed)
     c->samp_thr = static_cast<u32>(std::min(t, 4294967295.0));
     c->samp_seed = seed;
     c->samp_kind = 2;
+    return 0;
+}
+
+int bh_set_monotone(bh_context *c, const int8_t *dirs, uint32_t n)
+{
+    if (!c) return fail(1, "null context");
+    if (dirs == nullptr && n != 0) return fail(1, "null monotone directions");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<int8_t> prev = std::move(c->mono_host);
+    c->mono_host.clear();
+    for (u32 f = 0; f < n; ++f) c->mono_host.push_back(dirs[f] > 0 ? 1 : (dirs[f] < 0 ? -1 : 0));
+    drop_graph(c); // the finders' configuration changes
+    if (int r = arm_monotone(c)) {
+        c->mono_host = std::move(prev);
+        (void)arm_monotone(c);
+        return r;
+    }
     return 0;
 }
 

@@ -116,6 +116,10 @@ __device__ __forceinline__ void write_leaf(DevNode *tree, u32 id, float v)
 __device__ __forceinline__ float node_leaf_value(NodeInfo const &n, GrowCfg const &c,
                                                  double const *scales)
 {
+    if (c.mono) // write_leaf's bounded weight (src/step/primitives.hpp:42-50)
+        return c.quantized ? fd_leaf_value(n.id, static_cast<double>(n.itg) * scales[0],
+                                           static_cast<double>(n.ith) * scales[1], c)
+                           : fd_leaf_value(n.id, n.tg, n.th, c);
     if (c.quantized)
         return leaf_weight(static_cast<double>(n.itg) * scales[0],
                            static_cast<double>(n.ith) * scales[1], c.l1, c.l2);
@@ -1109,6 +1113,7 @@ __global__ __launch_bounds__(1024) void k_plan_level(NodeInfo const *__restrict_
                 op.right_id = rid;
                 op.alive = 1;
                 ops[rank] = op;
+                fd_propagate_bounds(cfg, scales, nd.id, s, lid, rid);
             } else {
                 write_leaf(tree, nd.id, node_leaf_value(nd, cfg, scales));
             }
@@ -1900,6 +1905,7 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
                     tree[nd.id] = pn;
                     tree[lid].cover = static_cast<float>(nL);
                     tree[rid].cover = static_cast<float>(nR);
+                    fd_propagate_bounds(cfg, scales, nd.id, sp, lid, rid);
                     PartOp op{};
                     op.parent = i;
                     op.feature = sp.feature;
@@ -2450,6 +2456,20 @@ __global__ __launch_bounds__(256) void k_feature_draw(TreeCtl const *__restrict_
         }
     }
     if (f < F) mask[f] = rank < k ? 1 : 0;
+}
+
+// The root's monotone box is unbounded (SplitInput lo/hi defaults, include/bonsai/split.hpp).
+__global__ void k_bounds_root(double *__restrict__ bounds)
+{
+    if (threadIdx.x == 0) {
+        bounds[0] = -__builtin_huge_val();
+        bounds[1] = __builtin_huge_val();
+    }
+}
+
+void launch_bounds_root(hipStream_t s, double *bounds)
+{
+    hipLaunchKernelGGL(k_bounds_root, dim3(1), dim3(64), 0, s, bounds);
 }
 
 void launch_feature_draw(hipStream_t s, TreeCtl const *ctl, u32 F, u32 k, u64 seed, u8 *mask)

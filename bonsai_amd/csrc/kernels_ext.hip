@@ -41,6 +41,10 @@ __device__ __forceinline__ float leaf_weight(double g, double h, double l1, doub
 }
 __device__ __forceinline__ float node_value(NodeInfo const &n, GrowCfg const &c, double const *scales)
 {
+    if (c.mono) // write_leaf's bounded weight (src/step/primitives.hpp:42-50)
+        return c.quantized ? fd_leaf_value(n.id, static_cast<double>(n.itg) * scales[0],
+                                           static_cast<double>(n.ith) * scales[1], c)
+                           : fd_leaf_value(n.id, n.tg, n.th, c);
     if (c.quantized)
         return leaf_weight(static_cast<double>(n.itg) * scales[0], static_cast<double>(n.ith) * scales[1], c.l1,
                            c.l2);
@@ -805,6 +809,7 @@ __device__ __forceinline__ void lw_select_round(TreeCtl *__restrict__ ctl, LwCtl
         tree[rid] = leaf;
         parent_out[0] = c.node;
         split_out[0] = c.split;
+        fd_propagate_bounds(cfg, scales, c.node.id, c.split, lid, rid);
         PartOp op{};
         op.parent = 0;
         op.feature = c.split.feature;

@@ -165,6 +165,16 @@ int bh_set_feature_mask(bh_context *ctx, const uint8_t *mask, uint32_t n_feature
  * reference's std::mt19937 (bh_set_feature_mask takes that one's selection).  Replaces
  * a feature mask. */
 int bh_set_feature_fraction(bh_context *ctx, float fraction, uint64_t seed);
+/* Monotone constraints (tree.monotone_constraints, include/bonsai/config/tree_config.hpp:
+ * 20-22): dirs[f] > 0 increasing, < 0 decreasing, 0 free; features past n are free
+ * (monotone_constraint_of, include/bonsai/split.hpp:160-166).  The finders skip a
+ * candidate whose bounded child weights break its feature's direction
+ * (src/split.cpp:143-153), children inherit the parent's leaf-weight box fenced at the
+ * midpoint of the child weights (propagate_monotone_bounds, src/grower_impl.hpp:47-75),
+ * and leaf values are clamped to their box (bounded_leaf_weight,
+ * include/bonsai/split.hpp:141-150).  Kept across bh_configure; refused with the
+ * levelwise grower (src/grower_impl.hpp:583-588).  dirs NULL / n 0 clears. */
+int bh_set_monotone(bh_context *ctx, const int8_t *dirs, uint32_t n);
 /* Trees grown so far: nodes[t * max_nodes + i], sizes[t]; returns via *n_trees. */
 int bh_get_trees(bh_context *ctx, bh_node *nodes, uint32_t max_nodes, uint32_t *sizes,
                  uint32_t cap_trees, uint32_t *n_trees);

@@ -141,6 +141,14 @@ class Oracle:
         m = np.ascontiguousarray(mask, np.uint8)
         self.L.orc_set_feature_mask(P(m), C.c_uint32(m.size))
 
+    def set_monotone(self, dirs):
+        """Monotone directions for the following depthwise/leafwise growers (None: none)."""
+        if dirs is None:
+            self.L.orc_set_monotone(None, C.c_uint32(0))
+            return
+        d = np.ascontiguousarray(dirs, np.int8)
+        self.L.orc_set_monotone(P(d), C.c_uint32(d.size))
+
     def feature_draw(self, n_features, fraction, seed, it):
         """The device's per-iteration feature subsample as a 0/1 mask."""
         out = np.zeros(n_features, np.uint8)

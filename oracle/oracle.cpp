@@ -47,11 +47,16 @@ inline double score(double g, double h, double l1, double l2)
     double const d = h + l2;
     return d > 0.0 ? (t * t) / d : 0.0;
 }
-inline double bounded_leaf_weight(double g, double h, double l1, double l2)
+constexpr double kInf = std::numeric_limits<double>::infinity();
+
+// bounded_leaf_weight (include/bonsai/split.hpp:141-150): the Newton weight clamped to
+// the node's monotone box (+-inf without constraints).
+inline double bounded_leaf_weight(double g, double h, double l1, double l2, double lo = -kInf,
+                                  double hi = kInf)
 {
     double const d = h + l2;
     double const w = d > 0.0 ? -l1_thresholded(g, l1) / d : 0.0;
-    return w; // bounds are +-inf without monotone constraints
+    return std::clamp(w, lo, hi);
 }
 
 // Routing truth: include/bonsai/dataset.hpp:234-238.
@@ -257,6 +262,24 @@ struct Cfg {
 // same rows (SplitInput::totals takes the first selected one's, include/bonsai/split.hpp:
 // 39-53; quantized sums are equal integers, FP32 ones differ in summation order only).
 std::vector<u8> g_feature_mask;
+
+// Monotone directions (tree.monotone_constraints; monotone_constraint_of,
+// include/bonsai/split.hpp:160-166: features beyond the list are free).  Set by
+// orc_set_monotone for the depthwise and leafwise growers that follow.
+std::vector<int8_t> g_monotone;
+inline int monotone_of(u32 f) { return f < g_monotone.size() ? g_monotone[f] : 0; }
+
+// src/split.cpp:143-153: a candidate whose bounded child weights break the feature's
+// direction is skipped.
+inline bool breaks_monotone(u32 f, double gL, double hL, double gR, double hR, double lo, double hi,
+                            double l1, double l2)
+{
+    int const mc = monotone_of(f);
+    if (mc == 0) return false;
+    double const wl = bounded_leaf_weight(gL, hL, l1, l2, lo, hi);
+    double const wr = bounded_leaf_weight(gR, hR, l1, l2, lo, hi);
+    return static_cast<double>(mc) * (wr - wl) < 0.0;
+}
 inline bool feature_selected(u32 f)
 {
     return g_feature_mask.empty() || (f < g_feature_mask.size() && g_feature_mask[f] != 0);
@@ -307,7 +330,7 @@ inline bool candidate(double lg, double lh, double mg, double mh, double rg, dou
 }
 
 Split find_node_f32(float const *hist, u16 const *n_bins, u32 F, u32 SB, float tg, float th,
-                    u64 n_rows, Cfg const &c)
+                    u64 n_rows, Cfg const &c, double lo = -kInf, double hi = kInf)
 {
     Split best;
     if (n_rows < 2 * static_cast<u64>(c.min_data_in_leaf)) return best;
@@ -331,6 +354,10 @@ Split find_node_f32(float const *hist, u16 const *n_bins, u32 F, u32 SB, float t
                 bool const dl = d == 0;
                 double cs;
                 if (!candidate(lg, lh, mg, mh, real_g, real_h, dl, c, &cs)) continue;
+                if (breaks_monotone(f, lg + (dl ? mg : 0.0), lh + (dl ? mh : 0.0),
+                                    (real_g - lg) + (!dl ? mg : 0.0), (real_h - lh) + (!dl ? mh : 0.0), lo,
+                                    hi, c.l1, c.l2))
+                    continue;
                 if (update_best(cand, cs - node_score, f, b, dl, c)) {
                     cand.gL = lg + (dl ? mg : 0.0);
                     cand.hL = lh + (dl ? mh : 0.0);
@@ -347,7 +374,7 @@ Split find_node_f32(float const *hist, u16 const *n_bins, u32 F, u32 SB, float t
 // Integer cells: the same candidate order, sums exact in int64 and converted to double
 // by the per-iteration scale before scoring.
 Split find_node_int(i64 const *hist, u16 const *n_bins, u32 F, u32 SB, i64 tg, i64 th,
-                    double sg, double sh, u64 n_rows, Cfg const &c)
+                    double sg, double sh, u64 n_rows, Cfg const &c, double lo = -kInf, double hi = kInf)
 {
     Split best;
     if (n_rows < 2 * static_cast<u64>(c.min_data_in_leaf)) return best;
@@ -377,6 +404,7 @@ Split find_node_int(i64 const *hist, u16 const *n_bins, u32 F, u32 SB, i64 tg, i
                 double const dgR = static_cast<double>(gR) * sg;
                 double const dhR = static_cast<double>(hR) * sh;
                 if (dhL < c.min_child_hess || dhR < c.min_child_hess) continue;
+                if (breaks_monotone(f, dgL, dhL, dgR, dhR, lo, hi, c.l1, c.l2)) continue;
                 double const cs = score(dgL, dhL, c.l1, c.l2) + score(dgR, dhR, c.l1, c.l2);
                 if (update_best(cand, cs - node_score, f, b, dl, c)) {
                     cand.igL = gL;
@@ -544,6 +572,7 @@ struct NodeState {
     float tg = 0.0F, th = 0.0F;
     i64 itg = 0, ith = 0;
     int depth = 0;
+    double blo = -kInf, bhi = kInf; // monotone box of the node's leaf weight
 };
 
 struct Engine {
@@ -628,9 +657,39 @@ struct Engine {
     {
         if (mode == ORC_HIST_INT)
             return find_node_int(n.hi.data(), ds.n_bins, ds.n_features, ds.bin_stride, n.itg,
-                                 n.ith, gr.sg, gr.sh, n.rows.size(), cfg);
+                                 n.ith, gr.sg, gr.sh, n.rows.size(), cfg, n.blo, n.bhi);
         return find_node_f32(n.hf.data(), ds.n_bins, ds.n_features, ds.bin_stride, n.tg, n.th,
-                             n.rows.size(), cfg);
+                             n.rows.size(), cfg, n.blo, n.bhi);
+    }
+
+    // propagate_monotone_bounds (src/grower_impl.hpp:47-75): both children inherit the
+    // parent's box; a split on a monotone feature fences them at the midpoint of the
+    // child weights bounded by the parent's box.  The child sums are the split's, as the
+    // finder scored them (the device planes carry find's child sums,
+    // src/step/level.hpp:197-205).
+    void propagate(NodeState const &par, Split const &s, NodeState &L, NodeState &R) const
+    {
+        L.blo = R.blo = par.blo;
+        L.bhi = R.bhi = par.bhi;
+        int const mc = monotone_of(s.feature);
+        if (mc == 0) return;
+        double gL = s.gL, hL = s.hL, gR = s.gR, hR = s.hR;
+        if (mode == ORC_HIST_INT) {
+            gL = static_cast<double>(s.igL) * gr.sg;
+            hL = static_cast<double>(s.ihL) * gr.sh;
+            gR = static_cast<double>(s.igR) * gr.sg;
+            hR = static_cast<double>(s.ihR) * gr.sh;
+        }
+        double const wL = bounded_leaf_weight(gL, hL, cfg.l1, cfg.l2, par.blo, par.bhi);
+        double const wR = bounded_leaf_weight(gR, hR, cfg.l1, cfg.l2, par.blo, par.bhi);
+        double const mid = 0.5 * (wL + wR);
+        if (mc > 0) {
+            L.bhi = std::min(L.bhi, mid);
+            R.blo = std::max(R.blo, mid);
+        } else {
+            L.blo = std::max(L.blo, mid);
+            R.bhi = std::min(R.bhi, mid);
+        }
     }
 
     // Children that can never split (they sit at max_depth) take their totals from the
@@ -660,7 +719,7 @@ struct Engine {
             g = n.tg;
             h = n.th;
         }
-        return static_cast<float>(bounded_leaf_weight(g, h, cfg.l1, cfg.l2));
+        return static_cast<float>(bounded_leaf_weight(g, h, cfg.l1, cfg.l2, n.blo, n.bhi));
     }
 
     // Stable partition (src/step/primitives.hpp:113-151): left rows ascending, then
@@ -783,6 +842,7 @@ std::vector<orc_node> grow_depthwise(Engine const &e, std::vector<u32> *leaf_of_
             p.L.id = p.lid;
             p.R.id = p.rid;
             p.L.depth = p.R.depth = par.depth + 1;
+            e.propagate(par, splits[p.parent], p.L, p.R);
         }
         // demote_empty_splits: a split that left one child empty is a leaf again.
         std::vector<Pending> kept;
@@ -867,6 +927,7 @@ std::vector<orc_node> grow_leafwise(Engine const &e, std::vector<u32> *leaf_of_r
         L.id = lid;
         R.id = rid;
         L.depth = R.depth = c.node.depth + 1;
+        e.propagate(c.node, c.split, L, R);
         if (L.rows.empty() || R.rows.empty()) {
             write_leaf(nodes, e, c.node);
             nodes[c.node.id].gain = 0.0F;
@@ -1334,6 +1395,12 @@ void orc_set_feature_mask(const uint8_t *mask, uint32_t n)
 // smallest 64-bit keys splitmix64(seed, iter, f, channel 4), ties to the lower id.
 // fraction >= 1 or F <= 1: every feature.  Pinned by tests/test_oracle_sampler.py (k
 // selected, each feature about equally often, independent draws per iteration).
+void orc_set_monotone(const int8_t *dirs, uint32_t n)
+{
+    g_monotone.clear();
+    for (u32 f = 0; dirs && f < n; ++f) g_monotone.push_back(dirs[f] > 0 ? 1 : (dirs[f] < 0 ? -1 : 0));
+}
+
 void orc_feature_draw(uint32_t n_features, float fraction, uint64_t seed, uint32_t iter, uint8_t *mask)
 {
     if (fraction >= 1.0F || n_features <= 1) {
