@@ -150,6 +150,7 @@ def lib() -> C.CDLL:
         "bh_set_feature_mask": [P, P, u32],
         "bh_set_feature_fraction": [P, C.c_float, u64],
         "bh_set_monotone": [P, P, u32],
+        "bh_set_interaction": [P, P, P, u32],
         "bh_get_trees": [P, P, u32, P, u32, C.POINTER(u32)],
         "bh_max_nodes": [P, C.POINTER(u32)],
         "bh_set_gradients": [P, P, P, u32, P, P, P],
@@ -214,6 +215,21 @@ def device_count() -> int:
     n = C.c_int(0)
     rc = lib().bh_device_count(C.byref(n))
     return n.value if rc == 0 else 0
+
+
+def parse_interaction_groups(entries) -> list[list[int]]:
+    """parse_interaction_groups (src/grower_impl.hpp:84-107): each entry a string of ids
+    separated by ',' or '+' (empty pieces skipped), or a sequence of ids; each group
+    sorted; empty groups dropped."""
+    out = []
+    for e in entries:
+        if isinstance(e, str):
+            ids = [int(p) for p in e.replace("+", ",").split(",") if p != ""]
+        else:
+            ids = [int(v) for v in e]
+        if ids:
+            out.append(sorted(ids))
+    return out
 
 
 class HipContext:
@@ -424,6 +440,17 @@ class HipContext:
             return
         d = np.ascontiguousarray(np.sign(np.asarray(dirs)), np.int8)
         _check(lib().bh_set_monotone(self._h, _ptr(d), d.size))
+
+    def set_interaction_constraints(self, groups) -> None:
+        """Interaction constraints: a list of groups, each a list of feature ids or a
+        string such as "0,1" / "0+1" (the config's form).  None or []: no constraints."""
+        parsed = parse_interaction_groups(groups or [])
+        if not parsed:
+            _check(lib().bh_set_interaction(self._h, None, None, 0))
+            return
+        members = np.ascontiguousarray(np.concatenate([np.asarray(g, np.uint32) for g in parsed]), np.uint32)
+        sizes = np.array([len(g) for g in parsed], np.uint32)
+        _check(lib().bh_set_interaction(self._h, _ptr(members), _ptr(sizes), sizes.size))
 
     def set_feature_fraction(self, fraction: float, seed: int = 0) -> None:
         """Device per-tree feature draw: tree t selects max(1, ceil(fraction * F))

@@ -218,6 +218,10 @@ struct FindScratch {
 };
 
 // Tree-growth settings shared by every kernel of a fit.
+// Interaction constraints: at most kMaxGroups groups over at most 32 * kMaxPathWords
+// features.
+constexpr u32 kMaxGroups = 256, kMaxPathWords = 128;
+
 struct GrowCfg {
     double min_child_hess, min_gain, l2, l1;
     int32_t max_depth, min_data_in_leaf, max_leaves, quantized;
@@ -229,6 +233,12 @@ struct GrowCfg {
     // [lo, hi] leaf-weight box of every tree node id (2 doubles per node).
     int8_t const *mono;
     double *bounds;
+    // Interaction constraints (tree.interaction_constraints, tree_config.hpp:23-26):
+    // n_groups feature sets as path_words-word bitmasks, and every tree node id's path
+    // (the distinct features split on above it) in the same form; n_groups 0: none.
+    u32 const *groups;
+    u32 *paths;
+    u32 n_groups, path_words;
 };
 
 // Compact node for the bin-space walks (score update, predict).

@@ -74,6 +74,19 @@ __device__ __forceinline__ void fd_propagate_bounds(GrowCfg const &c, double con
     c.bounds[2 * rid + 1] = rhi;
 }
 
+// propagate_interaction_state (src/grower_impl.hpp:143-161): both children's path is
+// the parent's plus the split feature.
+__device__ __forceinline__ void fd_propagate_path(GrowCfg const &c, u32 pid, u32 feature, u32 lid, u32 rid)
+{
+    if (!c.n_groups) return;
+    u32 const W = c.path_words;
+    for (u32 w = 0; w < W; ++w) {
+        u32 const v = c.paths[pid * W + w] | (w == (feature >> 5) ? 1u << (feature & 31u) : 0u);
+        c.paths[lid * W + w] = v;
+        c.paths[rid * W + w] = v;
+    }
+}
+
 // A node's leaf weight as float, clamped to its box when constraints are on.
 __device__ __forceinline__ float fd_leaf_value(u32 id, double g, double h, GrowCfg const &c)
 {
@@ -104,6 +117,7 @@ __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restric
     __shared__ i64 s_itot[2];
     __shared__ Acc s_mg[kFindTile], s_mh[kFindTile];
     __shared__ u32 s_ncut[kFindTile], s_dirs[kFindTile], s_nb[kFindTile], s_mc[kFindTile];
+    __shared__ u32 s_allow[kMaxPathWords], s_cover[kMaxGroups], s_single;
 
     // Every launch's frontier buffer holds gridDim.y entries past front_base, so the
     // node loads beside the frontier size.
@@ -209,14 +223,49 @@ __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restric
         }
         return;
     }
+    // Interaction constraints: the features this node may split on (allowed_features,
+    // src/grower_impl.hpp:111-137): the union of every group that contains the node's
+    // whole path, plus the path's feature when the path has one; an empty path allows all.
+    u32 const W = cfg.path_words;
+    if (cfg.n_groups) {
+        u32 const *path = cfg.paths + static_cast<size_t>(node.id) * W;
+        if (threadIdx.x == 0) {
+            u32 pc = 0;
+            for (u32 w = 0; w < W; ++w) pc += __popc(path[w]);
+            s_single = pc;
+        }
+        for (u32 g = threadIdx.x; g < cfg.n_groups; g += blockDim.x) {
+            u32 const *grp = cfg.groups + static_cast<size_t>(g) * W;
+            u32 miss = 0;
+            for (u32 w = 0; w < W; ++w) miss |= path[w] & ~grp[w];
+            s_cover[g] = miss == 0u;
+        }
+        __syncthreads();
+        for (u32 w = threadIdx.x; w < W; w += blockDim.x) {
+            u32 a = 0;
+            if (s_single == 0) {
+                a = ~0u;
+            } else {
+                for (u32 g = 0; g < cfg.n_groups; ++g)
+                    if (s_cover[g]) a |= cfg.groups[static_cast<size_t>(g) * W + w];
+                if (s_single == 1) a |= path[w];
+            }
+            s_allow[w] = a;
+        }
+        // (the tile loop's first barrier publishes s_allow)
+    }
     // Block bx scans feature tiles bx, bx + G, ...
     for (u32 f0 = bx * kFindTile; f0 < F; f0 += G * kFindTile) {
         u32 const nt = min(kFindTile, F - f0);
         __syncthreads(); // the previous tile (or phase 0) is done with the LDS
         // Stage the tile's cells, bin counts and count cells (coalesced, one round trip).
-        // An unselected feature offers no cut (zero bins).
-        if (threadIdx.x < nt)
-            s_nb[threadIdx.x] = (cfg.fmask && !cfg.fmask[f0 + threadIdx.x]) ? 0u : n_bins[f0 + threadIdx.x];
+        // An unselected or disallowed feature offers no cut (zero bins).
+        if (threadIdx.x < nt) {
+            u32 const f = f0 + threadIdx.x;
+            bool const off = (cfg.fmask && !cfg.fmask[f]) ||
+                             (cfg.n_groups && !((s_allow[f >> 5] >> (f & 31u)) & 1u));
+            s_nb[threadIdx.x] = off ? 0u : n_bins[f];
+        }
         for (u32 k = threadIdx.x; k < nt * SB; k += blockDim.x) s_cells[k] = hcells[static_cast<size_t>(f0) * SB + k];
         if (hcnt)
             for (u32 k = threadIdx.x; k < nt * SB; k += blockDim.x) s_cn[k] = hcnt[static_cast<size_t>(f0) * SB + k];

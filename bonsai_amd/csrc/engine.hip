@@ -119,6 +119,10 @@ struct bh_context {
     std::vector<int8_t> mono_host;
     int8_t *mono_dev = nullptr;  // [F]
     double *bounds = nullptr;    // [max_nodes][2]
+    // interaction constraints (bh_set_interaction): groups as feature bitmasks, node paths
+    std::vector<std::vector<u32>> groups_host;
+    u32 *groups_dev = nullptr;   // [n_groups][path_words]
+    u32 *paths = nullptr;        // [max_nodes][path_words]
     // configuration
     bh_fit_config fc{};
     GrowCfg gc{};
@@ -168,13 +172,16 @@ void free_fit(bh_context *c)
                     c->hist[0], c->hist[1], c->partials, c->ctl, c->front[0], c->front[1], c->splits, c->tree,
                     c->ops, c->pch, c->chunk_left, c->hch, c->fops, c->trees_out, c->tree_sizes, c->leaf_rows,
                     c->fscr.part, c->fscr.done, c->cnt[0], c->cnt[1], c->ops4, c->chunk_cls, c->mono_dev,
-                    c->bounds};
+                    c->bounds, c->groups_dev, c->paths};
     for (void *p : ptrs)
         if (p) hipFree(p);
     c->mono_dev = nullptr;
     c->bounds = nullptr;
     c->gc.mono = nullptr;
     c->gc.bounds = nullptr;
+    c->groups_dev = c->paths = nullptr;
+    c->gc.groups = c->gc.paths = nullptr;
+    c->gc.n_groups = c->gc.path_words = 0;
     c->gh0f = c->ghwf[0] = c->ghwf[1] = nullptr;
     c->gh0q = c->ghwq[0] = c->ghwq[1] = nullptr;
     c->maxbits = nullptr;
@@ -230,6 +237,37 @@ int arm_monotone(bh_context *c)
     HIP_TRY(hipMemcpy(c->mono_dev, d.data(), c->F, hipMemcpyHostToDevice));
     c->gc.mono = c->mono_dev;
     c->gc.bounds = c->bounds;
+    return 0;
+}
+
+// Device copies of the interaction groups and the per-node paths for the configured
+// fit; none without groups.  The levelwise grower refuses them
+// (src/grower_impl.hpp:589-595).
+int arm_interaction(bh_context *c)
+{
+    if (c->groups_dev) (void)hipFree(c->groups_dev);
+    if (c->paths) (void)hipFree(c->paths);
+    c->groups_dev = c->paths = nullptr;
+    c->gc.groups = c->gc.paths = nullptr;
+    c->gc.n_groups = c->gc.path_words = 0;
+    if (c->groups_host.empty() || !c->configured) return 0;
+    if (c->fc.grower == BH_LEVELWISE)
+        return fail(1, "interaction_constraints are not supported by the levelwise grower");
+    u32 const W = (c->F + 31u) / 32u;
+    if (W > kMaxPathWords) return fail(1, "interaction constraints support at most 4096 features");
+    std::vector<u32> bits(c->groups_host.size() * W, 0u);
+    for (size_t g = 0; g < c->groups_host.size(); ++g)
+        for (u32 f : c->groups_host[g]) {
+            if (f >= c->F) return fail(1, "interaction group names a feature beyond n_features");
+            bits[g * W + f / 32u] |= 1u << (f % 32u);
+        }
+    HIP_TRY(dalloc(&c->groups_dev, bits.size()));
+    HIP_TRY(dalloc(&c->paths, static_cast<size_t>(c->max_nodes) * W));
+    HIP_TRY(hipMemcpy(c->groups_dev, bits.data(), bits.size() * sizeof(u32), hipMemcpyHostToDevice));
+    c->gc.groups = c->groups_dev;
+    c->gc.paths = c->paths;
+    c->gc.n_groups = static_cast<u32>(c->groups_host.size());
+    c->gc.path_words = W;
     return 0;
 }
 
@@ -311,7 +349,8 @@ int alloc_fit(bh_context *c)
     c->gc.max_leaves = f.max_leaves;
     c->gc.quantized = f.quantized;
     c->configured = true;
-    return arm_monotone(c);
+    if (int r = arm_monotone(c)) return r;
+    return arm_interaction(c);
 }
 
 int ensure_tree_out(bh_context *c, u32 n_more)
@@ -545,7 +584,7 @@ void enqueue_depthwise(bh_context *c)
 void enqueue_grow(bh_context *c)
 {
     enqueue_sample(c);
-    if (c->gc.mono) launch_bounds_root(c->stream, c->bounds);
+    if (c->gc.mono || c->gc.n_groups) launch_constraints_root(c->stream, c->bounds, c->paths, c->gc.path_words);
     if (c->fc.grower == BH_DEPTHWISE) enqueue_depthwise(c);
     else if (c->fc.grower == BH_LEVELWISE) enqueue_levelwise(c->ext, c->stream, c->quant, grow_view(c));
     else enqueue_leafwise(c->ext, c->stream, c->quant, grow_view(c));
@@ -1038,6 +1077,30 @@ int bh_set_monotone(bh_context *c, const int8_t *dirs, uint32_t n)
     if (int r = arm_monotone(c)) {
         c->mono_host = std::move(prev);
         (void)arm_monotone(c);
+        return r;
+    }
+    return 0;
+}
+
+int bh_set_interaction(bh_context *c, const uint32_t *members, const uint32_t *group_sizes, uint32_t n_groups)
+{
+    if (!c) return fail(1, "null context");
+    if (n_groups > kMaxGroups) return fail(1, "at most 256 interaction groups");
+    if (n_groups && (!members || !group_sizes)) return fail(1, "null interaction groups");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<std::vector<u32>> prev = std::move(c->groups_host);
+    c->groups_host.clear();
+    size_t k = 0;
+    for (u32 g = 0; g < n_groups; ++g) {
+        std::vector<u32> grp(members + k, members + k + group_sizes[g]);
+        k += group_sizes[g];
+        if (!grp.empty()) c->groups_host.push_back(std::move(grp)); // empty entries are dropped
+    }
+    drop_graph(c);
+    if (int r = arm_interaction(c)) {
+        c->groups_host = std::move(prev);
+        (void)arm_interaction(c);
         return r;
     }
     return 0;

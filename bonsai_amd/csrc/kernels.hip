@@ -1114,6 +1114,7 @@ __global__ __launch_bounds__(1024) void k_plan_level(NodeInfo const *__restrict_
                 op.alive = 1;
                 ops[rank] = op;
                 fd_propagate_bounds(cfg, scales, nd.id, s, lid, rid);
+                fd_propagate_path(cfg, nd.id, s.feature, lid, rid);
             } else {
                 write_leaf(tree, nd.id, node_leaf_value(nd, cfg, scales));
             }
@@ -1906,6 +1907,7 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
                     tree[lid].cover = static_cast<float>(nL);
                     tree[rid].cover = static_cast<float>(nR);
                     fd_propagate_bounds(cfg, scales, nd.id, sp, lid, rid);
+                    fd_propagate_path(cfg, nd.id, sp.feature, lid, rid);
                     PartOp op{};
                     op.parent = i;
                     op.feature = sp.feature;
@@ -2458,18 +2460,21 @@ __global__ __launch_bounds__(256) void k_feature_draw(TreeCtl const *__restrict_
     if (f < F) mask[f] = rank < k ? 1 : 0;
 }
 
-// The root's monotone box is unbounded (SplitInput lo/hi defaults, include/bonsai/split.hpp).
-__global__ void k_bounds_root(double *__restrict__ bounds)
+// The root's monotone box is unbounded and its interaction path empty (SplitInput
+// defaults, include/bonsai/split.hpp:19-34).
+__global__ void k_constraints_root(double *__restrict__ bounds, u32 *__restrict__ paths, u32 path_words)
 {
-    if (threadIdx.x == 0) {
+    if (bounds && threadIdx.x == 0) {
         bounds[0] = -__builtin_huge_val();
         bounds[1] = __builtin_huge_val();
     }
+    if (paths)
+        for (u32 w = threadIdx.x; w < path_words; w += blockDim.x) paths[w] = 0u;
 }
 
-void launch_bounds_root(hipStream_t s, double *bounds)
+void launch_constraints_root(hipStream_t s, double *bounds, u32 *paths, u32 path_words)
 {
-    hipLaunchKernelGGL(k_bounds_root, dim3(1), dim3(64), 0, s, bounds);
+    hipLaunchKernelGGL(k_constraints_root, dim3(1), dim3(64), 0, s, bounds, paths, path_words);
 }
 
 void launch_feature_draw(hipStream_t s, TreeCtl const *ctl, u32 F, u32 k, u64 seed, u8 *mask)

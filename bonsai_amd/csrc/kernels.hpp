@@ -74,8 +74,9 @@ void launch_bernoulli(hipStream_t s, TreeCtl *ctl, u32 n, u64 seed, u32 thr, u32
 // Per-tree feature selection for iteration ctl->iter: mask[f] = 1 for the k smallest
 // (key, f) pairs.
 void launch_feature_draw(hipStream_t s, TreeCtl const *ctl, u32 F, u32 k, u64 seed, u8 *mask);
-// Monotone constraints: node 0's [lo, hi] box = (-inf, +inf) at the start of a tree.
-void launch_bounds_root(hipStream_t s, double *bounds);
+// Monotone / interaction constraints at the start of a tree: node 0's [lo, hi] box =
+// (-inf, +inf) and its path empty (either pointer may be null).
+void launch_constraints_root(hipStream_t s, double *bounds, u32 *paths, u32 path_words);
 // maxbits (training, quantized): also the next iteration's gradient maxima, from the
 // updated scores and the labels, into slot (ctl->iter + 1) & 1.
 void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode const *tree, TreeCtl const *ctl,

@@ -810,6 +810,7 @@ __device__ __forceinline__ void lw_select_round(TreeCtl *__restrict__ ctl, LwCtl
         parent_out[0] = c.node;
         split_out[0] = c.split;
         fd_propagate_bounds(cfg, scales, c.node.id, c.split, lid, rid);
+        fd_propagate_path(cfg, c.node.id, c.split.feature, lid, rid);
         PartOp op{};
         op.parent = 0;
         op.feature = c.split.feature;

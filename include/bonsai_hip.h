@@ -175,6 +175,15 @@ int bh_set_feature_fraction(bh_context *ctx, float fraction, uint64_t seed);
  * include/bonsai/split.hpp:141-150).  Kept across bh_configure; refused with the
  * levelwise grower (src/grower_impl.hpp:583-588).  dirs NULL / n 0 clears. */
 int bh_set_monotone(bh_context *ctx, const int8_t *dirs, uint32_t n);
+/* Interaction constraints (tree.interaction_constraints, parsed as
+ * parse_interaction_groups does, src/grower_impl.hpp:84-107): n_groups groups, group g
+ * holding group_sizes[g] feature ids, concatenated in members (empty groups dropped; at
+ * most 256 groups, 4,096 features).  A node may split on the union of the groups that
+ * contain every distinct feature on its path, plus that feature when the path has one;
+ * the root on any (allowed_features, src/grower_impl.hpp:111-137; the finder's skip,
+ * src/split.cpp:111-114).  Kept across bh_configure; refused with the levelwise grower
+ * (src/grower_impl.hpp:589-595) and for ids beyond n_features.  n_groups 0 clears. */
+int bh_set_interaction(bh_context *ctx, const uint32_t *members, const uint32_t *group_sizes, uint32_t n_groups);
 /* Trees grown so far: nodes[t * max_nodes + i], sizes[t]; returns via *n_trees. */
 int bh_get_trees(bh_context *ctx, bh_node *nodes, uint32_t max_nodes, uint32_t *sizes,
                  uint32_t cap_trees, uint32_t *n_trees);

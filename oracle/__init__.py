@@ -149,6 +149,17 @@ class Oracle:
         d = np.ascontiguousarray(dirs, np.int8)
         self.L.orc_set_monotone(P(d), C.c_uint32(d.size))
 
+    def set_interaction(self, groups):
+        """Interaction groups (lists of feature ids) for the following depthwise/leafwise
+        growers (None or []: none)."""
+        groups = [list(g) for g in (groups or []) if len(g)]
+        if not groups:
+            self.L.orc_set_interaction(None, None, C.c_uint32(0))
+            return
+        m = np.ascontiguousarray(np.concatenate([np.asarray(g, np.uint32) for g in groups]), np.uint32)
+        s = np.array([len(g) for g in groups], np.uint32)
+        self.L.orc_set_interaction(P(m), P(s), C.c_uint32(s.size))
+
     def feature_draw(self, n_features, fraction, seed, it):
         """The device's per-iteration feature subsample as a 0/1 mask."""
         out = np.zeros(n_features, np.uint8)

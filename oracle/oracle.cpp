@@ -269,6 +269,25 @@ std::vector<u8> g_feature_mask;
 std::vector<int8_t> g_monotone;
 inline int monotone_of(u32 f) { return f < g_monotone.size() ? g_monotone[f] : 0; }
 
+// Interaction groups (parse_interaction_groups, src/grower_impl.hpp:84-107: each sorted,
+// empty ones dropped).  Set by orc_set_interaction for the depthwise and leafwise
+// growers that follow.
+std::vector<std::vector<u32>> g_groups;
+
+// allowed_features (src/grower_impl.hpp:111-137): every group containing the whole
+// path contributes its members, and a one-feature path may keep splitting on that
+// feature; empty result = all allowed (no groups or an empty path).
+std::vector<u8> allowed_features(std::vector<u32> const &path, u32 n_features)
+{
+    if (g_groups.empty() || path.empty()) return {};
+    std::vector<u8> allowed(n_features, 0);
+    for (auto const &g : g_groups)
+        if (std::includes(g.begin(), g.end(), path.begin(), path.end()))
+            for (u32 f : g) allowed[f] = 1;
+    if (path.size() == 1) allowed[path.front()] = 1;
+    return allowed;
+}
+
 // src/split.cpp:143-153: a candidate whose bounded child weights break the feature's
 // direction is skipped.
 inline bool breaks_monotone(u32 f, double gL, double hL, double gR, double hR, double lo, double hi,
@@ -330,13 +349,15 @@ inline bool candidate(double lg, double lh, double mg, double mh, double rg, dou
 }
 
 Split find_node_f32(float const *hist, u16 const *n_bins, u32 F, u32 SB, float tg, float th,
-                    u64 n_rows, Cfg const &c, double lo = -kInf, double hi = kInf)
+                    u64 n_rows, Cfg const &c, double lo = -kInf, double hi = kInf,
+                    u8 const *allowed = nullptr)
 {
     Split best;
     if (n_rows < 2 * static_cast<u64>(c.min_data_in_leaf)) return best;
     double const node_score = score(tg, th, c.l1, c.l2);
     for (u32 f = 0; f < F; ++f) {
         if (!feature_selected(f)) continue;
+        if (allowed && !allowed[f]) continue; // src/split.cpp:111-114
         Split cand;
         u32 const nb = n_bins[f];
         if (nb == 0) continue;
@@ -374,7 +395,8 @@ Split find_node_f32(float const *hist, u16 const *n_bins, u32 F, u32 SB, float t
 // Integer cells: the same candidate order, sums exact in int64 and converted to double
 // by the per-iteration scale before scoring.
 Split find_node_int(i64 const *hist, u16 const *n_bins, u32 F, u32 SB, i64 tg, i64 th,
-                    double sg, double sh, u64 n_rows, Cfg const &c, double lo = -kInf, double hi = kInf)
+                    double sg, double sh, u64 n_rows, Cfg const &c, double lo = -kInf, double hi = kInf,
+                    u8 const *allowed = nullptr)
 {
     Split best;
     if (n_rows < 2 * static_cast<u64>(c.min_data_in_leaf)) return best;
@@ -382,6 +404,7 @@ Split find_node_int(i64 const *hist, u16 const *n_bins, u32 F, u32 SB, i64 tg, i
         score(static_cast<double>(tg) * sg, static_cast<double>(th) * sh, c.l1, c.l2);
     for (u32 f = 0; f < F; ++f) {
         if (!feature_selected(f)) continue;
+        if (allowed && !allowed[f]) continue; // src/split.cpp:111-114
         Split cand;
         u32 const nb = n_bins[f];
         if (nb == 0) continue;
@@ -573,6 +596,8 @@ struct NodeState {
     i64 itg = 0, ith = 0;
     int depth = 0;
     double blo = -kInf, bhi = kInf; // monotone box of the node's leaf weight
+    std::vector<u32> path;          // distinct features split on above (sorted)
+    std::vector<u8> allowed;        // interaction-allowed features (empty: all)
 };
 
 struct Engine {
@@ -657,9 +682,11 @@ struct Engine {
     {
         if (mode == ORC_HIST_INT)
             return find_node_int(n.hi.data(), ds.n_bins, ds.n_features, ds.bin_stride, n.itg,
-                                 n.ith, gr.sg, gr.sh, n.rows.size(), cfg, n.blo, n.bhi);
+                                 n.ith, gr.sg, gr.sh, n.rows.size(), cfg, n.blo, n.bhi,
+                                 n.allowed.empty() ? nullptr : n.allowed.data());
         return find_node_f32(n.hf.data(), ds.n_bins, ds.n_features, ds.bin_stride, n.tg, n.th,
-                             n.rows.size(), cfg, n.blo, n.bhi);
+                             n.rows.size(), cfg, n.blo, n.bhi,
+                                 n.allowed.empty() ? nullptr : n.allowed.data());
     }
 
     // propagate_monotone_bounds (src/grower_impl.hpp:47-75): both children inherit the
@@ -671,6 +698,15 @@ struct Engine {
     {
         L.blo = R.blo = par.blo;
         L.bhi = R.bhi = par.bhi;
+        if (!g_groups.empty()) { // propagate_interaction_state (src/grower_impl.hpp:143-161)
+            std::vector<u32> path = par.path;
+            if (!std::binary_search(path.begin(), path.end(), s.feature))
+                path.insert(std::upper_bound(path.begin(), path.end(), s.feature), s.feature);
+            L.allowed = allowed_features(path, ds.n_features);
+            R.allowed = L.allowed;
+            L.path = path;
+            R.path = std::move(path);
+        }
         int const mc = monotone_of(s.feature);
         if (mc == 0) return;
         double gL = s.gL, hL = s.hL, gR = s.gR, hR = s.hR;
@@ -1399,6 +1435,18 @@ void orc_set_monotone(const int8_t *dirs, uint32_t n)
 {
     g_monotone.clear();
     for (u32 f = 0; dirs && f < n; ++f) g_monotone.push_back(dirs[f] > 0 ? 1 : (dirs[f] < 0 ? -1 : 0));
+}
+
+void orc_set_interaction(const uint32_t *members, const uint32_t *group_sizes, uint32_t n_groups)
+{
+    g_groups.clear();
+    size_t k = 0;
+    for (u32 g = 0; g < n_groups; ++g) {
+        std::vector<u32> grp(members + k, members + k + group_sizes[g]);
+        k += group_sizes[g];
+        std::sort(grp.begin(), grp.end());
+        if (!grp.empty()) g_groups.push_back(std::move(grp));
+    }
 }
 
 void orc_feature_draw(uint32_t n_features, float fraction, uint64_t seed, uint32_t iter, uint8_t *mask)

@@ -97,6 +97,9 @@ void orc_set_feature_mask(const uint8_t *mask, uint32_t n);
 /* Monotone directions (+1, -1, 0 by sign) for the depthwise and leafwise growers that
  * follow; NULL or n = 0 clears. */
 void orc_set_monotone(const int8_t *dirs, uint32_t n);
+/* Interaction groups (group g: group_sizes[g] ids, concatenated in members) for the
+ * depthwise and leafwise growers that follow; n_groups 0 clears. */
+void orc_set_interaction(const uint32_t *members, const uint32_t *group_sizes, uint32_t n_groups);
 /* The product's device draw for feature_fraction < 1: k = max(1, ceil(fraction * F))
  * features with the smallest counter-based keys of (seed, iter, f); mask out [F]. */
 void orc_feature_draw(uint32_t n_features, float fraction, uint64_t seed, uint32_t iter, uint8_t *mask);
