@@ -103,6 +103,11 @@ class FitConfig:
     lambda_l1: float = 0.0
     learning_rate: float = 0.05
     seed: int = 42
+    # TreeConfig's sampling and constraint fields, applied by HipContext.configure
+    feature_fraction: float = 1.0         # per-tree feature draw on the device; 1 = all
+    feature_seed: int = 2
+    monotone_constraints: tuple = ()      # +1 / -1 / 0 per feature, trailing entries free
+    interaction_constraints: tuple = ()   # "0,1" / "0+1" strings or id lists
 
     def to_c(self) -> FitConfigC:
         if self.grower not in GROWERS:
@@ -294,6 +299,9 @@ class HipContext:
     def configure(self, cfg: FitConfig) -> None:
         c = cfg.to_c()
         _check(lib().bh_configure(self._h, C.byref(c)))
+        self.set_feature_fraction(cfg.feature_fraction, cfg.feature_seed)
+        self.set_monotone(list(cfg.monotone_constraints) or None)
+        self.set_interaction_constraints(list(cfg.interaction_constraints) or None)
         self.cfg = cfg
 
     @property

@@ -96,3 +96,26 @@ def test_interaction_refusals_and_clearing(hip, orc, tall):
     b = ctx.grow_tree()[0]
     ctx.close()
     assert a.tobytes() == b.tobytes()
+
+
+def test_fit_config_fields_equal_the_setters(hip, orc, tall):
+    """FitConfig's TreeConfig fields (feature_fraction / feature_seed,
+    monotone_constraints, interaction_constraints) reach the device through configure."""
+    bins, nb, X, y = tall
+    g, h = grad_of(orc, y, seed=41)
+    fields = dict(feature_fraction=0.6, feature_seed=9, monotone_constraints=(1, 0, -1),
+                  interaction_constraints=("0,1,2,10", "3+4+24", "2,5,6,7,8,9"))
+    a_ctx = ctx_for(hip, bins, nb, grower="hip_depthwise", quantized=True, max_depth=6, **fields)
+    a_ctx.set_gradients(g, h)
+    a = a_ctx.grow_tree()[0]
+    a_ctx.close()
+    b_ctx = ctx_for(hip, bins, nb, grower="hip_depthwise", quantized=True, max_depth=6)
+    b_ctx.set_gradients(g, h)
+    b_ctx.set_feature_fraction(0.6, seed=9)
+    b_ctx.set_monotone([1, 0, -1])
+    b_ctx.set_interaction_constraints(GROUPS)
+    b = b_ctx.grow_tree()[0]
+    b_ctx.close()
+    assert a.tobytes() == b.tobytes()
+    with pytest.raises(hip.ConfigError):
+        ctx_for(hip, bins, nb, grower="hip_levelwise", quantized=True, max_depth=4, monotone_constraints=(1,))
