@@ -108,6 +108,9 @@ class FitConfig:
     feature_seed: int = 2
     monotone_constraints: tuple = ()      # +1 / -1 / 0 per feature, trailing entries free
     interaction_constraints: tuple = ()   # "0,1" / "0+1" strings or id lists
+    # SamplerConfig.subsample (include/bonsai/config/sampler_config.hpp:8): the device
+    # Bernoulli row draw per iteration, seeded by `seed`; 1 = every row
+    subsample: float = 1.0
 
     def to_c(self) -> FitConfigC:
         if self.grower not in GROWERS:
@@ -299,6 +302,7 @@ class HipContext:
     def configure(self, cfg: FitConfig) -> None:
         c = cfg.to_c()
         _check(lib().bh_configure(self._h, C.byref(c)))
+        self.set_bernoulli(cfg.subsample, cfg.seed)
         self.set_feature_fraction(cfg.feature_fraction, cfg.feature_seed)
         self.set_monotone(list(cfg.monotone_constraints) or None)
         self.set_interaction_constraints(list(cfg.interaction_constraints) or None)

@@ -357,3 +357,21 @@ def test_feature_sampling_rejects_bad_input(hip, tall):
     ctx.set_feature_mask(None)
     ctx.set_feature_fraction(1.0)
     ctx.close()
+
+
+def test_fit_config_subsample_is_the_bernoulli_sampler(hip, orc, tall):
+    bins, nb, X, y = tall
+    g, h = grad_of(orc, y, seed=43)
+    out = []
+    for via_config in (True, False):
+        kw = dict(subsample=0.45, seed=12) if via_config else dict(seed=12)
+        ctx = ctx_for(hip, bins, nb, grower="hip_depthwise", quantized=True, max_depth=5, **kw)
+        ctx.set_gradients(g, h)
+        if not via_config:
+            ctx.set_bernoulli(0.45, seed=12)
+        out.append(ctx.grow_tree()[0])
+        ctx.close()
+    assert out[0].tobytes() == out[1].tobytes()
+    assert out[0]["cover"][0] == orc.bernoulli_rows(bins.shape[0], 0.45, 12, 0).size
+    with pytest.raises(hip.ConfigError):
+        ctx_for(hip, bins, nb, grower="hip_depthwise", quantized=True, max_depth=5, subsample=0.0)
