@@ -98,7 +98,9 @@ __device__ __forceinline__ float fd_leaf_value(u32 id, double g, double h, GrowC
     return static_cast<float>(fd_bounded_weight(g, h, c.l1, c.l2, lo, hi));
 }
 
-template <bool Q>
+// CON: the feature mask and the monotone / interaction constraints are compiled in
+// (launches without any use the leaner CON = false body).
+template <bool Q, bool CON>
 __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restrict__ ctl, void const *__restrict__ hist,
                           u16 const *__restrict__ n_bins, u32 F, u32 SB, GrowCfg cfg, double const *__restrict__ scales,
                           SplitRec *__restrict__ out, u32 front_base, u32 n_front_override,
@@ -151,7 +153,7 @@ __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restric
     __shared__ u32 s_last;
     double node_score = 0.0, tg = 0.0, th = 0.0;
     double node_lo = -__builtin_huge_val(), node_hi = __builtin_huge_val(); // monotone box
-    if (cfg.mono) {
+    if (CON && cfg.mono) {
         node_lo = cfg.bounds[2 * node.id];
         node_hi = cfg.bounds[2 * node.id + 1];
     }
@@ -227,7 +229,7 @@ __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restric
     // src/grower_impl.hpp:111-137): the union of every group that contains the node's
     // whole path, plus the path's feature when the path has one; an empty path allows all.
     u32 const W = cfg.path_words;
-    if (cfg.n_groups) {
+    if (CON && cfg.n_groups) {
         u32 const *path = cfg.paths + static_cast<size_t>(node.id) * W;
         if (threadIdx.x == 0) {
             u32 pc = 0;
@@ -262,8 +264,8 @@ __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restric
         // An unselected or disallowed feature offers no cut (zero bins).
         if (threadIdx.x < nt) {
             u32 const f = f0 + threadIdx.x;
-            bool const off = (cfg.fmask && !cfg.fmask[f]) ||
-                             (cfg.n_groups && !((s_allow[f >> 5] >> (f & 31u)) & 1u));
+            bool const off = CON && ((cfg.fmask && !cfg.fmask[f]) ||
+                             (cfg.n_groups && !((s_allow[f >> 5] >> (f & 31u)) & 1u)));
             s_nb[threadIdx.x] = off ? 0u : n_bins[f];
         }
         for (u32 k = threadIdx.x; k < nt * SB; k += blockDim.x) s_cells[k] = hcells[static_cast<size_t>(f0) * SB + k];
@@ -434,7 +436,7 @@ __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restric
                     hR = right_h + (!dl ? mh : 0.0);
                 }
                 if (hL < cfg.min_child_hess || hR < cfg.min_child_hess) continue;
-                if (cfg.mono) {
+                if (CON && cfg.mono) {
                     // src/split.cpp:143-153: a candidate whose bounded child weights
                     // break the feature's direction is skipped.
                     int const mc = cfg.mono[f0 + tt];

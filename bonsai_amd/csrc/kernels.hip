@@ -1021,7 +1021,7 @@ u32 find_blocks_per_node(u32 F)
     return std::max(1u, std::min<u32>(std::min(tuning().find_blocks, kFindMaxBlocks), (F + kFindTile - 1) / kFindTile));
 }
 
-template <bool Q>
+template <bool Q, bool CON>
 __global__ __launch_bounds__(kFindThreads) void k_find(NodeInfo *__restrict__ front, TreeCtl const *__restrict__ ctl,
                                                      void const *__restrict__ hist, u16 const *__restrict__ n_bins,
                                                      u32 F, u32 SB, GrowCfg cfg, double const *__restrict__ scales,
@@ -1029,7 +1029,8 @@ __global__ __launch_bounds__(kFindThreads) void k_find(NodeInfo *__restrict__ fr
                                                      SplitRec *__restrict__ part, u32 *__restrict__ done,
                                                      u32 const *__restrict__ cnt)
 {
-    find_node<Q>(front, ctl, hist, n_bins, F, SB, cfg, scales, out, front_base, n_front_override, part, done, cnt);
+    find_node<Q, CON>(front, ctl, hist, n_bins, F, SB, cfg, scales, out, front_base, n_front_override, part, done,
+                      cnt);
 }
 
 // ---------------------------------------------------------------------------------
@@ -2680,8 +2681,10 @@ hipError_t set_hist_lds_limits()
     raise_lds(reinterpret_cast<void const *>(&k_hist_q), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_hist_q2), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_hist<false, 4>), max_shared);
-    raise_lds(reinterpret_cast<void const *>(&k_find<true>), max_shared);
-    raise_lds(reinterpret_cast<void const *>(&k_find<false>), max_shared);
+    raise_lds(reinterpret_cast<void const *>(&k_find<true, false>), max_shared);
+    raise_lds(reinterpret_cast<void const *>(&k_find<false, false>), max_shared);
+    raise_lds(reinterpret_cast<void const *>(&k_find<true, true>), max_shared);
+    raise_lds(reinterpret_cast<void const *>(&k_find<false, true>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_part_scatter4), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_part_scatter_v), max_shared);
     return hipSuccess;
@@ -2723,12 +2726,11 @@ void launch_find(hipStream_t s, bool q, NodeInfo *front, TreeCtl const *ctl, voi
                                                 hipSuccess)
         cus = 256;
     u32 const threads = static_cast<u64>(G) * max_front > static_cast<u64>(cus) ? kFindThreads / 2u : kFindThreads;
-    if (q)
-        hipLaunchKernelGGL(k_find<true>, grid, dim3(threads), lds, s, front, ctl, hist, n_bins, F, SB, cfg, scales, out,
-                           front_base, n_front_override, fs.part, fs.done, cnt);
-    else
-        hipLaunchKernelGGL(k_find<false>, grid, dim3(threads), lds, s, front, ctl, hist, n_bins, F, SB, cfg, scales, out,
-                           front_base, n_front_override, fs.part, fs.done, nullptr);
+    bool const con = cfg.fmask || cfg.mono || cfg.n_groups;
+    auto const fn = q ? (con ? &k_find<true, true> : &k_find<true, false>)
+                      : (con ? &k_find<false, true> : &k_find<false, false>);
+    hipLaunchKernelGGL(fn, grid, dim3(threads), lds, s, front, ctl, hist, n_bins, F, SB, cfg, scales, out, front_base,
+                       n_front_override, fs.part, fs.done, q ? cnt : nullptr);
 }
 
 void launch_plan_level(hipStream_t s, NodeInfo const *front, SplitRec const *splits, TreeCtl *ctl, DevNode *tree,

@@ -886,7 +886,7 @@ struct LwNext {
     u32 next;
 };
 
-template <bool Q, bool COUNTED>
+template <bool Q, bool COUNTED, bool CON>
 __global__ __launch_bounds__(kFindThreads) void k_lw_find(NodeInfo *__restrict__ kids, TreeCtl *__restrict__ ctl,
                                                         LwCtl *__restrict__ lw, void const *__restrict__ hist,
                                                         u16 const *__restrict__ n_bins, u32 F, u32 SB, GrowCfg cfg,
@@ -895,7 +895,7 @@ __global__ __launch_bounds__(kFindThreads) void k_lw_find(NodeInfo *__restrict__
                                                         u32 *__restrict__ done, u32 const *__restrict__ cnt, LwNext nx)
 {
     __shared__ u32 s_last_wg;
-    find_node<Q>(kids, ctl, hist, n_bins, F, SB, cfg, scales, splits, 0u, 0u, part, done, cnt);
+    find_node<Q, CON>(kids, ctl, hist, n_bins, F, SB, cfg, scales, splits, 0u, 0u, part, done, cnt);
     if (!nx.next) return;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1085,8 +1085,10 @@ hipError_t ext_set_limits()
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&max_shared, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (e != hipSuccess) return e;
-    void const *fns[] = {reinterpret_cast<void const *>(&k_lw_find<true, true>),
-                         reinterpret_cast<void const *>(&k_lw_find<false, false>)};
+    void const *fns[] = {reinterpret_cast<void const *>(&k_lw_find<true, true, false>),
+                         reinterpret_cast<void const *>(&k_lw_find<false, false, false>),
+                         reinterpret_cast<void const *>(&k_lw_find<true, true, true>),
+                         reinterpret_cast<void const *>(&k_lw_find<false, false, true>)};
     for (void const *fn : fns) {
         hipFuncAttributes a{};
         if (hipFuncGetAttributes(&a, fn) != hipSuccess) continue;
@@ -1166,8 +1168,10 @@ void launch_lw_find(hipStream_t st, GrowView const &v, LwCtl *lw, u32 *cnt, LwNe
 {
     size_t const lds = static_cast<size_t>(kFindTile) * v.SB * (Q ? (cnt ? 36 : 32) : 24);
     dim3 const grid(find_blocks_per_node(v.F), 2);
-    hipLaunchKernelGGL((k_lw_find<Q, COUNTED>), grid, dim3(kFindThreads), lds, st, v.front[0], v.ctl, lw, v.hist[0],
-                       v.d_nbins, v.F, v.SB, v.gc, v.scales, v.splits, v.fscr.part, v.fscr.done, cnt, nx);
+    bool const con = v.gc.fmask || v.gc.mono || v.gc.n_groups;
+    auto const fn = con ? &k_lw_find<Q, COUNTED, true> : &k_lw_find<Q, COUNTED, false>;
+    hipLaunchKernelGGL(fn, grid, dim3(kFindThreads), lds, st, v.front[0], v.ctl, lw, v.hist[0], v.d_nbins, v.F, v.SB,
+                       v.gc, v.scales, v.splits, v.fscr.part, v.fscr.done, cnt, nx);
 }
 
 void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
