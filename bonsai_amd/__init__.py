@@ -162,6 +162,7 @@ def lib() -> C.CDLL:
         "bh_get_trees": [P, P, u32, P, u32, C.POINTER(u32)],
         "bh_max_nodes": [P, C.POINTER(u32)],
         "bh_set_gradients": [P, P, P, u32, P, P, P],
+        "bh_get_gradients": [P, P, P, P, P, P],
         "bh_grow_tree": [P, P, u32, C.POINTER(u32), P],
         "bh_histogram": [P, P, u64, P, P],
         "bh_find_split": [P, P, P, u64, u32, i32, C.c_double, C.c_double, P, P],
@@ -300,7 +301,15 @@ class HipContext:
         _check(lib().bh_upload_labels(self._h, _ptr(labels)))
 
     def configure(self, cfg: FitConfig) -> None:
+        """Configure the fit.  The FitConfig owns the sampling and constraint fields:
+        whatever an earlier configure or setter left (row sampler, feature selection,
+        monotone and interaction constraints) is cleared and cfg's are applied."""
         c = cfg.to_c()
+        if cfg.grower == "hip_levelwise" and (any(int(v) for v in cfg.monotone_constraints)
+                                               or parse_interaction_groups(cfg.interaction_constraints)):
+            raise ConfigError("monotone / interaction constraints are not supported by the levelwise grower")
+        self.set_monotone(None)
+        self.set_interaction_constraints(None)
         _check(lib().bh_configure(self._h, C.byref(c)))
         self.set_bernoulli(cfg.subsample, cfg.seed)
         self.set_feature_fraction(cfg.feature_fraction, cfg.feature_seed)
@@ -359,6 +368,20 @@ class HipContext:
         sc = np.zeros(2, np.float64)
         _check(lib().bh_set_gradients(self._h, _ptr(g), _ptr(h), it, _ptr(gq), _ptr(hq), _ptr(sc)))
         return gq, hq, sc
+
+    def get_gradients(self):
+        """The current iteration's gradients in row order: (g, h) float32 for an FP32
+        fit, (gq, hq, scales) for a quantized one."""
+        if self.cfg.quantized:
+            gq = np.zeros(self.n_rows, np.int8)
+            hq = np.zeros(self.n_rows, np.int16)
+            sc = np.zeros(2, np.float64)
+            _check(lib().bh_get_gradients(self._h, None, None, _ptr(gq), _ptr(hq), _ptr(sc)))
+            return gq, hq, sc
+        g = np.zeros(self.n_rows, np.float32)
+        h = np.zeros(self.n_rows, np.float32)
+        _check(lib().bh_get_gradients(self._h, _ptr(g), _ptr(h), None, None, None))
+        return g, h
 
     def grow_tree(self) -> tuple[np.ndarray, np.ndarray]:
         cap = self.max_nodes()

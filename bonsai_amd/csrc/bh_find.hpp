@@ -161,11 +161,15 @@ __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restric
     // Quantized totals are exact integers, equal over every feature's cells: a block that
     // scans takes them from the first feature of its first tile (no extra round trip).
     bool const tot_from_tile = Q && can && !had_totals;
-    // Phase 0: node totals from the first feature's cells (Histogram::totals,
-    // include/bonsai/histogram.hpp:76-87) unless a parent split already set them.
+    // Phase 0: node totals from the first selected feature's cells (Histogram::totals,
+    // include/bonsai/histogram.hpp:76-87; SplitInput::totals skips unselected features,
+    // include/bonsai/split.hpp:36-53) unless a parent split already set them.
     if (!had_totals && !tot_from_tile) {
-        u32 const nb0 = n_bins[0];
-        for (u32 k = threadIdx.x; k < nb0; k += blockDim.x) s_cells[k] = hcells[k];
+        u32 f0 = 0;
+        if (CON && cfg.fmask)
+            while (f0 + 1u < F && !cfg.fmask[f0]) ++f0;
+        u32 const nb0 = n_bins[f0];
+        for (u32 k = threadIdx.x; k < nb0; k += blockDim.x) s_cells[k] = hcells[static_cast<size_t>(f0) * SB + k];
         __syncthreads();
         if (Q) {
             // Exact integer totals: any order (one wave reduces feature 0's cells).

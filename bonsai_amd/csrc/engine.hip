@@ -349,8 +349,10 @@ int alloc_fit(bh_context *c)
     c->gc.max_leaves = f.max_leaves;
     c->gc.quantized = f.quantized;
     c->configured = true;
-    if (int r = arm_monotone(c)) return r;
-    return arm_interaction(c);
+    int r = arm_monotone(c);
+    if (r == 0) r = arm_interaction(c);
+    if (r != 0) free_fit(c); // a fit whose constraints could not be armed is no fit
+    return r;
 }
 
 int ensure_tree_out(bh_context *c, u32 n_more)
@@ -930,6 +932,15 @@ int bh_configure(bh_context *c, const bh_fit_config *cfg)
     if (cfg->grower == BH_LEAFWISE && (cfg->max_leaves < 2 || cfg->max_leaves > 1024))
         return fail(1, "max_leaves must be in [2, 1024] for the leafwise grower");
     if (cfg->grower == BH_LEVELWISE && cfg->max_depth > 11) return fail(1, "levelwise max_depth must be <= 11");
+    if (cfg->grower == BH_LEVELWISE) {
+        // refused before anything is freed, so the previous fit stays as it was
+        // (src/grower_impl.hpp:583-595)
+        bool mono = false;
+        for (int8_t v : c->mono_host) mono = mono || v != 0;
+        if (mono) return fail(1, "monotone_constraints are not supported by the levelwise grower");
+        if (!c->groups_host.empty())
+            return fail(1, "interaction_constraints are not supported by the levelwise grower");
+    }
     HIP_TRY(hipSetDevice(c->device));
     hipStreamSynchronize(c->stream);
     free_fit(c);
@@ -1430,6 +1441,36 @@ int bh_set_gradients(bh_context *c, const float *grad, const float *hess, uint32
         scales[0] = sc[0];
         scales[1] = sc[1];
     }
+    if (gq || hq) {
+        std::vector<u32> q(c->N);
+        HIP_TRY(hipMemcpy(q.data(), c->gh0q, c->N * sizeof(u32), hipMemcpyDeviceToHost));
+        for (u64 i = 0; i < c->N; ++i) {
+            if (gq) gq[i] = static_cast<int8_t>(static_cast<int16_t>(q[i] >> 16));
+            if (hq) hq[i] = static_cast<int16_t>(q[i] & 0xFFFFu);
+        }
+    }
+    return 0;
+}
+
+int bh_get_gradients(bh_context *c, float *grad, float *hess, int8_t *gq, int16_t *hq, double *scales)
+{
+    if (int r = check_ready(c)) return r;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (!c->quant) {
+        if (gq || hq || scales) return fail(1, "an FP32 fit keeps float gradients: pass grad / hess");
+        if (grad || hess) {
+            std::vector<float2> gh(c->N);
+            HIP_TRY(hipMemcpy(gh.data(), c->gh0f, c->N * sizeof(float2), hipMemcpyDeviceToHost));
+            for (u64 i = 0; i < c->N; ++i) {
+                if (grad) grad[i] = gh[i].x;
+                if (hess) hess[i] = gh[i].y;
+            }
+        }
+        return 0;
+    }
+    if (grad || hess) return fail(1, "a quantized fit keeps integer gradients: pass gq / hq / scales");
+    if (scales) HIP_TRY(hipMemcpy(scales, c->scales, 2 * sizeof(double), hipMemcpyDeviceToHost));
     if (gq || hq) {
         std::vector<u32> q(c->N);
         HIP_TRY(hipMemcpy(q.data(), c->gh0q, c->N * sizeof(u32), hipMemcpyDeviceToHost));

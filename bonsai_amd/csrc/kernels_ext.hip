@@ -60,20 +60,25 @@ __device__ __forceinline__ void put_leaf(DevNode *tree, u32 id, float v)
 }
 
 // Node totals of a frontier (Histogram::totals, include/bonsai/histogram.hpp:76-87)
-// when no parent split supplied them.  One workgroup per node.
+// from the first selected feature's cells (include/bonsai/split.hpp:36-53) when no
+// parent split supplied them.  One workgroup per node.
 template <bool Q>
 __global__ __launch_bounds__(64) void k_totals(NodeInfo *__restrict__ front, TreeCtl const *__restrict__ ctl,
                                                void const *__restrict__ hist, u16 const *__restrict__ n_bins, u32 F,
-                                               u32 SB)
+                                               u32 SB, u8 const *__restrict__ fmask)
 {
     u32 const i = blockIdx.x;
     if (i >= ctl->n_front || ctl->done) return;
     NodeInfo nd = front[i];
     if (nd.totals_ready) return;
     size_t const nc = static_cast<size_t>(F) * SB;
-    u32 const nb0 = n_bins[0];
+    u32 f0 = 0;
+    if (fmask)
+        while (f0 + 1u < F && !fmask[f0]) ++f0;
+    u32 const nb0 = n_bins[f0];
+    size_t const off = static_cast<size_t>(f0) * SB;
     if (Q) {
-        i64 const *c = reinterpret_cast<i64 const *>(hist) + static_cast<size_t>(nd.slot) * nc * 2;
+        i64 const *c = reinterpret_cast<i64 const *>(hist) + (static_cast<size_t>(nd.slot) * nc + off) * 2;
         i64 g = 0, h = 0;
         for (u32 b = threadIdx.x; b < nb0; b += 64) {
             g += c[2 * b];
@@ -90,7 +95,7 @@ __global__ __launch_bounds__(64) void k_totals(NodeInfo *__restrict__ front, Tre
             front[i] = nd;
         }
     } else if (threadIdx.x == 0) {
-        float2 const *c = reinterpret_cast<float2 const *>(hist) + static_cast<size_t>(nd.slot) * nc;
+        float2 const *c = reinterpret_cast<float2 const *>(hist) + static_cast<size_t>(nd.slot) * nc + off;
         double g = 0.0, h = 0.0;
         u32 b = 0;
         for (; b + 8 <= nb0; b += 8) {
@@ -1119,9 +1124,11 @@ void enqueue_levelwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
         u32 const width = 1u << d;
         bool const last = d + 1 == D;
         if (q)
-            hipLaunchKernelGGL(k_totals<true>, dim3(width), dim3(64), 0, st, cur, v.ctl, hcur, v.d_nbins, v.F, v.SB);
+            hipLaunchKernelGGL(k_totals<true>, dim3(width), dim3(64), 0, st, cur, v.ctl, hcur, v.d_nbins, v.F, v.SB,
+                               v.gc.fmask);
         else
-            hipLaunchKernelGGL(k_totals<false>, dim3(width), dim3(64), 0, st, cur, v.ctl, hcur, v.d_nbins, v.F, v.SB);
+            hipLaunchKernelGGL(k_totals<false>, dim3(width), dim3(64), 0, st, cur, v.ctl, hcur, v.d_nbins, v.F, v.SB,
+                               v.gc.fmask);
         if (q)
             hipLaunchKernelGGL(k_find_level<true>, dim3(v.F), dim3(256), lds_level, st, cur, v.ctl, hcur, v.d_nbins, v.F,
                                v.SB, v.gc, v.scales, s.feat_best);

@@ -173,7 +173,8 @@ int bh_set_feature_fraction(bh_context *ctx, float fraction, uint64_t seed);
  * midpoint of the child weights (propagate_monotone_bounds, src/grower_impl.hpp:47-75),
  * and leaf values are clamped to their box (bounded_leaf_weight,
  * include/bonsai/split.hpp:141-150).  Kept across bh_configure; refused with the
- * levelwise grower (src/grower_impl.hpp:583-588).  dirs NULL / n 0 clears. */
+ * levelwise grower (src/grower_impl.hpp:583-588): while directions are set, bh_configure
+ * refuses that grower and keeps the previous fit.  dirs NULL / n 0 clears. */
 int bh_set_monotone(bh_context *ctx, const int8_t *dirs, uint32_t n);
 /* Interaction constraints (tree.interaction_constraints, parsed as
  * parse_interaction_groups does, src/grower_impl.hpp:84-107): n_groups groups, group g
@@ -182,7 +183,8 @@ int bh_set_monotone(bh_context *ctx, const int8_t *dirs, uint32_t n);
  * contain every distinct feature on its path, plus that feature when the path has one;
  * the root on any (allowed_features, src/grower_impl.hpp:111-137; the finder's skip,
  * src/split.cpp:111-114).  Kept across bh_configure; refused with the levelwise grower
- * (src/grower_impl.hpp:589-595) and for ids beyond n_features.  n_groups 0 clears. */
+ * (src/grower_impl.hpp:589-595; while groups are set, bh_configure refuses that grower
+ * and keeps the previous fit) and for ids beyond n_features.  n_groups 0 clears. */
 int bh_set_interaction(bh_context *ctx, const uint32_t *members, const uint32_t *group_sizes, uint32_t n_groups);
 /* Trees grown so far: nodes[t * max_nodes + i], sizes[t]; returns via *n_trees. */
 int bh_get_trees(bh_context *ctx, bh_node *nodes, uint32_t max_nodes, uint32_t *sizes,
@@ -197,6 +199,13 @@ int bh_max_nodes(bh_context *ctx, uint32_t *max_nodes);
  * values and scales when gq/hq/scales are non-null. */
 int bh_set_gradients(bh_context *ctx, const float *grad, const float *hess, uint32_t iter,
                      int8_t *gq, int16_t *hq, double *scales);
+/* Read back the gradients of the current iteration in row order: after bh_train, those
+ * the last iteration computed on the device from the scores before its tree (the
+ * resident gh_from_scores_kernel, src/cuda/detail/kernels.cuh:972-1009, plus this
+ * build's per-iteration quantizer); after bh_set_gradients, those handed over.  FP32
+ * fits fill grad / hess; quantized fits fill gq / hq and scales (g = gq * scales[0],
+ * h = hq * scales[1]).  Any pointer may be NULL. */
+int bh_get_gradients(bh_context *ctx, float *grad, float *hess, int8_t *gq, int16_t *hq, double *scales);
 /* Grow one tree from the current gradients without touching scores.  leaf_of_row
  * (optional, n_rows entries) receives each row's leaf node id (stamp_kernel,
  * kernels.cuh:479-488); for the levelwise grower the leaf-table index. */

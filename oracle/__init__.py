@@ -111,6 +111,11 @@ class Oracle:
         y = np.ascontiguousarray(y, np.float32)
         return self.L.orc_init_score(C.c_int32(objective), P(y), C.c_uint64(y.size))
 
+    def logloss_eval(self, s, y):
+        s = np.ascontiguousarray(s, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        return self.L.orc_logloss_eval(P(s), P(y), C.c_uint64(s.size))
+
     def quantize(self, g, h, seed, it):
         g = np.ascontiguousarray(g, np.float32)
         h = np.ascontiguousarray(h, np.float32)

@@ -257,10 +257,10 @@ struct Cfg {
 // Per-tree feature subsample (sample_features, src/grower_impl.hpp:163-180): the finders
 // skip features outside it, as the reference's finder skips features whose histogram
 // was not built (src/split.cpp:106-110).  Empty: every feature.  Set by
-// orc_set_feature_mask for the growers and finders that follow.  Node totals still come
-// from feature 0's cells, built whether selected or not: every feature's cells sum the
-// same rows (SplitInput::totals takes the first selected one's, include/bonsai/split.hpp:
-// 39-53; quantized sums are equal integers, FP32 ones differ in summation order only).
+// orc_set_feature_mask for the growers and finders that follow.  Node totals come from
+// the first selected feature's cells (SplitInput::totals skips unselected placeholders,
+// include/bonsai/split.hpp:36-53): every feature's cells sum the same rows, so quantized
+// totals are the same integers whichever feature, FP32 ones differ in summation order.
 std::vector<u8> g_feature_mask;
 
 // Monotone directions (tree.monotone_constraints; monotone_constraint_of,
@@ -302,6 +302,14 @@ inline bool breaks_monotone(u32 f, double gL, double hL, double gR, double hR, d
 inline bool feature_selected(u32 f)
 {
     return g_feature_mask.empty() || (f < g_feature_mask.size() && g_feature_mask[f] != 0);
+}
+// The feature whose cells give a node's totals: the first selected one (feature 0
+// without a mask).
+inline u32 totals_feature(u32 n_features)
+{
+    for (u32 f = 0; f < n_features; ++f)
+        if (feature_selected(f)) return f;
+    return 0;
 }
 Cfg to_cfg(orc_tree_cfg const &c)
 {
@@ -560,8 +568,8 @@ Split find_level_int(i64 const *hists, i64 const *totals, u32 P, u16 const *n_bi
     return best;
 }
 
-// Histogram::totals (include/bonsai/histogram.hpp:76-87): double sum over the first
-// feature's cells, stored as float.
+// Histogram::totals (include/bonsai/histogram.hpp:76-87): double sum over one feature's
+// cells (the caller passes the first selected feature's), stored as float.
 void totals_f32(float const *hist, u32 nb0, float *tg, float *th)
 {
     double g = 0.0, h = 0.0;
@@ -625,7 +633,8 @@ struct Engine {
         if (mode == ORC_HIST_INT) {
             n.hi.assign(cells(), 0);
             fill_int(ds, gr, n.rows.data(), n.rows.size(), n.hi.data());
-            totals_int(n.hi.data(), ds.n_bins[0], &n.itg, &n.ith);
+            u32 const f0 = totals_feature(ds.n_features);
+            totals_int(n.hi.data() + static_cast<size_t>(f0) * ds.bin_stride * 2, ds.n_bins[f0], &n.itg, &n.ith);
         } else {
             n.hf.assign(cells(), 0.0F);
             if (mode == ORC_HIST_F32_CHUNKED)
@@ -634,7 +643,8 @@ struct Engine {
                 fill_f32_omp(n.rows.data(), n.rows.size(), n.hf.data());
             else
                 fill_f32_serial(ds, gr, n.rows.data(), n.rows.size(), n.hf.data());
-            totals_f32(n.hf.data(), ds.n_bins[0], &n.tg, &n.th);
+            u32 const f0 = totals_feature(ds.n_features);
+            totals_f32(n.hf.data() + static_cast<size_t>(f0) * ds.bin_stride * 2, ds.n_bins[f0], &n.tg, &n.th);
         }
     }
 
@@ -670,11 +680,15 @@ struct Engine {
         if (mode == ORC_HIST_INT) {
             large.hi = std::move(parent.hi);
             for (size_t i = 0; i < large.hi.size(); ++i) large.hi[i] -= small.hi[i];
-            totals_int(large.hi.data(), ds.n_bins[0], &large.itg, &large.ith);
+            u32 const f0 = totals_feature(ds.n_features);
+            totals_int(large.hi.data() + static_cast<size_t>(f0) * ds.bin_stride * 2, ds.n_bins[f0], &large.itg,
+                       &large.ith);
         } else {
             large.hf = std::move(parent.hf);
             for (size_t i = 0; i < large.hf.size(); ++i) large.hf[i] -= small.hf[i];
-            totals_f32(large.hf.data(), ds.n_bins[0], &large.tg, &large.th);
+            u32 const f0 = totals_feature(ds.n_features);
+            totals_f32(large.hf.data() + static_cast<size_t>(f0) * ds.bin_stride * 2, ds.n_bins[f0], &large.tg,
+                       &large.th);
         }
     }
 
@@ -1317,7 +1331,8 @@ void eval_paths(Elem const *elems, Head const *heads, int n_heads, int max_len,
 // =================================================================================
 extern "C" {
 
-// Pinned by tests/test_oracle_binning.py (golden cut lists of test_bin_mapper.cpp).
+// Pinned by tests/test_oracle_golden.py::test_bin_mapper (the cut lists of
+// tests/unit/test_bin_mapper.cpp) and tests/test_abi.py::test_host_binning_matches_oracle.
 int32_t orc_fit_cuts(const float *col, uint64_t n, uint64_t stride, int32_t max_bin,
                      float *cuts_out, int32_t cap)
 {
@@ -1341,7 +1356,8 @@ void orc_transform(const float *cuts, int32_t n_cuts, const float *x, uint64_t n
         out[i * out_stride] = static_cast<u8>(transform_one(cuts, n_cuts, x[i * x_stride]));
 }
 
-// src/objective.cpp:86-101.  Pinned by tests/test_oracle_objective.py.
+// src/objective.cpp:86-101.  Pinned by tests/test_oracle_golden.py::test_objective (the
+// reference's gradient cases) and tests/test_oracle_objective.py (double closed form).
 void orc_logloss_grad(const float *score, const float *label, uint64_t n, float *g, float *h)
 {
     for (u64 i = 0; i < n; ++i) {
@@ -1350,7 +1366,7 @@ void orc_logloss_grad(const float *score, const float *label, uint64_t n, float 
         h[i] = p * (1.0F - p);
     }
 }
-// src/objective.cpp:44-58.
+// src/objective.cpp:44-58.  Pinned by tests/test_oracle_objective.py (exact residual).
 void orc_mse_grad(const float *score, const float *target, uint64_t n, float *g, float *h)
 {
     for (u64 i = 0; i < n; ++i) {
@@ -1358,7 +1374,8 @@ void orc_mse_grad(const float *score, const float *target, uint64_t n, float *g,
         h[i] = 1.0F;
     }
 }
-// src/objective.cpp:77-84 (mse) and :122-130 (logloss): sequential float sums.
+// src/objective.cpp:77-84 (mse) and :122-130 (logloss): sequential float sums.  Pinned by
+// tests/test_oracle_objective.py (means and log odds in closed form).
 float orc_init_score(int32_t objective, const float *label, uint64_t n)
 {
     float s = 0.0F;
@@ -1367,7 +1384,8 @@ float orc_init_score(int32_t objective, const float *label, uint64_t n)
     if (objective == 1) return std::log(m / (1.0F - m));
     return m;
 }
-// src/objective.cpp:103-120.
+// src/objective.cpp:103-120.  Pinned by tests/test_oracle_objective.py (the reference's
+// three eval cases, tests/unit/test_objective.cpp:211-254, and the double cross entropy).
 float orc_logloss_eval(const float *score, const float *label, uint64_t n)
 {
     float s = 0.0F;
@@ -1378,7 +1396,8 @@ float orc_logloss_eval(const float *score, const float *label, uint64_t n)
     return s / static_cast<float>(n);
 }
 
-// Pinned by tests/test_oracle_objective.py (unbiasedness, bounds, reconstruction error).
+// Pinned by tests/test_oracle_invariants.py::test_quantizer_bounds_unbiased_deterministic
+// (bounds, unbiasedness, reconstruction error, determinism).
 void orc_quantize(const float *g, const float *h, uint64_t n, uint64_t seed, uint32_t iter,
                   int8_t *gq, int16_t *hq, float *max_g, float *max_h)
 {
@@ -1425,12 +1444,6 @@ void orc_set_feature_mask(const uint8_t *mask, uint32_t n)
     else g_feature_mask.assign(mask, mask + n);
 }
 
-// The product's per-iteration device draw for feature_fraction < 1 (the reference draws
-// with std::sample over std::mt19937, src/grower_impl.hpp:163-180): k = max(1,
-// ceil(fraction * F)) in float as the reference computes it, and the k features with the
-// smallest 64-bit keys splitmix64(seed, iter, f, channel 4), ties to the lower id.
-// fraction >= 1 or F <= 1: every feature.  Pinned by tests/test_oracle_sampler.py (k
-// selected, each feature about equally often, independent draws per iteration).
 void orc_set_monotone(const int8_t *dirs, uint32_t n)
 {
     g_monotone.clear();
@@ -1449,6 +1462,13 @@ void orc_set_interaction(const uint32_t *members, const uint32_t *group_sizes, u
     }
 }
 
+// The product's per-iteration device draw for feature_fraction < 1 (the reference draws
+// with std::sample over std::mt19937, src/grower_impl.hpp:163-180): k = max(1,
+// ceil(fraction * F)) in float as the reference computes it, and the k features with the
+// smallest 64-bit keys splitmix64(seed, iter, f, channel 4), ties to the lower id.
+// fraction >= 1 or F <= 1: every feature.  Pinned by tests/test_oracle_features.py (k
+// selected, all at fraction >= 1 or one feature, each feature about equally often,
+// independent draws per tree).
 void orc_feature_draw(uint32_t n_features, float fraction, uint64_t seed, uint32_t iter, uint8_t *mask)
 {
     if (fraction >= 1.0F || n_features <= 1) {
@@ -1518,7 +1538,8 @@ static void put(Split const &s, orc_split *o)
     o->valid = s.valid ? 1u : 0u;
 }
 
-// Pinned by tests/test_oracle_split.py (closed forms of test_split_node.cpp).
+// Pinned by tests/test_oracle_golden.py::test_split_node_f32 / _int_same_decisions (the
+// closed forms of tests/unit/test_split_node.cpp).
 void orc_find_node_f32(const float *hist, const uint16_t *n_bins, uint32_t n_features,
                        uint32_t bin_stride, float tot_g, float tot_h, uint64_t n_rows,
                        const orc_tree_cfg *cfg, orc_split *out)
@@ -1534,7 +1555,10 @@ void orc_find_node_int(const int64_t *hist, const uint16_t *n_bins, uint32_t n_f
                       n_rows, to_cfg(*cfg)),
         out);
 }
-// Pinned by tests/test_oracle_split.py (test_split_level.cpp cases).
+// Pinned by tests/test_oracle_golden.py::test_split_level_f32 / _int_same_decisions (the
+// cases of tests/unit/test_split_level.cpp), test_level_single_parent_equals_node_finder,
+// test_level_int_single_parent_equals_node_int_finder, and
+// tests/test_oracle_invariants.py::test_levelwise_int_equals_f32_when_exact.
 void orc_find_level_f32(const float *hists, const float *totals, uint32_t n_nodes,
                         const uint16_t *n_bins, uint32_t n_features, uint32_t bin_stride,
                         const orc_tree_cfg *cfg, orc_split *out)
@@ -1559,7 +1583,8 @@ float orc_leaf_value(double g, double h, const orc_tree_cfg *cfg)
     return static_cast<float>(bounded_leaf_weight(g, h, cfg->lambda_l1, cfg->lambda_l2));
 }
 
-// Pinned by tests/test_oracle_partition.py (definition + test_partition.cpp invariant).
+// Pinned by tests/test_oracle_invariants.py::test_partition_stable_and_routes_left (brute
+// force stable order with routes_left).
 uint64_t orc_partition(const uint8_t *plane, uint64_t row_stride, const uint32_t *rows,
                        uint64_t n, uint32_t feature, uint32_t split_bin, uint32_t default_left,
                        uint32_t last_bin, uint32_t *out)
@@ -1585,8 +1610,10 @@ static int32_t emit(std::vector<orc_node> const &v, orc_node *nodes, int32_t cap
     return static_cast<int32_t>(v.size());
 }
 
-// Pinned by tests/test_oracle_growers.py (separable trees of test_grower.cpp, brute
-// force best split on tiny data, invariants).
+// Pinned by tests/test_oracle_invariants.py (separable stump of test_grower.cpp, L1 soft
+// threshold, integer-exact gradients make the FP32 and integer growers agree, leafwise
+// and levelwise invariants) and the reference's grower cases in
+// tests/test_oracle_monotone.py / tests/test_oracle_interaction.py.
 int32_t orc_grow_depthwise(const uint8_t *plane, uint64_t row_stride, uint64_t n_rows,
                            uint32_t n_features, const uint16_t *n_bins, const float *g,
                            const float *h, const int8_t *gq, const int16_t *hq, double scale_g,
@@ -1729,7 +1756,8 @@ int32_t orc_train_depthwise(const uint8_t *plane, uint64_t row_stride, uint64_t 
     return static_cast<int32_t>(n_iters);
 }
 
-// Pinned by tests/test_oracle_predict.py (hand-built trees, tree-by-tree sums).
+// Pinned by tests/test_oracle_invariants.py::test_predict_walk_and_rounding (brute-force
+// walk with float rounding per step).
 void orc_predict(const orc_node *trees, const int32_t *tree_offsets, uint32_t n_trees,
                  const uint8_t *plane, uint64_t row_stride, uint64_t n_rows,
                  const uint16_t *last_bin, float learning_rate, float init_score, float *out)
@@ -1783,7 +1811,9 @@ double orc_tree_expected_value(const orc_node *nodes)
     return expected_impl(nodes, nullptr, nullptr, nullptr, 0);
 }
 
-// Pinned by tests/test_oracle_shap.py (brute-force Shapley, additivity, ice-cream tree).
+// Pinned by tests/test_oracle_golden.py::test_shap_weight_table / test_shap_golden_tree
+// (the ice-cream tree) and tests/test_oracle_invariants.py (brute-force Shapley values,
+// ensemble scaling, additivity).
 void orc_tree_shap(const orc_node *nodes, int32_t max_depth, const uint8_t *row_bins,
                    const uint16_t *last_bin, uint32_t n_features, double *phi)
 {
@@ -1859,7 +1889,8 @@ void orc_pred_contribs(const orc_node *trees, const int32_t *tree_offsets, uint3
     }
 }
 
-// Pinned by tests/test_oracle_metric.py (brute-force O(n^2) pair count).
+// Pinned by tests/test_oracle_golden.py::test_auc and
+// tests/test_oracle_invariants.py::test_auc_brute_force_pairs (O(n^2) pair count).
 float orc_auc(const float *probs, const float *labels, uint64_t n)
 {
     std::vector<size_t> order(n);

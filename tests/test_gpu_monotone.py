@@ -104,3 +104,53 @@ def test_monotone_refused_by_levelwise_and_clearable(hip, orc, tall):
         out.append(ctx.grow_tree()[0])
         ctx.close()
     assert out[0].tobytes() == out[1].tobytes()
+
+
+def test_reconfigure_owns_the_constraints(hip, orc, tall):
+    """configure clears the constraints an earlier fit or setter left: a depthwise
+    context with monotone and interaction constraints reconfigures to levelwise and
+    grows the unconstrained oblivious tree; back on depthwise the free tree."""
+    bins, nb, X, y = tall
+    g, h = grad_of(orc, y, seed=31)
+    ctx = ctx_for(hip, bins, nb, grower="hip_depthwise", quantized=True, max_depth=5,
+                  monotone_constraints=tuple(int(v) for v in DIRS), interaction_constraints=("0,1", "2+3"))
+    ctx.set_gradients(g, h)
+    constrained, _ = ctx.grow_tree()
+    ctx.configure(hip.FitConfig(grower="hip_levelwise", quantized=True, max_depth=4))
+    ctx.set_gradients(g, h)
+    lv, _ = ctx.grow_tree()
+    ref = hip.HipContext(0)
+    ref.upload_bins(bins, nb)
+    ref.configure(hip.FitConfig(grower="hip_levelwise", quantized=True, max_depth=4))
+    ref.set_gradients(g, h)
+    assert lv.tobytes() == ref.grow_tree()[0].tobytes()
+    ref.configure(hip.FitConfig(grower="hip_depthwise", quantized=True, max_depth=5))
+    ref.set_gradients(g, h)
+    free, _ = ref.grow_tree()
+    ref.close()
+    ctx.configure(hip.FitConfig(grower="hip_depthwise", quantized=True, max_depth=5))
+    ctx.set_gradients(g, h)
+    again, _ = ctx.grow_tree()
+    ctx.close()
+    assert again.tobytes() == free.tobytes() != constrained.tobytes()
+
+
+def test_refused_configure_keeps_the_previous_fit(hip, orc, tall):
+    """Through the C ABI: with directions set, bh_configure refuses the levelwise grower
+    before anything changes, so the constrained depthwise fit still grows its tree."""
+    import ctypes as C
+    bins, nb, X, y = tall
+    g, h = grad_of(orc, y, seed=37)
+    ctx = ctx_for(hip, bins, nb, grower="hip_depthwise", quantized=True, max_depth=6)
+    ctx.set_monotone(DIRS)
+    gq, hq, sc = ctx.set_gradients(g, h)
+    before, _ = ctx.grow_tree()
+    bad = hip.FitConfig(grower="hip_levelwise", quantized=True, max_depth=4).to_c()
+    assert hip.lib().bh_configure(ctx._h, C.byref(bad)) > 0
+    assert b"levelwise" in hip.lib().bh_last_error()
+    ctx.set_gradients(g, h)
+    after, _ = ctx.grow_tree()
+    ctx.close()
+    assert after.tobytes() == before.tobytes()
+    ref, _ = oracle_tree(orc, "hip_depthwise", bins, nb, g, h, gq, hq, sc, True, orc_cfg(max_depth=6))
+    assert_same_tree(after, ref)

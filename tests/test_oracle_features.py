@@ -7,12 +7,16 @@ distinct features with the product taken in float, every feature for fraction >=
 single feature, and a uniform spread (over many trees each feature is selected with
 probability k / F, checked within 6 standard deviations).  The skip is pinned by an
 independent route: a tree grown under a mask equals the tree grown on the selected
-columns alone, with the feature ids mapped back.
+columns alone, with the feature ids mapped back.  In FP32 that also pins where node
+totals come from: the first selected feature's cells, as SplitInput::totals skips the
+unselected placeholders (include/bonsai/split.hpp:36-53); float sums of another
+feature's cells differ in the last bits, which the bitwise comparison of gains and leaf
+values would show.
 """
 import numpy as np
 import pytest
 
-from oracle_lib import HIST_INT, cfg as orc_cfg
+from oracle_lib import HIST_F32_CHUNKED, HIST_INT, cfg as orc_cfg
 
 
 def expected_k(frac, F):
@@ -70,16 +74,28 @@ def masked(orc, fn, mask, *a, **kw):
         orc.set_feature_mask(None)
 
 
+def float_grads(orc, n, seed=3):
+    rng = np.random.default_rng(seed)
+    return orc.logloss_grad(rng.normal(0, 1, n).astype(np.float32), (rng.random(n) < 0.5).astype(np.float32))
+
+
+@pytest.mark.parametrize("mode", [HIST_INT, HIST_F32_CHUNKED])
 @pytest.mark.parametrize("grower", ["grow_depthwise", "grow_leafwise"])
-def test_masked_tree_equals_tree_on_selected_columns(orc, grower):
+def test_masked_tree_equals_tree_on_selected_columns(orc, grower, mode):
     bins, nb, gq, hq, sc = tiny(orc)
+    if mode == HIST_INT:
+        grads = dict(gq=gq, hq=hq, scales=sc)
+    else:
+        g, h = float_grads(orc, bins.shape[0])
+        g = (g + np.where(bins[:, 0] > 30, np.float32(0.5), np.float32(-0.5))).astype(np.float32)
+        grads = dict(g=g, h=h, chunk=1000)
     sel = np.array([1, 4, 5, 7, 9], np.int64)  # feature 0 (the strongest) left out
     mask = np.zeros(bins.shape[1], np.uint8)
     mask[sel] = 1
     c = orc_cfg(max_depth=6, max_leaves=31)
     fn = getattr(orc, grower)
-    full, flor = masked(orc, fn, mask, bins, nb, gq=gq, hq=hq, scales=sc, c=c, mode=HIST_INT)
-    sub, slor = fn(np.ascontiguousarray(bins[:, sel]), nb[sel], gq=gq, hq=hq, scales=sc, c=c, mode=HIST_INT)
+    full, flor = masked(orc, fn, mask, bins, nb, c=c, mode=mode, **grads)
+    sub, slor = fn(np.ascontiguousarray(bins[:, sel]), nb[sel], c=c, mode=mode, **grads)
     assert len(full) == len(sub) > 1
     inner = sub["feature"] >= 0
     assert np.array_equal(full["feature"][inner], sel[sub["feature"][inner]])
@@ -90,19 +106,24 @@ def test_masked_tree_equals_tree_on_selected_columns(orc, grower):
         assert np.array_equal(full[f].view(np.uint32), sub[f].view(np.uint32))
     assert np.array_equal(flor, slor)
     # and the mask changed the tree: the unmasked root splits on feature 0
-    plain, _ = fn(bins, nb, gq=gq, hq=hq, scales=sc, c=c, mode=HIST_INT)
+    plain, _ = fn(bins, nb, c=c, mode=mode, **grads)
     assert plain["feature"][0] == 0
 
 
-def test_masked_levelwise_equals_selected_columns(orc):
+@pytest.mark.parametrize("mode", [HIST_INT, HIST_F32_CHUNKED])
+def test_masked_levelwise_equals_selected_columns(orc, mode):
     bins, nb, gq, hq, sc = tiny(orc, seed=2)
+    if mode == HIST_INT:
+        grads = dict(gq=gq, hq=hq, scales=sc)
+    else:
+        g, h = float_grads(orc, bins.shape[0], seed=4)
+        grads = dict(g=g, h=h, chunk=1000)
     sel = np.array([2, 3, 7, 11], np.int64)
     mask = np.zeros(bins.shape[1], np.uint8)
     mask[sel] = 1
     c = orc_cfg(max_depth=5)
-    a = masked(orc, orc.grow_levelwise, mask, bins, nb, gq=gq, hq=hq, scales=sc, c=c, mode=HIST_INT)
-    b = orc.grow_levelwise(np.ascontiguousarray(bins[:, sel]), nb[sel], gq=gq, hq=hq, scales=sc, c=c,
-                           mode=HIST_INT)
+    a = masked(orc, orc.grow_levelwise, mask, bins, nb, c=c, mode=mode, **grads)
+    b = orc.grow_levelwise(np.ascontiguousarray(bins[:, sel]), nb[sel], c=c, mode=mode, **grads)
     la, lb = a[0].astype(np.int64), b[0].astype(np.int64)
     assert la.shape == lb.shape and la.shape[0] > 0
     assert np.array_equal(la[:, 0], sel[lb[:, 0]])

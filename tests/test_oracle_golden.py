@@ -80,6 +80,20 @@ def test_split_level_f32(orc, case):
     check_split(s, case["expect"])
 
 
+@pytest.mark.parametrize("case", CASES["split_level"], ids=lambda c: c["src"])
+def test_split_level_int_same_decisions(orc, case):
+    """The integer level finder on the same cases in fixed point (quantum 1/400 holds
+    every case's decimal values exactly): the same decision and gain."""
+    nodes = [as_hist(n) for n in case["frontier"]]
+    nb = nodes[0][1]
+    H = np.stack([h for h, _ in nodes])
+    q = np.rint(H.astype(np.float64) * 400).astype(np.int64)
+    assert np.allclose(q / 400.0, H.astype(np.float64), rtol=1e-7, atol=0)
+    T = np.array([[int(n[0, : nb[0], 0].sum()), int(n[0, : nb[0], 1].sum())] for n in q], np.int64)
+    s = orc.find_level_int(q, T, nb, 1 / 400, 1 / 400, tcfg(case["cfg"]))
+    check_split(s, case["expect"])
+
+
 def test_level_single_parent_equals_node_finder(orc):
     """test_split_level.cpp:26-48 on random histograms: a one-node frontier is the node finder."""
     rng = np.random.default_rng(0)
@@ -98,6 +112,27 @@ def test_level_single_parent_equals_node_finder(orc):
         assert (a.valid, a.feature, a.bin, a.default_left) == (b.valid, b.feature, b.bin, b.default_left)
         if a.valid:
             assert a.gain == pytest.approx(b.gain, rel=1e-6)
+
+
+def test_level_int_single_parent_equals_node_int_finder(orc):
+    """The integer twins: a one-node frontier of the integer level finder is the integer
+    node finder (decision and gain), on random fixed-point histograms."""
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        F, SB = 4, 11
+        base = np.zeros((SB, 2), np.int64)
+        base[:, 0] = rng.integers(-500, 500, SB)
+        base[:, 1] = rng.integers(50, 900, SB)
+        # every feature holds the same rows: permute one node's cells across bins
+        q = np.stack([base[rng.permutation(SB)] for _ in range(F)])
+        nb = np.full(F, SB, np.uint16)
+        tg, th = int(base[:, 0].sum()), int(base[:, 1].sum())
+        c = TreeCfg(0.0, 0.0, 1.0, 0.0, 6, 0, 31, 0)
+        a = orc.find_node_int(q, nb, tg, th, 0.01, 0.002, 0, c)
+        b = orc.find_level_int(q[None], np.array([[tg, th]], np.int64), nb, 0.01, 0.002, c)
+        assert (a.valid, a.feature, a.bin, a.default_left) == (b.valid, b.feature, b.bin, b.default_left)
+        if a.valid:
+            assert a.gain == pytest.approx(b.gain, rel=1e-12)
 
 
 @pytest.mark.parametrize("case", CASES["leaf_weight"], ids=lambda c: c["src"])

@@ -161,6 +161,25 @@ def test_depthwise_int_equals_f64_when_exact(orc):
     assert np.array_equal(la, lb) and np.array_equal(la, lq)
 
 
+def test_levelwise_int_equals_f32_when_exact(orc):
+    """The levelwise twin: with integer gradients the FP32 serial and the integer level
+    finders see the same exact sums, so the oblivious trees (level decisions, leaf values,
+    covers, leaf of every row) must agree."""
+    bins, nb = small_plane(n=6000, f=6, seed=30)
+    rng = np.random.default_rng(31)
+    gq = rng.integers(-127, 128, bins.shape[0]).astype(np.int8)
+    hq = rng.integers(1, 200, bins.shape[0]).astype(np.int16)
+    g = gq.astype(np.float32)
+    h = hq.astype(np.float32)
+    for depth in (1, 3, 5):
+        c = orc_cfg(max_depth=depth, min_data=5)
+        a = orc.grow_levelwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_SERIAL)
+        q = orc.grow_levelwise(bins, nb, gq=gq, hq=hq, scales=(1.0, 1.0), c=c, mode=HIST_INT)
+        assert a[0].shape[0] == depth
+        for x, y in zip(a, q):
+            assert np.array_equal(x, y)
+
+
 def test_leafwise_adaptive_chunks_keep_every_row(orc):
     """The leaf-sized FP32 units (ORC_CHUNK_ADAPTIVE: per built node, ceil(n / T) rows
     rounded up to 256, clamped to [1024, 8192]) regroup the float sums only: with integer
