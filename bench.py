@@ -13,11 +13,17 @@ headline value is the quantized (int8 g / int16 h, stochastic rounding) grower's
 for 500 iterations; the FP32 fixed-order grower runs the same iterations beside it and
 both models are scored by held-out AUC on 1.6M rows drawn from the same distribution.
 
-roofline: the histogram kernel, every launch of a few eager iterations timed with
-hipEvents on the engine stream; algorithmic bytes per launch = rows x (28 bin bytes +
-4 gradient-pair bytes) + 8 bytes per built (node, feature, bin) cell.
-cpu_baseline: the C++/OpenMP CPU restatement of the reference's depthwise grower on a
-row sample, scaled to 16M rows and 500 iterations.
+roofline: the histogram kernels, every launch of a few eager iterations timed with
+hipEvents on the engine stream.  achieved = algorithmic bytes / kernel time, where the
+algorithmic bytes of a launch are every built node's rows read once, rows x (28 bin
+bytes + 4 gradient-pair bytes), plus 8 bytes per built (node, feature, bin) cell; the
+rate of the bytes the launches actually scan (levels whose rows were not moved read
+each parent's whole segment) is reported beside it as scanned_GBps.
+Equal AUC: after the timed region (untimed), both precisions fit 500 iterations and
+their held-out AUCs are compared; the root histogram of both precisions is compared with
+an FP64 histogram of the same float gradients.
+cpu_baseline: the C++/OpenMP CPU restatement of the reference's depthwise grower on the
+full 16M-row plane for a few iterations, scaled to 500 iterations.
 """
 from __future__ import annotations
 
@@ -106,23 +112,73 @@ def held_out_auc(B, trees, n_bins, init, bins_h, yh):
     return synth.auc(margin, yh)
 
 
-def cpu_baseline(X, y, cuts, n_rows_full: int, sample_rows: int, sample_iters: int):
-    """The oracle's OpenMP depthwise grower on the first sample_rows rows."""
-    from bonsai_amd import binning
+def cpu_info() -> dict:
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    quota = None
+    try:
+        q, per = Path("/sys/fs/cgroup/cpu.max").read_text().split()
+        quota = None if q == "max" else round(int(q) / int(per), 2)
+    except (OSError, ValueError):
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "cgroup_quota_cpus": quota}
+
+
+def cpu_baseline(bins, nb, y, iters: int):
+    """The oracle's OpenMP depthwise grower (the reference's row-block fill) on the whole
+    plane for `iters` iterations, on min(cores, quota, 16) threads."""
     from oracle import HIST_F32_OMP, cfg, load_oracle
     orc = load_oracle()
     cores = host_cores()
-    bins = binning.transform(X[:sample_rows], cuts)
-    nb = binning.n_bins_of(cuts)
+    info = cpu_info()
+    log(f"cpu: {info['model']}, nproc {info['nproc']}, affinity {info['affinity']}, "
+        f"quota {info['cgroup_quota_cpus']}, using {cores} threads")
     t0 = time.perf_counter()
-    orc.train_depthwise(bins, nb, y[:sample_rows], 1, sample_iters, 0.1, cfg(max_depth=8), mode=HIST_F32_OMP,
-                        n_threads=cores)
+    orc.train_depthwise(bins, nb, y, 1, iters, 0.1, cfg(max_depth=8), mode=HIST_F32_OMP, n_threads=cores)
     dt = time.perf_counter() - t0
-    s_iter_full = dt / sample_iters * (n_rows_full / sample_rows)
-    return {"value": round(s_iter_full * FULL_ITERS, 3), "unit": "s (fit of 500 iterations, extrapolated)",
-            "cores": cores, "kind": "port",
-            "sample": f"{sample_iters} iterations on the first {sample_rows} rows ({dt:.1f}s), scaled "
-                      f"linearly to {n_rows_full} rows x {FULL_ITERS} iterations"}
+    return {"value": round(dt / iters * FULL_ITERS, 3), "unit": "s (fit of 500 iterations, extrapolated)",
+            "cores": cores, "kind": "port", "cpu": info,
+            "sample": f"{iters} iterations on all {bins.shape[0]} rows ({dt:.1f}s, {dt / iters:.2f} s per "
+                      f"iteration), scaled linearly to {FULL_ITERS} iterations"}
+
+
+def hist_error(B, ctx, scores, y, nb, bins, what: str):
+    """Root histogram of the logistic gradients at `scores` in both precisions against an
+    FP64 histogram of the same float gradients."""
+    from oracle import load_oracle
+    orc = load_oracle()
+    X_rows = scores.size
+    g, h = orc.logloss_grad(scores, y)
+    SB = int(nb.max())
+    ref = np.zeros((bins.shape[1], SB, 2), np.float64)
+    for f in range(bins.shape[1]):
+        ref[f, :, 0] = np.bincount(bins[:, f], weights=g.astype(np.float64), minlength=SB)[:SB]
+        ref[f, :, 1] = np.bincount(bins[:, f], weights=h.astype(np.float64), minlength=SB)[:SB]
+    rows = np.arange(X_rows, dtype=np.uint32)
+    out = {"rows": X_rows, "gradients": what}
+    for quantized in (True, False):
+        ctx.configure(B.FitConfig(grower="hip_depthwise", objective="logloss", quantized=quantized, max_depth=8))
+        if quantized:
+            gq, hq, sc = ctx.set_gradients(g, h)
+            got = ctx.histogram(rows).astype(np.float64) * np.array([sc[0], sc[1]])
+        else:
+            ctx.set_gradients(g, h)
+            got = ctx.histogram(rows).astype(np.float64)
+        err = np.abs(got - ref)
+        key = "quantized" if quantized else "fp32"
+        out[key] = {"max_abs_g": float(err[..., 0].max()), "max_abs_h": float(err[..., 1].max()),
+                    "rel_l1_g": float(err[..., 0].sum() / np.abs(ref[..., 0]).sum()),
+                    "rel_l1_h": float(err[..., 1].sum() / np.abs(ref[..., 1]).sum())}
+        if quantized:
+            out[key]["scales"] = [float(sc[0]), float(sc[1])]
+    return out
 
 
 def main() -> int:
@@ -137,8 +193,10 @@ def main() -> int:
     ap.add_argument("--tuning", default="", help="fixed launch shapes, e.g. scatter_lds_bytes=49152,find_blocks=4")
     ap.add_argument("--skip-fp32", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=4_000_000)
-    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--cpu-iters", type=int, default=3, help="CPU baseline iterations on the full plane")
+    ap.add_argument("--auc-iters", type=int, default=FULL_ITERS,
+                    help="untimed fit length of the equal-AUC check (0: skip)")
+    ap.add_argument("--skip-hist-error", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB calibration copy kernel (for HBM counter scaling) and exit")
     a = ap.parse_args()
@@ -177,9 +235,11 @@ def main() -> int:
     log(f"quantized: {a.steps} iters in {dt_q:.3f}s, held-out AUC {auc_q:.5f}")
     prof = ctx.profile_hist(a.prof_iters)
     hist_gbps = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9
+    scanned_gbps = prof["scanned_bytes"] / (prof["ms"] * 1e-3) / 1e9
     per_launch_bytes = prof["bytes"] / prof["launches"]
     per_launch_us = prof["ms"] * 1e3 / prof["launches"]
-    log(f"histogram: {prof['launches']} launches, {per_launch_us:.1f} us avg, {hist_gbps:.0f} GB/s")
+    log(f"histogram: {prof['launches']} launches, {per_launch_us:.1f} us avg, {hist_gbps:.0f} GB/s algorithmic "
+        f"({prof['rows']} built rows), {scanned_gbps:.0f} GB/s scanned ({prof['scanned_rows']} rows)")
 
     fp32 = None
     if not a.skip_fp32:
@@ -190,19 +250,48 @@ def main() -> int:
         fp32 = {"fit_s_500": round(dt_f / a.steps * FULL_ITERS, 3), "ms_per_step": round(dt_f / a.steps * 1e3, 3),
                 "held_out_auc": round(auc_f, 6), "speedup_quantized": round(dt_f / dt_q, 3),
                 "auc_delta_quantized": round(auc_q - auc_f, 6), "tuning": tuned_f}
+
+    # Equal AUC at the full fit length (untimed; the launch shapes stay as tuned).
+    auc_full = None
+    if a.auc_iters and not a.skip_fp32:
+        res = {}
+        for quantized, tuned in ((True, tuned_q), (False, fp32["tuning"])):
+            ctx.configure(B.FitConfig(grower="hip_depthwise", objective="logloss", quantized=quantized,
+                                      use_graph=True, max_depth=8, learning_rate=0.1, lambda_l2=1.0))
+            if tuned and tuned.get("shapes"):
+                ctx.set_tuning(**tuned["shapes"])
+            ctx.set_scores(np.full(a.rows, init, np.float32))
+            t0 = time.perf_counter()
+            ctx.train(a.auc_iters)
+            res["quantized" if quantized else "fp32"] = {
+                "held_out_auc": round(held_out_auc(B, ctx.trees(), nb, init, bins_h, yh), 6),
+                "fit_s": round(time.perf_counter() - t0, 3)}
+        auc_full = {"iterations": a.auc_iters, **res,
+                    "auc_delta": round(res["quantized"]["held_out_auc"] - res["fp32"]["held_out_auc"], 6)}
+        auc_full["within_0.001"] = abs(auc_full["auc_delta"]) <= 0.001
+        log(f"equal-AUC check at {a.auc_iters} iterations: {auc_full}")
+
+    bins = ctx.download_bins()
+    herr = None
+    if not a.skip_hist_error:
+        if auc_full:   # the FP32 fit's final scores: gradients with a realistic spread
+            scores, what = ctx.get_scores(), f"after {a.auc_iters} FP32 iterations"
+        else:
+            scores, what = np.full(a.rows, init, np.float32), "iteration 0 (score = init score)"
+        herr = hist_error(B, ctx, scores, y, nb, bins, what)
+        log(f"root histogram error vs FP64: {herr}")
     ctx.close()
 
     cpu = None
     if not a.skip_cpu:
-        rows = min(a.cpu_rows, a.rows)
-        cpu = cpu_baseline(X, y, cuts, a.rows, rows, a.cpu_iters)
+        cpu = cpu_baseline(bins, nb, y, a.cpu_iters)
         log(f"cpu baseline: {cpu['value']} s extrapolated on {cpu['cores']} threads")
 
     traffic = traffic_ratio = None
     if TRAFFIC.exists() and a.rows == 16_000_000:
         t = json.loads(TRAFFIC.read_text())
         traffic = t["traffic_bytes_per_launch"]
-        traffic_ratio = round(t["traffic_bytes_per_launch"] / t["algorithmic_bytes_per_launch"], 3)
+        traffic_ratio = round(traffic / per_launch_bytes, 3)
 
     out = {
         "metric": METRIC,
@@ -225,6 +314,8 @@ def main() -> int:
             "held_out_auc": round(auc_q, 6),
             "fp32": fp32,
             "hist_GBps": round(hist_gbps, 1),
+            "auc_full_fit": auc_full,
+            "hist_error_vs_fp64": herr,
             "tuning": tuned_q,
             "rows_per_s": round(a.rows * a.steps / dt_q, 1),
         },
@@ -234,6 +325,9 @@ def main() -> int:
             "frac": round(hist_gbps / HBM_PEAK_GBPS, 4),
             "bytes_per_launch": round(per_launch_bytes), "avg_launch_us": round(per_launch_us, 2),
             "launches_timed": prof["launches"],
+            "bytes_basis": "built-node rows x (28 + 4) + 8 B per built cell",
+            "scanned_GBps": round(scanned_gbps, 1),
+            "scanned_bytes_per_launch": round(prof["scanned_bytes"] / prof["launches"]),
             "copy_GBps": round(copy_gbps, 1),
             "traffic": traffic,
             "traffic_over_algorithmic": traffic_ratio,

@@ -38,7 +38,7 @@ class LibraryMissing(ImportError):
 
 
 GROWERS = {"hip_depthwise": 0, "hip_leafwise": 1, "hip_levelwise": 2}
-OBJECTIVES = {"mse": 0, "logloss": 1}
+OBJECTIVES = {"mse": 0, "logloss": 1, "poisson": 2}
 
 
 class FitConfigC(C.Structure):
@@ -145,6 +145,7 @@ def lib() -> C.CDLL:
         "bh_destroy": [P],
         "bh_upload_bins": [P, P, u64, u32, u64, P],
         "bh_upload_labels": [P, P],
+        "bh_upload_weights": [P, P],
         "bh_upload_features": [P, P, u64, u32, P, P],
         "bh_download_bins": [P, P, u64],
         "bh_configure": [P, C.POINTER(FitConfigC)],
@@ -170,7 +171,7 @@ def lib() -> C.CDLL:
         "bh_time_root_histogram": [P, u32, C.POINTER(C.c_float)],
         "bh_set_timing": [P, i32],
         "bh_profile_hist": [P, u32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(u64),
-                            C.POINTER(u64)],
+                            C.POINTER(u64), C.POINTER(C.c_double), C.POINTER(u64)],
         "bh_get_timing": [P, P, u32],
         "bh_set_tuning": [P, C.POINTER(TuningC)],
         "bh_get_tuning": [P, C.POINTER(TuningC)],
@@ -299,6 +300,16 @@ class HipContext:
     def upload_labels(self, labels: np.ndarray) -> None:
         labels = np.ascontiguousarray(labels, dtype=np.float32)
         _check(lib().bh_upload_labels(self._h, _ptr(labels)))
+
+    def upload_weights(self, weights: np.ndarray | None) -> None:
+        """Per-row weights scaling the device gradients (None: unweighted)."""
+        if weights is None:
+            _check(lib().bh_upload_weights(self._h, None))
+            return
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        if w.size != self.n_rows:
+            raise ConfigError("one weight per row")
+        _check(lib().bh_upload_weights(self._h, _ptr(w)))
 
     def configure(self, cfg: FitConfig) -> None:
         """Configure the fit.  The FitConfig owns the sampling and constraint fields:
@@ -516,12 +527,16 @@ class HipContext:
         return {k: getattr(t, k) for k, _ in TuningC._fields_}, ms.value
 
     def profile_hist(self, n_iters: int) -> dict:
-        """Per-launch hipEvent timing of every histogram launch over n_iters iterations."""
-        ms, by = C.c_double(0), C.c_double(0)
-        la, ro = C.c_uint64(0), C.c_uint64(0)
-        _check(lib().bh_profile_hist(self._h, n_iters, C.byref(ms), C.byref(by), C.byref(la), C.byref(ro)))
+        """Per-launch hipEvent timing of every histogram launch over n_iters iterations:
+        kernel ms, launches, algorithmic bytes / built rows (each built node's rows once)
+        and scanned bytes / rows (what the launches read)."""
+        ms, by, sb = C.c_double(0), C.c_double(0), C.c_double(0)
+        la, ro, sr = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+        _check(lib().bh_profile_hist(self._h, n_iters, C.byref(ms), C.byref(by), C.byref(la), C.byref(ro),
+                                     C.byref(sb), C.byref(sr)))
         self.n_trained += n_iters
-        return {"ms": ms.value, "bytes": by.value, "launches": la.value, "rows": ro.value}
+        return {"ms": ms.value, "bytes": by.value, "launches": la.value, "rows": ro.value,
+                "scanned_bytes": sb.value, "scanned_rows": sr.value}
 
     def time_root_histogram(self, reps: int = 10) -> float:
         ms = C.c_float(0)

@@ -61,6 +61,7 @@ struct bh_context {
     u8 *plane0 = nullptr, *work[2] = {nullptr, nullptr};
     u16 *d_nbins = nullptr;
     float *label = nullptr, *score = nullptr;
+    float *weight = nullptr; // per-row weights (bh_upload_weights), null: unweighted
     bool have_labels = false;
     // quantized fits: the gradient maxima of the current scores are not in maxbits yet
     // (scores, labels or the fit changed outside the iteration loop)
@@ -133,8 +134,8 @@ struct bh_context {
     hipGraphExec_t graph_exec = nullptr;
     // histogram profiling (per-launch hipEvents, non-graph iterations only)
     bool prof = false;
-    double prof_ms = 0.0, prof_bytes = 0.0;
-    u64 prof_launches = 0, prof_rows = 0;
+    double prof_ms = 0.0, prof_bytes = 0.0, prof_scanned_bytes = 0.0;
+    u64 prof_launches = 0, prof_rows = 0, prof_scanned = 0;
     // timing
     bool timing = false;
     std::vector<float> iter_ms;
@@ -398,12 +399,15 @@ hipError_t host_sync_point(bh_context *c)
     return hipStreamSynchronize(c->stream);
 }
 
-// Profiling: time one histogram launch with hipEvents on its stream and count its
-// algorithmic bytes: rows scanned x (F + gradient pair bytes) read + one 8-byte cell per
-// (built node, feature, bin) written.  Rows are physically compacted per node (or, for
-// a routed unit, per parent: it scans the parent's whole segment), so no row-index
-// bytes are read.
-void profiled_hist(bh_context *c, void *hist_next, u32 grid_units, u32 *cnt_next, PartOp const *routes, bool group)
+// Profiling: time one histogram launch with hipEvents on its stream; after its
+// completion op, count its bytes two ways.  Algorithmic (the scope table's B_hist):
+// every built node k reads its n_k rows once, n_k x (F bin bytes + gradient pair bytes),
+// and writes one 8-byte cell per (feature, bin); rows are physically compacted per node,
+// so no row-index bytes are read.  n_k comes from the built node's count cells (feature
+// 0's bins sum to its rows) or, without count cells, from its units (which then cover
+// exactly its rows).  Scanned: the rows the units actually read (a routed unit scans
+// its parent's whole segment and keeps the rows routed to the built child).
+float profiled_hist(bh_context *c, void *hist_next, u32 grid_units, u32 *cnt_next, PartOp const *routes, bool group)
 {
     (void)hipEventRecord(c->ev0, c->stream);
     launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
@@ -413,25 +417,45 @@ void profiled_hist(bh_context *c, void *hist_next, u32 grid_units, u32 *cnt_next
     (void)hipEventSynchronize(c->ev1);
     float ms = 0.0F;
     (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    return ms;
+}
+
+void profiled_count(bh_context *c, float ms, u32 const *cnt_next)
+{
+    (void)hipStreamSynchronize(c->stream);
     TreeCtl ctl;
     (void)hipMemcpy(&ctl, c->ctl, sizeof(ctl), hipMemcpyDeviceToHost);
-    if (ctl.n_hist_chunks == 0) return;
+    if (ctl.n_hist_chunks == 0 || ctl.n_finish == 0) return;
     std::vector<HistChunk> hc(ctl.n_hist_chunks);
     (void)hipMemcpy(hc.data(), c->hch, hc.size() * sizeof(HistChunk), hipMemcpyDeviceToHost);
-    u64 rows = 0;
-    std::vector<u32> slots;
-    for (auto const &u : hc) {
-        rows += u.end - u.begin;
-        slots.push_back(u.slot);
+    std::vector<FinishOp> fo(ctl.n_finish);
+    (void)hipMemcpy(fo.data(), c->fops, fo.size() * sizeof(FinishOp), hipMemcpyDeviceToHost);
+    u64 scanned = 0;
+    for (auto const &u : hc) scanned += u.end - u.begin;
+    u64 built = 0, nodes = 0;
+    u32 const nb0 = c->n_bins[0];
+    std::vector<u32> cn(nb0);
+    for (auto const &op : fo) {
+        if (op.empty) continue;
+        ++nodes;
+        if (cnt_next) {
+            (void)hipMemcpy(cn.data(), cnt_next + static_cast<size_t>(op.small_slot) * c->node_cells(),
+                            nb0 * sizeof(u32), hipMemcpyDeviceToHost);
+            for (u32 v : cn) built += v;
+        } else {
+            for (auto const &u : hc)
+                if (u.slot == op.small_slot) built += u.end - u.begin;
+        }
     }
-    std::sort(slots.begin(), slots.end());
-    u64 const nodes = static_cast<u64>(std::unique(slots.begin(), slots.end()) - slots.begin());
     double sum_bins = 0.0;
     for (u16 b : c->n_bins) sum_bins += b;
     double const cells = static_cast<double>(nodes) * sum_bins * 8.0;
+    double const row_bytes = c->F + (c->quant ? 4.0 : 8.0);
     c->prof_ms += ms;
-    c->prof_rows += rows;
-    c->prof_bytes += static_cast<double>(rows) * (c->F + (c->quant ? 4.0 : 8.0)) + cells;
+    c->prof_rows += built;
+    c->prof_scanned += scanned;
+    c->prof_bytes += static_cast<double>(built) * row_bytes + cells;
+    c->prof_scanned_bytes += static_cast<double>(scanned) * row_bytes + cells;
     c->prof_launches += 1;
 }
 
@@ -442,14 +466,16 @@ void enqueue_build(bh_context *c, void *hist_cur, void *hist_next, u32 max_ops, 
                    u32 const *cnt_cur = nullptr, u32 *cnt_next = nullptr, bool advance = false,
                    PartOp const *routes = nullptr, bool group = false)
 {
+    float ms = 0.0F;
     if (c->prof)
-        profiled_hist(c, hist_next, grid_units, cnt_next, routes, group);
+        ms = profiled_hist(c, hist_next, grid_units, cnt_next, routes, group);
     else
         launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
                     gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), grid_units,
                     c->partials, hist_next, cnt_next, routes, group);
     launch_finish(c->stream, c->quant, c->fops, c->ctl, c->partials, hist_cur, hist_next, c->node_cells(), max_ops,
                   cnt_cur, cnt_next, advance);
+    if (c->prof) profiled_count(c, ms, cnt_next);
 }
 
 u32 hist_grid(bh_context *c) { return std::max(1u, std::min(c->max_hch, 4096u)); }
@@ -597,17 +623,17 @@ void enqueue_grow(bh_context *c)
 void prime_gradients(bh_context *c)
 {
     if (!c->quant || !c->max_stale) return;
-    launch_grad_prime(c->stream, c->score, c->label, c->N, c->fc.objective, c->maxbits, c->ctl);
+    launch_grad_prime(c->stream, c->score, c->label, c->weight, c->N, c->fc.objective, c->maxbits, c->ctl);
     c->max_stale = false;
 }
 
 void enqueue_gradients(bh_context *c)
 {
     if (c->quant)
-        launch_grad_quant(c->stream, c->score, c->label, c->N, c->fc.objective, c->maxbits, c->fc.seed, c->ctl,
-                          c->gh0q, c->scales);
+        launch_grad_quant(c->stream, c->score, c->label, c->weight, c->N, c->fc.objective, c->maxbits, c->fc.seed,
+                          c->ctl, c->gh0q, c->scales);
     else
-        launch_grad_f32(c->stream, c->score, c->label, c->N, c->fc.objective, c->gh0f);
+        launch_grad_f32(c->stream, c->score, c->label, c->weight, c->N, c->fc.objective, c->gh0f);
 }
 
 // One boosting iteration: gradients, tree, score epilogue, tree copy-out.
@@ -616,7 +642,8 @@ void enqueue_iteration(bh_context *c)
     enqueue_gradients(c);
     enqueue_grow(c);
     launch_score_update(c->stream, c->plane0, c->Fs, c->N, c->tree, c->ctl, c->d_nbins, c->F, c->fc.learning_rate,
-                        c->score, nullptr, c->max_nodes, c->label, c->fc.objective, c->quant ? c->maxbits : nullptr);
+                        c->score, nullptr, c->max_nodes, c->label, c->fc.objective, c->quant ? c->maxbits : nullptr,
+                        c->weight);
     if (c->have_valid) {
         // the new tree on the validation plane, then the round's loss and the
         // early-stopping rule (eval_accumulate + the fit loop's check)
@@ -788,7 +815,8 @@ int bh_destroy(bh_context *c)
     hipStreamSynchronize(c->stream);
     free_fit(c);
     void *ptrs[] = {c->plane0, c->work[0], c->work[1], c->d_nbins, c->label, c->score,
-                    c->vplane, c->vlabel, c->vscore, c->vhist, c->vpartial, c->samp_rows, c->samp_cnt, c->fmask};
+                    c->vplane, c->vlabel, c->vscore, c->vhist, c->vpartial, c->samp_rows, c->samp_cnt, c->fmask,
+                    c->weight};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -813,9 +841,10 @@ static int alloc_plane(bh_context *c, uint64_t n_rows, uint32_t n_features, cons
     hipStreamSynchronize(c->stream);
     free_fit(c);
     void *ptrs[] = {c->plane0, c->work[0], c->work[1], c->d_nbins, c->label, c->score, c->samp_rows, c->samp_cnt,
-                    c->fmask};
+                    c->fmask, c->weight};
     for (void *p : ptrs)
         if (p) hipFree(p);
+    c->weight = nullptr;
     c->plane0 = c->work[0] = c->work[1] = nullptr;
     c->samp_rows = c->samp_cnt = nullptr;
     c->samp_kind = c->samp_n = 0;
@@ -922,11 +951,34 @@ int bh_upload_labels(bh_context *c, const float *labels)
     return 0;
 }
 
+static void drop_graph(bh_context *c);
+
+int bh_upload_weights(bh_context *c, const float *weights)
+{
+    if (!c || !c->plane0) return fail(1, "no bins uploaded");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (weights) {
+        for (u64 r = 0; r < c->N; ++r)
+            if (!(weights[r] >= 0.0F) || std::isinf(weights[r]))
+                return fail(1, "weights must be finite and non-negative");
+        if (!c->weight) HIP_TRY(dalloc(&c->weight, c->N));
+        HIP_TRY(hipMemcpy(c->weight, weights, c->N * sizeof(float), hipMemcpyHostToDevice));
+    } else if (c->weight) {
+        HIP_TRY(hipFree(c->weight));
+        c->weight = nullptr;
+    }
+    drop_graph(c); // the gradient kernels' arguments change
+    c->max_stale = true;
+    return 0;
+}
+
 int bh_configure(bh_context *c, const bh_fit_config *cfg)
 {
     if (!c || !c->plane0) return fail(1, "no bins uploaded");
     if (cfg->grower < 0 || cfg->grower > 2) return fail(1, "unknown grower");
-    if (cfg->objective != BH_MSE && cfg->objective != BH_LOGLOSS) return fail(1, "unknown objective");
+    if (cfg->objective != BH_MSE && cfg->objective != BH_LOGLOSS && cfg->objective != BH_POISSON)
+        return fail(1, "unknown objective");
     if (cfg->max_depth < 1 || cfg->max_depth > 12) return fail(1, "max_depth must be in [1, 12]");
     if (cfg->min_data_in_leaf < 0) return fail(1, "min_data_in_leaf < 0");
     if (cfg->grower == BH_LEAFWISE && (cfg->max_leaves < 2 || cfg->max_leaves > 1024))
@@ -948,7 +1000,6 @@ int bh_configure(bh_context *c, const bh_fit_config *cfg)
     return alloc_fit(c);
 }
 
-static void drop_graph(bh_context *c);
 
 int bh_set_scores(bh_context *c, const float *scores)
 {
@@ -1647,15 +1698,15 @@ int bh_partition(bh_context *c, const uint32_t *rows, uint64_t n, uint32_t featu
 }
 
 int bh_profile_hist(bh_context *c, uint32_t n_iters, double *ms_total, double *bytes_total, uint64_t *launches,
-                    uint64_t *rows_total)
+                    uint64_t *rows_total, double *scanned_bytes_total, uint64_t *scanned_rows_total)
 {
     if (int r = check_ready(c)) return r;
     if (c->fc.grower != BH_DEPTHWISE) return fail(1, "histogram profiling covers the depthwise grower");
     HIP_TRY(hipSetDevice(c->device));
     if (int r = ensure_tree_out(c, n_iters)) return r;
     c->prof = true;
-    c->prof_ms = c->prof_bytes = 0.0;
-    c->prof_launches = c->prof_rows = 0;
+    c->prof_ms = c->prof_bytes = c->prof_scanned_bytes = 0.0;
+    c->prof_launches = c->prof_rows = c->prof_scanned = 0;
     prime_gradients(c);
     for (u32 it = 0; it < n_iters; ++it) enqueue_iteration(c);
     c->prof = false;
@@ -1667,6 +1718,8 @@ int bh_profile_hist(bh_context *c, uint32_t n_iters, double *ms_total, double *b
     *bytes_total = c->prof_bytes;
     *launches = c->prof_launches;
     *rows_total = c->prof_rows;
+    if (scanned_bytes_total) *scanned_bytes_total = c->prof_scanned_bytes;
+    if (scanned_rows_total) *scanned_rows_total = c->prof_scanned;
     return 0;
 }
 

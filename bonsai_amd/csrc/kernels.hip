@@ -127,29 +127,49 @@ __device__ __forceinline__ float node_leaf_value(NodeInfo const &n, GrowCfg cons
 }
 
 // ---------------------------------------------------------------------------------
-// Objective: src/objective.cpp:44-101 (mse, logloss); device twin of
+// Objective: src/objective.cpp:44-101 (mse, logloss) and :253-278 (poisson, log link,
+// raw score clamped to +-30 before exp, include/bonsai/objective.hpp:47); device twin of
 // gh_from_scores_kernel (src/cuda/detail/kernels.cuh:972-1009) fused with the
-// per-iteration quantizer.
+// per-iteration quantizer.  Weighted rows scale g and h by w[r] (the kernel's Weighted
+// arm; the host multiply of include/bonsai/booster.hpp:546-553).
 // ---------------------------------------------------------------------------------
+constexpr float kPoissonMaxLog = 30.0F;
+
 __device__ __forceinline__ void grad_of(int objective, float s, float y, float &g, float &h)
 {
     if (objective == 1) {
         float const p = 1.0F / (1.0F + expf(-s));
         g = p - y;
         h = p * (1.0F - p);
+    } else if (objective == 2) {
+        float const mu = expf(fminf(fmaxf(s, -kPoissonMaxLog), kPoissonMaxLog));
+        g = mu - y;
+        h = mu;
     } else {
         g = s - y;
         h = 1.0F;
     }
 }
 
+// Row r's gradient pair, scaled by its weight when the fit is weighted.
+__device__ __forceinline__ void grad_of_row(int objective, float s, float y, float const *__restrict__ weight,
+                                            u64 r, float &g, float &h)
+{
+    grad_of(objective, s, y, g, h);
+    if (weight) {
+        float const w = weight[r];
+        g *= w;
+        h *= w;
+    }
+}
+
 __global__ void k_grad_f32(float const *__restrict__ score, float const *__restrict__ label,
-                           u64 n, int objective, float2 *__restrict__ gh)
+                           float const *__restrict__ weight, u64 n, int objective, float2 *__restrict__ gh)
 {
     for (u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; r < n;
          r += static_cast<u64>(gridDim.x) * blockDim.x) {
         float g, h;
-        grad_of(objective, score[r], label[r], g, h);
+        grad_of_row(objective, score[r], label[r], weight, r, g, h);
         gh[r] = make_float2(g, h);
     }
 }
@@ -188,14 +208,15 @@ __device__ __forceinline__ void block_max2_atomic(float mg, float mh, u32 *maxbi
 // score update of iteration t fills slot (t + 1) & 1 for the next iteration, so this
 // pass only primes slot t & 1 when scores or labels changed outside the loop.
 __global__ void k_grad_max(float const *__restrict__ score, float const *__restrict__ label,
-                           u64 n, int objective, u32 *__restrict__ maxbits, TreeCtl const *__restrict__ ctl)
+                           float const *__restrict__ weight, u64 n, int objective, u32 *__restrict__ maxbits,
+                           TreeCtl const *__restrict__ ctl)
 {
     maxbits += 2u * (ctl->iter & 1u);
     float mg = 0.0F, mh = 0.0F;
     for (u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; r < n;
          r += static_cast<u64>(gridDim.x) * blockDim.x) {
         float g, h;
-        grad_of(objective, score[r], label[r], g, h);
+        grad_of_row(objective, score[r], label[r], weight, r, g, h);
         mg = fmaxf(mg, fabsf(g));
         mh = fmaxf(mh, fabsf(h));
     }
@@ -206,7 +227,8 @@ __global__ void k_grad_max(float const *__restrict__ score, float const *__restr
 // stochastic rounding q = floor(x * (qmax / max) + u), u from a counter RNG keyed on
 // (seed, iteration, original row, channel).
 __global__ void k_grad_quant(float const *__restrict__ score, float const *__restrict__ label,
-                             u64 n, int objective, u32 *__restrict__ maxbits, u64 seed,
+                             float const *__restrict__ weight, u64 n, int objective, u32 *__restrict__ maxbits,
+                             u64 seed,
                              TreeCtl const *__restrict__ ctl, u32 *__restrict__ ghq,
                              double *__restrict__ scales)
 {
@@ -225,7 +247,7 @@ __global__ void k_grad_quant(float const *__restrict__ score, float const *__res
     for (u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; r < n;
          r += static_cast<u64>(gridDim.x) * blockDim.x) {
         float g, h;
-        grad_of(objective, score[r], label[r], g, h);
+        grad_of_row(objective, score[r], label[r], weight, r, g, h);
         float const tg = g * ig + sr_uniform(seed, iter, r, 0);
         float const th = h * ih + sr_uniform(seed, iter, r, 1);
         float qg = fminf(127.0F, fmaxf(-127.0F, floorf(tg)));
@@ -2193,7 +2215,8 @@ __global__ __launch_bounds__(256) void k_score_update(u8 const *__restrict__ pla
                                                       u16 const *__restrict__ n_bins, u32 F, float lr,
                                                       float *__restrict__ score, u32 *__restrict__ leaf_of_row,
                                                       u32 node_cap, u32 last_cap, float const *__restrict__ label,
-                                                      int objective, u32 *__restrict__ maxbits)
+                                                      float const *__restrict__ weight, int objective,
+                                                      u32 *__restrict__ maxbits)
 {
     // LDS: [node_cap WalkNode][last_cap u16][rows: 256 * Fs + 16, 16-byte aligned]
     extern __shared__ __align__(16) unsigned char smem[];
@@ -2237,7 +2260,7 @@ __global__ __launch_bounds__(256) void k_score_update(u8 const *__restrict__ pla
             score[r] = s;
             if (maxbits) {
                 float g, h;
-                grad_of(objective, s, y, g, h);
+                grad_of_row(objective, s, y, weight, r, g, h);
                 mg = fmaxf(mg, fabsf(g));
                 mh = fmaxf(mh, fabsf(h));
             }
@@ -2264,6 +2287,9 @@ __global__ __launch_bounds__(256) void k_eval_loss(float const *__restrict__ sco
         if (OBJ == 0) {
             double const d = static_cast<double>(s) - static_cast<double>(y);
             acc += d * d;
+        } else if (OBJ == 2) {
+            float const f = fminf(fmaxf(s, -kPoissonMaxLog), kPoissonMaxLog);
+            acc += static_cast<double>(expf(f)) - static_cast<double>(y) * static_cast<double>(f);
         } else {
             float const ax = fabsf(s);
             acc += fmaxf(0.0F, s) + log1pf(expf(-ax)) - y * s;
@@ -2586,21 +2612,22 @@ void launch_bin_rows(hipStream_t s, float const *X, u64 n, u32 F, u32 Fs, float 
     hipLaunchKernelGGL(k_bin_rows, dim3(grid), dim3(256), lds_cuts * 4u, s, X, n, F, Fs, cuts, coff, plane,
                        lds_cuts);
 }
-void launch_grad_f32(hipStream_t s, float const *score, float const *label, u64 n, int obj, float2 *gh)
+void launch_grad_f32(hipStream_t s, float const *score, float const *label, float const *weight, u64 n, int obj,
+                     float2 *gh)
 {
-    hipLaunchKernelGGL(k_grad_f32, dim3(2048), dim3(256), 0, s, score, label, n, obj, gh);
+    hipLaunchKernelGGL(k_grad_f32, dim3(2048), dim3(256), 0, s, score, label, weight, n, obj, gh);
 }
-void launch_grad_quant(hipStream_t s, float const *score, float const *label, u64 n, int obj, u32 *maxbits,
-                       u64 seed, TreeCtl const *ctl, u32 *ghq, double *scales)
+void launch_grad_quant(hipStream_t s, float const *score, float const *label, float const *weight, u64 n, int obj,
+                       u32 *maxbits, u64 seed, TreeCtl const *ctl, u32 *ghq, double *scales)
 {
-    hipLaunchKernelGGL(k_grad_quant, dim3(2048), dim3(256), 0, s, score, label, n, obj, maxbits, seed, ctl, ghq,
-                       scales);
+    hipLaunchKernelGGL(k_grad_quant, dim3(2048), dim3(256), 0, s, score, label, weight, n, obj, maxbits, seed, ctl,
+                       ghq, scales);
 }
-void launch_grad_prime(hipStream_t s, float const *score, float const *label, u64 n, int obj, u32 *maxbits,
-                       TreeCtl const *ctl)
+void launch_grad_prime(hipStream_t s, float const *score, float const *label, float const *weight, u64 n, int obj,
+                       u32 *maxbits, TreeCtl const *ctl)
 {
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, maxbits, 4u);
-    hipLaunchKernelGGL(k_grad_max, dim3(1024), dim3(256), 0, s, score, label, n, obj, maxbits, ctl);
+    hipLaunchKernelGGL(k_grad_max, dim3(1024), dim3(256), 0, s, score, label, weight, n, obj, maxbits, ctl);
 }
 void launch_quant_from(hipStream_t s, float2 const *gh, u64 n, u32 *maxbits, u64 seed, u32 iter, u32 *ghq,
                        double *scales_dev)
@@ -2859,17 +2886,18 @@ void launch_tree_begin(hipStream_t s, TreeCtl *ctl, NodeInfo *front, DevNode *tr
 
 void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode const *tree, TreeCtl const *ctl,
                          u16 const *n_bins, u32 F, float lr, float *score, u32 *leaf_of_row, u32 max_nodes,
-                         float const *label, int objective, u32 *maxbits)
+                         float const *label, int objective, u32 *maxbits, float const *weight)
 {
     u32 const node_cap = std::min(max_nodes, kWalkLds);
     u32 const last_cap = std::min(F, kWalkLds);
     size_t const head = (node_cap * sizeof(WalkNode) + last_cap * sizeof(u16) + 15u) & ~static_cast<size_t>(15);
     if (Fs <= 64)
         hipLaunchKernelGGL(k_score_update<true>, dim3(2048), dim3(256), head + 256u * Fs + 16u, s, plane, Fs, n, tree,
-                           ctl, n_bins, F, lr, score, leaf_of_row, node_cap, last_cap, label, objective, maxbits);
+                           ctl, n_bins, F, lr, score, leaf_of_row, node_cap, last_cap, label, weight, objective,
+                           maxbits);
     else
         hipLaunchKernelGGL(k_score_update<false>, dim3(2048), dim3(256), head, s, plane, Fs, n, tree, ctl, n_bins, F,
-                           lr, score, leaf_of_row, node_cap, last_cap, label, objective, maxbits);
+                           lr, score, leaf_of_row, node_cap, last_cap, label, weight, objective, maxbits);
 }
 
 u32 eval_blocks(u32 n) { return std::max(1u, std::min(n / 256u, 1024u)); }
@@ -2880,6 +2908,8 @@ void launch_eval(hipStream_t s, int objective, float const *scores, float const 
     u32 const blocks = eval_blocks(n);
     if (objective == 1)
         hipLaunchKernelGGL(k_eval_loss<1>, dim3(blocks), dim3(256), 0, s, scores, labels, n, partial);
+    else if (objective == 2)
+        hipLaunchKernelGGL(k_eval_loss<2>, dim3(blocks), dim3(256), 0, s, scores, labels, n, partial);
     else
         hipLaunchKernelGGL(k_eval_loss<0>, dim3(blocks), dim3(256), 0, s, scores, labels, n, partial);
     hipLaunchKernelGGL(k_eval_finish, dim3(1), dim3(256), 0, s, partial, blocks, n, ctl, hist, hist_cap, patience);

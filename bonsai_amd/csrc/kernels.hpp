@@ -5,15 +5,18 @@
 
 namespace bh {
 
-void launch_grad_f32(hipStream_t s, float const *score, float const *label, u64 n, int obj, float2 *gh);
+// weight: per-row weights scaling g and h (null: unweighted).  obj 0 mse, 1 logloss,
+// 2 poisson.
+void launch_grad_f32(hipStream_t s, float const *score, float const *label, float const *weight, u64 n, int obj,
+                     float2 *gh);
 // Quantized gradients of iteration ctl->iter with the max-abs in maxbits slot iter & 1
 // (two slots of two words); clears the other slot for the score update to fill.
-void launch_grad_quant(hipStream_t s, float const *score, float const *label, u64 n, int obj, u32 *maxbits,
-                       u64 seed, TreeCtl const *ctl, u32 *ghq, double *scales);
+void launch_grad_quant(hipStream_t s, float const *score, float const *label, float const *weight, u64 n, int obj,
+                       u32 *maxbits, u64 seed, TreeCtl const *ctl, u32 *ghq, double *scales);
 // Fills maxbits slot ctl->iter & 1 from the current scores and labels (when they were
 // set outside the iteration loop).
-void launch_grad_prime(hipStream_t s, float const *score, float const *label, u64 n, int obj, u32 *maxbits,
-                       TreeCtl const *ctl);
+void launch_grad_prime(hipStream_t s, float const *score, float const *label, float const *weight, u64 n, int obj,
+                       u32 *maxbits, TreeCtl const *ctl);
 void launch_quant_from(hipStream_t s, float2 const *gh, u64 n, u32 *maxbits, u64 seed, u32 iter, u32 *ghq,
                        double *scales_dev);
 hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl const *ctl, u8 const *pl0,
@@ -58,7 +61,7 @@ void launch_part_scatter4(hipStream_t s, PartChunk const *pch, PartOp4 const *op
 void launch_finalize(hipStream_t s, NodeInfo const *front, TreeCtl const *ctl, DevNode *tree, GrowCfg const &cfg,
                      double const *scales, u32 max_front);
 void launch_next_level(hipStream_t s, TreeCtl *ctl);
-// Validation loss of one round (objective 0 MSE, 1 logloss) and the early-stopping rule
+// Validation loss of one round (objective 0 MSE, 1 logloss, 2 poisson) and the early-stopping rule
 // in the control block; eval_blocks(n) partials of scratch.
 u32 eval_blocks(u32 n);
 void launch_eval(hipStream_t s, int objective, float const *scores, float const *labels, u32 n, double *partial,
@@ -81,7 +84,8 @@ void launch_constraints_root(hipStream_t s, double *bounds, u32 *paths, u32 path
 // updated scores and the labels, into slot (ctl->iter + 1) & 1.
 void launch_score_update(hipStream_t s, u8 const *plane, u32 Fs, u64 n, DevNode const *tree, TreeCtl const *ctl,
                          u16 const *n_bins, u32 F, float lr, float *score, u32 *leaf_of_row, u32 max_nodes,
-                         float const *label = nullptr, int objective = 0, u32 *maxbits = nullptr);
+                         float const *label = nullptr, int objective = 0, u32 *maxbits = nullptr,
+                         float const *weight = nullptr);
 void launch_iter_advance(hipStream_t s, TreeCtl *ctl);
 void launch_copy_tree(hipStream_t s, DevNode const *tree, TreeCtl const *ctl, DevNode *out_trees, u32 *out_sizes,
                       u32 max_nodes);

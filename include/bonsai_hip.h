@@ -30,8 +30,10 @@ typedef struct bh_predictor bh_predictor;
 /* Growth policies: the reference's cuda_depthwise / cuda_leafwise / cuda_levelwise
  * registry names (include/bonsai/registry/names.hpp:81-94), here hip_*. */
 enum { BH_DEPTHWISE = 0, BH_LEAFWISE = 1, BH_LEVELWISE = 2 };
-/* Objectives with a device gradient (DeviceObjectiveKind, kernels.cuh:972-1009). */
-enum { BH_MSE = 0, BH_LOGLOSS = 1 };
+/* Objectives with a device gradient (DeviceObjectiveKind,
+ * include/bonsai/objective_traits.hpp:21-27; gh_from_scores_kernel, kernels.cuh:972-1009):
+ * squared error, logistic, Poisson with a log link (raw score clamped to +-30 before exp). */
+enum { BH_MSE = 0, BH_LOGLOSS = 1, BH_POISSON = 2 };
 
 /* TreeConfig + BoosterConfig subset plus the device knobs
  * (include/bonsai/config/tree_config.hpp:9-29, booster_config.hpp:8-24). */
@@ -111,6 +113,13 @@ int bh_upload_features(bh_context *ctx, const float *X, uint64_t n_rows, uint32_
 int bh_download_bins(bh_context *ctx, uint8_t *out, uint64_t out_stride);
 /* Labels of the resident objective (resident_begin, device_context.cu:1541-1579). */
 int bh_upload_labels(bh_context *ctx, const float *labels);
+/* Per-row sample weights of the resident objective (resident_begin uploads them for a
+ * weighted Dataset, include/bonsai/cuda/histogram_engine.hpp:249-250): the device gradient
+ * scales g and h by w[r] (gh_from_scores_kernel's Weighted arm, kernels.cuh:972-1009; the
+ * host multiply of include/bonsai/booster.hpp:546-553).  n_rows finite non-negative
+ * weights; NULL clears (unweighted).  Host gradients (bh_set_gradients) are taken as
+ * given. */
+int bh_upload_weights(bh_context *ctx, const float *weights);
 /* Configure the fit; allocates every device buffer the fit needs (the reference's
  * capacity gates refuse_hist_budget / refuse_leaf_pool, device_context.cu:60-87). */
 int bh_configure(bh_context *ctx, const bh_fit_config *cfg);
@@ -234,11 +243,14 @@ int bh_partition(bh_context *ctx, const uint32_t *rows, uint64_t n, uint32_t fea
  * own stream. */
 int bh_time_root_histogram(bh_context *ctx, uint32_t reps, float *ms_mean);
 /* Run n_iters depthwise iterations without a graph, timing every histogram launch with
- * hipEvents on the engine stream; totals of kernel time (ms), algorithmic bytes
- * (rows x (features + gradient-pair bytes) + 8 bytes per built (node, feature, bin) cell), launches and
- * rows.  Trees are appended like bh_train. */
+ * hipEvents on the engine stream.  Totals of kernel time (ms), launches, and bytes two
+ * ways: algorithmic (every built node's rows read once, rows x (features + gradient-pair
+ * bytes), plus 8 bytes per built (node, feature, bin) cell) with the built rows, and
+ * scanned (the rows the launches actually read: a level whose rows were not moved scans
+ * each parent's segment) with the scanned rows.  Trees are appended like bh_train. */
 int bh_profile_hist(bh_context *ctx, uint32_t n_iters, double *ms_total, double *bytes_total,
-                    uint64_t *launches, uint64_t *rows_total);
+                    uint64_t *launches, uint64_t *rows_total, double *scanned_bytes_total,
+                    uint64_t *scanned_rows_total);
 /* Per-iteration timing of bh_train (the reference's lap profile, src/cuda/detail/profile.cuh:39-40):
  * when enabled, hipEvents bracket every iteration; bh_get_timing writes ms[i] = device
  * time of iteration i of the last bh_train call (0 beyond its count). */

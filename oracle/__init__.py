@@ -107,6 +107,14 @@ class Oracle:
         self.L.orc_mse_grad(P(s), P(y), C.c_uint64(s.size), P(g), P(h))
         return g, h
 
+    def poisson_grad(self, s, y):
+        s = np.ascontiguousarray(s, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.zeros_like(s)
+        h = np.zeros_like(s)
+        self.L.orc_poisson_grad(P(s), P(y), C.c_uint64(s.size), P(g), P(h))
+        return g, h
+
     def init_score(self, objective, y):
         y = np.ascontiguousarray(y, np.float32)
         return self.L.orc_init_score(C.c_int32(objective), P(y), C.c_uint64(y.size))

@@ -1374,10 +1374,28 @@ void orc_mse_grad(const float *score, const float *target, uint64_t n, float *g,
         h[i] = 1.0F;
     }
 }
-// src/objective.cpp:77-84 (mse) and :122-130 (logloss): sequential float sums.  Pinned by
-// tests/test_oracle_objective.py (means and log odds in closed form).
+// src/objective.cpp:253-278 (poisson, log link): the raw score clamped to +-30
+// (k_poisson_max_log, include/bonsai/objective.hpp:47) before exp, g = mu - y, h = mu.
+// Pinned by tests/test_oracle_objective.py (score 0, the clamp, the double closed form).
+void orc_poisson_grad(const float *score, const float *label, uint64_t n, float *g, float *h)
+{
+    for (u64 i = 0; i < n; ++i) {
+        float const mu = std::exp(std::min(30.0F, std::max(-30.0F, score[i])));
+        g[i] = mu - label[i];
+        h[i] = mu;
+    }
+}
+// src/objective.cpp:77-84 (mse) and :122-130 (logloss): sequential float sums; :293-308
+// (poisson): log of the double mean, floored at 1e-9.  Pinned by
+// tests/test_oracle_objective.py (means, log odds and log means in closed form).
 float orc_init_score(int32_t objective, const float *label, uint64_t n)
 {
+    if (objective == 2) {
+        double sum = 0.0;
+        for (u64 i = 0; i < n; ++i) sum += label[i];
+        double const mean = sum / static_cast<double>(std::max<u64>(n, 1));
+        return static_cast<float>(std::log(std::max(mean, 1e-9)));
+    }
     float s = 0.0F;
     for (u64 i = 0; i < n; ++i) s += label[i];
     float const m = s / static_cast<float>(n);

@@ -81,10 +81,11 @@ void  orc_logloss_grad(const float *score, const float *label, uint64_t n, float
                        float *h);
 void  orc_mse_grad(const float *score, const float *target, uint64_t n, float *g,
                    float *h);
-float orc_init_score(int32_t objective, const float *label, uint64_t n); /* 0 mse, 1 logloss */
+float orc_init_score(int32_t objective, const float *label, uint64_t n); /* 0 mse, 1 logloss, 2 poisson */
 float orc_logloss_eval(const float *score, const float *label, uint64_t n);
 
 /* Stochastic-rounding quantizer (int8 g, int16 h, per-call max-abs scale). */
+void orc_poisson_grad(const float *score, const float *label, uint64_t n, float *g, float *h);
 void orc_quantize(const float *g, const float *h, uint64_t n, uint64_t seed,
                   uint32_t iter, int8_t *gq, int16_t *hq, float *max_g, float *max_h);
 /* Bernoulli row sampler (BernoulliSampler::sample, src/sampler.cpp:38-55) with the

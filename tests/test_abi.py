@@ -62,7 +62,8 @@ def test_fit_config_validation():
     with pytest.raises(bonsai_amd.ConfigError):
         bonsai_amd.FitConfig(grower="cuda_depthwise").to_c()
     with pytest.raises(bonsai_amd.ConfigError):
-        bonsai_amd.FitConfig(objective="poisson").to_c()
+        bonsai_amd.FitConfig(objective="quantile").to_c()
+    assert bonsai_amd.FitConfig(objective="poisson").to_c().objective == 2
     c = bonsai_amd.FitConfig(grower="hip_levelwise", quantized=True).to_c()
     assert c.grower == 2 and c.quantized == 1
 

@@ -86,8 +86,8 @@ def assert_codes_agree(gq, hq, sc, ogq, ohq, osc):
         assert (d == 0).mean() >= 0.9999
 
 
-def oracle_tree(orc, grower, bins, nb, depth, leaves, **grad):
-    c = orc_cfg(max_depth=depth, max_leaves=leaves)
+def oracle_tree(orc, grower, bins, nb, depth, leaves, min_data=20, **grad):
+    c = orc_cfg(max_depth=depth, max_leaves=leaves, min_data=min_data)
     if grower == "hip_depthwise":
         mode = HIST_INT if "gq" in grad else HIST_F32_CHUNKED
         return orc.grow_depthwise(bins, nb, c=c, mode=mode, **grad)[0]
@@ -111,6 +111,63 @@ def assert_same_grown(grower, tree, ref):
     assert np.array_equal(dc, lc)
 
 
+OBJ_ID = {"mse": 0, "logloss": 1, "poisson": 2}
+
+
+def replay_fit(hip, orc, bins, nb, y, grower, depth, leaves, quantized, iters, objective="logloss", seed=777,
+               lr=0.1, min_data=20, check_every=1, weights=None):
+    """Train `iters` device iterations one at a time and replay each on the host: the
+    device gradients against the oracle's (those through exp within rounding, the
+    squared-error ones exactly; weighted rows scaled by w in float), the oracle's tree
+    from the device's own gradients against the device tree, the scores bit for bit.
+    check_every > 1 checks the gradients and trees of every check_every-th iteration
+    only (the scores of all)."""
+    lr = np.float32(lr)
+    ctx = ctx_for(hip, bins, nb, grower=grower, objective=objective, quantized=quantized, max_depth=depth,
+                  max_leaves=leaves, learning_rate=float(lr), seed=seed, min_data_in_leaf=min_data)
+    ctx.upload_labels(y)
+    if weights is not None:
+        ctx.upload_weights(weights)
+    init = np.float32(orc.init_score(OBJ_ID[objective], y))
+    s = np.full(y.size, init, np.float32)
+    ctx.set_scores(s)
+    grad_fn = {"logloss": orc.logloss_grad, "mse": orc.mse_grad, "poisson": orc.poisson_grad}[objective]
+    exact = objective == "mse"
+    for t in range(iters):
+        assert np.array_equal(ctx.get_scores().view(np.uint32), s.view(np.uint32)), t
+        ctx.train(1)
+        tree = ctx.trees()[t]
+        if t % check_every == 0:
+            g, h = grad_fn(s, y)
+            if weights is not None:
+                g = (g * weights).astype(np.float32)
+                h = (h * weights).astype(np.float32)
+            if quantized:
+                gq, hq, sc = ctx.get_gradients()
+                og, oh, osc = orc.quantize(g, h, seed, t)
+                if exact:
+                    assert np.array_equal(gq, og) and np.array_equal(hq, oh) and np.array_equal(sc, osc)
+                else:
+                    assert_codes_agree(gq, hq, sc, og, oh, osc)
+                ref = oracle_tree(orc, grower, bins, nb, depth, leaves, min_data, gq=gq, hq=hq, scales=sc)
+            else:
+                gd, hd = ctx.get_gradients()
+                if exact:
+                    assert np.array_equal(gd, g) and np.array_equal(hd, h)
+                else:
+                    # a few ulps of the exp-derived terms (|h| scales them) and of the weights
+                    w = np.ones_like(h) if weights is None else np.maximum(weights, 1.0)
+                    tol = 2.5e-7 * np.maximum(1.0, np.abs(h)) * w
+                    assert np.all(np.abs(gd - g) <= tol) and np.all(np.abs(hd - h) <= tol)
+                ref = oracle_tree(orc, grower, bins, nb, depth, leaves, min_data, g=gd, h=hd)
+            assert_same_grown(grower, tree, ref)
+        s = (s + lr * leaf_values(tree, bins, nb)).astype(np.float32)
+    assert np.array_equal(ctx.get_scores().view(np.uint32), s.view(np.uint32))
+    trees = ctx.trees()
+    ctx.close()
+    return trees, s
+
+
 @pytest.mark.parametrize("quantized", [True, False])
 @pytest.mark.parametrize("grower,depth,leaves", [("hip_depthwise", 6, 31), ("hip_leafwise", 10, 63),
                                                  ("hip_levelwise", 5, 31)])
@@ -119,30 +176,7 @@ def test_logloss_iterations_match_the_oracle(hip, orc, tall, grower, depth, leav
     the next iteration) against the oracle's logloss_grad + quantize of the replayed
     scores, then the tree grown from the device's own gradients."""
     bins, nb, X, y = tall
-    lr, seed, iters = np.float32(0.1), 777, 5
-    ctx = ctx_for(hip, bins, nb, grower=grower, objective="logloss", quantized=quantized, max_depth=depth,
-                  max_leaves=leaves, learning_rate=float(lr), seed=seed)
-    ctx.upload_labels(y)
-    init = np.float32(orc.init_score(1, y))
-    s = np.full(y.size, init, np.float32)
-    ctx.set_scores(s)
-    for t in range(iters):
-        assert np.array_equal(ctx.get_scores().view(np.uint32), s.view(np.uint32)), t
-        ctx.train(1)
-        g, h = orc.logloss_grad(s, y)
-        if quantized:
-            gq, hq, sc = ctx.get_gradients()
-            assert_codes_agree(gq, hq, sc, *orc.quantize(g, h, seed, t))
-            ref = oracle_tree(orc, grower, bins, nb, depth, leaves, gq=gq, hq=hq, scales=sc)
-        else:
-            gd, hd = ctx.get_gradients()
-            assert np.max(np.abs(gd - g)) <= 2e-7 and np.max(np.abs(hd - h)) <= 2e-7
-            ref = oracle_tree(orc, grower, bins, nb, depth, leaves, g=gd, h=hd)
-        tree = ctx.trees()[t]
-        assert_same_grown(grower, tree, ref)
-        s = (s + lr * leaf_values(tree, bins, nb)).astype(np.float32)
-    assert np.array_equal(ctx.get_scores().view(np.uint32), s.view(np.uint32))
-    ctx.close()
+    replay_fit(hip, orc, bins, nb, y, grower, depth, leaves, quantized, iters=5)
 
 
 def test_gradients_read_back_as_handed_over(hip, orc, tall):
@@ -160,3 +194,57 @@ def test_gradients_read_back_as_handed_over(hip, orc, tall):
     rg, rh = ctx.get_gradients()
     assert np.array_equal(rg, g) and np.array_equal(rh, h)
     ctx.close()
+
+
+@pytest.mark.parametrize("quantized", [True, False])
+@pytest.mark.parametrize("objective", ["logloss", "mse"])
+def test_weighted_iterations_match_the_oracle(hip, orc, tall, objective, quantized):
+    """Row weights scale the device gradients (gh_from_scores_kernel's Weighted arm,
+    src/cuda/detail/kernels.cuh:972-1009): g * w and h * w in float, then the same trees
+    as the oracle grown from them."""
+    bins, nb, X, y = tall
+    w = np.random.default_rng(21).uniform(0.25, 3.0, y.size).astype(np.float32)
+    w[::97] = 0.0                                    # zero-weight rows carry no gradient
+    target = y if objective == "logloss" else (y * np.float32(3.0) - np.float32(1.0)).astype(np.float32)
+    replay_fit(hip, orc, bins, nb, target, "hip_depthwise", 6, 31, quantized, iters=4, objective=objective,
+               weights=w)
+
+
+def poisson_labels(X):
+    z = np.where(np.isnan(X[:, 0]), np.float32(0), X[:, 0])
+    lam = np.exp(np.float32(0.4) * z + np.float32(0.3) * (X[:, 10] > 0))
+    return np.random.default_rng(23).poisson(lam).astype(np.float32)
+
+
+@pytest.mark.parametrize("quantized", [True, False])
+@pytest.mark.parametrize("grower,depth,leaves", [("hip_depthwise", 6, 31), ("hip_leafwise", 10, 63)])
+def test_poisson_iterations_match_the_oracle(hip, orc, tall, grower, depth, leaves, quantized):
+    """The Poisson device gradient (log link, score clamped to +-30, src/objective.cpp:253-278)."""
+    bins, nb, X, y = tall
+    replay_fit(hip, orc, bins, nb, poisson_labels(X), grower, depth, leaves, quantized, iters=4, objective="poisson")
+
+
+def test_weights_refused_and_cleared(hip, orc, tall):
+    bins, nb, X, y = tall
+    ctx = ctx_for(hip, bins, nb, quantized=True, max_depth=4)
+    bad = np.ones(y.size, np.float32)
+    bad[5] = -1.0
+    with pytest.raises(hip.ConfigError):
+        ctx.upload_weights(bad)
+    bad[5] = np.inf
+    with pytest.raises(hip.ConfigError):
+        ctx.upload_weights(bad)
+    ctx.upload_labels(y)
+    ctx.set_scores(np.zeros(y.size, np.float32))
+    ctx.upload_weights(np.full(y.size, 2.0, np.float32))
+    ctx.upload_weights(None)                         # cleared: the unweighted fit
+    ctx.train(2)
+    a = ctx.trees()
+    ctx.close()
+    ctx = ctx_for(hip, bins, nb, quantized=True, max_depth=4)
+    ctx.upload_labels(y)
+    ctx.set_scores(np.zeros(y.size, np.float32))
+    ctx.train(2)
+    b = ctx.trees()
+    ctx.close()
+    assert all(x.tobytes() == z.tobytes() for x, z in zip(a, b))

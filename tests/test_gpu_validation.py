@@ -28,6 +28,9 @@ def host_loss(scores, labels, objective):
     if objective == "logloss":
         t = np.maximum(np.float32(0), s) + np.log1p(np.exp(-np.abs(s))) - y * s
         return t.astype(np.float64).sum() / s.size
+    if objective == "poisson":   # src/objective.cpp:280-291
+        f = np.clip(s, np.float32(-30.0), np.float32(30.0)).astype(np.float64)
+        return (np.exp(f) - y.astype(np.float64) * f).sum() / s.size
     return ((s.astype(np.float64) - y.astype(np.float64)) ** 2).sum() / s.size
 
 
@@ -35,14 +38,17 @@ def fitted(hip, orc, bins, nb, y, objective, quantized, use_graph=True):
     ctx = ctx_for(hip, bins[:N_TR], nb, quantized=quantized, max_depth=5, learning_rate=0.1,
                   objective=objective, use_graph=use_graph)
     ctx.upload_labels(y[:N_TR])
-    init = orc.init_score(1 if objective == "logloss" else 0, y[:N_TR])
+    init = orc.init_score({"mse": 0, "logloss": 1, "poisson": 2}[objective], y[:N_TR])
     ctx.set_scores(np.full(N_TR, init, np.float32))
     return ctx, init
 
 
-@pytest.mark.parametrize("objective,quantized", [("logloss", True), ("logloss", False), ("mse", True)])
+@pytest.mark.parametrize("objective,quantized", [("logloss", True), ("logloss", False), ("mse", True),
+                                                 ("poisson", True)])
 def test_loss_history_matches_host(hip, orc, data, objective, quantized):
     bins, nb, y = data
+    if objective == "poisson":
+        y = np.random.default_rng(3).poisson(np.where(y > 0, 2.0, 0.5)).astype(np.float32)
     ctx, init = fitted(hip, orc, bins, nb, y, objective, quantized)
     yv = y[N_TR:]
     ctx.set_validation(bins[N_TR:], yv, init)

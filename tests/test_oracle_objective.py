@@ -82,3 +82,29 @@ def test_logloss_eval_is_the_mean_cross_entropy(orc):
     p = 1.0 / (1.0 + np.exp(-s.astype(np.float64)))
     ce = -(y * np.log(p) + (1 - y) * np.log1p(-p)).mean()
     assert orc.logloss_eval(s, y) == pytest.approx(ce, rel=1e-5)
+
+
+def test_poisson_grad_closed_form_and_clamp(orc):
+    """PoissonObjective::compute (src/objective.cpp:253-278): g = exp(s) - y, h = exp(s)
+    with the raw score clamped to +-30; s = 0 gives exactly 1 - y and 1."""
+    rng = np.random.default_rng(5)
+    s = np.concatenate([rng.normal(0, 2, 2000), [0.0, 31.0, 100.0, -31.0, -100.0]]).astype(np.float32)
+    y = rng.poisson(2.0, s.size).astype(np.float32)
+    g, h = orc.poisson_grad(s, y)
+    mu = np.exp(np.clip(s.astype(np.float64), -30.0, 30.0))
+    assert np.all(np.abs(h - mu) <= 2e-7 * mu)
+    assert np.all(np.abs(g - (mu - y)) <= 2e-7 * mu + 1e-6 * np.abs(y))
+    z = s == 0
+    assert np.array_equal(g[z], np.float32(1.0) - y[z]) and np.all(h[z] == 1.0)
+    assert h[s.size - 4] == h[s.size - 3] == np.float32(np.exp(np.float32(30.0)))   # clamped above
+    assert h[s.size - 2] == h[s.size - 1] == np.float32(np.exp(np.float32(-30.0)))  # and below
+
+
+def test_init_score_poisson_is_the_log_mean(orc):
+    """PoissonObjective::init_score (src/objective.cpp:293-308): log of the double mean,
+    floored at 1e-9 so all-zero labels stay finite."""
+    assert orc.init_score(2, np.array([1, 2, 3, 6], np.float32)) == pytest.approx(math.log(3.0), abs=1e-6)
+    assert orc.init_score(2, np.zeros(10, np.float32)) == pytest.approx(math.log(1e-9), rel=1e-6)
+    rng = np.random.default_rng(6)
+    y = rng.poisson(0.7, 10000).astype(np.float32)
+    assert orc.init_score(2, y) == pytest.approx(math.log(y.astype(np.float64).mean()), abs=1e-6)
