@@ -58,6 +58,16 @@ class TuningC(C.Structure):
                 ("hist_group", C.c_uint32)]
 
 
+class StepNodeC(C.Structure):
+    _fields_ = [("depth", C.c_uint32), ("_pad", C.c_uint32), ("rows", C.c_uint64), ("sum_g", C.c_double),
+                ("sum_h", C.c_double), ("isum_g", C.c_int64), ("isum_h", C.c_int64)]
+
+
+class LeafRoundC(C.Structure):
+    _fields_ = [("left_slot", C.c_uint32), ("right_slot", C.c_uint32), ("left_count", C.c_uint32),
+                ("right_count", C.c_uint32)]
+
+
 class ServeConfigC(C.Structure):
     _fields_ = [("n_slots", C.c_uint32), ("do_shap", C.c_uint32), ("max_batch_rows", C.c_uint64),
                 ("hbm_budget", C.c_uint64)]
@@ -169,6 +179,18 @@ def lib() -> C.CDLL:
         "bh_find_split": [P, P, P, u64, u32, i32, C.c_double, C.c_double, P, P],
         "bh_partition": [P, P, u64, u32, u32, u32, P, C.POINTER(u64)],
         "bh_time_root_histogram": [P, u32, C.POINTER(C.c_float)],
+        "bh_begin_root": [P, P, C.POINTER(u64)],
+        "bh_partition_level": [P, P, u32, P],
+        "bh_advance_level": [P, P, u32],
+        "bh_advance_layout_only": [P],
+        "bh_find_splits_many": [P, P, u32, P],
+        "bh_find_level_split": [P, P, u32, P],
+        "bh_stamp_leaves": [P, P, u32],
+        "bh_finalize_tree": [P, P, u32, P, P],
+        "bh_leaf_begin_root": [P, P, C.POINTER(u64)],
+        "bh_leaf_split": [P, u32, u32, u32, u32, C.POINTER(LeafRoundC)],
+        "bh_leaf_build": [P, u32, u32],
+        "bh_leaf_find": [P, P, P, u32, P],
         "bh_set_timing": [P, i32],
         "bh_profile_hist": [P, u32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(u64),
                             C.POINTER(u64), C.POINTER(C.c_double), C.POINTER(u64)],
@@ -537,6 +559,82 @@ class HipContext:
         self.n_trained += n_iters
         return {"ms": ms.value, "bytes": by.value, "launches": la.value, "rows": ro.value,
                 "scanned_bytes": sb.value, "scanned_rows": sr.value}
+
+    # -- engine steps (the reference's GPULevelEngine / GPULeafEngine methods) --
+    def begin_root(self, leaf_plane: bool = False):
+        """Root histogram into slot 0; returns ((sum_g, sum_h), rows)."""
+        sums = np.zeros(2, np.float64)
+        rows = C.c_uint64(0)
+        fn = lib().bh_leaf_begin_root if leaf_plane else lib().bh_begin_root
+        _check(fn(self._h, _ptr(sums), C.byref(rows)))
+        return (float(sums[0]), float(sums[1])), rows.value
+
+    def partition_level(self, ops) -> np.ndarray:
+        """ops: (parent_slot, left_slot, right_slot, feature, bin, default_left) tuples;
+        returns [n_left, n_right] per op."""
+        a = np.ascontiguousarray(np.array(ops, np.uint32).reshape(-1, 6))
+        counts = np.zeros(2 * len(a), np.uint32)
+        _check(lib().bh_partition_level(self._h, _ptr(a), len(a), _ptr(counts)))
+        return counts.reshape(-1, 2)
+
+    def advance_level(self, ops) -> None:
+        """ops: (parent_slot, small_slot, large_slot) tuples."""
+        a = np.ascontiguousarray(np.array(ops, np.uint32).reshape(-1, 3))
+        _check(lib().bh_advance_level(self._h, _ptr(a), len(a)))
+
+    def advance_layout_only(self) -> None:
+        _check(lib().bh_advance_layout_only(self._h))
+
+    @staticmethod
+    def _step_nodes(depths):
+        arr = (StepNodeC * max(1, len(depths)))()
+        for i, d in enumerate(depths):
+            arr[i].depth = int(d)
+        return arr
+
+    @staticmethod
+    def _step_out(arr, n):
+        return [(arr[i].rows, arr[i].sum_g, arr[i].sum_h, arr[i].isum_g, arr[i].isum_h) for i in range(n)]
+
+    def find_splits_many(self, depths):
+        """Split of every slot of the current level (node i = slot i); returns (splits,
+        [(rows, sum_g, sum_h, isum_g, isum_h)])."""
+        nodes = self._step_nodes(depths)
+        out = np.zeros(max(1, len(depths)), SPLIT_DTYPE)
+        _check(lib().bh_find_splits_many(self._h, nodes, len(depths), _ptr(out)))
+        return out[: len(depths)], self._step_out(nodes, len(depths))
+
+    def find_level_split(self, depths):
+        nodes = self._step_nodes(depths)
+        out = np.zeros(max(1, len(depths)), SPLIT_DTYPE)
+        _check(lib().bh_find_level_split(self._h, nodes, len(depths), _ptr(out)))
+        return out[: len(depths)], self._step_out(nodes, len(depths))
+
+    def stamp_leaves(self, stamps) -> None:
+        a = np.ascontiguousarray(np.array(stamps, np.uint32).reshape(-1, 2))
+        _check(lib().bh_stamp_leaves(self._h, _ptr(a), len(a)))
+
+    def finalize_tree(self, node_values):
+        v = np.ascontiguousarray(node_values, np.float32)
+        values = np.zeros(self.n_rows, np.float32)
+        ids = np.zeros(self.n_rows, np.uint32)
+        _check(lib().bh_finalize_tree(self._h, _ptr(v), v.size, _ptr(values), _ptr(ids)))
+        return values, ids
+
+    def leaf_split(self, parent_slot, feature, bin_, default_left):
+        r = LeafRoundC()
+        _check(lib().bh_leaf_split(self._h, parent_slot, feature, bin_, int(default_left), C.byref(r)))
+        return r.left_slot, r.right_slot, r.left_count, r.right_count
+
+    def leaf_build(self, small_slot, large_slot) -> None:
+        _check(lib().bh_leaf_build(self._h, small_slot, large_slot))
+
+    def leaf_find(self, depths, slots):
+        nodes = self._step_nodes(depths)
+        sl = np.ascontiguousarray(np.array(slots, np.uint32))
+        out = np.zeros(max(1, len(depths)), SPLIT_DTYPE)
+        _check(lib().bh_leaf_find(self._h, nodes, _ptr(sl), len(depths), _ptr(out)))
+        return out[: len(depths)], self._step_out(nodes, len(depths))
 
     def time_root_histogram(self, reps: int = 10) -> float:
         ms = C.c_float(0)

@@ -51,6 +51,13 @@ u32 round_up4(u32 x) { return (x + 3u) & ~3u; }
 
 void bh_set_error(std::string const &m) { g_err = m; }
 
+// One slot's rows for the engine steps: a segment of a plane (0/1 work, 2 pristine)
+// and the routing-tree node the rows sit in.
+struct StepSeg {
+    u32 off = 0, cnt = 0, src = 2, route = 0;
+    bool live = false;
+};
+
 struct bh_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -136,6 +143,14 @@ struct bh_context {
     bool prof = false;
     double prof_ms = 0.0, prof_bytes = 0.0, prof_scanned_bytes = 0.0;
     u64 prof_launches = 0, prof_rows = 0, prof_scanned = 0;
+    // engine steps (bh_begin_root ... bh_finalize_tree, bh_leaf_*): the segment of every
+    // slot of the current and next level (level plane) or of the slot pool (leaf plane),
+    // the histogram buffer of the current level, and the routing tree of the partitions
+    int step_mode = 0; // 0 none, 1 level plane, 2 leaf plane
+    std::vector<StepSeg> seg_cur, seg_next, pool;
+    u32 step_hist = 0, step_next_slot = 0;
+    std::vector<DevNode> route;
+    std::vector<u32> route_stamp;
     // timing
     bool timing = false;
     std::vector<float> iter_ms;
@@ -474,7 +489,7 @@ void enqueue_build(bh_context *c, void *hist_cur, void *hist_next, u32 max_ops, 
                     gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), grid_units,
                     c->partials, hist_next, cnt_next, routes, group);
     launch_finish(c->stream, c->quant, c->fops, c->ctl, c->partials, hist_cur, hist_next, c->node_cells(), max_ops,
-                  cnt_cur, cnt_next, advance);
+                  cnt_cur, cnt_next, advance, hist_transpose_F(c->quant, c->Fs, c->F, group));
     if (c->prof) profiled_count(c, ms, cnt_next);
 }
 
@@ -1310,7 +1325,7 @@ static int validate_tuning(bh_tuning const *t)
     if (t->hist_threads != 256u && t->hist_threads != 512u && t->hist_threads != 1024u)
         return fail(1, "hist_threads must be 256, 512 or 1024");
     if (t->part_grid < 64u || t->part_grid > 65536u) return fail(1, "part_grid must be in [64, 65536]");
-    if (t->hist_layout > 2u) return fail(1, "hist_layout must be 0, 1 or 2");
+    if (t->hist_layout > 3u) return fail(1, "hist_layout must be 0, 1, 2 or 3");
     if (t->pair_levels > 1u) return fail(1, "pair_levels must be 0 or 1");
     if (t->hist_group < 1u || t->hist_group > kHistGroupMax) return fail(1, "hist_group must be in [1, 4]");
     return 0;
@@ -1389,7 +1404,7 @@ int bh_autotune(bh_context *c, uint32_t iters, bh_tuning *chosen, float *ms_per_
     static const u32 thr_c[] = {256u, 512u, 1024u};
     static const u32 fb_c[] = {1u, 2u, 4u, 8u};
     static const u32 pg_c[] = {512u, 1024u, 2048u, 4096u};
-    static const u32 hl_c[] = {0u, 1u, 2u};
+    static const u32 hl_c[] = {0u, 1u, 2u, 3u};
     static const u32 pl_c[] = {0u, 1u};
     static const u32 hg_c[] = {1u, 2u, 4u};
     for (int knob = 0; rc == 0 && knob < 7; ++knob) {
@@ -1400,7 +1415,7 @@ int bh_autotune(bh_context *c, uint32_t iters, bh_tuning *chosen, float *ms_per_
                           : knob == 4 ? hl_c
                           : knob == 5 ? pl_c
                                       : hg_c;
-        int const nc = knob == 0 ? 7 : knob == 1 || knob == 4 || knob == 6 ? 3 : knob == 5 ? 2 : 4;
+        int const nc = knob == 0 ? 7 : knob == 1 || knob == 6 ? 3 : knob == 5 ? 2 : 4;
         for (int k = 0; rc == 0 && k < nc; ++k) {
             Tuning t = best;
             u32 &slot = knob == 0   ? t.scatter_lds
@@ -1745,6 +1760,558 @@ int bh_time_root_histogram(bh_context *c, uint32_t reps, float *ms_mean)
     }
     *ms_mean = reps ? total / static_cast<float>(reps) : 0.0F;
     return 0;
+}
+
+
+// =================================================================================
+// Engine steps: the reference's GPULevelEngine / GPULeafEngine methods
+// (include/bonsai/grower.hpp:91-162; CudaHistogramEngine,
+// include/bonsai/cuda/histogram_engine.hpp:90-218) for growers that keep the control loop
+// on the host (LevelStep / LeafStep, src/step/level.hpp:159-366, src/step/leaf.hpp:124-267).
+// Rows never cross the bus: a partition moves each parent's segment into the other work
+// plane (left rows ascending, then right rows, at the parent's offsets), histograms stay
+// in slot-indexed device buffers, and only ops, counts and split records move.  A host
+// routing tree records every partition, so finalize walks each row of the pristine plane
+// to its stamped leaf.
+// =================================================================================
+namespace {
+
+// The control block fields the step launches read.
+int step_ctl(bh_context *c, u32 n_front, u32 n_ops, u32 n_part_chunks, u32 n_hist_chunks, u32 n_finish)
+{
+    TreeCtl t{};
+    t.n_front = n_front;
+    t.n_ops = n_ops;
+    t.n_part_chunks = n_part_chunks;
+    t.n_hist_chunks = n_hist_chunks;
+    t.n_finish = n_finish;
+    HIP_TRY(hipMemcpy(c->ctl, &t, sizeof(t), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int step_ready(bh_context *c)
+{
+    if (int r = check_ready(c)) return r;
+    if (c->gc.mono || c->gc.n_groups)
+        return fail(1, "engine steps take no monotone or interaction constraints (the host grower owns them)");
+    if (c->fsamp_kind == 2)
+        return fail(1, "engine steps take a handed-over feature mask, not the device feature draw");
+    return 0;
+}
+
+// The first selected feature (its cells give node totals, include/bonsai/split.hpp:36-53).
+u32 step_totals_feature(bh_context *c)
+{
+    if (c->fsamp_kind != 1 || !c->fmask) return 0;
+    std::vector<u8> m(c->F);
+    if (hipMemcpy(m.data(), c->fmask, c->F, hipMemcpyDeviceToHost) != hipSuccess) return 0;
+    for (u32 f = 0; f < c->F; ++f)
+        if (m[f]) return f;
+    return 0;
+}
+
+// Build the histograms of `builds` (small slot, its segment, large slot, parent slot):
+// 8,192-row units, partials folded in unit order, large = parent - small.
+struct StepBuild {
+    u32 small, large, parent;
+    StepSeg seg;
+};
+int step_build(bh_context *c, std::vector<StepBuild> const &builds, void *hist_cur, void *hist_next)
+{
+    std::vector<HistChunk> hc;
+    std::vector<FinishOp> fo;
+    u32 parts = 0;
+    for (auto const &b : builds) {
+        u32 const units = (b.seg.cnt + kHistChunk - 1) / kHistChunk;
+        FinishOp f{};
+        f.small_slot = b.small;
+        f.large_slot = b.large;
+        f.parent_slot = b.parent;
+        f.part_first = parts;
+        f.part_count = units > 1 ? units : 0;
+        f.empty = b.seg.cnt == 0 ? 1u : 0u;
+        for (u32 k = 0; k < units; ++k) {
+            HistChunk h{};
+            h.slot = b.small;
+            h.src = b.seg.src;
+            h.begin = b.seg.off + k * kHistChunk;
+            h.end = b.seg.off + std::min(b.seg.cnt, (k + 1) * kHistChunk);
+            h.part = units > 1 ? parts + k : kNone;
+            hc.push_back(h);
+        }
+        if (units > 1) parts += units;
+        fo.push_back(f);
+    }
+    if (hc.size() > c->max_hch || parts > c->max_parts || fo.size() > c->max_ops + 2)
+        return fail(1, "engine step: more histogram units than the configured fit holds");
+    if (!hc.empty()) HIP_TRY(hipMemcpy(c->hch, hc.data(), hc.size() * sizeof(HistChunk), hipMemcpyHostToDevice));
+    if (!fo.empty()) HIP_TRY(hipMemcpy(c->fops, fo.data(), fo.size() * sizeof(FinishOp), hipMemcpyHostToDevice));
+    if (int r = step_ctl(c, 0, 0, 0, static_cast<u32>(hc.size()), static_cast<u32>(fo.size()))) return r;
+    HIP_TRY(launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
+                        gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(),
+                        std::max(1u, std::min(static_cast<u32>(hc.size()), 4096u)), c->partials, hist_next));
+    launch_finish(c->stream, c->quant, c->fops, c->ctl, c->partials, hist_cur, hist_next, c->node_cells(),
+                  std::max<u32>(1u, static_cast<u32>(fo.size())), nullptr, nullptr, false,
+                  hist_transpose_F(c->quant, c->Fs, c->F));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// Stable partition of each segment by its rule into the other work plane at the same
+// offsets; n_left per op out.
+struct StepPart {
+    StepSeg seg;
+    u32 feature, bin, default_left;
+};
+int step_partition(bh_context *c, std::vector<StepPart> const &parts, std::vector<u32> &n_left)
+{
+    u32 const chunk_rows = part_chunk_rows(c->Fs);
+    std::vector<PartOp> ops(parts.size());
+    std::vector<PartChunk> pch;
+    for (size_t i = 0; i < parts.size(); ++i) {
+        StepPart const &p = parts[i];
+        PartOp &o = ops[i];
+        o = PartOp{};
+        o.parent = static_cast<u32>(i);
+        o.feature = p.feature;
+        o.bin = p.bin;
+        o.default_left = p.default_left;
+        o.last_bin = static_cast<u32>(c->n_bins[p.feature]) - 1u;
+        o.off = p.seg.off;
+        o.cnt = p.seg.cnt;
+        o.src = p.seg.src;
+        o.dst = p.seg.src == 0 ? 1u : 0u;
+        o.chunk_first = static_cast<u32>(pch.size());
+        o.alive = 1;
+        for (u32 b = 0; b < p.seg.cnt; b += chunk_rows)
+            pch.push_back(PartChunk{static_cast<u32>(i), b, std::min(p.seg.cnt, b + chunk_rows), 0u});
+        o.chunk_count = static_cast<u32>(pch.size()) - o.chunk_first;
+    }
+    if (ops.size() > c->max_ops + 2 || pch.size() > c->max_pch)
+        return fail(1, "engine step: more partition work than the configured fit holds");
+    n_left.assign(parts.size(), 0u);
+    if (pch.empty()) return 0;
+    HIP_TRY(hipMemcpy(c->ops, ops.data(), ops.size() * sizeof(PartOp), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->pch, pch.data(), pch.size() * sizeof(PartChunk), hipMemcpyHostToDevice));
+    if (int r = step_ctl(c, 0, static_cast<u32>(ops.size()), static_cast<u32>(pch.size()), 0, 0)) return r;
+    launch_part_count(c->stream, c->pch, c->ops, c->ctl, c->work[0], c->work[1], c->plane0, c->Fs, c->chunk_left,
+                      part_grid(c));
+    std::vector<u32> cl(pch.size());
+    HIP_TRY(hipMemcpy(cl.data(), c->chunk_left, cl.size() * sizeof(u32), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < pch.size(); ++k) {
+        PartOp &o = ops[pch[k].op];
+        pch[k].left_before = o.n_left;
+        o.n_left += cl[k];
+    }
+    for (size_t i = 0; i < ops.size(); ++i) n_left[i] = ops[i].n_left;
+    HIP_TRY(hipMemcpy(c->ops, ops.data(), ops.size() * sizeof(PartOp), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->pch, pch.data(), pch.size() * sizeof(PartChunk), hipMemcpyHostToDevice));
+    u32 const gh_bytes = c->quant ? 4 : 8;
+    launch_part_scatter(c->stream, c->pch, c->ops, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
+                        gh_src(c, 1), gh_src(c, 2), c->work[0], c->work[1], const_cast<void *>(gh_src(c, 0)),
+                        const_cast<void *>(gh_src(c, 1)), c->Fs, gh_bytes, part_grid(c));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// Node totals as the finders take them: the first selected feature's cells of the node's
+// histogram, summed in bin order (FP32: double sums stored as float, Histogram::totals,
+// include/bonsai/histogram.hpp:76-87; quantized: exact integers, times the scales).
+int step_totals(bh_context *c, std::vector<NodeInfo> const &front, void const *hist, bh_step_node *nodes)
+{
+    u32 const f0 = step_totals_feature(c);
+    u32 const nb = c->n_bins[f0];
+    double sc[2] = {0.0, 0.0};
+    if (c->quant) HIP_TRY(hipMemcpy(sc, c->scales, sizeof(sc), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < front.size(); ++i) {
+        size_t const off = static_cast<size_t>(front[i].slot) * c->node_cells() + static_cast<size_t>(f0) * c->SB;
+        bh_step_node &o = nodes[i];
+        o.rows = front[i].cnt;
+        if (c->quant) {
+            std::vector<i64> cells(2 * nb);
+            HIP_TRY(hipMemcpy(cells.data(), static_cast<i64 const *>(hist) + 2 * off, cells.size() * sizeof(i64),
+                              hipMemcpyDeviceToHost));
+            i64 g = 0, h = 0;
+            for (u32 b = 0; b < nb; ++b) {
+                g += cells[2 * b];
+                h += cells[2 * b + 1];
+            }
+            o.isum_g = g;
+            o.isum_h = h;
+            o.sum_g = static_cast<double>(g) * sc[0];
+            o.sum_h = static_cast<double>(h) * sc[1];
+        } else {
+            std::vector<float> cells(2 * nb);
+            HIP_TRY(hipMemcpy(cells.data(), static_cast<float const *>(hist) + 2 * off,
+                              cells.size() * sizeof(float), hipMemcpyDeviceToHost));
+            double g = 0.0, h = 0.0;
+            for (u32 b = 0; b < nb; ++b) {
+                g += cells[2 * b];
+                h += cells[2 * b + 1];
+            }
+            o.sum_g = static_cast<float>(g);
+            o.sum_h = static_cast<float>(h);
+            o.isum_g = o.isum_h = 0;
+        }
+    }
+    return 0;
+}
+
+// The split records of nodes laid out as NodeInfo (slot = histogram slot), with their
+// totals and rows into nodes.
+int step_find(bh_context *c, std::vector<NodeInfo> const &front, void const *hist, bh_split *out,
+              bh_step_node *nodes)
+{
+    u32 const n = static_cast<u32>(front.size());
+    if (n == 0) return 0;
+    if (int r = step_totals(c, front, hist, nodes)) return r;
+    if (n > std::max(c->max_front * 2u, 4u)) return fail(1, "engine step: more nodes than the fit's frontier");
+    HIP_TRY(hipMemcpy(c->front[0], front.data(), n * sizeof(NodeInfo), hipMemcpyHostToDevice));
+    if (int r = step_ctl(c, n, 0, 0, 0, 0)) return r;
+    launch_find(c->stream, c->quant, c->front[0], c->ctl, hist, c->d_nbins, c->F, c->SB, c->gc, c->scales, c->splits,
+                n, 0, n, c->fscr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    static_assert(sizeof(SplitRec) == sizeof(bh_split), "bh_split layout");
+    HIP_TRY(hipMemcpy(out, c->splits, n * sizeof(SplitRec), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Root of a step-driven tree: every row (pristine plane) or the row subset (gathered
+// into work plane 1), its histogram into slot 0 of hist[0], the totals from it.
+int step_root(bh_context *c, double *sums, uint64_t *rows)
+{
+    StepSeg root;
+    root.live = true;
+    if (c->samp_kind == 2) return fail(1, "engine steps take a handed-over row subset, not the device Bernoulli draw");
+    if (c->samp_kind == 1) {
+        launch_gather_rows(c->stream, c->plane0, c->Fs, gh_src(c, 2), c->quant ? 4 : 8, c->samp_rows, c->samp_n,
+                           c->work[1], const_cast<void *>(gh_src(c, 1)), 0);
+        root.src = 1;
+        root.cnt = c->samp_n;
+    } else {
+        root.src = kSrcPlane0;
+        root.cnt = static_cast<u32>(c->N);
+    }
+    if (int r = step_build(c, {StepBuild{0u, kNone, kNone, root}}, c->hist[1] ? c->hist[1] : c->hist[0], c->hist[0]))
+        return r;
+    u32 const f0 = step_totals_feature(c);
+    u32 const nb = c->n_bins[f0];
+    size_t const off = static_cast<size_t>(f0) * c->SB;
+    double g = 0.0, h = 0.0;
+    if (c->quant) {
+        std::vector<i64> cells(2 * nb);
+        double sc[2];
+        HIP_TRY(hipMemcpy(cells.data(), static_cast<i64 const *>(c->hist[0]) + 2 * off, cells.size() * sizeof(i64),
+                          hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(sc, c->scales, sizeof(sc), hipMemcpyDeviceToHost));
+        i64 ig = 0, ih = 0;
+        for (u32 b = 0; b < nb; ++b) {
+            ig += cells[2 * b];
+            ih += cells[2 * b + 1];
+        }
+        g = static_cast<double>(ig) * sc[0];
+        h = static_cast<double>(ih) * sc[1];
+    } else {
+        std::vector<float> cells(2 * nb);
+        HIP_TRY(hipMemcpy(cells.data(), static_cast<float const *>(c->hist[0]) + 2 * off,
+                          cells.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (u32 b = 0; b < nb; ++b) {
+            g += cells[2 * b];
+            h += cells[2 * b + 1];
+        }
+        g = static_cast<float>(g); // Histogram::totals stores the double sums as float
+        h = static_cast<float>(h);
+    }
+    sums[0] = g;
+    sums[1] = h;
+    *rows = root.cnt;
+    c->route.assign(1, DevNode{-1, 0, 0, 0, 0, 0.0F, 0.0F, 0.0F});
+    c->route_stamp.assign(1, kNone);
+    return 0;
+}
+
+// Record a partition in the routing tree: the parent's node gets the split, its two
+// children are new leaves.
+void route_split(bh_context *c, u32 node, u32 feature, u32 bin, u32 dl, u32 &left, u32 &right)
+{
+    left = static_cast<u32>(c->route.size());
+    right = left + 1u;
+    DevNode &p = c->route[node];
+    p.feature = static_cast<int32_t>(feature);
+    p.bin = bin;
+    p.default_left = dl;
+    p.left = left;
+    p.right = right;
+    c->route.push_back(DevNode{-1, 0, 0, 0, 0, 0.0F, 0.0F, 0.0F});
+    c->route.push_back(DevNode{-1, 0, 0, 0, 0, 0.0F, 0.0F, 0.0F});
+    c->route_stamp.push_back(kNone);
+    c->route_stamp.push_back(kNone);
+}
+
+NodeInfo step_node(StepSeg const &g, u32 slot, u32 depth)
+{
+    NodeInfo nd{};
+    nd.id = g.route;
+    nd.off = g.off;
+    nd.cnt = g.cnt;
+    nd.src = g.src;
+    nd.slot = slot;
+    nd.depth = depth;
+    return nd;
+}
+
+} // namespace
+
+int bh_begin_root(bh_context *c, double *sums, uint64_t *rows)
+{
+    if (int r = step_ready(c)) return r;
+    if (c->fc.grower == BH_LEAFWISE) return fail(1, "the level plane needs a depthwise or levelwise fit");
+    if (!sums || !rows) return fail(1, "null output");
+    HIP_TRY(hipSetDevice(c->device));
+    c->step_mode = 0;
+    if (int r = step_root(c, sums, rows)) return r;
+    StepSeg root;
+    root.live = true;
+    root.src = c->samp_kind == 1 ? 1u : kSrcPlane0;
+    root.cnt = static_cast<u32>(*rows);
+    c->seg_cur.assign(1, root);
+    c->seg_next.clear();
+    c->step_hist = 0;
+    c->step_mode = 1;
+    return 0;
+}
+
+int bh_partition_level(bh_context *c, const bh_partition_op *ops, uint32_t n_ops, uint32_t *child_counts)
+{
+    if (!c || c->step_mode != 1) return fail(1, "no level-plane tree begun (bh_begin_root)");
+    if (n_ops > 0 && (!ops || !child_counts)) return fail(1, "null ops or counts");
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<StepPart> parts(n_ops);
+    u32 max_slot = 0;
+    for (u32 i = 0; i < n_ops; ++i) {
+        bh_partition_op const &o = ops[i];
+        if (o.parent_slot >= c->seg_cur.size() || !c->seg_cur[o.parent_slot].live)
+            return fail(1, "partition op names a slot with no rows at this level");
+        if (o.feature >= c->F) return fail(1, "partition op feature out of range");
+        if (o.left_slot == o.right_slot) return fail(1, "partition op children share a slot");
+        max_slot = std::max(max_slot, std::max(o.left_slot, o.right_slot));
+        parts[i] = StepPart{c->seg_cur[o.parent_slot], o.feature, o.bin, o.default_left ? 1u : 0u};
+    }
+    if (n_ops && max_slot >= std::max(c->max_front, 2u)) return fail(1, "child slot beyond the fit's frontier");
+    std::vector<u32> nl;
+    if (int r = step_partition(c, parts, nl)) return r;
+    c->seg_next.assign(n_ops ? max_slot + 1u : 0u, StepSeg{});
+    for (u32 i = 0; i < n_ops; ++i) {
+        bh_partition_op const &o = ops[i];
+        StepSeg const &p = c->seg_cur[o.parent_slot];
+        u32 const dst = p.src == 0 ? 1u : 0u;
+        u32 lr = 0, rr = 0;
+        route_split(c, p.route, o.feature, o.bin, o.default_left ? 1u : 0u, lr, rr);
+        StepSeg &L = c->seg_next[o.left_slot];
+        StepSeg &R = c->seg_next[o.right_slot];
+        if (L.live || R.live) return fail(1, "two partition ops name the same child slot");
+        L = StepSeg{p.off, nl[i], dst, lr, true};
+        R = StepSeg{p.off + nl[i], p.cnt - nl[i], dst, rr, true};
+        child_counts[2 * i] = nl[i];
+        child_counts[2 * i + 1] = p.cnt - nl[i];
+    }
+    return 0;
+}
+
+int bh_advance_level(bh_context *c, const bh_level_op *ops, uint32_t n_ops)
+{
+    if (!c || c->step_mode != 1) return fail(1, "no level-plane tree begun (bh_begin_root)");
+    if (n_ops > 0 && !ops) return fail(1, "null ops");
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<StepBuild> b(n_ops);
+    for (u32 i = 0; i < n_ops; ++i) {
+        bh_level_op const &o = ops[i];
+        if (o.parent_slot >= c->seg_cur.size() || o.small_slot >= c->seg_next.size() ||
+            o.large_slot >= c->seg_next.size() || !c->seg_next[o.small_slot].live)
+            return fail(1, "level op names a slot the partition did not make");
+        if (std::max(o.small_slot, o.large_slot) >= c->hist_slots)
+            return fail(1, "level op slot beyond the fit's histogram slots");
+        b[i] = StepBuild{o.small_slot, o.large_slot, o.parent_slot, c->seg_next[o.small_slot]};
+    }
+    void *cur = c->hist[c->step_hist], *nxt = c->hist[1u - c->step_hist];
+    if (!nxt) return fail(1, "the level plane needs two histogram buffers");
+    if (int r = step_build(c, b, cur, nxt)) return r;
+    c->seg_cur = std::move(c->seg_next);
+    c->seg_next.clear();
+    c->step_hist = 1u - c->step_hist;
+    return 0;
+}
+
+int bh_advance_layout_only(bh_context *c)
+{
+    if (!c || c->step_mode != 1) return fail(1, "no level-plane tree begun (bh_begin_root)");
+    c->seg_cur = std::move(c->seg_next);
+    c->seg_next.clear();
+    return 0;
+}
+
+int bh_find_splits_many(bh_context *c, bh_step_node *nodes, uint32_t n, bh_split *out)
+{
+    if (!c || c->step_mode != 1) return fail(1, "no level-plane tree begun (bh_begin_root)");
+    if (n > c->seg_cur.size()) return fail(1, "more nodes than slots at this level");
+    if (n > 0 && (!nodes || !out)) return fail(1, "null nodes or output");
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<NodeInfo> f(n);
+    for (u32 i = 0; i < n; ++i) {
+        if (!c->seg_cur[i].live) return fail(1, "a node's slot holds no rows at this level");
+        f[i] = step_node(c->seg_cur[i], i, nodes[i].depth);
+    }
+    return step_find(c, f, c->hist[c->step_hist], out, nodes);
+}
+
+int bh_find_level_split(bh_context *c, bh_step_node *nodes, uint32_t n, bh_split *out)
+{
+    if (!c || c->step_mode != 1) return fail(1, "no level-plane tree begun (bh_begin_root)");
+    if (c->fc.grower != BH_LEVELWISE) return fail(1, "the level split needs a levelwise fit");
+    if (n == 0 || n > c->seg_cur.size()) return fail(1, "node count out of range");
+    if (!nodes || !out) return fail(1, "null nodes or output");
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<NodeInfo> f(n);
+    for (u32 i = 0; i < n; ++i) {
+        if (!c->seg_cur[i].live) return fail(1, "a node's slot holds no rows at this level");
+        f[i] = step_node(c->seg_cur[i], i, nodes[i].depth);
+    }
+    if (int r = step_totals(c, f, c->hist[c->step_hist], nodes)) return r;
+    HIP_TRY(hipMemcpy(c->front[0], f.data(), n * sizeof(NodeInfo), hipMemcpyHostToDevice));
+    if (int r = step_ctl(c, n, 0, 0, 0, 0)) return r;
+    SplitRec *pick = c->splits + std::max(c->max_front * 2u, 4u) - 1u; // last record: the pick
+    if (n >= std::max(c->max_front * 2u, 4u)) return fail(1, "more nodes than the fit's frontier");
+    launch_level_find_step(c->ext, c->stream, c->quant, grow_view(c), c->front[0], n, c->hist[c->step_hist], pick);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    SplitRec p;
+    HIP_TRY(hipMemcpy(&p, pick, sizeof(p), hipMemcpyDeviceToHost));
+    std::vector<SplitRec> per(n);
+    if (p.valid) HIP_TRY(hipMemcpy(per.data(), c->splits, n * sizeof(SplitRec), hipMemcpyDeviceToHost));
+    for (u32 i = 0; i < n; ++i) {
+        SplitRec r = p.valid ? per[i] : p;
+        std::memcpy(&out[i], &r, sizeof(r));
+    }
+    return 0;
+}
+
+int bh_stamp_leaves(bh_context *c, const bh_leaf_stamp *stamps, uint32_t n)
+{
+    if (!c || c->step_mode == 0) return fail(1, "no tree begun");
+    if (n > 0 && !stamps) return fail(1, "null stamps");
+    std::vector<StepSeg> const &segs = c->step_mode == 1 ? c->seg_cur : c->pool;
+    for (u32 i = 0; i < n; ++i) {
+        if (stamps[i].slot >= segs.size() || !segs[stamps[i].slot].live)
+            return fail(1, "stamp names a slot with no rows");
+        c->route_stamp[segs[stamps[i].slot].route] = stamps[i].node_id;
+    }
+    return 0;
+}
+
+int bh_finalize_tree(bh_context *c, const float *node_values, uint32_t n_values, float *values,
+                     uint32_t *leaf_ids)
+{
+    if (!c || c->step_mode == 0) return fail(1, "no tree begun");
+    if (c->route.size() > c->max_nodes) return fail(1, "the routing tree exceeds the fit's node capacity");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMemcpy(c->tree, c->route.data(), c->route.size() * sizeof(DevNode), hipMemcpyHostToDevice));
+    TreeCtl t{};
+    t.n_nodes = static_cast<u32>(c->route.size());
+    HIP_TRY(hipMemcpy(c->ctl, &t, sizeof(t), hipMemcpyHostToDevice));
+    launch_score_update(c->stream, c->plane0, c->Fs, c->N, c->tree, c->ctl, c->d_nbins, c->F, 0.0F, nullptr,
+                        c->leaf_rows, c->max_nodes);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<u32> lr(c->N);
+    HIP_TRY(hipMemcpy(lr.data(), c->leaf_rows, c->N * sizeof(u32), hipMemcpyDeviceToHost));
+    for (u64 r = 0; r < c->N; ++r) {
+        u32 const id = c->route_stamp[lr[r]];
+        if (leaf_ids) leaf_ids[r] = id;
+        if (values) values[r] = node_values && id < n_values ? node_values[id] : 0.0F;
+    }
+    return 0;
+}
+
+int bh_leaf_begin_root(bh_context *c, double *sums, uint64_t *rows)
+{
+    if (int r = step_ready(c)) return r;
+    if (c->fc.grower != BH_LEAFWISE) return fail(1, "the leaf plane needs a leafwise fit");
+    if (!sums || !rows) return fail(1, "null output");
+    HIP_TRY(hipSetDevice(c->device));
+    c->step_mode = 0;
+    if (int r = step_root(c, sums, rows)) return r;
+    StepSeg root;
+    root.live = true;
+    root.src = c->samp_kind == 1 ? 1u : kSrcPlane0;
+    root.cnt = static_cast<u32>(*rows);
+    c->pool.assign(1, root);
+    c->step_next_slot = 1;
+    c->step_mode = 2;
+    return 0;
+}
+
+int bh_leaf_split(bh_context *c, uint32_t parent_slot, uint32_t feature, uint32_t bin, uint32_t default_left,
+                  bh_leaf_round *out)
+{
+    if (!c || c->step_mode != 2) return fail(1, "no leaf-plane tree begun (bh_leaf_begin_root)");
+    if (parent_slot >= c->pool.size() || !c->pool[parent_slot].live) return fail(1, "parent slot holds no rows");
+    if (feature >= c->F) return fail(1, "feature out of range");
+    if (!out) return fail(1, "null output");
+    HIP_TRY(hipSetDevice(c->device));
+    StepSeg const p = c->pool[parent_slot];
+    std::vector<u32> nl;
+    if (int r = step_partition(c, {StepPart{p, feature, bin, default_left ? 1u : 0u}}, nl)) return r;
+    u32 const dst = p.src == 0 ? 1u : 0u;
+    u32 const nr = p.cnt - nl[0];
+    *out = bh_leaf_round{0u, 0u, nl[0], nr};
+    if (nl[0] == 0 || nr == 0) {
+        // one side emptied: no slot taken, the parent keeps its (moved, same order) rows
+        c->pool[parent_slot].src = dst;
+        return 0;
+    }
+    // the smaller child (ties: the left) takes the next free slot, the larger inherits
+    // the parent's
+    bool const left_small = nl[0] <= nr;
+    u32 const small_slot = c->step_next_slot;
+    if (small_slot >= c->hist_slots) return fail(1, "leaf slot pool exhausted (max_leaves)");
+    c->step_next_slot += 1;
+    if (c->pool.size() <= small_slot) c->pool.resize(small_slot + 1u);
+    u32 lr = 0, rr = 0;
+    route_split(c, p.route, feature, bin, default_left ? 1u : 0u, lr, rr);
+    StepSeg const L{p.off, nl[0], dst, lr, true}, R{p.off + nl[0], nr, dst, rr, true};
+    u32 const ls = left_small ? small_slot : parent_slot, rs = left_small ? parent_slot : small_slot;
+    c->pool[ls] = L;
+    c->pool[rs] = R;
+    *out = bh_leaf_round{ls, rs, nl[0], nr};
+    return 0;
+}
+
+int bh_leaf_build(bh_context *c, uint32_t small_slot, uint32_t large_slot)
+{
+    if (!c || c->step_mode != 2) return fail(1, "no leaf-plane tree begun (bh_leaf_begin_root)");
+    if (small_slot >= c->pool.size() || large_slot >= c->pool.size() || !c->pool[small_slot].live ||
+        !c->pool[large_slot].live || small_slot == large_slot)
+        return fail(1, "leaf build names slots the split did not make");
+    HIP_TRY(hipSetDevice(c->device));
+    // large = parent - small in place: the large child inherited the parent's slot
+    return step_build(c, {StepBuild{small_slot, large_slot, large_slot, c->pool[small_slot]}}, c->hist[0],
+                      c->hist[0]);
+}
+
+int bh_leaf_find(bh_context *c, bh_step_node *nodes, const uint32_t *slots, uint32_t n, bh_split *out)
+{
+    if (!c || c->step_mode != 2) return fail(1, "no leaf-plane tree begun (bh_leaf_begin_root)");
+    if (n > 0 && (!nodes || !slots || !out)) return fail(1, "null nodes, slots or output");
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<NodeInfo> f(n);
+    for (u32 i = 0; i < n; ++i) {
+        if (slots[i] >= c->pool.size() || !c->pool[slots[i]].live) return fail(1, "slot holds no rows");
+        f[i] = step_node(c->pool[slots[i]], slots[i], nodes[i].depth);
+    }
+    return step_find(c, f, c->hist[0], out, nodes);
 }
 
 } // extern "C"

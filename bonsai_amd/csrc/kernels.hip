@@ -801,6 +801,114 @@ __global__ __launch_bounds__(512) void k_hist_q2(
     }
 }
 
+// Quantized histogram, rotated row-in-lane layout (rows of at most 64 bytes, one feature
+// slice, single 8192-row sub-units; same contract as k_hist_q otherwise).  Lane l owns
+// whole rows like k_hist_q3, but rotates which feature it adds at each atomic
+// instruction: its k-th dword load takes dword (k + l) mod DW of the row and byte
+// (j + l / DW) mod 4 of it, so the 64 lanes of one instruction spread over all 4 * DW
+// features instead of sharing one.  The LDS cells are feature-minor, cell (f, b) at
+// b * FS + f, so a cell's bank is a function of its feature alone: the lanes of an
+// instruction land on distinct bank pairs up to the ~2 lanes per feature, and same-cell
+// collisions (few-valued features) fall to those same ~2 lanes.  Multi-unit nodes write
+// packed partials in that (bin, feature) order (F cells per bin, coalesced); the
+// completion kernel reads them with the transposed index.  Single-unit nodes write
+// their slot directly.
+constexpr u32 hq4_fs(u32 F) { return F <= 16u ? 16u : (F <= 32u ? 32u : 64u); }
+
+template <u32 DW, u32 U>
+__global__ __launch_bounds__(kHqThreads) void k_hist_q4(
+    HistChunk const *__restrict__ chunks, TreeCtl const *__restrict__ ctl,
+    u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
+    void const *__restrict__ gw0, void const *__restrict__ gw1, void const *__restrict__ gwp,
+    u32 Fs, u32 F, u32 SB, void *__restrict__ partials, void *__restrict__ hist_next,
+    u32 *__restrict__ cnt_next, PartOp const *__restrict__ routes)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned long long *lh = reinterpret_cast<unsigned long long *>(smem); // SB x FS, feature-minor
+    constexpr u32 FS = hq4_fs(4u * DW);
+    u32 const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    u32 const NW = blockDim.x >> 6;
+    u32 const lcells = SB * FS;
+    u32 const n_units = ctl->n_hist_chunks;
+    size_t const node_cells = static_cast<size_t>(F) * SB;
+    // this lane's rotation: dword offset and byte offset
+    u32 const rot_d = lane % DW, rot_b = (lane / DW) & 3u;
+    for (u32 c = blockIdx.x; c < n_units; c += gridDim.x) {
+        HistChunk const ch = chunks[c];
+        for (u32 i = threadIdx.x; i < lcells; i += blockDim.x) lh[i] = 0ull;
+        __syncthreads();
+        u8 const *plane = ch.src == 0 ? pl0 : (ch.src == 1 ? pl1 : plp);
+        u32 const *ghv = static_cast<u32 const *>(ch.src == 0 ? gw0 : (ch.src == 1 ? gw1 : gwp));
+        bool const routed = ch.route != kNone;
+        PartOp rop{};
+        if (routed) rop = routes[ch.route];
+        bool const want_left = ch.side == 0;
+        u32 const rf_dw = rop.feature >> 2, rf_sh = 8u * (rop.feature & 3u);
+        for (u32 r0 = ch.begin + wave * (U * 64u); r0 < ch.end; r0 += NW * U * 64u) {
+            u32 dw[U][DW], v[U], rdw[U];
+            bool ok[U];
+#pragma unroll
+            for (u32 u = 0; u < U; ++u) {
+                u32 const row = r0 + u * 64u + lane;
+                ok[u] = row < ch.end;
+                u32 const *rp = reinterpret_cast<u32 const *>(plane + static_cast<size_t>(ok[u] ? row : ch.begin) * Fs);
+#pragma unroll
+                for (u32 k = 0; k < DW; ++k) {
+                    u32 m = rot_d + k;
+                    m = m >= DW ? m - DW : m;
+                    dw[u][k] = rp[m];
+                }
+                v[u] = ghv[ok[u] ? row : ch.begin];
+                rdw[u] = routed ? rp[rf_dw] : 0u;
+            }
+#pragma unroll
+            for (u32 u = 0; u < U; ++u) {
+                bool keep = ok[u];
+                if (routed)
+                    keep = keep && routes_left((rdw[u] >> rf_sh) & 0xFFu, rop.last_bin, rop.bin, rop.default_left) ==
+                                       want_left;
+                if (!keep) continue;
+                u64 const pk = pack_q(v[u]);
+#pragma unroll
+                for (u32 k = 0; k < DW; ++k) {
+                    u32 m = rot_d + k;
+                    m = m >= DW ? m - DW : m;
+#pragma unroll
+                    for (u32 j = 0; j < 4; ++j) {
+                        u32 const jj = (j + rot_b) & 3u;
+                        u32 const f = 4u * m + jj;
+                        if (f < F)
+                            atomicAdd(lh + ((dw[u][k] >> (8u * jj)) & 0xFFu) * FS + f,
+                                      static_cast<unsigned long long>(pk));
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        u32 const out_cells = F * SB;
+        if (ch.part != kNone) {
+            // packed partial, (bin, feature) order: coalesced stores, conflict-free LDS reads
+            u64 *P = static_cast<u64 *>(partials) + static_cast<size_t>(ch.part) * node_cells;
+            for (u32 i = threadIdx.x; i < out_cells; i += blockDim.x) {
+                u32 const b = i / F, f = i - b * F;
+                P[i] = lh[b * FS + f];
+            }
+        } else {
+            size_t const base = static_cast<size_t>(ch.slot) * node_cells;
+            i64 *dst = static_cast<i64 *>(hist_next) + base * 2;
+            for (u32 i = threadIdx.x; i < out_cells; i += blockDim.x) {
+                u32 const b = i / F, f = i - b * F;
+                u64 const v = lh[b * FS + f];
+                size_t const o = static_cast<size_t>(f) * SB + b;
+                dst[2 * o] = unpack_g(v);
+                dst[2 * o + 1] = unpack_h(v);
+                if (cnt_next) cnt_next[base + o] = unpack_cnt(v);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Fixed-order completion of built nodes (H6 reduce + H7 sibling subtraction,
 // src/step/primitives.hpp:156-173; device twin subtract_kernel, kernels.cuh:507-520):
 // small = p_0 + p_1 + ... in unit order, large = parent - small.
@@ -908,11 +1016,14 @@ constexpr u32 kFqGroups = 16;
 
 // advance != 0: the level ends with this launch, and workgroup (0, 0) also moves the
 // control block to the next frontier (no workgroup of the launch reads it).
+// tF != 0: the packed partials are in (bin, feature) order with tF features (k_hist_q4):
+// a workgroup's cells are consecutive in that order, and each cell's node-layout index
+// (feature * SB + bin) is used for the writes and the parent.
 __global__ __launch_bounds__(256) void k_finish_q(FinishOp const *__restrict__ ops, TreeCtl *__restrict__ ctl,
                                                   void const *__restrict__ partials, void const *__restrict__ hist_cur,
                                                   void *__restrict__ hist_next, u32 node_cells,
                                                   u32 const *__restrict__ cnt_cur, u32 *__restrict__ cnt_next,
-                                                  u32 advance)
+                                                  u32 advance, u32 tF)
 {
     __shared__ i64 s_g[kFqGroups][kFqCells], s_h[kFqGroups][kFqCells];
     __shared__ u32 s_n[kFqGroups][kFqCells];
@@ -927,12 +1038,17 @@ __global__ __launch_bounds__(256) void k_finish_q(FinishOp const *__restrict__ o
         FinishOp const op = ops[o];
         i64 *nx = reinterpret_cast<i64 *>(hist_next);
         for (u32 tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-            u32 const i = tile * kFqCells + j;
-            bool const in = i < node_cells;
+            u32 const pi_ = tile * kFqCells + j;          // cell index in the partials' order
+            bool const in = pi_ < node_cells;
+            u32 i = pi_;                                  // cell index in the node layout
+            if (tF) {
+                u32 const b = pi_ / tF, f = pi_ - b * tF;
+                i = f * (node_cells / tF) + b;
+            }
             i64 g = 0, h = 0;
             u32 n = 0;
             if (in && op.part_count > 0 && !op.wide) {
-                u64 const *pp = static_cast<u64 const *>(partials) + i;
+                u64 const *pp = static_cast<u64 const *>(partials) + pi_;
                 u32 p = grp;
                 for (; p + 3 * kFqGroups < op.part_count; p += 4 * kFqGroups) {
                     u64 v[4];
@@ -2638,6 +2754,11 @@ void launch_quant_from(hipStream_t s, float2 const *gh, u64 n, u32 *maxbits, u64
     (void)scales_dev;
 }
 
+u32 hist_transpose_F(bool q, u32 Fs, u32 F, bool group)
+{
+    return q && !group && Fs <= 64u && t_tune.hist_layout == 3u ? F : 0u;
+}
+
 hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl const *ctl, u8 const *pl0,
                        u8 const *pl1, u8 const *plp, void const *g0, void const *g1, void const *gp, u32 Fs, u32 F,
                        u32 SB, u32 fsl, u32 n_waves, u32 grid_x, void *partials, void *hist_next, u32 *cnt_next,
@@ -2649,7 +2770,23 @@ hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl c
     u32 const slices = (F + fsl - 1) / fsl;
     size_t const lds = static_cast<size_t>(fsl) * SB * 8;
     dim3 const grid(grid_x, slices), block(64 * n_waves);
-    if (q && Fs <= 64u && (routes || t_tune.hist_layout == 2u)) {
+    if (hist_transpose_F(q, Fs, F, group)) {
+        // rotated row-in-lane (single sub-units), every level of rows up to 64 bytes
+        u32 const Dw = Fs / 4u;
+        size_t const lds4 = static_cast<size_t>(hq4_fs(Fs)) * SB * 8u;
+#define BH_HQ4(DWV, UV)                                                                                          \
+    case DWV:                                                                                                     \
+        hipLaunchKernelGGL((k_hist_q4<DWV, UV>), dim3(grid_x), dim3(t_tune.hist_threads), lds4, s, chunks, ctl,  \
+                           pl0, pl1, plp, g0, g1, gp, Fs, F, SB, partials, hist_next, cnt_next, routes);          \
+        break;
+        switch (Dw) {
+            BH_HQ4(1, 4) BH_HQ4(2, 4) BH_HQ4(3, 4) BH_HQ4(4, 4) BH_HQ4(5, 4) BH_HQ4(6, 4) BH_HQ4(7, 4)
+            BH_HQ4(8, 3) BH_HQ4(9, 3) BH_HQ4(10, 2) BH_HQ4(11, 2) BH_HQ4(12, 2) BH_HQ4(13, 2) BH_HQ4(14, 2)
+            BH_HQ4(15, 2) BH_HQ4(16, 2)
+        default: break;
+        }
+#undef BH_HQ4
+    } else if (q && Fs <= 64u && (routes || t_tune.hist_layout == 2u)) {
         // row-in-lane: whole rows per lane, instantiated per row width in dwords.  Routed
         // levels always use it (a filtered row skips all its atomics at once).
         u32 const Dw = Fs / 4u;
@@ -2706,6 +2843,15 @@ hipError_t set_hist_lds_limits()
     e = hipDeviceGetAttribute(&max_shared, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (e != hipSuccess) return e;
     raise_lds(reinterpret_cast<void const *>(&k_hist_q), max_shared);
+    void const *q4[] = {reinterpret_cast<void const *>(&k_hist_q4<1, 4>),  reinterpret_cast<void const *>(&k_hist_q4<2, 4>),
+                        reinterpret_cast<void const *>(&k_hist_q4<3, 4>),  reinterpret_cast<void const *>(&k_hist_q4<4, 4>),
+                        reinterpret_cast<void const *>(&k_hist_q4<5, 4>),  reinterpret_cast<void const *>(&k_hist_q4<6, 4>),
+                        reinterpret_cast<void const *>(&k_hist_q4<7, 4>),  reinterpret_cast<void const *>(&k_hist_q4<8, 3>),
+                        reinterpret_cast<void const *>(&k_hist_q4<9, 3>),  reinterpret_cast<void const *>(&k_hist_q4<10, 2>),
+                        reinterpret_cast<void const *>(&k_hist_q4<11, 2>), reinterpret_cast<void const *>(&k_hist_q4<12, 2>),
+                        reinterpret_cast<void const *>(&k_hist_q4<13, 2>), reinterpret_cast<void const *>(&k_hist_q4<14, 2>),
+                        reinterpret_cast<void const *>(&k_hist_q4<15, 2>), reinterpret_cast<void const *>(&k_hist_q4<16, 2>)};
+    for (void const *fn : q4) raise_lds(fn, max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_hist_q2), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_hist<false, 4>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_find<true, false>), max_shared);
@@ -2719,7 +2865,7 @@ hipError_t set_hist_lds_limits()
 
 void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl *ctl, void const *partials,
                    void const *hist_cur, void *hist_next, u32 node_cells, u32 max_ops, u32 const *cnt_cur,
-                   u32 *cnt_next, bool advance)
+                   u32 *cnt_next, bool advance, u32 tF)
 {
     u32 const tiles = min(64u, (node_cells + 255) / 256);
     dim3 const grid(tiles, max(1u, min(max_ops, 1024u)));
@@ -2730,7 +2876,7 @@ void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl *ctl, voi
         u32 const cell_tiles = (node_cells + kFqCells - 1) / kFqCells;
         u32 const tx = max(1u, min(cell_tiles, max(1u, 4096u / ops_y)));
         hipLaunchKernelGGL(k_finish_q, dim3(tx, ops_y), dim3(256), 0, s, ops, ctl, partials, hist_cur, hist_next,
-                           node_cells, cnt_cur, cnt_next, advance ? 1u : 0u);
+                           node_cells, cnt_cur, cnt_next, advance ? 1u : 0u, tF);
     } else {
         hipLaunchKernelGGL(k_finish<false>, grid, dim3(256), 0, s, ops, ctl, partials, hist_cur, hist_next,
                            node_cells, nullptr, nullptr);

@@ -24,10 +24,13 @@ hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl c
                        u32 SB, u32 fsl, u32 n_waves, u32 grid_x, void *partials, void *hist_next,
                        u32 *cnt_next = nullptr, PartOp const *routes = nullptr, bool group = false);
 hipError_t set_hist_lds_limits();
+// F when launch_hist with these arguments runs the rotated row-in-lane kernel, whose unit
+// partials are in (bin, feature) order (launch_finish's tF), else 0.
+u32 hist_transpose_F(bool q, u32 Fs, u32 F, bool group = false);
 // advance: the level ends with this completion; the frontier moves on with it.
 void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl *ctl, void const *partials,
                    void const *hist_cur, void *hist_next, u32 node_cells, u32 max_ops,
-                   u32 const *cnt_cur = nullptr, u32 *cnt_next = nullptr, bool advance = false);
+                   u32 const *cnt_cur = nullptr, u32 *cnt_next = nullptr, bool advance = false, u32 tF = 0);
 void launch_find(hipStream_t s, bool q, NodeInfo *front, TreeCtl const *ctl, void const *hist, u16 const *n_bins,
                  u32 F, u32 SB, GrowCfg const &cfg, double const *scales, SplitRec *out, u32 max_front,
                  u32 front_base, u32 n_front_override, FindScratch const &fs, u32 const *cnt = nullptr);

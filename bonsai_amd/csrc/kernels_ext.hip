@@ -1115,7 +1115,8 @@ void enqueue_levelwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
     hipLaunchKernelGGL(k_obl_begin, dim3(1), dim3(64), 0, st, v.ctl);
     launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
                 v.fsl, v.waves, std::max(1u, std::min(v.max_hch, 4096u)), v.partials, v.hist[0]);
-    launch_finish(st, q, v.fops, v.ctl, v.partials, v.hist[1], v.hist[0], v.F * v.SB, 1);
+    launch_finish(st, q, v.fops, v.ctl, v.partials, v.hist[1], v.hist[0], v.F * v.SB, 1, nullptr, nullptr, false,
+                  hist_transpose_F(q, v.Fs, v.F));
     u32 const TN = q ? kLevelTileQ : kLevelTileF32;
     size_t const lds_level = static_cast<size_t>(TN) * v.SB * (q ? 16 : 8);
     for (u32 d = 0; d < D; ++d) {
@@ -1154,7 +1155,8 @@ void enqueue_levelwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
                                 gh_bytes, std::max(1u, std::min(v.max_pch, tuning().part_grid)));
             launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F,
                         v.SB, v.fsl, v.waves, std::max(1u, std::min(v.max_hch, 4096u)), v.partials, hnext);
-            launch_finish(st, q, v.fops, v.ctl, v.partials, hcur, hnext, v.F * v.SB, std::min(width, v.max_ops));
+            launch_finish(st, q, v.fops, v.ctl, v.partials, hcur, hnext, v.F * v.SB, std::min(width, v.max_ops),
+                          nullptr, nullptr, false, hist_transpose_F(q, v.Fs, v.F));
         }
         hipLaunchKernelGGL(k_next_level_obl, dim3(1), dim3(256), 0, st, v.ctl, cur, nxt);
         if (v.host_sync) (void)hipStreamSynchronize(st);
@@ -1165,6 +1167,37 @@ void enqueue_levelwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
     else
         hipLaunchKernelGGL(k_oblivious_tree<false>, dim3(1), dim3(1024), 0, st, v.front[D & 1u], v.ctl,
                            s.level_splits, s.level_gains, v.tree, v.gc, v.scales);
+}
+
+// find_level_split as one step over a frontier the host laid out (front[0 .. n) with
+// ctl->n_front = n): node totals, the per-feature level scan, the pick (pick[0], valid
+// = 0 when no split is feasible), and every node's child sums under the pick in
+// v.splits[i] (k_plan_oblivious's last-level arm; its partition ops are not used).
+void launch_level_find_step(ExtState &s, hipStream_t st, bool q, GrowView const &v, NodeInfo *front, u32 n,
+                            void const *hist, SplitRec *pick)
+{
+    u32 const TN = q ? kLevelTileQ : kLevelTileF32;
+    size_t const lds_level = static_cast<size_t>(TN) * v.SB * (q ? 16 : 8);
+    hipLaunchKernelGGL(k_obl_begin, dim3(1), dim3(64), 0, st, v.ctl);
+    if (q) {
+        hipLaunchKernelGGL(k_totals<true>, dim3(n), dim3(64), 0, st, front, v.ctl, hist, v.d_nbins, v.F, v.SB,
+                           v.gc.fmask);
+        hipLaunchKernelGGL(k_find_level<true>, dim3(v.F), dim3(256), lds_level, st, front, v.ctl, hist, v.d_nbins,
+                           v.F, v.SB, v.gc, v.scales, s.feat_best);
+    } else {
+        hipLaunchKernelGGL(k_totals<false>, dim3(n), dim3(64), 0, st, front, v.ctl, hist, v.d_nbins, v.F, v.SB,
+                           v.gc.fmask);
+        hipLaunchKernelGGL(k_find_level<false>, dim3(v.F), dim3(256), lds_level, st, front, v.ctl, hist, v.d_nbins,
+                           v.F, v.SB, v.gc, v.scales, s.feat_best);
+    }
+    hipLaunchKernelGGL(k_level_pick, dim3(1), dim3(256), 0, st, s.feat_best, v.F, v.ctl, s.level_splits, s.level_gains,
+                       pick);
+    if (q)
+        hipLaunchKernelGGL(k_plan_oblivious<true>, dim3(1), dim3(1024), 0, st, front, v.ctl, pick, hist, v.d_nbins, v.F,
+                           v.SB, v.splits, v.ops, v.pch, 0u, 1u, part_chunk_rows(v.Fs));
+    else
+        hipLaunchKernelGGL(k_plan_oblivious<false>, dim3(1), dim3(1024), 0, st, front, v.ctl, pick, hist, v.d_nbins,
+                           v.F, v.SB, v.splits, v.ops, v.pch, 0u, 1u, part_chunk_rows(v.Fs));
 }
 
 // LeafwiseGrower::grow as a launch sequence: the root, then max_leaves - 1 rounds
@@ -1198,7 +1231,8 @@ void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
                        s.hist_slots, v.root, v.lw_units);
     launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
                 v.fsl, v.waves, hgrid, v.partials, pool, cnt);
-    launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1, cnt, cnt);
+    launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1, cnt, cnt, false,
+                      hist_transpose_F(q, v.Fs, v.F));
     launch_find(st, q, kids, v.ctl, pool, v.d_nbins, v.F, v.SB, v.gc, v.scales, v.splits, 2, 0, 0, v.fscr, cnt);
     hipLaunchKernelGGL(k_lw_push_root, dim3(1), dim3(64), 0, st, lw, kids, v.splits, s.cands, s.free_slots, v.tree,
                        v.gc, v.scales);
@@ -1228,7 +1262,8 @@ void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
         }
         launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
                     v.fsl, v.waves, hgrid, v.partials, pool, cnt);
-        launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1, cnt, cnt);
+        launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1, cnt, cnt, false,
+                      hist_transpose_F(q, v.Fs, v.F));
         LwNext const nx{s.cands, s.free_slots, parent, psplit, v.tree, v.ops, v.pch, v.hch, v.fops, v.lw_units,
                         part_chunk_rows(v.Fs), r + 1 < rounds ? 1u : 0u};
         if (q) launch_lw_find<true, true>(st, v, lw, cnt, nx);

@@ -62,6 +62,10 @@ void ext_free(ExtState &s);
 hipError_t ext_set_limits();
 void enqueue_levelwise(ExtState &s, hipStream_t st, bool q, GrowView const &v);
 void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v);
+// The level finder as one step over a host-laid frontier (front[0 .. n), ctl->n_front =
+// n): pick[0] = the level's split, v.splits[i] = node i's child sums under it.
+void launch_level_find_step(ExtState &s, hipStream_t st, bool q, GrowView const &v, NodeInfo *front, u32 n,
+                            void const *hist, SplitRec *pick);
 
 } // namespace bh
 

@@ -257,6 +257,74 @@ int bh_profile_hist(bh_context *ctx, uint32_t n_iters, double *ms_total, double 
 int bh_set_timing(bh_context *ctx, int32_t enable);
 int bh_get_timing(bh_context *ctx, float *ms, uint32_t n);
 
+/* ---- engine steps: GPULevelEngine / GPULeafEngine (include/bonsai/grower.hpp:91-162;
+ *      CudaHistogramEngine, include/bonsai/cuda/histogram_engine.hpp:90-218) ----
+ * For growers that keep the control loop on the host (LevelStep / LeafStep,
+ * src/step/level.hpp:159-366, src/step/leaf.hpp:124-267), one host round trip per step.
+ * They grow from the current gradients (bh_set_gradients, or the last bh_train
+ * iteration's) over every row or the row subset (bh_set_row_subset); a feature mask
+ * (bh_set_feature_mask) narrows the finders.  Rows never cross the bus: partitions move
+ * each parent's segment into the other work plane (left rows ascending, then right rows,
+ * at the parent's offsets) and histograms stay in slot-indexed device buffers.  The
+ * device feature draw, the Bernoulli draw and monotone / interaction constraints are
+ * refused (the host grower owns them in this mode). */
+typedef struct bh_level_op {      /* LevelOp (histogram_engine.hpp:103-108) */
+    uint32_t parent_slot, small_slot, large_slot;
+} bh_level_op;
+typedef struct bh_partition_op {  /* PartitionOp (histogram_engine.hpp:112-120) */
+    uint32_t parent_slot, left_slot, right_slot, feature, bin, default_left;
+} bh_partition_op;
+typedef struct bh_leaf_stamp {    /* LeafStamp (histogram_engine.hpp:122-126) */
+    uint32_t slot, node_id;
+} bh_leaf_stamp;
+typedef struct bh_leaf_round {    /* LeafRound (histogram_engine.hpp:212-218) */
+    uint32_t left_slot, right_slot, left_count, right_count;
+} bh_leaf_round;
+/* A node handed to a finder (the SplitInput fields the device path reads, split.hpp:
+ * 15-53): depth in; the node's rows and totals out (Histogram::totals of its cells, the
+ * first selected feature's: FP32 double sums rounded to float; quantized exact integers,
+ * isum_*, times the scales, sum_*). */
+typedef struct bh_step_node {
+    uint32_t depth, _pad;
+    uint64_t rows;
+    double   sum_g, sum_h;
+    int64_t  isum_g, isum_h;
+} bh_step_node;
+
+/* begin_root (histogram_engine.hpp:144-151): the root's histogram into slot 0, its
+ * totals (as bh_step_node's) and rows out.  Level plane: depthwise or levelwise fits. */
+int bh_begin_root(bh_context *ctx, double *sums, uint64_t *rows);
+/* partition_level (:153-158): child_counts[2i], [2i+1] = (n_left, n_right) of ops[i]. */
+int bh_partition_level(bh_context *ctx, const bh_partition_op *ops, uint32_t n_ops, uint32_t *child_counts);
+/* advance_level (:159-160): small children built from their rows, large = parent - small;
+ * the child level becomes current. */
+int bh_advance_level(bh_context *ctx, const bh_level_op *ops, uint32_t n_ops);
+/* advance_layout_only (:161-163): the child level becomes current without histograms. */
+int bh_advance_layout_only(bh_context *ctx);
+/* find_splits_many (:169-176): node i is slot i of the current level. */
+int bh_find_splits_many(bh_context *ctx, bh_step_node *nodes, uint32_t n, bh_split *out);
+/* find_level_split (:177-183), levelwise fits: one split for the whole frontier; out[i]
+ * carries it with node i's child sums under it (valid = 0: no feasible split). */
+int bh_find_level_split(bh_context *ctx, bh_step_node *nodes, uint32_t n, bh_split *out);
+/* stamp_leaves / leaf_stamp (:150-152, :239-242): the rows of each named slot (current
+ * level, or the leaf pool) end in node node_id. */
+int bh_stamp_leaves(bh_context *ctx, const bh_leaf_stamp *stamps, uint32_t n);
+/* finalize_tree (:164-168): every row's stamped node id (0xFFFFFFFF: an unstamped leaf)
+ * and node_values[id] (0 beyond n_values); either output may be NULL.  Rows outside the
+ * row subset are routed through the recorded splits too. */
+int bh_finalize_tree(bh_context *ctx, const float *node_values, uint32_t n_values, float *values,
+                     uint32_t *leaf_ids);
+/* leaf_begin_root (:220-226), leafwise fits: the root into pool slot 0. */
+int bh_leaf_begin_root(bh_context *ctx, double *sums, uint64_t *rows);
+/* leaf_split (:227-228): both counts nonzero: the smaller child (ties: left) takes the
+ * next free slot and the larger inherits the parent's; otherwise no slot is taken. */
+int bh_leaf_split(bh_context *ctx, uint32_t parent_slot, uint32_t feature, uint32_t bin, uint32_t default_left,
+                  bh_leaf_round *out);
+/* leaf_build (:229-233): small built from its rows, large = parent - small in place. */
+int bh_leaf_build(bh_context *ctx, uint32_t small_slot, uint32_t large_slot);
+/* leaf_find (:234-238): nodes[i]'s histogram is pool slot slots[i]. */
+int bh_leaf_find(bh_context *ctx, bh_step_node *nodes, const uint32_t *slots, uint32_t n, bh_split *out);
+
 /* ---- launch-shape tuning (no counterpart in the reference, whose launch shapes are
  *      fixed constants, src/cuda/detail/device_context.cu:451-497) ---- */
 typedef struct bh_tuning {
@@ -265,7 +333,9 @@ typedef struct bh_tuning {
     uint32_t hist_threads;      /* quantized histogram workgroup size: 256, 512 or 1024 */
     uint32_t part_grid;         /* partition workgroups */
     uint32_t hist_layout;       /* quantized histogram: 0 row-dword lanes (default), 1 feature-in-lane (rows <= 32
-                                   bytes), 2 row-in-lane (rows <= 64 bytes); levels whose units filter their
+                                   bytes), 2 row-in-lane (rows <= 64 bytes), 3 rotated row-in-lane with
+                                   feature-minor LDS cells (rows <= 64 bytes, single-sub-unit levels; every
+                                   level, routed ones included); with 0-2, levels whose units filter their
                                    parent's rows use 2 whenever rows are <= 64 bytes */
     uint32_t pair_levels;       /* quantized depthwise, rows <= 64 bytes: 1 = move rows every second level
                                    (four-way partitions; the levels between scan parent segments) */

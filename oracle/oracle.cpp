@@ -648,29 +648,93 @@ struct Engine {
         }
     }
 
-    // Row blocks of 16384 with per-block partials folded in block order: the result
-    // depends on the block size, not on scheduling (src/fill/rows.hpp:199-344 pattern).
+    // The reference's row fill over the row-major mirror (src/fill/rows.hpp:199-344):
+    // 1024-row chunks (k_reduce_grain, :348) split into contiguous per-thread ranges;
+    // thread t sums its range in row order into its own partial (the first thread straight
+    // into the node), and the partials are added in thread order.  The result depends on
+    // the thread count, not on scheduling.
     void fill_f32_omp(u32 const *rows, u64 n, float *out) const
     {
-        u64 const B = 16384;
-        u64 const nb = (n + B - 1) / B;
-        if (nb <= 1) {
+        u64 const G = 1024;
+        u64 const n_chunks = (n + G - 1) / G;
+        u64 const T = std::min<u64>(static_cast<u64>(std::max(1, n_threads)), n_chunks);
+        if (T <= 1) {
             fill_f32_serial(ds, gr, rows, n, out);
             return;
         }
         size_t const C = cells();
-        std::vector<float> part(nb * C, 0.0F);
-#pragma omp parallel for schedule(dynamic, 1) num_threads(n_threads)
-        for (long b = 0; b < static_cast<long>(nb); ++b) {
-            u64 const k0 = static_cast<u64>(b) * B;
-            u64 const k1 = std::min(n, k0 + B);
-            fill_f32_serial(ds, gr, rows + k0, k1 - k0, part.data() + static_cast<u64>(b) * C);
+        std::vector<float> part((T - 1) * C, 0.0F);
+#pragma omp parallel for schedule(static, 1) num_threads(static_cast<int>(T))
+        for (long t = 0; t < static_cast<long>(T); ++t) {
+            u64 const k0 = static_cast<u64>(t) * n_chunks / T * G;
+            u64 const k1 = std::min(n, (static_cast<u64>(t) + 1) * n_chunks / T * G);
+            float *dst = t == 0 ? out : part.data() + static_cast<u64>(t - 1) * C;
+            fill_f32_serial(ds, gr, rows + k0, k1 - k0, dst);
         }
 #pragma omp parallel for schedule(static) num_threads(n_threads)
         for (long i = 0; i < static_cast<long>(C); ++i) {
-            float acc = 0.0F;
-            for (u64 b = 0; b < nb; ++b) acc += part[b * C + static_cast<u64>(i)];
+            float acc = out[i];
+            for (u64 t = 1; t < T; ++t) acc += part[(t - 1) * C + static_cast<u64>(i)];
             out[i] = acc;
+        }
+    }
+
+    // One level's fill of several nodes at once (CpuHistogramEngine::populate_many,
+    // src/grower.cpp:93-139, and plan_reduce / run_fill_reduce, src/fill/rows.hpp:199-344):
+    // node-major 1024-row chunks split into contiguous per-thread ranges, so a thread
+    // touches few nodes; the first thread on a node sums straight into it, later threads
+    // into partial slots added in thread order.
+    void fill_many_omp(std::vector<NodeState *> const &nodes_in) const
+    {
+        u64 const G = 1024;
+        size_t const C = cells();
+        struct Chunk { u32 node; u64 k0, k1; };
+        std::vector<Chunk> chunks;
+        for (u32 i = 0; i < nodes_in.size(); ++i) {
+            NodeState &nd = *nodes_in[i];
+            nd.hf.assign(C, 0.0F);
+            u64 const n = nd.rows.size();
+            for (u64 k = 0; k < n; k += G) chunks.push_back({i, k, std::min(n, k + G)});
+        }
+        u64 const nch = chunks.size();
+        u64 const T = std::max<u64>(1, std::min<u64>(static_cast<u64>(std::max(1, n_threads)), nch));
+        auto first_chunk = [&](u64 t) { return t * nch / T; };
+        // per node: the first and last thread that touch it, and its partial slots
+        std::vector<long> first_t(nodes_in.size(), -1), last_t(nodes_in.size(), -1);
+        for (u64 t = 0; t < T; ++t)
+            for (u64 c = first_chunk(t); c < first_chunk(t + 1); ++c) {
+                u32 const i = chunks[c].node;
+                if (first_t[i] < 0) first_t[i] = static_cast<long>(t);
+                last_t[i] = static_cast<long>(t);
+            }
+        std::vector<u64> slot0(nodes_in.size(), 0);
+        u64 n_slots = 0;
+        for (size_t i = 0; i < nodes_in.size(); ++i) {
+            slot0[i] = n_slots;
+            if (first_t[i] >= 0) n_slots += static_cast<u64>(last_t[i] - first_t[i]);
+        }
+        std::vector<float> part(n_slots * C, 0.0F);
+#pragma omp parallel for schedule(static, 1) num_threads(static_cast<int>(T))
+        for (long t = 0; t < static_cast<long>(T); ++t) {
+            for (u64 c = first_chunk(static_cast<u64>(t)); c < first_chunk(static_cast<u64>(t) + 1); ++c) {
+                Chunk const &ch = chunks[c];
+                NodeState &nd = *nodes_in[ch.node];
+                float *dst = t == first_t[ch.node]
+                                 ? nd.hf.data()
+                                 : part.data() + (slot0[ch.node] + static_cast<u64>(t - first_t[ch.node] - 1)) * C;
+                fill_f32_serial(ds, gr, nd.rows.data() + ch.k0, ch.k1 - ch.k0, dst);
+            }
+        }
+#pragma omp parallel for schedule(dynamic, 1) num_threads(n_threads)
+        for (long i = 0; i < static_cast<long>(nodes_in.size()); ++i) {
+            NodeState &nd = *nodes_in[i];
+            u64 const extra = first_t[i] < 0 ? 0 : static_cast<u64>(last_t[i] - first_t[i]);
+            for (u64 k = 0; k < extra; ++k) {
+                float const *src = part.data() + (slot0[i] + k) * C;
+                for (size_t j = 0; j < C; ++j) nd.hf[j] += src[j];
+            }
+            u32 const f0 = totals_feature(ds.n_features);
+            totals_f32(nd.hf.data() + static_cast<size_t>(f0) * ds.bin_stride * 2, ds.n_bins[f0], &nd.tg, &nd.th);
         }
     }
 
@@ -886,7 +950,9 @@ std::vector<orc_node> grow_depthwise(Engine const &e, std::vector<u32> *leaf_of_
             auto const [lid, rid] = commit_split(nodes, current[i].id, splits[i]);
             pend.push_back({i, {}, {}, lid, rid});
         }
-        for (auto &p : pend) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(e.n_threads) if (pend.size() > 1)
+        for (long k = 0; k < static_cast<long>(pend.size()); ++k) {
+            auto &p = pend[static_cast<size_t>(k)];
             NodeState &par = current[p.parent];
             e.partition(par.rows, splits[p.parent], p.L.rows, p.R.rows);
             p.L.id = p.lid;
@@ -910,6 +976,18 @@ std::vector<orc_node> grow_depthwise(Engine const &e, std::vector<u32> *leaf_of_
         }
         // build_children: smaller child filled (ties -> left), larger by subtraction.
         bool const last = depth + 1 >= e.cfg.max_depth;
+        if (e.mode == ORC_HIST_F32_OMP && !kept.empty()) {
+            // the level's smaller children filled together, then the subtractions
+            std::vector<NodeState *> smalls;
+            for (auto &p : kept) smalls.push_back(p.L.rows.size() <= p.R.rows.size() ? &p.L : &p.R);
+            e.fill_many_omp(smalls);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(e.n_threads)
+            for (long k = 0; k < static_cast<long>(kept.size()); ++k) {
+                auto &p = kept[static_cast<size_t>(k)];
+                bool const left_small = p.L.rows.size() <= p.R.rows.size();
+                e.subtract_into(left_small ? p.R : p.L, left_small ? p.L : p.R, current[p.parent]);
+            }
+        }
         for (auto &p : kept) {
             NodeState &par = current[p.parent];
             bool const left_small = p.L.rows.size() <= p.R.rows.size();
@@ -917,7 +995,7 @@ std::vector<orc_node> grow_depthwise(Engine const &e, std::vector<u32> *leaf_of_
             NodeState &large = left_small ? p.R : p.L;
             if (last && e.leaves_from_split()) {
                 e.totals_from_split(p.L, p.R, splits[p.parent]);
-            } else {
+            } else if (e.mode != ORC_HIST_F32_OMP) {
                 e.fill(small);
                 e.subtract_into(large, small, par);
             }
@@ -936,8 +1014,10 @@ std::vector<orc_node> grow_depthwise(Engine const &e, std::vector<u32> *leaf_of_
     }
     if (leaf_of_row) {
         leaf_of_row->assign(e.ds.n_rows, 0);
-        for (auto const &n : leaves)
-            for (u32 r : n.rows) (*leaf_of_row)[r] = n.id;
+        u32 *lor = leaf_of_row->data();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(e.n_threads)
+        for (long k = 0; k < static_cast<long>(leaves.size()); ++k)
+            for (u32 r : leaves[static_cast<size_t>(k)].rows) lor[r] = leaves[static_cast<size_t>(k)].id;
     }
     return nodes;
 }

@@ -46,6 +46,13 @@ SHAPES = [
          hist_group=3),
     dict(scatter_lds_bytes=20480, find_blocks=2, hist_threads=256, part_grid=2048, hist_layout=0, pair_levels=1,
          hist_group=1),
+    # rotated row-in-lane histogram with transposed unit partials, paired and unpaired
+    dict(scatter_lds_bytes=32768, find_blocks=4, hist_threads=512, part_grid=4096, hist_layout=3, pair_levels=1,
+         hist_group=1),
+    dict(scatter_lds_bytes=32768, find_blocks=4, hist_threads=1024, part_grid=4096, hist_layout=3, pair_levels=0,
+         hist_group=1),
+    dict(scatter_lds_bytes=32768, find_blocks=4, hist_threads=256, part_grid=4096, hist_layout=3, pair_levels=1,
+         hist_group=2),
 ]
 
 
@@ -81,7 +88,7 @@ def test_tuning_validation(hip, tall):
     with pytest.raises(hip.ConfigError):
         ctx.set_tuning(pair_levels=2)
     with pytest.raises(hip.ConfigError):
-        ctx.set_tuning(hist_layout=3)
+        ctx.set_tuning(hist_layout=4)
     with pytest.raises(hip.ConfigError):
         ctx.set_tuning(hist_group=5)
     ctx.close()
