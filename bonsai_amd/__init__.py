@@ -76,7 +76,8 @@ class ServeConfigC(C.Structure):
 class ServeStatsC(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("rows_per_s", C.c_double), ("p50_ms", C.c_double),
                 ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("rows", C.c_uint64),
-                ("requests", C.c_uint32), ("max_inflight", C.c_uint32)]
+                ("requests", C.c_uint32), ("max_inflight", C.c_uint32),
+                ("max_device_concurrency", C.c_uint32), ("_pad", C.c_uint32), ("device_overlap_ms", C.c_double)]
 
 
 NODE_DTYPE = np.dtype([
@@ -182,6 +183,7 @@ def lib() -> C.CDLL:
         "bh_begin_root": [P, P, C.POINTER(u64)],
         "bh_partition_level": [P, P, u32, P],
         "bh_advance_level": [P, P, u32],
+        "bh_advance_level_per_node": [P, P, u32],
         "bh_advance_layout_only": [P],
         "bh_find_splits_many": [P, P, u32, P],
         "bh_find_level_split": [P, P, u32, P],
@@ -577,10 +579,12 @@ class HipContext:
         _check(lib().bh_partition_level(self._h, _ptr(a), len(a), _ptr(counts)))
         return counts.reshape(-1, 2)
 
-    def advance_level(self, ops) -> None:
-        """ops: (parent_slot, small_slot, large_slot) tuples."""
+    def advance_level(self, ops, per_node: bool = False) -> None:
+        """ops: (parent_slot, small_slot, large_slot) tuples; per_node launches one build
+        and one subtraction per node (the reference's cadence) instead of one per level."""
         a = np.ascontiguousarray(np.array(ops, np.uint32).reshape(-1, 3))
-        _check(lib().bh_advance_level(self._h, _ptr(a), len(a)))
+        fn = lib().bh_advance_level_per_node if per_node else lib().bh_advance_level
+        _check(fn(self._h, _ptr(a), len(a)))
 
     def advance_layout_only(self) -> None:
         _check(lib().bh_advance_layout_only(self._h))

@@ -10,7 +10,9 @@
    histogram's error against an FP64 histogram of the unquantized gradients.
 3  1,000,000 x 2,000 u8 bins (64 values; 90% sparse features), regression, hip_levelwise
    depth 10: K iterations, one hipGraph per iteration against eager launches and against
-   a host round-trip after every level (the reference's cadence).
+   a host round-trip after every level; and one tree grown three ways (the device grower,
+   the host-driven engine steps with one histogram launch per level, and the steps with
+   one launch per node, the reference's cadence).
 4  Serving: 1,000 random depth-8 trees over 28 features (leaf values N(0, 0.01^2)),
    20,000 predict + TreeSHAP requests, lognormal sizes (median 256, sigma 1, clipped to
    [1, 65536]), Poisson arrivals at a fixed rate, admitted under an HBM budget:
@@ -183,8 +185,54 @@ def config3(args) -> dict:
             res.setdefault(mode, {})["quantized" if quantized else "fp32"] = {
                 "ms_per_iter": round(dt / args.iters * 1e3, 3),
                 "fit_s_300_extrapolated": round(dt / args.iters * 300, 3)}
+    # One tree from the same gradients three ways: the device-resident grower (every
+    # node of a level in one launch, no host round trip), the engine steps driven from
+    # the host with one launch per level, and the steps with one build and one
+    # subtraction launch per node (the reference's cadence).
+    g = (np.full(n, init, np.float32) - y).astype(np.float32)
+    h = np.ones(n, np.float32)
+    trees = {}
+    for quantized in (True, False):
+        ctx.configure(B.FitConfig(grower="hip_levelwise", objective="mse", quantized=quantized, max_depth=10,
+                                  learning_rate=0.1, lambda_l2=1.0))
+        ctx.set_gradients(g, h)
+        out = {}
+        for name, fn in (("device_grower", lambda: ctx.grow_tree()),
+                         ("steps_one_launch_per_level", lambda: levelwise_steps_tree(ctx, 10, False)),
+                         ("steps_one_launch_per_node", lambda: levelwise_steps_tree(ctx, 10, True))):
+            fn()
+            t = time.perf_counter()
+            reps = 3
+            for _ in range(reps):
+                fn()
+            out[name] = round((time.perf_counter() - t) / reps * 1e3, 3)
+        trees["quantized" if quantized else "fp32"] = out
+    res["one_tree_ms"] = trees
     ctx.close()
     return res
+
+
+def levelwise_steps_tree(ctx, depth: int, per_node: bool) -> None:
+    """An oblivious tree through the engine steps (find_level_split, partition_level,
+    advance_level), one histogram launch per level or per node."""
+    ctx.begin_root()
+    n = 1
+    for d in range(depth):
+        splits, _ = ctx.find_level_split([d] * n)
+        if not splits[0]["valid"]:
+            break
+        s0 = splits[0]
+        counts = ctx.partition_level([(i, 2 * i, 2 * i + 1, s0["feature"], s0["bin"], s0["default_left"])
+                                      for i in range(n)])
+        if d + 1 == depth:
+            ctx.advance_layout_only()
+        else:
+            ctx.advance_level([(i, 2 * i + (0 if counts[i][0] <= counts[i][1] else 1),
+                                2 * i + (1 if counts[i][0] <= counts[i][1] else 0)) for i in range(n)],
+                              per_node=per_node)
+        n *= 2
+    ctx.stamp_leaves([(i, i) for i in range(n)])
+    ctx.finalize_tree(np.zeros(n, np.float32))
 
 
 def random_ensemble(n_trees: int, depth: int, n_features: int, n_bins: np.ndarray, seed: int = 11):

@@ -2145,6 +2145,82 @@ int bh_advance_level(bh_context *c, const bh_level_op *ops, uint32_t n_ops)
     return 0;
 }
 
+int bh_advance_level_per_node(bh_context *c, const bh_level_op *ops, uint32_t n_ops)
+{
+    if (!c || c->step_mode != 1) return fail(1, "no level-plane tree begun (bh_begin_root)");
+    if (n_ops > 0 && !ops) return fail(1, "null ops");
+    HIP_TRY(hipSetDevice(c->device));
+    void *cur = c->hist[c->step_hist], *nxt = c->hist[1u - c->step_hist];
+    if (!nxt) return fail(1, "the level plane needs two histogram buffers");
+    // Every node's units and completion op uploaded once; then one histogram launch and
+    // one completion launch per node, each with its own unit count in the control block.
+    std::vector<HistChunk> hc;
+    std::vector<FinishOp> fo;
+    std::vector<u32> first(n_ops + 1, 0);
+    u32 parts = 0;
+    for (u32 i = 0; i < n_ops; ++i) {
+        bh_level_op const &o = ops[i];
+        if (o.parent_slot >= c->seg_cur.size() || o.small_slot >= c->seg_next.size() ||
+            o.large_slot >= c->seg_next.size() || !c->seg_next[o.small_slot].live)
+            return fail(1, "level op names a slot the partition did not make");
+        if (std::max(o.small_slot, o.large_slot) >= c->hist_slots)
+            return fail(1, "level op slot beyond the fit's histogram slots");
+        StepSeg const &g = c->seg_next[o.small_slot];
+        u32 const units = (g.cnt + kHistChunk - 1) / kHistChunk;
+        FinishOp f{};
+        f.small_slot = o.small_slot;
+        f.large_slot = o.large_slot;
+        f.parent_slot = o.parent_slot;
+        f.part_first = parts;
+        f.part_count = units > 1 ? units : 0;
+        f.empty = g.cnt == 0 ? 1u : 0u;
+        first[i] = static_cast<u32>(hc.size());
+        for (u32 k = 0; k < units; ++k) {
+            HistChunk h{};
+            h.slot = o.small_slot;
+            h.src = g.src;
+            h.begin = g.off + k * kHistChunk;
+            h.end = g.off + std::min(g.cnt, (k + 1) * kHistChunk);
+            h.part = units > 1 ? parts + k : kNone;
+            hc.push_back(h);
+        }
+        if (units > 1) parts += units;
+        fo.push_back(f);
+    }
+    first[n_ops] = static_cast<u32>(hc.size());
+    if (hc.size() > c->max_hch || parts > c->max_parts || fo.size() > c->max_ops + 2)
+        return fail(1, "engine step: more histogram units than the configured fit holds");
+    if (!hc.empty()) HIP_TRY(hipMemcpy(c->hch, hc.data(), hc.size() * sizeof(HistChunk), hipMemcpyHostToDevice));
+    if (!fo.empty()) HIP_TRY(hipMemcpy(c->fops, fo.data(), fo.size() * sizeof(FinishOp), hipMemcpyHostToDevice));
+    TreeCtl *hctl = nullptr;
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&hctl), std::max<u32>(1u, n_ops) * sizeof(TreeCtl), 0));
+    u32 const tF = hist_transpose_F(c->quant, c->Fs, c->F);
+    hipError_t e = hipSuccess;
+    for (u32 i = 0; i < n_ops && e == hipSuccess; ++i) {
+        hctl[i] = TreeCtl{};
+        hctl[i].n_hist_chunks = first[i + 1] - first[i];
+        hctl[i].n_finish = 1;
+        e = hipMemcpyAsync(c->ctl, &hctl[i], sizeof(TreeCtl), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = launch_hist(c->stream, c->quant, c->hch + first[i], c->ctl, c->work[0], c->work[1], c->plane0,
+                            gh_src(c, 0), gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(),
+                            c->hist_waves(), std::max(1u, std::min(first[i + 1] - first[i], 4096u)), c->partials,
+                            nxt);
+        if (e == hipSuccess)
+            launch_finish(c->stream, c->quant, c->fops + i, c->ctl, c->partials, cur, nxt, c->node_cells(), 1,
+                          nullptr, nullptr, false, tF);
+    }
+    hipError_t const s2 = hipStreamSynchronize(c->stream);
+    (void)hipHostFree(hctl);
+    if (e != hipSuccess) return fail(-1, std::string("hip: per-node build: ") + hipGetErrorString(e));
+    if (s2 != hipSuccess) return fail(-1, std::string("hip: per-node build: ") + hipGetErrorString(s2));
+    HIP_TRY(hipGetLastError());
+    c->seg_cur = std::move(c->seg_next);
+    c->seg_next.clear();
+    c->step_hist = 1u - c->step_hist;
+    return 0;
+}
+
 int bh_advance_layout_only(bh_context *c)
 {
     if (!c || c->step_mode != 1) return fail(1, "no level-plane tree begun (bh_begin_root)");

@@ -589,6 +589,7 @@ namespace {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
+    hipEvent_t k0 = nullptr, k1 = nullptr; // the batch's kernels: first start, last end
     u8 *h_rows = nullptr, *d_rows = nullptr;
     float *h_pred = nullptr, *d_pred = nullptr;
     double *h_shap = nullptr, *d_shap = nullptr;
@@ -620,6 +621,8 @@ extern "C" int bh_serve(bh_predictor *p, const uint8_t *rows, uint64_t row_strid
     for (auto &s : slots) {
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreate(&s.k0);
+        if (e == hipSuccess) e = hipEventCreate(&s.k1);
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&s.h_rows), cap * row_stride, 0);
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&s.h_pred), cap * sizeof(float), 0);
         if (e == hipSuccess) e = hipMalloc(&s.d_rows, cap * row_stride);
@@ -639,6 +642,8 @@ extern "C" int bh_serve(bh_predictor *p, const uint8_t *rows, uint64_t row_strid
             if (s.d_pred) (void)hipFree(s.d_pred);
             if (s.d_shap) (void)hipFree(s.d_shap);
             if (s.done) (void)hipEventDestroy(s.done);
+            if (s.k0) (void)hipEventDestroy(s.k0);
+            if (s.k1) (void)hipEventDestroy(s.k1);
             if (s.stream) (void)hipStreamDestroy(s.stream);
         }
     };
@@ -646,6 +651,19 @@ extern "C" int bh_serve(bh_predictor *p, const uint8_t *rows, uint64_t row_strid
         cleanup();
         return sfail(-1, std::string("hip: serve buffers: ") + hipGetErrorString(e));
     }
+    // Device-time intervals of every batch's kernels, from a base event on the first
+    // slot's stream: they show whether batches of different slots ran on the device at
+    // the same time.
+    hipEvent_t base = nullptr;
+    e = hipEventCreate(&base);
+    if (e == hipSuccess) e = hipEventRecord(base, slots[0].stream);
+    if (e == hipSuccess) e = hipEventSynchronize(base);
+    if (e != hipSuccess) {
+        if (base) (void)hipEventDestroy(base);
+        cleanup();
+        return sfail(-1, std::string("hip: serve events: ") + hipGetErrorString(e));
+    }
+    std::vector<std::pair<double, double>> kspans;
     using clk = std::chrono::steady_clock;
     auto const t0 = clk::now();
     auto now_s = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
@@ -664,6 +682,10 @@ extern "C" int bh_serve(bh_predictor *p, const uint8_t *rows, uint64_t row_strid
                 break;
             }
             double const t_done = now_s();
+            float a_ms = 0.0F, b_ms = 0.0F;
+            if (hipEventElapsedTime(&a_ms, base, s.k0) == hipSuccess &&
+                hipEventElapsedTime(&b_ms, base, s.k1) == hipSuccess)
+                kspans.emplace_back(a_ms, b_ms);
             u64 pos = 0;
             for (u32 r : s.reqs) {
                 u64 const a = req_offsets[r], nr = req_offsets[r + 1] - a;
@@ -706,11 +728,13 @@ extern "C" int bh_serve(bh_predictor *p, const uint8_t *rows, uint64_t row_strid
             max_inflight = std::max(max_inflight, inflight);
             total_rows += rows_in;
             e = hipMemcpyAsync(s.d_rows, s.h_rows, rows_in * row_stride, hipMemcpyHostToDevice, s.stream);
+            if (e == hipSuccess) e = hipEventRecord(s.k0, s.stream);
             if (e == hipSuccess) {
                 rc = launch_predict(p, s.d_rows, rows_in, row_stride, s.d_pred, s.stream);
                 if (rc == 0 && shap) rc = launch_shap(p, s.d_rows, rows_in, row_stride, s.d_shap, s.stream);
             }
             if (rc) break;
+            if (e == hipSuccess) e = hipEventRecord(s.k1, s.stream);
             if (e == hipSuccess) e = hipMemcpyAsync(s.h_pred, s.d_pred, rows_in * sizeof(float), hipMemcpyDeviceToHost, s.stream);
             if (e == hipSuccess && shap)
                 e = hipMemcpyAsync(s.h_shap, s.d_shap, rows_in * (F + 1) * sizeof(double), hipMemcpyDeviceToHost, s.stream);
@@ -726,7 +750,26 @@ extern "C" int bh_serve(bh_predictor *p, const uint8_t *rows, uint64_t row_strid
     }
     double const wall = now_s();
     cleanup();
+    (void)hipEventDestroy(base);
     if (rc) return rc;
+    // Sweep the kernel spans: time with two or more batches on the device, and the most
+    // at once.
+    std::vector<std::pair<double, int>> ev;
+    for (auto const &k : kspans) {
+        ev.emplace_back(k.first, +1);
+        ev.emplace_back(k.second, -1);
+    }
+    std::sort(ev.begin(), ev.end(), [](auto const &x, auto const &y) {
+        return x.first < y.first || (x.first == y.first && x.second < y.second);
+    });
+    int live = 0, peak = 0;
+    double overlap = 0.0, prev = 0.0;
+    for (auto const &v : ev) {
+        if (live >= 2) overlap += v.first - prev;
+        live += v.second;
+        peak = std::max(peak, live);
+        prev = v.first;
+    }
     std::vector<double> lat(latency_ms, latency_ms + n_req);
     std::sort(lat.begin(), lat.end());
     auto pct = [&](double q) {
@@ -744,5 +787,7 @@ extern "C" int bh_serve(bh_predictor *p, const uint8_t *rows, uint64_t row_strid
     stats->p99_ms = pct(0.99);
     stats->mean_ms = n_req ? mean / n_req : 0.0;
     stats->max_inflight = max_inflight;
+    stats->device_overlap_ms = overlap;
+    stats->max_device_concurrency = static_cast<uint32_t>(peak);
     return 0;
 }

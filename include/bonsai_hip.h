@@ -299,6 +299,10 @@ int bh_partition_level(bh_context *ctx, const bh_partition_op *ops, uint32_t n_o
 /* advance_level (:159-160): small children built from their rows, large = parent - small;
  * the child level becomes current. */
 int bh_advance_level(bh_context *ctx, const bh_level_op *ops, uint32_t n_ops);
+/* advance_level with the reference's launch cadence (one histogram build and one
+ * subtraction launch per node, src/cuda/detail/device_context.cu:913-988), for measuring
+ * against the one-launch-per-level build; same results. */
+int bh_advance_level_per_node(bh_context *ctx, const bh_level_op *ops, uint32_t n_ops);
 /* advance_layout_only (:161-163): the child level becomes current without histograms. */
 int bh_advance_layout_only(bh_context *ctx);
 /* find_splits_many (:169-176): node i is slot i of the current level. */
@@ -389,7 +393,10 @@ typedef struct bh_serve_stats {
     double   p50_ms, p99_ms, mean_ms;
     uint64_t rows;
     uint32_t requests;
-    uint32_t max_inflight;
+    uint32_t max_inflight;           /* requests admitted and not yet completed, at most */
+    uint32_t max_device_concurrency; /* batches whose kernels ran on the device at once, at most */
+    uint32_t _pad;
+    double   device_overlap_ms;      /* device time with two or more batches' kernels running */
 } bh_serve_stats;
 
 /* Serve n_req requests whose rows are rows[req_offsets[i] .. req_offsets[i+1]) and

@@ -157,7 +157,7 @@ def test_depthwise_steps_on_a_row_subset(hip, orc, tall):
     assert np.array_equal(ids[rows], rlor)
 
 
-def steps_levelwise(ctx, orc, c, tot):
+def steps_levelwise(ctx, orc, c, tot, per_node=False):
     ctx.begin_root()
     cur = [0]
     levels, gains = [], []
@@ -178,7 +178,8 @@ def steps_levelwise(ctx, orc, c, tot):
             ctx.advance_layout_only()
         else:
             ctx.advance_level([(i, 2 * i + (0 if counts[i][0] <= counts[i][1] else 1),
-                                2 * i + (1 if counts[i][0] <= counts[i][1] else 0)) for i in range(len(cur))])
+                                2 * i + (1 if counts[i][0] <= counts[i][1] else 0)) for i in range(len(cur))],
+                              per_node=per_node)
         cur = list(range(2 * len(cur)))
     if tots is None:   # the loop ran out of depth after a find: totals of the frontier
         _, info = ctx.find_splits_many([c.max_depth] * len(cur))
@@ -189,8 +190,10 @@ def steps_levelwise(ctx, orc, c, tot):
     return np.array(levels, np.int64).reshape(-1, 3), leaves, ids
 
 
+@pytest.mark.parametrize("per_node", [False, True])
 @pytest.mark.parametrize("quantized", [True, False])
-def test_levelwise_steps_match_the_oracle(hip, orc, tall, quantized):
+def test_levelwise_steps_match_the_oracle(hip, orc, tall, quantized, per_node):
+    """per_node: one build and one subtraction launch per node (the reference's cadence)."""
     bins, nb, X, y = tall
     g, h = grad_of(orc, y, seed=71)
     depth = 5
@@ -203,7 +206,7 @@ def test_levelwise_steps_match_the_oracle(hip, orc, tall, quantized):
         ctx.set_gradients(g, h)
         sc = (0.0, 0.0)
         ref = orc.grow_levelwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_CHUNKED)
-    levels, leaves, ids = steps_levelwise(ctx, orc, c, Totals(quantized, sc))
+    levels, leaves, ids = steps_levelwise(ctx, orc, c, Totals(quantized, sc), per_node)
     ctx.close()
     assert np.array_equal(levels, ref[0].astype(np.int64))
     assert np.array_equal(leaves.view(np.uint32), ref[1].view(np.uint32))

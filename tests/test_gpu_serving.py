@@ -118,3 +118,28 @@ def test_serve_results_identical_alone_and_in_the_mix(hip, orc, model):
         assert np.array_equal(alone.view(np.uint32), pred[a:b].view(np.uint32))
         sa = p.shap(rows[a:b])
         assert np.array_equal(sa.view(np.uint64), shap[a:b].view(np.uint64))
+
+
+def test_serve_overlaps_batches_on_the_device(hip, orc, model):
+    """Requests that arrive together are admitted into every free slot (several in
+    flight at once) and their kernels run on the device at the same time: the device-time
+    spans of batches on different streams intersect.  Results stay identical to the
+    requests served alone."""
+    trees, init, held, nb = model
+    p = hip.Predictor(trees, nb, 0.1, init)
+    rng = np.random.default_rng(12)
+    n_req, size = 16, 6000
+    idx = rng.integers(0, held.shape[0], n_req * size)
+    rows = held[idx]
+    offs = (np.arange(n_req + 1) * size).astype(np.uint64)
+    arrival = np.zeros(n_req)                      # a burst: everything arrives at once
+    pred, shap, lat, stats = p.serve(rows, offs, arrival, n_slots=4, do_shap=True, max_batch_rows=size,
+                                     hbm_budget=1 << 30)
+    assert stats["max_inflight"] >= 2
+    assert stats["max_device_concurrency"] >= 2
+    assert stats["device_overlap_ms"] > 0.0
+    for r in (0, 7, n_req - 1):
+        a, b = int(offs[r]), int(offs[r + 1])
+        assert np.array_equal(p.predict(rows[a:b]).view(np.uint32), pred[a:b].view(np.uint32))
+        assert np.array_equal(p.shap(rows[a:b]).view(np.uint64), shap[a:b].view(np.uint64))
+    p.close()
