@@ -11,9 +11,10 @@ carries each iteration's gradient maxima to the next.
 The logistic objective goes through expf on the device and std::exp in the oracle,
 which may differ in the last bit, so its replay reads each iteration's device gradients
 back (bh_get_gradients) and checks them against the oracle's within that rounding:
-float gradients within 2e-7 absolute (a few ulps of values in [-1, 1]); quantized
-maxima within one float ulp, at least 99.99% of the int8 / int16 codes equal and every
-other within one quantum.  The oracle's grower then takes the device's own gradients,
+float gradients within four ulps of max(1, |h|) (Poisson's h = exp(s) scales its
+terms); quantized maxima within one float ulp, at least 99.99% of the int8 g codes and
+99.9% of the int16 h codes equal (a last-bit difference crosses a floor boundary with
+probability about qmax * 2^-24), every other within one quantum.  The oracle's grower then takes the device's own gradients,
 and its tree must equal the device's bit for bit; the scores follow the host replay of
 s + lr * leaf exactly.
 """
@@ -76,14 +77,17 @@ def test_quantized_iterations_match_the_host_replay(hip, orc, tall, calls):
 
 
 def assert_codes_agree(gq, hq, sc, ogq, ohq, osc):
+    """A last-bit difference of exp moves x * qmax / max across a floor boundary with
+    probability about qmax * 2^-24: negligible for the int8 g (127 levels), up to ~0.1%
+    of rows for the int16 h (32767 levels).  Every code is within one quantum."""
     for k, qmax in ((0, 127.0), (1, 32767.0)):
         md = np.array([sc[k] * qmax], np.float32).view(np.int32)[0]
         mo = np.array([osc[k] * qmax], np.float32).view(np.int32)[0]
         assert abs(int(md) - int(mo)) <= 1, (k, sc, osc)
-    for a, b in ((gq, ogq), (hq, ohq)):
+    for a, b, frac in ((gq, ogq, 0.9999), (hq, ohq, 0.999)):
         d = np.abs(a.astype(np.int32) - b.astype(np.int32))
         assert d.max() <= 1
-        assert (d == 0).mean() >= 0.9999
+        assert (d == 0).mean() >= frac, (d == 0).mean()
 
 
 def oracle_tree(orc, grower, bins, nb, depth, leaves, min_data=20, **grad):
@@ -155,9 +159,9 @@ def replay_fit(hip, orc, bins, nb, y, grower, depth, leaves, quantized, iters, o
                 if exact:
                     assert np.array_equal(gd, g) and np.array_equal(hd, h)
                 else:
-                    # a few ulps of the exp-derived terms (|h| scales them) and of the weights
+                    # four ulps of the exp-derived terms (|h| scales them), times the weights
                     w = np.ones_like(h) if weights is None else np.maximum(weights, 1.0)
-                    tol = 2.5e-7 * np.maximum(1.0, np.abs(h)) * w
+                    tol = 4.0 * 2.0 ** -23 * np.maximum(1.0, np.abs(h)) * w
                     assert np.all(np.abs(gd - g) <= tol) and np.all(np.abs(hd - h) <= tol)
                 ref = oracle_tree(orc, grower, bins, nb, depth, leaves, min_data, g=gd, h=hd)
             assert_same_grown(grower, tree, ref)
