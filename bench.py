@@ -213,8 +213,16 @@ def main() -> int:
     copy_ms = B.copy_calibrate(1 << 30, 20)
     copy_gbps = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
     log(f"calibration copy of 1 GiB: {copy_ms * 1e3:.1f} us, {copy_gbps:.0f} GB/s")
+    # streaming ceilings: the best grid for a wide copy and a wide read of 1 GiB
+    wide = {}
+    for mode, name, traffic_x in ((1, "copy", 2), (2, "read", 1)):
+        best = max((traffic_x * (1 << 30) / (B.stream_calibrate(1 << 30, mode, g, 10) * 1e-3) / 1e9, g)
+                   for g in (1024, 2048, 4096, 8192, 16384))
+        wide[name] = best
+        log(f"wide {name} of 1 GiB: {best[0]:.0f} GB/s at {best[1]} workgroups")
     if a.copy_calibration:
-        print(json.dumps({"copy_bytes": 1 << 30, "copy_ms": copy_ms, "copy_GBps": round(copy_gbps, 1)}))
+        print(json.dumps({"copy_bytes": 1 << 30, "copy_ms": copy_ms, "copy_GBps": round(copy_gbps, 1),
+                          "wide_copy_GBps": round(wide["copy"][0], 1), "wide_read_GBps": round(wide["read"][0], 1)}))
         return 0
 
     X, y, Xh, yh, cuts = make_data(a.rows, a.held_out)
@@ -329,6 +337,8 @@ def main() -> int:
             "scanned_GBps": round(scanned_gbps, 1),
             "scanned_bytes_per_launch": round(prof["scanned_bytes"] / prof["launches"]),
             "copy_GBps": round(copy_gbps, 1),
+            "wide_copy_GBps": round(wide["copy"][0], 1),
+            "wide_read_GBps": round(wide["read"][0], 1),
             "traffic": traffic,
             "traffic_over_algorithmic": traffic_ratio,
             "traffic_source": "profiles/hist_quantized_16Mx28_counters.md" if traffic else None,

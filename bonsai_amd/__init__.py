@@ -152,6 +152,7 @@ def lib() -> C.CDLL:
         "bh_device_count": [C.POINTER(C.c_int)],
         "bh_synchronize": [P],
         "bh_copy_calibrate": [C.c_int, u64, u32, C.POINTER(C.c_float)],
+        "bh_stream_calibrate": [C.c_int, u64, u32, u32, u32, C.POINTER(C.c_float)],
         "bh_create": [C.c_int, C.POINTER(P)],
         "bh_destroy": [P],
         "bh_upload_bins": [P, P, u64, u32, u64, P],
@@ -237,6 +238,14 @@ def copy_calibrate(nbytes: int = 1 << 30, reps: int = 20, device: int = 0) -> fl
     """Mean time (ms) of a plain device copy of nbytes; bandwidth = 2 * nbytes / time."""
     ms = C.c_float(0)
     _check(lib().bh_copy_calibrate(device, nbytes, reps, C.byref(ms)))
+    return ms.value
+
+
+def stream_calibrate(nbytes: int, mode: int, grid: int, reps: int = 20, device: int = 0) -> float:
+    """Mean time (ms) of a bandwidth probe over nbytes: mode 1 a wide copy (traffic
+    2 * nbytes), mode 2 a wide read (traffic nbytes), over grid workgroups."""
+    ms = C.c_float(0)
+    _check(lib().bh_stream_calibrate(device, nbytes, reps, mode, grid, C.byref(ms)))
     return ms.value
 
 

@@ -745,10 +745,12 @@ extern "C" {
 
 const char *bh_last_error(void) { return g_err.c_str(); }
 
-int bh_copy_calibrate(int device, uint64_t bytes, uint32_t reps, float *ms_mean)
+static int stream_calibrate(int device, uint64_t bytes, uint32_t reps, uint32_t mode, uint32_t grid, float *ms_mean)
 {
     if (bytes < 16 || bytes % 16 != 0) return fail(1, "bytes must be a positive multiple of 16");
     if (reps == 0) return fail(1, "reps must be positive");
+    if (mode > 2) return fail(1, "mode must be 0 (plain copy), 1 (wide copy) or 2 (wide read)");
+    if (mode != 0 && (grid == 0 || grid > 65536)) return fail(1, "grid must be in [1, 65536]");
     HIP_TRY(hipSetDevice(device));
     void *src = nullptr, *dst = nullptr;
     hipStream_t st = nullptr;
@@ -760,9 +762,9 @@ int bh_copy_calibrate(int device, uint64_t bytes, uint32_t reps, float *ms_mean)
     if (e == hipSuccess) e = hipEventCreate(&e1);
     if (e == hipSuccess) e = hipMemsetAsync(src, 1, bytes, st);
     if (e == hipSuccess) {
-        launch_copy_calibrate(st, src, dst, bytes);   // warm
+        launch_copy_calibrate(st, src, dst, bytes, mode, grid);   // warm
         (void)hipEventRecord(e0, st);
-        for (u32 r = 0; r < reps; ++r) launch_copy_calibrate(st, src, dst, bytes);
+        for (u32 r = 0; r < reps; ++r) launch_copy_calibrate(st, src, dst, bytes, mode, grid);
         (void)hipEventRecord(e1, st);
         e = hipEventSynchronize(e1);
     }
@@ -776,6 +778,16 @@ int bh_copy_calibrate(int device, uint64_t bytes, uint32_t reps, float *ms_mean)
     if (e != hipSuccess) return fail(-1, std::string("hip: ") + hipGetErrorString(e));
     *ms_mean = ms / static_cast<float>(reps);
     return 0;
+}
+
+int bh_copy_calibrate(int device, uint64_t bytes, uint32_t reps, float *ms_mean)
+{
+    return stream_calibrate(device, bytes, reps, 0u, 0u, ms_mean);
+}
+
+int bh_stream_calibrate(int device, uint64_t bytes, uint32_t reps, uint32_t mode, uint32_t grid, float *ms_mean)
+{
+    return stream_calibrate(device, bytes, reps, mode, grid, ms_mean);
 }
 
 int bh_synchronize(bh_context *c)
@@ -1776,6 +1788,16 @@ int bh_time_root_histogram(bh_context *c, uint32_t reps, float *ms_mean)
 // =================================================================================
 namespace {
 
+// Host <-> device copies of the step path, ordered on the engine stream and complete on
+// return.  The engine stream is non-blocking, so a plain hipMemcpy (null stream) would
+// neither wait for the step's kernels before reading their output nor keep an upload
+// from landing while a kernel still reads the old contents.
+hipError_t step_copy(bh_context *c, void *dst, void const *src, size_t bytes, hipMemcpyKind kind)
+{
+    hipError_t const e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
+    return e == hipSuccess ? hipStreamSynchronize(c->stream) : e;
+}
+
 // The control block fields the step launches read.
 int step_ctl(bh_context *c, u32 n_front, u32 n_ops, u32 n_part_chunks, u32 n_hist_chunks, u32 n_finish)
 {
@@ -1785,7 +1807,7 @@ int step_ctl(bh_context *c, u32 n_front, u32 n_ops, u32 n_part_chunks, u32 n_his
     t.n_part_chunks = n_part_chunks;
     t.n_hist_chunks = n_hist_chunks;
     t.n_finish = n_finish;
-    HIP_TRY(hipMemcpy(c->ctl, &t, sizeof(t), hipMemcpyHostToDevice));
+    HIP_TRY(step_copy(c, c->ctl, &t, sizeof(t), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -1804,7 +1826,7 @@ u32 step_totals_feature(bh_context *c)
 {
     if (c->fsamp_kind != 1 || !c->fmask) return 0;
     std::vector<u8> m(c->F);
-    if (hipMemcpy(m.data(), c->fmask, c->F, hipMemcpyDeviceToHost) != hipSuccess) return 0;
+    if (step_copy(c, m.data(), c->fmask, c->F, hipMemcpyDeviceToHost) != hipSuccess) return 0;
     for (u32 f = 0; f < c->F; ++f)
         if (m[f]) return f;
     return 0;
@@ -1844,8 +1866,8 @@ int step_build(bh_context *c, std::vector<StepBuild> const &builds, void *hist_c
     }
     if (hc.size() > c->max_hch || parts > c->max_parts || fo.size() > c->max_ops + 2)
         return fail(1, "engine step: more histogram units than the configured fit holds");
-    if (!hc.empty()) HIP_TRY(hipMemcpy(c->hch, hc.data(), hc.size() * sizeof(HistChunk), hipMemcpyHostToDevice));
-    if (!fo.empty()) HIP_TRY(hipMemcpy(c->fops, fo.data(), fo.size() * sizeof(FinishOp), hipMemcpyHostToDevice));
+    if (!hc.empty()) HIP_TRY(step_copy(c, c->hch, hc.data(), hc.size() * sizeof(HistChunk), hipMemcpyHostToDevice));
+    if (!fo.empty()) HIP_TRY(step_copy(c, c->fops, fo.data(), fo.size() * sizeof(FinishOp), hipMemcpyHostToDevice));
     if (int r = step_ctl(c, 0, 0, 0, static_cast<u32>(hc.size()), static_cast<u32>(fo.size()))) return r;
     HIP_TRY(launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
                         gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(),
@@ -1892,21 +1914,22 @@ int step_partition(bh_context *c, std::vector<StepPart> const &parts, std::vecto
         return fail(1, "engine step: more partition work than the configured fit holds");
     n_left.assign(parts.size(), 0u);
     if (pch.empty()) return 0;
-    HIP_TRY(hipMemcpy(c->ops, ops.data(), ops.size() * sizeof(PartOp), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->pch, pch.data(), pch.size() * sizeof(PartChunk), hipMemcpyHostToDevice));
+    HIP_TRY(step_copy(c, c->ops, ops.data(), ops.size() * sizeof(PartOp), hipMemcpyHostToDevice));
+    HIP_TRY(step_copy(c, c->pch, pch.data(), pch.size() * sizeof(PartChunk), hipMemcpyHostToDevice));
     if (int r = step_ctl(c, 0, static_cast<u32>(ops.size()), static_cast<u32>(pch.size()), 0, 0)) return r;
     launch_part_count(c->stream, c->pch, c->ops, c->ctl, c->work[0], c->work[1], c->plane0, c->Fs, c->chunk_left,
                       part_grid(c));
     std::vector<u32> cl(pch.size());
-    HIP_TRY(hipMemcpy(cl.data(), c->chunk_left, cl.size() * sizeof(u32), hipMemcpyDeviceToHost));
+    HIP_TRY(step_copy(c, cl.data(), c->chunk_left, cl.size() * sizeof(u32), hipMemcpyDeviceToHost));
     for (size_t k = 0; k < pch.size(); ++k) {
+        if (cl[k] > pch[k].end - pch[k].begin) return fail(-2, "engine step: partition count beyond its chunk");
         PartOp &o = ops[pch[k].op];
         pch[k].left_before = o.n_left;
         o.n_left += cl[k];
     }
     for (size_t i = 0; i < ops.size(); ++i) n_left[i] = ops[i].n_left;
-    HIP_TRY(hipMemcpy(c->ops, ops.data(), ops.size() * sizeof(PartOp), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->pch, pch.data(), pch.size() * sizeof(PartChunk), hipMemcpyHostToDevice));
+    HIP_TRY(step_copy(c, c->ops, ops.data(), ops.size() * sizeof(PartOp), hipMemcpyHostToDevice));
+    HIP_TRY(step_copy(c, c->pch, pch.data(), pch.size() * sizeof(PartChunk), hipMemcpyHostToDevice));
     u32 const gh_bytes = c->quant ? 4 : 8;
     launch_part_scatter(c->stream, c->pch, c->ops, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
                         gh_src(c, 1), gh_src(c, 2), c->work[0], c->work[1], const_cast<void *>(gh_src(c, 0)),
@@ -1924,14 +1947,14 @@ int step_totals(bh_context *c, std::vector<NodeInfo> const &front, void const *h
     u32 const f0 = step_totals_feature(c);
     u32 const nb = c->n_bins[f0];
     double sc[2] = {0.0, 0.0};
-    if (c->quant) HIP_TRY(hipMemcpy(sc, c->scales, sizeof(sc), hipMemcpyDeviceToHost));
+    if (c->quant) HIP_TRY(step_copy(c, sc, c->scales, sizeof(sc), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < front.size(); ++i) {
         size_t const off = static_cast<size_t>(front[i].slot) * c->node_cells() + static_cast<size_t>(f0) * c->SB;
         bh_step_node &o = nodes[i];
         o.rows = front[i].cnt;
         if (c->quant) {
             std::vector<i64> cells(2 * nb);
-            HIP_TRY(hipMemcpy(cells.data(), static_cast<i64 const *>(hist) + 2 * off, cells.size() * sizeof(i64),
+            HIP_TRY(step_copy(c, cells.data(), static_cast<i64 const *>(hist) + 2 * off, cells.size() * sizeof(i64),
                               hipMemcpyDeviceToHost));
             i64 g = 0, h = 0;
             for (u32 b = 0; b < nb; ++b) {
@@ -1944,7 +1967,7 @@ int step_totals(bh_context *c, std::vector<NodeInfo> const &front, void const *h
             o.sum_h = static_cast<double>(h) * sc[1];
         } else {
             std::vector<float> cells(2 * nb);
-            HIP_TRY(hipMemcpy(cells.data(), static_cast<float const *>(hist) + 2 * off,
+            HIP_TRY(step_copy(c, cells.data(), static_cast<float const *>(hist) + 2 * off,
                               cells.size() * sizeof(float), hipMemcpyDeviceToHost));
             double g = 0.0, h = 0.0;
             for (u32 b = 0; b < nb; ++b) {
@@ -1968,14 +1991,14 @@ int step_find(bh_context *c, std::vector<NodeInfo> const &front, void const *his
     if (n == 0) return 0;
     if (int r = step_totals(c, front, hist, nodes)) return r;
     if (n > std::max(c->max_front * 2u, 4u)) return fail(1, "engine step: more nodes than the fit's frontier");
-    HIP_TRY(hipMemcpy(c->front[0], front.data(), n * sizeof(NodeInfo), hipMemcpyHostToDevice));
+    HIP_TRY(step_copy(c, c->front[0], front.data(), n * sizeof(NodeInfo), hipMemcpyHostToDevice));
     if (int r = step_ctl(c, n, 0, 0, 0, 0)) return r;
     launch_find(c->stream, c->quant, c->front[0], c->ctl, hist, c->d_nbins, c->F, c->SB, c->gc, c->scales, c->splits,
                 n, 0, n, c->fscr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     static_assert(sizeof(SplitRec) == sizeof(bh_split), "bh_split layout");
-    HIP_TRY(hipMemcpy(out, c->splits, n * sizeof(SplitRec), hipMemcpyDeviceToHost));
+    HIP_TRY(step_copy(c, out, c->splits, n * sizeof(SplitRec), hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -2004,9 +2027,9 @@ int step_root(bh_context *c, double *sums, uint64_t *rows)
     if (c->quant) {
         std::vector<i64> cells(2 * nb);
         double sc[2];
-        HIP_TRY(hipMemcpy(cells.data(), static_cast<i64 const *>(c->hist[0]) + 2 * off, cells.size() * sizeof(i64),
+        HIP_TRY(step_copy(c, cells.data(), static_cast<i64 const *>(c->hist[0]) + 2 * off, cells.size() * sizeof(i64),
                           hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(sc, c->scales, sizeof(sc), hipMemcpyDeviceToHost));
+        HIP_TRY(step_copy(c, sc, c->scales, sizeof(sc), hipMemcpyDeviceToHost));
         i64 ig = 0, ih = 0;
         for (u32 b = 0; b < nb; ++b) {
             ig += cells[2 * b];
@@ -2016,7 +2039,7 @@ int step_root(bh_context *c, double *sums, uint64_t *rows)
         h = static_cast<double>(ih) * sc[1];
     } else {
         std::vector<float> cells(2 * nb);
-        HIP_TRY(hipMemcpy(cells.data(), static_cast<float const *>(c->hist[0]) + 2 * off,
+        HIP_TRY(step_copy(c, cells.data(), static_cast<float const *>(c->hist[0]) + 2 * off,
                           cells.size() * sizeof(float), hipMemcpyDeviceToHost));
         for (u32 b = 0; b < nb; ++b) {
             g += cells[2 * b];
@@ -2190,8 +2213,8 @@ int bh_advance_level_per_node(bh_context *c, const bh_level_op *ops, uint32_t n_
     first[n_ops] = static_cast<u32>(hc.size());
     if (hc.size() > c->max_hch || parts > c->max_parts || fo.size() > c->max_ops + 2)
         return fail(1, "engine step: more histogram units than the configured fit holds");
-    if (!hc.empty()) HIP_TRY(hipMemcpy(c->hch, hc.data(), hc.size() * sizeof(HistChunk), hipMemcpyHostToDevice));
-    if (!fo.empty()) HIP_TRY(hipMemcpy(c->fops, fo.data(), fo.size() * sizeof(FinishOp), hipMemcpyHostToDevice));
+    if (!hc.empty()) HIP_TRY(step_copy(c, c->hch, hc.data(), hc.size() * sizeof(HistChunk), hipMemcpyHostToDevice));
+    if (!fo.empty()) HIP_TRY(step_copy(c, c->fops, fo.data(), fo.size() * sizeof(FinishOp), hipMemcpyHostToDevice));
     TreeCtl *hctl = nullptr;
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&hctl), std::max<u32>(1u, n_ops) * sizeof(TreeCtl), 0));
     u32 const tF = hist_transpose_F(c->quant, c->Fs, c->F);
@@ -2256,7 +2279,7 @@ int bh_find_level_split(bh_context *c, bh_step_node *nodes, uint32_t n, bh_split
         f[i] = step_node(c->seg_cur[i], i, nodes[i].depth);
     }
     if (int r = step_totals(c, f, c->hist[c->step_hist], nodes)) return r;
-    HIP_TRY(hipMemcpy(c->front[0], f.data(), n * sizeof(NodeInfo), hipMemcpyHostToDevice));
+    HIP_TRY(step_copy(c, c->front[0], f.data(), n * sizeof(NodeInfo), hipMemcpyHostToDevice));
     if (int r = step_ctl(c, n, 0, 0, 0, 0)) return r;
     SplitRec *pick = c->splits + std::max(c->max_front * 2u, 4u) - 1u; // last record: the pick
     if (n >= std::max(c->max_front * 2u, 4u)) return fail(1, "more nodes than the fit's frontier");
@@ -2264,9 +2287,9 @@ int bh_find_level_split(bh_context *c, bh_step_node *nodes, uint32_t n, bh_split
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     SplitRec p;
-    HIP_TRY(hipMemcpy(&p, pick, sizeof(p), hipMemcpyDeviceToHost));
+    HIP_TRY(step_copy(c, &p, pick, sizeof(p), hipMemcpyDeviceToHost));
     std::vector<SplitRec> per(n);
-    if (p.valid) HIP_TRY(hipMemcpy(per.data(), c->splits, n * sizeof(SplitRec), hipMemcpyDeviceToHost));
+    if (p.valid) HIP_TRY(step_copy(c, per.data(), c->splits, n * sizeof(SplitRec), hipMemcpyDeviceToHost));
     for (u32 i = 0; i < n; ++i) {
         SplitRec r = p.valid ? per[i] : p;
         std::memcpy(&out[i], &r, sizeof(r));
@@ -2293,16 +2316,16 @@ int bh_finalize_tree(bh_context *c, const float *node_values, uint32_t n_values,
     if (!c || c->step_mode == 0) return fail(1, "no tree begun");
     if (c->route.size() > c->max_nodes) return fail(1, "the routing tree exceeds the fit's node capacity");
     HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipMemcpy(c->tree, c->route.data(), c->route.size() * sizeof(DevNode), hipMemcpyHostToDevice));
+    HIP_TRY(step_copy(c, c->tree, c->route.data(), c->route.size() * sizeof(DevNode), hipMemcpyHostToDevice));
     TreeCtl t{};
     t.n_nodes = static_cast<u32>(c->route.size());
-    HIP_TRY(hipMemcpy(c->ctl, &t, sizeof(t), hipMemcpyHostToDevice));
+    HIP_TRY(step_copy(c, c->ctl, &t, sizeof(t), hipMemcpyHostToDevice));
     launch_score_update(c->stream, c->plane0, c->Fs, c->N, c->tree, c->ctl, c->d_nbins, c->F, 0.0F, nullptr,
                         c->leaf_rows, c->max_nodes);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     std::vector<u32> lr(c->N);
-    HIP_TRY(hipMemcpy(lr.data(), c->leaf_rows, c->N * sizeof(u32), hipMemcpyDeviceToHost));
+    HIP_TRY(step_copy(c, lr.data(), c->leaf_rows, c->N * sizeof(u32), hipMemcpyDeviceToHost));
     for (u64 r = 0; r < c->N; ++r) {
         u32 const id = c->route_stamp[lr[r]];
         if (leaf_ids) leaf_ids[r] = id;

@@ -2700,6 +2700,51 @@ __global__ __launch_bounds__(256) void k_copy_calibrate(uint4 const *__restrict_
         dst[i] = src[i];
 }
 
+// Bandwidth probes of known traffic with U independent 16-byte nontemporal loads in
+// flight per lane: a copy (reads and writes n * 16 bytes) and a read (n * 16 bytes
+// read; one dword per workgroup written so the loads stay live).
+typedef u32 u32x4v __attribute__((ext_vector_type(4)));
+template <int U>
+__global__ __launch_bounds__(256) void k_copy_wide(u32x4v const *__restrict__ src, u32x4v *__restrict__ dst, u64 n)
+{
+    u64 const stride = static_cast<u64>(gridDim.x) * 256u * U;
+    for (u64 base = blockIdx.x * 256ull * U + threadIdx.x; base < n; base += stride) {
+        u32x4v v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            u64 const i = base + static_cast<u64>(u) * 256u;
+            v[u] = i < n ? __builtin_nontemporal_load(src + i) : u32x4v{0u, 0u, 0u, 0u};
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            u64 const i = base + static_cast<u64>(u) * 256u;
+            if (i < n) __builtin_nontemporal_store(v[u], dst + i);
+        }
+    }
+}
+template <int U>
+__global__ __launch_bounds__(256) void k_read_wide(u32x4v const *__restrict__ src, u32 *__restrict__ sink, u64 n)
+{
+    __shared__ u32 acc;
+    if (threadIdx.x == 0) acc = 0u;
+    __syncthreads();
+    u32 x = 0u;
+    u64 const stride = static_cast<u64>(gridDim.x) * 256u * U;
+    for (u64 base = blockIdx.x * 256ull * U + threadIdx.x; base < n; base += stride) {
+        u32x4v v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            u64 const i = base + static_cast<u64>(u) * 256u;
+            v[u] = i < n ? __builtin_nontemporal_load(src + i) : u32x4v{0u, 0u, 0u, 0u};
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) x ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+    }
+    atomicXor(&acc, x);
+    __syncthreads();
+    if (threadIdx.x == 0) sink[blockIdx.x] = acc;
+}
+
 // ---------------------------------------------------------------------------------
 // Launch wrappers (host side).
 // ---------------------------------------------------------------------------------
@@ -2713,9 +2758,19 @@ static thread_local Tuning t_tune;
 void set_tuning(Tuning const &t) { t_tune = t; }
 Tuning const &tuning() { return t_tune; }
 
-void launch_copy_calibrate(hipStream_t s, void const *src, void *dst, u64 bytes)
+void launch_copy_calibrate(hipStream_t s, void const *src, void *dst, u64 bytes, u32 mode, u32 grid)
 {
     u64 const n = bytes / 16;
+    if (mode == 1u) {
+        hipLaunchKernelGGL(k_copy_wide<4>, dim3(grid), dim3(256), 0, s, static_cast<u32x4v const *>(src),
+                           static_cast<u32x4v *>(dst), n);
+        return;
+    }
+    if (mode == 2u) {
+        hipLaunchKernelGGL(k_read_wide<8>, dim3(grid), dim3(256), 0, s, static_cast<u32x4v const *>(src),
+                           static_cast<u32 *>(dst), n);
+        return;
+    }
     hipLaunchKernelGGL(k_copy_calibrate, dim3(static_cast<u32>(std::min<u64>((n + 255) / 256, 16384))), dim3(256), 0,
                        s, static_cast<uint4 const *>(src), static_cast<uint4 *>(dst), n);
 }

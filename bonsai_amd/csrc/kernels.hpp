@@ -94,7 +94,7 @@ void launch_copy_tree(hipStream_t s, DevNode const *tree, TreeCtl const *ctl, De
                       u32 max_nodes);
 void set_tuning(Tuning const &t);
 Tuning const &tuning();
-void launch_copy_calibrate(hipStream_t s, void const *src, void *dst, u64 bytes);
+void launch_copy_calibrate(hipStream_t s, void const *src, void *dst, u64 bytes, u32 mode = 0, u32 grid = 0);
 void launch_bin_rows(hipStream_t s, float const *X, u64 n, u32 F, u32 Fs, float const *cuts, u32 const *coff,
                      u32 n_cuts_total, u8 *plane);
 void launch_gather_rows(hipStream_t s, u8 const *plane, u32 Fs, void const *gh, u32 gh_bytes, u32 const *rows,

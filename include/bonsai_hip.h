@@ -89,6 +89,13 @@ int         bh_synchronize(bh_context *ctx);
  * mean kernel time (ms, hipEvents).  Its traffic is exactly 2 x bytes, which calibrates
  * HBM counters and gives the achievable copy bandwidth beside the histogram's. */
 int         bh_copy_calibrate(int device, uint64_t bytes, uint32_t reps, float *ms_mean);
+/* Calibration (no counterpart in the reference): the same timing for mode 0 (the plain
+ * copy above), 1 (copy with four independent 16-byte nontemporal loads in flight per
+ * lane; traffic 2 x bytes) or 2 (read only, eight loads in flight per lane; traffic
+ * bytes), over `grid` workgroups of 256 lanes (modes 1 and 2; 1 .. 65536).  The best
+ * of these is the bandwidth a streaming kernel reaches on the device. */
+int         bh_stream_calibrate(int device, uint64_t bytes, uint32_t reps, uint32_t mode, uint32_t grid,
+                                float *ms_mean);
 /* Message of the last failure on this thread. */
 const char *bh_last_error(void);
 
