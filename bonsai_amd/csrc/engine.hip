@@ -156,7 +156,15 @@ struct bh_context {
     std::vector<float> iter_ms;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
-    u32 hist_fsl() const { return F <= 64 ? round_up4(F) : 128u; }
+    // Features per histogram slice: all of them up to 64 (at most 64 x 256 x 8 B =
+    // 128 KiB of LDS cells), else up to 128 whose cells fit those 128 KiB.
+    u32 hist_fsl() const
+    {
+        if (F <= 64) return round_up4(F);
+        u32 f = 128;
+        while (f > 4 && static_cast<size_t>(f) * SB * 8u > (128u << 10)) f -= 4;
+        return f;
+    }
     u32 hist_waves() const
     {
         u32 const groups = (std::min(F, hist_fsl()) + 3u) / 4u;
