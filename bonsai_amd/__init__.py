@@ -178,6 +178,7 @@ def lib() -> C.CDLL:
         "bh_get_gradients": [P, P, P, P, P, P],
         "bh_grow_tree": [P, P, u32, C.POINTER(u32), P],
         "bh_histogram": [P, P, u64, P, P],
+        "bh_histogram_noalloc": [P, P, u64, P, P],
         "bh_find_split": [P, P, P, u64, u32, i32, C.c_double, C.c_double, P, P],
         "bh_partition": [P, P, u64, u32, u32, u32, P, C.POINTER(u64)],
         "bh_time_root_histogram": [P, u32, C.POINTER(C.c_float)],

@@ -11,6 +11,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -46,6 +47,78 @@ template <typename T> hipError_t dalloc(T **p, size_t count)
 }
 
 u32 round_up4(u32 x) { return (x + 3u) & ~3u; }
+
+} // namespace
+
+// ---------------------------------------------------------------------------------
+// Scratch of the no-allocation histogram entry point (bh_histogram_noalloc).  One copy
+// per GPU, held until the process ends: the histogram work list, its completion op and
+// control block, then the unit partials.  Host code never passes the array's name to a
+// kernel (on the host it is a stand-in, not a device address): bh_create resolves each
+// device's address once with hipGetSymbolAddress and every use goes through that.
+// ---------------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = 32ull << 20;
+constexpr size_t kNoAllocCtlOff = 0;                 // TreeCtl (128 bytes)
+constexpr size_t kNoAllocFinishOff = 256;            // FinishOp (64 bytes)
+constexpr size_t kNoAllocChunkOff = 512;             // HistChunk[kNoAllocMaxUnits]
+constexpr u32 kNoAllocMaxUnits = 2048;
+constexpr size_t kNoAllocPartOff =
+    (kNoAllocChunkOff + kNoAllocMaxUnits * sizeof(HistChunk) + 255) / 256 * 256; // unit partials
+static_assert(sizeof(TreeCtl) <= kNoAllocFinishOff - kNoAllocCtlOff, "TreeCtl fits its scratch slot");
+static_assert(sizeof(FinishOp) <= kNoAllocChunkOff - kNoAllocFinishOff, "FinishOp fits its scratch slot");
+
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+// Resolves this device's copy of the scratch array (the device must be current).
+int resolve_scratch(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) return fail(1, "device id beyond the scratch table");
+    if (g_scratch_ptr[device]) return 0;
+    void *ptr = nullptr;
+    hipError_t const e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+        return fail(-1, std::string("hip: scratch address: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<unsigned char *>(ptr);
+    return 0;
+}
+
+// One node over rows [0, n) of work plane 0: `units` histogram units of kHistChunk rows
+// (the same decomposition bh_histogram uploads), their completion op, and a control
+// block cleared except for the unit and op counts.  One workgroup.
+__global__ __launch_bounds__(256) void k_noalloc_plan(TreeCtl *__restrict__ ctl, FinishOp *__restrict__ fop,
+                                                      HistChunk *__restrict__ chunks, u32 units, u32 n)
+{
+    u32 *cw = reinterpret_cast<u32 *>(ctl);
+    for (u32 i = threadIdx.x; i < sizeof(TreeCtl) / 4u; i += blockDim.x) cw[i] = 0u;
+    for (u32 k = threadIdx.x; k < units; k += blockDim.x) {
+        HistChunk h{};
+        h.slot = 0;
+        h.src = 0;
+        h.begin = k * kHistChunk;
+        h.end = min(n, (k + 1u) * kHistChunk);
+        h.part = units > 1u ? k : kNone;
+        chunks[k] = h;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        FinishOp f{};
+        f.small_slot = 0;
+        f.large_slot = kNone;
+        f.part_first = 0;
+        f.part_count = units > 1u ? units : 0u;
+        f.empty = n == 0u ? 1u : 0u;
+        *fop = f;
+        ctl->n_hist_chunks = units;
+        ctl->n_finish = 1;
+    }
+}
 
 } // namespace
 
@@ -832,6 +905,11 @@ int bh_create(int device, bh_context **out)
     }
     hipEventCreate(&c->ev0);
     hipEventCreate(&c->ev1);
+    if (int r = resolve_scratch(device)) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return r;
+    }
     e = set_hist_lds_limits();
     if (e == hipSuccess) e = ext_set_limits();
     if (e != hipSuccess) {
@@ -1632,6 +1710,45 @@ int bh_histogram(bh_context *c, const uint32_t *rows, uint64_t n, float *out_f32
         if (!out_f32) return fail(1, "FP32 fit: pass out_f32");
         HIP_TRY(hipMemcpy(out_f32, c->hist[0], nc * 2 * sizeof(float), hipMemcpyDeviceToHost));
     }
+    return 0;
+}
+
+int bh_histogram_noalloc(bh_context *c, const uint32_t *d_rows, uint64_t n, void *d_out, void *hip_stream)
+{
+    if (int r = check_ready(c)) return r;
+    if (n > c->N) return fail(1, "more rows than the plane holds");
+    if (n > 0 && !d_rows) return fail(1, "null row list");
+    if (!d_out) return fail(1, "null output");
+    u32 const units = static_cast<u32>((n + kHistChunk - 1) / kHistChunk);
+    size_t const part_bytes = static_cast<size_t>(units > 1u ? units : 0u) * c->node_cells() * c->part_bytes();
+    if (units > kNoAllocMaxUnits || kNoAllocPartOff + part_bytes > kNoAllocScratchBytes)
+        return fail(1, "bh_histogram_noalloc: " + std::to_string(n) + " rows need " +
+                           std::to_string(kNoAllocPartOff + part_bytes) + " bytes of scratch, more than its " +
+                           std::to_string(kNoAllocScratchBytes) + " (at most " +
+                           std::to_string(std::min<size_t>(kNoAllocMaxUnits,
+                                                           (kNoAllocScratchBytes - kNoAllocPartOff) /
+                                                               (c->node_cells() * c->part_bytes()))) +
+                           " units of 8192 rows at this plane's features and bins)");
+    HIP_TRY(hipSetDevice(c->device));
+    unsigned char *const sp = g_scratch_ptr[c->device];
+    if (!sp) return fail(-1, "scratch not resolved for this device");
+    hipStream_t const st = hip_stream ? static_cast<hipStream_t>(hip_stream) : c->stream;
+    TreeCtl *const ctl = reinterpret_cast<TreeCtl *>(sp + kNoAllocCtlOff);
+    FinishOp *const fop = reinterpret_cast<FinishOp *>(sp + kNoAllocFinishOff);
+    HistChunk *const chunks = reinterpret_cast<HistChunk *>(sp + kNoAllocChunkOff);
+    void *const partials = sp + kNoAllocPartOff;
+    u32 const gh_bytes = c->quant ? 4 : 8;
+    if (n > 0)
+        launch_gather_rows(st, c->plane0, c->Fs, gh_src(c, 2), gh_bytes, d_rows, n, c->work[0],
+                           const_cast<void *>(gh_src(c, 0)), 0);
+    hipLaunchKernelGGL(k_noalloc_plan, dim3(1), dim3(256), 0, st, ctl, fop, chunks, units, static_cast<u32>(n));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_hist(st, c->quant, chunks, ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0), gh_src(c, 1),
+                        gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(),
+                        std::max(1u, std::min(units, 4096u)), partials, d_out));
+    launch_finish(st, c->quant, fop, ctl, partials, d_out, d_out, c->node_cells(), 1, nullptr, nullptr, false,
+                  hist_transpose_F(c->quant, c->Fs, c->F));
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 

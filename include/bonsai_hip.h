@@ -235,6 +235,20 @@ int bh_grow_tree(bh_context *ctx, bh_node *nodes, uint32_t cap, uint32_t *n_node
  * out_i64.  bin_stride = max n_bins. */
 int bh_histogram(bh_context *ctx, const uint32_t *rows, uint64_t n, float *out_f32,
                  int64_t *out_i64);
+/* No counterpart in the reference: bh_histogram for callers that cannot allocate (code
+ * under HIP graph capture, tight serving or engine loops, harnesses that forbid
+ * hipMalloc).  The row list (n row ids below n_rows, device memory) and the output (the
+ * histogram in bh_histogram's layout, device memory: int64 pairs for quantized fits,
+ * float pairs for FP32 ones) are the caller's; the work list, control block and unit
+ * partials live in a fixed 32 MiB per-GPU scratch array that the library holds until
+ * the process ends, so the call allocates nothing and returns once its kernels are
+ * queued on hip_stream (NULL: the context's stream).  The histogram equals bh_histogram's
+ * for the same rows bit for bit.  Row counts whose unit partials (8,192-row units of
+ * features x bins cells) exceed the scratch are refused with a capacity error naming the
+ * limit.  Every call on one GPU shares that GPU's scratch: calls that overlap on one GPU
+ * (other streams, other contexts) race, and like bh_histogram the call reuses the
+ * context's gather buffers. */
+int bh_histogram_noalloc(bh_context *ctx, const uint32_t *d_rows, uint64_t n, void *d_out, void *hip_stream);
 /* Node split find over a host histogram in the layout above (find_splits_many,
  * device_context.cu:997-1048).  Totals are computed from the histogram unless
  * tot_given != 0. */
