@@ -39,6 +39,18 @@ def _compile(src: str) -> Path:
     return obj
 
 
+PROBES = PKG / "probes"
+
+
+def build_probes() -> None:
+    """Stand-alone device probes (bonsai_amd/probes/*.hip -> executables beside them)."""
+    for src in sorted(PROBES.glob("*.hip")):
+        cmd = [HIPCC, *FLAGS, "-o", str(src.with_suffix("")), str(src)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stdout}{r.stderr}")
+
+
 def build(verbose: bool = True) -> Path:
     BUILD.mkdir(exist_ok=True)
     with ThreadPoolExecutor(max_workers=4) as ex:
@@ -49,6 +61,7 @@ def build(verbose: bool = True) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}{r.stderr}")
     os.replace(tmp, LIB)
+    build_probes()
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -158,6 +158,7 @@ struct bh_context {
     u32 *gh0q = nullptr, *ghwq[2] = {nullptr, nullptr};
     u32 *maxbits = nullptr;
     double *scales = nullptr;
+    u32 *fx = nullptr; // FP32 histogram grids: maxima of |g|, h (zero between passes), s_g, s_h
     // histograms and work lists
     void *hist[2] = {nullptr, nullptr};
     void *partials = nullptr;
@@ -229,13 +230,13 @@ struct bh_context {
     std::vector<float> iter_ms;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
-    // Features per histogram slice: all of them up to 64 (at most 64 x 256 x 8 B =
-    // 128 KiB of LDS cells), else up to 128 whose cells fit those 128 KiB.
+    // Features per histogram slice: all of them up to 64, else up to 128, as long as the
+    // slice's LDS cells fit 128 KiB (8 B per quantized cell, 16 B per FP32 g / h pair).
     u32 hist_fsl() const
     {
-        if (F <= 64) return round_up4(F);
-        u32 f = 128;
-        while (f > 4 && static_cast<size_t>(f) * SB * 8u > (128u << 10)) f -= 4;
+        u32 const cell = quant ? 8u : 16u;
+        u32 f = F <= 64 ? round_up4(F) : 128u;
+        while (f > 4 && static_cast<size_t>(f) * SB * cell > (128u << 10)) f -= 4;
         return f;
     }
     u32 hist_waves() const
@@ -265,10 +266,11 @@ void free_fit(bh_context *c)
     if (c->graph) hipGraphDestroy(c->graph);
     c->graph_exec = nullptr;
     c->graph = nullptr;
-    void *ptrs[] = {c->gh0f, c->ghwf[0], c->ghwf[1], c->gh0q, c->ghwq[0], c->ghwq[1], c->maxbits, c->scales,
+    void *ptrs[] = {c->gh0f, c->ghwf[0], c->ghwf[1], c->gh0q, c->ghwq[0], c->ghwq[1], c->maxbits, c->scales, c->fx,
                     c->hist[0], c->hist[1], c->partials, c->ctl, c->front[0], c->front[1], c->splits, c->tree,
                     c->ops, c->pch, c->chunk_left, c->hch, c->fops, c->trees_out, c->tree_sizes, c->leaf_rows,
-                    c->fscr.part, c->fscr.done, c->cnt[0], c->cnt[1], c->ops4, c->chunk_cls, c->mono_dev,
+                    c->fscr.part, c->fscr.done, c->cnt[0], c->cnt[1], c->ops4, c->chunk_cls,
+                    c->mono_dev,
                     c->bounds, c->groups_dev, c->paths};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -283,6 +285,7 @@ void free_fit(bh_context *c)
     c->gh0q = c->ghwq[0] = c->ghwq[1] = nullptr;
     c->maxbits = nullptr;
     c->scales = nullptr;
+    c->fx = nullptr;
     c->hist[0] = c->hist[1] = c->partials = nullptr;
     c->ctl = nullptr;
     c->front[0] = c->front[1] = nullptr;
@@ -408,6 +411,8 @@ int alloc_fit(bh_context *c)
     HIP_TRY(dalloc(&c->maxbits, 4));
     HIP_TRY(dalloc(&c->scales, 4));
     HIP_TRY(hipMemset(c->scales, 0, 4 * sizeof(double)));
+    HIP_TRY(dalloc(&c->fx, 4));
+    HIP_TRY(hipMemset(c->fx, 0, 4 * sizeof(u32)));
     HIP_TRY(hipMalloc(&c->hist[0], hist_bytes));
     if (f.grower != BH_LEAFWISE) HIP_TRY(hipMalloc(&c->hist[1], hist_bytes));
     HIP_TRY(hipMalloc(&c->partials, static_cast<size_t>(c->max_parts) * nc * c->part_bytes()));
@@ -508,7 +513,7 @@ float profiled_hist(bh_context *c, void *hist_next, u32 grid_units, u32 *cnt_nex
     (void)hipEventRecord(c->ev0, c->stream);
     launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
                 gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), grid_units,
-                c->partials, hist_next, cnt_next, routes, group);
+                c->partials, hist_next, c->fx, cnt_next, routes, group);
     (void)hipEventRecord(c->ev1, c->stream);
     (void)hipEventSynchronize(c->ev1);
     float ms = 0.0F;
@@ -568,7 +573,7 @@ void enqueue_build(bh_context *c, void *hist_cur, void *hist_next, u32 max_ops, 
     else
         launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
                     gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), grid_units,
-                    c->partials, hist_next, cnt_next, routes, group);
+                    c->partials, hist_next, c->fx, cnt_next, routes, group);
     launch_finish(c->stream, c->quant, c->fops, c->ctl, c->partials, hist_cur, hist_next, c->node_cells(), max_ops,
                   cnt_cur, cnt_next, advance, hist_transpose_F(c->quant, c->Fs, c->F, group));
     if (c->prof) profiled_count(c, ms, cnt_next);
@@ -729,7 +734,7 @@ void enqueue_gradients(bh_context *c)
         launch_grad_quant(c->stream, c->score, c->label, c->weight, c->N, c->fc.objective, c->maxbits, c->fc.seed,
                           c->ctl, c->gh0q, c->scales);
     else
-        launch_grad_f32(c->stream, c->score, c->label, c->weight, c->N, c->fc.objective, c->gh0f);
+        launch_grad_f32(c->stream, c->score, c->label, c->weight, c->N, c->fc.objective, c->gh0f, c->fx);
 }
 
 // One boosting iteration: gradients, tree, score epilogue, tree copy-out.
@@ -817,6 +822,7 @@ GrowView grow_view(bh_context *c)
     v.lw_units = c->lw_units;
     v.root = root_sel(c);
     v.fsl = c->hist_fsl();
+    v.fx = c->fx;
     v.waves = c->hist_waves();
     v.host_sync = c->fc.host_sync != 0;
     return v;
@@ -1585,7 +1591,10 @@ int bh_set_gradients(bh_context *c, const float *grad, const float *hess, uint32
     for (u64 i = 0; i < c->N; ++i) gh[i] = make_float2(grad[i], hess[i]);
     c->max_stale = true; // the host gradients' maxima overwrite slot 0
     if (!c->quant) {
-        HIP_TRY(hipMemcpy(c->gh0f, gh.data(), c->N * sizeof(float2), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpyAsync(c->gh0f, gh.data(), c->N * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+        launch_fx_from(c->stream, c->gh0f, c->N, c->fx);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(c->stream));
         return 0;
     }
     float2 *tmp = nullptr;
@@ -1745,7 +1754,7 @@ int bh_histogram_noalloc(bh_context *c, const uint32_t *d_rows, uint64_t n, void
     HIP_TRY(hipGetLastError());
     HIP_TRY(launch_hist(st, c->quant, chunks, ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0), gh_src(c, 1),
                         gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(),
-                        std::max(1u, std::min(units, 4096u)), partials, d_out));
+                        std::max(1u, std::min(units, 4096u)), partials, d_out, c->fx));
     launch_finish(st, c->quant, fop, ctl, partials, d_out, d_out, c->node_cells(), 1, nullptr, nullptr, false,
                   hist_transpose_F(c->quant, c->Fs, c->F));
     HIP_TRY(hipGetLastError());
@@ -1888,7 +1897,7 @@ int bh_time_root_histogram(bh_context *c, uint32_t reps, float *ms_mean)
         HIP_TRY(hipEventRecord(c->ev0, c->stream));
         launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
                     gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(), hist_grid(c),
-                    c->partials, c->hist[0]);
+                    c->partials, c->hist[0], c->fx);
         HIP_TRY(hipEventRecord(c->ev1, c->stream));
         HIP_TRY(hipEventSynchronize(c->ev1));
         float ms = 0.0F;
@@ -1996,7 +2005,7 @@ int step_build(bh_context *c, std::vector<StepBuild> const &builds, void *hist_c
     if (int r = step_ctl(c, 0, 0, 0, static_cast<u32>(hc.size()), static_cast<u32>(fo.size()))) return r;
     HIP_TRY(launch_hist(c->stream, c->quant, c->hch, c->ctl, c->work[0], c->work[1], c->plane0, gh_src(c, 0),
                         gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(), c->hist_waves(),
-                        std::max(1u, std::min(static_cast<u32>(hc.size()), 4096u)), c->partials, hist_next));
+                        std::max(1u, std::min(static_cast<u32>(hc.size()), 4096u)), c->partials, hist_next, c->fx));
     launch_finish(c->stream, c->quant, c->fops, c->ctl, c->partials, hist_cur, hist_next, c->node_cells(),
                   std::max<u32>(1u, static_cast<u32>(fo.size())), nullptr, nullptr, false,
                   hist_transpose_F(c->quant, c->Fs, c->F));
@@ -2353,7 +2362,7 @@ int bh_advance_level_per_node(bh_context *c, const bh_level_op *ops, uint32_t n_
             e = launch_hist(c->stream, c->quant, c->hch + first[i], c->ctl, c->work[0], c->work[1], c->plane0,
                             gh_src(c, 0), gh_src(c, 1), gh_src(c, 2), c->Fs, c->F, c->SB, c->hist_fsl(),
                             c->hist_waves(), std::max(1u, std::min(first[i + 1] - first[i], 4096u)), c->partials,
-                            nxt);
+                            nxt, c->fx);
         if (e == hipSuccess)
             launch_finish(c->stream, c->quant, c->fops + i, c->ctl, c->partials, cur, nxt, c->node_cells(), 1,
                           nullptr, nullptr, false, tF);

@@ -163,16 +163,6 @@ __device__ __forceinline__ void grad_of_row(int objective, float s, float y, flo
     }
 }
 
-__global__ void k_grad_f32(float const *__restrict__ score, float const *__restrict__ label,
-                           float const *__restrict__ weight, u64 n, int objective, float2 *__restrict__ gh)
-{
-    for (u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; r < n;
-         r += static_cast<u64>(gridDim.x) * blockDim.x) {
-        float g, h;
-        grad_of_row(objective, score[r], label[r], weight, r, g, h);
-        gh[r] = make_float2(g, h);
-    }
-}
 
 __global__ void k_zero_u32(u32 *p, u32 n)
 {
@@ -201,6 +191,24 @@ __device__ __forceinline__ void block_max2_atomic(float mg, float mh, u32 *maxbi
         atomicMax(&maxbits[0], __float_as_uint(mg));
         atomicMax(&maxbits[1], __float_as_uint(mh));
     }
+}
+
+// FP32 gradients; the maxima of |g| and h go to fx[0..1] for the histogram's
+// fixed-point grid (k_fx_shift).
+__global__ __launch_bounds__(256) void k_grad_f32(float const *__restrict__ score, float const *__restrict__ label,
+                                                  float const *__restrict__ weight, u64 n, int objective,
+                                                  float2 *__restrict__ gh, u32 *__restrict__ fx)
+{
+    float mg = 0.0F, mh = 0.0F;
+    for (u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; r < n;
+         r += static_cast<u64>(gridDim.x) * blockDim.x) {
+        float g, h;
+        grad_of_row(objective, score[r], label[r], weight, r, g, h);
+        gh[r] = make_float2(g, h);
+        mg = fmaxf(mg, fabsf(g));
+        mh = fmaxf(mh, fabsf(h));
+    }
+    block_max2_atomic(mg, mh, fx);
 }
 
 // Max-abs of g and h over every row, for the quantizer of the current iteration.  The
@@ -297,142 +305,126 @@ __global__ void k_gh_max(float2 const *__restrict__ gh, u64 n, u32 *__restrict__
 // (src/cuda/detail/kernels.cuh:186-277); CPU semantics of populate
 // (src/grower.cpp:93-139).
 //
-// One workgroup per work unit (a chunk of <= kHistChunk rows of one node's contiguous
-// segment) and feature slice.  The slice's sub-histogram lives in LDS.  Wave w owns
-// the 4-feature groups g = w, w+NW, ...: it reads one dword of every row of the chunk
-// (4 bins), in ascending row order, and is the only wave that touches those cells.
-// FP32: two ds_add_f32 per (row, feature), so every cell sums its rows in ascending
-// order (fixed order, no inter-wave races).  Quantized: one ds_add_u64 per
-// (row, feature) on the packed (g << 32) + h word, exact and order-free.
-// The unit's LDS histogram is written once to a partial (multi-unit nodes) or straight
-// into the node's slot (single-unit nodes).
+// One workgroup per work unit (at most kHistChunk rows of one node's contiguous segment)
+// and feature slice; the slice's sub-histogram lives in LDS and is written once to a
+// partial (multi-unit nodes) or straight into the node's slot (single-unit nodes).
+//
+// FP32 gradients (k_hist_fx): LDS float atomics cost about 30 times an integer atomic on
+// this device (192 against 6.5 clocks per wave instruction and CU, measured), so the
+// cells sum on a fixed-point grid instead.  Every g (h) goes on the grid 2^-s with
+// s = 49 - e, max|g| < 2^e over the iteration's rows, rounded to nearest even: a unit's
+// cells then sum exactly in int64 (8,192 x 2^49 < 2^63), in any order, and each cell is
+// rounded once to float when the unit is written.  Units fold in unit order and the
+// larger sibling is parent - small in float (k_finish<false>), as before.  Lanes are row
+// x dword as in k_hist_q: two ds_add_u64 per (row, feature).
 // ---------------------------------------------------------------------------------
-template <bool Q, int U>
-__global__ __launch_bounds__(1024) void k_hist(
+__device__ __forceinline__ int fx_shift_of(float m)
+{
+    if (!(m > 0.0F) || !(m <= 3.402823466e38F)) return 0;
+    int e = 0;
+    (void)frexpf(m, &e);
+    return 49 - e;
+}
+__device__ __forceinline__ long long fx_quant(float v, int s) { return static_cast<long long>(rintf(ldexpf(v, s))); }
+__device__ __forceinline__ float fx_dequant(long long q, int s) { return ldexpf(static_cast<float>(q), -s); }
+
+// The grids of the current gradients: fx[0..1] hold max|g|, max h as float bits (filled
+// by the gradient pass), fx[2..3] receive s_g, s_h, and the maxima are cleared for the
+// next pass.  One thread.
+__global__ void k_fx_shift(u32 *fx)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        fx[2] = static_cast<u32>(fx_shift_of(__uint_as_float(fx[0])));
+        fx[3] = static_cast<u32>(fx_shift_of(__uint_as_float(fx[1])));
+        fx[0] = 0u;
+        fx[1] = 0u;
+    }
+}
+
+constexpr int kFxUnroll = 4;
+
+__global__ __launch_bounds__(1024) void k_hist_fx(
     HistChunk const *__restrict__ chunks, TreeCtl const *__restrict__ ctl,
     u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
     void const *__restrict__ gw0, void const *__restrict__ gw1, void const *__restrict__ gwp,
-    u32 Fs, u32 F, u32 SB, u32 fsl, void *__restrict__ partials, void *__restrict__ hist_next)
+    u32 Fs, u32 F, u32 SB, u32 fsl, void *__restrict__ partials, void *__restrict__ hist_next,
+    u32 const *__restrict__ fx)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     u32 const f_begin = blockIdx.y * fsl;
     if (f_begin >= F) return;
     u32 const nf = min(F - f_begin, fsl);
-    u32 const groups = (nf + 3u) / 4u;
-    u32 const wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    u32 const NW = blockDim.x >> 6;
     u32 const cells = nf * SB;
+    unsigned long long *lg = reinterpret_cast<unsigned long long *>(smem); // nf x SB g cells
+    unsigned long long *lh = lg + cells;                                    // nf x SB h cells
+    int const sg = static_cast<int>(fx[2]), sh = static_cast<int>(fx[3]);
+    u32 const Dw = (nf + 3u) / 4u;             // dwords of the slice per row
+    u32 const R = 64u / Dw;                    // rows per wave instruction
+    u32 const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    u32 const NW = blockDim.x >> 6;
+    u32 const lr = lane / Dw, d = lane - lr * Dw;
+    bool const lane_ok = lr < R;
     u32 const n_units = ctl->n_hist_chunks;
     size_t const node_cells = static_cast<size_t>(F) * SB;
-
     for (u32 c = blockIdx.x; c < n_units; c += gridDim.x) {
         HistChunk const ch = chunks[c];
-        if (Q) {
-            u64 *lh = reinterpret_cast<u64 *>(smem);
-            for (u32 i = threadIdx.x; i < cells; i += blockDim.x) lh[i] = 0ull;
-        } else {
-            float *lg = reinterpret_cast<float *>(smem);
-            for (u32 i = threadIdx.x; i < 2u * cells; i += blockDim.x) lg[i] = 0.0F;
-        }
+        for (u32 i = threadIdx.x; i < 2u * cells; i += blockDim.x) lg[i] = 0ull;
         __syncthreads();
         u8 const *plane = ch.src == 0 ? pl0 : (ch.src == 1 ? pl1 : plp);
-        void const *ghv = ch.src == 0 ? gw0 : (ch.src == 1 ? gw1 : gwp);
-        // Waves per feature group: with more waves than groups, the waves sharing a
-        // group split its row stream (quantized cells only: order-free).
-        u32 const wpg = NW >= groups ? NW / groups : 1u;
-        u32 const sub = NW >= groups ? wave / groups : 0u;
-        u32 const g_first = NW >= groups ? wave % groups : wave;
-        u32 const g_step = NW >= groups ? groups : NW;
-        bool const active = NW < groups || wave < wpg * groups;
-        for (u32 base = ch.begin + sub * 64u * U; active && base < ch.end; base += wpg * 64u * U) {
-            u32 rows[U];
-            bool ok[U];
+        float2 const *ghv = static_cast<float2 const *>(ch.src == 0 ? gw0 : (ch.src == 1 ? gw1 : gwp));
+        u8 const *cb = plane + f_begin + 4u * d;
+        u32 const stride = NW * R;
+        auto add_row = [&](u32 dwv, float2 v) {
+            unsigned long long const qg = static_cast<unsigned long long>(fx_quant(v.x, sg));
+            unsigned long long const qh = static_cast<unsigned long long>(fx_quant(v.y, sh));
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                rows[u] = base + u * 64u + lane;
-                ok[u] = rows[u] < ch.end;
-            }
-            float gf[U], hf[U];
-            u64 pk[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (Q) {
-                    u32 const v = ok[u] ? reinterpret_cast<u32 const *>(ghv)[rows[u]] : 0u;
-                    i64 const g = static_cast<int16_t>(v >> 16);
-                    u64 const h = v & 0xFFFFu;
-                    pk[u] = static_cast<u64>(g * 4294967296LL) + h;
-                } else {
-                    float2 const v = ok[u] ? reinterpret_cast<float2 const *>(ghv)[rows[u]]
-                                           : make_float2(0.0F, 0.0F);
-                    gf[u] = v.x;
-                    hf[u] = v.y;
+            for (u32 k = 0; k < 4; ++k) {
+                u32 const fl = 4u * d + k;
+                if (fl < nf) {
+                    u32 const cell = fl * SB + ((dwv >> (8u * k)) & 0xFFu);
+                    atomicAdd(lg + cell, qg);
+                    atomicAdd(lh + cell, qh);
                 }
             }
-            for (u32 g = g_first; g < groups; g += g_step) {
-                u32 d[U];
+        };
+        if (lane_ok) {
+            u32 r = ch.begin + wave * R + lr;
+            for (; r + (kFxUnroll - 1) * stride < ch.end; r += kFxUnroll * stride) {
+                u32 dw[kFxUnroll];
+                float2 v[kFxUnroll];
 #pragma unroll
-                for (int u = 0; u < U; ++u)
-                    d[u] = ok[u] ? *reinterpret_cast<u32 const *>(
-                                       plane + static_cast<size_t>(rows[u]) * Fs + f_begin + 4u * g)
-                                 : 0u;
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    if (!ok[u]) continue;
-#pragma unroll
-                    for (u32 k = 0; k < 4; ++k) {
-                        u32 const fl = 4u * g + k;
-                        if (fl >= nf) break;
-                        u32 const cell = fl * SB + ((d[u] >> (8u * k)) & 0xFFu);
-                        if (Q) {
-                            atomicAdd(reinterpret_cast<unsigned long long *>(smem) + cell,
-                                      static_cast<unsigned long long>(pk[u]));
-                        } else {
-                            float *lg = reinterpret_cast<float *>(smem);
-                            atomicAdd(lg + cell, gf[u]);
-                            atomicAdd(lg + cells + cell, hf[u]);
-                        }
-                    }
+                for (int u = 0; u < kFxUnroll; ++u) {
+                    u32 const rr = r + u * stride;
+                    dw[u] = *reinterpret_cast<u32 const *>(cb + static_cast<size_t>(rr) * Fs);
+                    v[u] = ghv[rr];
                 }
+#pragma unroll
+                for (int u = 0; u < kFxUnroll; ++u) add_row(dw[u], v[u]);
             }
+            for (; r < ch.end; r += stride)
+                add_row(*reinterpret_cast<u32 const *>(cb + static_cast<size_t>(r) * Fs), ghv[r]);
         }
         __syncthreads();
-        // Flush: partial (packed) or the node's own slot (final cells).
-        if (Q) {
-            u64 const *lh = reinterpret_cast<u64 const *>(smem);
-            if (ch.part != kNone) {
-                u64 *dst = reinterpret_cast<u64 *>(partials) + static_cast<size_t>(ch.part) * node_cells +
-                           static_cast<size_t>(f_begin) * SB;
-                for (u32 i = threadIdx.x; i < cells; i += blockDim.x) dst[i] = lh[i];
-            } else {
-                i64 *dst = reinterpret_cast<i64 *>(hist_next) +
-                           (static_cast<size_t>(ch.slot) * node_cells + static_cast<size_t>(f_begin) * SB) * 2;
-                for (u32 i = threadIdx.x; i < cells; i += blockDim.x) {
-                    u64 const v = lh[i];
-                    dst[2 * i] = static_cast<int32_t>(static_cast<u32>(v >> 32));
-                    dst[2 * i + 1] = static_cast<i64>(v & 0xFFFFFFFFull);
-                }
-            }
-        } else {
-            float const *lg = reinterpret_cast<float const *>(smem);
-            float2 *dst = ch.part != kNone
-                              ? reinterpret_cast<float2 *>(partials) + static_cast<size_t>(ch.part) * node_cells +
-                                    static_cast<size_t>(f_begin) * SB
-                              : reinterpret_cast<float2 *>(hist_next) + static_cast<size_t>(ch.slot) * node_cells +
-                                    static_cast<size_t>(f_begin) * SB;
-            for (u32 i = threadIdx.x; i < cells; i += blockDim.x) dst[i] = make_float2(lg[i], lg[cells + i]);
-        }
+        float2 *dst = ch.part != kNone
+                          ? static_cast<float2 *>(partials) + static_cast<size_t>(ch.part) * node_cells +
+                                static_cast<size_t>(f_begin) * SB
+                          : static_cast<float2 *>(hist_next) + static_cast<size_t>(ch.slot) * node_cells +
+                                static_cast<size_t>(f_begin) * SB;
+        for (u32 i = threadIdx.x; i < cells; i += blockDim.x)
+            dst[i] = make_float2(fx_dequant(static_cast<long long>(lg[i]), sg),
+                                 fx_dequant(static_cast<long long>(lh[i]), sh));
         __syncthreads();
     }
 }
 
-// Quantized histogram build, row-and-dword lanes (H6, same contract as k_hist<true>).
+// Quantized histogram build, row-and-dword lanes (H6, same unit contract as k_hist_fx).
 // Lane l of a wave takes dword d = l % Dw of row l / Dw (Dw = the slice's dwords per
 // row), so one wave load reads 64 / Dw whole rows as one contiguous span and a lane's
 // four bins feed four ds_add_u64 on the packed (g << 32) + h cell.  Each atomic
 // instruction then spreads over 64 / Dw rows x Dw features (7 features x 9 rows at 28
 // features) instead of 64 rows of one feature, which removes most same-address
 // collisions.  Integer sums are exact and order-free.  The LDS histogram is the slice's
-// [feature][SB] cells; the flush is the one k_hist<true> uses.
+// [feature][SB] cells.
 constexpr u32 kHqThreads = 1024;
 constexpr int kHqUnroll = 8;
 
@@ -2784,9 +2776,15 @@ void launch_bin_rows(hipStream_t s, float const *X, u64 n, u32 F, u32 Fs, float 
                        lds_cuts);
 }
 void launch_grad_f32(hipStream_t s, float const *score, float const *label, float const *weight, u64 n, int obj,
-                     float2 *gh)
+                     float2 *gh, u32 *fx)
 {
-    hipLaunchKernelGGL(k_grad_f32, dim3(2048), dim3(256), 0, s, score, label, weight, n, obj, gh);
+    hipLaunchKernelGGL(k_grad_f32, dim3(2048), dim3(256), 0, s, score, label, weight, n, obj, gh, fx);
+    hipLaunchKernelGGL(k_fx_shift, dim3(1), dim3(64), 0, s, fx);
+}
+void launch_fx_from(hipStream_t s, float2 const *gh, u64 n, u32 *fx)
+{
+    hipLaunchKernelGGL(k_gh_max, dim3(1024), dim3(256), 0, s, gh, n, fx);
+    hipLaunchKernelGGL(k_fx_shift, dim3(1), dim3(64), 0, s, fx);
 }
 void launch_grad_quant(hipStream_t s, float const *score, float const *label, float const *weight, u64 n, int obj,
                        u32 *maxbits, u64 seed, TreeCtl const *ctl, u32 *ghq, double *scales)
@@ -2816,8 +2814,8 @@ u32 hist_transpose_F(bool q, u32 Fs, u32 F, bool group)
 
 hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl const *ctl, u8 const *pl0,
                        u8 const *pl1, u8 const *plp, void const *g0, void const *g1, void const *gp, u32 Fs, u32 F,
-                       u32 SB, u32 fsl, u32 n_waves, u32 grid_x, void *partials, void *hist_next, u32 *cnt_next,
-                       PartOp const *routes, bool group)
+                       u32 SB, u32 fsl, u32 n_waves, u32 grid_x, void *partials, void *hist_next, u32 const *fx,
+                       u32 *cnt_next, PartOp const *routes, bool group)
 {
     // group: units may span several 8192-row sub-units (the row-dword and row-in-lane
     // quantized kernels add each sub-unit into the unit's cells; the feature-in-lane one
@@ -2871,9 +2869,12 @@ hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl c
     } else if (q) {
         hipLaunchKernelGGL(k_hist_q, grid, dim3(t_tune.hist_threads), lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp,
                            Fs, F, SB, fsl, partials, hist_next, cnt_next, routes);
-    } else
-        hipLaunchKernelGGL((k_hist<false, 4>), grid, block, lds, s, chunks, ctl, pl0, pl1, plp, g0, g1, gp, Fs, F,
-                           SB, fsl, partials, hist_next);
+    } else {
+        if (!fx) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_hist_fx, grid, dim3(t_tune.hist_threads), lds * 2u, s, chunks, ctl, pl0, pl1, plp, g0,
+                           g1, gp, Fs, F, SB, fsl, partials, hist_next, fx);
+    }
+    (void)n_waves;
     return hipGetLastError();
 }
 
@@ -2908,7 +2909,7 @@ hipError_t set_hist_lds_limits()
                         reinterpret_cast<void const *>(&k_hist_q4<15, 2>), reinterpret_cast<void const *>(&k_hist_q4<16, 2>)};
     for (void const *fn : q4) raise_lds(fn, max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_hist_q2), max_shared);
-    raise_lds(reinterpret_cast<void const *>(&k_hist<false, 4>), max_shared);
+    raise_lds(reinterpret_cast<void const *>(&k_hist_fx), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_find<true, false>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_find<false, false>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_find<true, true>), max_shared);

@@ -7,8 +7,12 @@ namespace bh {
 
 // weight: per-row weights scaling g and h (null: unweighted).  obj 0 mse, 1 logloss,
 // 2 poisson.
+// fx (4 words): the FP32 histogram's fixed-point grids, s_g and s_h in fx[2..3], set from
+// these gradients' maxima (fx[0..1] are zero between passes).
 void launch_grad_f32(hipStream_t s, float const *score, float const *label, float const *weight, u64 n, int obj,
-                     float2 *gh);
+                     float2 *gh, u32 *fx);
+// The grids of host-supplied FP32 gradients.
+void launch_fx_from(hipStream_t s, float2 const *gh, u64 n, u32 *fx);
 // Quantized gradients of iteration ctl->iter with the max-abs in maxbits slot iter & 1
 // (two slots of two words); clears the other slot for the score update to fill.
 void launch_grad_quant(hipStream_t s, float const *score, float const *label, float const *weight, u64 n, int obj,
@@ -21,7 +25,7 @@ void launch_quant_from(hipStream_t s, float2 const *gh, u64 n, u32 *maxbits, u64
                        double *scales_dev);
 hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl const *ctl, u8 const *pl0,
                        u8 const *pl1, u8 const *plp, void const *g0, void const *g1, void const *gp, u32 Fs, u32 F,
-                       u32 SB, u32 fsl, u32 n_waves, u32 grid_x, void *partials, void *hist_next,
+                       u32 SB, u32 fsl, u32 n_waves, u32 grid_x, void *partials, void *hist_next, u32 const *fx,
                        u32 *cnt_next = nullptr, PartOp const *routes = nullptr, bool group = false);
 hipError_t set_hist_lds_limits();
 // F when launch_hist with these arguments runs the rotated row-in-lane kernel, whose unit

@@ -1114,7 +1114,7 @@ void enqueue_levelwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
     launch_tree_begin(st, v.ctl, v.front[0], v.tree, v.hch, v.fops, v.root);
     hipLaunchKernelGGL(k_obl_begin, dim3(1), dim3(64), 0, st, v.ctl);
     launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
-                v.fsl, v.waves, std::max(1u, std::min(v.max_hch, 4096u)), v.partials, v.hist[0]);
+                v.fsl, v.waves, std::max(1u, std::min(v.max_hch, 4096u)), v.partials, v.hist[0], v.fx);
     launch_finish(st, q, v.fops, v.ctl, v.partials, v.hist[1], v.hist[0], v.F * v.SB, 1, nullptr, nullptr, false,
                   hist_transpose_F(q, v.Fs, v.F));
     u32 const TN = q ? kLevelTileQ : kLevelTileF32;
@@ -1154,7 +1154,7 @@ void enqueue_levelwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
                                 v.work[0], v.work[1], const_cast<void *>(v.gh[0]), const_cast<void *>(v.gh[1]), v.Fs,
                                 gh_bytes, std::max(1u, std::min(v.max_pch, tuning().part_grid)));
             launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F,
-                        v.SB, v.fsl, v.waves, std::max(1u, std::min(v.max_hch, 4096u)), v.partials, hnext);
+                        v.SB, v.fsl, v.waves, std::max(1u, std::min(v.max_hch, 4096u)), v.partials, hnext, v.fx);
             launch_finish(st, q, v.fops, v.ctl, v.partials, hcur, hnext, v.F * v.SB, std::min(width, v.max_ops),
                           nullptr, nullptr, false, hist_transpose_F(q, v.Fs, v.F));
         }
@@ -1230,7 +1230,7 @@ void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
     hipLaunchKernelGGL(k_lw_begin, dim3(1), dim3(256), 0, st, v.ctl, lw, kids, v.tree, v.hch, v.fops, s.free_slots,
                        s.hist_slots, v.root, v.lw_units);
     launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
-                v.fsl, v.waves, hgrid, v.partials, pool, cnt);
+                v.fsl, v.waves, hgrid, v.partials, pool, v.fx, cnt);
     launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1, cnt, cnt, false,
                       hist_transpose_F(q, v.Fs, v.F));
     launch_find(st, q, kids, v.ctl, pool, v.d_nbins, v.F, v.SB, v.gc, v.scales, v.splits, 2, 0, 0, v.fscr, cnt);
@@ -1261,7 +1261,7 @@ void enqueue_leafwise(ExtState &s, hipStream_t st, bool q, GrowView const &v)
                                 const_cast<void *>(v.gh[1]), v.Fs, gh_bytes, pgrid);
         }
         launch_hist(st, q, v.hch, v.ctl, v.work[0], v.work[1], v.plane0, v.gh[0], v.gh[1], v.gh[2], v.Fs, v.F, v.SB,
-                    v.fsl, v.waves, hgrid, v.partials, pool, cnt);
+                    v.fsl, v.waves, hgrid, v.partials, pool, v.fx, cnt);
         launch_finish(st, q, v.fops, v.ctl, v.partials, pool, pool, v.F * v.SB, 1, cnt, cnt, false,
                       hist_transpose_F(q, v.Fs, v.F));
         LwNext const nx{s.cands, s.free_slots, parent, psplit, v.tree, v.ops, v.pch, v.hch, v.fops, v.lw_units,

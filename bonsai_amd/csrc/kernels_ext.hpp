@@ -27,6 +27,7 @@ struct GrowView {
     HistChunk *hch;
     FinishOp *fops;
     double *scales;
+    u32 const *fx;   // FP32 histogram grids (launch_hist)
     GrowCfg gc;
     bh_fit_config fc;
     u32 max_pch, max_hch, max_ops, fsl, waves;

@@ -24,7 +24,7 @@ NODE_DTYPE = np.dtype([
     ("cover", np.float32), ("gain", np.float32),
 ])
 
-HIST_F32_SERIAL, HIST_F32_CHUNKED, HIST_INT, HIST_F32_OMP = 0, 1, 2, 3
+HIST_F32_SERIAL, HIST_F32_CHUNKED, HIST_INT, HIST_F32_OMP, HIST_F32_FIXED = 0, 1, 2, 3, 4
 # Rows per device histogram work unit: the FP32 chunked order the device sums in.
 DEVICE_CHUNK = 8192
 CHUNK_ADAPTIVE = 0x80000000  # | unit target T: leaf-sized FP32 partials (oracle.h)
@@ -189,10 +189,23 @@ class Oracle:
                             P(np.ascontiguousarray(h, np.float32)), P(out))
         return out
 
+    def fx_shift(self, v) -> int:
+        """Fixed-point grid exponent of HIST_F32_FIXED for the values v (orc_fx_shift)."""
+        v = np.ascontiguousarray(v, np.float32)
+        return int(self.L.orc_fx_shift(P(v), C.c_uint64(v.size)))
+
     def hist_f32(self, bins, rows, g, h, bin_stride, mode=HIST_F32_SERIAL, chunk=DEVICE_CHUNK):
         n, f = bins.shape
         out = np.zeros((f, bin_stride, 2), np.float32)
         rows = np.ascontiguousarray(rows, np.uint32)
+        if mode == HIST_F32_FIXED:
+            # grids from every row's gradient, as the device takes them
+            sg, sh = self.fx_shift(g), self.fx_shift(h)
+            self.L.orc_hist_f32_fixed(P(bins), C.c_uint64(f), C.c_uint32(f), C.c_uint32(bin_stride), P(rows),
+                                      C.c_uint64(rows.size), P(np.ascontiguousarray(g, np.float32)),
+                                      P(np.ascontiguousarray(h, np.float32)), C.c_int32(sg), C.c_int32(sh),
+                                      C.c_uint32(chunk), P(out))
+            return out
         self.L.orc_hist_f32(P(bins), C.c_uint64(f), C.c_uint32(f), C.c_uint32(bin_stride), P(rows),
                             C.c_uint64(rows.size), P(np.ascontiguousarray(g, np.float32)),
                             P(np.ascontiguousarray(h, np.float32)), C.c_int32(mode), C.c_uint32(chunk), P(out))

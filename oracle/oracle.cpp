@@ -196,7 +196,22 @@ struct Grad {
     int16_t const *hq = nullptr;
     double        sg = 0.0;
     double        sh = 0.0;
+    int           fsg = 0; // fixed-point grids of the FP32 fixed fill
+    int           fsh = 0;
 };
+
+// The device's FP32 histogram numerics (ORC_HIST_F32_FIXED): every gradient on the grid
+// 2^-s with s = 49 - e, max|v| < 2^e, rounded to nearest even; a chunk of at most 8,192
+// rows sums exactly in int64 (|q| < 2^49) and each cell is rounded to float once.
+int fx_shift_of(float m)
+{
+    if (!(m > 0.0F) || !(m <= std::numeric_limits<float>::max())) return 0;
+    int e = 0;
+    (void)std::frexp(m, &e);
+    return 49 - e;
+}
+inline i64 fx_quant(float v, int s) { return static_cast<i64>(std::rint(std::ldexp(v, s))); }
+inline float fx_dequant(i64 q, int s) { return std::ldexp(static_cast<float>(q), -s); }
 
 // Serial float fill in row order: bit-identical to the reference's single-block fill
 // (tests/unit/test_populate.cpp:160-182).
@@ -229,6 +244,41 @@ void fill_f32_chunked(Dataset const &ds, Grad const &gr, u32 const *rows, u64 n,
         fill_f32_serial(ds, gr, rows + k0, k1 - k0, part.data());
         for (size_t i = 0; i < cells; ++i) out[i] += part[i];
     }
+}
+
+// The device's FP32 fill: per `chunk` rows exact fixed-point sums, each cell rounded
+// once to float, and the chunk partials folded in chunk order.
+void fill_f32_fixed(Dataset const &ds, Grad const &gr, u32 const *rows, u64 n, u32 chunk, float *out)
+{
+    u32 const F = ds.n_features, SB = ds.bin_stride;
+    size_t const cells = static_cast<size_t>(F) * SB * 2;
+    std::vector<i64> acc(cells);
+    std::fill(out, out + cells, 0.0F);
+    for (u64 k0 = 0; k0 < n; k0 += chunk) {
+        u64 const k1 = std::min<u64>(n, k0 + chunk);
+        std::fill(acc.begin(), acc.end(), 0);
+        for (u64 k = k0; k < k1; ++k) {
+            u32 const r   = rows[k];
+            u8 const *row = ds.plane + static_cast<u64>(r) * ds.stride;
+            i64 const g = fx_quant(gr.g[r], gr.fsg), h = fx_quant(gr.h[r], gr.fsh);
+            for (u32 f = 0; f < F; ++f) {
+                i64 *c = acc.data() + (static_cast<u64>(f) * SB + row[f]) * 2;
+                c[0] += g;
+                c[1] += h;
+            }
+        }
+        for (size_t i = 0; i < cells; i += 2) {
+            out[i] += fx_dequant(acc[i], gr.fsg);
+            out[i + 1] += fx_dequant(acc[i + 1], gr.fsh);
+        }
+    }
+}
+
+int max_abs_shift(float const *v, u64 n)
+{
+    float m = 0.0F;
+    for (u64 i = 0; i < n; ++i) m = std::fmax(m, std::fabs(v[i]));
+    return fx_shift_of(m);
 }
 
 void fill_int(Dataset const &ds, Grad const &gr, u32 const *rows, u64 n, i64 *out)
@@ -639,6 +689,8 @@ struct Engine {
             n.hf.assign(cells(), 0.0F);
             if (mode == ORC_HIST_F32_CHUNKED)
                 fill_f32_chunked(ds, gr, n.rows.data(), n.rows.size(), chunk_for(n.rows.size()), n.hf.data());
+            else if (mode == ORC_HIST_F32_FIXED)
+                fill_f32_fixed(ds, gr, n.rows.data(), n.rows.size(), chunk_for(n.rows.size()), n.hf.data());
             else if (mode == ORC_HIST_F32_OMP)
                 fill_f32_omp(n.rows.data(), n.rows.size(), n.hf.data());
             else
@@ -810,7 +862,10 @@ struct Engine {
     // parent split's sums instead of a histogram build: the device planes' rule
     // (src/step/level.hpp:197-205 carries find's child sums).  Exact for integer cells;
     // the float-cell device emulation follows it, the serial reference fill does not.
-    bool leaves_from_split() const { return mode == ORC_HIST_F32_CHUNKED || mode == ORC_HIST_INT; }
+    bool leaves_from_split() const
+    {
+        return mode == ORC_HIST_F32_CHUNKED || mode == ORC_HIST_F32_FIXED || mode == ORC_HIST_INT;
+    }
     void totals_from_split(NodeState &L, NodeState &R, Split const &s) const
     {
         L.tg = static_cast<float>(s.gL);
@@ -1155,6 +1210,13 @@ Engine make_engine(uint8_t const *plane, uint64_t row_stride, uint64_t n_rows,
     e.mode = hist_mode;
     e.chunk = chunk == 0 ? kDeviceChunk : chunk;
     e.n_threads = hist_mode == ORC_HIST_F32_OMP ? omp_get_max_threads() : 1;
+    if (hist_mode == ORC_HIST_F32_FIXED) {
+        // grids handed over in the scale slots (rows that are a subset of the device's),
+        // else from these rows' own maxima
+        bool const given = sg != 0.0 || sh != 0.0;
+        e.gr.fsg = given ? static_cast<int>(sg) : max_abs_shift(g, n_rows);
+        e.gr.fsh = given ? static_cast<int>(sh) : max_abs_shift(h, n_rows);
+    }
     return e;
 }
 
@@ -1613,6 +1675,21 @@ void orc_hist_f32(const uint8_t *plane, uint64_t row_stride, uint32_t n_features
     std::fill(out, out + static_cast<size_t>(n_features) * bin_stride * 2, 0.0F);
     if (mode == ORC_HIST_F32_CHUNKED) fill_f32_chunked(ds, gr, rows, n_rows, chunk, out);
     else fill_f32_serial(ds, gr, rows, n_rows, out);
+}
+
+int32_t orc_fx_shift(const float *v, uint64_t n) { return max_abs_shift(v, n); }
+
+void orc_hist_f32_fixed(const uint8_t *plane, uint64_t row_stride, uint32_t n_features,
+                        uint32_t bin_stride, const uint32_t *rows, uint64_t n_rows, const float *g,
+                        const float *h, int32_t sg, int32_t sh, uint32_t chunk, float *out)
+{
+    Dataset ds{plane, row_stride, 0, n_features, nullptr, bin_stride};
+    Grad gr;
+    gr.g = g;
+    gr.h = h;
+    gr.fsg = sg;
+    gr.fsh = sh;
+    fill_f32_fixed(ds, gr, rows, n_rows, chunk == 0 ? static_cast<u32>(kDeviceChunk) : chunk, out);
 }
 
 void orc_hist_int(const uint8_t *plane, uint64_t row_stride, uint32_t n_features,

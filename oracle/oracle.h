@@ -64,9 +64,23 @@ enum {
     ORC_HIST_F32_CHUNKED = 1,/* float cells: sequential per fixed row chunk, chunk partials
                                 folded in chunk order (the device's fixed-order reduction) */
     ORC_HIST_INT = 2,        /* int64 sums of int8/int16 quantized gradients (exact) */
-    ORC_HIST_F32_OMP = 3     /* float cells, OpenMP row blocks with per-thread partials
+    ORC_HIST_F32_OMP = 3,    /* float cells, OpenMP row blocks with per-thread partials
                                 reduced in block order (CPU baseline fill) */
+    ORC_HIST_F32_FIXED = 4   /* float cells: per fixed row chunk an exact fixed-point sum
+                                (grid 2^-s, s = orc_fx_shift of the gradients) rounded once
+                                to float, chunk partials folded in chunk order (the
+                                device's FP32 histogram) */
 };
+
+/* Fixed-point grid of ORC_HIST_F32_FIXED: s = 49 - e with max|v| < 2^e over v[0..n)
+ * (frexpf), 0 when every value is 0 or the maximum is not finite.  A value goes on the
+ * grid as rintf(ldexpf(v, s)) (|q| < 2^49, so 8,192 of them sum exactly in int64) and a
+ * chunk's sum comes back as ldexpf((float)sum, -s). */
+int32_t orc_fx_shift(const float *v, uint64_t n);
+/* ORC_HIST_F32_FIXED histogram of rows with the given grids (sg for g, sh for h). */
+void orc_hist_f32_fixed(const uint8_t *plane, uint64_t row_stride, uint32_t n_features,
+                        uint32_t bin_stride, const uint32_t *rows, uint64_t n_rows, const float *g,
+                        const float *h, int32_t sg, int32_t sh, uint32_t chunk, float *out);
 
 /* ---- bin mapping: src/bin_mapper.cpp ---- */
 /* Cuts for one column (n <= n_samples path: every non-NaN value), returns #cuts. */
@@ -144,7 +158,8 @@ uint64_t orc_partition(const uint8_t *plane, uint64_t row_stride, const uint32_t
 
 /* ---- growers: src/grower_impl.hpp ---- */
 /* Returns #nodes written (<= cap) or -1 on error.  hist_mode picks the cell type; for
- * ORC_HIST_INT gq/hq and the scales are read, otherwise g/h.  chunk: rows per
+ * ORC_HIST_INT gq/hq and the scales are read, otherwise g/h (ORC_HIST_F32_FIXED: nonzero
+ * scales are the fixed-point grids s_g, s_h, else orc_fx_shift of these rows' g and h).  chunk: rows per
  * ORC_HIST_F32_CHUNKED partial (0: 8192), or ORC_CHUNK_ADAPTIVE | T: each built node of
  * n rows uses ceil(ceil(n / T) / 256) * 256 rows per partial, clamped to [1024, 8192]
  * (the device's leafwise work units with unit target T). */

@@ -239,7 +239,7 @@ std::vector<orc_node> oracle_tree(Plane const &ds, bh_fit_config const &c, std::
     uint32_t const chunk = c.grower == BH_LEAFWISE ? (0x80000000u | 256u) : 8192u;
     n = fn(ds.bins.data(), ds.n_features, ds.n_rows, ds.n_features, ds.n_bins.data(), q ? nullptr : g->data(),
            q ? nullptr : h->data(), q ? gq.data() : nullptr, q ? hq.data() : nullptr, q ? sc[0] : 0.0,
-           q ? sc[1] : 0.0, &oc, q ? ORC_HIST_INT : ORC_HIST_F32_CHUNKED, chunk, out.data(),
+           q ? sc[1] : 0.0, &oc, q ? ORC_HIST_INT : ORC_HIST_F32_FIXED, chunk, out.data(),
            static_cast<int32_t>(out.size()), nullptr);
     if (n < 0) throw std::runtime_error("oracle grower failed");
     out.resize(static_cast<size_t>(n));

@@ -8,7 +8,7 @@ histograms.
 import numpy as np
 import pytest
 
-from oracle_lib import DEVICE_CHUNK, HIST_F32_CHUNKED, HIST_F32_SERIAL, HIST_INT, cfg as orc_cfg
+from oracle_lib import DEVICE_CHUNK, HIST_F32_FIXED, HIST_F32_SERIAL, HIST_INT, cfg as orc_cfg
 
 pytestmark = pytest.mark.gpu
 
@@ -88,9 +88,9 @@ def test_histogram_f32_fixed_order(hip, orc, tall):
     absg = orc.hist_f64(bins, rows, np.abs(g), np.abs(h), int(nb.max()))
     err = np.abs(d1.astype(np.float64) - ref64)
     assert np.all(err <= rows.size * 2 ** -24 * absg + 1e-30)
-    chunked = orc.hist_f32(bins, rows, g, h, int(nb.max()), HIST_F32_CHUNKED, DEVICE_CHUNK)
-    # The device sums each cell in ascending row order per DEVICE_CHUNK-row unit, units in
-    # order: the chunked oracle restates that order.
+    chunked = orc.hist_f32(bins, rows, g, h, int(nb.max()), HIST_F32_FIXED, DEVICE_CHUNK)
+    # The device sums each DEVICE_CHUNK-row unit exactly on a fixed-point grid and rounds
+    # each cell once to float, units folded in order: the FIXED oracle restates that.
     assert np.array_equal(d1.view(np.uint32), chunked.view(np.uint32))
 
 
@@ -182,7 +182,7 @@ def test_grow_depthwise_f32_matches_device_order_oracle(hip, orc, tall, depth):
     ctx = ctx_for(hip, bins, nb, quantized=False, max_depth=depth)
     ctx.set_gradients(g, h)
     nodes, lor = ctx.grow_tree()
-    ref, rlor = orc.grow_depthwise(bins, nb, g=g, h=h, c=orc_cfg(max_depth=depth), mode=HIST_F32_CHUNKED)
+    ref, rlor = orc.grow_depthwise(bins, nb, g=g, h=h, c=orc_cfg(max_depth=depth), mode=HIST_F32_FIXED)
     assert_same_tree(nodes, ref)
     assert np.array_equal(lor, rlor)
     # against the serial reference fill: same routing within the reference's 1e-4 band

@@ -18,7 +18,7 @@ gains and the per-row leaf are compared bit for bit in both precisions.
 import numpy as np
 import pytest
 
-from oracle_lib import CHUNK_ADAPTIVE, HIST_F32_CHUNKED, HIST_INT, NODE_DTYPE, cfg as orc_cfg
+from oracle_lib import CHUNK_ADAPTIVE, HIST_F32_FIXED, HIST_INT, NODE_DTYPE, cfg as orc_cfg
 from test_gpu_depthwise import assert_same_tree, ctx_for, grad_of, tall_data
 from test_gpu_growers import levelwise_view
 
@@ -130,7 +130,7 @@ def test_depthwise_steps_match_the_oracle(hip, orc, tall, quantized, depth):
     else:
         ctx.set_gradients(g, h)
         sc = (0.0, 0.0)
-        ref, rlor = orc.grow_depthwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_CHUNKED)
+        ref, rlor = orc.grow_depthwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_FIXED)
     tree, ids, values = steps_depthwise(ctx, orc, c, Totals(quantized, sc))
     coarse, clor = ctx.grow_tree()
     ctx.close()
@@ -205,7 +205,7 @@ def test_levelwise_steps_match_the_oracle(hip, orc, tall, quantized, per_node):
     else:
         ctx.set_gradients(g, h)
         sc = (0.0, 0.0)
-        ref = orc.grow_levelwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_CHUNKED)
+        ref = orc.grow_levelwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_FIXED)
     levels, leaves, ids = steps_levelwise(ctx, orc, c, Totals(quantized, sc), per_node)
     ctx.close()
     assert np.array_equal(levels, ref[0].astype(np.int64))
@@ -275,7 +275,7 @@ def test_leafwise_steps_match_the_oracle(hip, orc, tall, quantized, leaves, dept
         ctx.set_gradients(g, h)
         sc = (0.0, 0.0)
         # the steps build with 8,192-row units (the fixed-chunk order)
-        ref, rlor = orc.grow_leafwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_CHUNKED)
+        ref, rlor = orc.grow_leafwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_FIXED)
     tree, ids = steps_leafwise(ctx, orc, c, Totals(quantized, sc))
     ctx.close()
     assert_same_tree(tree, ref)

@@ -6,7 +6,7 @@ against the oracle restating the device's fixed summation order.
 import numpy as np
 import pytest
 
-from oracle_lib import CHUNK_ADAPTIVE, HIST_F32_CHUNKED, HIST_INT, cfg as orc_cfg
+from oracle_lib import CHUNK_ADAPTIVE, HIST_F32_FIXED, HIST_INT, cfg as orc_cfg
 from test_gpu_depthwise import assert_same_tree, ctx_for, grad_of, tall_data
 
 pytestmark = pytest.mark.gpu
@@ -58,7 +58,7 @@ def test_grow_leafwise_f32_matches_device_order_oracle(hip, orc, tall, leaves, d
     ctx.set_gradients(g, h)
     nodes, lor = ctx.grow_tree()
     ref, rlor = orc.grow_leafwise(bins, nb, g=g, h=h, c=orc_cfg(max_depth=depth, max_leaves=leaves),
-                                  mode=HIST_F32_CHUNKED, chunk=LW_F32_CHUNK)
+                                  mode=HIST_F32_FIXED, chunk=LW_F32_CHUNK)
     assert_same_tree(nodes, ref)
     assert np.array_equal(lor, rlor)
 
@@ -83,7 +83,7 @@ def test_grow_leafwise_600k_rows(hip, orc, tall600k, quantized, leaves, depth):
         ref, rlor = orc.grow_leafwise(bins, nb, gq=gq, hq=hq, scales=sc, c=c, mode=HIST_INT)
     else:
         ctx.set_gradients(g, h)
-        ref, rlor = orc.grow_leafwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_CHUNKED, chunk=LW_F32_CHUNK)
+        ref, rlor = orc.grow_leafwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_FIXED, chunk=LW_F32_CHUNK)
     nodes, lor = ctx.grow_tree()
     ctx.close()
     assert_same_tree(nodes, ref)
@@ -110,7 +110,7 @@ def check_levelwise(hip, orc, bins, nb, g, h, depth, quantized):
                                                          c=orc_cfg(max_depth=depth), mode=HIST_INT)
     else:
         levels, lv, lc, gains, rlor = orc.grow_levelwise(bins, nb, g=g, h=h, c=orc_cfg(max_depth=depth),
-                                                         mode=HIST_F32_CHUNKED)
+                                                         mode=HIST_F32_FIXED)
     D = levels.shape[0]
     assert len(nodes) == (2 << D) - 1
     dl, dv, dc = levelwise_view(nodes, D)
@@ -154,7 +154,7 @@ def test_grow_depthwise_wide_rows(hip, orc, wide, quantized):
         ref, rlor = orc.grow_depthwise(bins, nb, gq=gq, hq=hq, scales=sc, c=orc_cfg(max_depth=5), mode=HIST_INT)
     else:
         ctx.set_gradients(g, h)
-        ref, rlor = orc.grow_depthwise(bins, nb, g=g, h=h, c=orc_cfg(max_depth=5), mode=HIST_F32_CHUNKED)
+        ref, rlor = orc.grow_depthwise(bins, nb, g=g, h=h, c=orc_cfg(max_depth=5), mode=HIST_F32_FIXED)
     nodes, lor = ctx.grow_tree()
     assert_same_tree(nodes, ref)
     assert np.array_equal(lor, rlor)
@@ -211,7 +211,7 @@ def test_grow_leafwise_wide_rows(hip, orc, quantized):
         ref, rlor = orc.grow_leafwise(bins, nb, gq=gq, hq=hq, scales=sc, c=c, mode=HIST_INT)
     else:
         ctx.set_gradients(g, h)
-        ref, rlor = orc.grow_leafwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_CHUNKED, chunk=LW_F32_CHUNK)
+        ref, rlor = orc.grow_leafwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_FIXED, chunk=LW_F32_CHUNK)
     nodes, lor = ctx.grow_tree()
     ctx.close()
     assert_same_tree(nodes, ref)

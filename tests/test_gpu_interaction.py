@@ -6,7 +6,7 @@ path stays inside a group, and the levelwise grower and out-of-range ids are ref
 import numpy as np
 import pytest
 
-from oracle_lib import HIST_F32_CHUNKED, HIST_INT, cfg as orc_cfg
+from oracle_lib import HIST_F32_FIXED, HIST_INT, cfg as orc_cfg
 from test_gpu_depthwise import assert_same_tree, ctx_for, grad_of, tall_data
 from test_gpu_growers import LW_F32_CHUNK
 from test_oracle_interaction import violations
@@ -29,8 +29,8 @@ def oracle_tree(orc, grower, bins, nb, g, h, gq, hq, sc, quantized, c):
         if quantized:
             return fn(bins, nb, gq=gq, hq=hq, scales=sc, c=c, mode=HIST_INT)
         if grower == "hip_leafwise":
-            return fn(bins, nb, g=g, h=h, c=c, mode=HIST_F32_CHUNKED, chunk=LW_F32_CHUNK)
-        return fn(bins, nb, g=g, h=h, c=c, mode=HIST_F32_CHUNKED)
+            return fn(bins, nb, g=g, h=h, c=c, mode=HIST_F32_FIXED, chunk=LW_F32_CHUNK)
+        return fn(bins, nb, g=g, h=h, c=c, mode=HIST_F32_FIXED)
     finally:
         orc.set_interaction(None)
 

@@ -8,7 +8,7 @@ refuses constraints (src/grower_impl.hpp:583-588).
 import numpy as np
 import pytest
 
-from oracle_lib import HIST_F32_CHUNKED, HIST_INT, cfg as orc_cfg
+from oracle_lib import HIST_F32_FIXED, HIST_INT, cfg as orc_cfg
 from test_gpu_depthwise import assert_same_tree, ctx_for, grad_of, tall_data
 from test_gpu_growers import LW_F32_CHUNK
 from test_oracle_monotone import monotone_violations
@@ -33,10 +33,10 @@ def oracle_tree(orc, grower, bins, nb, g, h, gq, hq, sc, quantized, c):
         if grower == "hip_depthwise":
             if quantized:
                 return orc.grow_depthwise(bins, nb, gq=gq, hq=hq, scales=sc, c=c, mode=HIST_INT)
-            return orc.grow_depthwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_CHUNKED)
+            return orc.grow_depthwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_FIXED)
         if quantized:
             return orc.grow_leafwise(bins, nb, gq=gq, hq=hq, scales=sc, c=c, mode=HIST_INT)
-        return orc.grow_leafwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_CHUNKED, chunk=LW_F32_CHUNK)
+        return orc.grow_leafwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_FIXED, chunk=LW_F32_CHUNK)
     finally:
         orc.set_monotone(None)
 

@@ -8,7 +8,7 @@ restatement keeps.
 import numpy as np
 import pytest
 
-from oracle_lib import HIST_F32_CHUNKED, HIST_INT, cfg as orc_cfg
+from oracle_lib import HIST_F32_FIXED, HIST_INT, cfg as orc_cfg
 from test_gpu_depthwise import assert_same_tree, ctx_for, grad_of, tall_data
 from test_gpu_growers import LW_F32_CHUNK
 
@@ -44,9 +44,11 @@ def oracle_tree(orc, grower, bins, nb, rows, g, h, gq, hq, sc, quantized, c):
     fn = getattr(orc, GROWERS[grower])
     if quantized:
         return fn(bins[rows], nb, gq=gq[rows], hq=hq[rows], scales=sc, c=c, mode=HIST_INT)
+    # the device's fixed-point grids come from every row's gradients, not the subset's
+    grids = (orc.fx_shift(g), orc.fx_shift(h))
     if grower == "hip_leafwise":
-        return fn(bins[rows], nb, g=g[rows], h=h[rows], c=c, mode=HIST_F32_CHUNKED, chunk=LW_F32_CHUNK)
-    return fn(bins[rows], nb, g=g[rows], h=h[rows], c=c, mode=HIST_F32_CHUNKED)
+        return fn(bins[rows], nb, g=g[rows], h=h[rows], scales=grids, c=c, mode=HIST_F32_FIXED, chunk=LW_F32_CHUNK)
+    return fn(bins[rows], nb, g=g[rows], h=h[rows], scales=grids, c=c, mode=HIST_F32_FIXED)
 
 
 def grow_cfg(grower):

@@ -21,7 +21,7 @@ s + lr * leaf exactly.
 import numpy as np
 import pytest
 
-from oracle_lib import CHUNK_ADAPTIVE, HIST_F32_CHUNKED, HIST_INT, cfg as orc_cfg
+from oracle_lib import CHUNK_ADAPTIVE, HIST_F32_FIXED, HIST_INT, cfg as orc_cfg
 from test_gpu_depthwise import assert_same_tree, ctx_for, tall_data
 from test_gpu_growers import levelwise_view
 
@@ -93,13 +93,13 @@ def assert_codes_agree(gq, hq, sc, ogq, ohq, osc):
 def oracle_tree(orc, grower, bins, nb, depth, leaves, min_data=20, **grad):
     c = orc_cfg(max_depth=depth, max_leaves=leaves, min_data=min_data)
     if grower == "hip_depthwise":
-        mode = HIST_INT if "gq" in grad else HIST_F32_CHUNKED
+        mode = HIST_INT if "gq" in grad else HIST_F32_FIXED
         return orc.grow_depthwise(bins, nb, c=c, mode=mode, **grad)[0]
     if grower == "hip_leafwise":
         if "gq" in grad:
             return orc.grow_leafwise(bins, nb, c=c, mode=HIST_INT, **grad)[0]
-        return orc.grow_leafwise(bins, nb, c=c, mode=HIST_F32_CHUNKED, chunk=CHUNK_ADAPTIVE | 256, **grad)[0]
-    mode = HIST_INT if "gq" in grad else HIST_F32_CHUNKED
+        return orc.grow_leafwise(bins, nb, c=c, mode=HIST_F32_FIXED, chunk=CHUNK_ADAPTIVE | 256, **grad)[0]
+    mode = HIST_INT if "gq" in grad else HIST_F32_FIXED
     return orc.grow_levelwise(bins, nb, c=c, mode=mode, **grad)
 
 

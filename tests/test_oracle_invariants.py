@@ -5,12 +5,13 @@ forms, conservation laws and special cases, each chosen so that a plausible mist
 import ctypes as C
 import itertools
 import math
+from fractions import Fraction
 
 import numpy as np
 import pytest
 
-from oracle_lib import (CHUNK_ADAPTIVE, HIST_F32_CHUNKED, HIST_F32_SERIAL, HIST_INT, NODE_DTYPE, P, TreeCfg,
-                        cfg as orc_cfg)
+from oracle_lib import (CHUNK_ADAPTIVE, HIST_F32_CHUNKED, HIST_F32_FIXED, HIST_F32_SERIAL, HIST_INT, NODE_DTYPE, P,
+                        TreeCfg, cfg as orc_cfg)
 
 
 def small_plane(n=3000, f=5, nb=12, seed=0, miss=0.05):
@@ -69,6 +70,70 @@ def test_hist_f32_single_chunk_is_serial_order_and_error_bound(orc):
     absum = orc.hist_f64(bins, rows, np.abs(g), h, 12)
     bound = 5000 * np.finfo(np.float32).eps * absum
     assert np.all(np.abs(many - ref) <= bound + 1e-30)
+
+
+# ------------------------------------------------------- FP32 fixed-point unit sums
+def _f32_of_scaled(q: int, s: int) -> np.float32:
+    """q * 2^-s rounded once to float32 (nearest, ties to even), by integer arithmetic."""
+    if q == 0:
+        return np.float32(0.0)
+    a = abs(q)
+    shift = max(0, a.bit_length() - 24)
+    m, rem = a >> shift, a & ((1 << shift) - 1)
+    half = 1 << (shift - 1) if shift else 0
+    if shift and (rem > half or (rem == half and (m & 1))):
+        m += 1
+    return np.float32(math.copysign(math.ldexp(m, shift - s), q))
+
+
+def _fixed_cells(bins, rows, v, s, SB):
+    """Exact grid sums per (feature, bin): v on the grid 2^-s, rounded half to even."""
+    F = bins.shape[1]
+    acc = [[0] * SB for _ in range(F)]
+    for r in rows:
+        q = round(Fraction(float(v[r])) * (2 ** s) if s >= 0 else Fraction(float(v[r])) / (2 ** -s))
+        for f in range(F):
+            acc[f][int(bins[r, f])] += q
+    return acc
+
+
+def test_fx_shift_is_49_minus_the_max_exponent(orc):
+    """s = 49 - e with max|v| < 2^e: 1.0 -> 48, just below 1 -> 49, 0.25 -> 50, 3 -> 47,
+    the sign is ignored, an all-zero array gives 0."""
+    for vals, want in (([1.0], 48), ([0.999], 49), ([0.25, -0.1], 50), ([-3.0, 2.0], 47), ([0.0, 0.0], 0),
+                       ([1e-30], 49 - math.frexp(1e-30)[1]), ([7.5e8], 49 - 30)):
+        assert orc.fx_shift(np.array(vals, np.float32)) == want, vals
+
+
+def test_fixed_unit_is_the_correctly_rounded_exact_sum(orc):
+    """One unit: every cell is the exact sum of its rows' grid values rounded once to
+    float32 (tiny, huge-ratio and negative values included); several units: the rounded
+    unit cells added in unit order in float32.  A dropped rounding step, a wrong grid or
+    a fold out of order changes some cell."""
+    bins, nb = small_plane(n=2500, f=3, seed=40)
+    rng = np.random.default_rng(41)
+    n = bins.shape[0]
+    g = (rng.normal(0, 1, n) * 10.0 ** rng.uniform(-12, 0, n)).astype(np.float32)
+    h = rng.uniform(1e-9, 0.25, n).astype(np.float32)
+    rows = np.arange(n, dtype=np.uint32)
+    SB = 12
+    sg, sh = orc.fx_shift(g), orc.fx_shift(h)
+    for chunk in (8192, 1000):
+        got = orc.hist_f32(bins, rows, g, h, SB, HIST_F32_FIXED, chunk=chunk)
+        want = np.zeros((3, SB, 2), np.float32)
+        for k0 in range(0, n, chunk):
+            part = rows[k0:k0 + chunk]
+            ag, ah = _fixed_cells(bins, part, g, sg, SB), _fixed_cells(bins, part, h, sh, SB)
+            for f in range(3):
+                for b in range(SB):
+                    want[f, b, 0] = np.float32(want[f, b, 0] + _f32_of_scaled(ag[f][b], sg))
+                    want[f, b, 1] = np.float32(want[f, b, 1] + _f32_of_scaled(ah[f][b], sh))
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), chunk
+    # and it is at least as close to the FP64 sums as the serial float fill
+    ref = orc.hist_f64(bins, rows, g, h, SB)
+    ser = orc.hist_f32(bins, rows, g, h, SB, HIST_F32_SERIAL)
+    one = orc.hist_f32(bins, rows, g, h, SB, HIST_F32_FIXED, chunk=8192)
+    assert np.abs(one - ref).sum() <= np.abs(ser - ref).sum()
 
 
 # ----------------------------------------------------------------------------- quantizer
@@ -152,13 +217,14 @@ def test_depthwise_int_equals_f64_when_exact(orc):
     c = orc_cfg(max_depth=4, min_data=5)
     a, la = orc.grow_depthwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_SERIAL)
     b, lb = orc.grow_depthwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_CHUNKED, chunk=700)
+    x, lx = orc.grow_depthwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_FIXED, chunk=700)
     q, lq = orc.grow_depthwise(bins, nb, gq=gq, hq=hq, scales=(1.0, 1.0), c=c, mode=HIST_INT)
-    for t in (b, q):
+    for t in (b, x, q):
         assert np.array_equal(a["feature"], t["feature"])
         internal = a["feature"] >= 0
         assert np.array_equal(a["bin"][internal], t["bin"][internal])
         assert np.array_equal(a["value"], t["value"])
-    assert np.array_equal(la, lb) and np.array_equal(la, lq)
+    assert np.array_equal(la, lb) and np.array_equal(la, lx) and np.array_equal(la, lq)
 
 
 def test_levelwise_int_equals_f32_when_exact(orc):
@@ -175,9 +241,10 @@ def test_levelwise_int_equals_f32_when_exact(orc):
         c = orc_cfg(max_depth=depth, min_data=5)
         a = orc.grow_levelwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_SERIAL)
         q = orc.grow_levelwise(bins, nb, gq=gq, hq=hq, scales=(1.0, 1.0), c=c, mode=HIST_INT)
+        x = orc.grow_levelwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_FIXED)
         assert a[0].shape[0] == depth
-        for x, y in zip(a, q):
-            assert np.array_equal(x, y)
+        for u, v, w in zip(a, q, x):
+            assert np.array_equal(u, v) and np.array_equal(u, w)
 
 
 def test_leafwise_adaptive_chunks_keep_every_row(orc):
@@ -194,9 +261,10 @@ def test_leafwise_adaptive_chunks_keep_every_row(orc):
     c = orc_cfg(max_depth=8, min_data=5, max_leaves=40)
     a, la = orc.grow_leafwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_SERIAL)
     for target in (1, 4, 256):
-        b, lb = orc.grow_leafwise(bins, nb, g=g, h=h, c=c, mode=HIST_F32_CHUNKED, chunk=CHUNK_ADAPTIVE | target)
-        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
-        assert np.array_equal(la, lb)
+        for mode in (HIST_F32_CHUNKED, HIST_F32_FIXED):
+            b, lb = orc.grow_leafwise(bins, nb, g=g, h=h, c=c, mode=mode, chunk=CHUNK_ADAPTIVE | target)
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+            assert np.array_equal(la, lb)
 
 
 def leaf_groups(lor):
