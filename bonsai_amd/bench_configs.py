@@ -88,6 +88,18 @@ def config2(args) -> dict:
     ctx.upload_features(X, cuts)
     ctx.upload_labels(y)
     res = {"config": 2, "workload": "16M x 28 binary, hip_leafwise 255 leaves, depth 12", "iters": args.iters}
+    if args.find_blocks:
+        # workgroups per node of the round's split find, quantized fit
+        sweep = {}
+        for fb in (int(v) for v in args.find_blocks.split(",")):
+            ctx.configure(B.FitConfig(grower="hip_leafwise", objective="logloss", quantized=True, max_depth=12,
+                                      max_leaves=255, learning_rate=0.1, lambda_l2=1.0))
+            ctx.set_tuning(find_blocks=fb)
+            ctx.set_scores(np.full(n, init, np.float32))
+            ctx.train(1)
+            ctx.set_scores(np.full(n, init, np.float32))
+            sweep[fb] = round(timed_train(ctx, args.iters) / args.iters * 1e3, 3)
+        res["find_blocks_ms_per_iter"] = sweep
     for quantized in (True, False):
         ctx.configure(B.FitConfig(grower="hip_leafwise", objective="logloss", quantized=quantized, max_depth=12,
                                   max_leaves=255, learning_rate=0.1, lambda_l2=1.0))
@@ -256,6 +268,25 @@ def random_ensemble(n_trees: int, depth: int, n_features: int, n_bins: np.ndarra
     return trees
 
 
+def shap_path_work(trees):
+    """(paths, Σ_paths K(K+1)/2 + 3K) with K = distinct features on the root-to-leaf path + 1."""
+    n_paths, work = 0, 0
+    for t in trees:
+        stack = [(0, frozenset())]
+        while stack:
+            i, feats = stack.pop()
+            f = int(t["feature"][i])
+            if f < 0:
+                k = len(feats) + 1
+                n_paths += 1
+                work += k * (k + 1) // 2 + 3 * k
+                continue
+            nf = feats | {f}
+            stack.append((int(t["left"][i]), nf))
+            stack.append((int(t["right"][i]), nf))
+    return n_paths, work
+
+
 def config4(args) -> dict:
     X = synth.tall_features(200_000, config_id=0, stream=2)
     cuts = binning.fit_cuts(X, max_bin=256, n_samples=200_000)
@@ -273,6 +304,18 @@ def config4(args) -> dict:
     out = {"config": 4, "workload": "1,000 depth-8 trees, 28 features; 20,000 predict + TreeSHAP requests",
            "requests": n_req, "rows": int(rows.shape[0]), "offered_qps": args.qps,
            "size_median": float(np.median(sizes)), "size_max": int(sizes.max())}
+    # the TreeSHAP kernel alone on one 65,536-row batch (rocprof times k_shap itself):
+    # work = Σ over paths of K(K+1)/2 + 3K fused multiply-adds per row, K = the path's
+    # elements (its distinct features + 1)
+    n_paths, fma_per_row = shap_path_work(trees)
+    batch = pool[rng.integers(0, pool.shape[0], 65536)]
+    p.shap(batch)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        p.shap(batch)
+    dt = (time.perf_counter() - t0) / 5
+    out["shap_batch"] = {"rows": 65536, "paths": n_paths, "fma_per_row": fma_per_row,
+                         "flop_per_launch": 2 * fma_per_row * 65536, "wall_ms_incl_copies": round(dt * 1e3, 3)}
     for do_shap in (False, True):
         pred, shap, lat, st = p.serve(rows, offs, arrival, n_slots=args.slots, do_shap=do_shap,
                                       max_batch_rows=65536, hbm_budget=args.hbm_budget)
@@ -295,6 +338,7 @@ def main() -> int:
     ap.add_argument("--qps", type=float, default=2000.0)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--hbm-budget", type=int, default=2 << 30)
+    ap.add_argument("--find-blocks", default="", help="leafwise: comma list of find_blocks values to time (config 2)")
     a = ap.parse_args()
     if B.device_count() < 1:
         print("needs a HIP device", file=sys.stderr)

@@ -1429,7 +1429,7 @@ static int validate_tuning(bh_tuning const *t)
     if (t->hist_threads != 256u && t->hist_threads != 512u && t->hist_threads != 1024u)
         return fail(1, "hist_threads must be 256, 512 or 1024");
     if (t->part_grid < 64u || t->part_grid > 65536u) return fail(1, "part_grid must be in [64, 65536]");
-    if (t->hist_layout > 3u) return fail(1, "hist_layout must be 0, 1, 2 or 3");
+    if (t->hist_layout > 5u) return fail(1, "hist_layout must be 0 .. 5");
     if (t->pair_levels > 1u) return fail(1, "pair_levels must be 0 or 1");
     if (t->hist_group < 1u || t->hist_group > kHistGroupMax) return fail(1, "hist_group must be in [1, 4]");
     return 0;
@@ -1504,11 +1504,11 @@ int bh_autotune(bh_context *c, uint32_t iters, bh_tuning *chosen, float *ms_per_
     Tuning best = c->tune;
     float best_ms = 0.0F;
     int rc = trial(best, &best_ms);
-    static const u32 lds_c[] = {8192u, 12288u, 20480u, 32768u, 49152u, 98304u, 147456u};
+    static const u32 lds_c[] = {8192u, 12288u, 20480u, 32768u, 40992u, 49152u, 98304u, 147456u};
     static const u32 thr_c[] = {256u, 512u, 1024u};
     static const u32 fb_c[] = {1u, 2u, 4u, 8u};
     static const u32 pg_c[] = {512u, 1024u, 2048u, 4096u};
-    static const u32 hl_c[] = {0u, 1u, 2u, 3u};
+    static const u32 hl_c[] = {0u, 1u, 2u, 3u, 4u, 5u};
     static const u32 pl_c[] = {0u, 1u};
     static const u32 hg_c[] = {1u, 2u, 4u};
     for (int knob = 0; rc == 0 && knob < 7; ++knob) {
@@ -1519,7 +1519,7 @@ int bh_autotune(bh_context *c, uint32_t iters, bh_tuning *chosen, float *ms_per_
                           : knob == 4 ? hl_c
                           : knob == 5 ? pl_c
                                       : hg_c;
-        int const nc = knob == 0 ? 7 : knob == 1 || knob == 6 ? 3 : knob == 5 ? 2 : 4;
+        int const nc = knob == 0 ? 8 : knob == 1 || knob == 6 ? 3 : knob == 5 ? 2 : knob == 4 ? 6 : 4;
         for (int k = 0; rc == 0 && k < nc; ++k) {
             Tuning t = best;
             u32 &slot = knob == 0   ? t.scatter_lds

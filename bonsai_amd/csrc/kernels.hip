@@ -613,6 +613,128 @@ __global__ __launch_bounds__(kHqThreads) void k_hist_q(
     }
 }
 
+// Quantized histogram, rotated row-dword lanes (rows of at most 64 bytes, all features in
+// one slice, single 8192-row sub-units; same contract as k_hist_q otherwise).  Loads are
+// k_hist_q's: lane (r, d) reads dword d of row r, so one wave load is 64 / Dw whole rows
+// of contiguous bytes.  The LDS cells are feature-minor, cell (f, b) at b * FS + f, so a
+// cell's bank pair is a function of its feature alone, and atomic instruction k makes
+// lane (r, d) add byte (k + r) mod 4 of its dword: the lanes of one instruction then
+// cover every feature with about 64 / F lanes each instead of 64 / Dw rows of 4
+// features, so random bins no longer collide on banks.  Unit partials are written in
+// (bin, feature) order like k_hist_q4's (the completion kernel reads them transposed);
+// single-unit nodes write their slot directly.
+template <bool ROUTED>
+__device__ __forceinline__ void hist_q5_rows(unsigned long long *__restrict__ lh, u8 const *__restrict__ plane,
+                                             u32 const *__restrict__ ghv, u8 const *__restrict__ cb, u32 Fs,
+                                             u32 F, u32 FS, u32 d, u32 lr, u32 Dw, u32 r, u32 end, u32 stride,
+                                             HqRoute const &rt)
+{
+    u32 const rot = lr & 3u;
+    auto keep_row = [&](u32 dwv) -> bool {
+        if (!ROUTED) return true;
+        bool const mine = d == rt.lane_d &&
+                          routes_left((dwv >> rt.shift) & 0xFFu, rt.last, rt.bin, rt.dl) == (rt.want_left != 0);
+        u64 const m = __ballot(mine);
+        return ((m >> (lr * Dw + rt.lane_d)) & 1ull) != 0;
+    };
+    auto add = [&](u32 dwv, u32 v) {
+        u64 const pk = pack_q(v);
+#pragma unroll
+        for (u32 k = 0; k < 4; ++k) {
+            u32 const j = (k + rot) & 3u;
+            u32 const f = 4u * d + j;
+            if (f < F)
+                atomicAdd(lh + ((dwv >> (8u * j)) & 0xFFu) * FS + f, static_cast<unsigned long long>(pk));
+        }
+    };
+    for (; r + (kHqUnroll - 1) * stride < end; r += kHqUnroll * stride) {
+        u32 dw[kHqUnroll], v[kHqUnroll];
+#pragma unroll
+        for (int u = 0; u < kHqUnroll; ++u) {
+            u32 const rr = r + u * stride;
+            dw[u] = *reinterpret_cast<u32 const *>(cb + static_cast<size_t>(rr) * Fs);
+            v[u] = ghv[rr];
+        }
+#pragma unroll
+        for (int u = 0; u < kHqUnroll; ++u)
+            if (keep_row(dw[u])) add(dw[u], v[u]);
+    }
+    for (; r < end; r += stride) {
+        u32 const dw1 = *reinterpret_cast<u32 const *>(cb + static_cast<size_t>(r) * Fs);
+        u32 const v1 = ghv[r];
+        if (keep_row(dw1)) add(dw1, v1);
+    }
+}
+
+__global__ __launch_bounds__(kHqThreads) void k_hist_q5(
+    HistChunk const *__restrict__ chunks, TreeCtl const *__restrict__ ctl,
+    u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
+    void const *__restrict__ gw0, void const *__restrict__ gw1, void const *__restrict__ gwp,
+    u32 Fs, u32 F, u32 SB, u32 FS, void *__restrict__ partials, void *__restrict__ hist_next,
+    u32 *__restrict__ cnt_next, PartOp const *__restrict__ routes)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned long long *lh = reinterpret_cast<unsigned long long *>(smem); // SB x FS, feature-minor
+    u32 const Dw = Fs / 4u;
+    u32 const R = 64u / Dw;
+    u32 const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    u32 const NW = blockDim.x >> 6;
+    u32 const lr = lane / Dw, d = lane - lr * Dw;
+    bool const lane_ok = lr < R;
+    u32 const lcells = SB * FS;
+    u32 const n_units = ctl->n_hist_chunks;
+    size_t const node_cells = static_cast<size_t>(F) * SB;
+    for (u32 c = blockIdx.x; c < n_units; c += gridDim.x) {
+        HistChunk const ch = chunks[c];
+        for (u32 i = threadIdx.x; i < lcells; i += blockDim.x) lh[i] = 0ull;
+        __syncthreads();
+        u8 const *plane = ch.src == 0 ? pl0 : (ch.src == 1 ? pl1 : plp);
+        u32 const *ghv = static_cast<u32 const *>(ch.src == 0 ? gw0 : (ch.src == 1 ? gw1 : gwp));
+        u8 const *cb = plane + 4u * d;
+        u32 const stride = NW * R;
+        HqRoute rt{};
+        if (ch.route != kNone) {
+            PartOp const &op = routes[ch.route];
+            rt.feature = op.feature;
+            rt.bin = op.bin;
+            rt.dl = op.default_left;
+            rt.last = op.last_bin;
+            rt.want_left = ch.side == 0 ? 1u : 0u;
+            rt.in_slice = 1u;
+            rt.lane_d = op.feature >> 2;
+            rt.shift = 8u * (op.feature & 3u);
+        }
+        if (lane_ok) {
+            u32 const r = ch.begin + wave * R + lr;
+            if (ch.route != kNone)
+                hist_q5_rows<true>(lh, plane, ghv, cb, Fs, F, FS, d, lr, Dw, r, ch.end, stride, rt);
+            else
+                hist_q5_rows<false>(lh, plane, ghv, cb, Fs, F, FS, d, lr, Dw, r, ch.end, stride, rt);
+        }
+        __syncthreads();
+        u32 const out_cells = F * SB;
+        if (ch.part != kNone) {
+            u64 *P = static_cast<u64 *>(partials) + static_cast<size_t>(ch.part) * node_cells;
+            for (u32 i = threadIdx.x; i < out_cells; i += blockDim.x) {
+                u32 const b = i / F, f = i - b * F;
+                P[i] = lh[b * FS + f];
+            }
+        } else {
+            size_t const base = static_cast<size_t>(ch.slot) * node_cells;
+            i64 *dst = static_cast<i64 *>(hist_next) + base * 2;
+            for (u32 i = threadIdx.x; i < out_cells; i += blockDim.x) {
+                u32 const b = i / F, f = i - b * F;
+                u64 const v = lh[b * FS + f];
+                size_t const o = static_cast<size_t>(f) * SB + b;
+                dst[2 * o] = unpack_g(v);
+                dst[2 * o + 1] = unpack_h(v);
+                if (cnt_next) cnt_next[base + o] = unpack_cnt(v);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Quantized histogram, row-in-lane layout (rows of at most 64 bytes, one feature slice;
 // same contract as k_hist_q).  Lane l of a wave owns whole rows: dword k of 64
 // consecutive rows is one load instruction spanning 64 rows (the lines of a 64-row
@@ -1290,10 +1412,17 @@ __global__ __launch_bounds__(256) void k_part_count(PartChunk const *__restrict_
         PartOp const op = ops[ch.op];
         u8 const *plane = op.src == 0 ? pl0 : (op.src == 1 ? pl1 : plp);
         u32 cnt = 0;
-        for (u32 k = ch.begin + threadIdx.x; k < ch.end; k += blockDim.x) {
-            u32 const b = plane[static_cast<size_t>(op.off + k) * Fs + op.feature];
-            cnt += routes_left(b, op.last_bin, op.bin, op.default_left) ? 1u : 0u;
+        u8 const *col = plane + static_cast<size_t>(op.off) * Fs + op.feature;
+        u32 k = ch.begin + threadIdx.x;
+        for (; k + 7u * blockDim.x < ch.end; k += 8u * blockDim.x) {
+            u32 b[8];
+#pragma unroll
+            for (u32 u = 0; u < 8; ++u) b[u] = col[static_cast<size_t>(k + u * blockDim.x) * Fs];
+#pragma unroll
+            for (u32 u = 0; u < 8; ++u) cnt += routes_left(b[u], op.last_bin, op.bin, op.default_left) ? 1u : 0u;
         }
+        for (; k < ch.end; k += blockDim.x)
+            cnt += routes_left(col[static_cast<size_t>(k) * Fs], op.last_bin, op.bin, op.default_left) ? 1u : 0u;
         for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
         if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
         __syncthreads();
@@ -1735,14 +1864,18 @@ __global__ __launch_bounds__(256) void k_part_scatter_v(
 // Four-way partition (two levels of splits in one pass, PartOp4).  Row classes: 0 AL,
 // 1 AR, 2 BL, 3 BR, 4 = the row's child did not split (the row is not moved).
 // ---------------------------------------------------------------------------------
-__device__ __forceinline__ u32 route4(u8 const *row, PartOp4 const &q)
+__device__ __forceinline__ u32 route4_bins(u32 b0, u32 ba, u32 bb, PartOp4 const &q)
 {
-    if (routes_left(row[q.feature], q.last_bin, q.bin, q.default_left)) {
+    if (routes_left(b0, q.last_bin, q.bin, q.default_left)) {
         if (!(q.alive & 1u)) return 4u;
-        return routes_left(row[q.fa], q.la, q.ba, q.da) ? 0u : 1u;
+        return routes_left(ba, q.la, q.ba, q.da) ? 0u : 1u;
     }
     if (!(q.alive & 2u)) return 4u;
-    return routes_left(row[q.fb], q.lb, q.bb, q.db) ? 2u : 3u;
+    return routes_left(bb, q.lb, q.bb, q.db) ? 2u : 3u;
+}
+__device__ __forceinline__ u32 route4(u8 const *row, PartOp4 const &q)
+{
+    return route4_bins(row[q.feature], row[q.fa], row[q.fb], q);
 }
 
 // Rows per class of every partition chunk.
@@ -1760,13 +1893,28 @@ __global__ __launch_bounds__(256) void k_part_count4(PartChunk const *__restrict
         PartOp4 const q = ops4[ch.op];
         u8 const *plane = q.src == 0 ? pl0 : (q.src == 1 ? pl1 : plp);
         u32 k0 = 0, k1 = 0, k2 = 0, k3 = 0;
-        for (u32 k = ch.begin + threadIdx.x; k < ch.end; k += blockDim.x) {
-            u32 const cls = route4(plane + static_cast<size_t>(q.off + k) * Fs, q);
+        auto tally = [&](u32 cls) {
             k0 += cls == 0u;
             k1 += cls == 1u;
             k2 += cls == 2u;
             k3 += cls == 3u;
+        };
+        // four rows per thread with their three routing bytes loaded together
+        u32 k = ch.begin + threadIdx.x;
+        u8 const *base = plane + static_cast<size_t>(q.off) * Fs;
+        for (; k + 3u * blockDim.x < ch.end; k += 4u * blockDim.x) {
+            u32 b0[4], ba[4], bb[4];
+#pragma unroll
+            for (u32 u = 0; u < 4; ++u) {
+                u8 const *row = base + static_cast<size_t>(k + u * blockDim.x) * Fs;
+                b0[u] = row[q.feature];
+                ba[u] = row[q.fa];
+                bb[u] = row[q.fb];
+            }
+#pragma unroll
+            for (u32 u = 0; u < 4; ++u) tally(route4_bins(b0[u], ba[u], bb[u], q));
         }
+        for (; k < ch.end; k += blockDim.x) tally(route4(base + static_cast<size_t>(k) * Fs, q));
         for (int o = 32; o > 0; o >>= 1) {
             k0 += __shfl_xor(k0, o);
             k1 += __shfl_xor(k1, o);
@@ -2809,7 +2957,7 @@ void launch_quant_from(hipStream_t s, float2 const *gh, u64 n, u32 *maxbits, u64
 
 u32 hist_transpose_F(bool q, u32 Fs, u32 F, bool group)
 {
-    return q && !group && Fs <= 64u && t_tune.hist_layout == 3u ? F : 0u;
+    return q && !group && Fs <= 64u && (t_tune.hist_layout == 3u || t_tune.hist_layout == 5u) ? F : 0u;
 }
 
 hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl const *ctl, u8 const *pl0,
@@ -2823,7 +2971,13 @@ hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl c
     u32 const slices = (F + fsl - 1) / fsl;
     size_t const lds = static_cast<size_t>(fsl) * SB * 8;
     dim3 const grid(grid_x, slices), block(64 * n_waves);
-    if (hist_transpose_F(q, Fs, F, group)) {
+    if (hist_transpose_F(q, Fs, F, group) && t_tune.hist_layout == 5u) {
+        // rotated row-dword (single sub-units), every level of rows up to 64 bytes
+        u32 const FS = hq4_fs(Fs);
+        size_t const lds5 = static_cast<size_t>(FS) * SB * 8u;
+        hipLaunchKernelGGL(k_hist_q5, dim3(grid_x), dim3(t_tune.hist_threads), lds5, s, chunks, ctl, pl0, pl1, plp,
+                           g0, g1, gp, Fs, F, SB, FS, partials, hist_next, cnt_next, routes);
+    } else if (hist_transpose_F(q, Fs, F, group)) {
         // rotated row-in-lane (single sub-units), every level of rows up to 64 bytes
         u32 const Dw = Fs / 4u;
         size_t const lds4 = static_cast<size_t>(hq4_fs(Fs)) * SB * 8u;
@@ -2839,7 +2993,7 @@ hipError_t launch_hist(hipStream_t s, bool q, HistChunk const *chunks, TreeCtl c
         default: break;
         }
 #undef BH_HQ4
-    } else if (q && Fs <= 64u && (routes || t_tune.hist_layout == 2u)) {
+    } else if (q && Fs <= 64u && ((routes && t_tune.hist_layout != 4u) || t_tune.hist_layout == 2u)) {
         // row-in-lane: whole rows per lane, instantiated per row width in dwords.  Routed
         // levels always use it (a filtered row skips all its atomics at once).
         u32 const Dw = Fs / 4u;
@@ -2909,6 +3063,7 @@ hipError_t set_hist_lds_limits()
                         reinterpret_cast<void const *>(&k_hist_q4<15, 2>), reinterpret_cast<void const *>(&k_hist_q4<16, 2>)};
     for (void const *fn : q4) raise_lds(fn, max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_hist_q2), max_shared);
+    raise_lds(reinterpret_cast<void const *>(&k_hist_q5), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_hist_fx), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_find<true, false>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_find<false, false>), max_shared);

@@ -360,8 +360,10 @@ typedef struct bh_tuning {
     uint32_t hist_layout;       /* quantized histogram: 0 row-dword lanes (default), 1 feature-in-lane (rows <= 32
                                    bytes), 2 row-in-lane (rows <= 64 bytes), 3 rotated row-in-lane with
                                    feature-minor LDS cells (rows <= 64 bytes, single-sub-unit levels; every
-                                   level, routed ones included); with 0-2, levels whose units filter their
-                                   parent's rows use 2 whenever rows are <= 64 bytes */
+                                   level, routed ones included), 4 row-dword lanes on every level, routed
+                                   ones included, 5 rotated row-dword lanes with feature-minor LDS cells
+                                   (as 3 otherwise); with 0-2, levels whose units filter their parent's
+                                   rows use 2 whenever rows are <= 64 bytes */
     uint32_t pair_levels;       /* quantized depthwise, rows <= 64 bytes: 1 = move rows every second level
                                    (four-way partitions; the levels between scan parent segments) */
     uint32_t hist_group;        /* quantized depthwise: 8192-row sub-units per histogram work unit on levels

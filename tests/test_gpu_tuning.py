@@ -53,6 +53,18 @@ SHAPES = [
          hist_group=1),
     dict(scatter_lds_bytes=32768, find_blocks=4, hist_threads=256, part_grid=4096, hist_layout=3, pair_levels=1,
          hist_group=2),
+    # row-dword lanes on the routed levels too
+    dict(scatter_lds_bytes=32768, find_blocks=8, hist_threads=512, part_grid=4096, hist_layout=4, pair_levels=1,
+         hist_group=1),
+    dict(scatter_lds_bytes=32768, find_blocks=8, hist_threads=1024, part_grid=4096, hist_layout=4, pair_levels=1,
+         hist_group=2),
+    # rotated row-dword lanes, feature-minor cells, transposed partials
+    dict(scatter_lds_bytes=32768, find_blocks=8, hist_threads=512, part_grid=4096, hist_layout=5, pair_levels=1,
+         hist_group=1),
+    dict(scatter_lds_bytes=32768, find_blocks=8, hist_threads=256, part_grid=4096, hist_layout=5, pair_levels=0,
+         hist_group=1),
+    dict(scatter_lds_bytes=32768, find_blocks=8, hist_threads=1024, part_grid=4096, hist_layout=5, pair_levels=1,
+         hist_group=2),
 ]
 
 
@@ -88,7 +100,7 @@ def test_tuning_validation(hip, tall):
     with pytest.raises(hip.ConfigError):
         ctx.set_tuning(pair_levels=2)
     with pytest.raises(hip.ConfigError):
-        ctx.set_tuning(hist_layout=4)
+        ctx.set_tuning(hist_layout=6)
     with pytest.raises(hip.ConfigError):
         ctx.set_tuning(hist_group=5)
     ctx.close()
