@@ -1677,6 +1677,14 @@ void orc_hist_f32(const uint8_t *plane, uint64_t row_stride, uint32_t n_features
     else fill_f32_serial(ds, gr, rows, n_rows, out);
 }
 
+// The device's FP32 histogram numerics (no counterpart in the reference, whose float
+// cells sum in row order; this build sums exactly on a fixed-point grid, each unit cell
+// rounded once).  Pinned by tests/test_oracle_invariants.py::test_fx_shift_is_49_minus_the_max_exponent
+// (closed form) and ::test_fixed_unit_is_the_correctly_rounded_exact_sum (every unit cell
+// against an independent exact computation in Python integers, the unit fold order, and
+// closer to FP64 than the serial fill), plus the integer-gradient grower agreements
+// (::test_depthwise_int_equals_f64_when_exact, ::test_levelwise_int_equals_f32_when_exact,
+// ::test_leafwise_adaptive_chunks_keep_every_row).
 int32_t orc_fx_shift(const float *v, uint64_t n) { return max_abs_shift(v, n); }
 
 void orc_hist_f32_fixed(const uint8_t *plane, uint64_t row_stride, uint32_t n_features,
