@@ -295,11 +295,14 @@ def main() -> int:
         cpu = cpu_baseline(bins, nb, y, a.cpu_iters)
         log(f"cpu baseline: {cpu['value']} s extrapolated on {cpu['cores']} threads")
 
-    traffic = traffic_ratio = None
+    # HBM traffic per histogram launch from the counter runs, against the algorithmic
+    # bytes of the same launches (built-node and scanned bases)
+    traffic = traffic_ratio = traffic_scanned_ratio = None
     if TRAFFIC.exists() and a.rows == 16_000_000:
         t = json.loads(TRAFFIC.read_text())
         traffic = t["traffic_bytes_per_launch"]
-        traffic_ratio = round(traffic / per_launch_bytes, 3)
+        traffic_ratio = round(traffic / t["built_bytes_per_launch"], 3)
+        traffic_scanned_ratio = round(traffic / t["scanned_bytes_per_launch"], 3)
 
     out = {
         "metric": METRIC,
@@ -341,6 +344,7 @@ def main() -> int:
             "wide_read_GBps": round(wide["read"][0], 1),
             "traffic": traffic,
             "traffic_over_algorithmic": traffic_ratio,
+            "traffic_over_scanned": traffic_scanned_ratio,
             "traffic_source": "profiles/hist_quantized_16Mx28_counters.md" if traffic else None,
         },
         "cpu_baseline": cpu,
