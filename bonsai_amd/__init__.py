@@ -159,6 +159,9 @@ def lib() -> C.CDLL:
         "bh_upload_labels": [P, P],
         "bh_upload_weights": [P, P],
         "bh_upload_features": [P, P, u64, u32, P, P],
+        "bh_upload_feature_columns": [P, P, u64, u32, P, P],
+        "bh_upload_features_device": [P, P, u64, u32, P, P],
+        "bh_gather_rows": [C.c_int, P, u64, u32, P, u64, P],
         "bh_download_bins": [P, P, u64],
         "bh_configure": [P, C.POINTER(FitConfigC)],
         "bh_set_scores": [P, P],
@@ -233,6 +236,81 @@ def _check(rc: int) -> None:
 
 def _ptr(a: np.ndarray | None):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _cut_tables(cuts: list) -> tuple[np.ndarray, np.ndarray]:
+    """Per-feature cut arrays as one flat float32 table and its u32 offsets."""
+    flat = np.ascontiguousarray(np.concatenate(cuts).astype(np.float32))
+    offs = np.zeros(len(cuts) + 1, np.uint32)
+    offs[1:] = np.cumsum([c.size for c in cuts])
+    return flat, offs
+
+
+def _hip_runtime() -> C.CDLL:
+    """The HIP runtime libbonsai_hip.so links (loaded with it, not a second copy)."""
+    lib()
+    return C.CDLL("libamdhip64.so.7", mode=os.RTLD_NOLOAD | os.RTLD_GLOBAL)
+
+
+class DeviceMatrix:
+    """A caller-owned, row-major float32 matrix in device memory: the reference's
+    DeviceMatrix (include/bonsai/cuda/histogram_engine.hpp:41-50), whose producer there
+    is a DLPack capsule.  Built from a host array it is allocated and filled through the
+    HIP runtime the library itself links, so the pointer is valid for the device ingest
+    calls; ``wrap`` adopts a pointer allocated elsewhere in that runtime (not freed here).
+    A torch tensor is not accepted: the torch wheel carries a HIP runtime of its own, and
+    in a process that has loaded this library torch reports no HIP device."""
+
+    def __init__(self, X: np.ndarray | None = None, device: int = 0):
+        self.ptr, self.n_rows, self.n_features, self.device, self._owned = 0, 0, 0, device, False
+        if X is None:
+            return
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        if X.ndim != 2:
+            raise ConfigError("a device matrix is 2-d")
+        rt = _hip_runtime()
+        p = C.c_void_p()
+        if rt.hipSetDevice(C.c_int(device)) or rt.hipMalloc(C.byref(p), C.c_size_t(max(X.nbytes, 4))):
+            raise DeviceError("device matrix allocation failed")
+        self.ptr, self._owned = p.value, True
+        if rt.hipMemcpy(C.c_void_p(self.ptr), C.c_void_p(X.ctypes.data), C.c_size_t(X.nbytes), C.c_int(1)):
+            self.close()
+            raise DeviceError("device matrix upload failed")
+        self.n_rows, self.n_features = X.shape
+
+    @classmethod
+    def wrap(cls, ptr: int, n_rows: int, n_features: int, device: int = 0) -> "DeviceMatrix":
+        m = cls(None, device)
+        m.ptr, m.n_rows, m.n_features = int(ptr), int(n_rows), int(n_features)
+        return m
+
+    def close(self) -> None:
+        if self._owned and self.ptr:
+            _hip_runtime().hipFree(C.c_void_p(self.ptr))
+        self.ptr, self._owned = 0, False
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gather_rows(X: DeviceMatrix, rows: np.ndarray | None) -> np.ndarray:
+    """Rows of a device matrix copied to the host (cuda_gather_rows,
+    src/cuda/histogram_engine.cu:473-493): X[rows] for the sample rows, or the whole
+    matrix when rows is None or empty."""
+    r = np.zeros(0, np.uint32) if rows is None else np.ascontiguousarray(rows, dtype=np.uint32)
+    out = np.empty((r.size if r.size else X.n_rows, X.n_features), np.float32)
+    _check(lib().bh_gather_rows(X.device, C.c_void_p(X.ptr), X.n_rows, X.n_features, _ptr(r) if r.size else None,
+                                r.size, _ptr(out)))
+    return out
 
 
 def copy_calibrate(nbytes: int = 1 << 30, reps: int = 20, device: int = 0) -> float:
@@ -319,11 +397,34 @@ class HipContext:
         n, f = X.shape
         if len(cuts) != f:
             raise ConfigError("one cut table per feature")
-        flat = np.ascontiguousarray(np.concatenate(cuts).astype(np.float32))
-        offs = np.zeros(f + 1, np.uint32)
-        offs[1:] = np.cumsum([c.size for c in cuts])
+        flat, offs = _cut_tables(cuts)
         _check(lib().bh_upload_features(self._h, _ptr(X), n, f, _ptr(flat), _ptr(offs)))
         self.n_rows, self.n_features = n, f
+        self.n_bins = np.diff(offs).astype(np.uint16)
+
+    def upload_feature_columns(self, columns: list, cuts: list) -> None:
+        """Bin per-feature float columns (host arrays of equal length) on the device."""
+        cols = [np.ascontiguousarray(c, dtype=np.float32) for c in columns]
+        f = len(cols)
+        if f == 0 or len(cuts) != f:
+            raise ConfigError("one cut table per column")
+        n = cols[0].size
+        if any(c.ndim != 1 or c.size != n for c in cols):
+            raise ConfigError("columns must be 1-d and of equal length")
+        flat, offs = _cut_tables(cuts)
+        ptrs = (C.c_void_p * f)(*[c.ctypes.data for c in cols])
+        _check(lib().bh_upload_feature_columns(self._h, ptrs, n, f, _ptr(flat), _ptr(offs)))
+        self.n_rows, self.n_features = n, f
+        self.n_bins = np.diff(offs).astype(np.uint16)
+
+    def upload_features_device(self, X: DeviceMatrix, cuts: list) -> None:
+        """Bin a device-resident matrix where it lies (cuda_ingest_device)."""
+        if len(cuts) != X.n_features:
+            raise ConfigError("one cut table per feature")
+        flat, offs = _cut_tables(cuts)
+        _check(lib().bh_upload_features_device(self._h, C.c_void_p(X.ptr), X.n_rows, X.n_features, _ptr(flat),
+                                               _ptr(offs)))
+        self.n_rows, self.n_features = X.n_rows, X.n_features
         self.n_bins = np.diff(offs).astype(np.uint16)
 
     def download_bins(self) -> np.ndarray:

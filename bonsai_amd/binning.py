@@ -81,19 +81,34 @@ def create_cuts(sample: np.ndarray, max_bin: int) -> np.ndarray:
     return np.asarray(out, np.float32)
 
 
-def fit_cuts(X: np.ndarray, max_bin: int = 255, n_samples: int = 200_000, seed: int = 0):
-    """Per-feature cuts for a row-major float matrix."""
-    n, f = X.shape
+def sample_rows(n: int, n_samples: int = 200_000, seed: int = 0) -> np.ndarray | None:
+    """Ascending sample rows the cuts are fitted on (None: every row)."""
     if n > n_samples:
-        rows = np.sort(np.random.default_rng(seed).choice(n, n_samples, replace=False))
-        S = X[rows]
-    else:
-        S = X
+        return np.sort(np.random.default_rng(seed).choice(n, n_samples, replace=False))
+    return None
+
+
+def cuts_of_sample(S: np.ndarray, max_bin: int) -> list[np.ndarray]:
     cuts = []
-    for j in range(f):
+    for j in range(S.shape[1]):
         col = S[:, j]
         cuts.append(create_cuts(col[~np.isnan(col)], max_bin))
     return cuts
+
+
+def fit_cuts(X: np.ndarray, max_bin: int = 255, n_samples: int = 200_000, seed: int = 0):
+    """Per-feature cuts for a row-major float matrix."""
+    rows = sample_rows(X.shape[0], n_samples, seed)
+    return cuts_of_sample(X if rows is None else X[rows], max_bin)
+
+
+def fit_cuts_device(X, max_bin: int = 255, n_samples: int = 200_000, seed: int = 0):
+    """fit_cuts for a device-resident matrix (bonsai_amd.DeviceMatrix): only the sample
+    rows are gathered on the device and copied to the host (cuda_gather_rows,
+    src/cuda/histogram_engine.cu:473-493).  Same cuts as fit_cuts on the same matrix."""
+    from bonsai_amd import gather_rows
+    rows = sample_rows(X.n_rows, n_samples, seed)
+    return cuts_of_sample(gather_rows(X, rows), max_bin)
 
 
 def transform(X: np.ndarray, cuts: list[np.ndarray]) -> np.ndarray:

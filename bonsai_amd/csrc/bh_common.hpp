@@ -38,6 +38,8 @@ constexpr u64 kPackCntMask = (1ull << 14) - 1ull;
 // cells); the group size is a tuning knob (1..kHistGroupMax).
 constexpr u32 kHistGroupMax = 4;
 constexpr u32 kPartChunk  = 4096;  // rows per partition work unit
+constexpr u32 kCursorStride = 16;  // u32 between the class cursors of four-way partitions (64 B)
+constexpr u32 kCursorMinOps = 8;   // four-way ops of a level from which tiles reserve their runs
 // Rows wider than 64 bytes: partition chunks shrink (powers of two, at least 64 rows) to
 // about this many bytes, so a level of wide rows still has many chunks.
 constexpr u64 kPartChunkBytes = 256u * 1024u;

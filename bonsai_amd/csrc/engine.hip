@@ -46,6 +46,18 @@ template <typename T> hipError_t dalloc(T **p, size_t count)
     return hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
 }
 
+// Device buffers of one ingest call, freed on every return path.
+template <typename T> struct DevScratch {
+    T *p = nullptr;
+    DevScratch() = default;
+    DevScratch(DevScratch const &) = delete;
+    DevScratch &operator=(DevScratch const &) = delete;
+    ~DevScratch()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
 u32 round_up4(u32 x) { return (x + 3u) & ~3u; }
 
 } // namespace
@@ -168,7 +180,8 @@ struct bh_context {
     FindScratch fscr{nullptr, nullptr};
     u32 *cnt[2] = {nullptr, nullptr}; // quantized depthwise: row count per (slot, feature, bin)
     PartOp4 *ops4 = nullptr;          // quantized depthwise: four-way partition ops
-    uint4 *chunk_cls = nullptr;       // and their per-chunk class counts
+    uint4 *chunk_cls = nullptr;       // and their per-chunk class counts,
+    u32 *cursors4 = nullptr;          // or their class cursors (kCursorStride apart)
     Tuning tune;
     DevNode *tree = nullptr;
     PartOp *ops = nullptr;
@@ -269,7 +282,7 @@ void free_fit(bh_context *c)
     void *ptrs[] = {c->gh0f, c->ghwf[0], c->ghwf[1], c->gh0q, c->ghwq[0], c->ghwq[1], c->maxbits, c->scales, c->fx,
                     c->hist[0], c->hist[1], c->partials, c->ctl, c->front[0], c->front[1], c->splits, c->tree,
                     c->ops, c->pch, c->chunk_left, c->hch, c->fops, c->trees_out, c->tree_sizes, c->leaf_rows,
-                    c->fscr.part, c->fscr.done, c->cnt[0], c->cnt[1], c->ops4, c->chunk_cls,
+                    c->fscr.part, c->fscr.done, c->cnt[0], c->cnt[1], c->ops4, c->chunk_cls, c->cursors4,
                     c->mono_dev,
                     c->bounds, c->groups_dev, c->paths};
     for (void *p : ptrs)
@@ -303,6 +316,7 @@ void free_fit(bh_context *c)
     c->cnt[0] = c->cnt[1] = nullptr;
     c->ops4 = nullptr;
     c->chunk_cls = nullptr;
+    c->cursors4 = nullptr;
     ext_free(c->ext);
     c->configured = false;
     c->trees_cap = c->trees_done = 0;
@@ -422,6 +436,7 @@ int alloc_fit(bh_context *c)
         HIP_TRY(dalloc(&c->cnt[1], cells));
         HIP_TRY(dalloc(&c->ops4, c->max_ops / 2 + 2));
         HIP_TRY(dalloc(&c->chunk_cls, c->max_pch));
+        HIP_TRY(dalloc(&c->cursors4, (static_cast<size_t>(c->max_ops) / 2 + 2) * 4u * kCursorStride));
     }
     if (c->quant && f.grower == BH_LEAFWISE) HIP_TRY(dalloc(&c->cnt[0], static_cast<size_t>(c->hist_slots) * nc));
     HIP_TRY(dalloc(&c->ctl, 1));
@@ -638,7 +653,7 @@ void enqueue_depthwise_counted(bh_context *c)
                     c->fscr, ccur);
         launch_plan_counted(s, cur, c->splits, c->ctl, c->tree, c->ops, c->pch, nxt, c->hch, c->fops, c->d_nbins,
                             c->gc, c->scales, dst, last ? 1u : 0u, part_chunk_rows(c->Fs), mode, c->ops4,
-                            mode == 1 ? unit_rows : kHistChunk);
+                            mode == 1 ? unit_rows : kHistChunk, c->cursors4);
         if (c->fc.host_sync) host_sync_point(c);
         if (!last) {
             if (mode == 0) {
@@ -650,14 +665,19 @@ void enqueue_depthwise_counted(bh_context *c)
                                     gh_bytes, part_grid(c), c->chunk_left);
                 ++moves;
             } else if (mode == 2) {
-                launch_part_count4(s, c->pch, c->ops4, c->ctl, c->work[0], c->work[1], c->plane0, c->Fs,
-                                   c->chunk_cls, part_grid(c));
+                // A level of 2^(d-1) parents: with few of them every tile would meet on
+                // the same cursors, so those levels count first (stable order); wider
+                // levels reserve their runs and skip the counting pass.
+                bool const reserve = width / 2u >= kCursorMinOps;
+                if (!reserve)
+                    launch_part_count4(s, c->pch, c->ops4, c->ctl, c->work[0], c->work[1], c->plane0, c->Fs,
+                                       c->chunk_cls, part_grid(c));
                 launch_part_scatter4(s, c->pch, c->ops4, c->ctl, c->work[0], c->work[1], c->plane0,
                                      static_cast<u32 const *>(gh_src(c, 0)), static_cast<u32 const *>(gh_src(c, 1)),
                                      static_cast<u32 const *>(gh_src(c, 2)), c->work[0], c->work[1],
                                      static_cast<u32 *>(const_cast<void *>(gh_src(c, 0))),
                                      static_cast<u32 *>(const_cast<void *>(gh_src(c, 1))), c->Fs, part_grid(c),
-                                     c->chunk_cls);
+                                     c->chunk_cls, reserve ? c->cursors4 : nullptr);
                 ++moves;
             }
             enqueue_build(c, hcur, hnext, std::min(width, c->max_ops), hist_grid(c), ccur, cnext, true,
@@ -1008,11 +1028,16 @@ int bh_upload_bins(bh_context *c, const uint8_t *bins, uint64_t n_rows, uint32_t
     return 0;
 }
 
-int bh_upload_features(bh_context *c, const float *X, uint64_t n_rows, uint32_t n_features, const float *cuts,
-                       const uint32_t *cut_offsets)
+// What every ingest arm shares: the cut tables are validated (ascending per feature is
+// the caller's contract, as for BinMapper cuts), the plane is (re)allocated and the tables
+// are uploaded.  Rows are chunked so that staging stays near kIngestChunkBytes.
+constexpr u64 kIngestChunkBytes = u64{64} << 20;
+static int begin_ingest(bh_context *c, uint64_t n_rows, uint32_t n_features, const float *cuts,
+                        const uint32_t *cut_offsets, DevScratch<float> &d_cuts, DevScratch<u32> &d_off)
 {
     if (!c) return fail(1, "null context");
     if (n_features == 0) return fail(1, "empty plane");
+    if (!cuts || !cut_offsets) return fail(1, "null cut tables");
     if (cut_offsets[0] != 0) return fail(1, "cut_offsets[0] must be 0");
     std::vector<u16> nb(n_features);
     for (u32 f = 0; f < n_features; ++f) {
@@ -1023,29 +1048,126 @@ int bh_upload_features(bh_context *c, const float *X, uint64_t n_rows, uint32_t 
     }
     if (int r = alloc_plane(c, n_rows, n_features, nb.data())) return r;
     u32 const n_cuts = cut_offsets[n_features];
-    float *d_cuts = nullptr, *d_x = nullptr;
-    u32 *d_off = nullptr;
-    u64 const chunk = std::min<u64>(n_rows, std::max<u64>(1, (u64{256} << 20) / (u64{4} * n_features)));
-    auto cleanup = [&] {
-        if (d_cuts) (void)hipFree(d_cuts);
-        if (d_off) (void)hipFree(d_off);
-        if (d_x) (void)hipFree(d_x);
-    };
-    hipError_t e = dalloc(&d_cuts, n_cuts);
-    if (e == hipSuccess) e = dalloc(&d_off, n_features + 1);
-    if (e == hipSuccess) e = dalloc(&d_x, chunk * n_features);
-    if (e == hipSuccess) e = hipMemcpy(d_cuts, cuts, n_cuts * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(d_off, cut_offsets, (n_features + 1) * sizeof(u32), hipMemcpyHostToDevice);
-    for (u64 r0 = 0; e == hipSuccess && r0 < n_rows; r0 += chunk) {
-        u64 const n = std::min(chunk, n_rows - r0);
-        e = hipMemcpy(d_x, X + r0 * n_features, n * n_features * sizeof(float), hipMemcpyHostToDevice);
-        if (e != hipSuccess) break;
-        launch_bin_rows(c->stream, d_x, n, n_features, c->Fs, d_cuts, d_off, n_cuts, c->plane0 + r0 * c->Fs);
-        e = hipGetLastError();
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    HIP_TRY(dalloc(&d_cuts.p, n_cuts));
+    HIP_TRY(dalloc(&d_off.p, n_features + 1));
+    HIP_TRY(hipMemcpy(d_cuts.p, cuts, n_cuts * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_off.p, cut_offsets, (n_features + 1) * sizeof(u32), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// X_dev must be device memory of `device` (the reference's DeviceMatrix is a cudaMalloc'd
+// buffer); anything else is refused before a kernel could read it.
+static int check_device_matrix(const float *X_dev, int device)
+{
+    hipPointerAttribute_t a{};
+    hipError_t const e = hipPointerGetAttributes(&a, X_dev);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(1, "X_dev is not device memory");
     }
-    cleanup();
+    if (a.type != hipMemoryTypeDevice || a.device != device) return fail(1, "X_dev is not device memory of the device");
+    return 0;
+}
+
+int bh_upload_features(bh_context *c, const float *X, uint64_t n_rows, uint32_t n_features, const float *cuts,
+                       const uint32_t *cut_offsets)
+{
+    if (!X) return fail(1, "null matrix");
+    DevScratch<float> d_cuts, d_x;
+    DevScratch<u32> d_off;
+    if (int r = begin_ingest(c, n_rows, n_features, cuts, cut_offsets, d_cuts, d_off)) return r;
+    u64 const chunk = std::min<u64>(n_rows, std::max<u64>(1, (u64{256} << 20) / (u64{4} * n_features)));
+    HIP_TRY(dalloc(&d_x.p, chunk * n_features));
+    for (u64 r0 = 0; r0 < n_rows; r0 += chunk) {
+        u64 const n = std::min(chunk, n_rows - r0);
+        HIP_TRY(hipMemcpyAsync(d_x.p, X + r0 * n_features, n * n_features * sizeof(float), hipMemcpyHostToDevice,
+                               c->stream));
+        launch_bin_rows(c->stream, d_x.p, n, n_features, c->Fs, d_cuts.p, d_off.p, cut_offsets[n_features],
+                        c->plane0 + r0 * c->Fs);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+int bh_upload_feature_columns(bh_context *c, const float *const *columns, uint64_t n_rows, uint32_t n_features,
+                              const float *cuts, const uint32_t *cut_offsets)
+{
+    if (!columns) return fail(1, "null columns");
+    for (u32 f = 0; f < n_features; ++f)
+        if (!columns[f]) return fail(1, "null column");
+    DevScratch<float> d_cuts, d_cols;
+    DevScratch<u32> d_off;
+    if (int r = begin_ingest(c, n_rows, n_features, cuts, cut_offsets, d_cuts, d_off)) return r;
+    // four columns (one dword of every row) per staged group
+    u64 const chunk = std::min<u64>(n_rows, kIngestChunkBytes / (4u * sizeof(float)));
+    HIP_TRY(dalloc(&d_cols.p, 4u * chunk));
+    for (u32 f0 = 0; f0 < n_features; f0 += 4u) {
+        u32 const G = std::min(4u, n_features - f0);
+        for (u64 r0 = 0; r0 < n_rows; r0 += chunk) {
+            u64 const n = std::min(chunk, n_rows - r0);
+            for (u32 g = 0; g < G; ++g)
+                HIP_TRY(hipMemcpyAsync(d_cols.p + g * chunk, columns[f0 + g] + r0, n * sizeof(float),
+                                       hipMemcpyHostToDevice, c->stream));
+            launch_bin_cols4(c->stream, d_cols.p, chunk, n, G, d_cuts.p, d_off.p, f0, c->Fs, c->plane0 + r0 * c->Fs);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(c->stream));
+        }
+    }
+    return 0;
+}
+
+int bh_upload_features_device(bh_context *c, const float *X_dev, uint64_t n_rows, uint32_t n_features,
+                              const float *cuts, const uint32_t *cut_offsets)
+{
+    if (!c) return fail(1, "null context");
+    if (!X_dev) return fail(1, "null matrix");
+    HIP_TRY(hipSetDevice(c->device));
+    if (int r = check_device_matrix(X_dev, c->device)) return r;
+    DevScratch<float> d_cuts;
+    DevScratch<u32> d_off;
+    if (int r = begin_ingest(c, n_rows, n_features, cuts, cut_offsets, d_cuts, d_off)) return r;
+    launch_bin_rows(c->stream, X_dev, n_rows, n_features, c->Fs, d_cuts.p, d_off.p, cut_offsets[n_features],
+                    c->plane0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int bh_gather_rows(int device, const float *X_dev, uint64_t n_rows, uint32_t n_features, const uint32_t *rows,
+                   uint64_t n_sample, float *out)
+{
+    if (!X_dev || !out) return fail(1, "null matrix or output");
+    if (n_rows == 0 || n_features == 0) return fail(1, "empty matrix");
+    HIP_TRY(hipSetDevice(device));
+    if (int r = check_device_matrix(X_dev, device)) return r;
+    if (n_sample == 0) { // the whole matrix
+        HIP_TRY(hipMemcpy(out, X_dev, n_rows * n_features * sizeof(float), hipMemcpyDeviceToHost));
+        return 0;
+    }
+    if (!rows) return fail(1, "null sample rows");
+    for (u64 i = 0; i < n_sample; ++i)
+        if (rows[i] >= n_rows) return fail(1, "sample row out of range");
+    u64 const chunk = std::min<u64>(n_sample, std::max<u64>(1, kIngestChunkBytes / (u64{4} * n_features)));
+    DevScratch<u32> d_rows;
+    DevScratch<float> d_out;
+    HIP_TRY(dalloc(&d_rows.p, chunk));
+    HIP_TRY(dalloc(&d_out.p, chunk * n_features));
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipError_t e = hipSuccess;
+    for (u64 i0 = 0; e == hipSuccess && i0 < n_sample; i0 += chunk) {
+        u64 const n = std::min(chunk, n_sample - i0);
+        e = hipMemcpyAsync(d_rows.p, rows + i0, n * sizeof(u32), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) break;
+        launch_gather_f32_rows(s, X_dev, d_rows.p, n, n_features, d_out.p);
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out + i0 * n_features, d_out.p, n * n_features * sizeof(float), hipMemcpyDeviceToHost,
+                               s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    (void)hipStreamDestroy(s);
     if (e != hipSuccess) return fail(-1, std::string("hip: ") + hipGetErrorString(e));
     return 0;
 }
@@ -1511,26 +1633,21 @@ int bh_autotune(bh_context *c, uint32_t iters, bh_tuning *chosen, float *ms_per_
     static const u32 hl_c[] = {0u, 1u, 2u, 3u, 4u, 5u};
     static const u32 pl_c[] = {0u, 1u};
     static const u32 hg_c[] = {1u, 2u, 4u};
-    for (int knob = 0; rc == 0 && knob < 7; ++knob) {
-        u32 const *cand = knob == 0   ? lds_c
-                          : knob == 1 ? thr_c
-                          : knob == 2 ? fb_c
-                          : knob == 3 ? pg_c
-                          : knob == 4 ? hl_c
-                          : knob == 5 ? pl_c
-                                      : hg_c;
-        int const nc = knob == 0 ? 8 : knob == 1 || knob == 6 ? 3 : knob == 5 ? 2 : knob == 4 ? 6 : 4;
-        for (int k = 0; rc == 0 && k < nc; ++k) {
+    struct Knob {
+        u32 Tuning::*field;
+        u32 const *cand;
+        int n;
+    };
+    static const Knob knobs[] = {{&Tuning::scatter_lds, lds_c, 8}, {&Tuning::hist_threads, thr_c, 3},
+                                 {&Tuning::find_blocks, fb_c, 4},  {&Tuning::part_grid, pg_c, 4},
+                                 {&Tuning::hist_layout, hl_c, 6},  {&Tuning::pair_levels, pl_c, 2},
+                                 {&Tuning::hist_group, hg_c, 3}};
+    for (Knob const &kn : knobs) {
+        for (int k = 0; rc == 0 && k < kn.n; ++k) {
             Tuning t = best;
-            u32 &slot = knob == 0   ? t.scatter_lds
-                        : knob == 1 ? t.hist_threads
-                        : knob == 2 ? t.find_blocks
-                        : knob == 3 ? t.part_grid
-                        : knob == 4 ? t.hist_layout
-                        : knob == 5 ? t.pair_levels
-                                    : t.hist_group;
-            if (slot == cand[k]) continue;
-            slot = cand[k];
+            u32 &slot = t.*kn.field;
+            if (slot == kn.cand[k]) continue;
+            slot = kn.cand[k];
             float ms = 0.0F;
             rc = trial(t, &ms);
             if (rc == 0 && ms < best_ms) {

@@ -1980,12 +1980,25 @@ __device__ uint4 chunk_prefix4(uint4 const *__restrict__ chunk_cls, u32 first, u
 // Narrow rows (Fs <= 64), LDS-staged like k_part_scatter_v: a tile is read with
 // aligned 16-byte loads, its rows are ranked per class with wave ballots (stable within
 // the class), and each class run is written as one contiguous destination range.
+//
+// CUR = false: a tile's runs start where k_part_count4's counts of the op's earlier
+// chunks and the chunk's earlier tiles end, so the partition is stable.
+// CUR = true (no counting pass; levels with many ops): the class sizes are the split
+// finder's child counts (PartOp4), and each tile reserves its four runs with one atomic
+// add per class on the op's cursors (kCursorStride u32 apart, zeroed by the planner).
+// Rows keep their order inside a tile, but the tiles of a class follow the order of
+// their reservations, which can change from run to run.  Only the quantized depthwise
+// grower moves rows this way, and nothing downstream observes the order inside a node's
+// segment: its histogram cells are exact integer sums, and scores are routed from the
+// pristine plane.  With one op every tile of the level meets on the same four cursors,
+// so the planner keeps CUR for levels whose ops spread the reservations.
+template <bool CUR>
 __global__ __launch_bounds__(256) void k_part_scatter4(
     PartChunk const *__restrict__ pch, PartOp4 const *__restrict__ ops4, TreeCtl *__restrict__ ctl,
     u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
     u32 const *__restrict__ gw0, u32 const *__restrict__ gw1, u32 const *__restrict__ gwp,
     u8 *__restrict__ out0, u8 *__restrict__ out1, u32 *__restrict__ go0, u32 *__restrict__ go1, u32 Fs,
-    u32 tile, u64 magic_d, uint4 const *__restrict__ chunk_cls)
+    u32 tile, u64 magic_d, uint4 const *__restrict__ chunk_cls, u32 *__restrict__ cursors)
 {
     // Tiles fit the register prefetch: tile * Fs + 16 <= kPfRows * 16 * 256 bytes and
     // tile * 4 + 16 <= kPfGh * 16 * 256 (checked by the launcher).
@@ -1994,6 +2007,7 @@ __global__ __launch_bounds__(256) void k_part_scatter4(
     uint4 *s_gh = reinterpret_cast<uint4 *>(smem + tile * Fs + 16u);  // tile * 4 + 16
     u16 *s_idx = reinterpret_cast<u16 *>(smem + tile * (Fs + 4u) + 32u); // 4 x tile
     __shared__ u32 s_wsum[16];
+    __shared__ u32 s_res[4];
     u32 const n = ctl->n_part_chunks;
     u32 const D = Fs / 4u;
     u32 const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -2007,8 +2021,12 @@ __global__ __launch_bounds__(256) void k_part_scatter4(
         u32 *gout = q.dst == 0 ? go0 : go1;
         // Class run starts inside the segment and the room each class has.
         u32 const n_ar = q.n_a - q.n_al, n_b = q.cnt - q.n_a, n_br = n_b - q.n_bl;
-        uint4 run = chunk_prefix4(chunk_cls, q.chunk_first, c, s_wsum);
-        uint4 const mine = chunk_cls[c];
+        uint4 run = make_uint4(0, 0, 0, 0), mine = make_uint4(0, 0, 0, 0);
+        if (!CUR) {
+            run = chunk_prefix4(chunk_cls, q.chunk_first, c, s_wsum);
+            mine = chunk_cls[c];
+        }
+        u32 *cur = cursors + static_cast<size_t>(ch.op) * 4u * kCursorStride;
         bool const fits = q.n_a <= q.cnt && q.n_al <= q.n_a && q.n_bl <= n_b && run.x + mine.x <= q.n_al &&
                           run.y + mine.y <= n_ar && run.z + mine.z <= q.n_bl && run.w + mine.w <= n_br;
         if (!fits) {
@@ -2064,11 +2082,21 @@ __global__ __launch_bounds__(256) void k_part_scatter4(
             }
             u32 const *sr = reinterpret_cast<u32 const *>(s_rows);
             u32 const *sg = reinterpret_cast<u32 const *>(s_gh);
+            if (CUR) {
+                if (threadIdx.x < 4u) {
+                    u32 const m = threadIdx.x;
+                    u32 const tm = m == 0u ? tr[0] : (m == 1u ? tr[1] : (m == 2u ? tr[2] : tr[3]));
+                    s_res[m] = tm ? atomicAdd(cur + m * kCursorStride, tm) : 0u;
+                }
+                __syncthreads();
+                run = make_uint4(s_res[0], s_res[1], s_res[2], s_res[3]);
+            }
             u32 const done[4] = {run.x, run.y, run.z, run.w};
             // The tile's runs stay inside their class ranges (they do whenever the
-            // chunk's counts matched its rows; checked again before any store).
-            bool const tile_fits = run.x + tr[0] <= q.n_al && run.y + tr[1] <= n_ar && run.z + tr[2] <= q.n_bl &&
-                                   run.w + tr[3] <= n_br;
+            // chunk's counts, or the split's child counts, matched its rows; checked
+            // before any store).
+            bool const tile_fits = static_cast<u64>(run.x) + tr[0] <= q.n_al && static_cast<u64>(run.y) + tr[1] <= n_ar &&
+                                   static_cast<u64>(run.z) + tr[2] <= q.n_bl && static_cast<u64>(run.w) + tr[3] <= n_br;
             if (!tile_fits && threadIdx.x == 0) atomicExch(&ctl->part_error, 1u);
             if (tile_fits) { // uniform across the block
                 size_t const a0 = base[0] + done[0], a1 = base[1] + done[1], a2 = base[2] + done[2],
@@ -2121,7 +2149,7 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
     DevNode *__restrict__ tree, PartOp *__restrict__ ops, PartChunk *__restrict__ pch, NodeInfo *__restrict__ next,
     HistChunk *__restrict__ hch, FinishOp *__restrict__ fops, u16 const *__restrict__ n_bins, GrowCfg cfg,
     double const *__restrict__ scales, u32 dst, u32 is_last, u32 chunk_rows, u32 mode, PartOp4 *__restrict__ ops4,
-    u32 unit_rows)
+    u32 unit_rows, u32 *__restrict__ cursors)
 {
     __shared__ u32 scratch[1024];
     __shared__ u32 s_base_nodes, s_valid, s_alive, s_chunks, s_units, s_parts;
@@ -2305,6 +2333,9 @@ __global__ __launch_bounds__(1024) void k_plan_counted(
             q.chunk_first = cfirst;
             q.chunk_count = nch;
             ops4[j] = q;
+            // the four class cursors of k_part_scatter4<true>
+#pragma unroll
+            for (u32 m = 0; m < 4; ++m) cursors[(static_cast<size_t>(j) * 4u + m) * kCursorStride] = 0u;
             if (j < kPlanLdsOps) s_cfirst[j] = cfirst;
         }
         __syncthreads();
@@ -2831,6 +2862,63 @@ __global__ __launch_bounds__(256) void k_bin_rows(float const *__restrict__ X, u
     }
 }
 
+// lower_bound of x over one feature's ascending cuts C[0..len), NaN -> the last bin.
+__device__ __forceinline__ u32 bin_of(float x, float const *C, u32 len)
+{
+    if (isnan(x)) return len - 1u;
+    u32 first = 0;
+    while (len > 0) {
+        u32 const half = len >> 1;
+        if (C[first + half] < x) {
+            first += half + 1u;
+            len -= half + 1u;
+        } else {
+            len = half;
+        }
+    }
+    return first;
+}
+
+// Column ingest (the reference's bin_col_kernel, src/cuda/detail/ingest_kernels.cuh:89-102,
+// one column per launch there): up to four columns of the features f0..f0+3 (f0 a
+// multiple of 4) arrive column-major in `cols` ([g][ld] floats, rows r0..r0+n of the
+// plane); each lane bins one row's four cells against the features' cuts (staged in
+// LDS) and writes them as one aligned dword, so the plane's bytes of a row group are
+// written whole instead of one strided byte per feature.  Bytes past n_features are 0.
+__global__ __launch_bounds__(256) void k_bin_cols4(float const *__restrict__ cols, u64 ld, u64 n, u32 G,
+                                                   float const *__restrict__ cuts, u32 const *__restrict__ coff,
+                                                   u32 f0, u32 Fs, u8 *__restrict__ plane)
+{
+    __shared__ float s_cuts[4 * 256];
+    __shared__ u32 s_off[5];
+    if (threadIdx.x <= G) s_off[threadIdx.x] = coff[f0 + threadIdx.x] - coff[f0];
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < s_off[G]; i += blockDim.x) s_cuts[i] = cuts[coff[f0] + i];
+    __syncthreads();
+    for (u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; r < n;
+         r += static_cast<u64>(gridDim.x) * blockDim.x) {
+        u32 word = 0;
+        for (u32 g = 0; g < G; ++g) {
+            u32 const b = bin_of(cols[g * ld + r], s_cuts + s_off[g], s_off[g + 1] - s_off[g]);
+            word |= b << (8u * g);
+        }
+        *reinterpret_cast<u32 *>(plane + r * Fs + f0) = word;
+    }
+}
+
+// Sample gather for the host cut fitting (gather_rows_kernel, ingest_kernels.cuh:76-86):
+// out[i][:] = X[rows[i]][:], one lane per output float.
+__global__ __launch_bounds__(256) void k_gather_f32_rows(float const *__restrict__ X, u32 const *__restrict__ rows,
+                                                         u64 n_out, u32 F, float *__restrict__ out)
+{
+    u64 const cells = n_out * F;
+    for (u64 i = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; i < cells;
+         i += static_cast<u64>(gridDim.x) * blockDim.x) {
+        u64 const r = i / F;
+        out[i] = X[static_cast<u64>(rows[r]) * F + (i - r * F)];
+    }
+}
+
 // Plain 16-byte-per-lane copy: the calibration kernel of known traffic (reads and
 // writes n * 16 bytes) against which HBM counters and bandwidth are read.
 __global__ __launch_bounds__(256) void k_copy_calibrate(uint4 const *__restrict__ src, uint4 *__restrict__ dst, u64 n)
@@ -2922,6 +3010,18 @@ void launch_bin_rows(hipStream_t s, float const *X, u64 n, u32 F, u32 Fs, float 
     u32 const grid = static_cast<u32>(std::min<u64>((total + 255) / 256, 8192));
     hipLaunchKernelGGL(k_bin_rows, dim3(grid), dim3(256), lds_cuts * 4u, s, X, n, F, Fs, cuts, coff, plane,
                        lds_cuts);
+}
+void launch_bin_cols4(hipStream_t s, float const *cols, u64 ld, u64 n, u32 G, float const *cuts, u32 const *coff,
+                      u32 f0, u32 Fs, u8 *plane)
+{
+    u32 const grid = static_cast<u32>(std::max<u64>(1, std::min<u64>((n + 255) / 256, 8192)));
+    hipLaunchKernelGGL(k_bin_cols4, dim3(grid), dim3(256), 0, s, cols, ld, n, G, cuts, coff, f0, Fs, plane);
+}
+void launch_gather_f32_rows(hipStream_t s, float const *X, u32 const *rows, u64 n_out, u32 F, float *out)
+{
+    u64 const cells = n_out * F;
+    u32 const grid = static_cast<u32>(std::max<u64>(1, std::min<u64>((cells + 255) / 256, 8192)));
+    hipLaunchKernelGGL(k_gather_f32_rows, dim3(grid), dim3(256), 0, s, X, rows, n_out, F, out);
 }
 void launch_grad_f32(hipStream_t s, float const *score, float const *label, float const *weight, u64 n, int obj,
                      float2 *gh, u32 *fx)
@@ -3069,7 +3169,8 @@ hipError_t set_hist_lds_limits()
     raise_lds(reinterpret_cast<void const *>(&k_find<false, false>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_find<true, true>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_find<false, true>), max_shared);
-    raise_lds(reinterpret_cast<void const *>(&k_part_scatter4), max_shared);
+    raise_lds(reinterpret_cast<void const *>(&k_part_scatter4<false>), max_shared);
+    raise_lds(reinterpret_cast<void const *>(&k_part_scatter4<true>), max_shared);
     raise_lds(reinterpret_cast<void const *>(&k_part_scatter_v), max_shared);
     return hipSuccess;
 }
@@ -3187,10 +3288,10 @@ u32 scatter_tile_rows(u32 Fs, u32 gh_bytes)
 void launch_plan_counted(hipStream_t s, NodeInfo const *front, SplitRec const *splits, TreeCtl *ctl, DevNode *tree,
                          PartOp *ops, PartChunk *pch, NodeInfo *next, HistChunk *hch, FinishOp *fops,
                          u16 const *n_bins, GrowCfg const &cfg, double const *scales, u32 dst, u32 is_last,
-                         u32 chunk_rows, u32 mode, PartOp4 *ops4, u32 unit_rows)
+                         u32 chunk_rows, u32 mode, PartOp4 *ops4, u32 unit_rows, u32 *cursors)
 {
     hipLaunchKernelGGL(k_plan_counted, dim3(1), dim3(1024), 0, s, front, splits, ctl, tree, ops, pch, next, hch, fops,
-                       n_bins, cfg, scales, dst, is_last, chunk_rows, mode, ops4, unit_rows);
+                       n_bins, cfg, scales, dst, is_last, chunk_rows, mode, ops4, unit_rows, cursors);
 }
 
 void launch_part_count4(hipStream_t s, PartChunk const *pch, PartOp4 const *ops4, TreeCtl const *ctl, u8 const *pl0,
@@ -3211,14 +3312,18 @@ u32 scatter4_tile_rows(u32 Fs)
 
 void launch_part_scatter4(hipStream_t s, PartChunk const *pch, PartOp4 const *ops4, TreeCtl *ctl, u8 const *pl0,
                           u8 const *pl1, u8 const *plp, u32 const *g0, u32 const *g1, u32 const *gp, u8 *o0,
-                          u8 *o1, u32 *go0, u32 *go1, u32 Fs, u32 grid, uint4 const *chunk_cls)
+                          u8 *o1, u32 *go0, u32 *go1, u32 Fs, u32 grid, uint4 const *chunk_cls, u32 *cursors)
 {
     u32 const D = Fs / 4u;
     u32 const tile = scatter4_tile_rows(Fs);
     size_t const lds = static_cast<size_t>(tile) * (Fs + 12u) + 32u;
     u64 const magic_d = ((1ull << 32) + D - 1u) / D;
-    hipLaunchKernelGGL(k_part_scatter4, dim3(grid), dim3(256), lds, s, pch, ops4, ctl, pl0, pl1, plp, g0, g1, gp, o0,
-                       o1, go0, go1, Fs, tile, magic_d, chunk_cls);
+    if (cursors)
+        hipLaunchKernelGGL(k_part_scatter4<true>, dim3(grid), dim3(256), lds, s, pch, ops4, ctl, pl0, pl1, plp, g0, g1,
+                           gp, o0, o1, go0, go1, Fs, tile, magic_d, chunk_cls, cursors);
+    else
+        hipLaunchKernelGGL(k_part_scatter4<false>, dim3(grid), dim3(256), lds, s, pch, ops4, ctl, pl0, pl1, plp, g0,
+                           g1, gp, o0, o1, go0, go1, Fs, tile, magic_d, chunk_cls, cursors);
 }
 
 

@@ -57,14 +57,17 @@ u32 scatter_tile_rows(u32 Fs, u32 gh_bytes);
 void launch_plan_counted(hipStream_t s, NodeInfo const *front, SplitRec const *splits, TreeCtl *ctl, DevNode *tree,
                          PartOp *ops, PartChunk *pch, NodeInfo *next, HistChunk *hch, FinishOp *fops,
                          u16 const *n_bins, GrowCfg const &cfg, double const *scales, u32 dst, u32 is_last,
-                         u32 chunk_rows, u32 mode = 0, PartOp4 *ops4 = nullptr, u32 unit_rows = kHistChunk);
+                         u32 chunk_rows, u32 mode = 0, PartOp4 *ops4 = nullptr, u32 unit_rows = kHistChunk,
+                         u32 *cursors = nullptr);
 // Four-way partition of grandparent segments (quantized depthwise, rows of at most 64
-// bytes): per-chunk class counts, then the LDS-staged scatter.
-void launch_part_count4(hipStream_t s, PartChunk const *pch, PartOp4 const *ops4, TreeCtl const *ctl, u8 const *pl0,
-                        u8 const *pl1, u8 const *plp, u32 Fs, uint4 *chunk_cls, u32 grid);
+// bytes): per-chunk class counts, then the LDS-staged scatter; with cursors (zeroed by
+// the planner), no counting pass and tiles reserve their class runs atomically.
 void launch_part_scatter4(hipStream_t s, PartChunk const *pch, PartOp4 const *ops4, TreeCtl *ctl, u8 const *pl0,
                           u8 const *pl1, u8 const *plp, u32 const *g0, u32 const *g1, u32 const *gp, u8 *o0,
-                          u8 *o1, u32 *go0, u32 *go1, u32 Fs, u32 grid, uint4 const *chunk_cls);
+                          u8 *o1, u32 *go0, u32 *go1, u32 Fs, u32 grid, uint4 const *chunk_cls,
+                          u32 *cursors = nullptr);
+void launch_part_count4(hipStream_t s, PartChunk const *pch, PartOp4 const *ops4, TreeCtl const *ctl, u8 const *pl0,
+                        u8 const *pl1, u8 const *plp, u32 Fs, uint4 *chunk_cls, u32 grid);
 void launch_finalize(hipStream_t s, NodeInfo const *front, TreeCtl const *ctl, DevNode *tree, GrowCfg const &cfg,
                      double const *scales, u32 max_front);
 void launch_next_level(hipStream_t s, TreeCtl *ctl);
@@ -101,6 +104,11 @@ Tuning const &tuning();
 void launch_copy_calibrate(hipStream_t s, void const *src, void *dst, u64 bytes, u32 mode = 0, u32 grid = 0);
 void launch_bin_rows(hipStream_t s, float const *X, u64 n, u32 F, u32 Fs, float const *cuts, u32 const *coff,
                      u32 n_cuts_total, u8 *plane);
+// Four columns of features f0..f0+G-1 (column-major, leading dimension ld) into the plane.
+void launch_bin_cols4(hipStream_t s, float const *cols, u64 ld, u64 n, u32 G, float const *cuts, u32 const *coff,
+                      u32 f0, u32 Fs, u8 *plane);
+// out[i][:] = X[rows[i]][:] (float rows of F features).
+void launch_gather_f32_rows(hipStream_t s, float const *X, u32 const *rows, u64 n_out, u32 F, float *out);
 void launch_gather_rows(hipStream_t s, u8 const *plane, u32 Fs, void const *gh, u32 gh_bytes, u32 const *rows,
                         u64 n, u8 *out_plane, void *out_gh, int gh_is_row_id);
 

@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void k_shap(u8 const *__restrict__ bins, u64 n
         SHead const hd = heads[static_cast<size_t>(g) * 64u + lane];
         u32 const nel = hd.n;
         if (nel == 0) continue;
-        u32 feat[KM], lo[KM], hi[KM];
+        u32 feat[KM], lo[KM], hi[KM], lst[KM];
         bool miss[KM];
         float z[KM], wl[KM];
         float const *w = weights + (nel - 1) * nel / 2;
@@ -168,6 +168,7 @@ __global__ __launch_bounds__(256) void k_shap(u8 const *__restrict__ bins, u64 n
             feat[j] = 0;
             lo[j] = 1;
             hi[j] = 0;
+            lst[j] = 0;
             miss[j] = false;
             z[j] = 1.0F;
             wl[j] = 0.0F;
@@ -177,11 +178,15 @@ __global__ __launch_bounds__(256) void k_shap(u8 const *__restrict__ bins, u64 n
                 miss[j] = (e.feature & 0x8000u) != 0;
                 lo[j] = e.lo;
                 hi[j] = e.hi;
+                lst[j] = last[feat[j]]; // the element's missing bin, read once per path
                 z[j] = e.z;
                 wl[j] = w[j];
             }
         }
-        double const v = hd.value;
+        // The leaf value in fixed-point units, rounded once to float: each contribution
+        // is then a float product of the path's float polynomial terms, rounded to the
+        // 64-bit accumulator's integer grid (|value| * scale <= 2^59).
+        float const vs = static_cast<float>(static_cast<double>(hd.value) * scale);
         // Lanes start on different rows, so one atomic instruction spreads over the
         // rows' accumulators instead of piling onto one row's few features.
         u32 rr = lane % nr;
@@ -197,7 +202,7 @@ __global__ __launch_bounds__(256) void k_shap(u8 const *__restrict__ bins, u64 n
                 sat[j] = false;
                 if (static_cast<u32>(j) < nel) {
                     u32 const b = row[feat[j]];
-                    bool const one = b == last[feat[j]] ? miss[j] : (b >= lo[j] && b <= hi[j]);
+                    bool const one = b == lst[j] ? miss[j] : (b >= lo[j] && b <= hi[j]);
                     sat[j] = one;
                     // multiply by (z + o t)
 #pragma unroll
@@ -212,9 +217,9 @@ __global__ __launch_bounds__(256) void k_shap(u8 const *__restrict__ bins, u64 n
 #pragma unroll
             for (int k = 0; k < KM; ++k) {
                 if (static_cast<u32>(k) < nel) {
-                    double c;
+                    float c;
                     if (!sat[k]) {
-                        c = -v * static_cast<double>(unsat);
+                        c = -vs * unsat;
                     } else {
                         // synthetic division by (t + z_k): deflated[n-1] = P[n],
                         // deflated[i] = P[i+1] - z_k deflated[i+1]
@@ -227,9 +232,9 @@ __global__ __launch_bounds__(256) void k_shap(u8 const *__restrict__ bins, u64 n
                                 sdef += wl[i] * prev;
                             }
                         }
-                        c = v * static_cast<double>(1.0F - zk) * static_cast<double>(sdef);
+                        c = vs * (1.0F - zk) * sdef;
                     }
-                    atomicAdd(phi + feat[k], static_cast<unsigned long long>(__double2ll_rn(c * scale)));
+                    atomicAdd(phi + feat[k], static_cast<unsigned long long>(__float2ll_rn(c)));
                 }
             }
         }

@@ -116,6 +116,25 @@ int bh_upload_bins(bh_context *ctx, const uint8_t *bins, uint64_t n_rows, uint32
  * previous plane like bh_upload_bins. */
 int bh_upload_features(bh_context *ctx, const float *X, uint64_t n_rows, uint32_t n_features,
                        const float *cuts, const uint32_t *cut_offsets);
+/* Column ingest (cuda_ingest(ColumnBatch), src/cuda/histogram_engine.cu:420-470, with
+ * bin_col_kernel, src/cuda/detail/ingest_kernels.cuh:89-102): columns[f] points at feature
+ * f's n_rows floats in host memory; cuts as for bh_upload_features.  Columns are staged four
+ * at a time and each row's four bins are written as one dword.  Same plane as
+ * bh_upload_features on the row-major matrix. */
+int bh_upload_feature_columns(bh_context *ctx, const float *const *columns, uint64_t n_rows,
+                              uint32_t n_features, const float *cuts, const uint32_t *cut_offsets);
+/* Device-resident ingest (cuda_ingest_device, src/cuda/histogram_engine.cu:495-526): X_dev is
+ * a row-major float [n_rows][n_features] matrix in device memory of the context's device
+ * (other memory is refused), written completely before the call; no host copy is made. */
+int bh_upload_features_device(bh_context *ctx, const float *X_dev, uint64_t n_rows, uint32_t n_features,
+                              const float *cuts, const uint32_t *cut_offsets);
+/* Sample of a device-resident matrix for the host cut fitting (cuda_gather_rows,
+ * src/cuda/histogram_engine.cu:473-493, gather_rows_kernel, ingest_kernels.cuh:76-86):
+ * out[i][:] = X_dev[rows[i]][:] for i < n_sample (every rows[i] < n_rows, else an error
+ * before any copy); n_sample == 0 copies the whole matrix (out holds n_rows * n_features
+ * floats).  out is host memory. */
+int bh_gather_rows(int device, const float *X_dev, uint64_t n_rows, uint32_t n_features, const uint32_t *rows,
+                   uint64_t n_sample, float *out);
 /* Copy the resident pristine bin plane back (row-major, out_stride bytes per row). */
 int bh_download_bins(bh_context *ctx, uint8_t *out, uint64_t out_stride);
 /* Labels of the resident objective (resident_begin, device_context.cu:1541-1579). */

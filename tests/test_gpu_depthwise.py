@@ -175,6 +175,25 @@ def test_grow_depthwise_int_bit_exact(hip, orc, tall, depth, pair):
     assert np.array_equal(lor, rlor)
 
 
+@pytest.mark.parametrize("width", [10, 60])
+def test_grow_depthwise_int_paired_row_widths(hip, orc, tall, width):
+    """Rows of 12 and 60 bytes through the four-way partitions, whose tiles reserve
+    their class runs with atomic cursors (4,096 workgroups racing), give the oracle's
+    tree and leaf of every row."""
+    bins, nb, X, y = tall
+    reps = -(-width // bins.shape[1])
+    wb = np.ascontiguousarray(np.hstack([bins] * reps)[:, :width])
+    wnb = np.concatenate([nb] * reps)[:width]
+    g, h = grad_of(orc, y, seed=width)
+    ctx = ctx_for(hip, wb, wnb, quantized=True, max_depth=6, min_data_in_leaf=20)
+    ctx.set_tuning(pair_levels=1, part_grid=4096)
+    gq, hq, sc = ctx.set_gradients(g, h)
+    nodes, lor = ctx.grow_tree()
+    ref, rlor = orc.grow_depthwise(wb, wnb, gq=gq, hq=hq, scales=sc, c=orc_cfg(max_depth=6), mode=HIST_INT)
+    assert_same_tree(nodes, ref)
+    assert np.array_equal(lor, rlor)
+
+
 @pytest.mark.parametrize("depth", [3, 6])
 def test_grow_depthwise_f32_matches_device_order_oracle(hip, orc, tall, depth):
     bins, nb, X, y = tall
