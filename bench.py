@@ -255,9 +255,18 @@ def main() -> int:
                                      shapes)
         auc_f = held_out_auc(B, trees_f, nb, init, bins_h, yh)
         log(f"fp32: {a.steps} iters in {dt_f:.3f}s, held-out AUC {auc_f:.5f}")
+        # the FP32 histogram kernel against the same roofline (built-node bytes, 8-byte g/h)
+        prof_f = ctx.profile_hist(a.prof_iters)
+        hist_f_gbps = prof_f["bytes"] / (prof_f["ms"] * 1e-3) / 1e9
+        log(f"fp32 histogram: {prof_f['launches']} launches, {prof_f['ms'] * 1e3 / prof_f['launches']:.1f} us avg, "
+            f"{hist_f_gbps:.0f} GB/s algorithmic")
         fp32 = {"fit_s_500": round(dt_f / a.steps * FULL_ITERS, 3), "ms_per_step": round(dt_f / a.steps * 1e3, 3),
                 "held_out_auc": round(auc_f, 6), "speedup_quantized": round(dt_f / dt_q, 3),
-                "auc_delta_quantized": round(auc_q - auc_f, 6), "tuning": tuned_f}
+                "auc_delta_quantized": round(auc_q - auc_f, 6), "tuning": tuned_f,
+                "hist_roofline": {"kernel": "k_hist_fx", "achieved": round(hist_f_gbps, 1), "peak": HBM_PEAK_GBPS,
+                                  "unit": "GB/s", "frac": round(hist_f_gbps / HBM_PEAK_GBPS, 4),
+                                  "avg_launch_us": round(prof_f["ms"] * 1e3 / prof_f["launches"], 2),
+                                  "bytes_basis": f"built-node rows x ({X.shape[1]} + 8) + 8 B per built cell"}}
 
     # Equal AUC at the full fit length (untimed; the launch shapes stay as tuned).
     auc_full = None

@@ -55,6 +55,12 @@ def test_null_context_is_a_config_error(libpath):
     rc = L.bh_upload_labels(None, None)
     assert rc > 0
     assert b"context" in L.bh_last_error() or b"bins" in L.bh_last_error()
+    # the ingest arms refuse missing inputs before any device call
+    assert L.bh_upload_feature_columns(None, None, 10, 3, None, None) > 0
+    assert b"columns" in L.bh_last_error()
+    assert L.bh_upload_features_device(None, None, 10, 3, None, None) > 0
+    assert L.bh_gather_rows(0, None, 10, 3, None, 0, None) > 0
+    assert b"null" in L.bh_last_error()
 
 
 def test_fit_config_validation():
