@@ -1745,20 +1745,38 @@ __device__ __forceinline__ void store_runs(u32 *__restrict__ dst, Runs<NR> const
             }
         }
         u32 const lag = (t + T - start % T) % T; // first vector of this thread in the run
-        for (u32 v = lag; v < nvec; v += T) {
-            u32 const k0 = head + 4u * v;
-            u32 q = div_magic(k0, magic);
-            u32 rem = k0 - q * D;
-            u32 w[4];
+        if (D >= 4u) {
+            // four dwords span at most two rows: both row indices read once
+            for (u32 v = lag; v < nvec; v += T) {
+                u32 const k0 = head + 4u * v;
+                u32 const q = div_magic(k0, magic);
+                u32 const rem = k0 - q * D;
+                u32 const a0 = base + static_cast<u32>(im[q]) * D;
+                u32 const a1 = rem + 3u >= D ? base + static_cast<u32>(im[q + 1u]) * D - D : a0;
+                u32 w[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                w[j] = src[base + static_cast<u32>(im[q]) * D + rem];
-                if (++rem == D) {
-                    rem = 0;
-                    ++q;
+                for (u32 j = 0; j < 4; ++j) {
+                    u32 const c = rem + j;
+                    w[j] = src[(c < D ? a0 : a1) + c];
                 }
+                *reinterpret_cast<uint4 *>(dst + first + k0) = make_uint4(w[0], w[1], w[2], w[3]);
             }
-            *reinterpret_cast<uint4 *>(dst + first + k0) = make_uint4(w[0], w[1], w[2], w[3]);
+        } else {
+            for (u32 v = lag; v < nvec; v += T) {
+                u32 const k0 = head + 4u * v;
+                u32 q = div_magic(k0, magic);
+                u32 rem = k0 - q * D;
+                u32 w[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    w[j] = src[base + static_cast<u32>(im[q]) * D + rem];
+                    if (++rem == D) {
+                        rem = 0;
+                        ++q;
+                    }
+                }
+                *reinterpret_cast<uint4 *>(dst + first + k0) = make_uint4(w[0], w[1], w[2], w[3]);
+            }
         }
         start += nvec;
     }
@@ -1993,7 +2011,7 @@ __device__ uint4 chunk_prefix4(uint4 const *__restrict__ chunk_cls, u32 first, u
 // pristine plane.  With one op every tile of the level meets on the same four cursors,
 // so the planner keeps CUR for levels whose ops spread the reservations.
 template <bool CUR>
-__global__ __launch_bounds__(256) void k_part_scatter4(
+__global__ __launch_bounds__(256, 4) void k_part_scatter4(
     PartChunk const *__restrict__ pch, PartOp4 const *__restrict__ ops4, TreeCtl *__restrict__ ctl,
     u8 const *__restrict__ pl0, u8 const *__restrict__ pl1, u8 const *__restrict__ plp,
     u32 const *__restrict__ gw0, u32 const *__restrict__ gw1, u32 const *__restrict__ gwp,
@@ -2006,12 +2024,11 @@ __global__ __launch_bounds__(256) void k_part_scatter4(
     uint4 *s_rows = reinterpret_cast<uint4 *>(smem);                  // tile * Fs + 16 bytes
     uint4 *s_gh = reinterpret_cast<uint4 *>(smem + tile * Fs + 16u);  // tile * 4 + 16
     u16 *s_idx = reinterpret_cast<u16 *>(smem + tile * (Fs + 4u) + 32u); // 4 x tile
-    __shared__ u32 s_wsum[16];
+    __shared__ __align__(16) u32 s_wsum[16]; // [class][wave] rows of one 256-row step
     __shared__ u32 s_res[4];
     u32 const n = ctl->n_part_chunks;
     u32 const D = Fs / 4u;
     u32 const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    u32 const waves = blockDim.x / 64u;
     for (u32 c = blockIdx.x; c < n; c += gridDim.x) {
         PartChunk const ch = pch[c];
         PartOp4 const q = ops4[ch.op];
@@ -2069,15 +2086,21 @@ __global__ __launch_bounds__(256) void k_part_scatter4(
                     if (lane == 0) s_wsum[m * 4u + wave] = static_cast<u32>(__popcll(bal));
                 }
                 __syncthreads();
-                u32 pre = 0;
+                // the four waves' counts of each class in one 16-byte read per class
+                uint4 const c0 = reinterpret_cast<uint4 const *>(s_wsum)[0];
+                uint4 const c1 = reinterpret_cast<uint4 const *>(s_wsum)[1];
+                uint4 const c2 = reinterpret_cast<uint4 const *>(s_wsum)[2];
+                uint4 const c3 = reinterpret_cast<uint4 const *>(s_wsum)[3];
                 if (cls < 4u) {
-                    pre = cls == 0u ? tr[0] : (cls == 1u ? tr[1] : (cls == 2u ? tr[2] : tr[3]));
-                    for (u32 k = 0; k < wave; ++k) pre += s_wsum[cls * 4u + k];
+                    uint4 const cw = cls == 0u ? c0 : (cls == 1u ? c1 : (cls == 2u ? c2 : c3));
+                    u32 const before = (wave > 0u ? cw.x : 0u) + (wave > 1u ? cw.y : 0u) + (wave > 2u ? cw.z : 0u);
+                    u32 const pre = (cls == 0u ? tr[0] : (cls == 1u ? tr[1] : (cls == 2u ? tr[2] : tr[3]))) + before;
                     s_idx[cls * tile + pre + rank] = static_cast<u16>(r);
                 }
-#pragma unroll
-                for (u32 m = 0; m < 4; ++m)
-                    for (u32 k = 0; k < waves; ++k) tr[m] += s_wsum[m * 4u + k];
+                tr[0] += c0.x + c0.y + c0.z + c0.w;
+                tr[1] += c1.x + c1.y + c1.z + c1.w;
+                tr[2] += c2.x + c2.y + c2.z + c2.w;
+                tr[3] += c3.x + c3.y + c3.z + c3.w;
                 __syncthreads();
             }
             u32 const *sr = reinterpret_cast<u32 const *>(s_rows);
