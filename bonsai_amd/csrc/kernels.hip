@@ -1722,11 +1722,13 @@ template <int NR> struct Runs {
     u32 n[NR];        // dwords of each run
 };
 
+constexpr u32 kScatterThreads = 256; // workgroup size of the narrow scatters
 template <int NR>
 __device__ __forceinline__ void store_runs(u32 *__restrict__ dst, Runs<NR> const rs, u32 D, u64 magic,
                                            u32 const *src, u32 base, u16 const *idx, u32 idx_stride)
 {
-    u32 const T = blockDim.x, t = threadIdx.x;
+    constexpr u32 T = kScatterThreads;
+    u32 const t = threadIdx.x;
     u32 start = 0; // first vector of run m in the shared index space
 #pragma unroll
     for (int m = 0; m < NR; ++m) {
@@ -1759,6 +1761,15 @@ __device__ __forceinline__ void store_runs(u32 *__restrict__ dst, Runs<NR> const
                     u32 const c = rem + j;
                     w[j] = src[(c < D ? a0 : a1) + c];
                 }
+                *reinterpret_cast<uint4 *>(dst + first + k0) = make_uint4(w[0], w[1], w[2], w[3]);
+            }
+        } else if (D == 1u) {
+            // one dword per row (the gradient words)
+            for (u32 v = lag; v < nvec; v += T) {
+                u32 const k0 = head + 4u * v;
+                u32 w[4];
+#pragma unroll
+                for (u32 j = 0; j < 4; ++j) w[j] = src[base + static_cast<u32>(im[k0 + j])];
                 *reinterpret_cast<uint4 *>(dst + first + k0) = make_uint4(w[0], w[1], w[2], w[3]);
             }
         } else {
@@ -1896,13 +1907,19 @@ __device__ __forceinline__ u32 route4(u8 const *row, PartOp4 const &q)
     return route4_bins(row[q.feature], row[q.fa], row[q.fb], q);
 }
 
-// Rows per class of every partition chunk.
+// Rows per class of every partition chunk.  Only three bytes of a row route it, but
+// its cache lines are read whole either way, so the chunk is streamed like the scatter
+// does it: tiles of whole rows read with aligned 16-byte loads into registers (the next
+// tile's loads in flight while this one is counted), staged in LDS, and every lane
+// routes its rows from there.  tile * Fs + 16 <= kPfRows * 16 * 256 bytes (launcher).
 __global__ __launch_bounds__(256) void k_part_count4(PartChunk const *__restrict__ pch,
                                                      PartOp4 const *__restrict__ ops4,
                                                      TreeCtl const *__restrict__ ctl, u8 const *__restrict__ pl0,
                                                      u8 const *__restrict__ pl1, u8 const *__restrict__ plp, u32 Fs,
-                                                     uint4 *__restrict__ chunk_cls)
+                                                     u32 tile, uint4 *__restrict__ chunk_cls)
 {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint4 *s_rows = reinterpret_cast<uint4 *>(smem); // tile * Fs + 16 bytes
     __shared__ u32 red[4][4];
     u32 const n = ctl->n_part_chunks;
     u32 const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -1911,28 +1928,28 @@ __global__ __launch_bounds__(256) void k_part_count4(PartChunk const *__restrict
         PartOp4 const q = ops4[ch.op];
         u8 const *plane = q.src == 0 ? pl0 : (q.src == 1 ? pl1 : plp);
         u32 k0 = 0, k1 = 0, k2 = 0, k3 = 0;
-        auto tally = [&](u32 cls) {
-            k0 += cls == 0u;
-            k1 += cls == 1u;
-            k2 += cls == 2u;
-            k3 += cls == 3u;
-        };
-        // four rows per thread with their three routing bytes loaded together
-        u32 k = ch.begin + threadIdx.x;
-        u8 const *base = plane + static_cast<size_t>(q.off) * Fs;
-        for (; k + 3u * blockDim.x < ch.end; k += 4u * blockDim.x) {
-            u32 b0[4], ba[4], bb[4];
-#pragma unroll
-            for (u32 u = 0; u < 4; ++u) {
-                u8 const *row = base + static_cast<size_t>(k + u * blockDim.x) * Fs;
-                b0[u] = row[q.feature];
-                ba[u] = row[q.fa];
-                bb[u] = row[q.fb];
+        uint4 pr[kPfRows];
+        u32 pr_n, pr_sh;
+        pr_sh = pf_load(pr, pr_n, plane, (static_cast<size_t>(q.off) + ch.begin) * Fs,
+                        min(tile, ch.end - ch.begin) * Fs);
+        for (u32 t0 = ch.begin; t0 < ch.end; t0 += tile) {
+            u32 const rows = min(tile, ch.end - t0);
+            __syncthreads(); // the previous tile is counted
+            pf_store(pr, pr_n, s_rows);
+            u32 const sh = pr_sh;
+            __syncthreads();
+            if (t0 + tile < ch.end)
+                pr_sh = pf_load(pr, pr_n, plane, (static_cast<size_t>(q.off) + t0 + tile) * Fs,
+                                min(tile, ch.end - t0 - tile) * Fs);
+            u8 const *srb = reinterpret_cast<u8 const *>(s_rows) + sh;
+            for (u32 r = threadIdx.x; r < rows; r += blockDim.x) {
+                u32 const cls = route4(srb + r * Fs, q);
+                k0 += cls == 0u;
+                k1 += cls == 1u;
+                k2 += cls == 2u;
+                k3 += cls == 3u;
             }
-#pragma unroll
-            for (u32 u = 0; u < 4; ++u) tally(route4_bins(b0[u], ba[u], bb[u], q));
         }
-        for (; k < ch.end; k += blockDim.x) tally(route4(base + static_cast<size_t>(k) * Fs, q));
         for (int o = 32; o > 0; o >>= 1) {
             k0 += __shfl_xor(k0, o);
             k1 += __shfl_xor(k1, o);
@@ -3287,7 +3304,7 @@ void launch_part_scatter(hipStream_t s, PartChunk const *pch, PartOp const *ops,
         u32 const G = gh_bytes / 4u;
         u64 const magic_d = ((1ull << 32) + D - 1u) / D;
         u64 const magic_g = ((1ull << 32) + G - 1u) / G;
-        hipLaunchKernelGGL(k_part_scatter_v, dim3(grid), dim3(256), lds, s, pch, ops, ctl, pl0, pl1, plp, g0, g1,
+        hipLaunchKernelGGL(k_part_scatter_v, dim3(grid), dim3(kScatterThreads), lds, s, pch, ops, ctl, pl0, pl1, plp, g0, g1,
                            gp, o0, o1, go0, go1, Fs, gh_bytes, tile, magic_d, magic_g, chunk_left,
                            const_cast<TreeCtl *>(ctl));
         return;
@@ -3320,7 +3337,11 @@ void launch_plan_counted(hipStream_t s, NodeInfo const *front, SplitRec const *s
 void launch_part_count4(hipStream_t s, PartChunk const *pch, PartOp4 const *ops4, TreeCtl const *ctl, u8 const *pl0,
                         u8 const *pl1, u8 const *plp, u32 Fs, uint4 *chunk_cls, u32 grid)
 {
-    hipLaunchKernelGGL(k_part_count4, dim3(grid), dim3(256), 0, s, pch, ops4, ctl, pl0, pl1, plp, Fs, chunk_cls);
+    // whole tiles in the register prefetch (kPfRows 16-byte words per thread)
+    u32 const tile = std::max(64u, std::min(kPartChunk, ((kPfRows * 16u * 256u - 16u) / Fs) & ~63u));
+    size_t const lds = static_cast<size_t>(tile) * Fs + 16u;
+    hipLaunchKernelGGL(k_part_count4, dim3(grid), dim3(256), lds, s, pch, ops4, ctl, pl0, pl1, plp, Fs, tile,
+                       chunk_cls);
 }
 
 u32 scatter4_tile_rows(u32 Fs)
@@ -3342,10 +3363,10 @@ void launch_part_scatter4(hipStream_t s, PartChunk const *pch, PartOp4 const *op
     size_t const lds = static_cast<size_t>(tile) * (Fs + 12u) + 32u;
     u64 const magic_d = ((1ull << 32) + D - 1u) / D;
     if (cursors)
-        hipLaunchKernelGGL(k_part_scatter4<true>, dim3(grid), dim3(256), lds, s, pch, ops4, ctl, pl0, pl1, plp, g0, g1,
+        hipLaunchKernelGGL(k_part_scatter4<true>, dim3(grid), dim3(kScatterThreads), lds, s, pch, ops4, ctl, pl0, pl1, plp, g0, g1,
                            gp, o0, o1, go0, go1, Fs, tile, magic_d, chunk_cls, cursors);
     else
-        hipLaunchKernelGGL(k_part_scatter4<false>, dim3(grid), dim3(256), lds, s, pch, ops4, ctl, pl0, pl1, plp, g0,
+        hipLaunchKernelGGL(k_part_scatter4<false>, dim3(grid), dim3(kScatterThreads), lds, s, pch, ops4, ctl, pl0, pl1, plp, g0,
                            g1, gp, o0, o1, go0, go1, Fs, tile, magic_d, chunk_cls, cursors);
 }
 
