@@ -3222,11 +3222,12 @@ void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl *ctl, voi
     u32 const tiles = min(64u, (node_cells + 255) / 256);
     dim3 const grid(tiles, max(1u, min(max_ops, 1024u)));
     if (q) {
-        // Enough workgroups to cover the cells of one op several times over; ops beyond
-        // the grid's y extent loop.
+        // Enough workgroups to cover the cells of one op several times over (up to 16,384
+        // in all, so a level of many ops does not walk a long chain of cell tiles per
+        // workgroup); ops beyond the grid's y extent loop.
         u32 const ops_y = max(1u, min(max_ops, 1024u));
         u32 const cell_tiles = (node_cells + kFqCells - 1) / kFqCells;
-        u32 const tx = max(1u, min(cell_tiles, max(1u, 4096u / ops_y)));
+        u32 const tx = max(1u, min(cell_tiles, max(1u, 16384u / ops_y)));
         hipLaunchKernelGGL(k_finish_q, dim3(tx, ops_y), dim3(256), 0, s, ops, ctl, partials, hist_cur, hist_next,
                            node_cells, cnt_cur, cnt_next, advance ? 1u : 0u, tF);
     } else {
