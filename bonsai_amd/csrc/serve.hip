@@ -158,20 +158,34 @@ __global__ __launch_bounds__(256) void k_shap(u8 const *__restrict__ bins, u64 n
     for (u32 g = wave; g < n_groups; g += NW) {
         SHead const hd = heads[static_cast<size_t>(g) * 64u + lane];
         u32 const nel = hd.n;
-        if (nel == 0) continue;
+        // The group's longest path (wave-uniform): the KM = 16 body skips element slots
+        // past it; the KM = 8 body runs all eight.
+        u32 nmax = nel;
+        for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, static_cast<u32>(__shfl_xor(static_cast<int>(nmax), o)));
+        nmax = __builtin_amdgcn_readfirstlane(nmax);
+        if (nmax == 0) continue;
+        // Slots past a lane's path hold a neutral element: z = 1 and never satisfied
+        // (lo > hi, a missing bin no byte equals), so it multiplies P(t) by exactly 1 and
+        // its deflation terms are exact zeros (P's coefficients past the path are 0, as
+        // are its weights).  No per-lane guard is needed until the contribution's add.
         u32 feat[KM], lo[KM], hi[KM], lst[KM];
         bool miss[KM];
-        float z[KM], wl[KM];
+        float z[KM], wl[KM], vz[KM];
         float const *w = weights + (nel - 1) * nel / 2;
+        // The leaf value in fixed-point units, rounded once to float: each contribution
+        // is then a float product of the path's float polynomial terms, rounded to the
+        // 64-bit accumulator's integer grid (|value| * scale <= 2^59).
+        float const vs = static_cast<float>(static_cast<double>(hd.value) * scale);
 #pragma unroll
         for (int j = 0; j < KM; ++j) {
             feat[j] = 0;
             lo[j] = 1;
             hi[j] = 0;
-            lst[j] = 0;
+            lst[j] = 0xFFFFFFFFu;
             miss[j] = false;
             z[j] = 1.0F;
             wl[j] = 0.0F;
+            vz[j] = 0.0F;
             if (static_cast<u32>(j) < nel) {
                 SElem const e = elems[(static_cast<size_t>(g) * K + j) * 64u + lane];
                 feat[j] = e.feature & 0x7FFFu;
@@ -181,12 +195,9 @@ __global__ __launch_bounds__(256) void k_shap(u8 const *__restrict__ bins, u64 n
                 lst[j] = last[feat[j]]; // the element's missing bin, read once per path
                 z[j] = e.z;
                 wl[j] = w[j];
+                vz[j] = vs * (1.0F - e.z); // a satisfied element's factor before its sum
             }
         }
-        // The leaf value in fixed-point units, rounded once to float: each contribution
-        // is then a float product of the path's float polynomial terms, rounded to the
-        // 64-bit accumulator's integer grid (|value| * scale <= 2^59).
-        float const vs = static_cast<float>(static_cast<double>(hd.value) * scale);
         // Lanes start on different rows, so one atomic instruction spreads over the
         // rows' accumulators instead of piling onto one row's few features.
         u32 rr = lane % nr;
@@ -200,41 +211,40 @@ __global__ __launch_bounds__(256) void k_shap(u8 const *__restrict__ bins, u64 n
 #pragma unroll
             for (int j = 0; j < KM; ++j) {
                 sat[j] = false;
-                if (static_cast<u32>(j) < nel) {
+                if (KM <= 8 || static_cast<u32>(j) < nmax) {
                     u32 const b = row[feat[j]];
                     bool const one = b == lst[j] ? miss[j] : (b >= lo[j] && b <= hi[j]);
                     sat[j] = one;
                     // multiply by (z + o t)
+                    float const zj = z[j];
 #pragma unroll
                     for (int i = KM; i >= 1; --i)
-                        if (i <= j + 1) poly[i] = z[j] * poly[i] + (one ? poly[i - 1] : 0.0F);
-                    poly[0] *= z[j];
+                        if (i <= j + 1) poly[i] = __builtin_fmaf(zj, poly[i], one ? poly[i - 1] : 0.0F);
+                    poly[0] *= zj;
                 }
             }
             float unsat = 0.0F;
 #pragma unroll
-            for (int i = 0; i < KM; ++i) unsat += wl[i] * poly[i]; // wl[i] = 0 beyond the path
+            for (int i = 0; i < KM; ++i) unsat = __builtin_fmaf(wl[i], poly[i], unsat); // wl[i] = 0 beyond the path
+            float const nu = -vs * unsat;
 #pragma unroll
             for (int k = 0; k < KM; ++k) {
-                if (static_cast<u32>(k) < nel) {
-                    float c;
-                    if (!sat[k]) {
-                        c = -vs * unsat;
-                    } else {
-                        // synthetic division by (t + z_k): deflated[n-1] = P[n],
-                        // deflated[i] = P[i+1] - z_k deflated[i+1]
-                        float const zk = z[k];
-                        float sdef = 0.0F, prev = 0.0F;
+                if (KM <= 8 || static_cast<u32>(k) < nmax) {
+                    // synthetic division by (t + z_k), from the top: deflated[i] =
+                    // P[i+1] - z_k deflated[i+1], zero above the path's degree; used only
+                    // where element k is satisfied (P then holds the factor t + z_k)
+                    float const nzk = -z[k];
+                    float sdef = 0.0F, prev = 0.0F;
 #pragma unroll
-                        for (int i = KM - 1; i >= 0; --i) {
-                            if (static_cast<u32>(i) < nel) {
-                                prev = (static_cast<u32>(i) + 1 == nel) ? poly[i + 1] : poly[i + 1] - zk * prev;
-                                sdef += wl[i] * prev;
-                            }
+                    for (int i = KM - 1; i >= 0; --i) {
+                        if (KM <= 8 || static_cast<u32>(i) < nmax) {
+                            prev = __builtin_fmaf(nzk, prev, poly[i + 1]);
+                            sdef = __builtin_fmaf(wl[i], prev, sdef);
                         }
-                        c = vs * (1.0F - zk) * sdef;
                     }
-                    atomicAdd(phi + feat[k], static_cast<unsigned long long>(__float2ll_rn(c)));
+                    float const c = sat[k] ? vz[k] * sdef : nu;
+                    if (static_cast<u32>(k) < nel)
+                        atomicAdd(phi + feat[k], static_cast<unsigned long long>(__float2ll_rn(c)));
                 }
             }
         }
