@@ -7,6 +7,9 @@
 namespace bh {
 
 constexpr u32 kFindTile = 7;
+// The leafwise finder's tile: its two nodes per launch leave most of the chip idle, so
+// smaller tiles (7 workgroups per node at 28 features) shorten the critical path.
+constexpr u32 kLwFindTile = 4;
 // Threads per split-find workgroup: 16 waves, so each SIMD interleaves four waves of the
 // FP64 candidate scoring.
 constexpr u32 kFindThreads = 1024;
@@ -100,7 +103,7 @@ __device__ __forceinline__ float fd_leaf_value(u32 id, double g, double h, GrowC
 
 // CON: the feature mask and the monotone / interaction constraints are compiled in
 // (launches without any use the leaner CON = false body).
-template <bool Q, bool CON>
+template <bool Q, bool CON, u32 TILE = kFindTile>
 __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restrict__ ctl, void const *__restrict__ hist,
                           u16 const *__restrict__ n_bins, u32 F, u32 SB, GrowCfg cfg, double const *__restrict__ scales,
                           SplitRec *__restrict__ out, u32 front_base, u32 n_front_override,
@@ -109,16 +112,16 @@ __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restric
     typedef typename CellT<Q>::T Cell;
     typedef typename CellT<Q>::A Acc;
     extern __shared__ __align__(16) unsigned char smem[];
-    Cell *s_cells = reinterpret_cast<Cell *>(smem);              // [kFindTile][SB]
-    Acc *pg = reinterpret_cast<Acc *>(s_cells + kFindTile * SB); // [kFindTile][SB]
-    Acc *ph = pg + kFindTile * SB;                                // [kFindTile][SB]
-    u32 *s_cn = reinterpret_cast<u32 *>(ph + kFindTile * SB);    // [kFindTile][SB] counts, then their prefix
+    Cell *s_cells = reinterpret_cast<Cell *>(smem);              // [TILE][SB]
+    Acc *pg = reinterpret_cast<Acc *>(s_cells + TILE * SB); // [TILE][SB]
+    Acc *ph = pg + TILE * SB;                                // [TILE][SB]
+    u32 *s_cn = reinterpret_cast<u32 *>(ph + TILE * SB);    // [TILE][SB] counts, then their prefix
     __shared__ double s_best_gain[kFindThreads / 64];
     __shared__ u32 s_best_key[kFindThreads / 64];
     __shared__ double s_tot[2];
     __shared__ i64 s_itot[2];
-    __shared__ Acc s_mg[kFindTile], s_mh[kFindTile];
-    __shared__ u32 s_ncut[kFindTile], s_dirs[kFindTile], s_nb[kFindTile], s_mc[kFindTile];
+    __shared__ Acc s_mg[TILE], s_mh[TILE];
+    __shared__ u32 s_ncut[TILE], s_dirs[TILE], s_nb[TILE], s_mc[TILE];
     __shared__ u32 s_allow[kMaxPathWords], s_cover[kMaxGroups], s_single;
 
     // Every launch's frontier buffer holds gridDim.y entries past front_base, so the
@@ -261,8 +264,8 @@ __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restric
         // (the tile loop's first barrier publishes s_allow)
     }
     // Block bx scans feature tiles bx, bx + G, ...
-    for (u32 f0 = bx * kFindTile; f0 < F; f0 += G * kFindTile) {
-        u32 const nt = min(kFindTile, F - f0);
+    for (u32 f0 = bx * TILE; f0 < F; f0 += G * TILE) {
+        u32 const nt = min(TILE, F - f0);
         __syncthreads(); // the previous tile (or phase 0) is done with the LDS
         // Stage the tile's cells, bin counts and count cells (coalesced, one round trip).
         // An unselected or disallowed feature offers no cut (zero bins).
@@ -276,7 +279,7 @@ __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restric
         if (hcnt)
             for (u32 k = threadIdx.x; k < nt * SB; k += blockDim.x) s_cn[k] = hcnt[static_cast<size_t>(f0) * SB + k];
         __syncthreads();
-        if (tot_from_tile && f0 == bx * kFindTile) {
+        if (tot_from_tile && f0 == bx * TILE) {
             if (threadIdx.x < 64) {
                 i64 g = 0, h = 0;
                 for (u32 b = threadIdx.x; b < SB; b += 64) {
@@ -297,7 +300,7 @@ __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restric
             }
             __syncthreads();
         }
-        if (f0 == bx * kFindTile) {
+        if (f0 == bx * TILE) {
             tg = Q ? static_cast<double>(s_itot[0]) * sg : s_tot[0];
             th = Q ? static_cast<double>(s_itot[1]) * sh : s_tot[1];
             node_score = fd_score_of(tg, th, cfg.l1, cfg.l2);
@@ -307,7 +310,7 @@ __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restric
         // cells alongside, in place).
         if (Q) {
             u32 const wv = threadIdx.x >> 6, ln = threadIdx.x & 63u;
-            for (u32 t = wv; t < kFindTile; t += blockDim.x >> 6) {
+            for (u32 t = wv; t < TILE; t += blockDim.x >> 6) {
                 u32 ncut = 0, dirs = 1;
                 if (t < nt) {
                     u32 const nb = s_nb[t];
@@ -371,7 +374,7 @@ __device__ void find_node(NodeInfo *__restrict__ front, TreeCtl const *__restric
             }
         }
         u32 const t = threadIdx.x;
-        if (!Q && t < kFindTile) {
+        if (!Q && t < TILE) {
             u32 ncut = 0, dirs = 1;
             if (t < nt) {
                 u32 const nb = s_nb[t];

@@ -1268,9 +1268,9 @@ __global__ __launch_bounds__(256) void k_finish_q(FinishOp const *__restrict__ o
 // with the serial scan's tie order, and the node's last block merges the blocks'
 // winners the same way.
 // ---------------------------------------------------------------------------------
-u32 find_blocks_per_node(u32 F)
+u32 find_blocks_per_node(u32 F, u32 tile)
 {
-    return std::max(1u, std::min<u32>(std::min(tuning().find_blocks, kFindMaxBlocks), (F + kFindTile - 1) / kFindTile));
+    return std::max(1u, std::min<u32>(std::min(tuning().find_blocks, kFindMaxBlocks), (F + tile - 1) / tile));
 }
 
 template <bool Q, bool CON>
@@ -3243,7 +3243,7 @@ void launch_find(hipStream_t s, bool q, NodeInfo *front, TreeCtl const *ctl, voi
 {
     // cells + two prefix planes, and (quantized with count cells) the counts' prefix
     size_t const lds = static_cast<size_t>(kFindTile) * SB * (q ? (cnt ? 36 : 32) : 24);
-    u32 const G = find_blocks_per_node(F);
+    u32 const G = find_blocks_per_node(F, kFindTile);
     dim3 const grid(G, max_front);
     // A 1,024-thread workgroup occupies a whole CU (its register use leaves room for 16
     // waves): when the level has more workgroups than CUs, halve them so two fit per CU.

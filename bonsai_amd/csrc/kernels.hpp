@@ -38,7 +38,7 @@ void launch_finish(hipStream_t s, bool q, FinishOp const *ops, TreeCtl *ctl, voi
 void launch_find(hipStream_t s, bool q, NodeInfo *front, TreeCtl const *ctl, void const *hist, u16 const *n_bins,
                  u32 F, u32 SB, GrowCfg const &cfg, double const *scales, SplitRec *out, u32 max_front,
                  u32 front_base, u32 n_front_override, FindScratch const &fs, u32 const *cnt = nullptr);
-u32 find_blocks_per_node(u32 F);
+u32 find_blocks_per_node(u32 F, u32 tile); // workgroups per node for tile-feature tiles
 void launch_plan_level(hipStream_t s, NodeInfo const *front, SplitRec const *splits, TreeCtl *ctl, DevNode *tree,
                        PartOp *ops, PartChunk *pch, u16 const *n_bins, GrowCfg const &cfg, double const *scales,
                        u32 dst, u32 chunk_rows);

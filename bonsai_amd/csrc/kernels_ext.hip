@@ -900,7 +900,7 @@ __global__ __launch_bounds__(kFindThreads) void k_lw_find(NodeInfo *__restrict__
                                                         u32 *__restrict__ done, u32 const *__restrict__ cnt, LwNext nx)
 {
     __shared__ u32 s_last_wg;
-    find_node<Q, CON>(kids, ctl, hist, n_bins, F, SB, cfg, scales, splits, 0u, 0u, part, done, cnt);
+    find_node<Q, CON, kLwFindTile>(kids, ctl, hist, n_bins, F, SB, cfg, scales, splits, 0u, 0u, part, done, cnt);
     if (!nx.next) return;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1206,8 +1206,8 @@ void launch_level_find_step(ExtState &s, hipStream_t st, bool q, GrowView const 
 template <bool Q, bool COUNTED>
 void launch_lw_find(hipStream_t st, GrowView const &v, LwCtl *lw, u32 *cnt, LwNext const &nx)
 {
-    size_t const lds = static_cast<size_t>(kFindTile) * v.SB * (Q ? (cnt ? 36 : 32) : 24);
-    dim3 const grid(find_blocks_per_node(v.F), 2);
+    size_t const lds = static_cast<size_t>(kLwFindTile) * v.SB * (Q ? (cnt ? 36 : 32) : 24);
+    dim3 const grid(find_blocks_per_node(v.F, kLwFindTile), 2);
     bool const con = v.gc.fmask || v.gc.mono || v.gc.n_groups;
     auto const fn = con ? &k_lw_find<Q, COUNTED, true> : &k_lw_find<Q, COUNTED, false>;
     hipLaunchKernelGGL(fn, grid, dim3(kFindThreads), lds, st, v.front[0], v.ctl, lw, v.hist[0], v.d_nbins, v.F, v.SB,
