@@ -132,6 +132,15 @@ __global__ __launch_bounds__(256) void k_predict(u8 const *__restrict__ bins, u6
 // row's result does not depend on the order of the adds or on what shares its batch.
 // ---------------------------------------------------------------------------------
 constexpr u32 kShapRows = 32;
+// A launch with fewer row blocks than this splits each block's path groups over more
+// workgroups, so small batches still spread over the chip.
+constexpr u32 kShapFillBlocks = 2048;
+
+// φ of one cell from its 64-bit fixed-point sum.
+__device__ __forceinline__ double shap_from_fixed(unsigned long long v, double scale, float lr)
+{
+    return static_cast<double>(static_cast<long long>(v)) / scale * static_cast<double>(lr);
+}
 
 // KM: compile-time bound on the path length (8 covers depth-8 trees), so every per-path
 // array lives in registers.
@@ -154,8 +163,10 @@ __global__ __launch_bounds__(256) void k_shap(u8 const *__restrict__ bins, u64 n
         s_row[k] = bins[(r0 + rr) * stride + f];
     }
     __syncthreads();
+    // gridDim.y workgroups share a row block, each taking every gridDim.y-th set of
+    // NW path groups.
     u32 const wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, NW = blockDim.x >> 6;
-    for (u32 g = wave; g < n_groups; g += NW) {
+    for (u32 g = blockIdx.y * NW + wave; g < n_groups; g += NW * gridDim.y) {
         SHead const hd = heads[static_cast<size_t>(g) * 64u + lane];
         u32 const nel = hd.n;
         // The group's longest path (wave-uniform): the KM = 16 body skips element slots
@@ -250,12 +261,45 @@ __global__ __launch_bounds__(256) void k_shap(u8 const *__restrict__ bins, u64 n
         }
     }
     __syncthreads();
+    if (gridDim.y > 1) {
+        // Path groups split over workgroups: the block's fixed-point sums go into the
+        // zeroed output cells with 64-bit integer atomics (exact in any order), and
+        // k_shap_to_double converts them once every workgroup has added its share.
+        unsigned long long *acc = reinterpret_cast<unsigned long long *>(out);
+        for (u32 k = threadIdx.x; k < nr * F; k += blockDim.x) {
+            u32 const rr = k / F, f = k - rr * F;
+            if (s_phi[k] != 0ull) atomicAdd(acc + (r0 + rr) * (F + 1) + f, s_phi[k]);
+        }
+        return;
+    }
     for (u32 k = threadIdx.x; k < nr * F; k += blockDim.x) {
         u32 const rr = k / F, f = k - rr * F;
-        out[(r0 + rr) * (F + 1) + f] =
-            static_cast<double>(static_cast<long long>(s_phi[k])) / scale * static_cast<double>(lr);
+        out[(r0 + rr) * (F + 1) + f] = shap_from_fixed(s_phi[k], scale, lr);
     }
     for (u32 rr = threadIdx.x; rr < nr; rr += blockDim.x) out[(r0 + rr) * (F + 1) + F] = bias;
+}
+
+// Split launches: the output cells start at zero (a fill kernel, so the sequence stays
+// capturable in a graph) ...
+__global__ __launch_bounds__(256) void k_shap_zero(unsigned long long *__restrict__ acc, u64 cells)
+{
+    for (u64 i = blockIdx.x * 256ull + threadIdx.x; i < cells; i += static_cast<u64>(gridDim.x) * 256ull) acc[i] = 0ull;
+}
+
+// ... and end as doubles, each cell converted in place by the thread that reads it.
+__global__ __launch_bounds__(256) void k_shap_to_double(double *__restrict__ out, u64 n, u32 F, float lr, double bias,
+                                                        double scale)
+{
+    u64 const cells = n * (F + 1);
+    for (u64 i = blockIdx.x * 256ull + threadIdx.x; i < cells; i += static_cast<u64>(gridDim.x) * 256ull) {
+        u32 const f = static_cast<u32>(i % (F + 1));
+        if (f == F) {
+            out[i] = bias;
+        } else {
+            unsigned long long const v = reinterpret_cast<unsigned long long const *>(out)[i];
+            out[i] = shap_from_fixed(v, scale, lr);
+        }
+    }
 }
 
 struct ActiveElem {
@@ -369,15 +413,32 @@ int launch_shap(bh_predictor *p, u8 const *d_bins, u64 n, u64 stride, double *d_
     size_t const lds = static_cast<size_t>(kShapRows) * p->F * (sizeof(unsigned long long) + 1u);
     if (lds > 160u * 1024u) return sfail(1, "TreeSHAP: too many features for the LDS accumulators");
     u32 const blocks = static_cast<u32>((n + kShapRows - 1) / kShapRows);
+    // Split the path groups of each row block over `split` workgroups while the row
+    // blocks alone leave the chip idle; every workgroup keeps at least one group per wave.
+    u32 split = 1;
+    if (blocks < kShapFillBlocks)
+        split = std::min((kShapFillBlocks + blocks - 1) / blocks, std::max(1u, p->n_groups / 4u));
+    u64 const cells = n * (p->F + 1);
+    u32 const cgrid = static_cast<u32>(std::min<u64>((cells + 255) / 256, 4096));
+    if (split > 1) {
+        hipLaunchKernelGGL(k_shap_zero, dim3(cgrid), dim3(256), 0, s, reinterpret_cast<unsigned long long *>(d_out),
+                           cells);
+        SHIP(hipGetLastError());
+    }
+    dim3 const grid(blocks, split);
     if (p->max_len <= 8)
-        hipLaunchKernelGGL(k_shap<8>, dim3(blocks), dim3(256), lds, s, d_bins, n, stride, p->F, p->d_elems,
-                           p->d_heads, p->n_groups, p->max_len, p->d_w, p->d_last, p->lr, p->shap_bias,
-                           p->shap_scale, d_out);
+        hipLaunchKernelGGL(k_shap<8>, grid, dim3(256), lds, s, d_bins, n, stride, p->F, p->d_elems, p->d_heads,
+                           p->n_groups, p->max_len, p->d_w, p->d_last, p->lr, p->shap_bias, p->shap_scale, d_out);
     else
-        hipLaunchKernelGGL(k_shap<kMaxPath>, dim3(blocks), dim3(256), lds, s, d_bins, n, stride, p->F, p->d_elems,
+        hipLaunchKernelGGL(k_shap<kMaxPath>, grid, dim3(256), lds, s, d_bins, n, stride, p->F, p->d_elems,
                            p->d_heads, p->n_groups, p->max_len, p->d_w, p->d_last, p->lr, p->shap_bias,
                            p->shap_scale, d_out);
     SHIP(hipGetLastError());
+    if (split > 1) {
+        hipLaunchKernelGGL(k_shap_to_double, dim3(cgrid), dim3(256), 0, s, d_out, n, p->F, p->lr, p->shap_bias,
+                           p->shap_scale);
+        SHIP(hipGetLastError());
+    }
     return 0;
 }
 

@@ -143,3 +143,21 @@ def test_serve_overlaps_batches_on_the_device(hip, orc, model):
         assert np.array_equal(p.predict(rows[a:b]).view(np.uint32), pred[a:b].view(np.uint32))
         assert np.array_equal(p.shap(rows[a:b]).view(np.uint64), shap[a:b].view(np.uint64))
     p.close()
+
+
+def test_shap_split_launch_equals_single_pass(hip, orc, model):
+    """A batch of 65,536 or more rows runs one workgroup per 32-row block over every path;
+    smaller batches split each block's path groups over several workgroups that add their
+    fixed-point sums with 64-bit integer atomics.  Both give the same bits for every row."""
+    trees, init, held, nb = model
+    p = hip.Predictor(trees, nb, 0.1, init)
+    rng = np.random.default_rng(21)
+    big = held[rng.integers(0, held.shape[0], 65_536 + 37)]
+    whole = p.shap(big)                               # single pass: 2,050 row blocks
+    for a, b in ((0, 1), (5, 262), (65_000, 65_573), (1_000, 33_000)):
+        part = p.shap(big[a:b])                       # split launches (1 to 1,000 row blocks)
+        assert np.array_equal(part.view(np.uint64), whole[a:b].view(np.uint64)), (a, b)
+    ref = orc.pred_contribs(trees, big[:64], nb, 0.1, init, 6)
+    scale = np.abs(ref).sum(axis=1, keepdims=True)
+    assert np.max(np.abs(whole[:64] - ref) / scale) <= 1e-5
+    p.close()
