@@ -413,7 +413,11 @@ int bh_predict(bh_predictor *p, const uint8_t *bins, uint64_t n, uint64_t row_st
  * (include/bonsai/booster.hpp:902-941). */
 int bh_shap(bh_predictor *p, const uint8_t *bins, uint64_t n, uint64_t row_stride,
             double *out);
-/* Same two on device-resident inputs/outputs (already in HBM) on a caller stream. */
+/* Same two on device-resident inputs/outputs (already in HBM) on a caller stream.
+ * bh_shap_device may use d_out as its 64-bit fixed-point accumulator before writing
+ * the doubles (batches under 65,536 rows), so d_out must not overlap d_bins.  It
+ * allocates nothing and issues kernels only (no memset), so a caller may capture it
+ * in a graph. */
 int bh_predict_device(bh_predictor *p, const uint8_t *d_bins, uint64_t n, uint64_t row_stride,
                       float *d_out, void *hip_stream);
 int bh_shap_device(bh_predictor *p, const uint8_t *d_bins, uint64_t n, uint64_t row_stride,
